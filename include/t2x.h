@@ -1,0 +1,274 @@
+/*
+ * t2x.h — C ABI of the MI355X-native E-step of RELION's ML optimiser.
+ *
+ * The library (libt2x.so) replaces the accelerated expectation step of relion_refine:
+ * the per-thread E-step driver (MlOptimiserCuda), the device bundle (MlDeviceBundle),
+ * the projector textures (AccProjector) and the backprojector volumes (AccBackprojector).
+ * Every entry point takes plain pointers and sizes, returns an integer status
+ * (T2X_OK or a negative T2X_E* code) and never throws; t2x_last_error() returns a
+ * thread-local description of the last failure.
+ *
+ * Conventions shared with relion_refine's CPU path:
+ *   - Fourier images use the FFTW half layout: size x (size/2+1), row i holds
+ *     frequency ip = (i <= size/2) ? i : i - size, column j holds jp = j.
+ *   - Forward FFTs are normalised by 1/(number of real pixels) (fftw.cpp:335-358).
+ *   - Real-space images are N x N, row-major, pixel (i,j) has logical
+ *     coordinates (i - N/2, j - N/2) (Xmipp origin).
+ *   - Projector / backprojector volumes are pad x pad x (pad/2+1) arrays,
+ *     x from 0, y and z centred: index = (z + pad/2)*pad*(pad/2+1) + (y + pad/2)*(pad/2+1) + x
+ *     (projector.cpp:55-85).
+ *   - Offsets in the particle metadata are in pixels, the translational sampling
+ *     grid is in Angstrom (ml_optimiser.cpp:343-350), sigma2_offset is in A^2.
+ */
+#ifndef T2X_H
+#define T2X_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define T2X_ABI_VERSION 1
+
+enum {
+    T2X_OK = 0,
+    T2X_EINVAL = -1,      /* bad argument or unsupported configuration            */
+    T2X_ENOMEM = -2,      /* device or host allocation failed                     */
+    T2X_EDEVICE = -3,     /* HIP runtime / kernel launch failure                  */
+    T2X_ESTATE = -4,      /* call out of order (e.g. projector not set)           */
+    T2X_EZEROSUM = -5,    /* zero or NaN sum of weights (ml_optimiser.cpp:7834)   */
+    T2X_ENOSIG = -6       /* no significant coarse sample (ml_optimiser.cpp:7986) */
+};
+
+/* Which arithmetic the coarse/fine cross terms use. */
+enum {
+    T2X_PREC_FP32 = 0,        /* plain FP32 products, FP32 accumulation                */
+    T2X_PREC_BF16X3 = 1,      /* BF16 hi+lo split, 3 MFMA products, FP32 accumulation  */
+    T2X_PREC_FP16X3 = 2       /* FP16 hi+lo split, 3 MFMA products, FP32 accumulation  */
+};
+
+/* Orientational prior modes (ml_model.h NOPRIOR / PRIOR_ROTTILT_PSI). */
+enum { T2X_NOPRIOR = 0, T2X_PRIOR_ROTTILT_PSI = 1 };
+
+/* CTF parameters of one image (ctf.cpp:150-260, CTF::setValuesByGroup + initialise). */
+typedef struct t2x_ctf_params {
+    double voltage_kv;      /* acceleration voltage (kV)                 */
+    double cs_mm;           /* spherical aberration (mm)                 */
+    double q0;              /* amplitude contrast                        */
+    double defocus_u;       /* A                                         */
+    double defocus_v;       /* A                                         */
+    double defocus_angle;   /* degrees                                   */
+    double bfactor;         /* A^2                                       */
+    double phase_shift;     /* degrees                                   */
+    double scale;           /* CTF scale factor                          */
+} t2x_ctf_params;
+
+/* E-step configuration: the MlOptimiser / HealpixSampling state the E-step reads. */
+typedef struct t2x_estep_config {
+    int    ori_size;               /* box size N of the particle images                  */
+    double pixel_size;             /* A/px                                               */
+    int    current_size;           /* image_current_size (even)                          */
+    int    coarse_size;            /* image_coarse_size (even, <= current_size)          */
+    int    nr_classes;             /* K                                                  */
+    int    ref_dim;                /* 3: 3D references; 2: 2D class averages             */
+    int    healpix_order;          /* coarse HEALPix order (ref_dim 3)                   */
+    double psi_step;               /* coarse in-plane step in degrees                    */
+    double offset_range;           /* A                                                  */
+    double offset_step;            /* A                                                  */
+    int    adaptive_oversampling;  /* 0 or 1                                             */
+    double adaptive_fraction;      /* 0.999 by default                                   */
+    long   maximum_significants;   /* <= 0: unlimited                                    */
+    double particle_diameter;      /* A, circular soft mask                              */
+    int    width_mask_edge;        /* px                                                 */
+    double sigma2_fudge;           /* 1                                                  */
+    int    do_norm_correction;     /* 0/1                                                */
+    double avg_norm_correction;    /* mymodel.avg_norm_correction                        */
+    int    do_ctf_correction;      /* 0/1                                                */
+    double sigma2_offset;          /* mymodel.sigma2_offset (A^2)                        */
+    int    orientational_prior_mode; /* T2X_NOPRIOR only in this version                 */
+    int    precision;              /* T2X_PREC_* for the cross terms                     */
+    int    reserved[8];
+} t2x_estep_config;
+
+/* Model arrays of the current iteration (host, caller-owned, read-only). */
+typedef struct t2x_model {
+    const double* sigma2_noise;    /* [ori_size/2+1]            mymodel.sigma2_noise[0]  */
+    const double* pdf_class;       /* [K]                        mymodel.pdf_class       */
+    const double* pdf_direction;   /* [K * n_dir]                mymodel.pdf_direction   */
+    const double* prior_offset_class; /* [2K] x,y in px for 2D refs, may be NULL         */
+} t2x_model;
+
+/* Per-particle input metadata (one image per particle). */
+typedef struct t2x_particle_in {
+    t2x_ctf_params ctf;
+    double old_offset_x, old_offset_y;       /* px (METADATA_XOFF/YOFF)                    */
+    double prior_offset_x, prior_offset_y;   /* px, 999 = unset (METADATA_XOFF_PRIOR)     */
+    double norm_correction;                  /* METADATA_NORM                              */
+    double rot, tilt, psi;                   /* current angles (METADATA_ROT/TILT/PSI)     */
+} t2x_particle_in;
+
+/* Per-particle results (the exp_metadata row written by storeWeightedSums). */
+typedef struct t2x_particle_out {
+    double rot, tilt, psi;        /* best pose (degrees)                                  */
+    double xoff, yoff;            /* best offset (px) = rounded old offset + translation  */
+    int    iclass;                /* best class, 1-based                                  */
+    int    pad0;
+    double pmax;                  /* METADATA_PMAX                                        */
+    double dLL;                   /* METADATA_DLL                                         */
+    double norm_correction;       /* METADATA_NORM after the update                       */
+    double sum_weight;            /* exp_sum_weight after the fine pass                   */
+    double min_diff2;             /* exp_min_diff2 after the fine pass                    */
+    double significant_weight;    /* exp_significant_weight after the fine pass           */
+    long   nr_significant_coarse; /* METADATA_NR_SIGN                                     */
+    long   nr_fine_samples;       /* fine (orientation, translation) pairs scored         */
+    long   nr_fine_significant;   /* fine samples with weight >= significant_weight       */
+    long   best_hidden;           /* fine hidden index of the best pose (see DESIGN)      */
+} t2x_particle_out;
+
+/* Weighted sums the E-step adds into (host, caller-owned, double). Any member may be NULL. */
+typedef struct t2x_wsum {
+    double* sigma2_noise;         /* [ori_size/2+1] wsum_model.sigma2_noise[0]            */
+    double* sumw_group;           /* [1]                                                  */
+    double* pdf_class;            /* [K]                                                  */
+    double* pdf_direction;        /* [K * n_dir]                                          */
+    double* prior_offset_class;   /* [2K]  (2D references only)                           */
+    double* sigma2_offset;        /* [1]                                                  */
+    double* avg_norm_correction;  /* [1]                                                  */
+    double* LL;                   /* [1]                                                  */
+    double* ave_Pmax;             /* [1]                                                  */
+} t2x_wsum;
+
+/* Optional inspection buffers for parity tests (host, caller-owned; NULL = skip).
+ * Only the first n_debug particles of a call are reported. */
+typedef struct t2x_estep_debug {
+    int     n_debug;
+    float*  coarse_diff2;     /* [n_debug][K*n_dir*n_psi*n_trans] ihidden order; -999 where not computed */
+    unsigned char* coarse_significant; /* [n_debug][same] Mcoarse_significant                          */
+    long    fine_capacity;    /* capacity (entries) per particle of the two arrays below                 */
+    long*   fine_hidden;      /* [n_debug][fine_capacity] ihidden_over of each fine sample                */
+    float*  fine_diff2;       /* [n_debug][fine_capacity] fine diff2                                      */
+    long*   fine_count;       /* [n_debug] number of fine samples                                         */
+    double* coarse_min_diff2; /* [n_debug]                                                                */
+    double* coarse_sum_weight;/* [n_debug]                                                                */
+    double* coarse_sig_weight;/* [n_debug]                                                                */
+} t2x_estep_debug;
+
+/* Sampling sizes implied by a configuration (HealpixSampling::NrDirections etc.). */
+typedef struct t2x_sampling_info {
+    long nr_dir, nr_psi, nr_trans;          /* coarse                                 */
+    long nr_over_rot, nr_over_trans;        /* oversampling factors of the fine pass  */
+    long nr_coarse_per_class;               /* nr_dir * nr_psi                        */
+    long nr_hidden_coarse;                  /* K * nr_dir * nr_psi * nr_trans         */
+} t2x_sampling_info;
+
+/* Per-stage timing and work counters accumulated over t2x_estep calls (for reporting). */
+typedef struct t2x_stats {
+    int    n_calls;
+    long   n_particles;
+    double ms_preprocess, ms_coarse_project, ms_coarse_diff2, ms_coarse_weights;
+    double ms_fine_diff2, ms_fine_weights, ms_store;     /* GPU event time per stage          */
+    long   n_coarse_diff2_launches;
+    double coarse_diff2_flop;    /* algorithmic 4*T*O*P per particle, summed                  */
+    double coarse_diff2_bytes;   /* algorithmic HBM bytes of the coarse GEMM                  */
+    double fine_samples;         /* fine (o,t) pairs scored                                   */
+    double fine_flop;            /* algorithmic 4*S*P_f                                       */
+    double bp_orientations;      /* orientations backprojected                                */
+    double sig_coarse;           /* significant coarse samples                                */
+    int    coarse_size, current_size, P_coarse, P_fine;
+} t2x_stats;
+
+/* ---------------------------------------------------------------------------------------
+ * Device bundle (MlDeviceBundle, src/acc/cuda/cuda_ml_optimiser.h:18-81)
+ * ------------------------------------------------------------------------------------- */
+typedef struct t2x_device t2x_device;
+
+/* Replaces MlDeviceBundle::MlDeviceBundle + setDevice (cuda_ml_optimiser.cu:1153-1169). */
+int t2x_device_open(int device_id, t2x_device** out);
+/* Replaces the bundle teardown at the end of MlOptimiser::expectation (ml_optimiser.cpp:3505-3543). */
+int t2x_device_close(t2x_device* dev);
+/* Thread-local text of the last error (CRITICAL/RelionError text in cuda_settings.h:50-84). */
+const char* t2x_last_error(void);
+int t2x_abi_version(void);
+
+/* ---------------------------------------------------------------------------------------
+ * Projector (AccProjector, src/acc/acc_projector.h:14-87)
+ * Replaces AccProjector::setMdlDim + initMdl (acc_projector_impl.h:10-223) for class iclass:
+ * re/im are pad*pad*(pad/2+1) (3D) or pad*(pad/2+1) (2D) floats of PPref[iclass].data.
+ * --------------------------------------------------------------------------------------- */
+int t2x_projector_set(t2x_device* dev, int iclass, int ref_dim, int pad_size, int r_max,
+                      double padding_factor, const float* re, const float* im);
+
+/* ---------------------------------------------------------------------------------------
+ * Backprojector (AccBackprojector, src/acc/acc_backprojector.h:15-87)
+ * ------------------------------------------------------------------------------------- */
+/* Replaces AccBackprojector::setMdlDim + initMdl (acc_backprojector_impl.h:8-60): zeroed volumes. */
+int t2x_backprojector_init(t2x_device* dev, int iclass, int ref_dim, int pad_size, int r_max,
+                           double padding_factor);
+/* Replaces AccBackprojector::getMdlData (acc_backprojector_impl.h:84-99): device volumes -> host doubles. */
+int t2x_backprojector_get(t2x_device* dev, int iclass, double* re, double* im, double* weight);
+/* Replaces AccBackprojector::clear (acc_backprojector_impl.h:101-143): zero the volumes. */
+int t2x_backprojector_clear(t2x_device* dev, int iclass);
+
+/* ---------------------------------------------------------------------------------------
+ * Sampling helpers (HealpixSampling, src/healpix_sampling.cpp)
+ * ------------------------------------------------------------------------------------- */
+int t2x_sampling_info_get(const t2x_estep_config* cfg, t2x_sampling_info* info);
+
+/* ---------------------------------------------------------------------------------------
+ * E-step (MlOptimiserCuda::doThreadExpectationSomeParticles, cuda_ml_optimiser.cu:1345-1408,
+ * and its CPU twin MlOptimiser::expectationOneParticle, ml_optimiser.cpp:4252-4608).
+ *
+ * Runs getFourierTransformsAndCtfs, both sampling passes (getAllSquaredDifferences +
+ * convertAllSquaredDifferencesToWeights) and storeWeightedSums for n particles.
+ * images: n * ori_size^2 floats, host memory (images_on_device = 0) or device memory (1).
+ * Backprojected slices are added into the device backprojector of each class; the other
+ * weighted sums are added into *wsum; per-particle metadata goes to out[0..n).
+ * --------------------------------------------------------------------------------------- */
+int t2x_estep(t2x_device* dev, const t2x_estep_config* cfg, const t2x_model* model,
+              int n, const float* images, int images_on_device,
+              const t2x_particle_in* pin, t2x_particle_out* out,
+              t2x_wsum* wsum, t2x_estep_debug* dbg);
+
+/* Accumulated timing / work counters since the last reset. */
+int t2x_stats_get(t2x_device* dev, t2x_stats* st);
+int t2x_stats_reset(t2x_device* dev);
+
+/* ---------------------------------------------------------------------------------------
+ * Device memory helpers for callers that keep inputs resident in HBM.
+ * ------------------------------------------------------------------------------------- */
+int t2x_device_alloc(t2x_device* dev, size_t bytes, void** dptr);
+int t2x_device_free(t2x_device* dev, void* dptr);
+int t2x_memcpy_h2d(t2x_device* dev, void* dst, const void* src, size_t bytes);
+int t2x_memcpy_d2h(t2x_device* dev, void* dst, const void* src, size_t bytes);
+int t2x_device_synchronize(t2x_device* dev);
+
+/* ---------------------------------------------------------------------------------------
+ * Synthetic workloads (Gaussian-blob volumes, CTF-modulated noisy projections) synthesised on
+ * the GPU from caller-supplied blob lists and per-particle parameters.
+ * ------------------------------------------------------------------------------------- */
+typedef struct t2x_blob { double x, y, z, sigma, amplitude; } t2x_blob; /* px, centred */
+
+/* Padded Fourier reference of a blob volume (analytic transform on the projector grid), low-passed
+ * to lowpass_A (<= 0: none) with a raised-cosine edge of width 2 shells. Output float re/im of
+ * pad*pad*(pad/2+1) on the DEVICE (dre/dim device pointers), scaled like Projector::computeFourierTransformMap. */
+int t2x_synth_reference_device(t2x_device* dev, int ori_size, double pixel_size, int pad_size, int r_max,
+                               double padding_factor, const t2x_blob* blobs, int n_blobs,
+                               double lowpass_A, float* dre, float* dim);
+/* Particle images on the device: projection of blob set `class_blobs[true_class]` at the true pose,
+ * CTF-modulated, shifted by (true_x, true_y) px, white noise with variance var(signal)/snr[i]
+ * (var over the whole box); the noise of pixel q of particle i is the counter-based Gaussian
+ * sample (noise_seed, i, q). dimages: device n*ori_size^2 floats. */
+int t2x_synth_particles_device(t2x_device* dev, int ori_size, double pixel_size, int n,
+                               const t2x_blob* blobs, const int* blob_offsets, /* [K+1] */
+                               const t2x_particle_in* pin, const double* true_rot,
+                               const double* true_tilt, const double* true_psi,
+                               const double* true_x, const double* true_y, const int* true_class,
+                               const double* snr, uint64_t noise_seed, float* dimages,
+                               double* noise_var /* [n] host, may be NULL */);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* T2X_H */

@@ -1,0 +1,269 @@
+// t2x_device.hip — device bundle, projector and backprojector volumes, memory helpers.
+//
+// Replaces MlDeviceBundle (src/acc/cuda/cuda_ml_optimiser.h:18-81, .cu:1153-1313),
+// AccProjector::setMdlDim/initMdl (src/acc/acc_projector_impl.h:10-223) and
+// AccBackprojector::setMdlDim/initMdl/getMdlData/clear (src/acc/acc_backprojector_impl.h:8-143).
+// Volumes live in plain HBM arrays (no textures): the E-step kernels do their own exact
+// trilinear interpolation, as the CPU path does (projector.cpp:585-640).
+#include "t2x_internal.h"
+
+namespace t2x {
+
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+
+}  // namespace t2x
+
+using namespace t2x;
+
+extern "C" const char* t2x_last_error(void) { return g_last_error.c_str(); }
+extern "C" int t2x_abi_version(void) { return T2X_ABI_VERSION; }
+
+extern "C" int t2x_device_open(int device_id, t2x_device** out)
+{
+    if (!out) { set_error("null output handle"); return T2X_EINVAL; }
+    *out = nullptr;
+    int n = 0;
+    T2X_HIP(hipGetDeviceCount(&n));
+    if (device_id < 0 || device_id >= n) {
+        set_error("device id " + std::to_string(device_id) + " out of range (" + std::to_string(n) + " devices)");
+        return T2X_EINVAL;
+    }
+    T2X_HIP(hipSetDevice(device_id));
+    auto* d = new Device();
+    d->id = device_id;
+    if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete d;
+        set_error("hipStreamCreate failed");
+        return T2X_EDEVICE;
+    }
+    for (auto& e : d->ev) (void)hipEventCreate(&e);
+    *out = reinterpret_cast<t2x_device*>(d);
+    return T2X_OK;
+}
+
+static void release_class(ClassVolumes& c)
+{
+    if (c.proj) (void)hipFree(c.proj);
+    if (c.bp_data) (void)hipFree(c.bp_data);
+    if (c.bp_weight) (void)hipFree(c.bp_weight);
+    c.proj = nullptr;
+    c.bp_data = nullptr;
+    c.bp_weight = nullptr;
+}
+
+extern "C" int t2x_device_close(t2x_device* h)
+{
+    if (!h) return T2X_OK;
+    Device* d = reinterpret_cast<Device*>(h);
+    (void)hipSetDevice(d->id);
+    (void)hipStreamSynchronize(d->stream);
+    for (auto& kv : d->classes) release_class(kv.second);
+    DevBuf* bufs[] = {&d->img, &d->img2, &d->fft_out, &d->Fimg_cur, &d->Fimg_nm_cur, &d->Fctf_cur,
+                      &d->pscal, &d->power, &d->A, &d->A1, &d->N1, &d->diff2, &d->minb,
+                      &d->sel_out, &d->prior_o, &d->prior_t, &d->sig_pairs, &d->sig_count,
+                      &d->sig_orient, &d->sig_orient_count, &d->orient_pairs_off, &d->fine_items,
+                      &d->fine_diff2, &d->fine_min, &d->fine_sel, &d->ab_fine, &d->ab_coarse,
+                      &d->wsum_dev, &d->store_items, &d->best, &d->tables, &d->Y_fine,
+                      &d->dbg_tmp, &d->coarse_B, &d->coarse_B2};
+    for (DevBuf* b : bufs) b->release();
+    PassTables* tabs[] = {&d->coarse_tab, &d->fine_tab};
+    for (PassTables* t : tabs) {
+        t->d_pix.release(); t->d_win.release(); t->d_minv.release(); t->d_ires.release(); t->d_inproj.release();
+    }
+    if (d->fft_plan) (void)hipfftDestroy(d->fft_plan);
+    for (auto& e : d->ev) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(d->stream);
+    delete d;
+    return T2X_OK;
+}
+
+static size_t vol_elems(int ref_dim, int pad)
+{
+    size_t xdim = pad / 2 + 1;
+    return ref_dim == 3 ? (size_t)pad * pad * xdim : (size_t)pad * xdim;
+}
+
+static int check_dims(int iclass, int ref_dim, int pad, int r_max, double pf)
+{
+    if (iclass < 0) { set_error("negative class index"); return T2X_EINVAL; }
+    if (ref_dim != 3) { set_error("only 3D references are supported"); return T2X_EINVAL; }
+    if (pad < 3 || !(pad & 1)) { set_error("pad_size must be odd"); return T2X_EINVAL; }
+    if (r_max < 1 || pf <= 0) { set_error("bad r_max or padding factor"); return T2X_EINVAL; }
+    long need = 2L * ((long)std::lround(pf * r_max) + 1) + 1;
+    if (pad < need) { set_error("pad_size smaller than 2*(ROUND(pf*r_max)+1)+1"); return T2X_EINVAL; }
+    return T2X_OK;
+}
+
+__global__ void k_interleave(const float* __restrict__ re, const float* __restrict__ im,
+                             float2* __restrict__ out, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) out[i] = make_float2(re[i], im[i]);
+}
+
+extern "C" int t2x_projector_set(t2x_device* h, int iclass, int ref_dim, int pad_size, int r_max,
+                                 double pf, const float* re, const float* im)
+{
+    if (!h || !re || !im) { set_error("null argument"); return T2X_EINVAL; }
+    T2X_TRY(check_dims(iclass, ref_dim, pad_size, r_max, pf));
+    Device* d = reinterpret_cast<Device*>(h);
+    std::lock_guard<std::mutex> lk(d->mtx);
+    T2X_HIP(hipSetDevice(d->id));
+    ClassVolumes& c = d->classes[iclass];
+    size_t n = vol_elems(ref_dim, pad_size);
+    if (c.proj && (c.p_pad != pad_size || c.p_ref_dim != ref_dim)) {
+        (void)hipFree(c.proj);
+        c.proj = nullptr;
+    }
+    if (!c.proj) {
+        if (hipMalloc(&c.proj, n * sizeof(float2)) != hipSuccess) {
+            set_error("hipMalloc projector volume failed");
+            return T2X_ENOMEM;
+        }
+    }
+    c.p_ref_dim = ref_dim;
+    c.p_pad = pad_size;
+    c.p_rmax = r_max;
+    c.p_pf = pf;
+    c.proj_version = ++d->proj_counter;
+    d->coarse_key.clear();
+    // Stage the two planes and interleave on the device. Pointers may be host or device memory.
+    float *dre = nullptr, *dim = nullptr;
+    if (hipMalloc(&dre, n * sizeof(float)) != hipSuccess || hipMalloc(&dim, n * sizeof(float)) != hipSuccess) {
+        if (dre) (void)hipFree(dre);
+        set_error("hipMalloc staging failed");
+        return T2X_ENOMEM;
+    }
+    T2X_HIP(hipMemcpyAsync(dre, re, n * sizeof(float), hipMemcpyDefault, d->stream));
+    T2X_HIP(hipMemcpyAsync(dim, im, n * sizeof(float), hipMemcpyDefault, d->stream));
+    k_interleave<<<(unsigned)((n + 255) / 256), 256, 0, d->stream>>>(dre, dim, c.proj, n);
+    T2X_HIP(hipGetLastError());
+    T2X_HIP(hipStreamSynchronize(d->stream));
+    (void)hipFree(dre);
+    (void)hipFree(dim);
+    return T2X_OK;
+}
+
+extern "C" int t2x_backprojector_init(t2x_device* h, int iclass, int ref_dim, int pad_size,
+                                      int r_max, double pf)
+{
+    if (!h) { set_error("null handle"); return T2X_EINVAL; }
+    T2X_TRY(check_dims(iclass, ref_dim, pad_size, r_max, pf));
+    Device* d = reinterpret_cast<Device*>(h);
+    std::lock_guard<std::mutex> lk(d->mtx);
+    T2X_HIP(hipSetDevice(d->id));
+    ClassVolumes& c = d->classes[iclass];
+    size_t n = vol_elems(ref_dim, pad_size);
+    if (c.bp_data && (c.b_pad != pad_size || c.b_ref_dim != ref_dim)) {
+        (void)hipFree(c.bp_data);
+        (void)hipFree(c.bp_weight);
+        c.bp_data = nullptr;
+        c.bp_weight = nullptr;
+    }
+    if (!c.bp_data) {
+        if (hipMalloc(&c.bp_data, n * sizeof(float2)) != hipSuccess ||
+            hipMalloc(&c.bp_weight, n * sizeof(float)) != hipSuccess) {
+            set_error("hipMalloc backprojector volumes failed");
+            return T2X_ENOMEM;
+        }
+    }
+    c.b_ref_dim = ref_dim;
+    c.b_pad = pad_size;
+    c.b_rmax = r_max;
+    c.b_pf = pf;
+    T2X_HIP(hipMemsetAsync(c.bp_data, 0, n * sizeof(float2), d->stream));
+    T2X_HIP(hipMemsetAsync(c.bp_weight, 0, n * sizeof(float), d->stream));
+    T2X_HIP(hipStreamSynchronize(d->stream));
+    return T2X_OK;
+}
+
+extern "C" int t2x_backprojector_clear(t2x_device* h, int iclass)
+{
+    if (!h) { set_error("null handle"); return T2X_EINVAL; }
+    Device* d = reinterpret_cast<Device*>(h);
+    std::lock_guard<std::mutex> lk(d->mtx);
+    auto it = d->classes.find(iclass);
+    if (it == d->classes.end() || !it->second.bp_data) { set_error("backprojector not initialised"); return T2X_ESTATE; }
+    ClassVolumes& c = it->second;
+    size_t n = vol_elems(c.b_ref_dim, c.b_pad);
+    T2X_HIP(hipSetDevice(d->id));
+    T2X_HIP(hipMemsetAsync(c.bp_data, 0, n * sizeof(float2), d->stream));
+    T2X_HIP(hipMemsetAsync(c.bp_weight, 0, n * sizeof(float), d->stream));
+    T2X_HIP(hipStreamSynchronize(d->stream));
+    return T2X_OK;
+}
+
+extern "C" int t2x_backprojector_get(t2x_device* h, int iclass, double* re, double* im, double* w)
+{
+    if (!h) { set_error("null handle"); return T2X_EINVAL; }
+    Device* d = reinterpret_cast<Device*>(h);
+    std::lock_guard<std::mutex> lk(d->mtx);
+    auto it = d->classes.find(iclass);
+    if (it == d->classes.end() || !it->second.bp_data) { set_error("backprojector not initialised"); return T2X_ESTATE; }
+    ClassVolumes& c = it->second;
+    size_t n = vol_elems(c.b_ref_dim, c.b_pad);
+    T2X_HIP(hipSetDevice(d->id));
+    T2X_HIP(hipStreamSynchronize(d->stream));
+    std::vector<float2> hd(n);
+    std::vector<float> hw(n);
+    T2X_HIP(hipMemcpy(hd.data(), c.bp_data, n * sizeof(float2), hipMemcpyDeviceToHost));
+    T2X_HIP(hipMemcpy(hw.data(), c.bp_weight, n * sizeof(float), hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) {
+        if (re) re[i] = hd[i].x;
+        if (im) im[i] = hd[i].y;
+        if (w) w[i] = hw[i];
+    }
+    return T2X_OK;
+}
+
+extern "C" int t2x_stats_get(t2x_device* h, t2x_stats* st)
+{
+    if (!h || !st) { set_error("null argument"); return T2X_EINVAL; }
+    *st = reinterpret_cast<Device*>(h)->stats.s;
+    return T2X_OK;
+}
+
+extern "C" int t2x_stats_reset(t2x_device* h)
+{
+    if (!h) { set_error("null handle"); return T2X_EINVAL; }
+    reinterpret_cast<Device*>(h)->stats.s = t2x_stats{};
+    return T2X_OK;
+}
+
+extern "C" int t2x_device_alloc(t2x_device* h, size_t bytes, void** p)
+{
+    if (!h || !p) { set_error("null argument"); return T2X_EINVAL; }
+    T2X_HIP(hipSetDevice(reinterpret_cast<Device*>(h)->id));
+    if (hipMalloc(p, bytes) != hipSuccess) { set_error("hipMalloc failed"); return T2X_ENOMEM; }
+    return T2X_OK;
+}
+
+extern "C" int t2x_device_free(t2x_device* h, void* p)
+{
+    if (!h) { set_error("null handle"); return T2X_EINVAL; }
+    T2X_HIP(hipFree(p));
+    return T2X_OK;
+}
+
+extern "C" int t2x_memcpy_h2d(t2x_device* h, void* dst, const void* src, size_t bytes)
+{
+    if (!h) { set_error("null handle"); return T2X_EINVAL; }
+    T2X_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return T2X_OK;
+}
+
+extern "C" int t2x_memcpy_d2h(t2x_device* h, void* dst, const void* src, size_t bytes)
+{
+    if (!h) { set_error("null handle"); return T2X_EINVAL; }
+    T2X_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return T2X_OK;
+}
+
+extern "C" int t2x_device_synchronize(t2x_device* h)
+{
+    if (!h) { set_error("null handle"); return T2X_EINVAL; }
+    Device* d = reinterpret_cast<Device*>(h);
+    T2X_HIP(hipStreamSynchronize(d->stream));
+    return T2X_OK;
+}

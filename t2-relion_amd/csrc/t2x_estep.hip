@@ -1,0 +1,1337 @@
+// t2x_estep.hip — the E-step driver and its per-particle kernels.
+//
+// One t2x_estep call replaces MlOptimiserCuda::doThreadExpectationSomeParticles
+// (src/acc/cuda/cuda_ml_optimiser.cu:1345-1408) for a whole set of particles: instead of a
+// per-thread task graph interleaving 4 particles, particles are processed in large batches
+// with one launch per stage, sized for HBM:
+//   1. preprocessing (getFourierTransformsAndCtfs, ml_optimiser.cpp:5689-6593)
+//   2. coarse pass: slices of all coarse orientations projected once per call and reused by
+//      every particle; diff2 for all (particle, translation) x (class, orientation) as one
+//      FP32 MFMA GEMM (getAllSquaredDifferences ipass 0, ml_optimiser.cpp:6922-7507)
+//   3. FP64 weights + exact radix-select significance (convertAllSquaredDifferencesToWeights)
+//   4. fine pass over the significant coarse pairs (ipass 1)
+//   5. FP64 weights + significance of the fine samples, best pose
+//   6. weighted sums + backprojection (storeWeightedSums, ml_optimiser.cpp:8043-8992)
+#include <algorithm>
+#include <numeric>
+
+#include "t2x_kernels.h"
+
+namespace t2x {
+
+enum { EPI_STORE = 0, EPI_DIFF2 = 1 };
+struct EpiParams {
+    float* C;
+    long long ldc;
+    int M, N;
+    int T;
+    const double* Cb;
+    const float* N1;
+    long long ldn;
+    const double* prior_o;
+    float* rowmin;
+    int rowmin_ld;
+};
+int launch_gemm(int epi, const float* A, long long lda, const float* B, long long ldb, int K,
+                int Mrows_pad, int Nrows_pad, const EpiParams& ep, hipStream_t st);
+
+__global__ void k_pass_min(WeightSrc s, int B, SelResult* res);
+__global__ void k_select(WeightSrc s, SelParams prm, int B, SelResult* res);
+__global__ void k_sig_count(WeightSrc s, int B, const SelResult* res, long long* counts);
+__global__ void k_sig_write(WeightSrc s, int B, const SelResult* res, const long long* pair_off,
+                            const long long* orient_off, int2* pairs, int* orients,
+                            long long* orient_pair_start);
+__global__ void k_fine_argmax(WeightSrc s, int B, SelResult* res, long long* nfine_sig);
+
+static const double kPi = 3.14159265358979323846;
+
+static inline long long round_up(long long x, long long m) { return (x + m - 1) / m * m; }
+__host__ __device__ static inline int relion_round(double x) { return (x > 0) ? (int)(x + 0.5) : (int)(x - 0.5); }
+
+// ------------------------------------------------------------------------------------------
+// Preprocessing kernels
+// ------------------------------------------------------------------------------------------
+
+// Norm correction, integer re-centring with zero fill (selfTranslate DONT_WRAP on the rounded
+// old offset, ml_optimiser.cpp:6097-6098) and the soft circular mask with the average
+// background value (softMaskOutsideMap, mask.cpp:43-95). Writes the unmasked image to
+// out[b] and the masked image to out[B + b].
+__global__ __launch_bounds__(256) void k_prep_real(const float* __restrict__ img, int N, int B,
+                                                   const PartScalars* __restrict__ ps,
+                                                   double radius, double cosine_width,
+                                                   float* __restrict__ out)
+{
+    const int b = blockIdx.x;
+    const size_t NN = (size_t)N * N;
+    const float* src = img + (size_t)b * NN;
+    float* um = out + (size_t)b * NN;
+    float* mk = out + (size_t)(B + b) * NN;
+    const int sx = (int)ps[b].old_x, sy = (int)ps[b].old_y;
+    const float scale = (float)ps[b].norm_scale;
+    const double radius_p = radius + cosine_width;
+    double sbg = 0., sw = 0.;
+    for (size_t q = threadIdx.x; q < NN; q += blockDim.x) {
+        int i = (int)(q / N), j = (int)(q % N);
+        int si = i - sy, sj = j - sx;
+        float v = 0.f;
+        if (si >= 0 && si < N && sj >= 0 && sj < N) v = src[(size_t)si * N + sj] * scale;
+        um[q] = v;
+        double y = i - N / 2, x = j - N / 2;
+        double r = sqrt(x * x + y * y);
+        if (r < radius) continue;
+        if (r > radius_p) { sw += 1.; sbg += v; }
+        else {
+            double rc = 0.5 + 0.5 * cos(kPi * (radius_p - r) / cosine_width);
+            sw += rc;
+            sbg += rc * v;
+        }
+    }
+    __shared__ double r1[4], r2[4];
+    for (int o = 32; o > 0; o >>= 1) { sbg += __shfl_xor(sbg, o); sw += __shfl_xor(sw, o); }
+    if ((threadIdx.x & 63) == 0) { r1[threadIdx.x >> 6] = sbg; r2[threadIdx.x >> 6] = sw; }
+    __syncthreads();
+    __shared__ double bg;
+    if (threadIdx.x == 0) bg = (r1[0] + r1[1] + r1[2] + r1[3]) / (r2[0] + r2[1] + r2[2] + r2[3]);
+    __syncthreads();
+    for (size_t q = threadIdx.x; q < NN; q += blockDim.x) {
+        int i = (int)(q / N), j = (int)(q % N);
+        double y = i - N / 2, x = j - N / 2;
+        double r = sqrt(x * x + y * y);
+        float v = um[q];
+        if (r < radius) mk[q] = v;
+        else if (r > radius_p) mk[q] = (float)bg;
+        else {
+            double rc = 0.5 + 0.5 * cos(kPi * (radius_p - r) / cosine_width);
+            mk[q] = (float)((1 - rc) * v + rc * bg);
+        }
+    }
+}
+
+// From the two full transforms (unnormalised R2C of unmasked and masked images) to the
+// current-size windows with the RELION normalisation and CenterFFTbySign, the CTF image,
+// and the power spectrum / high-resolution Xi2 of the masked image (ml_optimiser.cpp:6209-6250).
+__global__ __launch_bounds__(256) void k_prep_fourier(const float2* __restrict__ F, int N, int B, int cs,
+                                                      const CtfConst* __restrict__ ctf, double angpix,
+                                                      int do_ctf, float2* __restrict__ Fimg,
+                                                      float2* __restrict__ Fimg_nm,
+                                                      float* __restrict__ Fctf,
+                                                      double* __restrict__ power,
+                                                      PartScalars* __restrict__ ps)
+{
+    extern __shared__ double spec[];   // [N/2+1]
+    const int b = blockIdx.x;
+    const int xN = N / 2 + 1, xc = cs / 2 + 1;
+    const float2* Fu = F + (size_t)b * N * xN;
+    const float2* Fm = F + (size_t)(B + b) * N * xN;
+    const double inv = 1.0 / ((double)N * N);
+    for (int i = threadIdx.x; i < xN; i += blockDim.x) spec[i] = 0.;
+    __syncthreads();
+    const CtfConst c = ctf[b];
+    for (int q = threadIdx.x; q < cs * xc; q += blockDim.x) {
+        int i = q / xc, j = q % xc;
+        int ip = (i < xc) ? i : i - cs;
+        int ii = ip < 0 ? ip + N : ip;
+        float sgn = ((i + j) & 1) ? -1.f : 1.f;
+        float2 u = Fu[(size_t)ii * xN + j], m = Fm[(size_t)ii * xN + j];
+        Fimg_nm[(size_t)b * cs * xc + q] = make_float2((float)(u.x * inv) * sgn, (float)(u.y * inv) * sgn);
+        Fimg[(size_t)b * cs * xc + q] = make_float2((float)(m.x * inv) * sgn, (float)(m.y * inv) * sgn);
+        double v = 1.0;
+        if (do_ctf) v = ctf_value(c, j / (N * angpix), ip / (N * angpix));
+        Fctf[(size_t)b * cs * xc + q] = (float)v;
+    }
+    double hres = 0.;
+    if (cs < N) {
+        for (int q = threadIdx.x; q < N * xN; q += blockDim.x) {
+            int i = q / xN, j = q % xN;
+            int ip = (i < xN) ? i : i - N;
+            int ires = relion_round(sqrt((double)(ip * ip + j * j)));
+            if (ires > 0 && ires < xN && !(j == 0 && ip < 0)) {
+                float2 m = Fm[q];
+                double nf = ((double)m.x * m.x + (double)m.y * m.y) * inv * inv;
+                atomicAdd(&spec[ires], nf);
+                if (ires >= cs / 2 + 1) hres += nf;
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) hres += __shfl_xor(hres, o);
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = hres;
+    __syncthreads();
+    if (threadIdx.x == 0) ps[b].highres_Xi2 = red[0] + red[1] + red[2] + red[3];
+    for (int i = threadIdx.x; i < xN; i += blockDim.x) power[(size_t)b * xN + i] = spec[i];
+}
+
+// Constant terms C_b = highres/2 + 1/2 sum_p minv |X|^2 for one pass table.
+__global__ __launch_bounds__(256) void k_const_term(const float2* __restrict__ Fimg, int Pw,
+                                                    const int* __restrict__ win,
+                                                    const float* __restrict__ minv, int npix,
+                                                    PartScalars* __restrict__ ps, int which)
+{
+    const int b = blockIdx.x;
+    double s = 0.;
+    for (int p = threadIdx.x; p < npix; p += blockDim.x) {
+        float2 x = Fimg[(size_t)b * Pw + win[p]];
+        s += 0.5 * (double)minv[p] * ((double)x.x * x.x + (double)x.y * x.y);
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    __shared__ double red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double v = ps[b].highres_Xi2 / 2. + red[0] + red[1] + red[2] + red[3];
+        if (which == 0) ps[b].C_coarse = v;
+        else ps[b].C_fine = v;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Coarse pass
+// ------------------------------------------------------------------------------------------
+
+// Slices of all coarse orientations (Projector::project, projector.cpp:524-691): one workgroup
+// per (class, orientation), threads over the compact coarse pixels. Writes re/im interleaved
+// into B[n][2p..2p+1] and |F|^2 into B2[n][p].
+__global__ __launch_bounds__(256) void k_project_coarse(const float2* const* __restrict__ vols,
+                                                        const int* __restrict__ pads,
+                                                        const float* __restrict__ Ai,   // [Norient][9]
+                                                        int Norient, const int2* __restrict__ pix,
+                                                        const unsigned char* __restrict__ inproj,
+                                                        int npix, float* __restrict__ Bm, long long ldb,
+                                                        float* __restrict__ B2, long long ldb2)
+{
+    const int n = blockIdx.x;
+    const int c = n / Norient, o = n - c * Norient;
+    const float2* vol = vols[c];
+    const int pad = pads[c];
+    float a[6];
+    for (int i = 0; i < 6; i++) a[i] = Ai[o * 9 + (i / 2) * 3 + (i % 2)];  // rows 0..2, cols 0..1
+    for (int p = threadIdx.x; p < npix; p += blockDim.x) {
+        float2 f = make_float2(0.f, 0.f);
+        if (inproj[p]) {
+            float x = (float)pix[p].x, y = (float)pix[p].y;
+            float xp = a[0] * x + a[1] * y, yp = a[2] * x + a[3] * y, zp = a[4] * x + a[5] * y;
+            float2 v;
+            if (trilinear_sample(vol, pad, xp, yp, zp, &v)) f = v;
+        }
+        Bm[(size_t)n * ldb + 2 * p] = f.x;
+        Bm[(size_t)n * ldb + 2 * p + 1] = f.y;
+        B2[(size_t)n * ldb2 + p] = f.x * f.x + f.y * f.y;
+    }
+}
+
+// Image side of the coarse GEMM: A[(b,t)][2p..] = minv ctf X_t and A1[b][p] = minv ctf^2 / 2,
+// with X_t = X * exp(-2 pi i (x sx + y sy)/N) (shiftImageInFourierTransform, fftw.cpp:842-930).
+__global__ __launch_bounds__(256) void k_coarse_imgside(const float2* __restrict__ Fimg,
+                                                        const float* __restrict__ Fctf, int Pw,
+                                                        const int* __restrict__ win,
+                                                        const float* __restrict__ minv, int npix,
+                                                        const float2* __restrict__ ab, int T,
+                                                        float* __restrict__ A, long long lda,
+                                                        float* __restrict__ A1, long long lda1)
+{
+    const int b = blockIdx.y, t = blockIdx.x;
+    for (int p = threadIdx.x; p < npix; p += blockDim.x) {
+        const size_t w = (size_t)b * Pw + win[p];
+        float2 x = Fimg[w];
+        float ctf = Fctf[w], m = minv[p];
+        float s = m * ctf;
+        float2 sh = ab[(size_t)t * npix + p];
+        float xr = x.x * sh.x - x.y * sh.y, xi = x.x * sh.y + x.y * sh.x;
+        A[((size_t)b * T + t) * lda + 2 * p] = s * xr;
+        A[((size_t)b * T + t) * lda + 2 * p + 1] = s * xi;
+        if (t == 0) A1[(size_t)b * lda1 + p] = 0.5f * m * ctf * ctf;
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Fine pass
+// ------------------------------------------------------------------------------------------
+
+struct FineGeom {
+    int Norient, Npsi, n_subdir, n_subpsi, over_rot, over_trans, T;
+    const double* sub_rot;
+    const double* sub_tilt;
+    const double* sub_psi;
+    double pf;
+};
+
+__device__ __forceinline__ int find_particle(const long long* off, int B, long long i)
+{
+    int lo = 0, hi = B;   // off[lo] <= i < off[hi]
+    while (hi - lo > 1) {
+        int mid = (lo + hi) >> 1;
+        if (off[mid] <= i) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ void fine_matrix(const FineGeom& g, int n, int ir, double* A, int* cls, int* idir)
+{
+    const int c = n / g.Norient, o = n - c * g.Norient;
+    const int d = o / g.Npsi, ps = o - d * g.Npsi;
+    const int sd = ir / g.n_subpsi, sp = ir - sd * g.n_subpsi;
+    euler_matrix(g.sub_rot[(size_t)d * g.n_subdir + sd], g.sub_tilt[(size_t)d * g.n_subdir + sd],
+                 g.sub_psi[(size_t)ps * g.n_subpsi + sp], A);
+    *cls = c;
+    *idir = d;
+}
+
+#define FINE_MAX_PIX 6144
+
+// Fine diff2 of every significant coarse pair: one workgroup per (particle, significant
+// orientation, oversampled rotation). The slice at current size is projected into LDS as
+// Z = F conj(minv ctf X); each wave then scores one oversampled translation at a time,
+// G_t = sum_p Re(Z conj(AB_t)), diff2 = C_b + 1/2 sum minv ctf^2 |F|^2 - G_t.
+__global__ __launch_bounds__(256) void k_fine_diff2(
+    const long long* __restrict__ item_off, int B, const long long* __restrict__ orient_off,
+    const int* __restrict__ orients, const long long* __restrict__ orient_pair_start,
+    const long long* __restrict__ pair_off, const int2* __restrict__ pairs, FineGeom g,
+    const float2* const* __restrict__ vols, const int* __restrict__ pads,
+    const int2* __restrict__ pix, const unsigned char* __restrict__ inproj,
+    const float* __restrict__ minv, const int* __restrict__ win, int npix,
+    const float2* __restrict__ Fimg, const float* __restrict__ Fctf, int Pw,
+    const float2* __restrict__ ab_fine, const PartScalars* __restrict__ ps,
+    float* __restrict__ fine_d2, unsigned int* __restrict__ fine_min)
+{
+    __shared__ float2 Z[FINE_MAX_PIX];
+    __shared__ double red[4];
+    __shared__ float wmin[4];
+    const long long item = blockIdx.x;
+    const int b = find_particle(item_off, B, item);
+    const long long local = item - item_off[b];
+    const long long j = orient_off[b] + local / g.over_rot;
+    const int ir = (int)(local % g.over_rot);
+    const int n = orients[j];
+    const long long p0 = orient_pair_start[j];
+    const long long p1 = (j + 1 < orient_off[b + 1]) ? orient_pair_start[j + 1] : pair_off[b + 1];
+    double A[9];
+    int c, idir;
+    fine_matrix(g, n, ir, A, &c, &idir);
+    const float pf = (float)g.pf;
+    const float a0 = (float)A[0] * pf, a1 = (float)A[3] * pf;   // Ai = A^T * pf
+    const float a2 = (float)A[1] * pf, a3 = (float)A[4] * pf;
+    const float a4 = (float)A[2] * pf, a5 = (float)A[5] * pf;
+    const float2* vol = vols[c];
+    const int pad = pads[c];
+    double n1 = 0.;
+    for (int p = threadIdx.x; p < npix; p += blockDim.x) {
+        float2 f = make_float2(0.f, 0.f);
+        if (inproj[p]) {
+            float x = (float)pix[p].x, y = (float)pix[p].y;
+            float2 v;
+            if (trilinear_sample(vol, pad, a0 * x + a1 * y, a2 * x + a3 * y, a4 * x + a5 * y, &v)) f = v;
+        }
+        const size_t w = (size_t)b * Pw + win[p];
+        float2 X = Fimg[w];
+        float ctf = Fctf[w], m = minv[p];
+        float s = m * ctf;
+        float yr = s * X.x, yi = s * X.y;       // Y = minv ctf X
+        Z[p] = make_float2(f.x * yr + f.y * yi, f.y * yr - f.x * yi);   // F conj(Y)
+        n1 += 0.5 * (double)m * (double)ctf * ctf * ((double)f.x * f.x + (double)f.y * f.y);
+    }
+    for (int o = 32; o > 0; o >>= 1) n1 += __shfl_xor(n1, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n1;
+    __syncthreads();
+    const double base = ps[b].C_fine + red[0] + red[1] + red[2] + red[3];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int nt = (int)(p1 - p0) * g.over_trans;
+    float lmin = 3.0e38f;
+    for (int q = wave; q < nt; q += 4) {
+        const long long k = p0 + q / g.over_trans;
+        const int iot = q % g.over_trans;
+        const int ft = pairs[k].y * g.over_trans + iot;
+        const float2* abt = ab_fine + (size_t)ft * npix;
+        float acc = 0.f;
+        for (int p = lane; p < npix; p += 64) {
+            float2 z = Z[p], s = abt[p];
+            acc += z.x * s.x + z.y * s.y;
+        }
+        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+        if (lane == 0) {
+            float d = (float)(base - (double)acc);
+            fine_d2[k * (g.over_rot * g.over_trans) + ir * g.over_trans + iot] = d;
+            lmin = fminf(lmin, d);
+        }
+    }
+    if (lane == 0) wmin[wave] = lmin;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float m = fminf(fminf(wmin[0], wmin[1]), fminf(wmin[2], wmin[3]));
+        if (m < 3.0e38f) atomicMin(&fine_min[b], float_to_ordered(m));
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Weighted sums and backprojection (storeWeightedSums, ml_optimiser.cpp:8043-8992;
+// BackProjector::backproject2Dto3D, backprojector.cpp:55-357)
+// ------------------------------------------------------------------------------------------
+
+struct StoreParams {
+    FineGeom g;
+    const float2* const* vols;
+    const int* pads;
+    float2* const* bp_data;
+    float* const* bp_weight;
+    const int* bp_pads;
+    const int* bp_rmax;
+    const double* prior_o;
+    const double* prior_t;
+    const int2* pix;
+    const unsigned char* inproj;
+    const float* minv_store;   // Minvsigma2 with the origin term (ml_optimiser.cpp:8099-8104)
+    const int* win;
+    const int* ires;
+    int npix;                  // store pixel set: every pixel with ires >= 0
+    const float2* Fimg;
+    const float2* Fimg_nm;
+    const float* Fctf;
+    int Pw;
+    const float2* ab_fine;     // [T*over_trans][npix] (same pixel order)
+    const double* ftx;
+    const double* fty;         // fine translations in px
+    const PartScalars* ps;
+    const SelResult* fine_res;
+    const float* fine_d2;
+    double pixel_size;
+    int nshell;                // N/2+1
+    int cs;
+    // accumulators
+    double* w_sigma2;          // [nshell]
+    double* w_sumw;            // [1]
+    double* w_pdf_class;       // [K]
+    double* w_pdf_dir;         // [K*Nd]
+    double* w_sigma2_offset;   // [1]
+    double* w_norm;            // [B] per particle sum of wdiff2
+    int Nd;
+};
+
+#define STORE_MAX_T 1024
+
+__global__ __launch_bounds__(256) void k_store(const long long* __restrict__ item_off, int B,
+                                               const long long* __restrict__ orient_off,
+                                               const int* __restrict__ orients,
+                                               const long long* __restrict__ orient_pair_start,
+                                               const long long* __restrict__ pair_off,
+                                               const int2* __restrict__ pairs, StoreParams P)
+{
+    __shared__ int s_ft[STORE_MAX_T];
+    __shared__ float s_w[STORE_MAX_T];
+    __shared__ int s_cnt;
+    __shared__ double shell[520];
+    __shared__ double red[4];
+    const FineGeom& g = P.g;
+    const long long item = blockIdx.x;
+    const int b = find_particle(item_off, B, item);
+    const long long local = item - item_off[b];
+    const long long j = orient_off[b] + local / g.over_rot;
+    const int ir = (int)(local % g.over_rot);
+    const int n = orients[j];
+    const long long p0 = orient_pair_start[j];
+    const long long p1 = (j + 1 < orient_off[b + 1]) ? orient_pair_start[j + 1] : pair_off[b + 1];
+    const double sumw = P.fine_res[b].sum_w, sig = P.fine_res[b].sig_w, mind2 = P.fine_res[b].min_d2;
+    const int nover = g.over_rot * g.over_trans;
+    if (threadIdx.x == 0) s_cnt = 0;
+    for (int i = threadIdx.x; i < P.nshell; i += blockDim.x) shell[i] = 0.;
+    __syncthreads();
+    // significant oversampled translations of this oversampled rotation (in CPU order)
+    const int nt = (int)(p1 - p0) * g.over_trans;
+    for (int q = threadIdx.x; q < nt; q += blockDim.x) {
+        const long long k = p0 + q / g.over_trans;
+        const int iot = q % g.over_trans;
+        const int2 nt2 = pairs[k];
+        double prior = P.prior_o[nt2.x] * P.prior_t[(size_t)b * g.T + nt2.y];
+        double w = 0.;
+        if (prior > 0.) {
+            double dd = (double)P.fine_d2[k * nover + ir * g.over_trans + iot] - mind2;
+            w = (dd > 700.) ? 0. : prior * exp(-dd);
+        }
+        if (w >= sig && w > 0.) {
+            int slot = atomicAdd(&s_cnt, 1);
+            if (slot < STORE_MAX_T) {
+                s_ft[slot] = nt2.y * g.over_trans + iot;
+                s_w[slot] = (float)(w / sumw);
+            }
+        }
+    }
+    __syncthreads();
+    const int cnt = min(s_cnt, STORE_MAX_T);
+    if (cnt == 0) return;
+    double A[9];
+    int c, idir;
+    fine_matrix(g, n, ir, A, &c, &idir);
+    // scalar sums (ml_optimiser.cpp:8505-8552), exact weights recomputed in double
+    if (threadIdx.x == 0) {
+        double sw = 0., soff = 0.;
+        const double px2 = P.pixel_size * P.pixel_size;
+        const PartScalars s = P.ps[b];
+        for (int q = 0; q < nt; q++) {
+            const long long k = p0 + q / g.over_trans;
+            const int iot = q % g.over_trans;
+            const int2 nt2 = pairs[k];
+            double prior = P.prior_o[nt2.x] * P.prior_t[(size_t)b * g.T + nt2.y];
+            if (!(prior > 0.)) continue;
+            double dd = (double)P.fine_d2[k * nover + ir * g.over_trans + iot] - mind2;
+            double w = (dd > 700.) ? 0. : prior * exp(-dd);
+            if (!(w >= sig && w > 0.)) continue;
+            w /= sumw;
+            const int ft = nt2.y * g.over_trans + iot;
+            sw += w;
+            double dx = s.prior_x - s.old_x - P.ftx[ft];
+            double dy = s.prior_y - s.old_y - P.fty[ft];
+            soff += w * px2 * dx * dx + w * px2 * dy * dy;
+        }
+        atomicAdd(P.w_sumw, sw);
+        atomicAdd(&P.w_pdf_class[c], sw);
+        atomicAdd(&P.w_pdf_dir[(size_t)c * P.Nd + idir], sw);
+        atomicAdd(P.w_sigma2_offset, soff);
+    }
+    // projection frame: Ai = A^T * pf (float for sampling, double for the scatter geometry)
+    const double pf = g.pf;
+    const float fa0 = (float)(A[0] * pf), fa1 = (float)(A[3] * pf), fa2 = (float)(A[1] * pf);
+    const float fa3 = (float)(A[4] * pf), fa4 = (float)(A[2] * pf), fa5 = (float)(A[5] * pf);
+    const double da0 = A[0] * pf, da1 = A[3] * pf, da2 = A[1] * pf, da3 = A[4] * pf, da4 = A[2] * pf, da5 = A[5] * pf;
+    const float2* vol = P.vols[c];
+    const int pad = P.pads[c];
+    float2* bpd = P.bp_data[c];
+    float* bpw = P.bp_weight[c];
+    const int bpad = P.bp_pads[c];
+    const double bpf = pf;
+    const int rr = relion_round(P.bp_rmax[c] * bpf);
+    const int max_r2 = rr * rr;
+    const int bxdim = bpad / 2 + 1, bh = bpad / 2;
+    // ellipse row bounds of backproject2Dto3D (backprojector.cpp:80-135) in double
+    const double AtA_xx = da0 * da0 + da2 * da2 + da4 * da4;
+    const double AtA_xy = da0 * da1 + da2 * da3 + da4 * da5;
+    const double AtA_yy = da1 * da1 + da3 * da3 + da5 * da5;
+    const double AtA_xy2 = AtA_xy * AtA_xy;
+    double norm_acc = 0.;
+    for (int p = threadIdx.x; p < P.npix; p += blockDim.x) {
+        const int x = P.pix[p].x, y = P.pix[p].y;
+        float2 f = make_float2(0.f, 0.f);
+        if (P.inproj[p]) {
+            float2 v;
+            if (trilinear_sample(vol, pad, fa0 * x + fa1 * y, fa2 * x + fa3 * y, fa4 * x + fa5 * y, &v)) f = v;
+        }
+        const size_t wi = (size_t)b * P.Pw + P.win[p];
+        const float ctf = P.Fctf[wi];
+        const float2 X = P.Fimg[wi], Xn = P.Fimg_nm[wi];
+        const float fr = f.x * ctf, fi = f.y * ctf;     // Frefctf
+        const float ms = P.minv_store[p];
+        float d2acc = 0.f, accr = 0.f, acci = 0.f, wacc = 0.f;
+        for (int q = 0; q < cnt; q++) {
+            const float w = s_w[q];
+            const float2 s = P.ab_fine[(size_t)s_ft[q] * P.npix + p];
+            const float xr = X.x * s.x - X.y * s.y, xi = X.x * s.y + X.y * s.x;
+            const float dr = fr - xr, di = fi - xi;
+            d2acc += w * (dr * dr + di * di);
+            const float nr = Xn.x * s.x - Xn.y * s.y, ni = Xn.x * s.y + Xn.y * s.x;
+            const float wx = w * ctf * ms;
+            accr += nr * wx;
+            acci += ni * wx;
+            wacc += wx * ctf;
+        }
+        const int ir_ = P.ires[p];
+        if (ir_ >= 0) {
+            atomicAdd(&shell[ir_], (double)d2acc);
+            norm_acc += d2acc;
+        }
+        if (!(wacc > 0.f)) continue;
+        // backprojection of this pixel (first_x/last_x ellipse bounds and r2 test in double)
+        {
+            double discr = AtA_xy2 * y * y - AtA_xx * (AtA_yy * y * y - max_r2);
+            if (discr < 0.0) continue;
+            double dd = sqrt(discr) / AtA_xx;
+            double qq = -AtA_xy * y / AtA_xx;
+            double lo = qq - dd, hi = qq + dd;
+            int first_x = (lo == (int)lo) ? (int)lo : ((lo > 0) ? (int)(lo + 1) : (int)lo);
+            int last_x = (hi == (int)hi) ? (int)hi : ((hi > 0) ? (int)hi : (int)(hi - 1));
+            if (x < first_x || x > last_x) continue;
+        }
+        double xp = da0 * x + da1 * y, yp = da2 * x + da3 * y, zp = da4 * x + da5 * y;
+        if (xp * xp + yp * yp + zp * zp > max_r2) continue;
+        bool neg = xp < 0;
+        if (neg) { xp = -xp; yp = -yp; zp = -zp; }
+        double fx0 = floor(xp), fy0 = floor(yp), fz0 = floor(zp);
+        int x0 = (int)fx0, y0 = (int)fy0 + bh, z0 = (int)fz0 + bh;
+        if (x0 < 0 || x0 + 1 >= bxdim || y0 < 0 || y0 + 1 >= bpad || z0 < 0 || z0 + 1 >= bpad) continue;
+        double fx = xp - fx0, fy = yp - fy0, fz = zp - fz0;
+        double mfx = 1. - fx, mfy = 1. - fy, mfz = 1. - fz;
+        float vr = accr, vi = neg ? -acci : acci;
+        const size_t sy = bxdim, sz = (size_t)bpad * bxdim;
+        const size_t base = (size_t)z0 * sz + (size_t)y0 * sy + x0;
+        const double wts[8] = {mfz * mfy * mfx, mfz * mfy * fx, mfz * fy * mfx, mfz * fy * fx,
+                               fz * mfy * mfx, fz * mfy * fx, fz * fy * mfx, fz * fy * fx};
+        const size_t offs[8] = {0, 1, sy, sy + 1, sz, sz + 1, sz + sy, sz + sy + 1};
+#pragma unroll
+        for (int k = 0; k < 8; k++) {
+            const float wk = (float)wts[k];
+            atomicAdd(&bpd[base + offs[k]].x, wk * vr);
+            atomicAdd(&bpd[base + offs[k]].y, wk * vi);
+            atomicAdd(&bpw[base + offs[k]], wk * wacc);
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) norm_acc += __shfl_xor(norm_acc, o);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = norm_acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&P.w_norm[b], red[0] + red[1] + red[2] + red[3]);
+    for (int i = threadIdx.x; i < P.nshell; i += blockDim.x)
+        if (shell[i] != 0.) atomicAdd(&P.w_sigma2[i], shell[i]);
+}
+
+__global__ void k_fill_u32(unsigned int* p, unsigned int v, int n)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_fill_f32(float* p, float v, long long n)
+{
+    long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+
+__global__ void k_items_from_orients(const long long* orient_off, int B, int over_rot, long long* item_off)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= B) item_off[i] = orient_off[i] * over_rot;
+}
+
+// ------------------------------------------------------------------------------------------
+// Host side
+// ------------------------------------------------------------------------------------------
+
+// Compact pixel table of one pass: every window pixel with ires >= 0 (Mresol,
+// ml_optimiser.cpp:5596-5625), ordered by window index.
+static void build_table(int size, int cur, const double* sigma2, int nsig2, double fudge, bool store,
+                        PassTables& t)
+{
+    const int xs = size / 2 + 1, xc = cur / 2 + 1;
+    t.size = size;
+    t.h_x.clear(); t.h_y.clear(); t.h_win.clear(); t.h_ires.clear(); t.h_minv.clear(); t.h_inproj.clear();
+    const int rmo = xs - 1;
+    for (int i = 0; i < size; i++) {
+        int ip = (i < xs) ? i : i - size;
+        for (int j = 0; j < xs; j++) {
+            int ires = relion_round(std::sqrt((double)(ip * ip + j * j)));
+            if (!(ires < xs && !(j == 0 && ip < 0))) continue;   // Mresol == -1
+            double mv = 0.;
+            if (ires > 0 && ires < nsig2) mv = 1. / (fudge * sigma2[ires]);
+            if (store && ires == 0) mv = 1. / (fudge * sigma2[0]);
+            if (!store && mv == 0.) continue;   // no contribution to diff2
+            int ii = ip < 0 ? ip + cur : ip;
+            t.h_x.push_back(j);
+            t.h_y.push_back(ip);
+            t.h_win.push_back(ii * xc + j);
+            t.h_ires.push_back(ires);
+            t.h_minv.push_back((float)mv);
+            // projected pixels: x <= FLOOR(sqrt(r_max_out^2 - y^2)), y = i for i <= r_max_out
+            int y_proj = (i <= rmo) ? i : i - size;
+            int xmax = (int)std::floor(std::sqrt((double)(rmo * rmo - y_proj * y_proj)));
+            bool in = (rmo * rmo - y_proj * y_proj >= 0) && j <= xmax;
+            t.h_inproj.push_back(in ? 1 : 0);
+            if (y_proj != ip) t.h_y.back() = y_proj;   // row size/2 is projected at +size/2
+        }
+    }
+    t.npix = (int)t.h_x.size();
+    t.npix_pad = (int)round_up(t.npix, 16);
+}
+
+static int upload_table(PassTables& t, hipStream_t st)
+{
+    T2X_TRY(t.d_pix.ensure(sizeof(int2) * t.npix));
+    T2X_TRY(t.d_win.ensure(sizeof(int) * t.npix));
+    T2X_TRY(t.d_minv.ensure(sizeof(float) * t.npix));
+    T2X_TRY(t.d_ires.ensure(sizeof(int) * t.npix));
+    T2X_TRY(t.d_inproj.ensure(t.npix));
+    std::vector<int2> xy(t.npix);
+    for (int i = 0; i < t.npix; i++) xy[i] = make_int2(t.h_x[i], t.h_y[i]);
+    T2X_HIP(hipMemcpyAsync(t.d_pix.ptr, xy.data(), sizeof(int2) * t.npix, hipMemcpyHostToDevice, st));
+    T2X_HIP(hipMemcpyAsync(t.d_win.ptr, t.h_win.data(), sizeof(int) * t.npix, hipMemcpyHostToDevice, st));
+    T2X_HIP(hipMemcpyAsync(t.d_minv.ptr, t.h_minv.data(), sizeof(float) * t.npix, hipMemcpyHostToDevice, st));
+    T2X_HIP(hipMemcpyAsync(t.d_ires.ptr, t.h_ires.data(), sizeof(int) * t.npix, hipMemcpyHostToDevice, st));
+    T2X_HIP(hipMemcpyAsync(t.d_inproj.ptr, t.h_inproj.data(), t.npix, hipMemcpyHostToDevice, st));
+    T2X_HIP(hipStreamSynchronize(st));
+    return T2X_OK;
+}
+
+// exp(-2 pi i (x sx + y sy)/N) for each translation and compact pixel (double -> float2).
+static std::vector<float2> phase_table(const PassTables& t, const std::vector<double>& sx,
+                                       const std::vector<double>& sy, int N)
+{
+    std::vector<float2> ab(sx.size() * t.npix);
+    for (size_t k = 0; k < sx.size(); k++) {
+        double xs = sx[k] / -(double)N, ys = sy[k] / -(double)N;
+        bool zero = std::fabs(xs) < 1e-6 && std::fabs(ys) < 1e-6;
+        for (int p = 0; p < t.npix; p++) {
+            // the phase uses the logical row of the window (i <= size/2 -> i)
+            int y = t.h_y[p];
+            double dotp = 2 * kPi * (t.h_x[p] * xs + y * ys);
+            double a = zero ? 1.0 : std::cos(dotp), bb = zero ? 0.0 : std::sin(dotp);
+            ab[k * t.npix + p] = make_float2((float)a, (float)bb);
+        }
+    }
+    return ab;
+}
+
+template <class T>
+static int upload(DevBuf& buf, const std::vector<T>& v, hipStream_t st)
+{
+    T2X_TRY(buf.ensure(sizeof(T) * std::max<size_t>(v.size(), 1)));
+    if (!v.empty())
+        T2X_HIP(hipMemcpyAsync(buf.ptr, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st));
+    return T2X_OK;
+}
+
+static int ensure_fft(Device* d, int N, int batch)
+{
+    if (d->fft_plan && d->fft_n == N && d->fft_batch == batch) return T2X_OK;
+    if (d->fft_plan) { (void)hipfftDestroy(d->fft_plan); d->fft_plan = 0; }
+    int dims[2] = {N, N};
+    if (hipfftPlanMany(&d->fft_plan, 2, dims, nullptr, 1, N * N, nullptr, 1, N * (N / 2 + 1),
+                       HIPFFT_R2C, batch) != HIPFFT_SUCCESS) {
+        set_error("hipfftPlanMany failed");
+        return T2X_EDEVICE;
+    }
+    if (hipfftSetStream(d->fft_plan, d->stream) != HIPFFT_SUCCESS) {
+        set_error("hipfftSetStream failed");
+        return T2X_EDEVICE;
+    }
+    d->fft_n = N;
+    d->fft_batch = batch;
+    return T2X_OK;
+}
+
+struct EventTimer {
+    Device* d;
+    bool on;
+    int idx = 0;
+    void mark() { if (on) (void)hipEventRecord(d->ev[idx++], d->stream); }
+};
+
+}  // namespace t2x
+
+using namespace t2x;
+
+extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_model* model, int n,
+                         const float* images, int images_on_device, const t2x_particle_in* pin,
+                         t2x_particle_out* out, t2x_wsum* wsum, t2x_estep_debug* dbg)
+{
+    if (!h || !cfg || !model || (n > 0 && (!images || !pin || !out))) {
+        set_error("null argument");
+        return T2X_EINVAL;
+    }
+    if (n < 0) { set_error("negative particle count"); return T2X_EINVAL; }
+    if (!model->sigma2_noise || !model->pdf_class || !model->pdf_direction) {
+        set_error("model arrays missing");
+        return T2X_EINVAL;
+    }
+    Device* d = reinterpret_cast<Device*>(h);
+    std::lock_guard<std::mutex> lk(d->mtx);
+    T2X_HIP(hipSetDevice(d->id));
+    HostSampling S;
+    T2X_TRY(build_sampling(cfg, S));
+    const int K = cfg->nr_classes;
+    const int N = cfg->ori_size, xN = N / 2 + 1;
+    const int cs = cfg->current_size, cc = cfg->coarse_size, xc = cs / 2 + 1;
+    const int Pw = cs * xc;
+    const long Nd = S.nr_dir, Npsi = S.nr_psi, T = S.nr_trans;
+    const long Norient = Nd * Npsi;
+    const long O_all = K * Norient;
+    const int over_rot = S.over_rot, over_trans = S.over_trans, nover = over_rot * over_trans;
+    hipStream_t st = d->stream;
+    if (T > 128) { set_error("more than 128 coarse translations"); return T2X_EINVAL; }
+    if (T * over_trans > STORE_MAX_T) { set_error("too many fine translations"); return T2X_EINVAL; }
+    std::vector<const float2*> h_vols(K);
+    std::vector<int> h_pads(K);
+    std::vector<float2*> h_bpd(K);
+    std::vector<float*> h_bpw(K);
+    std::vector<int> h_bpads(K), h_bprmax(K);
+    double pf = 0;
+    for (int c = 0; c < K; c++) {
+        auto it = d->classes.find(c);
+        if (it == d->classes.end() || !it->second.proj || !it->second.bp_data) {
+            set_error("projector or backprojector of class " + std::to_string(c) + " not set");
+            return T2X_ESTATE;
+        }
+        ClassVolumes& cv = it->second;
+        h_vols[c] = cv.proj;
+        h_pads[c] = cv.p_pad;
+        h_bpd[c] = cv.bp_data;
+        h_bpw[c] = cv.bp_weight;
+        h_bpads[c] = cv.b_pad;
+        h_bprmax[c] = cv.b_rmax;
+        if (c == 0) pf = cv.p_pf;
+        if (cv.p_pf != pf || cv.b_pf != pf) { set_error("padding factors differ between classes"); return T2X_EINVAL; }
+        if (cv.p_rmax * 2 < cs) {
+            // the CPU projector only fills r <= r_max; smaller r_max than current_size/2 is allowed
+        }
+    }
+    // ---- per-configuration tables ----
+    const double* sigma2 = model->sigma2_noise;
+    build_table(cc, cs, sigma2, xN, cfg->sigma2_fudge, false, d->coarse_tab);
+    build_table(cs, cs, sigma2, xN, cfg->sigma2_fudge, true, d->fine_tab);
+    PassTables& tc = d->coarse_tab;
+    PassTables& tf = d->fine_tab;
+    if (tf.npix > FINE_MAX_PIX) { set_error("current_size too large for the fine-pass LDS tile"); return T2X_EINVAL; }
+    T2X_TRY(upload_table(tc, st));
+    T2X_TRY(upload_table(tf, st));
+    // fine diff2 pixel subset: pixels with minv != 0 come first? (all store pixels are used;
+    // the diff2 kernel reads minv of the diff2 definition)
+    std::vector<float> minv_diff2(tf.npix);
+    for (int p = 0; p < tf.npix; p++) minv_diff2[p] = (tf.h_ires[p] > 0) ? tf.h_minv[p] : 0.f;
+    DevBuf d_minv_fd;
+    T2X_TRY(upload(d_minv_fd, minv_diff2, st));
+    // phase tables
+    std::vector<float2> abc = phase_table(tc, S.coarse_tx_px, S.coarse_ty_px, N);
+    std::vector<float2> abf = phase_table(tf, S.fine_tx_px, S.fine_ty_px, N);
+    T2X_TRY(upload(d->ab_coarse, abc, st));
+    T2X_TRY(upload(d->ab_fine, abf, st));
+    // orientation prior (NOPRIOR): pdf_direction / mean over all classes and orientations;
+    // 0 where the class or the direction has zero probability (never scored)
+    double mean_o = 0.;
+    for (int c = 0; c < K; c++)
+        for (long dd = 0; dd < Nd; dd++) mean_o += model->pdf_direction[(size_t)c * Nd + dd] * Npsi;
+    mean_o /= (double)(K * Norient);
+    std::vector<double> prior_o(O_all);
+    for (int c = 0; c < K; c++)
+        for (long o = 0; o < Norient; o++) {
+            double p = model->pdf_direction[(size_t)c * Nd + o / Npsi];
+            bool scored = model->pdf_class[c] > 0. && p > 0.;
+            if (mean_o != 0.) p /= mean_o;
+            prior_o[(size_t)c * Norient + o] = scored ? p : 0.;
+        }
+    T2X_TRY(upload(d->prior_o, prior_o, st));
+    // device copies of angle tables and class pointer arrays
+    DevBuf d_subrot, d_subtilt, d_subpsi, d_ftx, d_fty, d_vols, d_pads, d_bpd, d_bpw, d_bpads, d_bprmax;
+    T2X_TRY(upload(d_subrot, S.sub_rot, st));
+    T2X_TRY(upload(d_subtilt, S.sub_tilt, st));
+    T2X_TRY(upload(d_subpsi, S.sub_psi, st));
+    T2X_TRY(upload(d_ftx, S.fine_tx_px, st));
+    T2X_TRY(upload(d_fty, S.fine_ty_px, st));
+    T2X_TRY(upload(d_vols, h_vols, st));
+    T2X_TRY(upload(d_pads, h_pads, st));
+    T2X_TRY(upload(d_bpd, h_bpd, st));
+    T2X_TRY(upload(d_bpw, h_bpw, st));
+    T2X_TRY(upload(d_bpads, h_bpads, st));
+    T2X_TRY(upload(d_bprmax, h_bprmax, st));
+
+    // ---- coarse projections, cached across calls ----
+    const int Kc = tc.npix_pad * 2;            // GEMM K (re/im)
+    const int Kn = tc.npix_pad;                // N1 GEMM K
+    const long long O_pad = round_up(O_all, 128);
+    std::string key = std::to_string(cc) + "/" + std::to_string(cs) + "/" + std::to_string(K) + "/" +
+                      std::to_string(cfg->healpix_order) + "/" + std::to_string(cfg->psi_step) + "/" +
+                      std::to_string(tc.npix);
+    for (int c = 0; c < K; c++) key += "/" + std::to_string(d->classes[c].proj_version);
+    EventTimer et{d, true};
+    (void)hipEventRecord(d->ev[0], st);
+    if (key != d->coarse_key) {
+        T2X_TRY(d->coarse_B.ensure(sizeof(float) * O_pad * Kc));
+        T2X_TRY(d->coarse_B2.ensure(sizeof(float) * O_pad * Kn));
+        T2X_HIP(hipMemsetAsync(d->coarse_B.ptr, 0, sizeof(float) * O_pad * Kc, st));
+        T2X_HIP(hipMemsetAsync(d->coarse_B2.ptr, 0, sizeof(float) * O_pad * Kn, st));
+        std::vector<float> Ai(Norient * 9);
+        for (long dd = 0; dd < Nd; dd++)
+            for (long p = 0; p < Npsi; p++) {
+                double A[9];
+                euler_matrix(S.rot[dd], S.tilt[dd], S.psi[p], A);
+                float* a = &Ai[(dd * Npsi + p) * 9];
+                for (int r = 0; r < 3; r++)
+                    for (int c2 = 0; c2 < 3; c2++) a[r * 3 + c2] = (float)(A[c2 * 3 + r] * pf);
+            }
+        DevBuf d_Ai;
+        T2X_TRY(upload(d_Ai, Ai, st));
+        k_project_coarse<<<(unsigned)O_all, 256, 0, st>>>(
+            d_vols.as<const float2*>(), d_pads.as<int>(), d_Ai.as<float>(), (int)Norient,
+            tc.d_pix.as<int2>(), tc.d_inproj.as<unsigned char>(), tc.npix, d->coarse_B.as<float>(), Kc,
+            d->coarse_B2.as<float>(), Kn);
+        T2X_HIP(hipGetLastError());
+        T2X_HIP(hipStreamSynchronize(st));
+        d_Ai.release();
+        d->coarse_key = key;
+    }
+    (void)hipEventRecord(d->ev[1], st);
+    {
+        float ms = 0;
+        (void)hipEventSynchronize(d->ev[1]);
+        (void)hipEventElapsedTime(&ms, d->ev[0], d->ev[1]);
+        d->stats.s.ms_coarse_project += ms;
+    }
+
+    // ---- device accumulators ----
+    const size_t nws = xN + 1 + K + K * Nd + 1;
+    T2X_TRY(d->wsum_dev.ensure(sizeof(double) * nws));
+    T2X_HIP(hipMemsetAsync(d->wsum_dev.ptr, 0, sizeof(double) * nws, st));
+    double* w_sigma2 = d->wsum_dev.as<double>();
+    double* w_sumw = w_sigma2 + xN;
+    double* w_pdfc = w_sumw + 1;
+    double* w_pdfd = w_pdfc + K;
+    double* w_off = w_pdfd + K * Nd;
+
+    // ---- batch size: coarse diff2 dominates memory (B*T*O_pad floats) ----
+    const size_t per_particle = (size_t)T * O_pad * 4 + (size_t)T * Kc * 4 + (size_t)4 * N * N * 4 +
+                                (size_t)2 * N * xN * 8 * 2;
+    size_t budget = (size_t)24 << 30;   // 24 GiB of scratch for the batch
+    {
+        size_t fr = 0, tot = 0;
+        if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr / 3 < budget) budget = fr / 3;
+    }
+    int Bmax = (int)std::max<size_t>(1, std::min<size_t>(1024, budget / per_particle));
+    const char* env_b = getenv("T2X_BATCH");
+    if (env_b && atoi(env_b) > 0) Bmax = std::min(Bmax, atoi(env_b));
+
+    const double pix = cfg->pixel_size;
+    // logsigma2 of dLL (ml_optimiser.cpp:8888-8896)
+    double logsigma2 = 0.;
+    for (int i = 0; i < cs; i++) {
+        int ip = (i < xc) ? i : i - cs;
+        for (int j = 0; j < xc; j++) {
+            int ires = relion_round(std::sqrt((double)(ip * ip + j * j)));
+            if (!(ires < xc && !(j == 0 && ip < 0))) continue;
+            if (ires > 0 && ires < xN) logsigma2 += std::log(2. * kPi * sigma2[ires]);
+        }
+    }
+    std::vector<double> h_wnorm_all(n, 0.);
+    std::vector<double> h_power_sum(xN, 0.);
+    double LL = 0., Pmax = 0., avg_norm = 0.;
+    t2x_stats& stt = d->stats.s;
+    stt.coarse_size = cc;
+    stt.current_size = cs;
+    stt.P_coarse = tc.npix;
+    stt.P_fine = tf.npix;
+
+    for (int b0 = 0; b0 < n; b0 += Bmax) {
+        const int B = std::min(Bmax, n - b0);
+        const size_t NN = (size_t)N * N;
+        int ev = 0;
+        (void)hipEventRecord(d->ev[ev++], st);
+        // ---- inputs ----
+        const float* d_img;
+        if (images_on_device) d_img = images + (size_t)b0 * NN;
+        else {
+            T2X_TRY(d->img.ensure(sizeof(float) * B * NN));
+            T2X_HIP(hipMemcpyAsync(d->img.ptr, images + (size_t)b0 * NN, sizeof(float) * B * NN,
+                                   hipMemcpyHostToDevice, st));
+            d_img = d->img.as<float>();
+        }
+        std::vector<PartScalars> hps(B);
+        std::vector<CtfConst> hctf(B);
+        std::vector<double> prior_t((size_t)B * T);
+        for (int i = 0; i < B; i++) {
+            const t2x_particle_in& p = pin[b0 + i];
+            PartScalars& s = hps[i];
+            std::memset(&s, 0, sizeof(s));
+            s.old_x = relion_round(p.old_offset_x);
+            s.old_y = relion_round(p.old_offset_y);
+            double px_ = p.prior_offset_x, py_ = p.prior_offset_y;
+            if (px_ > 998.99 && px_ < 999.01) px_ = 0.;
+            if (py_ > 998.99 && py_ < 999.01) py_ = 0.;
+            s.prior_x = px_;
+            s.prior_y = py_;
+            s.norm_scale = cfg->do_norm_correction ? cfg->avg_norm_correction / p.norm_correction : 1.0;
+            hctf[i] = ctf_const(p.ctf);
+            // pdf_offset / mean (ml_optimiser.cpp:7638-7667, 7701-7725); offsets mix px and A as there
+            std::vector<double> pt(T);
+            double mean_t = 0.;
+            for (long t = 0; t < T; t++) {
+                double ox = s.old_x + S.tx[t], oy = s.old_y + S.ty[t];
+                double td2 = ((ox - s.prior_x) * (ox - s.prior_x) + (oy - s.prior_y) * (oy - s.prior_y)) * pix * pix;
+                double v = (cfg->sigma2_offset < 0.0001) ? ((td2 > 0.) ? 0. : 1.)
+                                                         : std::exp(td2 / (-2. * cfg->sigma2_offset)) / (2. * kPi * cfg->sigma2_offset);
+                pt[t] = v;
+                mean_t += v;
+            }
+            mean_t /= (double)T;
+            for (long t = 0; t < T; t++) prior_t[(size_t)i * T + t] = (mean_t > 0.) ? pt[t] / mean_t : pt[t];
+        }
+        T2X_TRY(upload(d->pscal, hps, st));
+        DevBuf d_ctf;
+        T2X_TRY(upload(d_ctf, hctf, st));
+        T2X_TRY(upload(d->prior_t, prior_t, st));
+        PartScalars* d_ps = d->pscal.as<PartScalars>();
+        // ---- preprocessing ----
+        T2X_TRY(d->img2.ensure(sizeof(float) * 2 * B * NN));
+        T2X_TRY(d->fft_out.ensure(sizeof(float2) * 2 * B * N * xN));
+        T2X_TRY(d->Fimg_cur.ensure(sizeof(float2) * B * Pw));
+        T2X_TRY(d->Fimg_nm_cur.ensure(sizeof(float2) * B * Pw));
+        T2X_TRY(d->Fctf_cur.ensure(sizeof(float) * B * Pw));
+        T2X_TRY(d->power.ensure(sizeof(double) * B * xN));
+        k_prep_real<<<B, 256, 0, st>>>(d_img, N, B, d_ps, cfg->particle_diameter / (2. * pix),
+                                      (double)cfg->width_mask_edge, d->img2.as<float>());
+        T2X_HIP(hipGetLastError());
+        T2X_TRY(ensure_fft(d, N, 2 * B));
+        if (hipfftExecR2C(d->fft_plan, d->img2.as<hipfftReal>(), d->fft_out.as<hipfftComplex>()) != HIPFFT_SUCCESS) {
+            set_error("hipfftExecR2C failed");
+            return T2X_EDEVICE;
+        }
+        k_prep_fourier<<<B, 256, sizeof(double) * xN, st>>>(
+            d->fft_out.as<float2>(), N, B, cs, d_ctf.as<CtfConst>(), pix, cfg->do_ctf_correction,
+            d->Fimg_cur.as<float2>(), d->Fimg_nm_cur.as<float2>(), d->Fctf_cur.as<float>(),
+            d->power.as<double>(), d_ps);
+        T2X_HIP(hipGetLastError());
+        k_const_term<<<B, 256, 0, st>>>(d->Fimg_cur.as<float2>(), Pw, tc.d_win.as<int>(),
+                                        tc.d_minv.as<float>(), tc.npix, d_ps, 0);
+        k_const_term<<<B, 256, 0, st>>>(d->Fimg_cur.as<float2>(), Pw, tf.d_win.as<int>(),
+                                        d_minv_fd.as<float>(), tf.npix, d_ps, 1);
+        T2X_HIP(hipGetLastError());
+        (void)hipEventRecord(d->ev[ev++], st);   // 1: preprocess done
+
+        // ---- coarse pass ----
+        const long long M = (long long)B * T;
+        const long long M_pad = round_up(M, 128);
+        const long long B_pad = round_up(B, 128);
+        T2X_TRY(d->A.ensure(sizeof(float) * M_pad * Kc));
+        T2X_TRY(d->A1.ensure(sizeof(float) * B_pad * Kn));
+        T2X_TRY(d->N1.ensure(sizeof(float) * B_pad * O_pad));
+        T2X_TRY(d->diff2.ensure(sizeof(float) * M_pad * O_pad));
+        const int rowmin_ld = (int)(O_pad / 64);
+        DevBuf& d_rowmin = d->minb;
+        T2X_TRY(d_rowmin.ensure(sizeof(float) * M_pad * rowmin_ld));
+        T2X_HIP(hipMemsetAsync(d->A.ptr, 0, sizeof(float) * M_pad * Kc, st));
+        T2X_HIP(hipMemsetAsync(d->A1.ptr, 0, sizeof(float) * B_pad * Kn, st));
+        {
+            dim3 grid((unsigned)T, (unsigned)B);
+            k_coarse_imgside<<<grid, 256, 0, st>>>(d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw,
+                                                   tc.d_win.as<int>(), tc.d_minv.as<float>(), tc.npix,
+                                                   d->ab_coarse.as<float2>(), (int)T, d->A.as<float>(), Kc,
+                                                   d->A1.as<float>(), Kn);
+            T2X_HIP(hipGetLastError());
+        }
+        {
+            EpiParams ep{};
+            ep.C = d->N1.as<float>();
+            ep.ldc = O_pad;
+            ep.M = B;
+            ep.N = (int)O_all;
+            T2X_TRY(launch_gemm(EPI_STORE, d->A1.as<float>(), Kn, d->coarse_B2.as<float>(), Kn, Kn,
+                                (int)B_pad, (int)O_pad, ep, st));
+        }
+        (void)hipEventRecord(d->ev[ev++], st);   // 2: coarse gemm start
+        {
+            EpiParams ep{};
+            ep.C = d->diff2.as<float>();
+            ep.ldc = O_pad;
+            ep.M = (int)M;
+            ep.N = (int)O_all;
+            ep.T = (int)T;
+            // C_b is read from the PartScalars array: pass a strided view via a packed copy
+            ep.Cb = nullptr;
+            DevBuf& d_cb = d->tables;
+            std::vector<double> dummy;   // filled on device below
+            T2X_TRY(d_cb.ensure(sizeof(double) * B));
+            // copy C_coarse out of the scalars (device to device, strided)
+            T2X_HIP(hipMemcpy2DAsync(d_cb.ptr, sizeof(double), (char*)d_ps + offsetof(PartScalars, C_coarse),
+                                     sizeof(PartScalars), sizeof(double), B, hipMemcpyDeviceToDevice, st));
+            ep.Cb = d_cb.as<double>();
+            ep.N1 = d->N1.as<float>();
+            ep.ldn = O_pad;
+            ep.prior_o = d->prior_o.as<double>();
+            ep.rowmin = d_rowmin.as<float>();
+            ep.rowmin_ld = rowmin_ld;
+            // rows beyond the valid columns of the last tile keep +inf
+            k_fill_f32<<<(unsigned)((M_pad * rowmin_ld + 255) / 256), 256, 0, st>>>(d_rowmin.as<float>(), 3.0e38f,
+                                                                                    M_pad * rowmin_ld);
+            T2X_TRY(launch_gemm(EPI_DIFF2, d->A.as<float>(), Kc, d->coarse_B.as<float>(), Kc, Kc,
+                                (int)M_pad, (int)O_pad, ep, st));
+        }
+        (void)hipEventRecord(d->ev[ev++], st);   // 3: coarse gemm end
+        // ---- coarse weights and significance ----
+        T2X_TRY(d->sel_out.ensure(sizeof(SelResult) * 2 * B));
+        SelResult* d_selc = d->sel_out.as<SelResult>();
+        SelResult* d_self = d_selc + B;
+        WeightSrc wc{};
+        wc.kind = SRC_COARSE;
+        wc.d2 = d->diff2.as<float>();
+        wc.ld = O_pad;
+        wc.T = (int)T;
+        wc.O_all = (int)O_all;
+        wc.prior_o = d->prior_o.as<double>();
+        wc.prior_t = d->prior_t.as<double>();
+        wc.rowmin = d_rowmin.as<float>();
+        wc.rowmin_ld = rowmin_ld;
+        wc.nover = nover;
+        wc.over_trans = over_trans;
+        k_pass_min<<<B, 1024, 0, st>>>(wc, B, d_selc);
+        SelParams sp{cfg->adaptive_fraction, (long long)cfg->maximum_significants, 1};
+        k_select<<<B, 1024, 0, st>>>(wc, sp, B, d_selc);
+        T2X_HIP(hipGetLastError());
+        T2X_TRY(d->sig_count.ensure(sizeof(long long) * 2 * B));
+        k_sig_count<<<B, 1024, 0, st>>>(wc, B, d_selc, d->sig_count.as<long long>());
+        T2X_HIP(hipGetLastError());
+        std::vector<long long> counts(2 * B);
+        std::vector<SelResult> hselc(B);
+        T2X_HIP(hipMemcpyAsync(counts.data(), d->sig_count.ptr, sizeof(long long) * 2 * B, hipMemcpyDeviceToHost, st));
+        T2X_HIP(hipMemcpyAsync(hselc.data(), d_selc, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
+        T2X_HIP(hipStreamSynchronize(st));
+        (void)hipEventRecord(d->ev[ev++], st);   // 4: coarse weights done
+        std::vector<long long> pair_off(B + 1, 0), orient_off(B + 1, 0);
+        for (int i = 0; i < B; i++) {
+            if (hselc[i].npos == 0 || !(hselc[i].sum_w > 0.) || std::isnan(hselc[i].sum_w)) {
+                set_error("zero sum of weights in the coarse pass for particle " + std::to_string(b0 + i));
+                return T2X_EZEROSUM;
+            }
+            if (hselc[i].nsig == 0) { set_error("no significant coarse sample"); return T2X_ENOSIG; }
+            pair_off[i + 1] = pair_off[i] + counts[2 * i];
+            orient_off[i + 1] = orient_off[i] + counts[2 * i + 1];
+        }
+        const long long npairs = pair_off[B], norients = orient_off[B];
+        DevBuf d_pair_off, d_orient_off, d_item_off;
+        T2X_TRY(upload(d_pair_off, pair_off, st));
+        T2X_TRY(upload(d_orient_off, orient_off, st));
+        T2X_TRY(d->sig_pairs.ensure(sizeof(int2) * std::max<long long>(npairs, 1)));
+        T2X_TRY(d->sig_orient.ensure(sizeof(int) * std::max<long long>(norients, 1)));
+        T2X_TRY(d->orient_pairs_off.ensure(sizeof(long long) * std::max<long long>(norients, 1)));
+        k_sig_write<<<B, 1024, 0, st>>>(wc, B, d_selc, d_pair_off.as<long long>(), d_orient_off.as<long long>(),
+                                        d->sig_pairs.as<int2>(), d->sig_orient.as<int>(),
+                                        d->orient_pairs_off.as<long long>());
+        T2X_HIP(hipGetLastError());
+        // debug: coarse diff2 / significance in ihidden order (class, dir, psi, trans)
+        if (dbg && dbg->n_debug > 0 && b0 < dbg->n_debug) {
+            int nd = std::min(B, dbg->n_debug - b0);
+            const long Nh = O_all * T;
+            std::vector<float> row((size_t)T * O_pad);
+            T2X_HIP(hipStreamSynchronize(st));
+            for (int i = 0; i < nd; i++) {
+                T2X_HIP(hipMemcpy(row.data(), d->diff2.as<float>() + (size_t)i * T * O_pad,
+                                  sizeof(float) * T * O_pad, hipMemcpyDeviceToHost));
+                for (long o = 0; o < O_all; o++)
+                    for (long t = 0; t < T; t++) {
+                        float v = row[(size_t)t * O_pad + o];
+                        if (!(prior_o[o] > 0.)) v = -999.f;
+                        if (dbg->coarse_diff2) dbg->coarse_diff2[(size_t)(b0 + i) * Nh + o * T + t] = v;
+                    }
+                if (dbg->coarse_min_diff2) dbg->coarse_min_diff2[b0 + i] = hselc[i].min_d2;
+                if (dbg->coarse_sum_weight) dbg->coarse_sum_weight[b0 + i] = hselc[i].sum_w;
+                if (dbg->coarse_sig_weight) dbg->coarse_sig_weight[b0 + i] = hselc[i].sig_w;
+            }
+            if (dbg->coarse_significant) {
+                std::vector<int2> hp(npairs);
+                T2X_HIP(hipMemcpy(hp.data(), d->sig_pairs.ptr, sizeof(int2) * npairs, hipMemcpyDeviceToHost));
+                for (int i = 0; i < nd; i++) {
+                    unsigned char* m = dbg->coarse_significant + (size_t)(b0 + i) * Nh;
+                    std::memset(m, 0, Nh);
+                    for (long long k = pair_off[i]; k < pair_off[i + 1]; k++) m[(long)hp[k].x * T + hp[k].y] = 1;
+                }
+            }
+        }
+        // ---- fine pass ----
+        T2X_TRY(d_item_off.ensure(sizeof(long long) * (B + 1)));
+        k_items_from_orients<<<(B + 256) / 256, 256, 0, st>>>(d_orient_off.as<long long>(), B, over_rot,
+                                                              d_item_off.as<long long>());
+        const long long nitems = norients * over_rot;
+        T2X_TRY(d->fine_diff2.ensure(sizeof(float) * std::max<long long>(npairs * nover, 1)));
+        T2X_TRY(d->fine_min.ensure(sizeof(unsigned int) * B));
+        k_fill_u32<<<(B + 255) / 256, 256, 0, st>>>(d->fine_min.as<unsigned int>(), 0xffffffffu, B);
+        FineGeom g{};
+        g.Norient = (int)Norient;
+        g.Npsi = (int)Npsi;
+        g.n_subdir = S.n_subdir;
+        g.n_subpsi = S.n_subpsi;
+        g.over_rot = over_rot;
+        g.over_trans = over_trans;
+        g.T = (int)T;
+        g.sub_rot = d_subrot.as<double>();
+        g.sub_tilt = d_subtilt.as<double>();
+        g.sub_psi = d_subpsi.as<double>();
+        g.pf = pf;
+        (void)hipEventRecord(d->ev[ev++], st);   // 5: fine start
+        if (nitems > 0) {
+            k_fine_diff2<<<(unsigned)nitems, 256, 0, st>>>(
+                d_item_off.as<long long>(), B, d_orient_off.as<long long>(), d->sig_orient.as<int>(),
+                d->orient_pairs_off.as<long long>(), d_pair_off.as<long long>(), d->sig_pairs.as<int2>(), g,
+                d_vols.as<const float2*>(), d_pads.as<int>(), tf.d_pix.as<int2>(), tf.d_inproj.as<unsigned char>(),
+                d_minv_fd.as<float>(), tf.d_win.as<int>(), tf.npix, d->Fimg_cur.as<float2>(),
+                d->Fctf_cur.as<float>(), Pw, d->ab_fine.as<float2>(), d_ps, d->fine_diff2.as<float>(),
+                d->fine_min.as<unsigned int>());
+            T2X_HIP(hipGetLastError());
+        }
+        (void)hipEventRecord(d->ev[ev++], st);   // 6: fine diff2 end
+        WeightSrc wf = wc;
+        wf.kind = SRC_FINE;
+        wf.d2 = d->fine_diff2.as<float>();
+        wf.pair_off = d_pair_off.as<long long>();
+        wf.pairs = d->sig_pairs.as<int2>();
+        wf.fine_min = d->fine_min.as<unsigned int>();
+        k_pass_min<<<B, 1024, 0, st>>>(wf, B, d_self);
+        SelParams spf{cfg->adaptive_fraction, (long long)cfg->maximum_significants, 0};
+        k_select<<<B, 1024, 0, st>>>(wf, spf, B, d_self);
+        T2X_TRY(d->fine_sel.ensure(sizeof(long long) * B));
+        k_fine_argmax<<<B, 1024, 0, st>>>(wf, B, d_self, d->fine_sel.as<long long>());
+        T2X_HIP(hipGetLastError());
+        (void)hipEventRecord(d->ev[ev++], st);   // 7: fine weights done
+        if (dbg && dbg->n_debug > 0 && b0 < dbg->n_debug && dbg->fine_count) {
+            int nd = std::min(B, dbg->n_debug - b0);
+            std::vector<float> fd(npairs * nover);
+            std::vector<int2> hp(npairs);
+            T2X_HIP(hipStreamSynchronize(st));
+            T2X_HIP(hipMemcpy(fd.data(), d->fine_diff2.ptr, sizeof(float) * npairs * nover, hipMemcpyDeviceToHost));
+            T2X_HIP(hipMemcpy(hp.data(), d->sig_pairs.ptr, sizeof(int2) * npairs, hipMemcpyDeviceToHost));
+            for (int i = 0; i < nd; i++) {
+                long cnt = 0;
+                // CPU order: orientation, oversampled rotation, translation, oversampled translation
+                long long k0 = pair_off[i], k1 = pair_off[i + 1];
+                long long k = k0;
+                while (k < k1) {
+                    long long ke = k;
+                    while (ke < k1 && hp[ke].x == hp[k].x) ke++;
+                    for (int ir = 0; ir < over_rot; ir++)
+                        for (long long kk = k; kk < ke; kk++)
+                            for (int iot = 0; iot < over_trans; iot++) {
+                                if (cnt < dbg->fine_capacity) {
+                                    size_t o = (size_t)(b0 + i) * dbg->fine_capacity + cnt;
+                                    if (dbg->fine_diff2) dbg->fine_diff2[o] = fd[kk * nover + ir * over_trans + iot];
+                                    if (dbg->fine_hidden)
+                                        dbg->fine_hidden[o] = ((long)hp[kk].x * T + hp[kk].y) * nover + ir * over_trans + iot;
+                                }
+                                cnt++;
+                            }
+                    k = ke;
+                }
+                dbg->fine_count[b0 + i] = cnt;
+            }
+        }
+        // ---- weighted sums and backprojection ----
+        T2X_TRY(d->best.ensure(sizeof(double) * B));
+        T2X_HIP(hipMemsetAsync(d->best.ptr, 0, sizeof(double) * B, st));
+        StoreParams P{};
+        P.g = g;
+        P.vols = d_vols.as<const float2*>();
+        P.pads = d_pads.as<int>();
+        P.bp_data = d_bpd.as<float2*>();
+        P.bp_weight = d_bpw.as<float*>();
+        P.bp_pads = d_bpads.as<int>();
+        P.bp_rmax = d_bprmax.as<int>();
+        P.prior_o = d->prior_o.as<double>();
+        P.prior_t = d->prior_t.as<double>();
+        P.pix = tf.d_pix.as<int2>();
+        P.inproj = tf.d_inproj.as<unsigned char>();
+        P.minv_store = tf.d_minv.as<float>();
+        P.win = tf.d_win.as<int>();
+        P.ires = tf.d_ires.as<int>();
+        P.npix = tf.npix;
+        P.Fimg = d->Fimg_cur.as<float2>();
+        P.Fimg_nm = d->Fimg_nm_cur.as<float2>();
+        P.Fctf = d->Fctf_cur.as<float>();
+        P.Pw = Pw;
+        P.ab_fine = d->ab_fine.as<float2>();
+        P.ftx = d_ftx.as<double>();
+        P.fty = d_fty.as<double>();
+        P.ps = d_ps;
+        P.fine_res = d_self;
+        P.fine_d2 = d->fine_diff2.as<float>();
+        P.pixel_size = pix;
+        P.nshell = xN;
+        P.cs = cs;
+        P.w_sigma2 = w_sigma2;
+        P.w_sumw = w_sumw;
+        P.w_pdf_class = w_pdfc;
+        P.w_pdf_dir = w_pdfd;
+        P.w_sigma2_offset = w_off;
+        P.w_norm = d->best.as<double>();
+        P.Nd = (int)Nd;
+        if (xN > 520) { set_error("box too large for the shell accumulator"); return T2X_EINVAL; }
+        if (nitems > 0) {
+            k_store<<<(unsigned)nitems, 256, 0, st>>>(d_item_off.as<long long>(), B, d_orient_off.as<long long>(),
+                                                      d->sig_orient.as<int>(), d->orient_pairs_off.as<long long>(),
+                                                      d_pair_off.as<long long>(), d->sig_pairs.as<int2>(), P);
+            T2X_HIP(hipGetLastError());
+        }
+        (void)hipEventRecord(d->ev[ev++], st);   // 8: store done
+        // ---- per-particle results ----
+        std::vector<SelResult> hself(B);
+        std::vector<long long> hnsig(B);
+        std::vector<double> hnorm(B), hpower((size_t)B * xN);
+        std::vector<PartScalars> hps2(B);
+        std::vector<int2> hpairs;
+        T2X_HIP(hipMemcpyAsync(hself.data(), d_self, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
+        T2X_HIP(hipMemcpyAsync(hnsig.data(), d->fine_sel.ptr, sizeof(long long) * B, hipMemcpyDeviceToHost, st));
+        T2X_HIP(hipMemcpyAsync(hnorm.data(), d->best.ptr, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+        T2X_HIP(hipMemcpyAsync(hpower.data(), d->power.ptr, sizeof(double) * B * xN, hipMemcpyDeviceToHost, st));
+        T2X_HIP(hipMemcpyAsync(hps2.data(), d_ps, sizeof(PartScalars) * B, hipMemcpyDeviceToHost, st));
+        T2X_HIP(hipStreamSynchronize(st));
+        {
+            float ms[8] = {0};
+            for (int e = 1; e < ev; e++) (void)hipEventElapsedTime(&ms[e - 1], d->ev[e - 1], d->ev[e]);
+            stt.ms_preprocess += ms[0];
+            stt.ms_coarse_diff2 += ms[1] + ms[2];
+            stt.ms_coarse_weights += ms[3];
+            stt.ms_fine_diff2 += ms[4] + ms[5];
+            stt.ms_fine_weights += ms[6];
+            stt.ms_store += ms[7];
+        }
+        stt.n_coarse_diff2_launches += 1;
+        stt.coarse_diff2_flop += 4.0 * (double)T * (double)O_all * (double)tc.npix * B;
+        stt.coarse_diff2_bytes += 4.0 * ((double)M * Kc + (double)O_pad * Kc + (double)M * O_pad);
+        stt.fine_samples += (double)npairs * nover;
+        stt.fine_flop += 4.0 * (double)npairs * nover * tf.npix;
+        stt.bp_orientations += (double)nitems;
+        for (int i = 0; i < B; i++) {
+            const int gi = b0 + i;
+            const SelResult& r = hself[i];
+            t2x_particle_out& o = out[gi];
+            std::memset(&o, 0, sizeof(o));
+            if (!(r.sum_w > 0.) || std::isnan(r.sum_w)) {
+                set_error("zero sum of weights in the fine pass for particle " + std::to_string(gi));
+                return T2X_EZEROSUM;
+            }
+            // decode the best key: (((n*over_rot + ir)*T + t)*over_trans + iot)
+            long long kkey = r.best_key;
+            const int iot = (int)(kkey % over_trans); kkey /= over_trans;
+            const long t = (long)(kkey % T); kkey /= T;
+            const int ir = (int)(kkey % over_rot); kkey /= over_rot;
+            const long nn = (long)kkey;
+            const int c = (int)(nn / Norient);
+            const long oo = nn % Norient, dd = oo / Npsi, pp = oo % Npsi;
+            o.rot = S.sub_rot[dd * S.n_subdir + ir / S.n_subpsi];
+            o.tilt = S.sub_tilt[dd * S.n_subdir + ir / S.n_subpsi];
+            o.psi = S.sub_psi[pp * S.n_subpsi + ir % S.n_subpsi];
+            o.xoff = hps[i].old_x + S.fine_tx_px[t * over_trans + iot];
+            o.yoff = hps[i].old_y + S.fine_ty_px[t * over_trans + iot];
+            o.iclass = c + 1;
+            o.pmax = r.max_w / r.sum_w;
+            o.best_hidden = ((nn * T + t) * over_rot + ir) * over_trans + iot;
+            o.sum_weight = r.sum_w;
+            o.min_diff2 = r.min_d2;
+            o.significant_weight = r.sig_w;
+            o.nr_significant_coarse = hselc[i].nsig;
+            o.nr_fine_samples = (pair_off[i + 1] - pair_off[i]) * nover;
+            o.nr_fine_significant = hnsig[i];
+            double wnorm = hnorm[i];
+            for (int ires = cs / 2 + 1; ires < xN; ires++) {
+                h_power_sum[ires] += hpower[(size_t)i * xN + ires];
+                wnorm += hpower[(size_t)i * xN + ires];
+            }
+            double normcorr = pin[gi].norm_correction;
+            if (cfg->do_norm_correction) {
+                double old = pin[gi].norm_correction / cfg->avg_norm_correction;
+                normcorr = old * std::sqrt(wnorm * 2.);
+                avg_norm += normcorr;
+            }
+            o.norm_correction = normcorr;
+            o.dLL = std::log(r.sum_w) - r.min_d2 - logsigma2;
+            LL += o.dLL;
+            Pmax += o.pmax;
+        }
+        stt.n_particles += B;
+        for (int i = 0; i < B; i++) stt.sig_coarse += (double)hselc[i].nsig;
+    }
+    stt.n_calls += 1;
+    // ---- weighted sums back to the caller ----
+    std::vector<double> hw(nws);
+    T2X_HIP(hipMemcpy(hw.data(), d->wsum_dev.ptr, sizeof(double) * nws, hipMemcpyDeviceToHost));
+    if (wsum) {
+        if (wsum->sigma2_noise)
+            for (int i = 0; i < xN; i++) wsum->sigma2_noise[i] += hw[i] + h_power_sum[i];
+        if (wsum->sumw_group) wsum->sumw_group[0] += hw[xN];
+        if (wsum->pdf_class)
+            for (int c = 0; c < K; c++) wsum->pdf_class[c] += hw[xN + 1 + c];
+        if (wsum->pdf_direction)
+            for (long i = 0; i < K * Nd; i++) wsum->pdf_direction[i] += hw[xN + 1 + K + i];
+        if (wsum->sigma2_offset) wsum->sigma2_offset[0] += hw[xN + 1 + K + K * Nd];
+        if (wsum->avg_norm_correction && cfg->do_norm_correction) wsum->avg_norm_correction[0] += avg_norm;
+        if (wsum->LL) wsum->LL[0] += LL;
+        if (wsum->ave_Pmax) wsum->ave_Pmax[0] += Pmax;
+    }
+    return T2X_OK;
+}

@@ -1,0 +1,242 @@
+// t2x_internal.h — device state and shared helpers of libt2x (MI355X / gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hipfft/hipfft.h>
+
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/t2x.h"
+
+namespace t2x {
+
+void set_error(const std::string& msg);
+
+#define T2X_HIP(call)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (call);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            ::t2x::set_error(std::string(#call) + ": " + hipGetErrorString(e_) + " at " +  \
+                             __FILE__ + ":" + std::to_string(__LINE__));                   \
+            return T2X_EDEVICE;                                                            \
+        }                                                                                  \
+    } while (0)
+
+#define T2X_TRY(expr)                     \
+    do {                                  \
+        int s_ = (expr);                  \
+        if (s_ != T2X_OK) return s_;      \
+    } while (0)
+
+// A device allocation that grows on demand and is reused across calls.
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    ~DevBuf() { release(); }
+    int ensure(size_t need)
+    {
+        if (need <= bytes) return T2X_OK;
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+        size_t b = need + need / 8 + 256;
+        if (hipMalloc(&ptr, b) != hipSuccess) {
+            set_error("hipMalloc of " + std::to_string(b) + " bytes failed");
+            return T2X_ENOMEM;
+        }
+        bytes = b;
+        return T2X_OK;
+    }
+    template <class T> T* as() const { return reinterpret_cast<T*>(ptr); }
+    void release()
+    {
+        if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+// One class of the model: the projector (PPref) and the backprojector (BPref).
+struct ClassVolumes {
+    // projector
+    int p_ref_dim = 0, p_pad = 0, p_rmax = 0;
+    double p_pf = 0;
+    float2* proj = nullptr;   // pad*pad*(pad/2+1) complex
+    uint64_t proj_version = 0;
+    // backprojector
+    int b_ref_dim = 0, b_pad = 0, b_rmax = 0;
+    double b_pf = 0;
+    float2* bp_data = nullptr;
+    float* bp_weight = nullptr;
+};
+
+// Host-side sampling grid (HealpixSampling), computed in double precision.
+struct HostSampling {
+    long nr_dir = 0, nr_psi = 0, nr_trans = 0;
+    int over_rot = 1, over_trans = 1;     // fine-pass oversampling factors
+    std::vector<double> rot, tilt, psi;   // coarse directions / psi
+    std::vector<double> tx, ty;           // coarse translations (A)
+    std::vector<long> ipix;
+    // fine pass: sub-directions per coarse direction (over_rot/over_psi of them) and sub-psi
+    int n_subdir = 1, n_subpsi = 1;
+    std::vector<double> sub_rot, sub_tilt;  // [nr_dir * n_subdir]
+    std::vector<double> sub_psi;            // [nr_psi * n_subpsi]
+    std::vector<double> fine_tx_px, fine_ty_px;  // [nr_trans * over_trans]
+    std::vector<double> coarse_tx_px, coarse_ty_px;
+};
+
+int build_sampling(const t2x_estep_config* cfg, HostSampling& s);
+int check_config(const t2x_estep_config* c);
+
+// Cached per-configuration device tables.
+struct PassTables {
+    int size = 0;          // image size of the pass (coarse or current)
+    int npix = 0;          // compact pixels with Minvsigma2 != 0
+    int npix_pad = 0;      // padded to a multiple of 16
+    DevBuf d_pix;          // int2 (x, y) logical frequencies, y signed
+    DevBuf d_win;          // int: index into the current-size window of each compact pixel
+    DevBuf d_minv;         // float Minvsigma2 of each compact pixel
+    DevBuf d_ires;         // int shell index of each compact pixel
+    DevBuf d_inproj;       // uchar: pixel inside the projected disc x^2+y^2 <= (size/2)^2
+    std::vector<int> h_x, h_y, h_win, h_ires;
+    std::vector<float> h_minv;
+    std::vector<unsigned char> h_inproj;
+};
+
+struct Stats {
+    t2x_stats s{};
+};
+
+struct Device {
+    int id = 0;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev[16];
+    std::map<int, ClassVolumes> classes;
+    uint64_t proj_counter = 0;
+
+    // FFT plan for batched N x N R2C
+    hipfftHandle fft_plan = 0;
+    int fft_n = 0, fft_batch = 0;
+
+    // cached coarse projections
+    std::string coarse_key;
+    DevBuf coarse_B;      // [O_all_pad][K_pad] float: re/im of the coarse slices
+    DevBuf coarse_B2;     // [O_all_pad][Kn_pad] float: |F|^2 of the coarse slices
+    int coarse_O_all = 0, coarse_O_pad = 0;
+
+    // scratch
+    DevBuf img, img2, fft_out, Fimg_cur, Fimg_nm_cur, Fctf_cur, pscal, power;
+    DevBuf A, A1, N1, diff2, minb, sel_out, prior_o, prior_t, sig_pairs, sig_count, sig_orient,
+        sig_orient_count, orient_pairs_off, fine_items, fine_diff2, fine_min, fine_sel, ab_fine,
+        ab_coarse, wsum_dev, store_items, best, tables, Y_fine, dbg_tmp;
+    PassTables coarse_tab, fine_tab;
+    std::string tab_key;
+
+    Stats stats;
+    std::mutex mtx;
+};
+
+// ---------------- device helpers ----------------
+
+// Order-preserving map of a float to uint32 for atomicMin.
+__device__ __forceinline__ unsigned int float_to_ordered(float f)
+{
+    unsigned int u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__host__ __device__ __forceinline__ float ordered_to_float(unsigned int u)
+{
+    unsigned int v = (u & 0x80000000u) ? (u & 0x7fffffffu) : ~u;
+    float f;
+    std::memcpy(&f, &v, 4);
+    return f;
+}
+
+// Euler_angles2matrix (euler.cpp:52-91) in double precision.
+__host__ __device__ inline void euler_matrix(double rot, double tilt, double psi, double* A)
+{
+    const double d2r = 3.14159265358979323846 / 180.0;
+    double ca, sa, cb, sb, cg, sg;
+    sincos(rot * d2r, &sa, &ca);
+    sincos(tilt * d2r, &sb, &cb);
+    sincos(psi * d2r, &sg, &cg);
+    double cc = cb * ca, cs = cb * sa, sc = sb * ca, ss = sb * sa;
+    A[0] = cg * cc - sg * sa;  A[1] = cg * cs + sg * ca;  A[2] = -cg * sb;
+    A[3] = -sg * cc - cg * sa; A[4] = -sg * cs + cg * ca; A[5] = sg * sb;
+    A[6] = sc;                 A[7] = ss;                 A[8] = cb;
+}
+
+// Trilinear sample of the half-complex projector volume at logical (xp, yp, zp) (already
+// multiplied by the padding factor), with the Friedel flip of projector.cpp:583-640.
+// Returns false where the reference skips the pixel (outside the box).
+__device__ __forceinline__ bool trilinear_sample(const float2* __restrict__ vol, int pad, float xp,
+                                                 float yp, float zp, float2* out)
+{
+    const int xdim = pad / 2 + 1, h = pad / 2;
+    bool neg = xp < 0.f;
+    if (neg) { xp = -xp; yp = -yp; zp = -zp; }
+    float fx0 = floorf(xp), fy0 = floorf(yp), fz0 = floorf(zp);
+    int x0 = (int)fx0, y0 = (int)fy0 + h, z0 = (int)fz0 + h;
+    float fx = xp - fx0, fy = yp - fy0, fz = zp - fz0;
+    if (x0 < 0 || x0 + 1 >= xdim || y0 < 0 || y0 + 1 >= pad || z0 < 0 || z0 + 1 >= pad) return false;
+    const size_t sy = xdim, sz = (size_t)pad * xdim;
+    const float2* p = vol + (size_t)z0 * sz + (size_t)y0 * sy + x0;
+    float2 d000 = p[0], d001 = p[1], d010 = p[sy], d011 = p[sy + 1];
+    float2 d100 = p[sz], d101 = p[sz + 1], d110 = p[sz + sy], d111 = p[sz + sy + 1];
+    float2 dx00 = make_float2(d000.x + (d001.x - d000.x) * fx, d000.y + (d001.y - d000.y) * fx);
+    float2 dx01 = make_float2(d100.x + (d101.x - d100.x) * fx, d100.y + (d101.y - d100.y) * fx);
+    float2 dx10 = make_float2(d010.x + (d011.x - d010.x) * fx, d010.y + (d011.y - d010.y) * fx);
+    float2 dx11 = make_float2(d110.x + (d111.x - d110.x) * fx, d110.y + (d111.y - d110.y) * fx);
+    float2 dxy0 = make_float2(dx00.x + (dx10.x - dx00.x) * fy, dx00.y + (dx10.y - dx00.y) * fy);
+    float2 dxy1 = make_float2(dx01.x + (dx11.x - dx01.x) * fy, dx01.y + (dx11.y - dx01.y) * fy);
+    float2 v = make_float2(dxy0.x + (dxy1.x - dxy0.x) * fz, dxy0.y + (dxy1.y - dxy0.y) * fz);
+    if (neg) v.y = -v.y;
+    *out = v;
+    return true;
+}
+
+// CTF::getCTF (ctf.h:181-230) with precomputed constants; double precision.
+struct CtfConst {
+    double K1, K2, K3, K4, K5, Axx, Axy, Ayy, scale;
+};
+
+inline CtfConst ctf_const(const t2x_ctf_params& p)
+{
+    CtfConst c;
+    double local_Cs = p.cs_mm * 1e7, local_kV = p.voltage_kv * 1e3;
+    double rad = p.defocus_angle * 3.14159265358979323846 / 180.;
+    double lambda = 12.2643247 / std::sqrt(local_kV * (1. + local_kV * 0.978466e-6));
+    c.K1 = 3.14159265358979323846 / 2 * 2 * lambda;
+    c.K2 = 3.14159265358979323846 / 2 * local_Cs * lambda * lambda * lambda;
+    c.K3 = std::atan(p.q0 / std::sqrt(1 - p.q0 * p.q0));
+    c.K4 = -p.bfactor / 4.;
+    c.K5 = p.phase_shift * 3.14159265358979323846 / 180.;
+    double s = std::sin(rad), co = std::cos(rad);
+    double dU = -p.defocus_u, dV = -p.defocus_v;
+    c.Axx = co * co * dU + s * s * dV;
+    c.Axy = co * s * dU - s * co * dV;
+    c.Ayy = s * s * dU + co * co * dV;
+    c.scale = p.scale;
+    return c;
+}
+
+__device__ __forceinline__ double ctf_value(const CtfConst& c, double X, double Y)
+{
+    double u2 = X * X + Y * Y;
+    double gamma = c.K1 * (c.Axx * X * X + 2.0 * c.Axy * X * Y + c.Ayy * Y * Y) + c.K2 * u2 * u2 - c.K5 - c.K3;
+    double r = -sin(gamma) * exp(c.K4 * u2) * c.scale;
+    if (fabs(r) < 1e-8) r = (r >= 0 ? 1e-8 : -1e-8);
+    return r;
+}
+
+}  // namespace t2x

@@ -1,0 +1,58 @@
+// t2x_kernels.h — kernel declarations shared by the E-step driver.
+#pragma once
+#include "t2x_internal.h"
+
+namespace t2x {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// Per-particle preprocessing scalars (device, double).
+struct PartScalars {
+    double highres_Xi2;   // sum |F|^2 beyond current_size (ml_optimiser.cpp:6215-6240)
+    double old_x, old_y;  // rounded old offset (px)
+    double prior_x, prior_y;
+    double norm_scale;    // avg_norm / normcorr (1 when norm correction is off)
+    double C_coarse;      // highres/2 + 1/2 sum_Mc minv |X|^2
+    double C_fine;        // highres/2 + 1/2 sum_Mf minv |X|^2
+};
+
+// Per-particle selection results (device).
+struct SelResult {
+    double sum_w;       // sum of weights
+    double sig_w;       // significant weight threshold
+    double min_d2;      // min diff2 used for the weights
+    long long nsig;     // nr of significant samples counted by the threshold loop
+    long long npos;     // nr of positive weights
+    double max_w;       // max weight
+    long long best_key; // CPU store-order key of the max weight (fine pass)
+    long long best_e;   // element index of the max weight
+};
+
+enum { SRC_COARSE = 0, SRC_FINE = 1 };
+
+// Description of the weights of one pass, read by the selection kernels.
+struct WeightSrc {
+    int kind;                  // SRC_COARSE / SRC_FINE
+    const float* d2;           // coarse: [B][T][ld]; fine: [total fine]
+    long long ld;              // coarse: row stride (orientations, padded)
+    int T;                     // coarse translations
+    int O_all;                 // coarse orientations (all classes)
+    const double* prior_o;     // [O_all]
+    const double* prior_t;     // [B][T]
+    const float* rowmin;       // coarse: [B*T][ld/64] per-row minima from the GEMM
+    int rowmin_ld;
+    // fine
+    const long long* pair_off; // [B+1] offsets into pair lists (pairs)
+    const int2* pairs;         // (n, t) per significant coarse pair
+    const unsigned int* fine_min; // [B] ordered-uint minima
+    int nover;                 // fine samples per pair (32)
+    int over_trans;            // 4
+};
+
+struct SelParams {
+    double fraction;
+    long long max_sig;
+    int count_sig;
+};
+
+}  // namespace t2x

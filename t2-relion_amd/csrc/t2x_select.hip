@@ -1,0 +1,522 @@
+// t2x_select.hip — FP64 weights and significance selection of one E-step pass.
+//
+// Restates convertAllSquaredDifferencesToWeights (ml_optimiser.cpp:7510-8041):
+//   w = pdf_orientation/mean * pdf_offset/mean * exp(-(diff2 - min_diff2)), 0 if diff2-min > 700;
+//   the significant weight is found by accumulating the positive weights in descending order
+//   until the running sum exceeds adaptive_fraction * sum (ml_optimiser.cpp:7947-7967).
+// Instead of sorting every weight (the CUB radix sort of acc_ml_optimiser_impl.h:1430-1469),
+// one workgroup per particle runs an exact radix *select* on the IEEE-754 bit pattern of the
+// positive FP64 weights: per-digit counts and weight sums locate the bin where the descending
+// cumulative sum crosses the target; the final bin (<= SEL_CAP elements) is sorted in LDS and
+// walked sequentially, so the threshold value and count equal those of the sorted walk.
+#include "t2x_kernels.h"
+
+namespace t2x {
+
+#define SEL_THREADS 1024
+#define SEL_BINS 4096
+#define SEL_CAP 1024
+
+__device__ __forceinline__ double pass_weight(const WeightSrc& s, int b, long long e, double mind2,
+                                              const long long* pair_off_b_cache)
+{
+    float d;
+    double prior;
+    if (s.kind == SRC_COARSE) {
+        long long t = e / s.O_all, n = e - t * s.O_all;
+        prior = s.prior_o[n] * s.prior_t[(size_t)b * s.T + t];
+        if (!(prior > 0.)) return 0.;
+        d = s.d2[((size_t)b * s.T + t) * s.ld + n];
+    } else {
+        long long base = pair_off_b_cache[0];
+        long long k = base + e / s.nover;
+        int2 nt = s.pairs[k];
+        prior = s.prior_o[nt.x] * s.prior_t[(size_t)b * s.T + nt.y];
+        if (!(prior > 0.)) return 0.;
+        d = s.d2[base * s.nover + e];
+    }
+    double dd = (double)d - mind2;
+    if (dd > 700.) return 0.;
+    return prior * exp(-dd);
+}
+
+__device__ __forceinline__ long long pass_count(const WeightSrc& s, int b)
+{
+    if (s.kind == SRC_COARSE) return (long long)s.T * s.O_all;
+    return (s.pair_off[b + 1] - s.pair_off[b]) * (long long)s.nover;
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long wave_sum_u64(unsigned long long v)
+{
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    return v;
+}
+
+__device__ __forceinline__ float wave_min(float v)
+{
+    for (int o = 32; o > 0; o >>= 1) v = fminf(v, __shfl_xor(v, o));
+    return v;
+}
+
+// digit layout of a positive double: 12+12+12+12+12+4 bits from the top
+__device__ __forceinline__ int digit_shift(int L) { return L < 5 ? 52 - 12 * L : 0; }
+__device__ __forceinline__ int digit_width(int L) { return L < 5 ? 12 : 4; }
+
+// Minimum diff2 of the pass: coarse from the GEMM row minima, fine from the atomic minimum.
+__global__ void k_pass_min(WeightSrc s, int B, SelResult* res)
+{
+    int b = blockIdx.x;
+    if (b >= B) return;
+    __shared__ float red[SEL_THREADS / 64];
+    float m = 3.0e38f;
+    if (s.kind == SRC_COARSE) {
+        long long nr = (long long)s.T * s.rowmin_ld;
+        const float* rm = s.rowmin + (size_t)b * s.T * s.rowmin_ld;
+        for (long long i = threadIdx.x; i < nr; i += blockDim.x) m = fminf(m, rm[i]);
+    } else if (threadIdx.x == 0) {
+        m = ordered_to_float(s.fine_min[b]);
+    }
+    m = wave_min(m);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float mm = red[0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++) mm = fminf(mm, red[i]);
+        res[b].min_d2 = (double)mm;
+    }
+}
+
+struct SelShared {
+    unsigned int cnt[SEL_BINS];
+    double sum[SEL_BINS];
+    double buf[SEL_CAP];
+    double red[SEL_THREADS / 64];
+    unsigned long long redu[SEL_THREADS / 64];
+    double acc;            // weight sum of elements in digits above the chosen bin
+    long long cnt_above;   // number of elements above the chosen bin
+    unsigned long long prefix;
+    int digit;
+    int found;
+    int mode;              // 0: cumulative-sum crossing, 1: rank
+    long long rank;        // target rank (1-based, descending) in rank mode
+    long long bin_count;
+};
+
+// Chooses the digit of level L from the LDS histogram (one wave; the highest digit first).
+__device__ void choose_digit(SelShared& S, double target, int nbins)
+{
+    // 64 lanes x (nbins/64) bins; lane 0 owns the highest bins
+    const int lane = threadIdx.x;  // called by wave 0 only
+    const int per = nbins / 64;
+    const int hi = nbins - 1 - lane * per;   // lane's highest bin
+    double s = 0.;
+    unsigned long long c = 0;
+    for (int i = 0; i < per; i++) {
+        s += S.sum[hi - i];
+        c += S.cnt[hi - i];
+    }
+    // exclusive prefix over lanes (descending bins)
+    double ps = s;
+    unsigned long long pc = c;
+    for (int o = 1; o < 64; o <<= 1) {
+        double ts = __shfl_up(ps, o);
+        unsigned long long tc = __shfl_up(pc, o);
+        if (lane >= o) { ps += ts; pc += tc; }
+    }
+    double excl_s = ps - s;
+    unsigned long long excl_c = pc - c;
+    bool hit;
+    if (S.mode == 0) hit = (S.acc + excl_s <= target) && (S.acc + ps > target);
+    else hit = (S.cnt_above + (long long)excl_c < S.rank) && (S.cnt_above + (long long)pc >= S.rank);
+    unsigned long long ball = __ballot(hit);
+    int src = ball ? (__ffsll((long long)ball) - 1) : -1;
+    if (src < 0) {
+        // no crossing inside this prefix bin (summation-order rounding): take the lowest
+        // non-empty digit, i.e. the last element of the bin in descending order
+        unsigned long long nz = __ballot(c > 0);
+        int last = nz ? (63 - __clzll((long long)nz)) : 0;
+        src = last;
+        if (lane == src) {
+            int dsel = -1;
+            double a = S.acc + excl_s;
+            long long ca = S.cnt_above + (long long)excl_c;
+            for (int i = 0; i < per; i++) {
+                int d = hi - i;
+                if (S.cnt[d] > 0) dsel = d;
+            }
+            // accumulate everything above dsel
+            for (int i = 0; i < per; i++) {
+                int d = hi - i;
+                if (d == dsel) break;
+                a += S.sum[d];
+                ca += S.cnt[d];
+            }
+            S.acc = a;
+            S.cnt_above = ca;
+            S.digit = dsel;
+            S.bin_count = S.cnt[dsel];
+            S.found = 0;
+        }
+        return;
+    }
+    if (lane == src) {
+        double a = S.acc + excl_s;
+        long long ca = S.cnt_above + (long long)excl_c;
+        int dsel = hi - per + 1;
+        for (int i = 0; i < per; i++) {
+            int d = hi - i;
+            bool h = (S.mode == 0) ? (a + S.sum[d] > target) : (ca + (long long)S.cnt[d] >= S.rank);
+            if (h) { dsel = d; break; }
+            a += S.sum[d];
+            ca += S.cnt[d];
+        }
+        S.acc = a;
+        S.cnt_above = ca;
+        S.digit = dsel;
+        S.bin_count = S.cnt[dsel];
+        S.found = 1;
+    }
+}
+
+// Bitonic sort (descending) of n <= SEL_CAP doubles in LDS; pads with -1.
+__device__ void sort_desc(double* a, int n)
+{
+    int m = 1;
+    while (m < n) m <<= 1;
+    for (int i = n + threadIdx.x; i < m; i += blockDim.x) a[i] = -1.0;
+    __syncthreads();
+    for (int k = 2; k <= m; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < m; i += blockDim.x) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    bool desc = ((i & k) == 0);
+                    double x = a[i], y = a[ixj];
+                    if (desc ? (x < y) : (x > y)) { a[i] = y; a[ixj] = x; }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// One selection run (crossing or rank mode) over the positive weights of particle b.
+// Returns the threshold in S.buf[0] slot via out params (thread 0).
+__device__ void select_run(const WeightSrc& s, int b, double mind2, long long n_elem,
+                           const long long* poff, SelShared& S, double target, int mode,
+                           long long rank, bool have_level0, double* out_w, long long* out_n)
+{
+    if (threadIdx.x == 0) {
+        S.acc = 0.;
+        S.cnt_above = 0;
+        S.prefix = 0;
+        S.mode = mode;
+        S.rank = rank;
+    }
+    __syncthreads();
+    int L = 0;
+    for (;; L++) {
+        const int shift = digit_shift(L), width = digit_width(L);
+        const int nb = 1 << width;
+        if (!(L == 0 && have_level0)) {
+            for (int i = threadIdx.x; i < SEL_BINS; i += blockDim.x) { S.cnt[i] = 0; S.sum[i] = 0.; }
+            __syncthreads();
+            const unsigned long long pref = S.prefix;
+            const int pshift = shift + width;
+            for (long long e = threadIdx.x; e < n_elem; e += blockDim.x) {
+                double w = pass_weight(s, b, e, mind2, poff);
+                if (!(w > 0.)) continue;
+                unsigned long long bits = (unsigned long long)__double_as_longlong(w);
+                if (L > 0 && (bits >> pshift) != pref) continue;
+                int d = (int)((bits >> shift) & (unsigned long long)(nb - 1));
+                atomicAdd(&S.cnt[d], 1u);
+                atomicAdd(&S.sum[d], w);
+            }
+            __syncthreads();
+        }
+        if (threadIdx.x < 64) choose_digit(S, target, nb < 64 ? 64 : nb);
+        __syncthreads();
+        if (threadIdx.x == 0) S.prefix = (S.prefix << width) | (unsigned long long)S.digit;
+        __syncthreads();
+        if (S.bin_count <= SEL_CAP || L == 5) break;
+    }
+    // gather the chosen bin
+    const int shift = digit_shift(L);
+    const unsigned long long pref = S.prefix;
+    const long long bc = S.bin_count;
+    if (bc <= SEL_CAP) {
+        __shared__ unsigned int gcount;
+        if (threadIdx.x == 0) gcount = 0;
+        __syncthreads();
+        for (long long e = threadIdx.x; e < n_elem; e += blockDim.x) {
+            double w = pass_weight(s, b, e, mind2, poff);
+            if (!(w > 0.)) continue;
+            unsigned long long bits = (unsigned long long)__double_as_longlong(w);
+            if ((bits >> shift) != pref) continue;
+            unsigned int slot = atomicAdd(&gcount, 1u);
+            if (slot < SEL_CAP) S.buf[slot] = w;
+        }
+        __syncthreads();
+        int n = (int)min((unsigned int)SEL_CAP, gcount);
+        sort_desc(S.buf, n);
+        if (threadIdx.x == 0) {
+            if (mode == 0) {
+                double frac = S.acc;
+                int i = 0;
+                for (; i < n; i++) {
+                    frac += S.buf[i];
+                    if (frac > target) break;
+                }
+                if (i >= n) i = n - 1;
+                *out_w = S.buf[i];
+                *out_n = S.cnt_above + i + 1;
+            } else {
+                long long r = S.rank - S.cnt_above - 1;
+                if (r < 0) r = 0;
+                if (r >= n) r = n - 1;
+                *out_w = S.buf[r];
+                *out_n = S.rank;
+            }
+        }
+    } else if (threadIdx.x == 0) {
+        // every element of the bin has the same value (all 64 bits fixed)
+        double v = __longlong_as_double((long long)pref);
+        if (mode == 0) {
+            double frac = S.acc;
+            long long i = 0;
+            for (; i < bc; i++) {
+                frac += v;
+                if (frac > target) break;
+            }
+            if (i >= bc) i = bc - 1;
+            *out_w = v;
+            *out_n = S.cnt_above + i + 1;
+        } else {
+            *out_w = v;
+            *out_n = S.rank;
+        }
+    }
+    __syncthreads();
+}
+
+__global__ __launch_bounds__(SEL_THREADS) void k_select(WeightSrc s, SelParams prm, int B, SelResult* res)
+{
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    __shared__ SelShared S;
+    __shared__ long long poff[2];
+    if (threadIdx.x == 0 && s.kind == SRC_FINE) { poff[0] = s.pair_off[b]; poff[1] = s.pair_off[b + 1]; }
+    __syncthreads();
+    const double mind2 = res[b].min_d2;
+    const long long n_elem = pass_count(s, b);
+    // level-0 histogram together with the total sum and the positive count
+    for (int i = threadIdx.x; i < SEL_BINS; i += blockDim.x) { S.cnt[i] = 0; S.sum[i] = 0.; }
+    __syncthreads();
+    double lsum = 0.;
+    unsigned long long lpos = 0;
+    double lmax = 0.;
+    for (long long e = threadIdx.x; e < n_elem; e += blockDim.x) {
+        double w = pass_weight(s, b, e, mind2, poff);
+        if (!(w > 0.)) continue;
+        lsum += w;
+        lpos++;
+        lmax = fmax(lmax, w);
+        unsigned long long bits = (unsigned long long)__double_as_longlong(w);
+        int d = (int)(bits >> 52);
+        atomicAdd(&S.cnt[d], 1u);
+        atomicAdd(&S.sum[d], w);
+    }
+    lsum = wave_sum(lsum);
+    lpos = wave_sum_u64(lpos);
+    for (int o = 32; o > 0; o >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, o));
+    __shared__ double smax[SEL_THREADS / 64];
+    if ((threadIdx.x & 63) == 0) {
+        S.red[threadIdx.x >> 6] = lsum;
+        S.redu[threadIdx.x >> 6] = lpos;
+        smax[threadIdx.x >> 6] = lmax;
+    }
+    __syncthreads();
+    __shared__ double total;
+    __shared__ long long npos;
+    __shared__ double gmax;
+    if (threadIdx.x == 0) {
+        double t = 0.;
+        unsigned long long c = 0;
+        double m = 0.;
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) { t += S.red[i]; c += S.redu[i]; m = fmax(m, smax[i]); }
+        total = t;
+        npos = (long long)c;
+        gmax = m;
+    }
+    __syncthreads();
+    if (npos == 0) {
+        if (threadIdx.x == 0) {
+            res[b].sum_w = total;
+            res[b].sig_w = 0.;
+            res[b].nsig = 0;
+            res[b].npos = 0;
+            res[b].max_w = 0.;
+        }
+        return;
+    }
+    const double target = prm.fraction * total;
+    __shared__ double thr_w;
+    __shared__ long long thr_n;
+    // crossing mode; when the total never exceeds the target the walk ends at the smallest weight
+    bool crossing_possible = total > target;
+    if (crossing_possible)
+        select_run(s, b, mind2, n_elem, poff, S, target, 0, 0, true, &thr_w, &thr_n);
+    else
+        select_run(s, b, mind2, n_elem, poff, S, target, 1, npos, true, &thr_w, &thr_n);
+    __syncthreads();
+    if (prm.count_sig && prm.max_sig > 0 && thr_n > prm.max_sig) {
+        select_run(s, b, mind2, n_elem, poff, S, target, 1, prm.max_sig, false, &thr_w, &thr_n);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        res[b].sum_w = total;
+        res[b].sig_w = thr_w;
+        res[b].nsig = thr_n;
+        res[b].npos = npos;
+        res[b].max_w = gmax;
+    }
+}
+
+// Counts, per particle, the coarse (orientation, translation) pairs with w >= sig_w and the
+// orientations having at least one such pair (Mcoarse_significant, ml_optimiser.cpp:7996-8004;
+// isSignificantAnyImageAnyTranslation, :6898-6919).
+__global__ __launch_bounds__(1024) void k_sig_count(WeightSrc s, int B, const SelResult* res,
+                                                    long long* counts /* [B][2] */)
+{
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    const double mind2 = res[b].min_d2, sig = res[b].sig_w;
+    unsigned long long np = 0, no = 0;
+    for (int n = threadIdx.x; n < s.O_all; n += blockDim.x) {
+        int c = 0;
+        for (int t = 0; t < s.T; t++) {
+            double w = pass_weight(s, b, (long long)t * s.O_all + n, mind2, nullptr);
+            if (w >= sig && w > 0.) c++;
+        }
+        np += c;
+        no += (c > 0);
+    }
+    np = wave_sum_u64(np);
+    no = wave_sum_u64(no);
+    __shared__ unsigned long long r1[16], r2[16];
+    if ((threadIdx.x & 63) == 0) { r1[threadIdx.x >> 6] = np; r2[threadIdx.x >> 6] = no; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long a = 0, o = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) { a += r1[i]; o += r2[i]; }
+        counts[2 * b] = (long long)a;
+        counts[2 * b + 1] = (long long)o;
+    }
+}
+
+// Writes the ordered (orientation-major, then translation) significant pairs and orientations.
+__global__ __launch_bounds__(1024) void k_sig_write(WeightSrc s, int B, const SelResult* res,
+                                                    const long long* pair_off, const long long* orient_off,
+                                                    int2* pairs, int* orients, long long* orient_pair_start)
+{
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    const double mind2 = res[b].min_d2, sig = res[b].sig_w;
+    __shared__ int scan_p[1024], scan_o[1024];
+    __shared__ long long run_p, run_o;
+    if (threadIdx.x == 0) { run_p = pair_off[b]; run_o = orient_off[b]; }
+    __syncthreads();
+    for (int base = 0; base < s.O_all; base += blockDim.x) {
+        int n = base + threadIdx.x;
+        unsigned long long mask[2] = {0ull, 0ull};   // up to 128 translations
+        int c = 0;
+        if (n < s.O_all) {
+            for (int t = 0; t < s.T; t++) {
+                double w = pass_weight(s, b, (long long)t * s.O_all + n, mind2, nullptr);
+                if (w >= sig && w > 0.) { mask[t >> 6] |= 1ull << (t & 63); c++; }
+            }
+        }
+        scan_p[threadIdx.x] = c;
+        scan_o[threadIdx.x] = c > 0;
+        __syncthreads();
+        for (int o = 1; o < (int)blockDim.x; o <<= 1) {
+            int vp = threadIdx.x >= o ? scan_p[threadIdx.x - o] : 0;
+            int vo = threadIdx.x >= o ? scan_o[threadIdx.x - o] : 0;
+            __syncthreads();
+            scan_p[threadIdx.x] += vp;
+            scan_o[threadIdx.x] += vo;
+            __syncthreads();
+        }
+        long long p0 = run_p + scan_p[threadIdx.x] - c;
+        long long o0 = run_o + scan_o[threadIdx.x] - (c > 0);
+        if (c > 0) {
+            orients[o0] = n;
+            orient_pair_start[o0] = p0;
+            int k = 0;
+            for (int t = 0; t < s.T; t++)
+                if (mask[t >> 6] & (1ull << (t & 63))) pairs[p0 + (k++)] = make_int2(n, t);
+        }
+        __syncthreads();
+        if (threadIdx.x == blockDim.x - 1) { run_p = p0 + c; run_o = o0 + (c > 0); }
+        __syncthreads();
+    }
+}
+
+// Max weight of the fine pass and its CPU store-order key (storeWeightedSums walks
+// class, direction, psi, oversampled rotation, translation, oversampled translation and keeps
+// the first strict maximum, ml_optimiser.cpp:8643-8700); also counts weights >= sig_w.
+__global__ __launch_bounds__(1024) void k_fine_argmax(WeightSrc s, int B, SelResult* res,
+                                                      long long* nfine_sig)
+{
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    __shared__ long long poff[2];
+    if (threadIdx.x == 0) { poff[0] = s.pair_off[b]; poff[1] = s.pair_off[b + 1]; }
+    __syncthreads();
+    const double mind2 = res[b].min_d2, sig = res[b].sig_w;
+    const long long n_elem = (poff[1] - poff[0]) * (long long)s.nover;
+    double bw = -1.;
+    long long bk = LLONG_MAX, be = -1;
+    unsigned long long cnt = 0;
+    const int ot = s.over_trans;
+    for (long long e = threadIdx.x; e < n_elem; e += blockDim.x) {
+        double w = pass_weight(s, b, e, mind2, poff);
+        if (w >= sig && w > 0.) cnt++;
+        int2 nt = s.pairs[poff[0] + e / s.nover];
+        int q = (int)(e % s.nover);
+        int ir = q / ot, iot = q % ot;
+        long long key = (((long long)nt.x * (s.nover / ot) + ir) * s.T + nt.y) * ot + iot;
+        if (w > bw || (w == bw && key < bk)) { bw = w; bk = key; be = e; }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        double ow = __shfl_xor(bw, o);
+        long long ok = __shfl_xor(bk, o), oe = __shfl_xor(be, o);
+        if (ow > bw || (ow == bw && ok < bk)) { bw = ow; bk = ok; be = oe; }
+    }
+    cnt = wave_sum_u64(cnt);
+    __shared__ double sw[16];
+    __shared__ long long sk[16], se[16];
+    __shared__ unsigned long long sc[16];
+    if ((threadIdx.x & 63) == 0) { sw[threadIdx.x >> 6] = bw; sk[threadIdx.x >> 6] = bk; se[threadIdx.x >> 6] = be; sc[threadIdx.x >> 6] = cnt; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double w = sw[0];
+        long long k = sk[0], ee = se[0];
+        unsigned long long c = sc[0];
+        for (int i = 1; i < (int)(blockDim.x >> 6); i++) {
+            if (sw[i] > w || (sw[i] == w && sk[i] < k)) { w = sw[i]; k = sk[i]; ee = se[i]; }
+            c += sc[i];
+        }
+        res[b].max_w = w;
+        res[b].best_key = k;
+        res[b].best_e = ee;
+        nfine_sig[b] = (long long)c;
+    }
+}
+
+}  // namespace t2x

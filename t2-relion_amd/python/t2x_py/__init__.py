@@ -1,0 +1,202 @@
+"""ctypes bindings of the t2x C ABI (include/t2x.h).
+
+The structs mirror include/t2x.h field by field. `load()` opens the in-tree libt2x.so and
+raises if it is missing: there is no CPU fallback for the E-step.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(os.path.dirname(HERE))          # .../t2-relion_amd
+REPO_ROOT = os.path.dirname(PKG_ROOT)
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libt2x.so")
+
+T2X_OK = 0
+T2X_EINVAL = -1
+T2X_ENOMEM = -2
+T2X_EDEVICE = -3
+T2X_ESTATE = -4
+T2X_EZEROSUM = -5
+T2X_ENOSIG = -6
+
+T2X_PREC_FP32 = 0
+T2X_PREC_BF16X3 = 1
+T2X_PREC_FP16X3 = 2
+
+
+class CtfParams(C.Structure):
+    _fields_ = [(n, C.c_double) for n in (
+        "voltage_kv", "cs_mm", "q0", "defocus_u", "defocus_v", "defocus_angle",
+        "bfactor", "phase_shift", "scale")]
+
+
+class EstepConfig(C.Structure):
+    _fields_ = [
+        ("ori_size", C.c_int), ("pixel_size", C.c_double),
+        ("current_size", C.c_int), ("coarse_size", C.c_int),
+        ("nr_classes", C.c_int), ("ref_dim", C.c_int),
+        ("healpix_order", C.c_int), ("psi_step", C.c_double),
+        ("offset_range", C.c_double), ("offset_step", C.c_double),
+        ("adaptive_oversampling", C.c_int), ("adaptive_fraction", C.c_double),
+        ("maximum_significants", C.c_long), ("particle_diameter", C.c_double),
+        ("width_mask_edge", C.c_int), ("sigma2_fudge", C.c_double),
+        ("do_norm_correction", C.c_int), ("avg_norm_correction", C.c_double),
+        ("do_ctf_correction", C.c_int), ("sigma2_offset", C.c_double),
+        ("orientational_prior_mode", C.c_int), ("precision", C.c_int),
+        ("reserved", C.c_int * 8),
+    ]
+
+
+class Model(C.Structure):
+    _fields_ = [("sigma2_noise", C.POINTER(C.c_double)), ("pdf_class", C.POINTER(C.c_double)),
+                ("pdf_direction", C.POINTER(C.c_double)),
+                ("prior_offset_class", C.POINTER(C.c_double))]
+
+
+class ParticleIn(C.Structure):
+    _fields_ = [("ctf", CtfParams),
+                ("old_offset_x", C.c_double), ("old_offset_y", C.c_double),
+                ("prior_offset_x", C.c_double), ("prior_offset_y", C.c_double),
+                ("norm_correction", C.c_double),
+                ("rot", C.c_double), ("tilt", C.c_double), ("psi", C.c_double)]
+
+
+class ParticleOut(C.Structure):
+    _fields_ = [("rot", C.c_double), ("tilt", C.c_double), ("psi", C.c_double),
+                ("xoff", C.c_double), ("yoff", C.c_double),
+                ("iclass", C.c_int), ("pad0", C.c_int),
+                ("pmax", C.c_double), ("dLL", C.c_double), ("norm_correction", C.c_double),
+                ("sum_weight", C.c_double), ("min_diff2", C.c_double),
+                ("significant_weight", C.c_double),
+                ("nr_significant_coarse", C.c_long), ("nr_fine_samples", C.c_long),
+                ("nr_fine_significant", C.c_long), ("best_hidden", C.c_long)]
+
+
+PD = C.POINTER(C.c_double)
+
+
+class Wsum(C.Structure):
+    _fields_ = [(n, PD) for n in (
+        "sigma2_noise", "sumw_group", "pdf_class", "pdf_direction", "prior_offset_class",
+        "sigma2_offset", "avg_norm_correction", "LL", "ave_Pmax")]
+
+
+class EstepDebug(C.Structure):
+    _fields_ = [("n_debug", C.c_int),
+                ("coarse_diff2", C.POINTER(C.c_float)),
+                ("coarse_significant", C.POINTER(C.c_ubyte)),
+                ("fine_capacity", C.c_long),
+                ("fine_hidden", C.POINTER(C.c_long)),
+                ("fine_diff2", C.POINTER(C.c_float)),
+                ("fine_count", C.POINTER(C.c_long)),
+                ("coarse_min_diff2", PD), ("coarse_sum_weight", PD), ("coarse_sig_weight", PD)]
+
+
+class SamplingInfo(C.Structure):
+    _fields_ = [(n, C.c_long) for n in (
+        "nr_dir", "nr_psi", "nr_trans", "nr_over_rot", "nr_over_trans",
+        "nr_coarse_per_class", "nr_hidden_coarse")]
+
+
+class Stats(C.Structure):
+    _fields_ = [("n_calls", C.c_int), ("n_particles", C.c_long)] + \
+        [(n, C.c_double) for n in ("ms_preprocess", "ms_coarse_project", "ms_coarse_diff2",
+                                   "ms_coarse_weights", "ms_fine_diff2", "ms_fine_weights",
+                                   "ms_store")] + \
+        [("n_coarse_diff2_launches", C.c_long)] + \
+        [(n, C.c_double) for n in ("coarse_diff2_flop", "coarse_diff2_bytes", "fine_samples",
+                                   "fine_flop", "bp_orientations", "sig_coarse")] + \
+        [(n, C.c_int) for n in ("coarse_size", "current_size", "P_coarse", "P_fine")]
+
+
+class Blob(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("x", "y", "z", "sigma", "amplitude")]
+
+
+def dptr(a):
+    """double* of a contiguous float64 numpy array (or NULL)."""
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(PD)
+
+
+def fptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def make_wsum(K, n_dir, ori_size):
+    """numpy accumulators + a Wsum struct pointing at them."""
+    arrs = {
+        "sigma2_noise": np.zeros(ori_size // 2 + 1), "sumw_group": np.zeros(1),
+        "pdf_class": np.zeros(K), "pdf_direction": np.zeros(K * n_dir),
+        "prior_offset_class": np.zeros(2 * K), "sigma2_offset": np.zeros(1),
+        "avg_norm_correction": np.zeros(1), "LL": np.zeros(1), "ave_Pmax": np.zeros(1)}
+    ws = Wsum(**{k: dptr(v) for k, v in arrs.items()})
+    return ws, arrs
+
+
+def outs_to_dict(outs):
+    names = [f[0] for f in ParticleOut._fields_ if f[0] != "pad0"]
+    return {n: np.array([getattr(o, n) for o in outs]) for n in names}
+
+
+_lib = None
+
+
+def load(path=None):
+    """Open libt2x.so; raise if it is not built (the E-step has no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    p = path or LIB_PATH
+    if not os.path.exists(p):
+        raise RuntimeError(f"libt2x.so not built at {p}: run __graft_entry__.build()")
+    lib = C.CDLL(p)
+    vp = C.c_void_p
+    sig = {
+        "t2x_abi_version": (C.c_int, []),
+        "t2x_last_error": (C.c_char_p, []),
+        "t2x_device_open": (C.c_int, [C.c_int, C.POINTER(vp)]),
+        "t2x_device_close": (C.c_int, [vp]),
+        "t2x_projector_set": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                        C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "t2x_backprojector_init": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
+        "t2x_backprojector_get": (C.c_int, [vp, C.c_int, PD, PD, PD]),
+        "t2x_backprojector_clear": (C.c_int, [vp, C.c_int]),
+        "t2x_sampling_info_get": (C.c_int, [C.POINTER(EstepConfig), C.POINTER(SamplingInfo)]),
+        "t2x_estep": (C.c_int, [vp, C.POINTER(EstepConfig), C.POINTER(Model), C.c_int, vp, C.c_int,
+                                C.POINTER(ParticleIn), C.POINTER(ParticleOut), C.POINTER(Wsum),
+                                C.POINTER(EstepDebug)]),
+        "t2x_stats_get": (C.c_int, [vp, C.POINTER(Stats)]),
+        "t2x_stats_reset": (C.c_int, [vp]),
+        "t2x_device_alloc": (C.c_int, [vp, C.c_size_t, C.POINTER(vp)]),
+        "t2x_device_free": (C.c_int, [vp, vp]),
+        "t2x_memcpy_h2d": (C.c_int, [vp, vp, vp, C.c_size_t]),
+        "t2x_memcpy_d2h": (C.c_int, [vp, vp, vp, C.c_size_t]),
+        "t2x_device_synchronize": (C.c_int, [vp]),
+        "t2x_synth_reference_device": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int,
+                                                 C.c_double, C.POINTER(Blob), C.c_int, C.c_double,
+                                                 vp, vp]),
+        "t2x_synth_particles_device": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.POINTER(Blob),
+                                                 C.POINTER(C.c_int), C.POINTER(ParticleIn), PD, PD,
+                                                 PD, PD, PD, C.POINTER(C.c_int), PD, C.c_uint64,
+                                                 vp, PD]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def check(status, what="t2x call"):
+    if status != T2X_OK:
+        msg = load().t2x_last_error()
+        raise RuntimeError(f"{what} failed ({status}): {msg.decode() if msg else ''}")
