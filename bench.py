@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""E-step throughput on one MI355X: the 3D-classification workload (20,000 particles of 256^2,
+K=4 references of 256^3 at 1.1 A/px, HEALPix order 3, offsets 6/1.5 px), plain FP32 path.
+
+A step is one full E-step iteration over all particles, which are synthesised once on the GPU
+and stay resident in HBM: reference upload + coarse projections, preprocessing, both sampling
+passes, significance, weighted sums and backprojection. The coarse-diff2 GEMM (the dominant
+kernel) is timed with HIP events on the library's own stream inside the step.
+
+The CPU baseline times the FP64 oracle (the restatement of relion_refine's CPU E-step) on a
+bounded sample of the same workload with one particle per thread.
+"""
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(REPO, "t2-relion_amd", "python"))
+import t2x_py as T  # noqa: E402
+from t2x_py import workloads as W  # noqa: E402
+
+METRIC = "E-step particles/s, 1 MI355X: plain FP32 vs mixed+learned at equal FSC-0.143"
+MFMA_FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_dataset(lib, dev, w, n, cfg, sz, seed):
+    rng = np.random.default_rng(seed)
+    K, nb = cfg.nr_classes, w["n_blobs"]
+    blob_sets = [W.gen_blobs(rng, nb, w["blob_radius"], w["blob_sigma"], w["blob_amp"]) for _ in range(K)]
+    allb = (T.Blob * (nb * K))()
+    for c in range(K):
+        for k in range(nb):
+            allb[c * nb + k] = blob_sets[c][k]
+    boff = (C.c_int * (K + 1))(*[c * nb for c in range(K + 1)])
+    pin, tr = W.gen_particles(rng, n, K, w["max_shift"])
+    N = cfg.ori_size
+    dimg = C.c_void_p()
+    T.check(lib.t2x_device_alloc(dev, n * N * N * 4, C.byref(dimg)), "alloc images")
+    snr = np.full(n, w["snr"])
+    noise_var = np.zeros(n)
+    T.check(lib.t2x_synth_particles_device(dev, N, cfg.pixel_size, n, allb, boff, pin,
+                                           T.dptr(tr["rot"]), T.dptr(tr["tilt"]), T.dptr(tr["psi"]),
+                                           T.dptr(tr["x"]), T.dptr(tr["y"]),
+                                           tr["cls"].ctypes.data_as(C.POINTER(C.c_int)),
+                                           T.dptr(snr), seed + 7, dimg, T.dptr(noise_var)),
+            "synth particles")
+    pad, rmax = sz["pad_size"], sz["r_max"]
+    nvox = pad * pad * (pad // 2 + 1)
+    refs = []
+    for c in range(K):
+        dre, dim = C.c_void_p(), C.c_void_p()
+        T.check(lib.t2x_device_alloc(dev, nvox * 4, C.byref(dre)), "alloc ref")
+        T.check(lib.t2x_device_alloc(dev, nvox * 4, C.byref(dim)), "alloc ref")
+        one = (T.Blob * nb)(*[blob_sets[c][k] for k in range(nb)])
+        T.check(lib.t2x_synth_reference_device(dev, N, cfg.pixel_size, pad, rmax, 2.0, one, nb,
+                                               w["resolution"], dre, dim), "synth reference")
+        refs.append((dre, dim))
+    info = T.SamplingInfo()
+    T.check(lib.t2x_sampling_info_get(C.byref(cfg), C.byref(info)), "sampling")
+    sigma2, pdf_class, pdf_dir = W.model_arrays(cfg, info.nr_dir, noise_var)
+    return dict(dimg=dimg, pin=pin, truth=tr, refs=refs, sigma2=sigma2, pdf_class=pdf_class,
+                pdf_dir=pdf_dir, info=info, allb=allb, boff=boff, blob_sets=blob_sets,
+                noise_var=noise_var)
+
+
+def run_step(lib, dev, cfg, sz, ds, n, outs, ws):
+    pad, rmax = sz["pad_size"], sz["r_max"]
+    for c, (dre, dim) in enumerate(ds["refs"]):
+        # AccProjector::initMdl of every iteration (device-resident model -> projector volume)
+        T.check(lib.t2x_projector_set(dev, c, 3, pad, rmax, 2.0, C.cast(dre, C.POINTER(C.c_float)),
+                                      C.cast(dim, C.POINTER(C.c_float))), "projector")
+        T.check(lib.t2x_backprojector_clear(dev, c), "bp clear")
+    model = T.Model(T.dptr(ds["sigma2"]), T.dptr(ds["pdf_class"]), T.dptr(ds["pdf_dir"]), None)
+    T.check(lib.t2x_estep(dev, C.byref(cfg), C.byref(model), n, ds["dimg"], 1, ds["pin"], outs,
+                          C.byref(ws), None), "estep")
+
+
+def cpu_baseline(w, cfg, sz, seconds_target=20.0):
+    """FP64 oracle on a bounded sample (same workload shape), one particle per thread."""
+    sys.path.insert(0, os.path.join(REPO, "oracle"))
+    sys.path.insert(0, os.path.join(REPO, "tests"))
+    try:
+        import estep_harness as H
+    except Exception as e:  # oracle not built
+        return {"value": None, "unit": "particles/s", "cores": 0, "kind": "port",
+                "sample": f"unavailable: {e}"}
+    ncpu = len(os.sched_getaffinity(0))
+    threads = max(1, min(16, ncpu))
+    # one particle per thread, scaled down when a single particle would exceed the target
+    wl = H.Workload(w_name(w), n=threads, seed_offset=99, current_size=cfg.current_size,
+                    coarse_size=cfg.coarse_size)
+    t0 = time.time()
+    wl.run_oracle(nthreads=threads)
+    dt = time.time() - t0
+    return {"value": threads / dt, "unit": "particles/s", "cores": threads, "kind": "port",
+            "sample": f"{threads} particles of the same workload (FP64 oracle, {threads} OpenMP "
+                      f"threads, one particle each), {dt:.1f} s"}
+
+
+def w_name(w):
+    for k, v in W.WORKLOADS.items():
+        if v is w or v == w:
+            return k
+    return "class256"
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="class256")
+    ap.add_argument("--particles", type=int, default=0, help="default: the workload's count")
+    ap.add_argument("--warmup-particles", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the E-step "
+              "shards over particles with one process per GPU, which is not part of this build.",
+              file=sys.stderr)
+        sys.exit(2)
+    w = W.WORKLOADS[args.workload]
+    n = args.particles or w["n_particles"]
+    cfg, sz = W.make_config(w)
+    lib = T.load()
+    dev = C.c_void_p()
+    T.check(lib.t2x_device_open(0, C.byref(dev)), "open")
+    log(f"workload {args.workload}: n={n} N={cfg.ori_size} K={cfg.nr_classes} current={cfg.current_size} "
+        f"coarse={cfg.coarse_size} pad={sz['pad_size']}")
+    t0 = time.time()
+    ds = build_dataset(lib, dev, w, n, cfg, sz, w["seed"])
+    log(f"dataset synthesised on the GPU in {time.time() - t0:.1f} s")
+    for c in range(cfg.nr_classes):
+        T.check(lib.t2x_backprojector_init(dev, c, 3, sz["pad_size"], sz["r_max"], 2.0), "bp init")
+    outs = (T.ParticleOut * n)()
+    ws, wa = T.make_wsum(cfg.nr_classes, ds["info"].nr_dir, cfg.ori_size)
+    nw = min(n, args.warmup_particles)
+    for i in range(args.warmup):
+        t = time.time()
+        run_step(lib, dev, cfg, sz, ds, nw, outs, ws)
+        T.check(lib.t2x_device_synchronize(dev))
+        log(f"warmup {i}: {nw} particles in {time.time() - t:.2f} s")
+    T.check(lib.t2x_stats_reset(dev))
+    T.check(lib.t2x_device_synchronize(dev))
+    t0 = time.time()
+    for i in range(args.steps):
+        run_step(lib, dev, cfg, sz, ds, n, outs, ws)
+        log(f"step {i} done at {time.time() - t0:.2f} s")
+    T.check(lib.t2x_device_synchronize(dev))
+    dt = time.time() - t0
+    st = T.Stats()
+    T.check(lib.t2x_stats_get(dev, C.byref(st)))
+    value = n * args.steps / dt
+    # dominant kernel: coarse diff2 GEMM; algorithmic 4*T*O*P flops per particle
+    gemm_ms = st.ms_coarse_gemm if hasattr(st, "ms_coarse_gemm") else st.ms_coarse_diff2
+    launches = max(1, st.n_coarse_diff2_launches)
+    flop_per_launch = st.coarse_diff2_flop / launches
+    avg_ms = gemm_ms / launches
+    achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    traffic = os.environ.get("T2X_GEMM_TRAFFIC_BYTES")
+    roof = {"bound": "mfma", "achieved": achieved, "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+            "frac": achieved / MFMA_FP32_PEAK_TFLOPS, "traffic": float(traffic) if traffic else None,
+            "kernel": "k_gemm_tn<EPI_DIFF2> (coarse diff2 cross term, FP32 MFMA 16x16x4)",
+            "avg_launch_ms": avg_ms, "launches": launches, "flop_per_launch": flop_per_launch}
+    stages = {k: getattr(st, k) for k in ("ms_preprocess", "ms_coarse_project", "ms_coarse_diff2",
+                                           "ms_coarse_weights", "ms_fine_diff2", "ms_fine_weights",
+                                           "ms_store")}
+    nsig = st.sig_coarse / max(1, st.n_particles)
+    cpu = None if args.no_cpu else cpu_baseline(w, cfg, sz)
+    line = {
+        "metric": METRIC, "value": value, "unit": "particles/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "data": "synthetic (GPU-generated blob volumes and CTF-modulated noisy projections)",
+        "config": {"workload": f"{args.workload}: {n} particles {cfg.ori_size}^2 @ {cfg.pixel_size} A/px, "
+                               f"K={cfg.nr_classes}, HEALPix order {cfg.healpix_order}, offsets "
+                               f"{w['offset_range_px']}/{w['offset_step_px']} px, current_size "
+                               f"{cfg.current_size}, coarse_size {cfg.coarse_size}, plain FP32 path",
+                   "global_batch": n, "parallelism": "single GPU",
+                   "P_coarse": st.P_coarse, "P_fine": st.P_fine,
+                   "mean_significant_coarse": nsig,
+                   "fine_samples_per_particle": st.fine_samples / max(1, st.n_particles),
+                   "stage_ms": stages},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line), flush=True)
+    lib.t2x_device_close(dev)
+
+
+if __name__ == "__main__":
+    main()

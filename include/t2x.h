@@ -169,6 +169,7 @@ typedef struct t2x_stats {
     long   n_particles;
     double ms_preprocess, ms_coarse_project, ms_coarse_diff2, ms_coarse_weights;
     double ms_fine_diff2, ms_fine_weights, ms_store;     /* GPU event time per stage          */
+    double ms_coarse_gemm;       /* the coarse cross-term GEMM alone (HIP events on its stream) */
     long   n_coarse_diff2_launches;
     double coarse_diff2_flop;    /* algorithmic 4*T*O*P per particle, summed                  */
     double coarse_diff2_bytes;   /* algorithmic HBM bytes of the coarse GEMM                  */
@@ -215,6 +216,15 @@ int t2x_backprojector_clear(t2x_device* dev, int iclass);
  * Sampling helpers (HealpixSampling, src/healpix_sampling.cpp)
  * ------------------------------------------------------------------------------------- */
 int t2x_sampling_info_get(const t2x_estep_config* cfg, t2x_sampling_info* info);
+/* Coarse grid (HealpixSampling rot_angles/tilt_angles/psi_angles/translations_x/y,
+ * healpix_sampling.cpp:290-608): rot/tilt [nr_dir], psi [nr_psi], tx/ty in A [nr_trans]. */
+int t2x_sampling_coarse_get(const t2x_estep_config* cfg, double* rot, double* tilt, double* psi,
+                            double* tx, double* ty);
+/* Fine grid (getOrientations / getTranslationsInPixel with oversampling 1,
+ * healpix_sampling.cpp:1800-2050): sub_rot/sub_tilt [nr_dir*4] (NEST children, y outer),
+ * sub_psi [nr_psi*2], ftx/fty in px [nr_trans*4]. */
+int t2x_sampling_fine_get(const t2x_estep_config* cfg, double* sub_rot, double* sub_tilt,
+                          double* sub_psi, double* ftx_px, double* fty_px);
 
 /* ---------------------------------------------------------------------------------------
  * E-step (MlOptimiserCuda::doThreadExpectationSomeParticles, cuda_ml_optimiser.cu:1345-1408,
@@ -243,6 +253,10 @@ int t2x_device_free(t2x_device* dev, void* dptr);
 int t2x_memcpy_h2d(t2x_device* dev, void* dst, const void* src, size_t bytes);
 int t2x_memcpy_d2h(t2x_device* dev, void* dst, const void* src, size_t bytes);
 int t2x_device_synchronize(t2x_device* dev);
+
+/* Diagnostic copy-out of internal E-step buffers for parity tests (no reference counterpart):
+ * which = 0 coarse slices (re/im), 1 coarse |F|^2, 2 N1, 3 image-side operand, 4 A1, 5 coarse diff2. */
+int t2x_debug_copy(t2x_device* dev, int which, size_t offset, size_t count, float* dst);
 
 /* ---------------------------------------------------------------------------------------
  * Synthetic workloads (Gaussian-blob volumes, CTF-modulated noisy projections) synthesised on

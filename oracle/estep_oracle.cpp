@@ -301,7 +301,8 @@ extern "C" int orc_sampling_oversampled_translations(const t2x_estep_config* cfg
  * ===================================================================================== */
 extern "C" void orc_euler_angles2matrix(double alpha, double beta, double gamma, double* A)
 {
-    alpha *= PI / 180.; beta *= PI / 180.; gamma *= PI / 180.;
+    /* DEG2RAD(d) = ((d) * PI / 180) (macros.h:275) */
+    alpha = alpha * PI / 180; beta = beta * PI / 180; gamma = gamma * PI / 180;
     double ca = std::cos(alpha), cb = std::cos(beta), cg = std::cos(gamma);
     double sa = std::sin(alpha), sb = std::sin(beta), sg = std::sin(gamma);
     double cc = cb * ca, cs = cb * sa, sc = sb * ca, ss = sb * sa;
@@ -447,9 +448,27 @@ extern "C" void orc_fft2_c2r(int n, const double* in, double* out)
     for (size_t i = 0; i < a.size(); i++) out[i] = a[i].real();
 }
 
+/* Matrix2D::inv for 3x3 matrices (matrix2d.h:1108-1123): adjugate divided by the determinant.
+ * For a rotation this equals the transpose up to rounding; the reference uses this form. */
+static void inv3(const double* A, double* R)
+{
+    auto M = [&](int r, int c) { return A[r * 3 + c]; };
+    R[0] = M(2, 2) * M(1, 1) - M(2, 1) * M(1, 2);
+    R[1] = -(M(2, 2) * M(0, 1) - M(2, 1) * M(0, 2));
+    R[2] = M(1, 2) * M(0, 1) - M(1, 1) * M(0, 2);
+    R[3] = -(M(2, 2) * M(1, 0) - M(2, 0) * M(1, 2));
+    R[4] = M(2, 2) * M(0, 0) - M(2, 0) * M(0, 2);
+    R[5] = -(M(1, 2) * M(0, 0) - M(1, 0) * M(0, 2));
+    R[6] = M(2, 1) * M(1, 0) - M(2, 0) * M(1, 1);
+    R[7] = -(M(2, 1) * M(0, 0) - M(2, 0) * M(0, 1));
+    R[8] = M(1, 1) * M(0, 0) - M(1, 0) * M(0, 1);
+    double tmp = M(0, 0) * R[0] + M(1, 0) * R[1] + M(2, 0) * R[2];
+    for (int i = 0; i < 9; i++) R[i] /= tmp;
+}
+
 /* =====================================================================================
  * Projector::project (projector.cpp:524-691), trilinear branch.
- * Ainv = A^-1 * padding_factor (A is a rotation, so A^-1 = A^T).
+ * Ainv = A.inv() * padding_factor.
  * ===================================================================================== */
 static inline bool project_pixel(const orc_volume* v, const double* Ai, int x, int y, bool clean,
                                  cplx* res)
@@ -500,8 +519,8 @@ static inline bool project_pixel(const orc_volume* v, const double* Ai, int x, i
 static void project_into(const orc_volume* v, const double* A, int size, cplx* f2d, bool clean)
 {
     double Ai[9];
-    for (int r = 0; r < 3; r++)
-        for (int c = 0; c < 3; c++) Ai[r * 3 + c] = A[c * 3 + r] * v->padding_factor;
+    inv3(A, Ai);
+    for (int i = 0; i < 9; i++) Ai[i] *= v->padding_factor;
     const int xs = size / 2 + 1;
     const int r_max_out = xs - 1;
     const int r_max_out_2 = r_max_out * r_max_out;
@@ -528,8 +547,8 @@ static void backproject_from(orc_volume* v, const cplx* f2d, const double* fw, i
 {
     const double pf = v->padding_factor;
     double Ai[9];
-    for (int r = 0; r < 3; r++)
-        for (int c = 0; c < 3; c++) Ai[r * 3 + c] = A[c * 3 + r] * pf;
+    inv3(A, Ai);
+    for (int i = 0; i < 9; i++) Ai[i] *= pf;
     const int max_r2 = R_ROUND(v->r_max * pf) * R_ROUND(v->r_max * pf);
     const double Am_Xx = Ai[0], Am_Xy = Ai[1], Am_Yx = Ai[3], Am_Yy = Ai[4], Am_Zx = Ai[6], Am_Zy = Ai[7];
     const double AtA_xx = Am_Xx * Am_Xx + Am_Yx * Am_Yx + Am_Zx * Am_Zx;

@@ -267,3 +267,18 @@ extern "C" int t2x_device_synchronize(t2x_device* h)
     T2X_HIP(hipStreamSynchronize(d->stream));
     return T2X_OK;
 }
+
+// Diagnostic copy-out of internal buffers (tests only): 0 coarse slices (re/im), 1 coarse |F|^2,
+// 2 N1, 3 image-side A, 4 A1, 5 coarse diff2. Offsets and counts in floats.
+extern "C" int t2x_debug_copy(t2x_device* h, int which, size_t offset, size_t count, float* dst)
+{
+    if (!h || !dst) { set_error("null argument"); return T2X_EINVAL; }
+    Device* d = reinterpret_cast<Device*>(h);
+    DevBuf* bufs[] = {&d->coarse_B, &d->coarse_B2, &d->N1, &d->A, &d->A1, &d->diff2};
+    if (which < 0 || which > 5) { set_error("bad buffer id"); return T2X_EINVAL; }
+    DevBuf* b = bufs[which];
+    if ((offset + count) * sizeof(float) > b->bytes) { set_error("range outside buffer"); return T2X_EINVAL; }
+    T2X_HIP(hipStreamSynchronize(d->stream));
+    T2X_HIP(hipMemcpy(dst, b->as<float>() + offset, count * sizeof(float), hipMemcpyDeviceToHost));
+    return T2X_OK;
+}

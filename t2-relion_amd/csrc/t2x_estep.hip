@@ -249,9 +249,8 @@ __global__ __launch_bounds__(256) void k_coarse_imgside(const float2* __restrict
 
 struct FineGeom {
     int Norient, Npsi, n_subdir, n_subpsi, over_rot, over_trans, T;
-    const double* sub_rot;
-    const double* sub_tilt;
-    const double* sub_psi;
+    const double* dir_sc;   // [Nd*n_subdir][4] sin/cos of rot and tilt
+    const double* psi_sc;   // [Npsi*n_subpsi][2] sin/cos of psi
     double pf;
 };
 
@@ -271,8 +270,9 @@ __device__ __forceinline__ void fine_matrix(const FineGeom& g, int n, int ir, do
     const int c = n / g.Norient, o = n - c * g.Norient;
     const int d = o / g.Npsi, ps = o - d * g.Npsi;
     const int sd = ir / g.n_subpsi, sp = ir - sd * g.n_subpsi;
-    euler_matrix(g.sub_rot[(size_t)d * g.n_subdir + sd], g.sub_tilt[(size_t)d * g.n_subdir + sd],
-                 g.sub_psi[(size_t)ps * g.n_subpsi + sp], A);
+    const double* a = g.dir_sc + ((size_t)d * g.n_subdir + sd) * 4;
+    const double* p = g.psi_sc + ((size_t)ps * g.n_subpsi + sp) * 2;
+    euler_from_sincos(a[0], a[1], a[2], a[3], p[0], p[1], A);
     *cls = c;
     *idir = d;
 }
@@ -305,13 +305,13 @@ __global__ __launch_bounds__(256) void k_fine_diff2(
     const int n = orients[j];
     const long long p0 = orient_pair_start[j];
     const long long p1 = (j + 1 < orient_off[b + 1]) ? orient_pair_start[j + 1] : pair_off[b + 1];
-    double A[9];
+    double A[9], Ai[9];
     int c, idir;
     fine_matrix(g, n, ir, A, &c, &idir);
-    const float pf = (float)g.pf;
-    const float a0 = (float)A[0] * pf, a1 = (float)A[3] * pf;   // Ai = A^T * pf
-    const float a2 = (float)A[1] * pf, a3 = (float)A[4] * pf;
-    const float a4 = (float)A[2] * pf, a5 = (float)A[5] * pf;
+    inv3(A, Ai);   // Ainv = A.inv() * pf (projector.cpp:530-531)
+    const float a0 = (float)(Ai[0] * g.pf), a1 = (float)(Ai[1] * g.pf);
+    const float a2 = (float)(Ai[3] * g.pf), a3 = (float)(Ai[4] * g.pf);
+    const float a4 = (float)(Ai[6] * g.pf), a5 = (float)(Ai[7] * g.pf);
     const float2* vol = vols[c];
     const int pad = pads[c];
     double n1 = 0.;
@@ -360,6 +360,36 @@ __global__ __launch_bounds__(256) void k_fine_diff2(
         float m = fminf(fminf(wmin[0], wmin[1]), fminf(wmin[2], wmin[3]));
         if (m < 3.0e38f) atomicMin(&fine_min[b], float_to_ordered(m));
     }
+}
+
+// Which pixels backproject2Dto3D touches (backprojector.cpp:114-170): the ellipse bounds of
+// the row and the r^2 test, evaluated in double with the reference's operation order and no
+// contraction so that pixels on the r_max sphere are kept or dropped exactly as there.
+__device__ __noinline__ bool bp_pixel_inside(double AtA_xx, double AtA_xy, double AtA_yy,
+                                             double AtA_xy2, int max_r2, double a0, double a1,
+                                             double a2, double a3, double a4, double a5, int x, int y)
+{
+#pragma clang fp contract(off)
+    double discr = AtA_xy2 * y * y - AtA_xx * (AtA_yy * y * y - max_r2);
+    if (discr < 0.0) return false;
+    double dd = sqrt(discr) / AtA_xx;
+    double qq = -AtA_xy * y / AtA_xx;
+    double lo = qq - dd, hi = qq + dd;
+    int first_x = (lo == (int)lo) ? (int)lo : ((lo > 0) ? (int)(lo + 1) : (int)lo);
+    int last_x = (hi == (int)hi) ? (int)hi : ((hi > 0) ? (int)hi : (int)(hi - 1));
+    if (x < first_x || x > last_x) return false;
+    double xp = a0 * x + a1 * y, yp = a2 * x + a3 * y, zp = a4 * x + a5 * y;
+    return !(xp * xp + yp * yp + zp * zp > max_r2);
+}
+
+__device__ __noinline__ void bp_ata(double a0, double a1, double a2, double a3, double a4, double a5,
+                                    double* xx, double* xy, double* yy, double* xy2)
+{
+#pragma clang fp contract(off)
+    *xx = a0 * a0 + a2 * a2 + a4 * a4;
+    *xy = a0 * a1 + a2 * a3 + a4 * a5;
+    *yy = a1 * a1 + a3 * a3 + a5 * a5;
+    *xy2 = *xy * *xy;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -457,9 +487,10 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
     __syncthreads();
     const int cnt = min(s_cnt, STORE_MAX_T);
     if (cnt == 0) return;
-    double A[9];
+    double A[9], Ai[9];
     int c, idir;
     fine_matrix(g, n, ir, A, &c, &idir);
+    inv3(A, Ai);
     // scalar sums (ml_optimiser.cpp:8505-8552), exact weights recomputed in double
     if (threadIdx.x == 0) {
         double sw = 0., soff = 0.;
@@ -486,11 +517,11 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
         atomicAdd(&P.w_pdf_dir[(size_t)c * P.Nd + idir], sw);
         atomicAdd(P.w_sigma2_offset, soff);
     }
-    // projection frame: Ai = A^T * pf (float for sampling, double for the scatter geometry)
+    // projection frame: Ainv = A.inv() * pf (float for sampling, double for the scatter geometry)
     const double pf = g.pf;
-    const float fa0 = (float)(A[0] * pf), fa1 = (float)(A[3] * pf), fa2 = (float)(A[1] * pf);
-    const float fa3 = (float)(A[4] * pf), fa4 = (float)(A[2] * pf), fa5 = (float)(A[5] * pf);
-    const double da0 = A[0] * pf, da1 = A[3] * pf, da2 = A[1] * pf, da3 = A[4] * pf, da4 = A[2] * pf, da5 = A[5] * pf;
+    const double da0 = Ai[0] * pf, da1 = Ai[1] * pf, da2 = Ai[3] * pf, da3 = Ai[4] * pf, da4 = Ai[6] * pf, da5 = Ai[7] * pf;
+    const float fa0 = (float)da0, fa1 = (float)da1, fa2 = (float)da2;
+    const float fa3 = (float)da3, fa4 = (float)da4, fa5 = (float)da5;
     const float2* vol = P.vols[c];
     const int pad = P.pads[c];
     float2* bpd = P.bp_data[c];
@@ -501,10 +532,8 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
     const int max_r2 = rr * rr;
     const int bxdim = bpad / 2 + 1, bh = bpad / 2;
     // ellipse row bounds of backproject2Dto3D (backprojector.cpp:80-135) in double
-    const double AtA_xx = da0 * da0 + da2 * da2 + da4 * da4;
-    const double AtA_xy = da0 * da1 + da2 * da3 + da4 * da5;
-    const double AtA_yy = da1 * da1 + da3 * da3 + da5 * da5;
-    const double AtA_xy2 = AtA_xy * AtA_xy;
+    double AtA_xx, AtA_xy, AtA_yy, AtA_xy2;
+    bp_ata(da0, da1, da2, da3, da4, da5, &AtA_xx, &AtA_xy, &AtA_yy, &AtA_xy2);
     double norm_acc = 0.;
     for (int p = threadIdx.x; p < P.npix; p += blockDim.x) {
         const int x = P.pix[p].x, y = P.pix[p].y;
@@ -537,19 +566,14 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
             norm_acc += d2acc;
         }
         if (!(wacc > 0.f)) continue;
-        // backprojection of this pixel (first_x/last_x ellipse bounds and r2 test in double)
+        if (!bp_pixel_inside(AtA_xx, AtA_xy, AtA_yy, AtA_xy2, max_r2, da0, da1, da2, da3, da4, da5, x, y)) continue;
+        double xp, yp, zp;
         {
-            double discr = AtA_xy2 * y * y - AtA_xx * (AtA_yy * y * y - max_r2);
-            if (discr < 0.0) continue;
-            double dd = sqrt(discr) / AtA_xx;
-            double qq = -AtA_xy * y / AtA_xx;
-            double lo = qq - dd, hi = qq + dd;
-            int first_x = (lo == (int)lo) ? (int)lo : ((lo > 0) ? (int)(lo + 1) : (int)lo);
-            int last_x = (hi == (int)hi) ? (int)hi : ((hi > 0) ? (int)hi : (int)(hi - 1));
-            if (x < first_x || x > last_x) continue;
+#pragma clang fp contract(off)
+            xp = da0 * x + da1 * y;
+            yp = da2 * x + da3 * y;
+            zp = da4 * x + da5 * y;
         }
-        double xp = da0 * x + da1 * y, yp = da2 * x + da3 * y, zp = da4 * x + da5 * y;
-        if (xp * xp + yp * yp + zp * zp > max_r2) continue;
         bool neg = xp < 0;
         if (neg) { xp = -xp; yp = -yp; zp = -zp; }
         double fx0 = floor(xp), fy0 = floor(yp), fz0 = floor(zp);
@@ -803,10 +827,9 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         }
     T2X_TRY(upload(d->prior_o, prior_o, st));
     // device copies of angle tables and class pointer arrays
-    DevBuf d_subrot, d_subtilt, d_subpsi, d_ftx, d_fty, d_vols, d_pads, d_bpd, d_bpw, d_bpads, d_bprmax;
-    T2X_TRY(upload(d_subrot, S.sub_rot, st));
-    T2X_TRY(upload(d_subtilt, S.sub_tilt, st));
-    T2X_TRY(upload(d_subpsi, S.sub_psi, st));
+    DevBuf d_dirsc, d_psisc, d_ftx, d_fty, d_vols, d_pads, d_bpd, d_bpw, d_bpads, d_bprmax;
+    T2X_TRY(upload(d_dirsc, S.sub_dir_sc, st));
+    T2X_TRY(upload(d_psisc, S.sub_psi_sc, st));
     T2X_TRY(upload(d_ftx, S.fine_tx_px, st));
     T2X_TRY(upload(d_fty, S.fine_ty_px, st));
     T2X_TRY(upload(d_vols, h_vols, st));
@@ -834,11 +857,11 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         std::vector<float> Ai(Norient * 9);
         for (long dd = 0; dd < Nd; dd++)
             for (long p = 0; p < Npsi; p++) {
-                double A[9];
-                euler_matrix(S.rot[dd], S.tilt[dd], S.psi[p], A);
+                double A[9], R[9];
+                euler_matrix_host(S.rot[dd], S.tilt[dd], S.psi[p], A);
+                inv3(A, R);
                 float* a = &Ai[(dd * Npsi + p) * 9];
-                for (int r = 0; r < 3; r++)
-                    for (int c2 = 0; c2 < 3; c2++) a[r * 3 + c2] = (float)(A[c2 * 3 + r] * pf);
+                for (int k = 0; k < 9; k++) a[k] = (float)(R[k] * pf);
             }
         DevBuf d_Ai;
         T2X_TRY(upload(d_Ai, Ai, st));
@@ -1131,9 +1154,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         g.over_rot = over_rot;
         g.over_trans = over_trans;
         g.T = (int)T;
-        g.sub_rot = d_subrot.as<double>();
-        g.sub_tilt = d_subtilt.as<double>();
-        g.sub_psi = d_subpsi.as<double>();
+        g.dir_sc = d_dirsc.as<double>();
+        g.psi_sc = d_psisc.as<double>();
         g.pf = pf;
         (void)hipEventRecord(d->ev[ev++], st);   // 5: fine start
         if (nitems > 0) {
@@ -1255,6 +1277,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             for (int e = 1; e < ev; e++) (void)hipEventElapsedTime(&ms[e - 1], d->ev[e - 1], d->ev[e]);
             stt.ms_preprocess += ms[0];
             stt.ms_coarse_diff2 += ms[1] + ms[2];
+            stt.ms_coarse_gemm += ms[2];
             stt.ms_coarse_weights += ms[3];
             stt.ms_fine_diff2 += ms[4] + ms[5];
             stt.ms_fine_weights += ms[6];

@@ -91,6 +91,8 @@ struct HostSampling {
     int n_subdir = 1, n_subpsi = 1;
     std::vector<double> sub_rot, sub_tilt;  // [nr_dir * n_subdir]
     std::vector<double> sub_psi;            // [nr_psi * n_subpsi]
+    std::vector<double> sub_dir_sc;         // [nr_dir * n_subdir][4]: sin/cos of rot, tilt
+    std::vector<double> sub_psi_sc;         // [nr_psi * n_subpsi][2]: sin/cos of psi
     std::vector<double> fine_tx_px, fine_ty_px;  // [nr_trans * over_trans]
     std::vector<double> coarse_tx_px, coarse_ty_px;
 };
@@ -162,18 +164,43 @@ __host__ __device__ __forceinline__ float ordered_to_float(unsigned int u)
     return f;
 }
 
-// Euler_angles2matrix (euler.cpp:52-91) in double precision.
-__host__ __device__ inline void euler_matrix(double rot, double tilt, double psi, double* A)
+// Euler_angles2matrix (euler.cpp:52-91) from precomputed sines and cosines, with the
+// reference's operation order and no fused multiply-add, so that host and device produce the
+// same bits as the CPU path (whose x86-64 build does not contract a*b+c).
+__host__ __device__ inline void euler_from_sincos(double sa, double ca, double sb, double cb,
+                                                  double sg, double cg, double* A)
 {
-    const double d2r = 3.14159265358979323846 / 180.0;
-    double ca, sa, cb, sb, cg, sg;
-    sincos(rot * d2r, &sa, &ca);
-    sincos(tilt * d2r, &sb, &cb);
-    sincos(psi * d2r, &sg, &cg);
+#pragma clang fp contract(off)
     double cc = cb * ca, cs = cb * sa, sc = sb * ca, ss = sb * sa;
     A[0] = cg * cc - sg * sa;  A[1] = cg * cs + sg * ca;  A[2] = -cg * sb;
     A[3] = -sg * cc - cg * sa; A[4] = -sg * cs + cg * ca; A[5] = sg * sb;
     A[6] = sc;                 A[7] = ss;                 A[8] = cb;
+}
+
+// Host only: angles in degrees, DEG2RAD(d) = ((d) * PI / 180) (macros.h:275).
+inline void euler_matrix_host(double rot, double tilt, double psi, double* A)
+{
+    const double kP = 3.14159265358979323846;
+    double a = rot * kP / 180, b = tilt * kP / 180, g = psi * kP / 180;
+    euler_from_sincos(std::sin(a), std::cos(a), std::sin(b), std::cos(b), std::sin(g), std::cos(g), A);
+}
+
+// Matrix2D::inv for 3x3 matrices (matrix2d.h:1108-1123): adjugate over the determinant,
+// reference operation order, no contraction.
+__host__ __device__ inline void inv3(const double* A, double* R)
+{
+#pragma clang fp contract(off)
+    R[0] = A[8] * A[4] - A[7] * A[5];
+    R[1] = -(A[8] * A[1] - A[7] * A[2]);
+    R[2] = A[5] * A[1] - A[4] * A[2];
+    R[3] = -(A[8] * A[3] - A[6] * A[5]);
+    R[4] = A[8] * A[0] - A[6] * A[2];
+    R[5] = -(A[5] * A[0] - A[3] * A[2]);
+    R[6] = A[7] * A[3] - A[6] * A[4];
+    R[7] = -(A[7] * A[0] - A[6] * A[1]);
+    R[8] = A[4] * A[0] - A[3] * A[1];
+    double tmp = A[0] * R[0] + A[3] * R[1] + A[6] * R[2];
+    for (int i = 0; i < 9; i++) R[i] /= tmp;
 }
 
 // Trilinear sample of the half-complex projector volume at logical (xp, yp, zp) (already

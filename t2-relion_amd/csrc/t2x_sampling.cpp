@@ -65,7 +65,8 @@ void pixel_centre(int order, long pix, double* z, double* phi)
         shift = 0;
     } else {                         // equatorial belt
         nr = ns;
-        *z = (double)(2 * ns - ring) * (double)(2 * ns) * f2;
+        const double f1 = (double)(2 * ns) * f2;
+        *z = (double)(2 * ns - ring) * f1;
         shift = (ring - ns) & 1;
     }
     long jp = (phase_of_face[face] * nr + ix - iy + 1 + shift) / 2;
@@ -180,6 +181,21 @@ int build_sampling(const t2x_estep_config* cfg, HostSampling& s)
     for (int i = 0; i < nr_psi; i++)
         for (int k = 0; k < s.n_subpsi; k++)
             s.sub_psi[i * s.n_subpsi + k] = s.psi[i] - 0.5 * psi_step + (0.5 + k) * psi_step / s.n_subpsi;
+    // sines and cosines of the fine angles (host libm, as the CPU path's Euler_angles2matrix)
+    s.sub_dir_sc.resize(s.sub_rot.size() * 4);
+    for (size_t k = 0; k < s.sub_rot.size(); k++) {
+        double a = s.sub_rot[k] * kPi / 180, b = s.sub_tilt[k] * kPi / 180;
+        s.sub_dir_sc[4 * k + 0] = std::sin(a);
+        s.sub_dir_sc[4 * k + 1] = std::cos(a);
+        s.sub_dir_sc[4 * k + 2] = std::sin(b);
+        s.sub_dir_sc[4 * k + 3] = std::cos(b);
+    }
+    s.sub_psi_sc.resize(s.sub_psi.size() * 2);
+    for (size_t k = 0; k < s.sub_psi.size(); k++) {
+        double g = s.sub_psi[k] * kPi / 180;
+        s.sub_psi_sc[2 * k + 0] = std::sin(g);
+        s.sub_psi_sc[2 * k + 1] = std::cos(g);
+    }
     const double pix = cfg->pixel_size;
     for (long t = 0; t < s.nr_trans; t++) {
         s.coarse_tx_px.push_back(s.tx[t] / pix);
@@ -210,5 +226,27 @@ extern "C" int t2x_sampling_info_get(const t2x_estep_config* cfg, t2x_sampling_i
     info->nr_over_trans = s.over_trans;
     info->nr_coarse_per_class = s.nr_dir * s.nr_psi;
     info->nr_hidden_coarse = (long)cfg->nr_classes * s.nr_dir * s.nr_psi * s.nr_trans;
+    return T2X_OK;
+}
+
+extern "C" int t2x_sampling_coarse_get(const t2x_estep_config* cfg, double* rot, double* tilt,
+                                       double* psi, double* tx, double* ty)
+{
+    t2x::HostSampling s;
+    T2X_TRY(t2x::build_sampling(cfg, s));
+    for (long i = 0; i < s.nr_dir; i++) { if (rot) rot[i] = s.rot[i]; if (tilt) tilt[i] = s.tilt[i]; }
+    for (long i = 0; i < s.nr_psi; i++) if (psi) psi[i] = s.psi[i];
+    for (long i = 0; i < s.nr_trans; i++) { if (tx) tx[i] = s.tx[i]; if (ty) ty[i] = s.ty[i]; }
+    return T2X_OK;
+}
+
+extern "C" int t2x_sampling_fine_get(const t2x_estep_config* cfg, double* sub_rot, double* sub_tilt,
+                                     double* sub_psi, double* ftx_px, double* fty_px)
+{
+    t2x::HostSampling s;
+    T2X_TRY(t2x::build_sampling(cfg, s));
+    for (size_t i = 0; i < s.sub_rot.size(); i++) { if (sub_rot) sub_rot[i] = s.sub_rot[i]; if (sub_tilt) sub_tilt[i] = s.sub_tilt[i]; }
+    for (size_t i = 0; i < s.sub_psi.size(); i++) if (sub_psi) sub_psi[i] = s.sub_psi[i];
+    for (size_t i = 0; i < s.fine_tx_px.size(); i++) { if (ftx_px) ftx_px[i] = s.fine_tx_px[i]; if (fty_px) fty_px[i] = s.fine_ty_px[i]; }
     return T2X_OK;
 }

@@ -197,7 +197,7 @@ extern "C" int t2x_synth_particles_device(t2x_device* h, int N, double angpix, i
         int m = std::min(chunk, n - p0);
         std::vector<SynthPart> hp(m);
         for (int i = 0; i < m; i++) {
-            euler_matrix(trot[p0 + i], ttilt[p0 + i], tpsi[p0 + i], hp[i].A);
+            euler_matrix_host(trot[p0 + i], ttilt[p0 + i], tpsi[p0 + i], hp[i].A);
             hp[i].tx = tx[p0 + i];
             hp[i].ty = ty[p0 + i];
             hp[i].ctf = ctf_const(pin[p0 + i].ctf);

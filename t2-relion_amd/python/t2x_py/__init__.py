@@ -104,7 +104,7 @@ class Stats(C.Structure):
     _fields_ = [("n_calls", C.c_int), ("n_particles", C.c_long)] + \
         [(n, C.c_double) for n in ("ms_preprocess", "ms_coarse_project", "ms_coarse_diff2",
                                    "ms_coarse_weights", "ms_fine_diff2", "ms_fine_weights",
-                                   "ms_store")] + \
+                                   "ms_store", "ms_coarse_gemm")] + \
         [("n_coarse_diff2_launches", C.c_long)] + \
         [(n, C.c_double) for n in ("coarse_diff2_flop", "coarse_diff2_bytes", "fine_samples",
                                    "fine_flop", "bp_orientations", "sig_coarse")] + \
@@ -170,6 +170,8 @@ def load(path=None):
         "t2x_backprojector_get": (C.c_int, [vp, C.c_int, PD, PD, PD]),
         "t2x_backprojector_clear": (C.c_int, [vp, C.c_int]),
         "t2x_sampling_info_get": (C.c_int, [C.POINTER(EstepConfig), C.POINTER(SamplingInfo)]),
+        "t2x_sampling_coarse_get": (C.c_int, [C.POINTER(EstepConfig), PD, PD, PD, PD, PD]),
+        "t2x_sampling_fine_get": (C.c_int, [C.POINTER(EstepConfig), PD, PD, PD, PD, PD]),
         "t2x_estep": (C.c_int, [vp, C.POINTER(EstepConfig), C.POINTER(Model), C.c_int, vp, C.c_int,
                                 C.POINTER(ParticleIn), C.POINTER(ParticleOut), C.POINTER(Wsum),
                                 C.POINTER(EstepDebug)]),
@@ -180,6 +182,7 @@ def load(path=None):
         "t2x_memcpy_h2d": (C.c_int, [vp, vp, vp, C.c_size_t]),
         "t2x_memcpy_d2h": (C.c_int, [vp, vp, vp, C.c_size_t]),
         "t2x_device_synchronize": (C.c_int, [vp]),
+        "t2x_debug_copy": (C.c_int, [vp, C.c_int, C.c_size_t, C.c_size_t, C.POINTER(C.c_float)]),
         "t2x_synth_reference_device": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int,
                                                  C.c_double, C.POINTER(Blob), C.c_int, C.c_double,
                                                  vp, vp]),
