@@ -289,3 +289,18 @@ def test_weights_and_threshold(orc, seed, maxsig):
     rs, rsig, rn = _threshold_numpy(w, 0.999, maxsig, True)
     assert s[0] == pytest.approx(rs, rel=1e-12)
     assert sig[0] == rsig and cnt.value == rn
+
+
+def test_stale_projection_buffer_is_immaterial():
+    """The CPU diff2 loop reuses its projection buffer (ml_optimiser.cpp:7017-7065): pixels that
+    the floating-point r_max test skips keep the previous orientation's values. The oracle's
+    ORC_PROJ_STALE mode reproduces that; on the refine64 workload it changes nothing beyond
+    rounding, so the intended semantics (zero outside r_max) used by libt2x match the reference."""
+    import estep_harness as H
+    wl = H.Workload("refine64", n=3)
+    a = wl.run_oracle(n_debug=3, fine_cap=8192, proj_mode=O.ORC_PROJ_CLEAN)
+    b = wl.run_oracle(n_debug=3, fine_cap=8192, proj_mode=O.ORC_PROJ_STALE)
+    v = a["dbg"]["coarse_diff2"] > -998
+    assert H.rel(b["dbg"]["coarse_diff2"][v], a["dbg"]["coarse_diff2"][v]) < 1e-12
+    assert np.array_equal(a["out"]["best_hidden"], b["out"]["best_hidden"])
+    assert H.rel_norm(b["bp"][0][1], a["bp"][0][1]) < 1e-12
