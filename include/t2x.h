@@ -170,6 +170,7 @@ typedef struct t2x_stats {
     double ms_preprocess, ms_coarse_project, ms_coarse_diff2, ms_coarse_weights;
     double ms_fine_diff2, ms_fine_weights, ms_store;     /* GPU event time per stage          */
     double ms_coarse_gemm;       /* the coarse cross-term GEMM alone (HIP events on its stream) */
+    long   n_select_fallbacks;   /* batches whose coarse selection used the radix-select path   */
     long   n_coarse_diff2_launches;
     double coarse_diff2_flop;    /* algorithmic 4*T*O*P per particle, summed                  */
     double coarse_diff2_bytes;   /* algorithmic HBM bytes of the coarse GEMM                  */

@@ -42,6 +42,16 @@ __global__ void k_sig_write(WeightSrc s, int B, const SelResult* res, const long
                             const long long* orient_off, int2* pairs, int* orients,
                             long long* orient_pair_start);
 __global__ void k_fine_argmax(WeightSrc s, int B, SelResult* res, long long* nfine_sig);
+__global__ void k_coarse_sum(WeightSrc s, int B, SelResult* res);
+__global__ void k_coarse_candidates(WeightSrc s, int B, SelParams prm, const SelResult* res,
+                                    double* cand_w, int* cand_e, int* cand_count);
+__global__ void k_cand_select(WeightSrc s, int B, SelParams prm, SelResult* res, const double* cand_w,
+                              const int* cand_e, const int* cand_count, int2* pairs_strided,
+                              long long* counts, int* overflow);
+__global__ void k_cand_scatter(int B, const int2* pairs_strided, const long long* pair_off,
+                               const long long* orient_off, int2* pairs, int* orients,
+                               long long* orient_pair_start);
+static const int kCandCap = 4096;   // CAND_CAP of t2x_select.hip
 
 static const double kPi = 3.14159265358979323846;
 
@@ -1077,13 +1087,37 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         wc.over_trans = over_trans;
         k_pass_min<<<B, 1024, 0, st>>>(wc, B, d_selc);
         SelParams sp{cfg->adaptive_fraction, (long long)cfg->maximum_significants, 1};
-        k_select<<<B, 1024, 0, st>>>(wc, sp, B, d_selc);
-        T2X_HIP(hipGetLastError());
+        // candidate path: sum pass, compaction of the weights that can be significant, exact
+        // sorted walk per particle; the radix-select path handles batches that overflow
         T2X_TRY(d->sig_count.ensure(sizeof(long long) * 2 * B));
-        k_sig_count<<<B, 1024, 0, st>>>(wc, B, d_selc, d->sig_count.as<long long>());
+        T2X_TRY(d->cand_w.ensure(sizeof(double) * B * kCandCap));
+        T2X_TRY(d->cand_e.ensure(sizeof(int) * B * kCandCap));
+        T2X_TRY(d->cand_n.ensure(sizeof(int) * 2 * B));
+        T2X_TRY(d->pairs_strided.ensure(sizeof(int2) * B * kCandCap));
+        int* d_cand_count = d->cand_n.as<int>();
+        int* d_overflow = d_cand_count + B;
+        T2X_HIP(hipMemsetAsync(d_overflow, 0, sizeof(int) * B, st));
+        k_coarse_sum<<<B, 1024, 0, st>>>(wc, B, d_selc);
+        k_coarse_candidates<<<B, 1024, 0, st>>>(wc, B, sp, d_selc, d->cand_w.as<double>(), d->cand_e.as<int>(),
+                                                d_cand_count);
+        k_cand_select<<<B, 1024, 0, st>>>(wc, B, sp, d_selc, d->cand_w.as<double>(), d->cand_e.as<int>(),
+                                          d_cand_count, d->pairs_strided.as<int2>(),
+                                          d->sig_count.as<long long>(), d_overflow);
         T2X_HIP(hipGetLastError());
         std::vector<long long> counts(2 * B);
         std::vector<SelResult> hselc(B);
+        std::vector<int> hover(B);
+        T2X_HIP(hipMemcpyAsync(hover.data(), d_overflow, sizeof(int) * B, hipMemcpyDeviceToHost, st));
+        T2X_HIP(hipStreamSynchronize(st));
+        bool fallback = false;
+        for (int i = 0; i < B; i++) fallback |= hover[i] != 0;
+        if (getenv("T2X_FORCE_RADIX_SELECT")) fallback = true;   // exercises the fallback in tests
+        if (fallback) {
+            k_select<<<B, 1024, 0, st>>>(wc, sp, B, d_selc);
+            k_sig_count<<<B, 1024, 0, st>>>(wc, B, d_selc, d->sig_count.as<long long>());
+            T2X_HIP(hipGetLastError());
+            stt.n_select_fallbacks += 1;
+        }
         T2X_HIP(hipMemcpyAsync(counts.data(), d->sig_count.ptr, sizeof(long long) * 2 * B, hipMemcpyDeviceToHost, st));
         T2X_HIP(hipMemcpyAsync(hselc.data(), d_selc, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
         T2X_HIP(hipStreamSynchronize(st));
@@ -1105,9 +1139,14 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         T2X_TRY(d->sig_pairs.ensure(sizeof(int2) * std::max<long long>(npairs, 1)));
         T2X_TRY(d->sig_orient.ensure(sizeof(int) * std::max<long long>(norients, 1)));
         T2X_TRY(d->orient_pairs_off.ensure(sizeof(long long) * std::max<long long>(norients, 1)));
-        k_sig_write<<<B, 1024, 0, st>>>(wc, B, d_selc, d_pair_off.as<long long>(), d_orient_off.as<long long>(),
-                                        d->sig_pairs.as<int2>(), d->sig_orient.as<int>(),
-                                        d->orient_pairs_off.as<long long>());
+        if (fallback)
+            k_sig_write<<<B, 1024, 0, st>>>(wc, B, d_selc, d_pair_off.as<long long>(), d_orient_off.as<long long>(),
+                                            d->sig_pairs.as<int2>(), d->sig_orient.as<int>(),
+                                            d->orient_pairs_off.as<long long>());
+        else
+            k_cand_scatter<<<B, 256, 0, st>>>(B, d->pairs_strided.as<int2>(), d_pair_off.as<long long>(),
+                                              d_orient_off.as<long long>(), d->sig_pairs.as<int2>(),
+                                              d->sig_orient.as<int>(), d->orient_pairs_off.as<long long>());
         T2X_HIP(hipGetLastError());
         // debug: coarse diff2 / significance in ihidden order (class, dir, psi, trans)
         if (dbg && dbg->n_debug > 0 && b0 < dbg->n_debug) {

@@ -520,3 +520,245 @@ __global__ __launch_bounds__(1024) void k_fine_argmax(WeightSrc s, int B, SelRes
 }
 
 }  // namespace t2x
+
+namespace t2x {
+
+// ------------------------------------------------------------------------------------------
+// Coarse selection by candidate compaction.
+//
+// With the positive weights sorted descending u_1 >= u_2 >= ... >= u_n, the walk of
+// convertAllSquaredDifferencesToWeights stops at the first k with u_1+...+u_k > f*S. Then
+// u_k+...+u_n >= (1-f)*S and so u_k >= (1-f)*S/n: every weight the walk can stop at, and
+// every weight above it, is at least (1-f)*S/n. Pass 1 computes S, pass 2 keeps the weights
+// above half that bound (margin for summation order), and one workgroup per particle sorts
+// them in LDS and walks them exactly. A particle whose candidates overflow the buffer sends
+// the batch to the radix-select path above.
+// ------------------------------------------------------------------------------------------
+
+#define CAND_CAP 4096
+
+__global__ __launch_bounds__(1024) void k_coarse_sum(WeightSrc s, int B, SelResult* res)
+{
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    const double mind2 = res[b].min_d2;
+    double sum = 0., mx = 0.;
+    unsigned long long npos = 0;
+    for (int t = 0; t < s.T; t++) {
+        const double pt = s.prior_t[(size_t)b * s.T + t];
+        const float* row = s.d2 + ((size_t)b * s.T + t) * s.ld;
+        for (int n = threadIdx.x; n < s.O_all; n += blockDim.x) {
+            double prior = s.prior_o[n] * pt;
+            if (!(prior > 0.)) continue;
+            double dd = (double)row[n] - mind2;
+            if (dd > 700.) continue;
+            double w = prior * exp(-dd);
+            if (!(w > 0.)) continue;
+            sum += w;
+            npos++;
+            mx = fmax(mx, w);
+        }
+    }
+    sum = wave_sum(sum);
+    npos = wave_sum_u64(npos);
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
+    __shared__ double rs[16], rm[16];
+    __shared__ unsigned long long rn[16];
+    if ((threadIdx.x & 63) == 0) { rs[threadIdx.x >> 6] = sum; rm[threadIdx.x >> 6] = mx; rn[threadIdx.x >> 6] = npos; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0., m = 0.;
+        unsigned long long c = 0;
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) { a += rs[i]; m = fmax(m, rm[i]); c += rn[i]; }
+        res[b].sum_w = a;
+        res[b].max_w = m;
+        res[b].npos = (long long)c;
+    }
+}
+
+__global__ __launch_bounds__(1024) void k_coarse_candidates(WeightSrc s, int B, SelParams prm,
+                                                            const SelResult* res, double* cand_w,
+                                                            int* cand_e, int* cand_count)
+{
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    const double mind2 = res[b].min_d2;
+    const long long npos = res[b].npos;
+    const double thr = (npos > 0) ? 0.5 * (1.0 - prm.fraction) * res[b].sum_w / (double)npos : 0.;
+    __shared__ int cnt;
+    if (threadIdx.x == 0) cnt = 0;
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    for (int t = 0; t < s.T; t++) {
+        const double pt = s.prior_t[(size_t)b * s.T + t];
+        const float* row = s.d2 + ((size_t)b * s.T + t) * s.ld;
+        for (int base = 0; base < s.O_all; base += blockDim.x) {
+            const int n = base + threadIdx.x;
+            bool keep = false;
+            double w = 0.;
+            if (n < s.O_all) {
+                double prior = s.prior_o[n] * pt;
+                if (prior > 0.) {
+                    double dd = (double)row[n] - mind2;
+                    if (dd <= 700.) {
+                        w = prior * exp(-dd);
+                        keep = (w > 0.) && (w >= thr);
+                    }
+                }
+            }
+            unsigned long long m = __ballot(keep);
+            if (m) {
+                int first = 0;
+                if (lane == __ffsll((long long)m) - 1) first = atomicAdd(&cnt, __popcll(m));
+                first = __shfl(first, __ffsll((long long)m) - 1);
+                if (keep) {
+                    int slot = first + __popcll(m & ((1ull << lane) - 1));
+                    if (slot < CAND_CAP) {
+                        cand_w[(size_t)b * CAND_CAP + slot] = w;
+                        cand_e[(size_t)b * CAND_CAP + slot] = t * s.O_all + n;
+                    }
+                }
+            }
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) cand_count[b] = cnt;
+}
+
+// Bitonic sort in LDS of n <= CAND_CAP (key, payload) pairs; desc on doubles or asc on ints.
+__device__ void sort_pairs_desc(double* w, int* e, int n)
+{
+    int m = 1;
+    while (m < n) m <<= 1;
+    for (int i = n + threadIdx.x; i < m; i += blockDim.x) { w[i] = -1.0; e[i] = 0x7fffffff; }
+    __syncthreads();
+    for (int k = 2; k <= m; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < m; i += blockDim.x) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    bool desc = ((i & k) == 0);
+                    double x = w[i], y = w[ixj];
+                    bool sw = desc ? (x < y) : (x > y);
+                    if (sw) { w[i] = y; w[ixj] = x; int t = e[i]; e[i] = e[ixj]; e[ixj] = t; }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+__device__ void sort_int_asc(int* a, int n)
+{
+    int m = 1;
+    while (m < n) m <<= 1;
+    for (int i = n + threadIdx.x; i < m; i += blockDim.x) a[i] = 0x7fffffff;
+    __syncthreads();
+    for (int k = 2; k <= m; k <<= 1)
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < m; i += blockDim.x) {
+                int ixj = i ^ j;
+                if (ixj > i) {
+                    bool asc = ((i & k) == 0);
+                    int x = a[i], y = a[ixj];
+                    if (asc ? (x > y) : (x < y)) { a[i] = y; a[ixj] = x; }
+                }
+            }
+            __syncthreads();
+        }
+}
+
+// Exact walk over the sorted candidates; the significant pairs, ordered by
+// (orientation, translation), go to a per-particle strided list.
+__global__ __launch_bounds__(1024) void k_cand_select(WeightSrc s, int B, SelParams prm,
+                                                      SelResult* res, const double* cand_w,
+                                                      const int* cand_e, const int* cand_count,
+                                                      int2* pairs_strided, long long* counts,
+                                                      int* overflow)
+{
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    __shared__ double w[CAND_CAP];
+    __shared__ int e[CAND_CAP];
+    __shared__ int nsig_pairs;
+    const int n = cand_count[b];
+    if (n > CAND_CAP || n <= 0) {
+        if (threadIdx.x == 0) overflow[b] = 1;
+        return;
+    }
+    for (int i = threadIdx.x; i < n; i += blockDim.x) {
+        w[i] = cand_w[(size_t)b * CAND_CAP + i];
+        e[i] = cand_e[(size_t)b * CAND_CAP + i];
+    }
+    __syncthreads();
+    sort_pairs_desc(w, e, n);
+    if (threadIdx.x == 0) {
+        const double target = prm.fraction * res[b].sum_w;
+        double frac = 0., sig = 0.;
+        long long cnt = 0;
+        int i = 0;
+        bool crossed = false;
+        for (; i < n; i++) {
+            if (prm.max_sig > 0) {
+                if (cnt < prm.max_sig) {
+                    if (prm.count_sig) cnt++;
+                    sig = w[i];
+                }
+            } else {
+                if (prm.count_sig) cnt++;
+                sig = w[i];
+            }
+            frac += w[i];
+            if (frac > target) { crossed = true; break; }
+        }
+        if (!crossed) overflow[b] = 2;   // the walk must end inside the candidates
+        res[b].sig_w = sig;
+        res[b].nsig = cnt;
+        int m = 0;
+        while (m < n && w[m] >= sig) m++;
+        nsig_pairs = m;
+    }
+    __syncthreads();
+    const int m = nsig_pairs;
+    // order the significant prefix by (orientation, translation): key = n*T + t
+    for (int i = threadIdx.x; i < m; i += blockDim.x) {
+        int ee = e[i];
+        int t = ee / s.O_all, nn = ee - t * s.O_all;
+        e[i] = nn * s.T + t;
+    }
+    __syncthreads();
+    sort_int_asc(e, m);
+    for (int i = threadIdx.x; i < m; i += blockDim.x)
+        pairs_strided[(size_t)b * CAND_CAP + i] = make_int2(e[i] / s.T, e[i] % s.T);
+    if (threadIdx.x == 0) {
+        int no = 0;
+        for (int i = 0; i < m; i++)
+            if (i == 0 || e[i] / s.T != e[i - 1] / s.T) no++;
+        counts[2 * b] = m;
+        counts[2 * b + 1] = no;
+    }
+}
+
+// Strided per-particle lists -> compact arrays at the host-computed offsets.
+__global__ void k_cand_scatter(int B, const int2* pairs_strided, const long long* pair_off,
+                               const long long* orient_off, int2* pairs, int* orients,
+                               long long* orient_pair_start)
+{
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    const long long p0 = pair_off[b], m = pair_off[b + 1] - p0;
+    for (long long i = threadIdx.x; i < m; i += blockDim.x)
+        pairs[p0 + i] = pairs_strided[(size_t)b * CAND_CAP + i];
+    if (threadIdx.x == 0) {
+        long long o = orient_off[b];
+        for (long long i = 0; i < m; i++) {
+            int2 pr = pairs_strided[(size_t)b * CAND_CAP + i];
+            if (i == 0 || pr.x != pairs_strided[(size_t)b * CAND_CAP + i - 1].x) {
+                orients[o] = pr.x;
+                orient_pair_start[o] = p0 + i;
+                o++;
+            }
+        }
+    }
+}
+
+}  // namespace t2x

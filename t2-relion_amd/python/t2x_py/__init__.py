@@ -105,7 +105,7 @@ class Stats(C.Structure):
         [(n, C.c_double) for n in ("ms_preprocess", "ms_coarse_project", "ms_coarse_diff2",
                                    "ms_coarse_weights", "ms_fine_diff2", "ms_fine_weights",
                                    "ms_store", "ms_coarse_gemm")] + \
-        [("n_coarse_diff2_launches", C.c_long)] + \
+        [("n_select_fallbacks", C.c_long), ("n_coarse_diff2_launches", C.c_long)] + \
         [(n, C.c_double) for n in ("coarse_diff2_flop", "coarse_diff2_bytes", "fine_samples",
                                    "fine_flop", "bp_orientations", "sig_coarse")] + \
         [(n, C.c_int) for n in ("coarse_size", "current_size", "P_coarse", "P_fine")]

@@ -112,3 +112,26 @@ def test_full_size_properties(gpu_device):
     assert g["wsum"]["pdf_class"].sum() == pytest.approx(wl.n, rel=1e-6)
     assert (out["iclass"] - 1 == wl.truth["cls"]).mean() >= 0.99
     assert C is not None and T is not None
+
+
+def test_radix_select_fallback_matches(gpu_device, monkeypatch):
+    """The radix-select path (used when the candidate buffer overflows) gives the same
+    significant sets and poses as the candidate path and the oracle."""
+    monkeypatch.setenv("T2X_FORCE_RADIX_SELECT", "1")
+    wl = H.Workload("refine64", n=8, seed_offset=11)
+    o = wl.run_oracle(n_debug=8, fine_cap=16384)
+    g = wl.run_gpu(dev=gpu_device, n_debug=8, fine_cap=16384)
+    assert (o["dbg"]["coarse_significant"] == g["dbg"]["coarse_significant"]).all(axis=1).mean() >= 0.99
+    assert (o["out"]["best_hidden"] == g["out"]["best_hidden"]).mean() >= 0.999
+    assert H.rel_norm(g["bp"][0][1], o["bp"][0][1]) < 1e-4
+
+
+def test_maximum_significants(gpu_device):
+    """--maxsig caps the coarse significant samples (ml_optimiser.cpp:7950-7961)."""
+    wl = H.Workload("refine64", n=8, seed_offset=12, maximum_significants=3)
+    o = wl.run_oracle(n_debug=8, fine_cap=16384)
+    g = wl.run_gpu(dev=gpu_device, n_debug=8, fine_cap=16384)
+    assert np.all(o["out"]["nr_significant_coarse"] <= 3)
+    assert np.array_equal(o["out"]["nr_significant_coarse"], g["out"]["nr_significant_coarse"])
+    assert (o["dbg"]["coarse_significant"] == g["dbg"]["coarse_significant"]).all(axis=1).mean() >= 0.99
+    assert (o["out"]["best_hidden"] == g["out"]["best_hidden"]).mean() >= 0.999
