@@ -1,6 +1,7 @@
 #!/usr/bin/env python3
 """E-step throughput on one MI355X: the 3D-classification workload (20,000 particles of 256^2,
-K=4 references of 256^3 at 1.1 A/px, HEALPix order 3, offsets 6/1.5 px), plain FP32 path.
+K=4 references of 256^3 at 1.1 A/px, HEALPix order 3, offsets 6/1.5 px): the plain FP32 path
+and the split-precision path (FP16 hi+lo cross term on MFMA) timed in the same run.
 
 A step is one full E-step iteration over all particles, which are synthesised once on the GPU
 and stay resident in HBM: reference upload + coarse projections, preprocessing, both sampling
@@ -25,7 +26,10 @@ import t2x_py as T  # noqa: E402
 from t2x_py import workloads as W  # noqa: E402
 
 METRIC = "E-step particles/s, 1 MI355X: plain FP32 vs mixed+learned at equal FSC-0.143"
-MFMA_FP32_PEAK_TFLOPS = 157.3
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2500.0, "fp16x3": 2500.0}
+PRECISIONS = {"fp32": 0, "bf16x3": 1, "fp16x3": 2}
+DTYPES = {"fp32": "fp32", "bf16x3": "bf16x3 (bf16 hi+lo MFMA, fp32 accumulate, fp64 normalisation)",
+          "fp16x3": "fp16x3 (fp16 hi+lo MFMA, fp32 accumulate, fp64 normalisation)"}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -123,6 +127,9 @@ def main():
     ap.add_argument("--particles", type=int, default=0, help="default: the workload's count")
     ap.add_argument("--warmup-particles", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--precision", default="fp32,fp16x3",
+                    help="comma list of cross-term variants to time (fp32, bf16x3, fp16x3); the "
+                         "last one is the headline value, the others are reported beside it")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the E-step "
@@ -145,52 +152,73 @@ def main():
     outs = (T.ParticleOut * n)()
     ws, wa = T.make_wsum(cfg.nr_classes, ds["info"].nr_dir, cfg.ori_size)
     nw = min(n, args.warmup_particles)
-    for i in range(args.warmup):
-        t = time.time()
-        run_step(lib, dev, cfg, sz, ds, nw, outs, ws)
+    runs = {}
+    for pname in args.precision.split(","):
+        cfg.precision = PRECISIONS[pname]
+        for i in range(args.warmup):
+            t = time.time()
+            run_step(lib, dev, cfg, sz, ds, nw, outs, ws)
+            T.check(lib.t2x_device_synchronize(dev))
+            log(f"[{pname}] warmup {i}: {nw} particles in {time.time() - t:.2f} s")
+        T.check(lib.t2x_stats_reset(dev))
         T.check(lib.t2x_device_synchronize(dev))
-        log(f"warmup {i}: {nw} particles in {time.time() - t:.2f} s")
-    T.check(lib.t2x_stats_reset(dev))
-    T.check(lib.t2x_device_synchronize(dev))
-    t0 = time.time()
-    for i in range(args.steps):
-        run_step(lib, dev, cfg, sz, ds, n, outs, ws)
-        log(f"step {i} done at {time.time() - t0:.2f} s")
-    T.check(lib.t2x_device_synchronize(dev))
-    dt = time.time() - t0
-    st = T.Stats()
-    T.check(lib.t2x_stats_get(dev, C.byref(st)))
-    value = n * args.steps / dt
-    # dominant kernel: coarse diff2 GEMM; algorithmic 4*T*O*P flops per particle
-    gemm_ms = st.ms_coarse_gemm if hasattr(st, "ms_coarse_gemm") else st.ms_coarse_diff2
+        t0 = time.time()
+        for i in range(args.steps):
+            run_step(lib, dev, cfg, sz, ds, n, outs, ws)
+            log(f"[{pname}] step {i} done at {time.time() - t0:.2f} s")
+        T.check(lib.t2x_device_synchronize(dev))
+        dt = time.time() - t0
+        st = T.Stats()
+        T.check(lib.t2x_stats_get(dev, C.byref(st)))
+        best = np.array([o.best_hidden for o in outs])
+        runs[pname] = dict(dt=dt, st=st, value=n * args.steps / dt, best=best)
+        log(f"[{pname}] {runs[pname]['value']:.1f} particles/s")
+    head = args.precision.split(",")[-1]
+    r = runs[head]
+    st, dt, value = r["st"], r["dt"], r["value"]
+    # dominant kernel: coarse diff2 GEMM; algorithmic 4*T*O*P flops per particle (x3 products
+    # on 16-bit MFMA for the split-precision variants)
     launches = max(1, st.n_coarse_diff2_launches)
-    flop_per_launch = st.coarse_diff2_flop / launches
-    avg_ms = gemm_ms / launches
+    nprod = 1 if head == "fp32" else 3
+    flop_per_launch = nprod * st.coarse_diff2_flop / launches
+    avg_ms = st.ms_coarse_gemm / launches
+    peak = MFMA_PEAK_TFLOPS[head]
     achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
     traffic = os.environ.get("T2X_GEMM_TRAFFIC_BYTES")
-    roof = {"bound": "mfma", "achieved": achieved, "peak": MFMA_FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
-            "frac": achieved / MFMA_FP32_PEAK_TFLOPS, "traffic": float(traffic) if traffic else None,
-            "kernel": "k_gemm_tn<EPI_DIFF2> (coarse diff2 cross term, FP32 MFMA 16x16x4)",
+    kname = {"fp32": "k_gemm_tn<EPI_DIFF2> (FP32 MFMA 16x16x4)",
+             "bf16x3": "k_gemm_x3<bf16, EPI_DIFF2> (BF16 MFMA 16x16x32, 3 products)",
+             "fp16x3": "k_gemm_x3<f16, EPI_DIFF2> (FP16 MFMA 16x16x32, 3 products)"}[head]
+    roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+            "frac": achieved / peak, "traffic": float(traffic) if traffic else None,
+            "kernel": "coarse diff2 cross term, " + kname,
             "avg_launch_ms": avg_ms, "launches": launches, "flop_per_launch": flop_per_launch}
     stages = {k: getattr(st, k) for k in ("ms_preprocess", "ms_coarse_project", "ms_coarse_diff2",
                                            "ms_coarse_weights", "ms_fine_diff2", "ms_fine_weights",
                                            "ms_store")}
     nsig = st.sig_coarse / max(1, st.n_particles)
     cpu = None if args.no_cpu else cpu_baseline(w, cfg, sz)
+    variants = {}
+    for pname, rr in runs.items():
+        variants[pname] = {"particles_per_s": rr["value"], "ms_per_step": rr["dt"] * 1e3 / args.steps,
+                           "coarse_gemm_ms": rr["st"].ms_coarse_gemm,
+                           "pose_agreement_vs_fp32": (float((rr["best"] == runs["fp32"]["best"]).mean())
+                                                      if "fp32" in runs else None)}
     line = {
         "metric": METRIC, "value": value, "unit": "particles/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps,
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "fp32",
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": DTYPES[head],
         "data": "synthetic (GPU-generated blob volumes and CTF-modulated noisy projections)",
         "config": {"workload": f"{args.workload}: {n} particles {cfg.ori_size}^2 @ {cfg.pixel_size} A/px, "
                                f"K={cfg.nr_classes}, HEALPix order {cfg.healpix_order}, offsets "
                                f"{w['offset_range_px']}/{w['offset_step_px']} px, current_size "
-                               f"{cfg.current_size}, coarse_size {cfg.coarse_size}, plain FP32 path",
+                               f"{cfg.current_size}, coarse_size {cfg.coarse_size}, {head} cross term",
                    "global_batch": n, "parallelism": "single GPU",
                    "P_coarse": st.P_coarse, "P_fine": st.P_fine,
                    "mean_significant_coarse": nsig,
                    "fine_samples_per_particle": st.fine_samples / max(1, st.n_particles),
                    "stage_ms": stages},
+        "variants": variants,
+        "speedup_vs_plain_fp32": (value / runs["fp32"]["value"]) if "fp32" in runs else None,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

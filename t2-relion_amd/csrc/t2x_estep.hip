@@ -34,6 +34,11 @@ struct EpiParams {
 };
 int launch_gemm(int epi, const float* A, long long lda, const float* B, long long ldb, int K,
                 int Mrows_pad, int Nrows_pad, const EpiParams& ep, hipStream_t st);
+int launch_split(int prec, const float* X, long long ldx, int K, int rows, int valid_rows, void* hi,
+                 void* lo, long long ld, float* inv_scale, hipStream_t st);
+int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const float* As, long long lda,
+                   const void* Bh, const void* Bl, const float* Bs, long long ldb, int K, int Mrows_pad,
+                   int Nrows_pad, const EpiParams& ep, hipStream_t st);
 
 __global__ void k_pass_min(WeightSrc s, int B, SelResult* res);
 __global__ void k_select(WeightSrc s, SelParams prm, int B, SelResult* res);
@@ -855,7 +860,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     const long long O_pad = round_up(O_all, 128);
     std::string key = std::to_string(cc) + "/" + std::to_string(cs) + "/" + std::to_string(K) + "/" +
                       std::to_string(cfg->healpix_order) + "/" + std::to_string(cfg->psi_step) + "/" +
-                      std::to_string(tc.npix);
+                      std::to_string(tc.npix) + "/p" + std::to_string(cfg->precision);
+    const int prec = cfg->precision;
     for (int c = 0; c < K; c++) key += "/" + std::to_string(d->classes[c].proj_version);
     EventTimer et{d, true};
     (void)hipEventRecord(d->ev[0], st);
@@ -880,6 +886,13 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             tc.d_pix.as<int2>(), tc.d_inproj.as<unsigned char>(), tc.npix, d->coarse_B.as<float>(), Kc,
             d->coarse_B2.as<float>(), Kn);
         T2X_HIP(hipGetLastError());
+        if (prec != T2X_PREC_FP32) {
+            T2X_TRY(d->coarse_Bh.ensure(sizeof(uint16_t) * O_pad * Kc));
+            T2X_TRY(d->coarse_Bl.ensure(sizeof(uint16_t) * O_pad * Kc));
+            T2X_TRY(d->coarse_Bs.ensure(sizeof(float) * O_pad));
+            T2X_TRY(launch_split(prec, d->coarse_B.as<float>(), Kc, Kc, (int)O_pad, (int)O_all, d->coarse_Bh.ptr,
+                                 d->coarse_Bl.ptr, Kc, d->coarse_Bs.as<float>(), st));
+        }
         T2X_HIP(hipStreamSynchronize(st));
         d_Ai.release();
         d->coarse_key = key;
@@ -903,7 +916,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     double* w_off = w_pdfd + K * Nd;
 
     // ---- batch size: coarse diff2 dominates memory (B*T*O_pad floats) ----
-    const size_t per_particle = (size_t)T * O_pad * 4 + (size_t)T * Kc * 4 + (size_t)4 * N * N * 4 +
+    const size_t per_particle = (size_t)T * O_pad * 4 + (size_t)T * Kc * (prec == T2X_PREC_FP32 ? 4 : 8) +
+                                (size_t)4 * N * N * 4 +
                                 (size_t)2 * N * xN * 8 * 2;
     size_t budget = (size_t)24 << 30;   // 24 GiB of scratch for the batch
     {
@@ -1065,8 +1079,19 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             // rows beyond the valid columns of the last tile keep +inf
             k_fill_f32<<<(unsigned)((M_pad * rowmin_ld + 255) / 256), 256, 0, st>>>(d_rowmin.as<float>(), 3.0e38f,
                                                                                     M_pad * rowmin_ld);
-            T2X_TRY(launch_gemm(EPI_DIFF2, d->A.as<float>(), Kc, d->coarse_B.as<float>(), Kc, Kc,
-                                (int)M_pad, (int)O_pad, ep, st));
+            if (prec == T2X_PREC_FP32) {
+                T2X_TRY(launch_gemm(EPI_DIFF2, d->A.as<float>(), Kc, d->coarse_B.as<float>(), Kc, Kc,
+                                    (int)M_pad, (int)O_pad, ep, st));
+            } else {
+                T2X_TRY(d->Ah.ensure(sizeof(uint16_t) * M_pad * Kc));
+                T2X_TRY(d->Al.ensure(sizeof(uint16_t) * M_pad * Kc));
+                T2X_TRY(d->As.ensure(sizeof(float) * M_pad));
+                T2X_TRY(launch_split(prec, d->A.as<float>(), Kc, Kc, (int)M_pad, (int)M, d->Ah.ptr, d->Al.ptr, Kc,
+                                     d->As.as<float>(), st));
+                T2X_TRY(launch_gemm_x3(prec, EPI_DIFF2, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc,
+                                       d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc,
+                                       (int)M_pad, (int)O_pad, ep, st));
+            }
         }
         (void)hipEventRecord(d->ev[ev++], st);   // 3: coarse gemm end
         // ---- coarse weights and significance ----

@@ -134,6 +134,7 @@ struct Device {
     std::string coarse_key;
     DevBuf coarse_B;      // [O_all_pad][K_pad] float: re/im of the coarse slices
     DevBuf coarse_B2;     // [O_all_pad][Kn_pad] float: |F|^2 of the coarse slices
+    DevBuf coarse_Bh, coarse_Bl, coarse_Bs;   // 16-bit hi/lo split of coarse_B + row scales
     int coarse_O_all = 0, coarse_O_pad = 0;
 
     // scratch
@@ -141,7 +142,7 @@ struct Device {
     DevBuf A, A1, N1, diff2, minb, sel_out, prior_o, prior_t, sig_pairs, sig_count, sig_orient,
         sig_orient_count, orient_pairs_off, fine_items, fine_diff2, fine_min, fine_sel, ab_fine,
         ab_coarse, wsum_dev, store_items, best, tables, Y_fine, dbg_tmp, cand_w, cand_e, cand_n,
-        pairs_strided;
+        pairs_strided, Ah, Al, As;
     PassTables coarse_tab, fine_tab;
     std::string tab_key;
 

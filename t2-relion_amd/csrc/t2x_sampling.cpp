@@ -111,7 +111,7 @@ int check_config(const t2x_estep_config* c)
         set_error("only global searches (NOPRIOR) are supported");
         return T2X_EINVAL;
     }
-    if (c->precision != T2X_PREC_FP32) {
+    if (c->precision != T2X_PREC_FP32 && c->precision != T2X_PREC_BF16X3 && c->precision != T2X_PREC_FP16X3) {
         set_error("precision mode not supported");
         return T2X_EINVAL;
     }
