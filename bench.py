@@ -127,6 +127,9 @@ def main():
     ap.add_argument("--particles", type=int, default=0, help="default: the workload's count")
     ap.add_argument("--warmup-particles", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run one 1 GiB device-to-device copy kernel whose FETCH_SIZE/WRITE_SIZE "
+                         "counters calibrate the per-dispatch HBM traffic of the GEMM in --pmc runs")
     ap.add_argument("--precision", default="fp32,fp16x3",
                     help="comma list of cross-term variants to time (fp32, bf16x3, fp16x3); the "
                          "last one is the headline value, the others are reported beside it")
@@ -151,6 +154,15 @@ def main():
         T.check(lib.t2x_backprojector_init(dev, c, 3, sz["pad_size"], sz["r_max"], 2.0), "bp init")
     outs = (T.ParticleOut * n)()
     ws, wa = T.make_wsum(cfg.nr_classes, ds["info"].nr_dir, cfg.ori_size)
+    if args.copy_calibration:
+        src, dst = C.c_void_p(), C.c_void_p()
+        T.check(lib.t2x_device_alloc(dev, 1 << 30, C.byref(src)), "alloc")
+        T.check(lib.t2x_device_alloc(dev, 1 << 30, C.byref(dst)), "alloc")
+        T.check(lib.t2x_device_copy(dev, dst, src, 1 << 30), "copy")
+        T.check(lib.t2x_device_synchronize(dev))
+        T.check(lib.t2x_device_free(dev, src))
+        T.check(lib.t2x_device_free(dev, dst))
+        log("copy calibration: 1 GiB device-to-device copy (t2x::k_copy16) done")
     nw = min(n, args.warmup_particles)
     runs = {}
     for pname in args.precision.split(","):

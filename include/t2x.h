@@ -254,6 +254,10 @@ int t2x_device_free(t2x_device* dev, void* dptr);
 int t2x_memcpy_h2d(t2x_device* dev, void* dst, const void* src, size_t bytes);
 int t2x_memcpy_d2h(t2x_device* dev, void* dst, const void* src, size_t bytes);
 int t2x_device_synchronize(t2x_device* dev);
+/* Device-to-device copy by a plain vector-load/store kernel on the handle's stream (bytes a
+ * multiple of 16, 16-byte aligned pointers). Its HBM counters read exactly 2*bytes of traffic,
+ * which calibrates per-dispatch FETCH_SIZE / WRITE_SIZE figures. No reference counterpart. */
+int t2x_device_copy(t2x_device* dev, void* dst, const void* src, size_t bytes);
 
 /* Diagnostic copy-out of internal E-step buffers for parity tests (no reference counterpart):
  * which = 0 coarse slices (re/im), 1 coarse |F|^2, 2 N1, 3 image-side operand, 4 A1, 5 coarse diff2. */
@@ -282,6 +286,40 @@ int t2x_synth_particles_device(t2x_device* dev, int ori_size, double pixel_size,
                                const double* true_x, const double* true_y, const int* true_class,
                                const double* snr, uint64_t noise_seed, float* dimages,
                                double* noise_var /* [n] host, may be NULL */);
+
+/* ---------------------------------------------------------------------------------------
+ * Reconstruction (M-step side of the backprojector), all arithmetic in FP64 on the GPU.
+ * ------------------------------------------------------------------------------------- */
+typedef struct t2x_recon_params {
+    int    ori_size;        /* box size of the output map                                     */
+    int    pad_size;        /* BackProjector::pad_size = 2*(ROUND(padding_factor*r_max)+1)+1   */
+    int    r_max;           /* BackProjector::r_max (Fourier pixels of the original box)       */
+    double padding_factor;
+    int    ref_dim;         /* 3                                                              */
+    int    data_dim;        /* 2 for particle images (selects the FFT normalisation)          */
+    int    do_map;          /* add 1/(pf^3 tau2_fudge tau2) to the weights (MAP regularisation) */
+    double tau2_fudge;
+    int    minres_map;      /* shells below this one get no tau2 term                         */
+    int    skip_gridding;   /* must be 1: relion_refine's default (no --dont_skip_gridding)    */
+    int    reserved[6];
+} t2x_recon_params;
+
+/* Replaces BackProjector::reconstruct (backprojector.cpp:1344-1770) on its default
+ * skip_gridding branch: decenter (projector.h:242-253), MAP tau2 term (:1416-1457), radially
+ * averaged weight floor and data/weight division (:1459-1534), windowToOridimRealSpace
+ * (:2486-2600: Fourier window to 2*ori, inverse FFT, real-space window, FFT normalisation,
+ * softMaskOutsideMap mask.cpp:43-97) and griddingCorrect (projector.cpp Projector::griddingCorrect,
+ * trilinear sinc^2). Inputs are BackProjector::data (re, im) and ::weight in the projector
+ * layout pad*pad*(pad/2+1) [z][y][x], y and z centred at pad/2; tau2 has ori_size/2+1 shells
+ * (used when do_map). vol_out: ori_size^3 doubles, [z][y][x], centre at ori_size/2.
+ * The x = 0 and Nyquist planes are Hermitian-symmetrised before the half-complex inverse
+ * transform, which is what such a transform does with their imaginary parts. */
+int t2x_reconstruct(t2x_device* dev, const t2x_recon_params* p, const double* data_re,
+                    const double* data_im, const double* weight, const double* tau2, double* vol_out);
+
+/* Replaces getFSC(MultidimArray<RFLOAT>&, MultidimArray<RFLOAT>&, fsc) (fftw.cpp:481-523): Fourier
+ * shell correlation of two n^3 real maps ([z][y][x]); fsc has n/2+1 shells, shell = ROUND(|k|). */
+int t2x_fsc(t2x_device* dev, int n, const double* map1, const double* map2, double* fsc);
 
 #ifdef __cplusplus
 }

@@ -105,6 +105,12 @@ int orc_weights_from_diff2(long n, const double* diff2, const double* prior, dou
                            double* weights, double* sum_weight, double* significant_weight,
                            long* nr_significant);
 
+/* --- reconstruction (recon_oracle.cpp): BackProjector::reconstruct, skip_gridding branch;
+ *     getFSC of two real maps. Layouts as t2x_reconstruct / t2x_fsc. --- */
+int orc_reconstruct(const t2x_recon_params* p, const double* data_re, const double* data_im,
+                    const double* weight, const double* tau2, double* vol_out);
+int orc_fsc(int n, const double* m1, const double* m2, double* fsc);
+
 #ifdef __cplusplus
 }
 #endif

@@ -6,6 +6,7 @@
 // Volumes live in plain HBM arrays (no textures): the E-step kernels do their own exact
 // trilinear interpolation, as the CPU path does (projector.cpp:585-640).
 #include "t2x_internal.h"
+#include <algorithm>
 
 namespace t2x {
 
@@ -258,6 +259,28 @@ extern "C" int t2x_memcpy_d2h(t2x_device* h, void* dst, const void* src, size_t 
 {
     if (!h) { set_error("null handle"); return T2X_EINVAL; }
     T2X_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+    return T2X_OK;
+}
+
+__global__ void k_copy16(const uint4* __restrict__ src, uint4* __restrict__ dst, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+extern "C" int t2x_device_copy(t2x_device* h, void* dst, const void* src, size_t bytes)
+{
+    if (!h || !dst || !src) { set_error("null argument"); return T2X_EINVAL; }
+    if (bytes % 16 || (reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) % 16) {
+        set_error("t2x_device_copy: bytes and pointers must be multiples of 16");
+        return T2X_EINVAL;
+    }
+    Device* d = reinterpret_cast<Device*>(h);
+    const size_t n = bytes / 16;
+    if (n == 0) return T2X_OK;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 256 * 64);
+    k_copy16<<<grid, 256, 0, d->stream>>>(reinterpret_cast<const uint4*>(src), reinterpret_cast<uint4*>(dst), n);
+    T2X_HIP(hipGetLastError());
     return T2X_OK;
 }
 

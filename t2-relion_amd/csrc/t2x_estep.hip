@@ -38,7 +38,7 @@ int launch_split(int prec, const float* X, long long ldx, int K, int rows, int v
                  void* lo, long long ld, float* inv_scale, hipStream_t st);
 int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const float* As, long long lda,
                    const void* Bh, const void* Bl, const float* Bs, long long ldb, int K, int Mrows_pad,
-                   int Nrows_pad, const EpiParams& ep, hipStream_t st);
+                   int Nrows_pad, const EpiParams& ep, hipStream_t st, int tile);
 
 __global__ void k_pass_min(WeightSrc s, int B, SelResult* res);
 __global__ void k_select(WeightSrc s, SelParams prm, int B, SelResult* res);
@@ -47,9 +47,12 @@ __global__ void k_sig_write(WeightSrc s, int B, const SelResult* res, const long
                             const long long* orient_off, int2* pairs, int* orients,
                             long long* orient_pair_start);
 __global__ void k_fine_argmax(WeightSrc s, int B, SelResult* res, long long* nfine_sig);
-__global__ void k_coarse_sum(WeightSrc s, int B, SelResult* res);
+struct SumPart;
+__global__ void k_coarse_sum_rows(WeightSrc s, int B, const SelResult* res, SumPart* part);
+__global__ void k_coarse_sum_final(WeightSrc s, int B, const SumPart* part, SelResult* res);
 __global__ void k_coarse_candidates(WeightSrc s, int B, SelParams prm, const SelResult* res,
                                     double* cand_w, int* cand_e, int* cand_count);
+size_t sum_part_bytes();
 __global__ void k_cand_select(WeightSrc s, int B, SelParams prm, SelResult* res, const double* cand_w,
                               const int* cand_e, const int* cand_count, int2* pairs_strided,
                               long long* counts, int* overflow);
@@ -857,7 +860,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     // ---- coarse projections, cached across calls ----
     const int Kc = tc.npix_pad * 2;            // GEMM K (re/im)
     const int Kn = tc.npix_pad;                // N1 GEMM K
-    const long long O_pad = round_up(O_all, 128);
+    const long long O_pad = round_up(O_all, 256);   // multiple of both GEMM tile widths
     std::string key = std::to_string(cc) + "/" + std::to_string(cs) + "/" + std::to_string(K) + "/" +
                       std::to_string(cfg->healpix_order) + "/" + std::to_string(cfg->psi_step) + "/" +
                       std::to_string(tc.npix) + "/p" + std::to_string(cfg->precision);
@@ -1026,7 +1029,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
 
         // ---- coarse pass ----
         const long long M = (long long)B * T;
-        const long long M_pad = round_up(M, 128);
+        const long long M_pad = round_up(M, 256);
         const long long B_pad = round_up(B, 128);
         T2X_TRY(d->A.ensure(sizeof(float) * M_pad * Kc));
         T2X_TRY(d->A1.ensure(sizeof(float) * B_pad * Kn));
@@ -1090,7 +1093,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                                      d->As.as<float>(), st));
                 T2X_TRY(launch_gemm_x3(prec, EPI_DIFF2, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc,
                                        d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc,
-                                       (int)M_pad, (int)O_pad, ep, st));
+                                       (int)M_pad, (int)O_pad, ep, st, 0));
             }
         }
         (void)hipEventRecord(d->ev[ev++], st);   // 3: coarse gemm end
@@ -1121,10 +1124,15 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         T2X_TRY(d->pairs_strided.ensure(sizeof(int2) * B * kCandCap));
         int* d_cand_count = d->cand_n.as<int>();
         int* d_overflow = d_cand_count + B;
-        T2X_HIP(hipMemsetAsync(d_overflow, 0, sizeof(int) * B, st));
-        k_coarse_sum<<<B, 1024, 0, st>>>(wc, B, d_selc);
-        k_coarse_candidates<<<B, 1024, 0, st>>>(wc, B, sp, d_selc, d->cand_w.as<double>(), d->cand_e.as<int>(),
-                                                d_cand_count);
+        T2X_HIP(hipMemsetAsync(d_cand_count, 0, sizeof(int) * 2 * B, st));   // counts and overflow flags
+        T2X_TRY(d->sum_part.ensure(sum_part_bytes() * B * T));
+        {
+            const dim3 grid((unsigned)T, (unsigned)B);
+            k_coarse_sum_rows<<<grid, 256, 0, st>>>(wc, B, d_selc, d->sum_part.as<SumPart>());
+            k_coarse_sum_final<<<B, 64, 0, st>>>(wc, B, d->sum_part.as<SumPart>(), d_selc);
+            k_coarse_candidates<<<grid, 256, 0, st>>>(wc, B, sp, d_selc, d->cand_w.as<double>(),
+                                                      d->cand_e.as<int>(), d_cand_count);
+        }
         k_cand_select<<<B, 1024, 0, st>>>(wc, B, sp, d_selc, d->cand_w.as<double>(), d->cand_e.as<int>(),
                                           d_cand_count, d->pairs_strided.as<int2>(),
                                           d->sig_count.as<long long>(), d_overflow);

@@ -4,10 +4,10 @@
 // getAllSquaredDifferences (ml_optimiser.cpp:7265-7273) is recast as
 //   diff2[b,t,n] = C_b + N1[b,n] - sum_k A[(b,t),k] B[n,k]
 // with rows (particle, translation), columns (class, orientation) and k = (pixel, re/im).
-// Tiles: 128x128 per workgroup, 4 waves of 64x64, K step 16, v_mfma_f32_16x16x4f32.
-// Workgroups are remapped so that each XCD gets a contiguous range of tiles and walks the M
-// dimension fastest inside a column group, keeping B tiles hot in that XCD's L2.
+// Tiles: 128x128 per workgroup, 4 waves of 64x64, K step 16, v_mfma_f32_16x16x4f32 (FP32) or
+// K step 32, v_mfma_f32_16x16x32_{bf16,f16} on split hi/lo operands (split precision).
 #include "t2x_kernels.h"
+#include <algorithm>
 
 namespace t2x {
 
@@ -29,35 +29,109 @@ struct EpiParams {
 
 #define GBM 128
 #define GBN 128
+
+// Tile order. Workgroups are dealt round-robin to the 8 XCDs, and each XCD keeps ~64 of them
+// resident. Tiles are walked in groups of `group_m` tile-rows with M fastest, so with
+// group_m = 64 the 512 workgroups in flight cover 64 tile-rows x 8 tile-columns and each XCD
+// gets 8 rows x 8 columns of them: every A and B tile slice fetched into an XCD's L2 feeds 8
+// workgroups there, and all XCDs walk the same B columns at the same time (one HBM read of B
+// per row group, shared through the MALL). The map is a bijection for any tiles_m, tiles_n.
+__device__ __forceinline__ void tile_coords(int pid, int tiles_m, int tiles_n, int group_m, int& tm, int& tn)
+{
+    const int per_group = group_m * tiles_n;
+    const int gid = pid / per_group;
+    const int first_m = gid * group_m;
+    const int gsize = min(tiles_m - first_m, group_m);
+    const int in_group = pid - gid * per_group;
+    tm = first_m + in_group % gsize;
+    tn = in_group / gsize;
+}
+
+// Shared epilogue of the GEMM kernels for one wave's block of MI x 4 MFMA tiles (lane holds
+// C[4*(lane/16)+r][lane%16] of each 16x16 tile). Rows start at mbase, columns at ncol0 (a
+// multiple of 64). EPI_STORE writes C = acc * row scale * column scale; EPI_DIFF2 writes
+// diff2 = C_b + N1[b][n] - cross (b = row / T, rounded once from FP64) and the min over the
+// 64 columns of each row restricted to scored orientations (prior > 0). Each 16-row block's
+// inputs are gathered before use so their loads overlap.
+template <int MI, int EPI>
+__device__ __forceinline__ void gemm_epilogue(const floatx4 (&acc)[MI][4], long long mbase, long long ncol0,
+                                              int lane, const float* __restrict__ a_scale,
+                                              const float* __restrict__ b_scale, const EpiParams& ep)
+{
+    const int col_in = lane & 15, row_grp = (lane >> 4) * 4;
+    float sb[4];
+    bool nval[4], scored[4];
+    long long ncol[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        ncol[j] = ncol0 + j * 16 + col_in;
+        nval[j] = ncol[j] < ep.N;
+        sb[j] = (b_scale && nval[j]) ? b_scale[ncol[j]] : 1.f;
+        scored[j] = (EPI == EPI_DIFF2) && nval[j] && ep.prior_o[ncol[j]] > 0.;
+    }
+    const float invT = (EPI == EPI_DIFF2) ? 1.f / (float)ep.T : 0.f;
+#pragma unroll
+    for (int i = 0; i < MI; i++) {
+        int mrow[4];
+        bool mval[4];
+        float sa[4], n1[4][4];
+        double cb[4];
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const long long m = mbase + i * 16 + row_grp + r;
+            mval[r] = m < ep.M;
+            const int mm = mval[r] ? (int)m : 0;
+            mrow[r] = mm;
+            sa[r] = a_scale ? a_scale[mm] : 1.f;
+            if (EPI == EPI_DIFF2) {
+                int b = (int)((float)mm * invT);   // particle index m / T: float estimate, then exact
+                while (b > 0 && b * ep.T > mm) b--;
+                while ((b + 1) * ep.T <= mm) b++;
+                cb[r] = ep.Cb[b];
+                const float* n1row = ep.N1 + (long long)b * ep.ldn;
+#pragma unroll
+                for (int j = 0; j < 4; j++) n1[r][j] = n1row[nval[j] ? ncol[j] : 0];
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            float rmin = 3.0e38f;
+            float* crow = ep.C + (long long)mrow[r] * ep.ldc;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (mval[r] && nval[j]) {
+                    const float v = acc[i][j][r] * sa[r] * sb[j];
+                    if (EPI == EPI_STORE) {
+                        crow[ncol[j]] = v;
+                    } else {
+                        const float d = (float)(cb[r] + (double)n1[r][j] - (double)v);
+                        crow[ncol[j]] = d;
+                        if (scored[j]) rmin = fminf(rmin, d);
+                    }
+                }
+            }
+            if (EPI == EPI_DIFF2) {
+                for (int o = 1; o < 16; o <<= 1) rmin = fminf(rmin, __shfl_xor(rmin, o));
+                if (col_in == 0 && mval[r]) ep.rowmin[(long long)mrow[r] * ep.rowmin_ld + ncol0 / 64] = rmin;
+            }
+        }
+    }
+}
 #define GBK 16
 #define GPAD 4
 
 template <int EPI>
 __global__ __launch_bounds__(256) void k_gemm_tn(const float* __restrict__ A, long long lda,
                                                  const float* __restrict__ B, long long ldb,
-                                                 int K, int tiles_m, int tiles_n, EpiParams ep)
+                                                 int K, int tiles_m, int tiles_n, int group_m, EpiParams ep)
 {
     __shared__ float As[GBK][GBM + GPAD];
     __shared__ float Bs[GBK][GBN + GPAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
 
-    // XCD-aware tile order: workgroup ids are dealt round-robin to 8 XCDs
-    const int nwg = tiles_m * tiles_n;
-    int pid = blockIdx.x;
-    if ((nwg & 7) == 0) {   // bijective only when the grid divides evenly over the 8 XCDs
-        const int per = nwg / 8;
-        pid = (pid & 7) * per + (pid >> 3);
-    }
-    // within the linear order: groups of 8 tile-columns, M fastest
-    const int group = 8;
-    const int gid = pid / (group * tiles_m);
-    const int first_n = gid * group;
-    const int gsize = min(tiles_n - first_n, group);
-    const int in_group = pid % (group * tiles_m);
-    const int tm = in_group % tiles_m;
-    const int tn = first_n + in_group / tiles_m;
-    if (tn >= tiles_n || gsize <= 0) return;
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, group_m, tm, tn);
     const long long m0 = (long long)tm * GBM, n0 = (long long)tn * GBN;
 
     floatx4 acc[4][4];
@@ -106,46 +180,22 @@ __global__ __launch_bounds__(256) void k_gemm_tn(const float* __restrict__ A, lo
     }
 
     // epilogue: lane holds C[4*(lane/16)+r][lane%16] of each 16x16 sub-tile
-    const int col_in = lane & 15, row_grp = (lane >> 4) * 4;
-#pragma unroll
-    for (int i = 0; i < 4; i++) {
-#pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const long long m = m0 + wm * 64 + i * 16 + row_grp + r;
-            float rmin = 3.0e38f;
-            const bool mvalid = m < ep.M;
-            int bidx = 0;
-            double cb = 0.;
-            if (EPI == EPI_DIFF2 && mvalid) {
-                bidx = (int)(m / ep.T);
-                cb = ep.Cb[bidx];
-            }
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const long long n = n0 + wn * 64 + j * 16 + col_in;
-                float v = acc[i][j][r];
-                if (mvalid && n < ep.N) {
-                    if (EPI == EPI_STORE) {
-                        ep.C[m * ep.ldc + n] = v;
-                    } else {
-                        float d = (float)(cb + (double)ep.N1[(long long)bidx * ep.ldn + n] - (double)v);
-                        ep.C[m * ep.ldc + n] = d;
-                        if (ep.prior_o[n] > 0.) rmin = fminf(rmin, d);
-                    }
-                }
-            }
-            if (EPI == EPI_DIFF2) {
-                // min over the 16 lanes sharing this row (same lane>>4 group)
-                for (int o = 1; o < 16; o <<= 1) rmin = fminf(rmin, __shfl_xor(rmin, o));
-                if (col_in == 0 && mvalid)
-                    ep.rowmin[m * ep.rowmin_ld + (n0 + wn * 64) / 64] = rmin;
-            }
-        }
-    }
+    gemm_epilogue<4, EPI>(acc, m0 + wm * 64, n0 + wn * 64, lane, nullptr, nullptr, ep);
 }
 
-template __global__ void k_gemm_tn<EPI_STORE>(const float*, long long, const float*, long long, int, int, int, EpiParams);
-template __global__ void k_gemm_tn<EPI_DIFF2>(const float*, long long, const float*, long long, int, int, int, EpiParams);
+template __global__ void k_gemm_tn<EPI_STORE>(const float*, long long, const float*, long long, int, int, int, int, EpiParams);
+template __global__ void k_gemm_tn<EPI_DIFF2>(const float*, long long, const float*, long long, int, int, int, int, EpiParams);
+
+// group_m of the tile walk; T2X_GEMM_GROUP_M overrides it for tile-order sweeps
+static int gemm_group_m()
+{
+    static int g = [] {
+        const char* e = getenv("T2X_GEMM_GROUP_M");
+        int v = e ? atoi(e) : 0;
+        return v > 0 ? v : 64;
+    }();
+    return g;
+}
 
 // Host launcher. A must have tiles_m*128 rows, B tiles_n*128 rows, both K (multiple of 16)
 // columns, zero padded.
@@ -160,9 +210,9 @@ int launch_gemm(int epi, const float* A, long long lda, const float* B, long lon
     const long long nwg = (long long)tm * tn;
     if (nwg > 0x7fffffffLL) { set_error("gemm: grid too large"); return T2X_EINVAL; }
     if (epi == EPI_STORE)
-        k_gemm_tn<EPI_STORE><<<(unsigned)nwg, 256, 0, st>>>(A, lda, B, ldb, K, tm, tn, ep);
+        k_gemm_tn<EPI_STORE><<<(unsigned)nwg, 256, 0, st>>>(A, lda, B, ldb, K, tm, tn, gemm_group_m(), ep);
     else
-        k_gemm_tn<EPI_DIFF2><<<(unsigned)nwg, 256, 0, st>>>(A, lda, B, ldb, K, tm, tn, ep);
+        k_gemm_tn<EPI_DIFF2><<<(unsigned)nwg, 256, 0, st>>>(A, lda, B, ldb, K, tm, tn, gemm_group_m(), ep);
     T2X_HIP(hipGetLastError());
     return T2X_OK;
 }
@@ -207,7 +257,7 @@ struct SplitOperand {
 
 template <class H, int EPI>
 __global__ __launch_bounds__(256) void k_gemm_x3(SplitOperand A, SplitOperand B, int K, int tiles_m,
-                                                 int tiles_n, EpiParams ep)
+                                                 int tiles_n, int group_m, EpiParams ep)
 {
     typedef typename Vec8<H>::type v8;
     __shared__ __attribute__((aligned(16))) H Ash[GBM][XK + XPAD];
@@ -216,19 +266,8 @@ __global__ __launch_bounds__(256) void k_gemm_x3(SplitOperand A, SplitOperand B,
     __shared__ __attribute__((aligned(16))) H Bsl[GBN][XK + XPAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const int nwg = tiles_m * tiles_n;
-    int pid = blockIdx.x;
-    if ((nwg & 7) == 0) {
-        const int per = nwg / 8;
-        pid = (pid & 7) * per + (pid >> 3);
-    }
-    const int group = 8;
-    const int gid = pid / (group * tiles_m);
-    const int first_n = gid * group;
-    const int in_group = pid % (group * tiles_m);
-    const int tm = in_group % tiles_m;
-    const int tn = first_n + in_group / tiles_m;
-    if (tn >= tiles_n) return;
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, group_m, tm, tn);
     const long long m0 = (long long)tm * GBM, n0 = (long long)tn * GBN;
 
     floatx4 acc[4][4];
@@ -283,54 +322,214 @@ __global__ __launch_bounds__(256) void k_gemm_x3(SplitOperand A, SplitOperand B,
             bl[j] = *reinterpret_cast<const v8*>(&Bsl[wn * 64 + j * 16 + (lane & 15)][kc]);
         }
 #pragma unroll
-        for (int i = 0; i < 4; i++)
+        for (int i = 0; i < 4; i++) {
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                acc[i][j] = mfma16x16x32<H>(al[i], bh[j], acc[i][j]);
-                acc[i][j] = mfma16x16x32<H>(ah[i], bl[j], acc[i][j]);
-                acc[i][j] = mfma16x16x32<H>(ah[i], bh[j], acc[i][j]);
-            }
-    }
-
-    const int col_in = lane & 15, row_grp = (lane >> 4) * 4;
+            for (int j = 0; j < 4; j++) acc[i][j] = mfma16x16x32<H>(al[i], bh[j], acc[i][j]);
 #pragma unroll
-    for (int i = 0; i < 4; i++) {
+            for (int j = 0; j < 4; j++) acc[i][j] = mfma16x16x32<H>(ah[i], bl[j], acc[i][j]);
 #pragma unroll
-        for (int r = 0; r < 4; r++) {
-            const long long m = m0 + wm * 64 + i * 16 + row_grp + r;
-            float rmin = 3.0e38f;
-            const bool mvalid = m < ep.M;
-            int bidx = 0;
-            double cb = 0.;
-            float sa = 1.f;
-            if (mvalid) {
-                if (A.inv_scale) sa = A.inv_scale[m];
-                if (EPI == EPI_DIFF2) {
-                    bidx = (int)(m / ep.T);
-                    cb = ep.Cb[bidx];
-                }
-            }
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const long long n = n0 + wn * 64 + j * 16 + col_in;
-                if (mvalid && n < ep.N) {
-                    float v = acc[i][j][r] * sa;
-                    if (B.inv_scale) v *= B.inv_scale[n];
-                    if (EPI == EPI_STORE) {
-                        ep.C[m * ep.ldc + n] = v;
-                    } else {
-                        float d = (float)(cb + (double)ep.N1[(long long)bidx * ep.ldn + n] - (double)v);
-                        ep.C[m * ep.ldc + n] = d;
-                        if (ep.prior_o[n] > 0.) rmin = fminf(rmin, d);
-                    }
-                }
-            }
-            if (EPI == EPI_DIFF2) {
-                for (int o = 1; o < 16; o <<= 1) rmin = fminf(rmin, __shfl_xor(rmin, o));
-                if (col_in == 0 && mvalid) ep.rowmin[m * ep.rowmin_ld + (n0 + wn * 64) / 64] = rmin;
-            }
+            for (int j = 0; j < 4; j++) acc[i][j] = mfma16x16x32<H>(ah[i], bh[j], acc[i][j]);
         }
     }
+
+    gemm_epilogue<4, EPI>(acc, m0 + wm * 64, n0 + wn * 64, lane, A.inv_scale, B.inv_scale, ep);
+}
+
+// Wide variant: 256x256 tile per workgroup of 8 waves, each wave 128x64 (8x4 MFMA tiles), so
+// each wave issues 24 LDS fragment reads per 96 MFMAs (the 128x128 kernel: 16 per 48) and each
+// operand byte fetched from L2 feeds twice the flops. LDS rows are 32 halfs (64 B, 4 chunks of
+// 16 B) with chunk c of row r stored at c ^ ((r >> 1) & 3): conflict-free 16-byte reads for the
+// MFMA fragment pattern (16 rows x 4 chunks per wave) and for the row-major tile stores.
+#define WBM 256
+#define WBN 256
+
+__device__ __forceinline__ int swz(int r, int c) { return r * 32 + ((c ^ ((r >> 1) & 3)) << 3); }
+
+template <class H, int EPI>
+__global__ __launch_bounds__(512) void k_gemm_x3w(SplitOperand A, SplitOperand B, int K, int tiles_m,
+                                                  int tiles_n, int group_m, EpiParams ep)
+{
+    typedef typename Vec8<H>::type v8;
+    // two LDS stages (2 x 64 KiB): the tile slices for step k+1 are stored while step k's MFMAs
+    // run, so each K step needs a single barrier
+    __shared__ __attribute__((aligned(16))) H lds[2][4][WBM * 32];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave & 1, wn = wave >> 1;   // 2 waves along M (128 rows), 4 along N (64 cols)
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, group_m, tm, tn);
+    const long long m0 = (long long)tm * WBM, n0 = (long long)tn * WBN;
+
+    floatx4 acc[8][4];
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    // loader: rows lrow and lrow + 128, 16-byte chunk lc of the 64-byte K slice
+    const int lrow = tid >> 2, lc = tid & 3;
+    const H* Ah = reinterpret_cast<const H*>(A.hi);
+    const H* Al = reinterpret_cast<const H*>(A.lo);
+    const H* Bh = reinterpret_cast<const H*>(B.hi);
+    const H* Bl = reinterpret_cast<const H*>(B.lo);
+    const size_t ra0 = (size_t)(m0 + lrow) * A.ld + lc * 8, ra1 = (size_t)(m0 + lrow + 128) * A.ld + lc * 8;
+    const size_t rb0 = (size_t)(n0 + lrow) * B.ld + lc * 8, rb1 = (size_t)(n0 + lrow + 128) * B.ld + lc * 8;
+    const int s0 = swz(lrow, lc), s1 = swz(lrow + 128, lc);
+    uint4 p0, p1, p2, p3, p4, p5, p6, p7;
+#define X3W_GLOAD(k)                                               \
+    do {                                                           \
+        p0 = *reinterpret_cast<const uint4*>(Ah + ra0 + (k));      \
+        p1 = *reinterpret_cast<const uint4*>(Ah + ra1 + (k));      \
+        p2 = *reinterpret_cast<const uint4*>(Al + ra0 + (k));      \
+        p3 = *reinterpret_cast<const uint4*>(Al + ra1 + (k));      \
+        p4 = *reinterpret_cast<const uint4*>(Bh + rb0 + (k));      \
+        p5 = *reinterpret_cast<const uint4*>(Bh + rb1 + (k));      \
+        p6 = *reinterpret_cast<const uint4*>(Bl + rb0 + (k));      \
+        p7 = *reinterpret_cast<const uint4*>(Bl + rb1 + (k));      \
+    } while (0)
+#define X3W_LSTORE(b)                                              \
+    do {                                                           \
+        *reinterpret_cast<uint4*>(&lds[b][0][s0]) = p0;            \
+        *reinterpret_cast<uint4*>(&lds[b][0][s1]) = p1;            \
+        *reinterpret_cast<uint4*>(&lds[b][1][s0]) = p2;            \
+        *reinterpret_cast<uint4*>(&lds[b][1][s1]) = p3;            \
+        *reinterpret_cast<uint4*>(&lds[b][2][s0]) = p4;            \
+        *reinterpret_cast<uint4*>(&lds[b][2][s1]) = p5;            \
+        *reinterpret_cast<uint4*>(&lds[b][3][s0]) = p6;            \
+        *reinterpret_cast<uint4*>(&lds[b][3][s1]) = p7;            \
+    } while (0)
+    X3W_GLOAD(0);
+    X3W_LSTORE(0);
+    __syncthreads();
+
+    const int fr = lane & 15, fc = lane >> 4;   // fragment row within 16, 16-byte chunk 0..3
+    int buf = 0;
+    for (int k0 = 0; k0 < K; k0 += XK) {
+        const bool more = k0 + XK < K;
+        if (more) X3W_GLOAD(k0 + XK);
+        const H* Ash = lds[buf][0];
+        const H* Asl = lds[buf][1];
+        const H* Bsh = lds[buf][2];
+        const H* Bsl = lds[buf][3];
+        v8 bh[4], bl[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int o = swz(wn * 64 + j * 16 + fr, fc);
+            bh[j] = *reinterpret_cast<const v8*>(&Bsh[o]);
+            bl[j] = *reinterpret_cast<const v8*>(&Bsl[o]);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int o = swz(wm * 128 + i * 16 + fr, fc);
+            const v8 ah = *reinterpret_cast<const v8*>(&Ash[o]);
+            const v8 al = *reinterpret_cast<const v8*>(&Asl[o]);
+            // consecutive MFMAs go to 4 different accumulators; each element still sums
+            // lo.hi, hi.lo, hi.hi in that order
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = mfma16x16x32<H>(al, bh[j], acc[i][j]);
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = mfma16x16x32<H>(ah, bl[j], acc[i][j]);
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = mfma16x16x32<H>(ah, bh[j], acc[i][j]);
+        }
+        if (more) X3W_LSTORE(buf ^ 1);
+        __syncthreads();
+        buf ^= 1;
+    }
+
+#undef X3W_GLOAD
+#undef X3W_LSTORE
+    gemm_epilogue<8, EPI>(acc, m0 + wm * 128, n0 + wn * 64, lane, A.inv_scale, B.inv_scale, ep);
+}
+
+// Direct-load variant: 256x128 tile per workgroup of 4 waves (each 128x64), operand slices
+// moved from global memory straight into LDS with global_load_lds_dwordx4 (no register
+// staging), single LDS stage of 48 KiB, so two workgroups share a CU and each one's load and
+// barrier phases overlap the other's MFMAs. One wave instruction fills 1 KiB of LDS (16 rows x
+// 4 chunks); lane q writes physical chunk q & 3 of row q >> 2 and therefore fetches logical
+// chunk (q & 3) ^ ((row >> 1) & 3), which yields the same swizzled layout as swz().
+#define DBM 256
+#define DBN 128
+
+typedef const void __attribute__((address_space(1)))* gptr_t;
+typedef void __attribute__((address_space(3)))* lptr_t;
+
+template <class H>
+__device__ __forceinline__ void load_rows_lds(const H* __restrict__ g, long long ld, long long row0, int k0,
+                                              H* lds_rows, int lane)
+{
+    // 16 rows starting at row0 (global) -> lds_rows (16 x 32 halfs, swizzled)
+    const int r = lane >> 2, pc = lane & 3;
+    const int c = pc ^ ((r >> 1) & 3);
+    const H* src = g + (size_t)(row0 + r) * ld + k0 + c * 8;
+    __builtin_amdgcn_global_load_lds((gptr_t)src, (lptr_t)lds_rows, 16, 0, 0);
+}
+
+template <class H, int EPI>
+__global__ __launch_bounds__(256, 2) void k_gemm_x3d(SplitOperand A, SplitOperand B, int K, int tiles_m,
+                                                     int tiles_n, int group_m, EpiParams ep)
+{
+    typedef typename Vec8<H>::type v8;
+    __shared__ __attribute__((aligned(16))) H Ash[DBM * 32];
+    __shared__ __attribute__((aligned(16))) H Asl[DBM * 32];
+    __shared__ __attribute__((aligned(16))) H Bsh[DBN * 32];
+    __shared__ __attribute__((aligned(16))) H Bsl[DBN * 32];
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave & 1, wn = wave >> 1;   // 2 waves along M (128 rows), 2 along N (64 cols)
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, group_m, tm, tn);
+    const long long m0 = (long long)tm * DBM, n0 = (long long)tn * DBN;
+    const H* Ah = reinterpret_cast<const H*>(A.hi);
+    const H* Al = reinterpret_cast<const H*>(A.lo);
+    const H* Bh = reinterpret_cast<const H*>(B.hi);
+    const H* Bl = reinterpret_cast<const H*>(B.lo);
+
+    floatx4 acc[8][4];
+#pragma unroll
+    for (int i = 0; i < 8; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    const int fr = lane & 15, fc = lane >> 4;
+    for (int k0 = 0; k0 < K; k0 += XK) {
+        // each wave moves 4 A row blocks (hi and lo) and 2 B row blocks (hi and lo) of 16 rows
+#pragma unroll
+        for (int q = 0; q < 4; q++) {
+            const int rb = (wave * 4 + q) * 16;
+            load_rows_lds<H>(Ah, A.ld, m0 + rb, k0, &Ash[rb * 32], lane);
+            load_rows_lds<H>(Al, A.ld, m0 + rb, k0, &Asl[rb * 32], lane);
+        }
+#pragma unroll
+        for (int q = 0; q < 2; q++) {
+            const int rb = (wave * 2 + q) * 16;
+            load_rows_lds<H>(Bh, B.ld, n0 + rb, k0, &Bsh[rb * 32], lane);
+            load_rows_lds<H>(Bl, B.ld, n0 + rb, k0, &Bsl[rb * 32], lane);
+        }
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        v8 bh[4], bl[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int o = swz(wn * 64 + j * 16 + fr, fc);
+            bh[j] = *reinterpret_cast<const v8*>(&Bsh[o]);
+            bl[j] = *reinterpret_cast<const v8*>(&Bsl[o]);
+        }
+#pragma unroll
+        for (int i = 0; i < 8; i++) {
+            const int o = swz(wm * 128 + i * 16 + fr, fc);
+            const v8 ah = *reinterpret_cast<const v8*>(&Ash[o]);
+            const v8 al = *reinterpret_cast<const v8*>(&Asl[o]);
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = mfma16x16x32<H>(al, bh[j], acc[i][j]);
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = mfma16x16x32<H>(ah, bl[j], acc[i][j]);
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = mfma16x16x32<H>(ah, bh[j], acc[i][j]);
+        }
+        __syncthreads();
+    }
+
+    gemm_epilogue<8, EPI>(acc, m0 + wm * 128, n0 + wn * 64, lane, A.inv_scale, B.inv_scale, ep);
 }
 
 // hi/lo split of a row-major float matrix (rows x K), optionally with a power-of-two scale per
@@ -379,24 +578,55 @@ int launch_split(int prec, const float* X, long long ldx, int K, int rows, int v
     return T2X_OK;
 }
 
+// Tile variant of the split-precision GEMM: 128 (k_gemm_x3), 256 (k_gemm_x3w, 256x256) or 257
+// (k_gemm_x3d, 256x128 with direct-to-LDS loads); 0 = default.
+int gemm_x3_tile(int requested)
+{
+    if (requested == 128 || requested == 256 || requested == 257) return requested;
+    const char* e = getenv("T2X_GEMM_TILE");
+    const int v = e ? atoi(e) : 0;
+    return (v == 128 || v == 256 || v == 257) ? v : 256;
+}
+
 int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const float* As, long long lda,
                    const void* Bh, const void* Bl, const float* Bs, long long ldb, int K, int Mrows_pad,
-                   int Nrows_pad, const EpiParams& ep, hipStream_t st)
+                   int Nrows_pad, const EpiParams& ep, hipStream_t st, int tile)
 {
-    if (K % XK != 0 || Mrows_pad % GBM != 0 || Nrows_pad % GBN != 0) {
+    tile = gemm_x3_tile(tile);
+    const int tile_m = tile == 257 ? DBM : tile, tile_n = tile == 257 ? DBN : tile;
+    if (K % XK != 0 || Mrows_pad % tile_m != 0 || Nrows_pad % tile_n != 0) {
         set_error("gemm_x3: operands not padded to tile multiples");
         return T2X_EINVAL;
     }
-    const int tm = Mrows_pad / GBM, tn = Nrows_pad / GBN;
+    const int tm = Mrows_pad / tile_m, tn = Nrows_pad / tile_n;
     const long long nwg = (long long)tm * tn;
     if (nwg > 0x7fffffffLL) { set_error("gemm: grid too large"); return T2X_EINVAL; }
     SplitOperand A{Ah, Al, As, lda}, B{Bh, Bl, Bs, ldb};
-    if (prec == T2X_PREC_BF16X3) {
-        if (epi == EPI_STORE) k_gemm_x3<__bf16, EPI_STORE><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, ep);
-        else k_gemm_x3<__bf16, EPI_DIFF2><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, ep);
+    const int g = gemm_group_m();
+    if (tile == 257) {
+        const int gw = std::max(1, g / 2);
+        if (prec == T2X_PREC_BF16X3) {
+            if (epi == EPI_STORE) k_gemm_x3d<__bf16, EPI_STORE><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, gw, ep);
+            else k_gemm_x3d<__bf16, EPI_DIFF2><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, gw, ep);
+        } else {
+            if (epi == EPI_STORE) k_gemm_x3d<_Float16, EPI_STORE><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, gw, ep);
+            else k_gemm_x3d<_Float16, EPI_DIFF2><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, gw, ep);
+        }
+    } else if (tile == 256) {
+        const int gw = std::max(1, g / 2);
+        if (prec == T2X_PREC_BF16X3) {
+            if (epi == EPI_STORE) k_gemm_x3w<__bf16, EPI_STORE><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep);
+            else k_gemm_x3w<__bf16, EPI_DIFF2><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep);
+        } else {
+            if (epi == EPI_STORE) k_gemm_x3w<_Float16, EPI_STORE><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep);
+            else k_gemm_x3w<_Float16, EPI_DIFF2><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep);
+        }
+    } else if (prec == T2X_PREC_BF16X3) {
+        if (epi == EPI_STORE) k_gemm_x3<__bf16, EPI_STORE><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, g, ep);
+        else k_gemm_x3<__bf16, EPI_DIFF2><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, g, ep);
     } else {
-        if (epi == EPI_STORE) k_gemm_x3<_Float16, EPI_STORE><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, ep);
-        else k_gemm_x3<_Float16, EPI_DIFF2><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, ep);
+        if (epi == EPI_STORE) k_gemm_x3<_Float16, EPI_STORE><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, g, ep);
+        else k_gemm_x3<_Float16, EPI_DIFF2><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, g, ep);
     }
     T2X_HIP(hipGetLastError());
     return T2X_OK;

@@ -537,82 +537,133 @@ namespace t2x {
 
 #define CAND_CAP 4096
 
-__global__ __launch_bounds__(1024) void k_coarse_sum(WeightSrc s, int B, SelResult* res)
+// Weights with diff2 - min > kTinyDD are below prior * e^-60 and are neither summed nor
+// exponentiated: their total is at most e^-60 * (sum of all priors), below FP64 rounding of S
+// unless the prior at the minimum is itself below ~1e-8 of the prior mass.
+#define kTinyDD 60.0
+
+struct SumPart {
+    double sum, max;
+    long long npos, nbound;
+};
+
+size_t sum_part_bytes() { return sizeof(SumPart); }
+
+// One workgroup per (translation row, particle): partial sum, max and counts of the row.
+__global__ __launch_bounds__(256) void k_coarse_sum_rows(WeightSrc s, int B, const SelResult* res, SumPart* part)
 {
-    const int b = blockIdx.x;
-    if (b >= B) return;
+    const int t = blockIdx.x, b = blockIdx.y;
+    if (b >= B || t >= s.T) return;
     const double mind2 = res[b].min_d2;
+    const double pt = s.prior_t[(size_t)b * s.T + t];
+    const float* row = s.d2 + ((size_t)b * s.T + t) * s.ld;
     double sum = 0., mx = 0.;
-    unsigned long long npos = 0;
-    for (int t = 0; t < s.T; t++) {
-        const double pt = s.prior_t[(size_t)b * s.T + t];
-        const float* row = s.d2 + ((size_t)b * s.T + t) * s.ld;
-        for (int n = threadIdx.x; n < s.O_all; n += blockDim.x) {
-            double prior = s.prior_o[n] * pt;
-            if (!(prior > 0.)) continue;
-            double dd = (double)row[n] - mind2;
-            if (dd > 700.) continue;
-            double w = prior * exp(-dd);
-            if (!(w > 0.)) continue;
-            sum += w;
-            npos++;
-            mx = fmax(mx, w);
+    long long npos = 0, nbound = 0;
+    if (pt > 0.) {
+        for (int n4 = threadIdx.x * 4; n4 < s.O_all; n4 += blockDim.x * 4) {
+            const float4 v = *reinterpret_cast<const float4*>(row + n4);   // rows padded to 256
+            const float dv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+            for (int c = 0; c < 4; c++) {
+                const int n = n4 + c;
+                if (n >= s.O_all) break;
+                const double po = s.prior_o[n];
+                if (!(po > 0.)) continue;
+                const double dd = (double)dv[c] - mind2;
+                if (dd > 700.) continue;
+                npos++;
+                if (dd > kTinyDD) continue;
+                const double w = (po * pt) * exp(-dd);
+                if (!(w > 0.)) continue;
+                nbound++;
+                sum += w;
+                mx = fmax(mx, w);
+            }
         }
     }
     sum = wave_sum(sum);
-    npos = wave_sum_u64(npos);
+    npos = (long long)wave_sum_u64((unsigned long long)npos);
+    nbound = (long long)wave_sum_u64((unsigned long long)nbound);
     for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
-    __shared__ double rs[16], rm[16];
-    __shared__ unsigned long long rn[16];
-    if ((threadIdx.x & 63) == 0) { rs[threadIdx.x >> 6] = sum; rm[threadIdx.x >> 6] = mx; rn[threadIdx.x >> 6] = npos; }
+    __shared__ SumPart red[4];
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = SumPart{sum, mx, npos, nbound};
     __syncthreads();
     if (threadIdx.x == 0) {
-        double a = 0., m = 0.;
-        unsigned long long c = 0;
-        for (int i = 0; i < (int)(blockDim.x >> 6); i++) { a += rs[i]; m = fmax(m, rm[i]); c += rn[i]; }
-        res[b].sum_w = a;
-        res[b].max_w = m;
-        res[b].npos = (long long)c;
+        SumPart r{0., 0., 0, 0};
+        for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
+            r.sum += red[i].sum;
+            r.max = fmax(r.max, red[i].max);
+            r.npos += red[i].npos;
+            r.nbound += red[i].nbound;
+        }
+        part[(size_t)b * s.T + t] = r;
     }
 }
 
-__global__ __launch_bounds__(1024) void k_coarse_candidates(WeightSrc s, int B, SelParams prm,
-                                                            const SelResult* res, double* cand_w,
-                                                            int* cand_e, int* cand_count)
+// Fixed-order combination of the row partials (deterministic S). npos counts the weights that
+// are positive; nbound (kept in nsig until the walk overwrites it) the non-negligible ones.
+__global__ void k_coarse_sum_final(WeightSrc s, int B, const SumPart* part, SelResult* res)
 {
     const int b = blockIdx.x;
-    if (b >= B) return;
-    const double mind2 = res[b].min_d2;
-    const long long npos = res[b].npos;
-    const double thr = (npos > 0) ? 0.5 * (1.0 - prm.fraction) * res[b].sum_w / (double)npos : 0.;
-    __shared__ int cnt;
-    if (threadIdx.x == 0) cnt = 0;
-    __syncthreads();
-    const int lane = threadIdx.x & 63;
+    if (b >= B || threadIdx.x != 0) return;
+    SumPart r{0., 0., 0, 0};
     for (int t = 0; t < s.T; t++) {
-        const double pt = s.prior_t[(size_t)b * s.T + t];
-        const float* row = s.d2 + ((size_t)b * s.T + t) * s.ld;
-        for (int base = 0; base < s.O_all; base += blockDim.x) {
-            const int n = base + threadIdx.x;
+        const SumPart q = part[(size_t)b * s.T + t];
+        r.sum += q.sum;
+        r.max = fmax(r.max, q.max);
+        r.npos += q.npos;
+        r.nbound += q.nbound;
+    }
+    res[b].sum_w = r.sum;
+    res[b].max_w = r.max;
+    res[b].npos = r.npos;
+    res[b].nsig = r.nbound;
+}
+
+// Candidates of the exact walk: weights >= 0.5 (1-f) S / nbound. Every weight the walk can stop
+// at satisfies u_k * nbound >= (1-f) S - (negligible mass) (see above), so none is lost.
+// One workgroup per (translation row, particle); slots are reserved per wave with one atomic.
+__global__ __launch_bounds__(256) void k_coarse_candidates(WeightSrc s, int B, SelParams prm,
+                                                           const SelResult* res, double* cand_w,
+                                                           int* cand_e, int* cand_count)
+{
+    const int t = blockIdx.x, b = blockIdx.y;
+    if (b >= B || t >= s.T) return;
+    const double mind2 = res[b].min_d2;
+    const long long nb = res[b].nsig;
+    const double thr = (nb > 0) ? 0.5 * (1.0 - prm.fraction) * res[b].sum_w / (double)nb : 0.;
+    const double pt = s.prior_t[(size_t)b * s.T + t];
+    const float* row = s.d2 + ((size_t)b * s.T + t) * s.ld;
+    const int lane = threadIdx.x & 63;
+    if (!(pt > 0.)) return;
+    for (int base = 0; base < s.O_all; base += blockDim.x * 4) {
+        const int n4 = base + threadIdx.x * 4;
+        float dv[4] = {3.0e38f, 3.0e38f, 3.0e38f, 3.0e38f};
+        if (n4 < s.O_all) {
+            const float4 v = *reinterpret_cast<const float4*>(row + n4);
+            dv[0] = v.x; dv[1] = v.y; dv[2] = v.z; dv[3] = v.w;
+        }
+#pragma unroll
+        for (int c = 0; c < 4; c++) {
+            const int n = n4 + c;
             bool keep = false;
             double w = 0.;
             if (n < s.O_all) {
-                double prior = s.prior_o[n] * pt;
-                if (prior > 0.) {
-                    double dd = (double)row[n] - mind2;
-                    if (dd <= 700.) {
-                        w = prior * exp(-dd);
-                        keep = (w > 0.) && (w >= thr);
-                    }
+                const double po = s.prior_o[n];
+                const double dd = (double)dv[c] - mind2;
+                if (po > 0. && dd <= kTinyDD) {
+                    w = (po * pt) * exp(-dd);
+                    keep = (w > 0.) && (w >= thr);
                 }
             }
-            unsigned long long m = __ballot(keep);
+            const unsigned long long m = __ballot(keep);
             if (m) {
+                const int leader = __ffsll((long long)m) - 1;
                 int first = 0;
-                if (lane == __ffsll((long long)m) - 1) first = atomicAdd(&cnt, __popcll(m));
-                first = __shfl(first, __ffsll((long long)m) - 1);
+                if (lane == leader) first = atomicAdd(&cand_count[b], __popcll(m));
+                first = __shfl(first, leader);
                 if (keep) {
-                    int slot = first + __popcll(m & ((1ull << lane) - 1));
+                    const int slot = first + __popcll(m & ((1ull << lane) - 1));
                     if (slot < CAND_CAP) {
                         cand_w[(size_t)b * CAND_CAP + slot] = w;
                         cand_e[(size_t)b * CAND_CAP + slot] = t * s.O_all + n;
@@ -621,8 +672,6 @@ __global__ __launch_bounds__(1024) void k_coarse_candidates(WeightSrc s, int B, 
             }
         }
     }
-    __syncthreads();
-    if (threadIdx.x == 0) cand_count[b] = cnt;
 }
 
 // Bitonic sort in LDS of n <= CAND_CAP (key, payload) pairs; desc on doubles or asc on ints.
@@ -639,8 +688,11 @@ __device__ void sort_pairs_desc(double* w, int* e, int n)
                 if (ixj > i) {
                     bool desc = ((i & k) == 0);
                     double x = w[i], y = w[ixj];
-                    bool sw = desc ? (x < y) : (x > y);
-                    if (sw) { w[i] = y; w[ixj] = x; int t = e[i]; e[i] = e[ixj]; e[ixj] = t; }
+                    int ex = e[i], ey = e[ixj];
+                    // order: weight descending, then sample index ascending (deterministic ties)
+                    bool x_after_y = (x < y) || (x == y && ex > ey);
+                    bool sw = desc ? x_after_y : !x_after_y && !(x == y && ex == ey);
+                    if (sw) { w[i] = y; w[ixj] = x; e[i] = ey; e[ixj] = ex; }
                 }
             }
             __syncthreads();

@@ -182,6 +182,7 @@ def load(path=None):
         "t2x_memcpy_h2d": (C.c_int, [vp, vp, vp, C.c_size_t]),
         "t2x_memcpy_d2h": (C.c_int, [vp, vp, vp, C.c_size_t]),
         "t2x_device_synchronize": (C.c_int, [vp]),
+        "t2x_device_copy": (C.c_int, [vp, vp, vp, C.c_size_t]),
         "t2x_debug_copy": (C.c_int, [vp, C.c_int, C.c_size_t, C.c_size_t, C.POINTER(C.c_float)]),
         "t2x_synth_reference_device": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int,
                                                  C.c_double, C.POINTER(Blob), C.c_int, C.c_double,
