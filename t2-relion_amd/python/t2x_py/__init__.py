@@ -111,6 +111,21 @@ class Stats(C.Structure):
         [(n, C.c_int) for n in ("coarse_size", "current_size", "P_coarse", "P_fine")]
 
 
+class ReconParams(C.Structure):
+    _fields_ = [("ori_size", C.c_int), ("pad_size", C.c_int), ("r_max", C.c_int),
+                ("padding_factor", C.c_double), ("ref_dim", C.c_int), ("data_dim", C.c_int),
+                ("do_map", C.c_int), ("tau2_fudge", C.c_double), ("minres_map", C.c_int),
+                ("skip_gridding", C.c_int), ("reserved", C.c_int * 6)]
+
+
+def recon_params(ori_size, r_max, padding_factor=2.0, do_map=False, tau2_fudge=1.0, minres_map=0):
+    """Parameters of BackProjector::reconstruct for a 3D reference from 2D images."""
+    pad = 2 * (int(np.floor(padding_factor * r_max + 0.5)) + 1) + 1
+    return ReconParams(ori_size=ori_size, pad_size=pad, r_max=r_max, padding_factor=padding_factor,
+                       ref_dim=3, data_dim=2, do_map=int(do_map), tau2_fudge=tau2_fudge,
+                       minres_map=minres_map, skip_gridding=1)
+
+
 class Blob(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("x", "y", "z", "sigma", "amplitude")]
 
@@ -184,6 +199,8 @@ def load(path=None):
         "t2x_device_synchronize": (C.c_int, [vp]),
         "t2x_device_copy": (C.c_int, [vp, vp, vp, C.c_size_t]),
         "t2x_debug_copy": (C.c_int, [vp, C.c_int, C.c_size_t, C.c_size_t, C.POINTER(C.c_float)]),
+        "t2x_reconstruct": (C.c_int, [vp, C.POINTER(ReconParams), PD, PD, PD, PD, PD]),
+        "t2x_fsc": (C.c_int, [vp, C.c_int, PD, PD, PD]),
         "t2x_synth_reference_device": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int,
                                                  C.c_double, C.POINTER(Blob), C.c_int, C.c_double,
                                                  vp, vp]),
