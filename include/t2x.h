@@ -89,7 +89,11 @@ typedef struct t2x_estep_config {
     double sigma2_offset;          /* mymodel.sigma2_offset (A^2)                        */
     int    orientational_prior_mode; /* T2X_NOPRIOR only in this version                 */
     int    precision;              /* T2X_PREC_* for the cross terms                     */
-    int    reserved[8];
+    int    proposal_topk;          /* > 0: coarse pass scores only the top-k bins of the */
+                                   /* trained proposal network (t2x_proposal_train)      */
+    int    proposal_reserved;
+    double proposal_min_conf;      /* max softmax below this: exhaustive coarse search   */
+    int    reserved[4];
 } t2x_estep_config;
 
 /* Model arrays of the current iteration (host, caller-owned, read-only). */
@@ -179,6 +183,10 @@ typedef struct t2x_stats {
     double bp_orientations;      /* orientations backprojected                                */
     double sig_coarse;           /* significant coarse samples                                */
     int    coarse_size, current_size, P_coarse, P_fine;
+    double ms_proposal;          /* network inference + restricted work lists (learned path)  */
+    long   n_proposal_particles; /* particles that went through the proposal network         */
+    long   n_proposal_fallback;  /* of those, searched exhaustively (low confidence)          */
+    double proposal_bins_scored; /* sum over particles of the bins scored in the coarse pass  */
 } t2x_stats;
 
 /* ---------------------------------------------------------------------------------------
@@ -286,6 +294,56 @@ int t2x_synth_particles_device(t2x_device* dev, int ori_size, double pixel_size,
                                const double* true_x, const double* true_y, const int* true_class,
                                const double* snr, uint64_t noise_seed, float* dimages,
                                double* noise_var /* [n] host, may be NULL */);
+
+/* ---------------------------------------------------------------------------------------
+ * Learned pose proposal. No counterpart in the reference (it has no learned component): a small
+ * CNN (64x64 input, four 3x3 stride-2 convolutions of 32/64/128/128 channels, linear head) maps a
+ * particle to a distribution over bins = classes x HEALPix directions one order below the coarse
+ * search, plus a 2-D shift regression. It is trained on the GPU on noisy CTF-modulated projections
+ * of the current references (the device projectors), with the model's noise spectrum; with
+ * cfg->proposal_topk > 0 the E-step's coarse pass scores only the top-k bins of each particle and
+ * searches exhaustively when the top softmax probability is below cfg->proposal_min_conf.
+ * ------------------------------------------------------------------------------------- */
+typedef struct t2x_proposal_train_params {
+    int      steps;            /* optimiser steps (2000)                                     */
+    int      batch;            /* samples per step (256)                                      */
+    int      n_projections;    /* distinct projection parameter sets, cycled (200000)        */
+    int      bf16;             /* 1: BF16 operands on MFMA, FP32 accumulate; 0: FP32 operands */
+    double   lr;               /* Adam learning rate                                         */
+    double   shift_weight;     /* weight of the shift regression term                        */
+    double   max_shift_px;     /* shifts uniform in +-max_shift_px; targets are shift/max    */
+    double   defocus_min, defocus_max;   /* A, defocus U and V uniform in this range          */
+    double   voltage, cs, q0;  /* kV, mm, amplitude contrast of the training CTFs            */
+    uint64_t seed;
+    int      reserved[6];
+} t2x_proposal_train_params;
+
+typedef struct t2x_proposal_train_stats {
+    double ms_total;           /* wall time of t2x_proposal_train                            */
+    double ms_setup;           /* parameter sets, labels, uploads                            */
+    double loss_first, loss_last;   /* mean loss over the first / last 20 steps             */
+    double top1_last;          /* training top-1 bin accuracy over the last 20 steps         */
+    int    steps;
+    int    n_bins;
+    int    reserved[6];
+} t2x_proposal_train_stats;
+
+/* (Re)initialise the network (He-uniform, seeded) for n_bins outputs. */
+int t2x_proposal_init(t2x_device* dev, int n_bins, uint64_t seed);
+int t2x_proposal_num_params(t2x_device* dev, long long* n);
+int t2x_proposal_get_params(t2x_device* dev, float* host, long long n);
+int t2x_proposal_set_params(t2x_device* dev, const float* host, long long n);
+/* One forward + backward pass on host data (tests): x [B][64][64], label [B], shift_target
+ * [B][2]; writes the mean loss, the gradient (n_params floats, optional) and the outputs
+ * [B][n_bins+2] (optional). bf16 selects the operand precision of the layer GEMMs. */
+int t2x_proposal_loss_grad(t2x_device* dev, int B, const float* x, const int* label,
+                           const float* shift_target, double shift_weight, int bf16, double* loss,
+                           float* grad, float* outputs);
+/* Train on the GPU from the projectors of classes 0..K-1 set on this handle; cfg gives the box,
+ * pixel size, classes and the coarse HEALPix order (bins are one order below it); model gives
+ * the noise spectrum. */
+int t2x_proposal_train(t2x_device* dev, const t2x_estep_config* cfg, const t2x_model* model,
+                       const t2x_proposal_train_params* tp, t2x_proposal_train_stats* out);
 
 /* ---------------------------------------------------------------------------------------
  * Reconstruction (M-step side of the backprojector), all arithmetic in FP64 on the GPU.

@@ -1,0 +1,530 @@
+// t2x_cnn.hip — the pose-proposal network: forward, backward and Adam on the GPU.
+//
+// Network: 64x64 input, four 3x3 stride-2 pad-1 convolutions (32/64/128/128 channels, ReLU),
+// flatten (y, x, c), linear 2048 -> n_bins + 2 (bin logits over classes x HEALPix order-2
+// directions, and a 2-D shift regression in units of the maximum shift). Activations are
+// channels-last [B][H][W][C] in FP32; weights [Cout][ky][kx][Cin] (FP32 master copy, Adam state).
+//
+// Every layer pass is one GEMM C[m][n] = sum_k A(m,k) B(n,k) of a single kernel template whose
+// operands are either strided views of FP32 arrays or implicit im2col views of an activation
+// tensor (no im2col buffer for the forward pass or the weight gradients). Operands are rounded
+// to BF16 in LDS and multiplied on v_mfma_f32_16x16x32_bf16 with FP32 accumulation (BF16
+// weights, FP32 accumulate); an FP32 operand mode (v_mfma_f32_16x16x4f32) exists for gradient
+// checks against the float64 oracle. Epilogues fuse bias and ReLU.
+#include "t2x_cnn.h"
+
+#include <cmath>
+#include <vector>
+
+namespace t2x {
+namespace cnn {
+
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+enum { OP_STRIDED = 0, OP_IM2COL = 1, OP_IM2COL_T = 2 };
+enum { EPI_STORE = 0, EPI_BIAS = 1, EPI_BIAS_RELU = 2 };
+
+// An operand view X(r, k), r the output row (A: m, B: n), k the reduction index.
+struct Operand {
+    int mode;
+    const float* p;
+    long long sr, sk;                 // OP_STRIDED: p[r*sr + k*sk]
+    int H, W, C, Ho, Wo;              // im2col: activation [Bn][H][W][C], output grid Ho x Wo
+};
+
+struct GemmArgs {
+    int M, N, K;
+    Operand a, b;
+    float* c;
+    long long ldc;
+    const float* bias;
+};
+
+// im2col element: row index p = (b, y, x) over the output grid, column q = (ky, kx, ci)
+__device__ __forceinline__ float im2col_at(const Operand& o, long long p, long long q)
+{
+    const int ci = (int)(q % o.C);
+    const int kk = (int)(q / o.C);
+    const int kx = kk % 3, ky = kk / 3;
+    const long long x = p % o.Wo, y = (p / o.Wo) % o.Ho, b = p / ((long long)o.Wo * o.Ho);
+    const long long yy = 2 * y + ky - 1, xx = 2 * x + kx - 1;
+    if (yy < 0 || yy >= o.H || xx < 0 || xx >= o.W) return 0.f;
+    return o.p[((b * o.H + yy) * o.W + xx) * o.C + ci];
+}
+
+__device__ __forceinline__ float op_at(const Operand& o, long long r, long long k)
+{
+    if (o.mode == OP_STRIDED) return o.p[r * o.sr + k * o.sk];
+    if (o.mode == OP_IM2COL) return im2col_at(o, r, k);
+    return im2col_at(o, k, r);   // OP_IM2COL_T: r is the im2col column, k the pixel
+}
+
+#define CT 64
+#define CK 32
+#define CLD 36
+
+template <bool BF16, int EPI>
+__global__ __launch_bounds__(256) void k_cnn_gemm(GemmArgs g)
+{
+    __shared__ __attribute__((aligned(16))) float As[CT][CLD];
+    __shared__ __attribute__((aligned(16))) float Bs[CT][CLD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const long long m0 = (long long)blockIdx.x * CT, n0 = (long long)blockIdx.y * CT;
+    floatx4 acc[2][2];
+    for (int i = 0; i < 2; i++)
+        for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int lr = tid >> 2, lk = (tid & 3) * 8;
+    for (int k0 = 0; k0 < g.K; k0 += CK) {
+        __syncthreads();
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const long long k = k0 + lk + q;
+            const long long m = m0 + lr, n = n0 + lr;
+            As[lr][lk + q] = (m < g.M && k < g.K) ? op_at(g.a, m, k) : 0.f;
+            Bs[lr][lk + q] = (n < g.N && k < g.K) ? op_at(g.b, n, k) : 0.f;
+        }
+        __syncthreads();
+        if (BF16) {
+            const int kc = (lane >> 4) * 8;
+            bf16x8 a[2], b[2];
+#pragma unroll
+            for (int i = 0; i < 2; i++) {
+                const float* s = &As[wm * 32 + i * 16 + (lane & 15)][kc];
+#pragma unroll
+                for (int e = 0; e < 8; e++) a[i][e] = (__bf16)s[e];
+            }
+#pragma unroll
+            for (int j = 0; j < 2; j++) {
+                const float* s = &Bs[wn * 32 + j * 16 + (lane & 15)][kc];
+#pragma unroll
+                for (int e = 0; e < 8; e++) b[j][e] = (__bf16)s[e];
+            }
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        } else {
+#pragma unroll
+            for (int s = 0; s < CK / 4; s++) {
+                const int k = 4 * s + (lane >> 4);
+                float a[2], b[2];
+#pragma unroll
+                for (int i = 0; i < 2; i++) a[i] = As[wm * 32 + i * 16 + (lane & 15)][k];
+#pragma unroll
+                for (int j = 0; j < 2; j++) b[j] = Bs[wn * 32 + j * 16 + (lane & 15)][k];
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a[i], b[j], acc[i][j], 0, 0, 0);
+            }
+        }
+    }
+    const int col = lane & 15, rg = (lane >> 4) * 4;
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const long long m = m0 + wm * 32 + i * 16 + rg + r;
+                const long long n = n0 + wn * 32 + j * 16 + col;
+                if (m >= g.M || n >= g.N) continue;
+                float v = acc[i][j][r];
+                if (EPI != EPI_STORE) v += g.bias[n];
+                if (EPI == EPI_BIAS_RELU) v = fmaxf(v, 0.f);
+                g.c[m * g.ldc + n] = v;
+            }
+}
+
+// dz = da * (out > 0)
+__global__ void k_relu_bwd(const float* __restrict__ da, const float* __restrict__ out, float* __restrict__ dz,
+                           long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        dz[i] = out[i] > 0.f ? da[i] : 0.f;
+}
+
+// column sums of x [M][N] in a fixed order: partial[s][n] over row slabs, then the slabs
+#define CS_SLABS 64
+__global__ void k_colsum_partial(const float* __restrict__ x, long long M, int N, float* __restrict__ part)
+{
+    const int n = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int sub = threadIdx.x >> 6;   // 4 row phases
+    const int slab = blockIdx.y;
+    const long long per = (M + CS_SLABS - 1) / CS_SLABS;
+    const long long m_begin = slab * per, m_end = min(M, m_begin + per);
+    double s = 0.;
+    if (n < N)
+        for (long long m = m_begin + sub; m < m_end; m += 4) s += x[m * N + n];
+    __shared__ double red[4][64];
+    red[sub][threadIdx.x & 63] = s;
+    __syncthreads();
+    if (sub == 0 && n < N)
+        part[(size_t)slab * N + n] = (float)(red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] +
+                                             red[3][threadIdx.x]);
+}
+
+__global__ void k_colsum_final(const float* __restrict__ part, int N, float* __restrict__ out)
+{
+    const int n = blockIdx.x * blockDim.x + threadIdx.x;
+    if (n >= N) return;
+    double s = 0.;
+    for (int slab = 0; slab < CS_SLABS; slab++) s += part[(size_t)slab * N + n];
+    out[n] = (float)s;
+}
+
+// dact[b][yi][xi][ci] = sum over the (<= 4) output pixels whose 3x3 window covers (yi, xi)
+__global__ void k_col2im(const float* __restrict__ dcols, int Bn, int H, int W, int C, float* __restrict__ dact)
+{
+    const int Ho = H / 2, Wo = W / 2;
+    const long long n = (long long)Bn * H * W * C;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (long long)gridDim.x * blockDim.x) {
+        const int ci = (int)(e % C);
+        const long long pix = e / C;
+        const int xi = (int)(pix % W), yi = (int)((pix / W) % H);
+        const long long b = pix / ((long long)W * H);
+        float s = 0.f;
+        for (int ky = 0; ky < 3; ky++) {
+            const int t = yi + 1 - ky;
+            if (t < 0 || (t & 1)) continue;
+            const int y = t >> 1;
+            if (y >= Ho) continue;
+            for (int kx = 0; kx < 3; kx++) {
+                const int u = xi + 1 - kx;
+                if (u < 0 || (u & 1)) continue;
+                const int x = u >> 1;
+                if (x >= Wo) continue;
+                s += dcols[((b * Ho + y) * Wo + x) * (9LL * C) + (ky * 3 + kx) * C + ci];
+            }
+        }
+        dact[e] = s;
+    }
+}
+
+// softmax cross-entropy on the bins + shift_weight * squared shift error; dy = dloss/dy / B.
+// One workgroup per sample; loss per sample into loss_out (summed on the host in order).
+__global__ void k_cnn_loss(const float* __restrict__ y, int nbins, const int* __restrict__ label,
+                           const float* __restrict__ shift_t, float shift_weight, int Bn, float* __restrict__ dy,
+                           double* __restrict__ loss_out, int* __restrict__ correct)
+{
+    const int b = blockIdx.x;
+    const int ld = nbins + 2;
+    const float* yb = y + (size_t)b * ld;
+    __shared__ float red[256];
+    __shared__ int redi[256];
+    float mx = -3.0e38f;
+    int arg = 0;
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x)
+        if (yb[i] > mx) { mx = yb[i]; arg = i; }
+    red[threadIdx.x] = mx;
+    redi[threadIdx.x] = arg;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const float o = red[threadIdx.x + s];
+            const int oi = redi[threadIdx.x + s];
+            if (o > red[threadIdx.x] || (o == red[threadIdx.x] && oi < redi[threadIdx.x])) {
+                red[threadIdx.x] = o;
+                redi[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    const float m = red[0];
+    const int am = redi[0];
+    __syncthreads();
+    float se = 0.f;
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) se += expf(yb[i] - m);
+    red[threadIdx.x] = se;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    const float Z = red[0];
+    const int lab = label[b];
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
+        const float p = expf(yb[i] - m) / Z;
+        dy[(size_t)b * ld + i] = (p - (i == lab ? 1.f : 0.f)) / (float)Bn;
+    }
+    if (threadIdx.x < 2) {
+        const float d = yb[nbins + threadIdx.x] - shift_t[2 * b + threadIdx.x];
+        dy[(size_t)b * ld + nbins + threadIdx.x] = 2.f * shift_weight * d / (float)Bn;
+    }
+    if (threadIdx.x == 0) {
+        const double dx = (double)yb[nbins] - shift_t[2 * b], dyv = (double)yb[nbins + 1] - shift_t[2 * b + 1];
+        loss_out[b] = -((double)yb[lab] - m - log((double)Z)) + shift_weight * (dx * dx + dyv * dyv);
+        if (correct) correct[b] = (am == lab);
+    }
+}
+
+__global__ void k_adam(float* __restrict__ p, const float* __restrict__ g, float* __restrict__ m1,
+                       float* __restrict__ m2, long long n, float lr, float b1, float b2, float eps, float c1,
+                       float c2)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float a = b1 * m1[i] + (1.f - b1) * gi;
+        const float v = b2 * m2[i] + (1.f - b2) * gi * gi;
+        m1[i] = a;
+        m2[i] = v;
+        p[i] -= lr * (a / c1) / (sqrtf(v / c2) + eps);
+    }
+}
+
+// ---------------------------------------------------------------------------------------------
+
+static unsigned grid_for(long long n, int bs = 256)
+{
+    long long g = (n + bs - 1) / bs;
+    return (unsigned)std::min<long long>(std::max<long long>(g, 1), 65536);
+}
+
+int gemm(bool bf16, int epi, const GemmArgs& a, hipStream_t st)
+{
+    if (a.M <= 0 || a.N <= 0 || a.K <= 0) return T2X_OK;
+    const dim3 grid((unsigned)((a.M + CT - 1) / CT), (unsigned)((a.N + CT - 1) / CT));
+    if (grid.y > 65535) { set_error("cnn gemm: N too large"); return T2X_EINVAL; }
+    if (bf16) {
+        if (epi == EPI_STORE) k_cnn_gemm<true, EPI_STORE><<<grid, 256, 0, st>>>(a);
+        else if (epi == EPI_BIAS) k_cnn_gemm<true, EPI_BIAS><<<grid, 256, 0, st>>>(a);
+        else k_cnn_gemm<true, EPI_BIAS_RELU><<<grid, 256, 0, st>>>(a);
+    } else {
+        if (epi == EPI_STORE) k_cnn_gemm<false, EPI_STORE><<<grid, 256, 0, st>>>(a);
+        else if (epi == EPI_BIAS) k_cnn_gemm<false, EPI_BIAS><<<grid, 256, 0, st>>>(a);
+        else k_cnn_gemm<false, EPI_BIAS_RELU><<<grid, 256, 0, st>>>(a);
+    }
+    T2X_HIP(hipGetLastError());
+    return T2X_OK;
+}
+
+static Operand strided(const float* p, long long sr, long long sk)
+{
+    Operand o{};
+    o.mode = OP_STRIDED;
+    o.p = p;
+    o.sr = sr;
+    o.sk = sk;
+    return o;
+}
+
+static Operand im2col_view(int mode, const float* act, int H, int W, int C)
+{
+    Operand o{};
+    o.mode = mode;
+    o.p = act;
+    o.H = H;
+    o.W = W;
+    o.C = C;
+    o.Ho = H / 2;
+    o.Wo = W / 2;
+    return o;
+}
+
+void Net::layout(int n_bins_)
+{
+    n_bins = n_bins_;
+    long long off = 0;
+    for (int l = 0; l < 4; l++) {
+        w_off[l] = off;
+        off += (long long)kChannels[l + 1] * 9 * kChannels[l];
+        b_off[l] = off;
+        off += kChannels[l + 1];
+    }
+    wh_off = off;
+    off += (long long)(n_bins + 2) * kFeat;
+    bh_off = off;
+    off += n_bins + 2;
+    n_params = off;
+}
+
+int Net::ensure_batch(int B)
+{
+    if (B <= cap) return T2X_OK;
+    long long a = 0;
+    for (int l = 1; l <= 4; l++) {
+        const int s = kIn >> l;
+        a += (long long)B * s * s * kChannels[l];
+    }
+    (void)a;
+    for (int l = 0; l < 5; l++) {
+        const int s = kIn >> l;
+        T2X_TRY(act[l].ensure(sizeof(float) * (size_t)B * s * s * kChannels[l]));
+        T2X_TRY(dact[l].ensure(sizeof(float) * (size_t)B * s * s * kChannels[l]));
+    }
+    size_t dc = 0;
+    for (int l = 1; l < 4; l++) {   // dcols of layers 1..3 (input gradients of layers 2..4)
+        const int s = kIn >> (l + 1);
+        dc = std::max(dc, sizeof(float) * (size_t)B * s * s * 9 * kChannels[l]);
+    }
+    T2X_TRY(dcols.ensure(dc));
+    T2X_TRY(dz.ensure(sizeof(float) * (size_t)B * (kIn / 2) * (kIn / 2) * kChannels[1]));
+    T2X_TRY(y.ensure(sizeof(float) * (size_t)B * (n_bins + 2)));
+    T2X_TRY(dy.ensure(sizeof(float) * (size_t)B * (n_bins + 2)));
+    T2X_TRY(part.ensure(sizeof(float) * (size_t)CS_SLABS * std::max(n_bins + 2, kChannels[4])));
+    T2X_TRY(loss.ensure(sizeof(double) * B));
+    T2X_TRY(correct.ensure(sizeof(int) * B));
+    T2X_TRY(label.ensure(sizeof(int) * B));
+    T2X_TRY(shift_t.ensure(sizeof(float) * 2 * B));
+    cap = B;
+    return T2X_OK;
+}
+
+// act[0] holds the input [B][64][64][1]; writes act[1..4] and y
+int Net::forward(int B, bool bf16, hipStream_t st)
+{
+    const float* P = params.as<float>();
+    for (int l = 0; l < 4; l++) {
+        const int H = kIn >> l;
+        GemmArgs g{};
+        g.M = B * (H / 2) * (H / 2);
+        g.N = kChannels[l + 1];
+        g.K = 9 * kChannels[l];
+        g.a = im2col_view(OP_IM2COL, act[l].as<float>(), H, H, kChannels[l]);
+        g.b = strided(P + w_off[l], g.K, 1);
+        g.c = act[l + 1].as<float>();
+        g.ldc = g.N;
+        g.bias = P + b_off[l];
+        T2X_TRY(gemm(bf16, EPI_BIAS_RELU, g, st));
+    }
+    GemmArgs h{};
+    h.M = B;
+    h.N = n_bins + 2;
+    h.K = kFeat;
+    h.a = strided(act[4].as<float>(), kFeat, 1);
+    h.b = strided(P + wh_off, kFeat, 1);
+    h.c = y.as<float>();
+    h.ldc = h.N;
+    h.bias = P + bh_off;
+    return gemm(bf16, EPI_BIAS, h, st);
+}
+
+static int colsum(const float* x, long long M, int N, float* part, float* out, hipStream_t st)
+{
+    k_colsum_partial<<<dim3((unsigned)((N + 63) / 64), CS_SLABS), 256, 0, st>>>(x, M, N, part);
+    k_colsum_final<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(part, N, out);
+    T2X_HIP(hipGetLastError());
+    return T2X_OK;
+}
+
+// dy must hold dloss/dy; fills grad (n_params floats)
+int Net::backward(int B, bool bf16, hipStream_t st)
+{
+    const float* P = params.as<float>();
+    float* G = grad.as<float>();
+    const int ldy = n_bins + 2;
+    // head
+    {
+        GemmArgs g{};   // dWh[o][k] = sum_b dy[b][o] flat[b][k]
+        g.M = ldy;
+        g.N = kFeat;
+        g.K = B;
+        g.a = strided(dy.as<float>(), 1, ldy);
+        g.b = strided(act[4].as<float>(), 1, kFeat);
+        g.c = G + wh_off;
+        g.ldc = kFeat;
+        T2X_TRY(gemm(bf16, EPI_STORE, g, st));
+        T2X_TRY(colsum(dy.as<float>(), B, ldy, part.as<float>(), G + bh_off, st));
+        GemmArgs d{};   // dflat[b][k] = sum_o dy[b][o] Wh[o][k]
+        d.M = B;
+        d.N = kFeat;
+        d.K = ldy;
+        d.a = strided(dy.as<float>(), ldy, 1);
+        d.b = strided(P + wh_off, 1, kFeat);
+        d.c = dact[4].as<float>();
+        d.ldc = kFeat;
+        T2X_TRY(gemm(bf16, EPI_STORE, d, st));
+    }
+    for (int l = 3; l >= 0; l--) {
+        const int H = kIn >> l, Ho = H / 2;
+        const long long M = (long long)B * Ho * Ho;
+        const int Co = kChannels[l + 1], Ci = kChannels[l], K = 9 * Ci;
+        float* dzp = dz.as<float>();
+        k_relu_bwd<<<grid_for(M * Co), 256, 0, st>>>(dact[l + 1].as<float>(), act[l + 1].as<float>(), dzp, M * Co);
+        T2X_HIP(hipGetLastError());
+        GemmArgs g{};   // dW[co][q] = sum_p dz[p][co] im2col(act_l)[p][q]
+        g.M = Co;
+        g.N = K;
+        g.K = (int)M;
+        g.a = strided(dzp, 1, Co);
+        g.b = im2col_view(OP_IM2COL_T, act[l].as<float>(), H, H, Ci);
+        g.c = G + w_off[l];
+        g.ldc = K;
+        T2X_TRY(gemm(bf16, EPI_STORE, g, st));
+        T2X_TRY(colsum(dzp, M, Co, part.as<float>(), G + b_off[l], st));
+        if (l > 0) {
+            GemmArgs d{};   // dcols[p][q] = sum_co dz[p][co] W[co][q]
+            d.M = (int)M;
+            d.N = K;
+            d.K = Co;
+            d.a = strided(dzp, Co, 1);
+            d.b = strided(P + w_off[l], 1, K);
+            d.c = dcols.as<float>();
+            d.ldc = K;
+            T2X_TRY(gemm(bf16, EPI_STORE, d, st));
+            k_col2im<<<grid_for((long long)B * H * H * Ci), 256, 0, st>>>(dcols.as<float>(), B, H, H, Ci,
+                                                                          dact[l].as<float>());
+            T2X_HIP(hipGetLastError());
+        }
+    }
+    return T2X_OK;
+}
+
+int Net::loss_grad(int B, bool bf16, float shift_weight, hipStream_t st)
+{
+    T2X_TRY(forward(B, bf16, st));
+    k_cnn_loss<<<B, 256, 0, st>>>(y.as<float>(), n_bins, label.as<int>(), shift_t.as<float>(), shift_weight, B,
+                                  dy.as<float>(), loss.as<double>(), correct.as<int>());
+    T2X_HIP(hipGetLastError());
+    return backward(B, bf16, st);
+}
+
+int Net::adam_step(float lr, hipStream_t st)
+{
+    adam_t += 1;
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    const float c1 = 1.f - std::pow(b1, (float)adam_t), c2 = 1.f - std::pow(b2, (float)adam_t);
+    k_adam<<<grid_for(n_params), 256, 0, st>>>(params.as<float>(), grad.as<float>(), m1.as<float>(), m2.as<float>(),
+                                                n_params, lr, b1, b2, eps, c1, c2);
+    T2X_HIP(hipGetLastError());
+    return T2X_OK;
+}
+
+// He-uniform initialisation from a counter-based generator (host), zero biases
+int Net::init(int n_bins_, uint64_t seed, hipStream_t st)
+{
+    layout(n_bins_);
+    std::vector<float> h((size_t)n_params, 0.f);
+    auto uni = [&](uint64_t i) {
+        uint64_t z = seed + 0x9E3779B97F4A7C15ULL * (i + 1);
+        z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+        z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+        z ^= z >> 31;
+        return (double)(z >> 11) * (1.0 / 9007199254740992.0);
+    };
+    for (int l = 0; l < 4; l++) {
+        const long long n = (long long)kChannels[l + 1] * 9 * kChannels[l];
+        const double lim = std::sqrt(6.0 / (9.0 * kChannels[l]));
+        for (long long i = 0; i < n; i++) h[w_off[l] + i] = (float)((2.0 * uni(w_off[l] + i) - 1.0) * lim);
+    }
+    const double limh = std::sqrt(6.0 / kFeat);
+    for (long long i = 0; i < (long long)(n_bins + 2) * kFeat; i++)
+        h[wh_off + i] = (float)((2.0 * uni(wh_off + i) - 1.0) * limh);
+    T2X_TRY(params.ensure(sizeof(float) * n_params));
+    T2X_TRY(grad.ensure(sizeof(float) * n_params));
+    T2X_TRY(m1.ensure(sizeof(float) * n_params));
+    T2X_TRY(m2.ensure(sizeof(float) * n_params));
+    T2X_HIP(hipMemcpyAsync(params.ptr, h.data(), sizeof(float) * n_params, hipMemcpyHostToDevice, st));
+    T2X_HIP(hipMemsetAsync(m1.ptr, 0, sizeof(float) * n_params, st));
+    T2X_HIP(hipMemsetAsync(m2.ptr, 0, sizeof(float) * n_params, st));
+    adam_t = 0;
+    trained = false;
+    return T2X_OK;
+}
+
+}  // namespace cnn
+}  // namespace t2x

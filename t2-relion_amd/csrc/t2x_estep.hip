@@ -16,6 +16,9 @@
 #include <numeric>
 
 #include "t2x_kernels.h"
+#include "t2x_cnn.h"
+
+#include <chrono>
 
 namespace t2x {
 
@@ -36,6 +39,16 @@ int launch_gemm(int epi, const float* A, long long lda, const float* B, long lon
                 int Mrows_pad, int Nrows_pad, const EpiParams& ep, hipStream_t st);
 int launch_split(int prec, const float* X, long long ldx, int K, int rows, int valid_rows, void* hi,
                  void* lo, long long ld, float* inv_scale, hipStream_t st);
+int launch_gemm_x3_bins(int prec, const void* Ah, const void* Al, const float* As, long long lda, const void* Bh,
+                        const void* Bl, const float* Bs, long long ldb, int K, const int* plist, const void* work,
+                        int n_work, int T, long long n_clamp, const EpiParams& ep, hipStream_t st);
+struct BinWorkHost {
+    int pstart, cnt, rt, col0, ncols;
+};
+namespace cnn {
+int proposal_infer(Device* d, const float2* Fimg, int cs, int B, int topk, double min_conf, std::vector<int>& bins,
+                   std::vector<float>& maxp, std::vector<int>& fallback);
+}
 int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const float* As, long long lda,
                    const void* Bh, const void* Bl, const float* Bs, long long ldb, int K, int Mrows_pad,
                    int Nrows_pad, const EpiParams& ep, hipStream_t st, int tile);
@@ -783,6 +796,24 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     const int over_rot = S.over_rot, over_trans = S.over_trans, nover = over_rot * over_trans;
     hipStream_t st = d->stream;
     if (T > 128) { set_error("more than 128 coarse translations"); return T2X_EINVAL; }
+    // learned proposal: the coarse pass scores only the top-k bins (classes x HEALPix directions
+    // one order down = blocks of 4 directions x Npsi orientations) of each particle
+    const bool use_prop = cfg->proposal_topk > 0;
+    const int n_bins = (int)(K * (Nd / 4)), bin_cols = (int)(4 * Npsi);
+    int topk = 0;
+    if (use_prop) {
+        if (cfg->precision == T2X_PREC_FP32) {
+            set_error("the learned proposal runs with a split-precision cross term (BF16X3 or FP16X3)");
+            return T2X_EINVAL;
+        }
+        if (!d->net || !d->net->trained) { set_error("proposal network not trained on this handle"); return T2X_ESTATE; }
+        if (Nd % 4 || d->net->n_bins != n_bins) {
+            set_error("proposal network bins do not match classes x directions / 4");
+            return T2X_EINVAL;
+        }
+        if (bin_cols % 64) { set_error("learned proposal needs 4 * nr_psi to be a multiple of 64"); return T2X_EINVAL; }
+        topk = std::min(cfg->proposal_topk, n_bins);
+    }
     if (T * over_trans > STORE_MAX_T) { set_error("too many fine translations"); return T2X_EINVAL; }
     std::vector<const float2*> h_vols(K);
     std::vector<int> h_pads(K);
@@ -1057,6 +1088,52 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             T2X_TRY(launch_gemm(EPI_STORE, d->A1.as<float>(), Kn, d->coarse_B2.as<float>(), Kn, Kn,
                                 (int)B_pad, (int)O_pad, ep, st));
         }
+        // ---- learned proposal: bins per particle, work lists of the restricted coarse GEMM ----
+        int n_work = 0;
+        if (use_prop) {
+            const auto tp0 = std::chrono::steady_clock::now();
+            std::vector<int> pbins, pfb;
+            std::vector<float> pmax;
+            T2X_TRY(cnn::proposal_infer(d, d->Fimg_cur.as<float2>(), cs, B, topk, cfg->proposal_min_conf, pbins, pmax,
+                                        pfb));
+            const int mw = (n_bins + 31) / 32;
+            std::vector<unsigned> mask((size_t)B * mw, 0u);
+            std::vector<std::vector<int>> members(n_bins);
+            long long scored = 0;
+            for (int i = 0; i < B; i++) {
+                if (pfb[i]) {
+                    for (int j = 0; j < n_bins; j++) { mask[(size_t)i * mw + (j >> 5)] |= 1u << (j & 31); members[j].push_back(i); }
+                    scored += n_bins;
+                    d->stats.s.n_proposal_fallback++;
+                } else {
+                    for (int k = 0; k < topk; k++) {
+                        const int j = pbins[(size_t)i * topk + k];
+                        mask[(size_t)i * mw + (j >> 5)] |= 1u << (j & 31);
+                        members[j].push_back(i);
+                    }
+                    scored += topk;
+                }
+            }
+            std::vector<int> plist;
+            std::vector<BinWorkHost> work;
+            for (int j = 0; j < n_bins; j++) {
+                if (members[j].empty()) continue;
+                const int pstart = (int)plist.size(), cnt = (int)members[j].size();
+                plist.insert(plist.end(), members[j].begin(), members[j].end());
+                const long long rows = (long long)cnt * T;
+                for (int rt = 0; (long long)rt * 128 < rows; rt++)
+                    for (int c0 = 0; c0 < bin_cols; c0 += 128)
+                        work.push_back(BinWorkHost{pstart, cnt, rt, j * bin_cols + c0, std::min(128, bin_cols - c0)});
+            }
+            T2X_TRY(upload(d->prop_mask, mask, st));
+            T2X_TRY(upload(d->prop_plist, plist, st));
+            T2X_TRY(upload(d->prop_work, work, st));
+            n_work = (int)work.size();
+            d->stats.s.n_proposal_particles += B;
+            d->stats.s.proposal_bins_scored += (double)scored;
+            d->stats.s.ms_proposal +=
+                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp0).count();
+        }
         (void)hipEventRecord(d->ev[ev++], st);   // 2: coarse gemm start
         {
             EpiParams ep{};
@@ -1091,9 +1168,15 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                 T2X_TRY(d->As.ensure(sizeof(float) * M_pad));
                 T2X_TRY(launch_split(prec, d->A.as<float>(), Kc, Kc, (int)M_pad, (int)M, d->Ah.ptr, d->Al.ptr, Kc,
                                      d->As.as<float>(), st));
-                T2X_TRY(launch_gemm_x3(prec, EPI_DIFF2, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc,
-                                       d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc,
-                                       (int)M_pad, (int)O_pad, ep, st, 0));
+                if (use_prop)
+                    T2X_TRY(launch_gemm_x3_bins(prec, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc, d->coarse_Bh.ptr,
+                                                d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc,
+                                                d->prop_plist.as<int>(), d->prop_work.ptr, n_work, (int)T,
+                                                O_pad - 1, ep, st));
+                else
+                    T2X_TRY(launch_gemm_x3(prec, EPI_DIFF2, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc,
+                                           d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc,
+                                           (int)M_pad, (int)O_pad, ep, st, 0));
             }
         }
         (void)hipEventRecord(d->ev[ev++], st);   // 3: coarse gemm end
@@ -1103,6 +1186,11 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         SelResult* d_self = d_selc + B;
         WeightSrc wc{};
         wc.kind = SRC_COARSE;
+        if (use_prop) {
+            wc.bin_mask = d->prop_mask.as<unsigned>();
+            wc.mask_words = (n_bins + 31) / 32;
+            wc.bin_cols = bin_cols;
+        }
         wc.d2 = d->diff2.as<float>();
         wc.ld = O_pad;
         wc.T = (int)T;

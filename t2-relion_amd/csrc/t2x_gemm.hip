@@ -532,6 +532,143 @@ __global__ __launch_bounds__(256, 2) void k_gemm_x3d(SplitOperand A, SplitOperan
     gemm_epilogue<8, EPI>(acc, m0 + wm * 128, n0 + wn * 64, lane, A.inv_scale, B.inv_scale, ep);
 }
 
+// Learned-proposal variant: one workgroup per work item = (bin, row tile, column tile). The
+// rows of a bin are the T translation rows of each particle that scores the bin (particle list
+// plist[pstart .. pstart+cnt)), the columns its contiguous block of orientations [col0,
+// col0+ncols) (ncols a multiple of 64, so each 64-column row minimum belongs to one work item).
+// 128x128 tiles, 4 waves of 64x64, swizzled LDS, split-precision operands as k_gemm_x3w.
+struct BinWork {
+    int pstart, cnt, rt, col0, ncols;
+};
+
+template <class H>
+__global__ __launch_bounds__(256) void k_gemm_x3_bins(SplitOperand A, SplitOperand B, int K, const int* __restrict__ plist,
+                                                      const BinWork* __restrict__ work, int T, long long n_clamp,
+                                                      EpiParams ep)
+{
+    typedef typename Vec8<H>::type v8;
+    __shared__ __attribute__((aligned(16))) H lds[4][128 * 32];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const BinWork wk = work[blockIdx.x];
+    const int nrows = wk.cnt * T;
+    // A row of local row r (clamped to a valid row when beyond the bin's rows)
+    auto arow = [&](int r) -> long long {
+        const int ml = wk.rt * 128 + r;
+        if (ml >= nrows) return 0;
+        return (long long)plist[wk.pstart + ml / T] * T + ml % T;
+    };
+    const int lrow = tid >> 2, lc = tid & 3;
+    const long long ma0 = arow(lrow), ma1 = arow(lrow + 64);
+    const long long nb0 = min((long long)wk.col0 + lrow, n_clamp), nb1 = min((long long)wk.col0 + lrow + 64, n_clamp);
+    const H* Ah = reinterpret_cast<const H*>(A.hi);
+    const H* Al = reinterpret_cast<const H*>(A.lo);
+    const H* Bh = reinterpret_cast<const H*>(B.hi);
+    const H* Bl = reinterpret_cast<const H*>(B.lo);
+    const size_t ra0 = (size_t)ma0 * A.ld + lc * 8, ra1 = (size_t)ma1 * A.ld + lc * 8;
+    const size_t rb0 = (size_t)nb0 * B.ld + lc * 8, rb1 = (size_t)nb1 * B.ld + lc * 8;
+    const int s0 = swz(lrow, lc), s1 = swz(lrow + 64, lc);
+    floatx4 acc[4][4];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int j = 0; j < 4; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    const int fr = lane & 15, fc = lane >> 4;
+    for (int k0 = 0; k0 < K; k0 += XK) {
+        const uint4 p0 = *reinterpret_cast<const uint4*>(Ah + ra0 + k0), p1 = *reinterpret_cast<const uint4*>(Ah + ra1 + k0);
+        const uint4 p2 = *reinterpret_cast<const uint4*>(Al + ra0 + k0), p3 = *reinterpret_cast<const uint4*>(Al + ra1 + k0);
+        const uint4 p4 = *reinterpret_cast<const uint4*>(Bh + rb0 + k0), p5 = *reinterpret_cast<const uint4*>(Bh + rb1 + k0);
+        const uint4 p6 = *reinterpret_cast<const uint4*>(Bl + rb0 + k0), p7 = *reinterpret_cast<const uint4*>(Bl + rb1 + k0);
+        __syncthreads();
+        *reinterpret_cast<uint4*>(&lds[0][s0]) = p0;
+        *reinterpret_cast<uint4*>(&lds[0][s1]) = p1;
+        *reinterpret_cast<uint4*>(&lds[1][s0]) = p2;
+        *reinterpret_cast<uint4*>(&lds[1][s1]) = p3;
+        *reinterpret_cast<uint4*>(&lds[2][s0]) = p4;
+        *reinterpret_cast<uint4*>(&lds[2][s1]) = p5;
+        *reinterpret_cast<uint4*>(&lds[3][s0]) = p6;
+        *reinterpret_cast<uint4*>(&lds[3][s1]) = p7;
+        __syncthreads();
+        v8 bh[4], bl[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int o = swz(wn * 64 + j * 16 + fr, fc);
+            bh[j] = *reinterpret_cast<const v8*>(&lds[2][o]);
+            bl[j] = *reinterpret_cast<const v8*>(&lds[3][o]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int o = swz(wm * 64 + i * 16 + fr, fc);
+            const v8 ah = *reinterpret_cast<const v8*>(&lds[0][o]);
+            const v8 al = *reinterpret_cast<const v8*>(&lds[1][o]);
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = mfma16x16x32<H>(al, bh[j], acc[i][j]);
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = mfma16x16x32<H>(ah, bl[j], acc[i][j]);
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = mfma16x16x32<H>(ah, bh[j], acc[i][j]);
+        }
+    }
+    // epilogue (diff2): rows gathered through the particle list
+    const int col_in = lane & 15, row_grp = (lane >> 4) * 4;
+    const int cbase = wn * 64;
+    if (cbase >= wk.ncols) return;   // this wave's 64 columns are outside the bin
+    const long long ncol0 = (long long)wk.col0 + cbase;
+    float sb[4];
+    bool scored[4];
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const long long n = ncol0 + j * 16 + col_in;
+        sb[j] = B.inv_scale ? B.inv_scale[n] : 1.f;
+        scored[j] = ep.prior_o[n] > 0.;
+    }
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int ml = wk.rt * 128 + wm * 64 + i * 16 + row_grp + r;
+            const bool mval = ml < nrows;
+            int p = 0;
+            long long m = 0;
+            if (mval) {
+                p = plist[wk.pstart + ml / T];
+                m = (long long)p * T + ml % T;
+            }
+            const float sa = (mval && A.inv_scale) ? A.inv_scale[m] : 1.f;
+            const double cb = mval ? ep.Cb[p] : 0.;
+            float rmin = 3.0e38f;
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const long long n = ncol0 + j * 16 + col_in;
+                if (mval) {
+                    const float v = acc[i][j][r] * sa * sb[j];
+                    const float d = (float)(cb + (double)ep.N1[(long long)p * ep.ldn + n] - (double)v);
+                    ep.C[m * ep.ldc + n] = d;
+                    if (scored[j]) rmin = fminf(rmin, d);
+                }
+            }
+            for (int o = 1; o < 16; o <<= 1) rmin = fminf(rmin, __shfl_xor(rmin, o));
+            if (col_in == 0 && mval) ep.rowmin[m * ep.rowmin_ld + ncol0 / 64] = rmin;
+        }
+    }
+}
+
+int launch_gemm_x3_bins(int prec, const void* Ah, const void* Al, const float* As, long long lda, const void* Bh,
+                        const void* Bl, const float* Bs, long long ldb, int K, const int* plist, const void* work,
+                        int n_work, int T, long long n_clamp, const EpiParams& ep, hipStream_t st)
+{
+    if (K % XK != 0) { set_error("gemm_x3_bins: K not padded"); return T2X_EINVAL; }
+    if (n_work <= 0) return T2X_OK;
+    SplitOperand A{Ah, Al, As, lda}, B{Bh, Bl, Bs, ldb};
+    const BinWork* w = reinterpret_cast<const BinWork*>(work);
+    if (prec == T2X_PREC_BF16X3)
+        k_gemm_x3_bins<__bf16><<<(unsigned)n_work, 256, 0, st>>>(A, B, K, plist, w, T, n_clamp, ep);
+    else
+        k_gemm_x3_bins<_Float16><<<(unsigned)n_work, 256, 0, st>>>(A, B, K, plist, w, T, n_clamp, ep);
+    T2X_HIP(hipGetLastError());
+    return T2X_OK;
+}
+
 // hi/lo split of a row-major float matrix (rows x K), optionally with a power-of-two scale per
 // row chosen so the row maximum maps to [0.5, 1); rows beyond `valid_rows` are written as zero.
 template <class H>

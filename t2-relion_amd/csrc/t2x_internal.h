@@ -119,6 +119,8 @@ struct Stats {
     t2x_stats s{};
 };
 
+namespace cnn { struct Net; }
+
 struct Device {
     int id = 0;
     hipStream_t stream = nullptr;
@@ -148,6 +150,10 @@ struct Device {
 
     Stats stats;
     std::mutex mtx;
+    std::shared_ptr<cnn::Net> net;   // pose-proposal network (t2x_proposal_*)
+    DevBuf prop_crop, prop_img, prop_out, prop_mask, prop_plist, prop_work;
+    hipfftHandle prop_plan = 0;
+    int prop_plan_b = 0;
 };
 
 // ---------------- device helpers ----------------

@@ -47,7 +47,19 @@ struct WeightSrc {
     const unsigned int* fine_min; // [B] ordered-uint minima
     int nover;                 // fine samples per pair (32)
     int over_trans;            // 4
+    // coarse, learned proposal: bit per (particle, bin) of the scored orientation blocks; null
+    // = all orientations scored. bin of orientation n = n / bin_cols.
+    const unsigned int* bin_mask;
+    int mask_words;            // 32-bit words per particle
+    int bin_cols;              // orientations per bin
 };
+
+__device__ __forceinline__ bool coarse_allowed(const WeightSrc& s, int b, long long n)
+{
+    if (!s.bin_mask) return true;
+    const int bin = (int)(n / s.bin_cols);
+    return (s.bin_mask[(size_t)b * s.mask_words + (bin >> 5)] >> (bin & 31)) & 1u;
+}
 
 struct SelParams {
     double fraction;

@@ -24,6 +24,7 @@ __device__ __forceinline__ double pass_weight(const WeightSrc& s, int b, long lo
     double prior;
     if (s.kind == SRC_COARSE) {
         long long t = e / s.O_all, n = e - t * s.O_all;
+        if (!coarse_allowed(s, b, n)) return 0.;
         prior = s.prior_o[n] * s.prior_t[(size_t)b * s.T + t];
         if (!(prior > 0.)) return 0.;
         d = s.d2[((size_t)b * s.T + t) * s.ld + n];
@@ -568,7 +569,7 @@ __global__ __launch_bounds__(256) void k_coarse_sum_rows(WeightSrc s, int B, con
                 const int n = n4 + c;
                 if (n >= s.O_all) break;
                 const double po = s.prior_o[n];
-                if (!(po > 0.)) continue;
+                if (!(po > 0.) || !coarse_allowed(s, b, n)) continue;
                 const double dd = (double)dv[c] - mind2;
                 if (dd > 700.) continue;
                 npos++;
@@ -651,7 +652,7 @@ __global__ __launch_bounds__(256) void k_coarse_candidates(WeightSrc s, int B, S
             if (n < s.O_all) {
                 const double po = s.prior_o[n];
                 const double dd = (double)dv[c] - mind2;
-                if (po > 0. && dd <= kTinyDD) {
+                if (po > 0. && dd <= kTinyDD && coarse_allowed(s, b, n)) {
                     w = (po * pt) * exp(-dd);
                     keep = (w > 0.) && (w >= thr);
                 }

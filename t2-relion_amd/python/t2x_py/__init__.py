@@ -45,7 +45,8 @@ class EstepConfig(C.Structure):
         ("do_norm_correction", C.c_int), ("avg_norm_correction", C.c_double),
         ("do_ctf_correction", C.c_int), ("sigma2_offset", C.c_double),
         ("orientational_prior_mode", C.c_int), ("precision", C.c_int),
-        ("reserved", C.c_int * 8),
+        ("proposal_topk", C.c_int), ("proposal_reserved", C.c_int),
+        ("proposal_min_conf", C.c_double), ("reserved", C.c_int * 4),
     ]
 
 
@@ -108,7 +109,9 @@ class Stats(C.Structure):
         [("n_select_fallbacks", C.c_long), ("n_coarse_diff2_launches", C.c_long)] + \
         [(n, C.c_double) for n in ("coarse_diff2_flop", "coarse_diff2_bytes", "fine_samples",
                                    "fine_flop", "bp_orientations", "sig_coarse")] + \
-        [(n, C.c_int) for n in ("coarse_size", "current_size", "P_coarse", "P_fine")]
+        [(n, C.c_int) for n in ("coarse_size", "current_size", "P_coarse", "P_fine")] + \
+        [("ms_proposal", C.c_double), ("n_proposal_particles", C.c_long), ("n_proposal_fallback", C.c_long),
+         ("proposal_bins_scored", C.c_double)]
 
 
 class ReconParams(C.Structure):
@@ -124,6 +127,19 @@ def recon_params(ori_size, r_max, padding_factor=2.0, do_map=False, tau2_fudge=1
     return ReconParams(ori_size=ori_size, pad_size=pad, r_max=r_max, padding_factor=padding_factor,
                        ref_dim=3, data_dim=2, do_map=int(do_map), tau2_fudge=tau2_fudge,
                        minres_map=minres_map, skip_gridding=1)
+
+
+class ProposalTrainParams(C.Structure):
+    _fields_ = [("steps", C.c_int), ("batch", C.c_int), ("n_projections", C.c_int), ("bf16", C.c_int),
+                ("lr", C.c_double), ("shift_weight", C.c_double), ("max_shift_px", C.c_double),
+                ("defocus_min", C.c_double), ("defocus_max", C.c_double), ("voltage", C.c_double),
+                ("cs", C.c_double), ("q0", C.c_double), ("seed", C.c_uint64), ("reserved", C.c_int * 6)]
+
+
+class ProposalTrainStats(C.Structure):
+    _fields_ = [("ms_total", C.c_double), ("ms_setup", C.c_double), ("loss_first", C.c_double),
+                ("loss_last", C.c_double), ("top1_last", C.c_double), ("steps", C.c_int),
+                ("n_bins", C.c_int), ("reserved", C.c_int * 6)]
 
 
 class Blob(C.Structure):
@@ -199,6 +215,15 @@ def load(path=None):
         "t2x_device_synchronize": (C.c_int, [vp]),
         "t2x_device_copy": (C.c_int, [vp, vp, vp, C.c_size_t]),
         "t2x_debug_copy": (C.c_int, [vp, C.c_int, C.c_size_t, C.c_size_t, C.POINTER(C.c_float)]),
+        "t2x_proposal_init": (C.c_int, [vp, C.c_int, C.c_uint64]),
+        "t2x_proposal_num_params": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
+        "t2x_proposal_get_params": (C.c_int, [vp, C.POINTER(C.c_float), C.c_longlong]),
+        "t2x_proposal_set_params": (C.c_int, [vp, C.POINTER(C.c_float), C.c_longlong]),
+        "t2x_proposal_loss_grad": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int),
+                                             C.POINTER(C.c_float), C.c_double, C.c_int, PD,
+                                             C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "t2x_proposal_train": (C.c_int, [vp, C.POINTER(EstepConfig), C.POINTER(Model),
+                                         C.POINTER(ProposalTrainParams), C.POINTER(ProposalTrainStats)]),
         "t2x_reconstruct": (C.c_int, [vp, C.POINTER(ReconParams), PD, PD, PD, PD, PD]),
         "t2x_fsc": (C.c_int, [vp, C.c_int, PD, PD, PD]),
         "t2x_synth_reference_device": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int,
