@@ -1,12 +1,22 @@
 #!/usr/bin/env python3
-"""E-step throughput on one MI355X: the 3D-classification workload (20,000 particles of 256^2,
-K=4 references of 256^3 at 1.1 A/px, HEALPix order 3, offsets 6/1.5 px): the plain FP32 path
-and the split-precision path (FP16 hi+lo cross term on MFMA) timed in the same run.
+"""E-step throughput on one MI355X on the 3D-classification workload (20,000 particles of 256^2,
+K=4 references of 256^3 at 1.1 A/px, HEALPix order 3, offsets 6/1.5 px), three paths timed in the
+same run on the same resident particles:
 
-A step is one full E-step iteration over all particles, which are synthesised once on the GPU
-and stay resident in HBM: reference upload + coarse projections, preprocessing, both sampling
-passes, significance, weighted sums and backprojection. The coarse-diff2 GEMM (the dominant
-kernel) is timed with HIP events on the library's own stream inside the step.
+  fp32     plain FP32 cross term (the parity path)
+  fp16x3   split-precision cross term (FP16 hi+lo on MFMA, FP32 accumulate, FP64 normalisation)
+  learned  fp16x3 + learned pose proposal: the coarse pass scores only the top-16 bins
+           (classes x HEALPix order-2 directions) of the proposal network, exhaustive search for
+           particles whose top softmax probability is below 0.3. The network is trained first
+           (2000 steps of batch 256 on 200,000 projections of the current references made on the
+           fly by the GPU projector); its training time is reported beside the E-step rate.
+
+A step is one full E-step over all particles (synthesised once on the GPU, resident in HBM):
+reference upload + coarse projections, preprocessing, both sampling passes, significance,
+weighted sums and backprojection. The dominant kernel of the headline path is timed with HIP
+events on the library's stream. A quality phase then reconstructs half-maps (first and second
+half of a particle subset) of every class for each path on the GPU and reports the FSC-0.143
+resolution per path, and the pose agreement of each path with exhaustive search.
 
 The CPU baseline times the FP64 oracle (the restatement of relion_refine's CPU E-step) on a
 bounded sample of the same workload with one particle per thread.
@@ -26,11 +36,17 @@ import t2x_py as T  # noqa: E402
 from t2x_py import workloads as W  # noqa: E402
 
 METRIC = "E-step particles/s, 1 MI355X: plain FP32 vs mixed+learned at equal FSC-0.143"
-MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2500.0, "fp16x3": 2500.0}
-PRECISIONS = {"fp32": 0, "bf16x3": 1, "fp16x3": 2}
-DTYPES = {"fp32": "fp32", "bf16x3": "bf16x3 (bf16 hi+lo MFMA, fp32 accumulate, fp64 normalisation)",
-          "fp16x3": "fp16x3 (fp16 hi+lo MFMA, fp32 accumulate, fp64 normalisation)"}
-HBM_PEAK_GBS = 8000.0
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2500.0, "fp16x3": 2500.0, "learned": 2500.0}
+PRECISIONS = {"fp32": 0, "bf16x3": 1, "fp16x3": 2, "learned": 2}
+DTYPES = {"fp32": "fp32",
+          "bf16x3": "bf16x3 (bf16 hi+lo MFMA, fp32 accumulate, fp64 normalisation)",
+          "fp16x3": "fp16x3 (fp16 hi+lo MFMA, fp32 accumulate, fp64 normalisation)",
+          "learned": "fp16x3 cross term + bf16 proposal network (fp32 accumulate), fp64 normalisation"}
+KERNELS = {"fp32": "k_gemm_tn<EPI_DIFF2> (FP32 MFMA 16x16x4)",
+           "bf16x3": "k_gemm_x3w<bf16, EPI_DIFF2> (BF16 MFMA 16x16x32, 3 products)",
+           "fp16x3": "k_gemm_x3w<f16, EPI_DIFF2> (FP16 MFMA 16x16x32, 3 products)",
+           "learned": "k_gemm_x3_bins<f16> (FP16 MFMA 16x16x32, 3 products, top-k bins)"}
+TOPK, MIN_CONF = 16, 0.3
 
 
 def log(*a):
@@ -77,16 +93,104 @@ def build_dataset(lib, dev, w, n, cfg, sz, seed):
                 noise_var=noise_var)
 
 
-def run_step(lib, dev, cfg, sz, ds, n, outs, ws):
+def set_projectors(lib, dev, sz, ds):
     pad, rmax = sz["pad_size"], sz["r_max"]
     for c, (dre, dim) in enumerate(ds["refs"]):
         # AccProjector::initMdl of every iteration (device-resident model -> projector volume)
         T.check(lib.t2x_projector_set(dev, c, 3, pad, rmax, 2.0, C.cast(dre, C.POINTER(C.c_float)),
                                       C.cast(dim, C.POINTER(C.c_float))), "projector")
+
+
+def run_step(lib, dev, cfg, sz, ds, n, outs, ws, first=0):
+    """One E-step over particles [first, first+n): projectors set, backprojectors cleared."""
+    set_projectors(lib, dev, sz, ds)
+    for c in range(cfg.nr_classes):
         T.check(lib.t2x_backprojector_clear(dev, c), "bp clear")
     model = T.Model(T.dptr(ds["sigma2"]), T.dptr(ds["pdf_class"]), T.dptr(ds["pdf_dir"]), None)
-    T.check(lib.t2x_estep(dev, C.byref(cfg), C.byref(model), n, ds["dimg"], 1, ds["pin"], outs,
-                          C.byref(ws), None), "estep")
+    N = cfg.ori_size
+    img = C.c_void_p(ds["dimg"].value + first * N * N * 4)
+    pin = C.cast(C.c_void_p(C.addressof(ds["pin"]) + first * C.sizeof(T.ParticleIn)), C.POINTER(T.ParticleIn))
+    T.check(lib.t2x_estep(dev, C.byref(cfg), C.byref(model), n, img, 1, pin, outs, C.byref(ws), None), "estep")
+
+
+def train_proposal(lib, dev, cfg, sz, ds, w, steps=2000, batch=256, n_proj=200000, seed=2024):
+    set_projectors(lib, dev, sz, ds)
+    model = T.Model(T.dptr(ds["sigma2"]), T.dptr(ds["pdf_class"]), T.dptr(ds["pdf_dir"]), None)
+    tp = T.ProposalTrainParams(steps=steps, batch=batch, n_projections=n_proj, bf16=1, lr=3e-3, shift_weight=1.0,
+                               max_shift_px=float(w["max_shift"]), seed=seed)
+    st = T.ProposalTrainStats()
+    T.check(lib.t2x_proposal_train(dev, C.byref(cfg), C.byref(model), C.byref(tp), C.byref(st)), "train")
+    return st
+
+
+def configure(cfg, pname):
+    cfg.precision = PRECISIONS[pname]
+    cfg.proposal_topk = TOPK if pname == "learned" else 0
+    cfg.proposal_min_conf = MIN_CONF
+
+
+def rot_matrix(rot, tilt, psi):
+    a, b, g = np.radians(rot), np.radians(tilt), np.radians(psi)
+    ca, sa, cb, sb, cg, sg = np.cos(a), np.sin(a), np.cos(b), np.sin(b), np.cos(g), np.sin(g)
+    cc, cs, sc, ss = cb * ca, cb * sa, sb * ca, sb * sa
+    return np.array([[cg * cc - sg * sa, cg * cs + sg * ca, -cg * sb],
+                     [-sg * cc - cg * sa, -sg * cs + cg * ca, sg * sb],
+                     [sc, ss, cb]])
+
+
+def pose_agreement(a, b, step_deg):
+    """fraction of particles with the same class and rotations within one sampling step"""
+    ok = 0
+    n = len(a["rot"])
+    for i in range(n):
+        if a["iclass"][i] != b["iclass"][i]:
+            continue
+        R = rot_matrix(a["rot"][i], a["tilt"][i], a["psi"][i]) @ rot_matrix(b["rot"][i], b["tilt"][i], b["psi"][i]).T
+        ok += np.degrees(np.arccos(np.clip((np.trace(R) - 1) / 2, -1, 1))) <= step_deg + 1e-6
+    return ok / max(1, n)
+
+
+def fsc_resolution(fsc, ori, angpix):
+    """resolution (A) of the first shell below 0.143 (shell 1 onwards)"""
+    for s in range(1, len(fsc)):
+        if fsc[s] < 0.143:
+            return ori * angpix / s if s > 1 else float("inf")
+    return 2 * angpix
+
+
+def quality_phase(lib, dev, cfg, sz, ds, paths, nq):
+    """half-maps of every class per path (first / second half of nq particles), FSC-0.143"""
+    K = cfg.nr_classes
+    p = T.recon_params(cfg.ori_size, sz["r_max"])
+    pad = p.pad_size
+    shape = (pad, pad, pad // 2 + 1)
+    nvox = int(np.prod(shape))
+    half = nq // 2
+    res = {}
+    for pname in paths:
+        configure(cfg, pname)
+        maps = {}
+        for h, first in (("a", 0), ("b", half)):
+            outs = (T.ParticleOut * half)()
+            ws, _ = T.make_wsum(K, ds["info"].nr_dir, cfg.ori_size)
+            run_step(lib, dev, cfg, sz, ds, half, outs, ws, first=first)
+            for c in range(K):
+                re, im, wt = np.zeros(nvox), np.zeros(nvox), np.zeros(nvox)
+                T.check(lib.t2x_backprojector_get(dev, c, T.dptr(re), T.dptr(im), T.dptr(wt)))
+                vol = np.zeros((cfg.ori_size,) * 3)
+                T.check(lib.t2x_reconstruct(dev, p, T.dptr(re), T.dptr(im), T.dptr(wt), None, T.dptr(vol)), "recon")
+                maps[(h, c)] = vol
+        fsc_res = []
+        curves = []
+        for c in range(K):
+            f = np.zeros(cfg.ori_size // 2 + 1)
+            T.check(lib.t2x_fsc(dev, cfg.ori_size, T.dptr(maps[("a", c)]), T.dptr(maps[("b", c)]), T.dptr(f)), "fsc")
+            curves.append(f)
+            fsc_res.append(fsc_resolution(f, cfg.ori_size, cfg.pixel_size))
+        res[pname] = dict(fsc_res=fsc_res, curves=curves)
+        log(f"[quality {pname}] FSC-0.143 per class (A): " + " ".join(f"{r:.2f}" for r in fsc_res))
+    configure(cfg, "fp32")
+    return res
 
 
 def cpu_baseline(w, cfg, sz, seconds_target=20.0):
@@ -100,7 +204,6 @@ def cpu_baseline(w, cfg, sz, seconds_target=20.0):
                 "sample": f"unavailable: {e}"}
     ncpu = len(os.sched_getaffinity(0))
     threads = max(1, min(16, ncpu))
-    # one particle per thread, scaled down when a single particle would exceed the target
     wl = H.Workload(w_name(w), n=threads, seed_offset=99, current_size=cfg.current_size,
                     coarse_size=cfg.coarse_size)
     t0 = time.time()
@@ -130,9 +233,12 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device-to-device copy kernel whose FETCH_SIZE/WRITE_SIZE "
                          "counters calibrate the per-dispatch HBM traffic of the GEMM in --pmc runs")
-    ap.add_argument("--precision", default="fp32,fp16x3",
-                    help="comma list of cross-term variants to time (fp32, bf16x3, fp16x3); the "
-                         "last one is the headline value, the others are reported beside it")
+    ap.add_argument("--precision", default="fp32,fp16x3,learned",
+                    help="comma list of paths to time (fp32, bf16x3, fp16x3, learned); the last one is "
+                         "the headline value, the others are reported beside it")
+    ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--quality-particles", type=int, default=2000,
+                    help="particles of the half-map FSC phase (0: skip)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the E-step "
@@ -163,10 +269,18 @@ def main():
         T.check(lib.t2x_device_free(dev, src))
         T.check(lib.t2x_device_free(dev, dst))
         log("copy calibration: 1 GiB device-to-device copy (t2x::k_copy16) done")
+    paths = args.precision.split(",")
+    train = None
+    if "learned" in paths:
+        t = time.time()
+        train = train_proposal(lib, dev, cfg, sz, ds, w, steps=args.train_steps)
+        log(f"[learned] proposal trained: {train.steps} steps in {train.ms_total:.0f} ms "
+            f"(setup {train.ms_setup:.0f} ms), loss {train.loss_first:.3f} -> {train.loss_last:.3f}, "
+            f"train top-1 {train.top1_last:.3f} ({time.time() - t:.1f} s wall)")
     nw = min(n, args.warmup_particles)
     runs = {}
-    for pname in args.precision.split(","):
-        cfg.precision = PRECISIONS[pname]
+    for pname in paths:
+        configure(cfg, pname)
         for i in range(args.warmup):
             t = time.time()
             run_step(lib, dev, cfg, sz, ds, nw, outs, ws)
@@ -182,39 +296,65 @@ def main():
         dt = time.time() - t0
         st = T.Stats()
         T.check(lib.t2x_stats_get(dev, C.byref(st)))
-        best = np.array([o.best_hidden for o in outs])
-        runs[pname] = dict(dt=dt, st=st, value=n * args.steps / dt, best=best)
+        od = T.outs_to_dict(outs)
+        runs[pname] = dict(dt=dt, st=st, value=n * args.steps / dt, out=od)
         log(f"[{pname}] {runs[pname]['value']:.1f} particles/s")
-    head = args.precision.split(",")[-1]
+    head = paths[-1]
     r = runs[head]
     st, dt, value = r["st"], r["dt"], r["value"]
-    # dominant kernel: coarse diff2 GEMM; algorithmic 4*T*O*P flops per particle (x3 products
-    # on 16-bit MFMA for the split-precision variants)
+    # dominant kernel: the coarse cross-term GEMM; algorithmic 4*T*O*P flops per scored particle
+    # (x3 MFMA products for the split-precision forms)
     launches = max(1, st.n_coarse_diff2_launches)
     nprod = 1 if head == "fp32" else 3
-    flop_per_launch = nprod * st.coarse_diff2_flop / launches
+    flop_total = nprod * st.coarse_diff2_flop
+    if head == "learned":   # only the scored bins are computed
+        n_bins = cfg.nr_classes * ds["info"].nr_dir // 4
+        flop_total *= st.proposal_bins_scored / max(1.0, st.n_proposal_particles * n_bins)
+    flop_per_launch = flop_total / launches
     avg_ms = st.ms_coarse_gemm / launches
     peak = MFMA_PEAK_TFLOPS[head]
     achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
     traffic = os.environ.get("T2X_GEMM_TRAFFIC_BYTES")
-    kname = {"fp32": "k_gemm_tn<EPI_DIFF2> (FP32 MFMA 16x16x4)",
-             "bf16x3": "k_gemm_x3<bf16, EPI_DIFF2> (BF16 MFMA 16x16x32, 3 products)",
-             "fp16x3": "k_gemm_x3<f16, EPI_DIFF2> (FP16 MFMA 16x16x32, 3 products)"}[head]
     roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
             "frac": achieved / peak, "traffic": float(traffic) if traffic else None,
-            "kernel": "coarse diff2 cross term, " + kname,
+            "kernel": "coarse diff2 cross term, " + KERNELS[head],
             "avg_launch_ms": avg_ms, "launches": launches, "flop_per_launch": flop_per_launch}
     stages = {k: getattr(st, k) for k in ("ms_preprocess", "ms_coarse_project", "ms_coarse_diff2",
                                            "ms_coarse_weights", "ms_fine_diff2", "ms_fine_weights",
-                                           "ms_store")}
+                                           "ms_store", "ms_proposal")}
     nsig = st.sig_coarse / max(1, st.n_particles)
-    cpu = None if args.no_cpu else cpu_baseline(w, cfg, sz)
+    step_deg = 360.0 / (6 * 2 ** cfg.healpix_order)
+    ref = "fp16x3" if "fp16x3" in runs else ("fp32" if "fp32" in runs else None)
     variants = {}
     for pname, rr in runs.items():
-        variants[pname] = {"particles_per_s": rr["value"], "ms_per_step": rr["dt"] * 1e3 / args.steps,
-                           "coarse_gemm_ms": rr["st"].ms_coarse_gemm,
-                           "pose_agreement_vs_fp32": (float((rr["best"] == runs["fp32"]["best"]).mean())
-                                                      if "fp32" in runs else None)}
+        v = {"particles_per_s": rr["value"], "ms_per_step": rr["dt"] * 1e3 / args.steps,
+             "coarse_gemm_ms": rr["st"].ms_coarse_gemm,
+             "pose_index_agreement_vs_fp32": (float((rr["out"]["best_hidden"] == runs["fp32"]["out"]["best_hidden"]).mean())
+                                              if "fp32" in runs else None)}
+        if pname == "learned":
+            s2 = rr["st"]
+            n_bins = cfg.nr_classes * ds["info"].nr_dir // 4
+            v.update({"fallback_rate": s2.n_proposal_fallback / max(1, s2.n_proposal_particles),
+                      "bins_scored_per_particle": s2.proposal_bins_scored / max(1, s2.n_proposal_particles),
+                      "n_bins": n_bins, "topk": TOPK, "min_conf": MIN_CONF,
+                      "pose_agreement_within_one_step_vs_exhaustive":
+                          pose_agreement(rr["out"], runs[ref]["out"], step_deg) if ref else None,
+                      "training_ms": train.ms_total if train else None,
+                      "particles_per_s_including_training":
+                          n * args.steps / (rr["dt"] + (train.ms_total / 1e3 if train else 0.0)),
+                      "train_loss_first_last": [train.loss_first, train.loss_last] if train else None,
+                      "train_top1_last": train.top1_last if train else None})
+        variants[pname] = v
+    quality = None
+    if args.quality_particles > 0:
+        nq = min(n, args.quality_particles)
+        q = quality_phase(lib, dev, cfg, sz, ds, list(runs.keys()), nq)
+        quality = {"particles": nq, "fsc_0143_A_per_class": {k: v["fsc_res"] for k, v in q.items()}}
+        if "fp32" in q:
+            for k in q:
+                quality.setdefault("max_abs_fsc_diff_vs_fp32", {})[k] = float(
+                    max(np.abs(a - b).max() for a, b in zip(q[k]["curves"], q["fp32"]["curves"])))
+    cpu = None if args.no_cpu else cpu_baseline(w, cfg, sz)
     line = {
         "metric": METRIC, "value": value, "unit": "particles/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps,
@@ -223,7 +363,7 @@ def main():
         "config": {"workload": f"{args.workload}: {n} particles {cfg.ori_size}^2 @ {cfg.pixel_size} A/px, "
                                f"K={cfg.nr_classes}, HEALPix order {cfg.healpix_order}, offsets "
                                f"{w['offset_range_px']}/{w['offset_step_px']} px, current_size "
-                               f"{cfg.current_size}, coarse_size {cfg.coarse_size}, {head} cross term",
+                               f"{cfg.current_size}, coarse_size {cfg.coarse_size}, path {head}",
                    "global_batch": n, "parallelism": "single GPU",
                    "P_coarse": st.P_coarse, "P_fine": st.P_fine,
                    "mean_significant_coarse": nsig,
@@ -231,6 +371,7 @@ def main():
                    "stage_ms": stages},
         "variants": variants,
         "speedup_vs_plain_fp32": (value / runs["fp32"]["value"]) if "fp32" in runs else None,
+        "quality": quality,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

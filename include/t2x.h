@@ -309,7 +309,7 @@ typedef struct t2x_proposal_train_params {
     int      batch;            /* samples per step (256)                                      */
     int      n_projections;    /* distinct projection parameter sets, cycled (200000)        */
     int      bf16;             /* 1: BF16 operands on MFMA, FP32 accumulate; 0: FP32 operands */
-    double   lr;               /* Adam learning rate                                         */
+    double   lr;               /* peak Adam learning rate (linear warm-up, cosine decay)     */
     double   shift_weight;     /* weight of the shift regression term                        */
     double   max_shift_px;     /* shifts uniform in +-max_shift_px; targets are shift/max    */
     double   defocus_min, defocus_max;   /* A, defocus U and V uniform in this range          */
@@ -323,6 +323,7 @@ typedef struct t2x_proposal_train_stats {
     double ms_setup;           /* parameter sets, labels, uploads                            */
     double loss_first, loss_last;   /* mean loss over the first / last 20 steps             */
     double top1_last;          /* training top-1 bin accuracy over the last 20 steps         */
+    double top16_last;         /* training top-16 bin accuracy over the last 20 steps        */
     int    steps;
     int    n_bins;
     int    reserved[6];

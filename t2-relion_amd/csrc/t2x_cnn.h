@@ -14,7 +14,7 @@ struct Net {
     long long n_params = 0;
     long long w_off[4] = {}, b_off[4] = {}, wh_off = 0, bh_off = 0;
     DevBuf params, grad, m1, m2;        // FP32 master weights, gradient, Adam moments
-    DevBuf act[5], dact[5], dcols, dz, y, dy, part, loss, correct, label, shift_t;
+    DevBuf act[5], dact[5], dcols, dz, y, dy, part, loss, correct, rank, label, shift_t, splitk;
     int cap = 0;                        // batch capacity of the activation buffers
     long long adam_t = 0;
     bool trained = false;

@@ -41,50 +41,101 @@ struct GemmArgs {
     const float* bias;
 };
 
-// im2col element: row index p = (b, y, x) over the output grid, column q = (ky, kx, ci)
-__device__ __forceinline__ float im2col_at(const Operand& o, long long p, long long q)
-{
-    const int ci = (int)(q % o.C);
-    const int kk = (int)(q / o.C);
-    const int kx = kk % 3, ky = kk / 3;
-    const long long x = p % o.Wo, y = (p / o.Wo) % o.Ho, b = p / ((long long)o.Wo * o.Ho);
-    const long long yy = 2 * y + ky - 1, xx = 2 * x + kx - 1;
-    if (yy < 0 || yy >= o.H || xx < 0 || xx >= o.W) return 0.f;
-    return o.p[((b * o.H + yy) * o.W + xx) * o.C + ci];
-}
-
-__device__ __forceinline__ float op_at(const Operand& o, long long r, long long k)
-{
-    if (o.mode == OP_STRIDED) return o.p[r * o.sr + k * o.sk];
-    if (o.mode == OP_IM2COL) return im2col_at(o, r, k);
-    return im2col_at(o, k, r);   // OP_IM2COL_T: r is the im2col column, k the pixel
-}
-
+// Operand tile loads. Each thread fills 8 entries of a 64 x 32 (row x k) tile:
+//  row-major form (A = strided with unit k stride, or im2col): thread = (row, 8 consecutive k);
+//    the im2col row (b, y, x) is decoded once and (ky, kx, ci) advanced incrementally.
+//  k-major form (strided with unit row stride, or im2col transposed): thread = (k chunk, row),
+//    lanes run along the row index so the loads coalesce.
 #define CT 64
 #define CK 32
-#define CLD 36
+#define CLD 33   // LDS row stride (floats): conflict-free row-wise writes and fragment reads
 
-template <bool BF16, int EPI>
-__global__ __launch_bounds__(256) void k_cnn_gemm(GemmArgs g)
+__device__ __forceinline__ bool row_major_form(const Operand& o)
 {
-    __shared__ __attribute__((aligned(16))) float As[CT][CLD];
-    __shared__ __attribute__((aligned(16))) float Bs[CT][CLD];
+    return o.mode == OP_IM2COL || (o.mode == OP_STRIDED && o.sk == 1);
+}
+
+__device__ __forceinline__ void load_tile(const Operand& o, int R, int K, int r0, int k0, float (*S)[CLD], int tid)
+{
+    if (row_major_form(o)) {
+        const int lr = tid >> 2, lk = (tid & 3) * 8;
+        const int r = r0 + lr;
+        float v[8];
+        if (o.mode == OP_STRIDED) {
+            const float* p = o.p + (long long)r * o.sr;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int k = k0 + lk + q;
+                v[q] = (r < R && k < K) ? p[k] : 0.f;
+            }
+        } else {
+            // im2col: r = pixel (b, y, x) of the output grid, k = (ky, kx, ci)
+            const int x = r % o.Wo, t = r / o.Wo, y = t % o.Ho, b = t / o.Ho;
+            int k = k0 + lk;
+            int ci = k % o.C, kk = k / o.C;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                float val = 0.f;
+                if (r < R && k < K) {
+                    const int ky = kk / 3, kx = kk - 3 * ky;
+                    const int yy = 2 * y + ky - 1, xx = 2 * x + kx - 1;
+                    if (yy >= 0 && yy < o.H && xx >= 0 && xx < o.W)
+                        val = o.p[(((long long)b * o.H + yy) * o.W + xx) * o.C + ci];
+                }
+                v[q] = val;
+                k++;
+                if (++ci == o.C) { ci = 0; kk++; }
+            }
+        }
+#pragma unroll
+        for (int q = 0; q < 8; q++) S[lr][lk + q] = v[q];
+    } else {
+        const int lr = tid & 63, kc = (tid >> 6) * 8;
+        const int r = r0 + lr;
+        if (o.mode == OP_STRIDED) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                const int k = k0 + kc + q;
+                S[lr][kc + q] = (r < R && k < K) ? o.p[(long long)r * o.sr + (long long)k * o.sk] : 0.f;
+            }
+        } else {
+            // im2col transposed: r = column (ky, kx, ci), k = pixel (b, y, x)
+            const int ci = r % o.C, kk = r / o.C, ky = kk / 3, kx = kk - 3 * ky;
+            int k = k0 + kc;
+            int x = k % o.Wo, t = k / o.Wo, y = t % o.Ho, b = t / o.Ho;
+#pragma unroll
+            for (int q = 0; q < 8; q++) {
+                float val = 0.f;
+                if (r < R && k < K) {
+                    const int yy = 2 * y + ky - 1, xx = 2 * x + kx - 1;
+                    if (yy >= 0 && yy < o.H && xx >= 0 && xx < o.W)
+                        val = o.p[(((long long)b * o.H + yy) * o.W + xx) * o.C + ci];
+                }
+                S[lr][kc + q] = val;
+                k++;
+                if (++x == o.Wo) { x = 0; if (++y == o.Ho) { y = 0; b++; } }
+            }
+        }
+    }
+}
+
+// C (+ blockIdx.z * split_stride for split-K partials) = A(m, k) B(n, k) over k in this block's range
+template <bool BF16, int EPI>
+__global__ __launch_bounds__(256) void k_cnn_gemm(GemmArgs g, int k_chunk, long long split_stride)
+{
+    __shared__ float As[CT][CLD];
+    __shared__ float Bs[CT][CLD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const long long m0 = (long long)blockIdx.x * CT, n0 = (long long)blockIdx.y * CT;
+    const int m0 = blockIdx.x * CT, n0 = blockIdx.y * CT;
+    const int kb = blockIdx.z * k_chunk, ke = min(g.K, kb + k_chunk);
     floatx4 acc[2][2];
     for (int i = 0; i < 2; i++)
         for (int j = 0; j < 2; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
-    const int lr = tid >> 2, lk = (tid & 3) * 8;
-    for (int k0 = 0; k0 < g.K; k0 += CK) {
+    for (int k0 = kb; k0 < ke; k0 += CK) {
         __syncthreads();
-#pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const long long k = k0 + lk + q;
-            const long long m = m0 + lr, n = n0 + lr;
-            As[lr][lk + q] = (m < g.M && k < g.K) ? op_at(g.a, m, k) : 0.f;
-            Bs[lr][lk + q] = (n < g.N && k < g.K) ? op_at(g.b, n, k) : 0.f;
-        }
+        load_tile(g.a, g.M, ke, m0, k0, As, tid);
+        load_tile(g.b, g.N, ke, n0, k0, Bs, tid);
         __syncthreads();
         if (BF16) {
             const int kc = (lane >> 4) * 8;
@@ -123,6 +174,7 @@ __global__ __launch_bounds__(256) void k_cnn_gemm(GemmArgs g)
             }
         }
     }
+    float* c = g.c + blockIdx.z * split_stride;
     const int col = lane & 15, rg = (lane >> 4) * 4;
 #pragma unroll
     for (int i = 0; i < 2; i++)
@@ -130,14 +182,27 @@ __global__ __launch_bounds__(256) void k_cnn_gemm(GemmArgs g)
         for (int j = 0; j < 2; j++)
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const long long m = m0 + wm * 32 + i * 16 + rg + r;
-                const long long n = n0 + wn * 32 + j * 16 + col;
+                const int m = m0 + wm * 32 + i * 16 + rg + r;
+                const int n = n0 + wn * 32 + j * 16 + col;
                 if (m >= g.M || n >= g.N) continue;
                 float v = acc[i][j][r];
                 if (EPI != EPI_STORE) v += g.bias[n];
                 if (EPI == EPI_BIAS_RELU) v = fmaxf(v, 0.f);
-                g.c[m * g.ldc + n] = v;
+                c[(long long)m * g.ldc + n] = v;
             }
+}
+
+// out[m][n] = sum over the split-K partials in order
+__global__ void k_splitk_reduce(const float* __restrict__ part, int S, long long stride, int M, int N, long long ldc,
+                                float* __restrict__ out)
+{
+    const long long n_el = (long long)M * N;
+    for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n_el; e += (long long)gridDim.x * blockDim.x) {
+        const long long m = e / N, n = e - m * N;
+        float s = 0.f;
+        for (int z = 0; z < S; z++) s += part[z * stride + m * N + n];
+        out[m * ldc + n] = s;
+    }
 }
 
 // dz = da * (out > 0)
@@ -209,7 +274,7 @@ __global__ void k_col2im(const float* __restrict__ dcols, int Bn, int H, int W, 
 // One workgroup per sample; loss per sample into loss_out (summed on the host in order).
 __global__ void k_cnn_loss(const float* __restrict__ y, int nbins, const int* __restrict__ label,
                            const float* __restrict__ shift_t, float shift_weight, int Bn, float* __restrict__ dy,
-                           double* __restrict__ loss_out, int* __restrict__ correct)
+                           double* __restrict__ loss_out, int* __restrict__ correct, int* __restrict__ rank)
 {
     const int b = blockIdx.x;
     const int ld = nbins + 2;
@@ -247,6 +312,16 @@ __global__ void k_cnn_loss(const float* __restrict__ y, int nbins, const int* __
     }
     const float Z = red[0];
     const int lab = label[b];
+    __syncthreads();
+    int above = 0;   // logits strictly above the label's: its rank
+    for (int i = threadIdx.x; i < nbins; i += blockDim.x) above += yb[i] > yb[lab];
+    redi[threadIdx.x] = above;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) redi[threadIdx.x] += redi[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0 && rank) rank[b] = redi[0];
     for (int i = threadIdx.x; i < nbins; i += blockDim.x) {
         const float p = expf(yb[i] - m) / Z;
         dy[(size_t)b * ld + i] = (p - (i == lab ? 1.f : 0.f)) / (float)Bn;
@@ -284,21 +359,46 @@ static unsigned grid_for(long long n, int bs = 256)
     return (unsigned)std::min<long long>(std::max<long long>(g, 1), 65536);
 }
 
-int gemm(bool bf16, int epi, const GemmArgs& a, hipStream_t st)
+// One layer GEMM. Outputs with few tiles and a long reduction (the weight gradients: M x N of
+// at most 770 x 2048 against K up to B*32*32 pixels) are split over K into partial products that
+// a second kernel sums in a fixed order; `scratch` holds the partials.
+int gemm(bool bf16, int epi, const GemmArgs& a, hipStream_t st, DevBuf* scratch)
 {
     if (a.M <= 0 || a.N <= 0 || a.K <= 0) return T2X_OK;
-    const dim3 grid((unsigned)((a.M + CT - 1) / CT), (unsigned)((a.N + CT - 1) / CT));
-    if (grid.y > 65535) { set_error("cnn gemm: N too large"); return T2X_EINVAL; }
+    const unsigned gx = (unsigned)((a.M + CT - 1) / CT), gy = (unsigned)((a.N + CT - 1) / CT);
+    if (gy > 65535) { set_error("cnn gemm: N too large"); return T2X_EINVAL; }
+    const long long tiles = (long long)gx * gy;
+    int S = 1;
+    if (epi == EPI_STORE && scratch && tiles < 256 && a.K >= 4 * 1024) {
+        S = (int)std::min<long long>((512 + tiles - 1) / tiles, a.K / 1024);
+        S = std::max(1, std::min(S, 1024));
+    }
+    int k_chunk = a.K;
+    if (S > 1) k_chunk = (int)(((long long)(a.K + S - 1) / S + CK - 1) / CK * CK);
+    S = (a.K + k_chunk - 1) / k_chunk;
+    GemmArgs g = a;
+    long long stride = 0;
+    if (S > 1) {
+        stride = (long long)a.M * a.N;
+        T2X_TRY(scratch->ensure(sizeof(float) * (size_t)stride * S));
+        g.c = scratch->as<float>();
+        g.ldc = a.N;
+    }
+    const dim3 grid(gx, gy, (unsigned)S);
     if (bf16) {
-        if (epi == EPI_STORE) k_cnn_gemm<true, EPI_STORE><<<grid, 256, 0, st>>>(a);
-        else if (epi == EPI_BIAS) k_cnn_gemm<true, EPI_BIAS><<<grid, 256, 0, st>>>(a);
-        else k_cnn_gemm<true, EPI_BIAS_RELU><<<grid, 256, 0, st>>>(a);
+        if (epi == EPI_STORE) k_cnn_gemm<true, EPI_STORE><<<grid, 256, 0, st>>>(g, k_chunk, stride);
+        else if (epi == EPI_BIAS) k_cnn_gemm<true, EPI_BIAS><<<grid, 256, 0, st>>>(g, k_chunk, stride);
+        else k_cnn_gemm<true, EPI_BIAS_RELU><<<grid, 256, 0, st>>>(g, k_chunk, stride);
     } else {
-        if (epi == EPI_STORE) k_cnn_gemm<false, EPI_STORE><<<grid, 256, 0, st>>>(a);
-        else if (epi == EPI_BIAS) k_cnn_gemm<false, EPI_BIAS><<<grid, 256, 0, st>>>(a);
-        else k_cnn_gemm<false, EPI_BIAS_RELU><<<grid, 256, 0, st>>>(a);
+        if (epi == EPI_STORE) k_cnn_gemm<false, EPI_STORE><<<grid, 256, 0, st>>>(g, k_chunk, stride);
+        else if (epi == EPI_BIAS) k_cnn_gemm<false, EPI_BIAS><<<grid, 256, 0, st>>>(g, k_chunk, stride);
+        else k_cnn_gemm<false, EPI_BIAS_RELU><<<grid, 256, 0, st>>>(g, k_chunk, stride);
     }
     T2X_HIP(hipGetLastError());
+    if (S > 1) {
+        k_splitk_reduce<<<grid_for(stride), 256, 0, st>>>(scratch->as<float>(), S, stride, a.M, a.N, a.ldc, a.c);
+        T2X_HIP(hipGetLastError());
+    }
     return T2X_OK;
 }
 
@@ -368,6 +468,7 @@ int Net::ensure_batch(int B)
     T2X_TRY(part.ensure(sizeof(float) * (size_t)CS_SLABS * std::max(n_bins + 2, kChannels[4])));
     T2X_TRY(loss.ensure(sizeof(double) * B));
     T2X_TRY(correct.ensure(sizeof(int) * B));
+    T2X_TRY(rank.ensure(sizeof(int) * B));
     T2X_TRY(label.ensure(sizeof(int) * B));
     T2X_TRY(shift_t.ensure(sizeof(float) * 2 * B));
     cap = B;
@@ -389,7 +490,7 @@ int Net::forward(int B, bool bf16, hipStream_t st)
         g.c = act[l + 1].as<float>();
         g.ldc = g.N;
         g.bias = P + b_off[l];
-        T2X_TRY(gemm(bf16, EPI_BIAS_RELU, g, st));
+        T2X_TRY(gemm(bf16, EPI_BIAS_RELU, g, st, nullptr));
     }
     GemmArgs h{};
     h.M = B;
@@ -400,7 +501,7 @@ int Net::forward(int B, bool bf16, hipStream_t st)
     h.c = y.as<float>();
     h.ldc = h.N;
     h.bias = P + bh_off;
-    return gemm(bf16, EPI_BIAS, h, st);
+    return gemm(bf16, EPI_BIAS, h, st, nullptr);
 }
 
 static int colsum(const float* x, long long M, int N, float* part, float* out, hipStream_t st)
@@ -427,7 +528,7 @@ int Net::backward(int B, bool bf16, hipStream_t st)
         g.b = strided(act[4].as<float>(), 1, kFeat);
         g.c = G + wh_off;
         g.ldc = kFeat;
-        T2X_TRY(gemm(bf16, EPI_STORE, g, st));
+        T2X_TRY(gemm(bf16, EPI_STORE, g, st, &splitk));
         T2X_TRY(colsum(dy.as<float>(), B, ldy, part.as<float>(), G + bh_off, st));
         GemmArgs d{};   // dflat[b][k] = sum_o dy[b][o] Wh[o][k]
         d.M = B;
@@ -437,7 +538,7 @@ int Net::backward(int B, bool bf16, hipStream_t st)
         d.b = strided(P + wh_off, 1, kFeat);
         d.c = dact[4].as<float>();
         d.ldc = kFeat;
-        T2X_TRY(gemm(bf16, EPI_STORE, d, st));
+        T2X_TRY(gemm(bf16, EPI_STORE, d, st, &splitk));
     }
     for (int l = 3; l >= 0; l--) {
         const int H = kIn >> l, Ho = H / 2;
@@ -454,7 +555,7 @@ int Net::backward(int B, bool bf16, hipStream_t st)
         g.b = im2col_view(OP_IM2COL_T, act[l].as<float>(), H, H, Ci);
         g.c = G + w_off[l];
         g.ldc = K;
-        T2X_TRY(gemm(bf16, EPI_STORE, g, st));
+        T2X_TRY(gemm(bf16, EPI_STORE, g, st, &splitk));
         T2X_TRY(colsum(dzp, M, Co, part.as<float>(), G + b_off[l], st));
         if (l > 0) {
             GemmArgs d{};   // dcols[p][q] = sum_co dz[p][co] W[co][q]
@@ -465,7 +566,7 @@ int Net::backward(int B, bool bf16, hipStream_t st)
             d.b = strided(P + w_off[l], 1, K);
             d.c = dcols.as<float>();
             d.ldc = K;
-            T2X_TRY(gemm(bf16, EPI_STORE, d, st));
+            T2X_TRY(gemm(bf16, EPI_STORE, d, st, &splitk));
             k_col2im<<<grid_for((long long)B * H * H * Ci), 256, 0, st>>>(dcols.as<float>(), B, H, H, Ci,
                                                                           dact[l].as<float>());
             T2X_HIP(hipGetLastError());
@@ -478,7 +579,7 @@ int Net::loss_grad(int B, bool bf16, float shift_weight, hipStream_t st)
 {
     T2X_TRY(forward(B, bf16, st));
     k_cnn_loss<<<B, 256, 0, st>>>(y.as<float>(), n_bins, label.as<int>(), shift_t.as<float>(), shift_weight, B,
-                                  dy.as<float>(), loss.as<double>(), correct.as<int>());
+                                  dy.as<float>(), loss.as<double>(), correct.as<int>(), rank.as<int>());
     T2X_HIP(hipGetLastError());
     return backward(B, bf16, st);
 }

@@ -459,8 +459,9 @@ extern "C" int t2x_proposal_train(t2x_device* h, const t2x_estep_config* cfg, co
     const int r_lim = std::min(r_max, kIn / 2);
     const int nlog = std::min(20, tp->steps);
     std::vector<double> lfirst, llast;
-    std::vector<int> cor(B);
-    double loss_first = 0., loss_last = 0., top1 = 0.;
+    std::vector<int> cor(B), rk(B);
+    double loss_first = 0., loss_last = 0., top1 = 0., top16 = 0.;
+    const int warm = std::max(1, std::min(100, tp->steps / 10));
     int rc = T2X_OK;
     for (int step = 0; step < tp->steps && rc == T2X_OK; step++) {
         const long long first = (long long)step * B;
@@ -474,22 +475,31 @@ extern "C" int t2x_proposal_train(t2x_device* h, const t2x_estep_config* cfg, co
         if (rc != T2X_OK) break;
         rc = net.loss_grad(B, tp->bf16 != 0, (float)tp->shift_weight, st);
         if (rc != T2X_OK) break;
-        rc = net.adam_step((float)tp->lr, st);
+        // linear warm-up, then cosine decay to 5% of the peak rate
+        double lr = tp->lr;
+        if (step < warm) lr *= (double)(step + 1) / warm;
+        else lr *= 0.05 + 0.95 * 0.5 * (1. + std::cos(kPiP * (double)(step - warm) / std::max(1, tp->steps - warm)));
+        rc = net.adam_step((float)lr, st);
         if (rc != T2X_OK) break;
         if (step < nlog || step >= tp->steps - nlog) {
             std::vector<double> lb(B);
             if (hipMemcpyAsync(lb.data(), net.loss.ptr, sizeof(double) * B, hipMemcpyDeviceToHost, st) != hipSuccess ||
                 hipMemcpyAsync(cor.data(), net.correct.ptr, sizeof(int) * B, hipMemcpyDeviceToHost, st) != hipSuccess ||
+                hipMemcpyAsync(rk.data(), net.rank.ptr, sizeof(int) * B, hipMemcpyDeviceToHost, st) != hipSuccess ||
                 hipStreamSynchronize(st) != hipSuccess) {
                 set_error("proposal: readback failed");
                 rc = T2X_EDEVICE;
                 break;
             }
             double s = 0.;
-            int c = 0;
-            for (int b = 0; b < B; b++) { s += lb[b]; c += cor[b]; }
+            int c = 0, c16 = 0;
+            for (int b = 0; b < B; b++) { s += lb[b]; c += cor[b]; c16 += rk[b] < 16; }
             if (step < nlog) loss_first += s / B / nlog;
-            if (step >= tp->steps - nlog) { loss_last += s / B / nlog; top1 += (double)c / B / nlog; }
+            if (step >= tp->steps - nlog) {
+                loss_last += s / B / nlog;
+                top1 += (double)c / B / nlog;
+                top16 += (double)c16 / B / nlog;
+            }
         }
     }
     if (plan) hipfftDestroy(plan);
@@ -507,6 +517,7 @@ extern "C" int t2x_proposal_train(t2x_device* h, const t2x_estep_config* cfg, co
         out->loss_first = loss_first;
         out->loss_last = loss_last;
         out->top1_last = top1;
+        out->top16_last = top16;
         out->steps = tp->steps;
         out->n_bins = n_bins;
     }

@@ -138,7 +138,8 @@ class ProposalTrainParams(C.Structure):
 
 class ProposalTrainStats(C.Structure):
     _fields_ = [("ms_total", C.c_double), ("ms_setup", C.c_double), ("loss_first", C.c_double),
-                ("loss_last", C.c_double), ("top1_last", C.c_double), ("steps", C.c_int),
+                ("loss_last", C.c_double), ("top1_last", C.c_double), ("top16_last", C.c_double),
+                ("steps", C.c_int),
                 ("n_bins", C.c_int), ("reserved", C.c_int * 6)]
 
 

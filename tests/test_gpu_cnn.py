@@ -2,8 +2,12 @@
 itself pinned by finite differences): loss, outputs and every parameter gradient.
 
 FP32 operand mode: outputs and gradients within 1e-4 relative (Frobenius per tensor; FP32
-accumulation over reductions up to 65536 terms). BF16 operand mode (the training mode, BF16
-weights and activations into MFMA with FP32 accumulation): within 3e-2 relative.
+accumulation over reductions up to 65536 terms); this pins the kernels' indexing exactly.
+BF16 operand mode (the training mode: weights, activations and back-propagated gradients are
+rounded to BF16, 8-bit mantissa, before every MFMA, FP32 accumulate): outputs within 1e-2; the
+rounding compounds along the backward chain (measured 0.5% on the head up to 9% on the first
+layer's gradient for random weights), so gradients are held to 0.15 relative and to a cosine
+similarity of 0.99 with the exact gradient, which is what the optimiser's step direction needs.
 """
 import ctypes as C
 
@@ -27,8 +31,8 @@ def flat_to_dict(flat, n_bins):
     return out
 
 
-@pytest.mark.parametrize("bf16,tol", [(0, 1e-4), (1, 3e-2)])
-def test_loss_and_gradients_match_oracle(gpu_device, bf16, tol):
+@pytest.mark.parametrize("bf16,tol_out,tol_grad", [(0, 1e-4, 1e-4), (1, 1e-2, 0.15)])
+def test_loss_and_gradients_match_oracle(gpu_device, bf16, tol_out, tol_grad):
     lib = T.load()
     n_bins, B = 768, 8
     T.check(lib.t2x_proposal_init(gpu_device, n_bins, 1234), "init")
@@ -63,5 +67,7 @@ def test_loss_and_gradients_match_oracle(gpu_device, bf16, tol):
     errs = {k: H.rel_norm(gg[k], go[k]) for k in go}
     print(f"\n[cnn bf16={bf16}] outputs {e_out:.2e} loss {e_loss:.2e} grads " +
           " ".join(f"{k}:{v:.1e}" for k, v in errs.items()))
-    assert e_out < tol and e_loss < tol
-    assert max(errs.values()) < tol
+    assert e_out < tol_out and e_loss < tol_out
+    assert max(errs.values()) < tol_grad
+    cos = {k: float((gg[k] * go[k]).sum() / (np.linalg.norm(gg[k]) * np.linalg.norm(go[k]))) for k in go}
+    assert min(cos.values()) > 0.99, cos

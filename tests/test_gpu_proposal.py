@@ -2,7 +2,8 @@
 
 1. Machinery: with every bin scored (top-k = all bins), or every particle sent to the exhaustive
    fallback, the restricted coarse pass (grouped GEMM over bins, per-particle bin masks in the
-   selection) reproduces the dense split-precision E-step exactly.
+   selection) reproduces the dense split-precision E-step: identical poses and dLL (bit for bit),
+   backprojections equal up to the ordering of their FP32 atomic additions (1e-6).
 2. Gain: after training on projections of the current references, the top-16 restricted E-step
    agrees with exhaustive search (same class, rotation within one coarse step of 7.5 deg) on far
    more particles than the same restriction driven by the untrained network.
@@ -63,7 +64,7 @@ def train(s, steps, seed=5):
         T.check(lib.t2x_projector_set(dev, c, 3, pad, rmax, 2.0, C.cast(dre, C.POINTER(C.c_float)),
                                       C.cast(dim, C.POINTER(C.c_float))))
     model = T.Model(T.dptr(ds["sigma2"]), T.dptr(ds["pdf_class"]), T.dptr(ds["pdf_dir"]), None)
-    tp = T.ProposalTrainParams(steps=steps, batch=256, n_projections=200000, bf16=1, lr=1e-3, shift_weight=1.0,
+    tp = T.ProposalTrainParams(steps=steps, batch=256, n_projections=200000, bf16=1, lr=3e-3, shift_weight=1.0,
                                max_shift_px=float(s["w"]["max_shift"]), seed=seed)
     st = T.ProposalTrainStats()
     T.check(lib.t2x_proposal_train(dev, C.byref(cfg), C.byref(model), C.byref(tp), C.byref(st)), "train")
@@ -99,9 +100,11 @@ def test_all_bins_and_all_fallback_reproduce_dense(setup):
         o, wa, bp = run(s, topk=topk, min_conf=conf)
         assert np.array_equal(o["best_hidden"], dense[0]["best_hidden"]), (topk, conf)
         assert np.array_equal(o["dLL"], dense[0]["dLL"])
+        # backprojection accumulates with FP32 atomics (order differs run to run)
         for c in range(len(bp)):
             for q in range(3):
-                assert np.array_equal(bp[c][q], dense[2][c][q])
+                d = np.linalg.norm(bp[c][q] - dense[2][c][q]) / max(np.linalg.norm(dense[2][c][q]), 1e-300)
+                assert d < 1e-6, (c, q, d)
 
 
 def test_trained_proposal_beats_untrained(setup):
@@ -116,8 +119,9 @@ def test_trained_proposal_beats_untrained(setup):
     untrained = run(s, topk=16, min_conf=0.0)[0]
     T.check(lib.t2x_proposal_init(dev, n_bins, 99))
     st = train(s, TRAIN_STEPS)
+    trained = run(s, topk=16, min_conf=0.0)[0]          # no fallback: the network alone
     T.check(lib.t2x_stats_reset(dev))
-    trained = run(s, topk=16, min_conf=0.3)[0]
+    run(s, topk=16, min_conf=0.3)
     stats = T.Stats()
     T.check(lib.t2x_stats_get(dev, C.byref(stats)))
     a_u, a_t = agreement(untrained, exhaustive), agreement(trained, exhaustive)
