@@ -51,7 +51,7 @@ int proposal_infer(Device* d, const float2* Fimg, int cs, int B, int topk, doubl
 }
 int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const float* As, long long lda,
                    const void* Bh, const void* Bl, const float* Bs, long long ldb, int K, int Mrows_pad,
-                   int Nrows_pad, const EpiParams& ep, hipStream_t st, int tile);
+                   int Nrows_pad, const EpiParams& ep, hipStream_t st, int tile, const int* rowmap);
 
 __global__ void k_pass_min(WeightSrc s, int B, SelResult* res);
 __global__ void k_select(WeightSrc s, SelParams prm, int B, SelResult* res);
@@ -66,6 +66,7 @@ __global__ void k_coarse_sum_final(WeightSrc s, int B, const SumPart* part, SelR
 __global__ void k_coarse_candidates(WeightSrc s, int B, SelParams prm, const SelResult* res,
                                     double* cand_w, int* cand_e, int* cand_count);
 size_t sum_part_bytes();
+__global__ void k_log_prior(const double* p, long long n, double* lp);
 __global__ void k_cand_select(WeightSrc s, int B, SelParams prm, SelResult* res, const double* cand_w,
                               const int* cand_e, const int* cand_count, int2* pairs_strided,
                               long long* counts, int* overflow);
@@ -875,6 +876,10 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             prior_o[(size_t)c * Norient + o] = scored ? p : 0.;
         }
     T2X_TRY(upload(d->prior_o, prior_o, st));
+    T2X_TRY(d->log_prior_o.ensure(sizeof(double) * std::max<long>(1, O_all)));
+    k_log_prior<<<(unsigned)std::min<long>((O_all + 255) / 256, 65536), 256, 0, st>>>(d->prior_o.as<double>(), O_all,
+                                                                                  d->log_prior_o.as<double>());
+    T2X_HIP(hipGetLastError());
     // device copies of angle tables and class pointer arrays
     DevBuf d_dirsc, d_psisc, d_ftx, d_fty, d_vols, d_pads, d_bpd, d_bpw, d_bpads, d_bprmax;
     T2X_TRY(upload(d_dirsc, S.sub_dir_sc, st));
@@ -1089,7 +1094,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                                 (int)B_pad, (int)O_pad, ep, st));
         }
         // ---- learned proposal: bins per particle, work lists of the restricted coarse GEMM ----
-        int n_work = 0;
+        int n_work = 0, n_fb_rows = 0;
         if (use_prop) {
             const auto tp0 = std::chrono::steady_clock::now();
             std::vector<int> pbins, pfb;
@@ -1099,10 +1104,12 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             const int mw = (n_bins + 31) / 32;
             std::vector<unsigned> mask((size_t)B * mw, 0u);
             std::vector<std::vector<int>> members(n_bins);
+            std::vector<int> fb_rows;   // exhaustive particles: all their rows through the dense kernel
             long long scored = 0;
             for (int i = 0; i < B; i++) {
                 if (pfb[i]) {
-                    for (int j = 0; j < n_bins; j++) { mask[(size_t)i * mw + (j >> 5)] |= 1u << (j & 31); members[j].push_back(i); }
+                    for (int j = 0; j < mw; j++) mask[(size_t)i * mw + j] = ~0u;
+                    for (long t = 0; t < T; t++) fb_rows.push_back((int)(i * T + t));
                     scored += n_bins;
                     d->stats.s.n_proposal_fallback++;
                 } else {
@@ -1128,7 +1135,9 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             T2X_TRY(upload(d->prop_mask, mask, st));
             T2X_TRY(upload(d->prop_plist, plist, st));
             T2X_TRY(upload(d->prop_work, work, st));
+            T2X_TRY(upload(d->prop_rows, fb_rows, st));
             n_work = (int)work.size();
+            n_fb_rows = (int)fb_rows.size();
             d->stats.s.n_proposal_particles += B;
             d->stats.s.proposal_bins_scored += (double)scored;
             d->stats.s.ms_proposal +=
@@ -1168,15 +1177,23 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                 T2X_TRY(d->As.ensure(sizeof(float) * M_pad));
                 T2X_TRY(launch_split(prec, d->A.as<float>(), Kc, Kc, (int)M_pad, (int)M, d->Ah.ptr, d->Al.ptr, Kc,
                                      d->As.as<float>(), st));
-                if (use_prop)
+                if (use_prop) {
+                    if (n_fb_rows > 0) {
+                        EpiParams ef = ep;
+                        ef.M = n_fb_rows;
+                        T2X_TRY(launch_gemm_x3(prec, EPI_DIFF2, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc,
+                                               d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc,
+                                               (int)round_up(n_fb_rows, 256), (int)O_pad, ef, st, 256,
+                                               d->prop_rows.as<int>()));
+                    }
                     T2X_TRY(launch_gemm_x3_bins(prec, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc, d->coarse_Bh.ptr,
                                                 d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc,
                                                 d->prop_plist.as<int>(), d->prop_work.ptr, n_work, (int)T,
                                                 O_pad - 1, ep, st));
-                else
+                } else
                     T2X_TRY(launch_gemm_x3(prec, EPI_DIFF2, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc,
                                            d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc,
-                                           (int)M_pad, (int)O_pad, ep, st, 0));
+                                           (int)M_pad, (int)O_pad, ep, st, 0, nullptr));
             }
         }
         (void)hipEventRecord(d->ev[ev++], st);   // 3: coarse gemm end
@@ -1196,6 +1213,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         wc.T = (int)T;
         wc.O_all = (int)O_all;
         wc.prior_o = d->prior_o.as<double>();
+        wc.log_prior_o = d->log_prior_o.as<double>();
         wc.prior_t = d->prior_t.as<double>();
         wc.rowmin = d_rowmin.as<float>();
         wc.rowmin_ld = rowmin_ld;

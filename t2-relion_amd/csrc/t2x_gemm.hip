@@ -52,11 +52,13 @@ __device__ __forceinline__ void tile_coords(int pid, int tiles_m, int tiles_n, i
 // multiple of 64). EPI_STORE writes C = acc * row scale * column scale; EPI_DIFF2 writes
 // diff2 = C_b + N1[b][n] - cross (b = row / T, rounded once from FP64) and the min over the
 // 64 columns of each row restricted to scored orientations (prior > 0). Each 16-row block's
-// inputs are gathered before use so their loads overlap.
+// inputs are gathered before use so their loads overlap. With a row map, GEMM row m (< ep.M)
+// is output row rowmap[m].
 template <int MI, int EPI>
 __device__ __forceinline__ void gemm_epilogue(const floatx4 (&acc)[MI][4], long long mbase, long long ncol0,
                                               int lane, const float* __restrict__ a_scale,
-                                              const float* __restrict__ b_scale, const EpiParams& ep)
+                                              const float* __restrict__ b_scale, const EpiParams& ep,
+                                              const int* __restrict__ rowmap = nullptr)
 {
     const int col_in = lane & 15, row_grp = (lane >> 4) * 4;
     float sb[4];
@@ -80,7 +82,7 @@ __device__ __forceinline__ void gemm_epilogue(const floatx4 (&acc)[MI][4], long 
         for (int r = 0; r < 4; r++) {
             const long long m = mbase + i * 16 + row_grp + r;
             mval[r] = m < ep.M;
-            const int mm = mval[r] ? (int)m : 0;
+            const int mm = mval[r] ? (rowmap ? rowmap[m] : (int)m) : 0;   // output row
             mrow[r] = mm;
             sa[r] = a_scale ? a_scale[mm] : 1.f;
             if (EPI == EPI_DIFF2) {
@@ -347,7 +349,8 @@ __device__ __forceinline__ int swz(int r, int c) { return r * 32 + ((c ^ ((r >> 
 
 template <class H, int EPI>
 __global__ __launch_bounds__(512) void k_gemm_x3w(SplitOperand A, SplitOperand B, int K, int tiles_m,
-                                                  int tiles_n, int group_m, EpiParams ep)
+                                                  int tiles_n, int group_m, EpiParams ep,
+                                                  const int* __restrict__ rowmap)
 {
     typedef typename Vec8<H>::type v8;
     // two LDS stages (2 x 64 KiB): the tile slices for step k+1 are stored while step k's MFMAs
@@ -371,7 +374,10 @@ __global__ __launch_bounds__(512) void k_gemm_x3w(SplitOperand A, SplitOperand B
     const H* Al = reinterpret_cast<const H*>(A.lo);
     const H* Bh = reinterpret_cast<const H*>(B.hi);
     const H* Bl = reinterpret_cast<const H*>(B.lo);
-    const size_t ra0 = (size_t)(m0 + lrow) * A.ld + lc * 8, ra1 = (size_t)(m0 + lrow + 128) * A.ld + lc * 8;
+    // A rows: through the row map when given (rows beyond ep.M read row 0, their output is dropped)
+    const long long am0 = rowmap ? ((m0 + lrow < ep.M) ? rowmap[m0 + lrow] : 0) : m0 + lrow;
+    const long long am1 = rowmap ? ((m0 + lrow + 128 < ep.M) ? rowmap[m0 + lrow + 128] : 0) : m0 + lrow + 128;
+    const size_t ra0 = (size_t)am0 * A.ld + lc * 8, ra1 = (size_t)am1 * A.ld + lc * 8;
     const size_t rb0 = (size_t)(n0 + lrow) * B.ld + lc * 8, rb1 = (size_t)(n0 + lrow + 128) * B.ld + lc * 8;
     const int s0 = swz(lrow, lc), s1 = swz(lrow + 128, lc);
     uint4 p0, p1, p2, p3, p4, p5, p6, p7;
@@ -438,7 +444,7 @@ __global__ __launch_bounds__(512) void k_gemm_x3w(SplitOperand A, SplitOperand B
 
 #undef X3W_GLOAD
 #undef X3W_LSTORE
-    gemm_epilogue<8, EPI>(acc, m0 + wm * 128, n0 + wn * 64, lane, A.inv_scale, B.inv_scale, ep);
+    gemm_epilogue<8, EPI>(acc, m0 + wm * 128, n0 + wn * 64, lane, A.inv_scale, B.inv_scale, ep, rowmap);
 }
 
 // Direct-load variant: 256x128 tile per workgroup of 4 waves (each 128x64), operand slices
@@ -727,9 +733,9 @@ int gemm_x3_tile(int requested)
 
 int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const float* As, long long lda,
                    const void* Bh, const void* Bl, const float* Bs, long long ldb, int K, int Mrows_pad,
-                   int Nrows_pad, const EpiParams& ep, hipStream_t st, int tile)
+                   int Nrows_pad, const EpiParams& ep, hipStream_t st, int tile, const int* rowmap)
 {
-    tile = gemm_x3_tile(tile);
+    tile = rowmap ? 256 : gemm_x3_tile(tile);   // row-mapped launches use the 256x256 kernel
     const int tile_m = tile == 257 ? DBM : tile, tile_n = tile == 257 ? DBN : tile;
     if (K % XK != 0 || Mrows_pad % tile_m != 0 || Nrows_pad % tile_n != 0) {
         set_error("gemm_x3: operands not padded to tile multiples");
@@ -752,11 +758,11 @@ int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const floa
     } else if (tile == 256) {
         const int gw = std::max(1, g / 2);
         if (prec == T2X_PREC_BF16X3) {
-            if (epi == EPI_STORE) k_gemm_x3w<__bf16, EPI_STORE><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep);
-            else k_gemm_x3w<__bf16, EPI_DIFF2><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep);
+            if (epi == EPI_STORE) k_gemm_x3w<__bf16, EPI_STORE><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep, rowmap);
+            else k_gemm_x3w<__bf16, EPI_DIFF2><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep, rowmap);
         } else {
-            if (epi == EPI_STORE) k_gemm_x3w<_Float16, EPI_STORE><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep);
-            else k_gemm_x3w<_Float16, EPI_DIFF2><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep);
+            if (epi == EPI_STORE) k_gemm_x3w<_Float16, EPI_STORE><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep, rowmap);
+            else k_gemm_x3w<_Float16, EPI_DIFF2><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep, rowmap);
         }
     } else if (prec == T2X_PREC_BF16X3) {
         if (epi == EPI_STORE) k_gemm_x3<__bf16, EPI_STORE><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, g, ep);

@@ -38,6 +38,7 @@ struct WeightSrc {
     int T;                     // coarse translations
     int O_all;                 // coarse orientations (all classes)
     const double* prior_o;     // [O_all]
+    const double* log_prior_o; // [O_all] log(prior_o) (-inf where 0), coarse candidates only
     const double* prior_t;     // [B][T]
     const float* rowmin;       // coarse: [B*T][ld/64] per-row minima from the GEMM
     int rowmin_ld;

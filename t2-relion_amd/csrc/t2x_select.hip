@@ -550,6 +550,12 @@ struct SumPart {
 
 size_t sum_part_bytes() { return sizeof(SumPart); }
 
+__global__ void k_log_prior(const double* __restrict__ p, long long n, double* __restrict__ lp)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        lp[i] = p[i] > 0. ? log(p[i]) : -INFINITY;
+}
+
 // One workgroup per (translation row, particle): partial sum, max and counts of the row.
 __global__ __launch_bounds__(256) void k_coarse_sum_rows(WeightSrc s, int B, const SelResult* res, SumPart* part)
 {
@@ -636,7 +642,10 @@ __global__ __launch_bounds__(256) void k_coarse_candidates(WeightSrc s, int B, S
     const double pt = s.prior_t[(size_t)b * s.T + t];
     const float* row = s.d2 + ((size_t)b * s.T + t) * s.ld;
     const int lane = threadIdx.x & 63;
-    if (!(pt > 0.)) return;
+    if (!(pt > 0.) || !(thr > 0.)) return;
+    // w >= thr  <=>  dd <= log(prior_o) + log(pt) - log(thr): compared in the log domain (relaxed by
+    // 1e-9, far below the factor-2 margin of thr), exact test on the few that pass
+    const double lbias = log(pt) - log(thr) + 1e-9;
     for (int base = 0; base < s.O_all; base += blockDim.x * 4) {
         const int n4 = base + threadIdx.x * 4;
         float dv[4] = {3.0e38f, 3.0e38f, 3.0e38f, 3.0e38f};
@@ -650,11 +659,11 @@ __global__ __launch_bounds__(256) void k_coarse_candidates(WeightSrc s, int B, S
             bool keep = false;
             double w = 0.;
             if (n < s.O_all) {
-                const double po = s.prior_o[n];
                 const double dd = (double)dv[c] - mind2;
-                if (po > 0. && dd <= kTinyDD && coarse_allowed(s, b, n)) {
+                if (dd <= kTinyDD && dd <= s.log_prior_o[n] + lbias && coarse_allowed(s, b, n)) {
+                    const double po = s.prior_o[n];
                     w = (po * pt) * exp(-dd);
-                    keep = (w > 0.) && (w >= thr);
+                    keep = (po > 0.) && (w > 0.) && (w >= thr);
                 }
             }
             const unsigned long long m = __ballot(keep);

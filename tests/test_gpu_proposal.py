@@ -23,7 +23,7 @@ from t2x_py import workloads as W  # noqa: E402
 
 pytestmark = pytest.mark.gpu
 
-TRAIN_STEPS = int(os.environ.get("T2X_TEST_PROPOSAL_STEPS", "300"))
+TRAIN_STEPS = int(os.environ.get("T2X_TEST_PROPOSAL_STEPS", "2000"))
 
 
 @pytest.fixture(scope="module")
