@@ -31,6 +31,7 @@ struct Operand {
     const float* p;
     long long sr, sk;                 // OP_STRIDED: p[r*sr + k*sk]
     int H, W, C, Ho, Wo;              // im2col: activation [Bn][H][W][C], output grid Ho x Wo
+    int ones_r;                       // >= 0: row ones_r reads 1 (bias gradients as an extra column)
 };
 
 struct GemmArgs {
@@ -39,6 +40,8 @@ struct GemmArgs {
     float* c;
     long long ldc;
     const float* bias;
+    int extra_col;                    // >= 0: output column extra_col goes to c_extra[m] instead
+    float* c_extra;
 };
 
 // Operand tile loads. Each thread fills 8 entries of a 64 x 32 (row x k) tile:
@@ -61,7 +64,10 @@ __device__ __forceinline__ void load_tile(const Operand& o, int R, int K, int r0
         const int lr = tid >> 2, lk = (tid & 3) * 8;
         const int r = r0 + lr;
         float v[8];
-        if (o.mode == OP_STRIDED) {
+        if (r == o.ones_r) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) v[q] = (k0 + lk + q < K) ? 1.f : 0.f;
+        } else if (o.mode == OP_STRIDED) {
             const float* p = o.p + (long long)r * o.sr;
 #pragma unroll
             for (int q = 0; q < 8; q++) {
@@ -92,7 +98,10 @@ __device__ __forceinline__ void load_tile(const Operand& o, int R, int K, int r0
     } else {
         const int lr = tid & 63, kc = (tid >> 6) * 8;
         const int r = r0 + lr;
-        if (o.mode == OP_STRIDED) {
+        if (r == o.ones_r) {
+#pragma unroll
+            for (int q = 0; q < 8; q++) S[lr][kc + q] = (k0 + kc + q < K) ? 1.f : 0.f;
+        } else if (o.mode == OP_STRIDED) {
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 const int k = k0 + kc + q;
@@ -188,20 +197,22 @@ __global__ __launch_bounds__(256) void k_cnn_gemm(GemmArgs g, int k_chunk, long 
                 float v = acc[i][j][r];
                 if (EPI != EPI_STORE) v += g.bias[n];
                 if (EPI == EPI_BIAS_RELU) v = fmaxf(v, 0.f);
-                c[(long long)m * g.ldc + n] = v;
+                if (split_stride == 0 && n == g.extra_col) g.c_extra[m] = v;
+                else c[(long long)m * (split_stride ? g.N : g.ldc) + n] = v;
             }
 }
 
 // out[m][n] = sum over the split-K partials in order
 __global__ void k_splitk_reduce(const float* __restrict__ part, int S, long long stride, int M, int N, long long ldc,
-                                float* __restrict__ out)
+                                float* __restrict__ out, int extra_col, float* __restrict__ out_extra)
 {
     const long long n_el = (long long)M * N;
     for (long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x; e < n_el; e += (long long)gridDim.x * blockDim.x) {
         const long long m = e / N, n = e - m * N;
         float s = 0.f;
         for (int z = 0; z < S; z++) s += part[z * stride + m * N + n];
-        out[m * ldc + n] = s;
+        if (n == extra_col) out_extra[m] = s;
+        else out[m * ldc + n] = s;
     }
 }
 
@@ -211,35 +222,6 @@ __global__ void k_relu_bwd(const float* __restrict__ da, const float* __restrict
 {
     for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
         dz[i] = out[i] > 0.f ? da[i] : 0.f;
-}
-
-// column sums of x [M][N] in a fixed order: partial[s][n] over row slabs, then the slabs
-#define CS_SLABS 64
-__global__ void k_colsum_partial(const float* __restrict__ x, long long M, int N, float* __restrict__ part)
-{
-    const int n = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int sub = threadIdx.x >> 6;   // 4 row phases
-    const int slab = blockIdx.y;
-    const long long per = (M + CS_SLABS - 1) / CS_SLABS;
-    const long long m_begin = slab * per, m_end = min(M, m_begin + per);
-    double s = 0.;
-    if (n < N)
-        for (long long m = m_begin + sub; m < m_end; m += 4) s += x[m * N + n];
-    __shared__ double red[4][64];
-    red[sub][threadIdx.x & 63] = s;
-    __syncthreads();
-    if (sub == 0 && n < N)
-        part[(size_t)slab * N + n] = (float)(red[0][threadIdx.x] + red[1][threadIdx.x] + red[2][threadIdx.x] +
-                                             red[3][threadIdx.x]);
-}
-
-__global__ void k_colsum_final(const float* __restrict__ part, int N, float* __restrict__ out)
-{
-    const int n = blockIdx.x * blockDim.x + threadIdx.x;
-    if (n >= N) return;
-    double s = 0.;
-    for (int slab = 0; slab < CS_SLABS; slab++) s += part[(size_t)slab * N + n];
-    out[n] = (float)s;
 }
 
 // dact[b][yi][xi][ci] = sum over the (<= 4) output pixels whose 3x3 window covers (yi, xi)
@@ -383,6 +365,7 @@ int gemm(bool bf16, int epi, const GemmArgs& a, hipStream_t st, DevBuf* scratch)
         T2X_TRY(scratch->ensure(sizeof(float) * (size_t)stride * S));
         g.c = scratch->as<float>();
         g.ldc = a.N;
+        g.extra_col = -1;   // the reduction routes the extra column
     }
     const dim3 grid(gx, gy, (unsigned)S);
     if (bf16) {
@@ -396,15 +379,17 @@ int gemm(bool bf16, int epi, const GemmArgs& a, hipStream_t st, DevBuf* scratch)
     }
     T2X_HIP(hipGetLastError());
     if (S > 1) {
-        k_splitk_reduce<<<grid_for(stride), 256, 0, st>>>(scratch->as<float>(), S, stride, a.M, a.N, a.ldc, a.c);
+        k_splitk_reduce<<<grid_for(stride), 256, 0, st>>>(scratch->as<float>(), S, stride, a.M, a.N, a.ldc, a.c,
+                                                          a.extra_col, a.c_extra);
         T2X_HIP(hipGetLastError());
     }
     return T2X_OK;
 }
 
-static Operand strided(const float* p, long long sr, long long sk)
+static Operand strided(const float* p, long long sr, long long sk, int ones_r = -1)
 {
     Operand o{};
+    o.ones_r = ones_r;
     o.mode = OP_STRIDED;
     o.p = p;
     o.sr = sr;
@@ -412,9 +397,10 @@ static Operand strided(const float* p, long long sr, long long sk)
     return o;
 }
 
-static Operand im2col_view(int mode, const float* act, int H, int W, int C)
+static Operand im2col_view(int mode, const float* act, int H, int W, int C, int ones_r = -1)
 {
     Operand o{};
+    o.ones_r = ones_r;
     o.mode = mode;
     o.p = act;
     o.H = H;
@@ -465,7 +451,6 @@ int Net::ensure_batch(int B)
     T2X_TRY(dz.ensure(sizeof(float) * (size_t)B * (kIn / 2) * (kIn / 2) * kChannels[1]));
     T2X_TRY(y.ensure(sizeof(float) * (size_t)B * (n_bins + 2)));
     T2X_TRY(dy.ensure(sizeof(float) * (size_t)B * (n_bins + 2)));
-    T2X_TRY(part.ensure(sizeof(float) * (size_t)CS_SLABS * std::max(n_bins + 2, kChannels[4])));
     T2X_TRY(loss.ensure(sizeof(double) * B));
     T2X_TRY(correct.ensure(sizeof(int) * B));
     T2X_TRY(rank.ensure(sizeof(int) * B));
@@ -482,6 +467,7 @@ int Net::forward(int B, bool bf16, hipStream_t st)
     for (int l = 0; l < 4; l++) {
         const int H = kIn >> l;
         GemmArgs g{};
+        g.extra_col = -1;
         g.M = B * (H / 2) * (H / 2);
         g.N = kChannels[l + 1];
         g.K = 9 * kChannels[l];
@@ -493,6 +479,7 @@ int Net::forward(int B, bool bf16, hipStream_t st)
         T2X_TRY(gemm(bf16, EPI_BIAS_RELU, g, st, nullptr));
     }
     GemmArgs h{};
+    h.extra_col = -1;
     h.M = B;
     h.N = n_bins + 2;
     h.K = kFeat;
@@ -504,13 +491,6 @@ int Net::forward(int B, bool bf16, hipStream_t st)
     return gemm(bf16, EPI_BIAS, h, st, nullptr);
 }
 
-static int colsum(const float* x, long long M, int N, float* part, float* out, hipStream_t st)
-{
-    k_colsum_partial<<<dim3((unsigned)((N + 63) / 64), CS_SLABS), 256, 0, st>>>(x, M, N, part);
-    k_colsum_final<<<(unsigned)((N + 255) / 256), 256, 0, st>>>(part, N, out);
-    T2X_HIP(hipGetLastError());
-    return T2X_OK;
-}
 
 // dy must hold dloss/dy; fills grad (n_params floats)
 int Net::backward(int B, bool bf16, hipStream_t st)
@@ -520,17 +500,19 @@ int Net::backward(int B, bool bf16, hipStream_t st)
     const int ldy = n_bins + 2;
     // head
     {
-        GemmArgs g{};   // dWh[o][k] = sum_b dy[b][o] flat[b][k]
+        GemmArgs g{};   // dWh[o][k] = sum_b dy[b][o] flat[b][k]; column kFeat (ones) gives dbh
         g.M = ldy;
-        g.N = kFeat;
+        g.N = kFeat + 1;
         g.K = B;
         g.a = strided(dy.as<float>(), 1, ldy);
-        g.b = strided(act[4].as<float>(), 1, kFeat);
+        g.b = strided(act[4].as<float>(), 1, kFeat, kFeat);
         g.c = G + wh_off;
         g.ldc = kFeat;
+        g.extra_col = kFeat;
+        g.c_extra = G + bh_off;
         T2X_TRY(gemm(bf16, EPI_STORE, g, st, &splitk));
-        T2X_TRY(colsum(dy.as<float>(), B, ldy, part.as<float>(), G + bh_off, st));
         GemmArgs d{};   // dflat[b][k] = sum_o dy[b][o] Wh[o][k]
+        d.extra_col = -1;
         d.M = B;
         d.N = kFeat;
         d.K = ldy;
@@ -547,18 +529,20 @@ int Net::backward(int B, bool bf16, hipStream_t st)
         float* dzp = dz.as<float>();
         k_relu_bwd<<<grid_for(M * Co), 256, 0, st>>>(dact[l + 1].as<float>(), act[l + 1].as<float>(), dzp, M * Co);
         T2X_HIP(hipGetLastError());
-        GemmArgs g{};   // dW[co][q] = sum_p dz[p][co] im2col(act_l)[p][q]
+        GemmArgs g{};   // dW[co][q] = sum_p dz[p][co] im2col(act_l)[p][q]; column K (ones) gives db
         g.M = Co;
-        g.N = K;
+        g.N = K + 1;
         g.K = (int)M;
         g.a = strided(dzp, 1, Co);
-        g.b = im2col_view(OP_IM2COL_T, act[l].as<float>(), H, H, Ci);
+        g.b = im2col_view(OP_IM2COL_T, act[l].as<float>(), H, H, Ci, K);
         g.c = G + w_off[l];
         g.ldc = K;
+        g.extra_col = K;
+        g.c_extra = G + b_off[l];
         T2X_TRY(gemm(bf16, EPI_STORE, g, st, &splitk));
-        T2X_TRY(colsum(dzp, M, Co, part.as<float>(), G + b_off[l], st));
         if (l > 0) {
             GemmArgs d{};   // dcols[p][q] = sum_co dz[p][co] W[co][q]
+            d.extra_col = -1;
             d.M = (int)M;
             d.N = K;
             d.K = Co;

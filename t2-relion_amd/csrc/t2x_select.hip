@@ -556,59 +556,70 @@ __global__ void k_log_prior(const double* __restrict__ p, long long n, double* _
         lp[i] = p[i] > 0. ? log(p[i]) : -INFINITY;
 }
 
-// One workgroup per (translation row, particle): partial sum, max and counts of the row.
+// One workgroup per (translation row, particle): partial sum, max and counts of the row. The
+// GEMM epilogue left the minimum of every 64-column block of the row (scored columns only, +inf
+// for blocks not computed); a block whose minimum is more than kTinyDD above the particle's
+// minimum holds only negligible weights and is skipped without reading it. Each wave takes 64
+// blocks at a time, one lane per block minimum, and walks the surviving blocks with one lane
+// per column.
 __global__ __launch_bounds__(256) void k_coarse_sum_rows(WeightSrc s, int B, const SelResult* res, SumPart* part)
 {
     const int t = blockIdx.x, b = blockIdx.y;
     if (b >= B || t >= s.T) return;
     const double mind2 = res[b].min_d2;
     const double pt = s.prior_t[(size_t)b * s.T + t];
-    const float* row = s.d2 + ((size_t)b * s.T + t) * s.ld;
+    const size_t row_id = (size_t)b * s.T + t;
+    const float* row = s.d2 + row_id * s.ld;
+    const float* rmin = s.rowmin + row_id * s.rowmin_ld;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
+    const int nblk = (s.O_all + 63) / 64;
     double sum = 0., mx = 0.;
-    long long npos = 0, nbound = 0;
+    long long npos = 0;
     if (pt > 0.) {
-        for (int n4 = threadIdx.x * 4; n4 < s.O_all; n4 += blockDim.x * 4) {
-            const float4 v = *reinterpret_cast<const float4*>(row + n4);   // rows padded to 256
-            const float dv[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-            for (int c = 0; c < 4; c++) {
-                const int n = n4 + c;
-                if (n >= s.O_all) break;
-                const double po = s.prior_o[n];
-                if (!(po > 0.) || !coarse_allowed(s, b, n)) continue;
-                const double dd = (double)dv[c] - mind2;
-                if (dd > 700.) continue;
-                npos++;
-                if (dd > kTinyDD) continue;
-                const double w = (po * pt) * exp(-dd);
-                if (!(w > 0.)) continue;
-                nbound++;
-                sum += w;
-                mx = fmax(mx, w);
+        for (int j0 = wave * 64; j0 < nblk; j0 += nw * 64) {
+            const int j = j0 + lane;
+            const bool live = j < nblk && (double)rmin[j] - mind2 <= kTinyDD;
+            unsigned long long m = __ballot(live);
+            while (m) {
+                const int jb = j0 + __ffsll((long long)m) - 1;
+                m &= m - 1;
+                const int n = jb * 64 + lane;
+                if (n < s.O_all) {
+                    const double po = s.prior_o[n];
+                    const double dd = (double)row[n] - mind2;
+                    if (po > 0. && dd <= kTinyDD && coarse_allowed(s, b, n)) {
+                        const double w = (po * pt) * exp(-dd);
+                        if (w > 0.) {
+                            npos++;
+                            sum += w;
+                            mx = fmax(mx, w);
+                        }
+                    }
+                }
             }
         }
     }
     sum = wave_sum(sum);
     npos = (long long)wave_sum_u64((unsigned long long)npos);
-    nbound = (long long)wave_sum_u64((unsigned long long)nbound);
     for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o));
     __shared__ SumPart red[4];
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = SumPart{sum, mx, npos, nbound};
+    if (lane == 0) red[wave] = SumPart{sum, mx, npos, npos};
     __syncthreads();
     if (threadIdx.x == 0) {
         SumPart r{0., 0., 0, 0};
-        for (int i = 0; i < (int)(blockDim.x >> 6); i++) {
+        for (int i = 0; i < nw; i++) {
             r.sum += red[i].sum;
             r.max = fmax(r.max, red[i].max);
             r.npos += red[i].npos;
             r.nbound += red[i].nbound;
         }
-        part[(size_t)b * s.T + t] = r;
+        part[row_id] = r;
     }
 }
 
-// Fixed-order combination of the row partials (deterministic S). npos counts the weights that
-// are positive; nbound (kept in nsig until the walk overwrites it) the non-negligible ones.
+// Fixed-order combination of the row partials (deterministic S). npos and nbound (kept in nsig
+// until the walk overwrites it) count the positive non-negligible weights (diff2 - min <= kTinyDD);
+// npos only serves the "no positive weight" check, for which the minimum itself always counts.
 __global__ void k_coarse_sum_final(WeightSrc s, int B, const SumPart* part, SelResult* res)
 {
     const int b = blockIdx.x;
@@ -640,26 +651,27 @@ __global__ __launch_bounds__(256) void k_coarse_candidates(WeightSrc s, int B, S
     const long long nb = res[b].nsig;
     const double thr = (nb > 0) ? 0.5 * (1.0 - prm.fraction) * res[b].sum_w / (double)nb : 0.;
     const double pt = s.prior_t[(size_t)b * s.T + t];
-    const float* row = s.d2 + ((size_t)b * s.T + t) * s.ld;
-    const int lane = threadIdx.x & 63;
+    const size_t row_id = (size_t)b * s.T + t;
+    const float* row = s.d2 + row_id * s.ld;
+    const float* rmin = s.rowmin + row_id * s.rowmin_ld;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     if (!(pt > 0.) || !(thr > 0.)) return;
     // w >= thr  <=>  dd <= log(prior_o) + log(pt) - log(thr): compared in the log domain (relaxed by
     // 1e-9, far below the factor-2 margin of thr), exact test on the few that pass
     const double lbias = log(pt) - log(thr) + 1e-9;
-    for (int base = 0; base < s.O_all; base += blockDim.x * 4) {
-        const int n4 = base + threadIdx.x * 4;
-        float dv[4] = {3.0e38f, 3.0e38f, 3.0e38f, 3.0e38f};
-        if (n4 < s.O_all) {
-            const float4 v = *reinterpret_cast<const float4*>(row + n4);
-            dv[0] = v.x; dv[1] = v.y; dv[2] = v.z; dv[3] = v.w;
-        }
-#pragma unroll
-        for (int c = 0; c < 4; c++) {
-            const int n = n4 + c;
+    const int nblk = (s.O_all + 63) / 64;
+    for (int j0 = wave * 64; j0 < nblk; j0 += nw * 64) {
+        const int j = j0 + lane;
+        const bool live = j < nblk && (double)rmin[j] - mind2 <= kTinyDD;
+        unsigned long long mb = __ballot(live);
+        while (mb) {
+            const int jb = j0 + __ffsll((long long)mb) - 1;
+            mb &= mb - 1;
+            const int n = jb * 64 + lane;
             bool keep = false;
             double w = 0.;
             if (n < s.O_all) {
-                const double dd = (double)dv[c] - mind2;
+                const double dd = (double)row[n] - mind2;
                 if (dd <= kTinyDD && dd <= s.log_prior_o[n] + lbias && coarse_allowed(s, b, n)) {
                     const double po = s.prior_o[n];
                     w = (po * pt) * exp(-dd);
