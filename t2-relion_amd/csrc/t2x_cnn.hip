@@ -58,7 +58,8 @@ __device__ __forceinline__ bool row_major_form(const Operand& o)
     return o.mode == OP_IM2COL || (o.mode == OP_STRIDED && o.sk == 1);
 }
 
-__device__ __forceinline__ void load_tile(const Operand& o, int R, int K, int r0, int k0, float (*S)[CLD], int tid)
+template <class LT, int LD>
+__device__ __forceinline__ void load_tile(const Operand& o, int R, int K, int r0, int k0, LT (*S)[LD], int tid)
 {
     if (row_major_form(o)) {
         const int lr = tid >> 2, lk = (tid & 3) * 8;
@@ -93,19 +94,26 @@ __device__ __forceinline__ void load_tile(const Operand& o, int R, int K, int r0
                 if (++ci == o.C) { ci = 0; kk++; }
             }
         }
+        if constexpr (sizeof(LT) == 2) {
+            bf16x8 pk;
 #pragma unroll
-        for (int q = 0; q < 8; q++) S[lr][lk + q] = v[q];
+            for (int q = 0; q < 8; q++) pk[q] = (__bf16)v[q];
+            *reinterpret_cast<bf16x8*>(&S[lr][lk]) = pk;   // one 16-byte store
+        } else {
+#pragma unroll
+            for (int q = 0; q < 8; q++) S[lr][lk + q] = v[q];
+        }
     } else {
         const int lr = tid & 63, kc = (tid >> 6) * 8;
         const int r = r0 + lr;
         if (r == o.ones_r) {
 #pragma unroll
-            for (int q = 0; q < 8; q++) S[lr][kc + q] = (k0 + kc + q < K) ? 1.f : 0.f;
+            for (int q = 0; q < 8; q++) S[lr][kc + q] = (LT)((k0 + kc + q < K) ? 1.f : 0.f);
         } else if (o.mode == OP_STRIDED) {
 #pragma unroll
             for (int q = 0; q < 8; q++) {
                 const int k = k0 + kc + q;
-                S[lr][kc + q] = (r < R && k < K) ? o.p[(long long)r * o.sr + (long long)k * o.sk] : 0.f;
+                S[lr][kc + q] = (LT)((r < R && k < K) ? o.p[(long long)r * o.sr + (long long)k * o.sk] : 0.f);
             }
         } else {
             // im2col transposed: r = column (ky, kx, ci), k = pixel (b, y, x)
@@ -120,7 +128,7 @@ __device__ __forceinline__ void load_tile(const Operand& o, int R, int K, int r0
                     if (yy >= 0 && yy < o.H && xx >= 0 && xx < o.W)
                         val = o.p[(((long long)b * o.H + yy) * o.W + xx) * o.C + ci];
                 }
-                S[lr][kc + q] = val;
+                S[lr][kc + q] = (LT)val;
                 k++;
                 if (++x == o.Wo) { x = 0; if (++y == o.Ho) { y = 0; b++; } }
             }
@@ -129,11 +137,18 @@ __device__ __forceinline__ void load_tile(const Operand& o, int R, int K, int r0
 }
 
 // C (+ blockIdx.z * split_stride for split-K partials) = A(m, k) B(n, k) over k in this block's range
+// LDS tiles hold BF16 (row stride 40: 16-byte aligned fragment reads) or FP32 (stride 33)
+template <bool BF16> struct LdsT;
+template <> struct LdsT<true> { typedef __bf16 type; static constexpr int ld = 40; };
+template <> struct LdsT<false> { typedef float type; static constexpr int ld = CLD; };
+
 template <bool BF16, int EPI>
 __global__ __launch_bounds__(256) void k_cnn_gemm(GemmArgs g, int k_chunk, long long split_stride)
 {
-    __shared__ float As[CT][CLD];
-    __shared__ float Bs[CT][CLD];
+    typedef typename LdsT<BF16>::type LT;
+    constexpr int LD = LdsT<BF16>::ld;
+    __shared__ __attribute__((aligned(16))) LT As[CT][LD];
+    __shared__ __attribute__((aligned(16))) LT Bs[CT][LD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
     const int m0 = blockIdx.x * CT, n0 = blockIdx.y * CT;
@@ -146,21 +161,13 @@ __global__ __launch_bounds__(256) void k_cnn_gemm(GemmArgs g, int k_chunk, long 
         load_tile(g.a, g.M, ke, m0, k0, As, tid);
         load_tile(g.b, g.N, ke, n0, k0, Bs, tid);
         __syncthreads();
-        if (BF16) {
+        if constexpr (BF16) {
             const int kc = (lane >> 4) * 8;
             bf16x8 a[2], b[2];
 #pragma unroll
-            for (int i = 0; i < 2; i++) {
-                const float* s = &As[wm * 32 + i * 16 + (lane & 15)][kc];
+            for (int i = 0; i < 2; i++) a[i] = *reinterpret_cast<const bf16x8*>(&As[wm * 32 + i * 16 + (lane & 15)][kc]);
 #pragma unroll
-                for (int e = 0; e < 8; e++) a[i][e] = (__bf16)s[e];
-            }
-#pragma unroll
-            for (int j = 0; j < 2; j++) {
-                const float* s = &Bs[wn * 32 + j * 16 + (lane & 15)][kc];
-#pragma unroll
-                for (int e = 0; e < 8; e++) b[j][e] = (__bf16)s[e];
-            }
+            for (int j = 0; j < 2; j++) b[j] = *reinterpret_cast<const bf16x8*>(&Bs[wn * 32 + j * 16 + (lane & 15)][kc]);
 #pragma unroll
             for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -170,7 +177,7 @@ __global__ __launch_bounds__(256) void k_cnn_gemm(GemmArgs g, int k_chunk, long 
 #pragma unroll
             for (int s = 0; s < CK / 4; s++) {
                 const int k = 4 * s + (lane >> 4);
-                float a[2], b[2];
+                float a[2], b[2];   // FP32 operand mode
 #pragma unroll
                 for (int i = 0; i < 2; i++) a[i] = As[wm * 32 + i * 16 + (lane & 15)][k];
 #pragma unroll

@@ -23,18 +23,6 @@
 namespace t2x {
 
 enum { EPI_STORE = 0, EPI_DIFF2 = 1 };
-struct EpiParams {
-    float* C;
-    long long ldc;
-    int M, N;
-    int T;
-    const double* Cb;
-    const float* N1;
-    long long ldn;
-    const double* prior_o;
-    float* rowmin;
-    int rowmin_ld;
-};
 int launch_gemm(int epi, const float* A, long long lda, const float* B, long long ldb, int K,
                 int Mrows_pad, int Nrows_pad, const EpiParams& ep, hipStream_t st);
 int launch_split(int prec, const float* X, long long ldx, int K, int rows, int valid_rows, void* hi,
@@ -1165,6 +1153,11 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             ep.prior_o = d->prior_o.as<double>();
             ep.rowmin = d_rowmin.as<float>();
             ep.rowmin_ld = rowmin_ld;
+            // running per-particle minima for the lazy stores of negligible blocks (all values are
+            // stored when diagnostic copies of the coarse diff2 are requested)
+            T2X_TRY(d->pmin.ensure(sizeof(unsigned) * B));
+            k_fill_u32<<<(unsigned)((B + 255) / 256), 256, 0, st>>>(d->pmin.as<unsigned>(), 0xFF800000u, B);  // +inf
+            ep.pmin = (dbg && dbg->n_debug > 0) ? nullptr : d->pmin.as<unsigned>();
             // rows beyond the valid columns of the last tile keep +inf
             k_fill_f32<<<(unsigned)((M_pad * rowmin_ld + 255) / 256), 256, 0, st>>>(d_rowmin.as<float>(), 3.0e38f,
                                                                                     M_pad * rowmin_ld);

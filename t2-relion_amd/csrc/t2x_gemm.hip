@@ -13,19 +13,6 @@ namespace t2x {
 
 enum { EPI_STORE = 0, EPI_DIFF2 = 1 };
 
-struct EpiParams {
-    float* C;            // output
-    long long ldc;
-    int M, N;            // valid extent
-    // EPI_DIFF2
-    int T;               // translations per particle (rows per particle)
-    const double* Cb;    // [B] constant term per particle (read as double)
-    const float* N1;     // [B][ldn]
-    long long ldn;
-    const double* prior_o;  // [N] orientation prior (0 = not scored)
-    float* rowmin;       // [M][N_pad/64]
-    int rowmin_ld;
-};
 
 #define GBM 128
 #define GBN 128
@@ -74,7 +61,7 @@ __device__ __forceinline__ void gemm_epilogue(const floatx4 (&acc)[MI][4], long 
     const float invT = (EPI == EPI_DIFF2) ? 1.f / (float)ep.T : 0.f;
 #pragma unroll
     for (int i = 0; i < MI; i++) {
-        int mrow[4];
+        int mrow[4], pb[4];
         bool mval[4];
         float sa[4], n1[4][4];
         double cb[4];
@@ -89,6 +76,7 @@ __device__ __forceinline__ void gemm_epilogue(const floatx4 (&acc)[MI][4], long 
                 int b = (int)((float)mm * invT);   // particle index m / T: float estimate, then exact
                 while (b > 0 && b * ep.T > mm) b--;
                 while ((b + 1) * ep.T <= mm) b++;
+                pb[r] = b;
                 cb[r] = ep.Cb[b];
                 const float* n1row = ep.N1 + (long long)b * ep.ldn;
 #pragma unroll
@@ -99,21 +87,32 @@ __device__ __forceinline__ void gemm_epilogue(const floatx4 (&acc)[MI][4], long 
         for (int r = 0; r < 4; r++) {
             float rmin = 3.0e38f;
             float* crow = ep.C + (long long)mrow[r] * ep.ldc;
+            if (EPI == EPI_STORE) {
 #pragma unroll
-            for (int j = 0; j < 4; j++) {
-                if (mval[r] && nval[j]) {
+                for (int j = 0; j < 4; j++)
+                    if (mval[r] && nval[j]) crow[ncol[j]] = acc[i][j][r] * sa[r] * sb[j];
+            } else {
+                float dv[4];
+#pragma unroll
+                for (int j = 0; j < 4; j++) {
                     const float v = acc[i][j][r] * sa[r] * sb[j];
-                    if (EPI == EPI_STORE) {
-                        crow[ncol[j]] = v;
-                    } else {
-                        const float d = (float)(cb[r] + (double)n1[r][j] - (double)v);
-                        crow[ncol[j]] = d;
-                        if (scored[j]) rmin = fminf(rmin, d);
-                    }
+                    dv[j] = (float)(cb[r] + (double)n1[r][j] - (double)v);
+                    if (mval[r] && nval[j] && scored[j]) rmin = fminf(rmin, dv[j]);
                 }
-            }
-            if (EPI == EPI_DIFF2) {
                 for (int o = 1; o < 16; o <<= 1) rmin = fminf(rmin, __shfl_xor(rmin, o));
+                bool store = true;
+                if (ep.pmin && mval[r]) {
+                    // the particle's final minimum is <= est, so a block more than the cut (60,
+                    // plus one for rounding) above est is negligible whatever the final minimum
+                    const float est = ordered_to_float(ep.pmin[pb[r]]);
+                    store = !(rmin > est + 61.f);
+                    if (rmin < est && col_in == 0) atomicMin(&ep.pmin[pb[r]], float_to_ordered(rmin));
+                }
+                if (store) {
+#pragma unroll
+                    for (int j = 0; j < 4; j++)
+                        if (mval[r] && nval[j]) crow[ncol[j]] = dv[j];
+                }
                 if (col_in == 0 && mval[r]) ep.rowmin[(long long)mrow[r] * ep.rowmin_ld + ncol0 / 64] = rmin;
             }
         }

@@ -62,6 +62,25 @@ __device__ __forceinline__ bool coarse_allowed(const WeightSrc& s, int b, long l
     return (s.bin_mask[(size_t)b * s.mask_words + (bin >> 5)] >> (bin & 31)) & 1u;
 }
 
+// Epilogue parameters of the coarse cross-term GEMMs.
+struct EpiParams {
+    float* C;            // output
+    long long ldc;
+    int M, N;            // valid extent
+    // EPI_DIFF2
+    int T;               // translations per particle (rows per particle)
+    const double* Cb;    // [B] constant term per particle (read as double)
+    const float* N1;     // [B][ldn]
+    long long ldn;
+    const double* prior_o;  // [N] orientation prior (0 = not scored)
+    float* rowmin;       // [M][N_pad/64]
+    int rowmin_ld;
+    // running per-particle minimum (order-preserving uint of the float); when set, the values of a
+    // 64-column block whose minimum exceeds it by more than the negligible-weight cut are not
+    // stored (the block minimum still is, so every reader skips the block). null: store all.
+    unsigned int* pmin;
+};
+
 struct SelParams {
     double fraction;
     long long max_sig;

@@ -25,6 +25,8 @@ __device__ __forceinline__ double pass_weight(const WeightSrc& s, int b, long lo
     if (s.kind == SRC_COARSE) {
         long long t = e / s.O_all, n = e - t * s.O_all;
         if (!coarse_allowed(s, b, n)) return 0.;
+        // blocks beyond the negligible-weight cut (their values may not have been stored)
+        if ((double)s.rowmin[((size_t)b * s.T + t) * s.rowmin_ld + n / 64] - mind2 > 60.0) return 0.;
         prior = s.prior_o[n] * s.prior_t[(size_t)b * s.T + t];
         if (!(prior > 0.)) return 0.;
         d = s.d2[((size_t)b * s.T + t) * s.ld + n];
