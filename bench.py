@@ -237,6 +237,9 @@ def main():
                     help="comma list of paths to time (fp32, bf16x3, fp16x3, learned); the last one is "
                          "the headline value, the others are reported beside it")
     ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--save-proposal", default="", help="write the trained proposal parameters (.npy)")
+    ap.add_argument("--load-proposal", default="",
+                    help="use proposal parameters saved by an earlier run instead of training")
     ap.add_argument("--quality-particles", type=int, default=2000,
                     help="particles of the half-map FSC phase (0: skip)")
     args = ap.parse_args()
@@ -271,12 +274,25 @@ def main():
         log("copy calibration: 1 GiB device-to-device copy (t2x::k_copy16) done")
     paths = args.precision.split(",")
     train = None
-    if "learned" in paths:
+    if "learned" in paths and args.load_proposal:
+        flat = np.ascontiguousarray(np.load(args.load_proposal), dtype=np.float32)
+        n_bins = cfg.nr_classes * ds["info"].nr_dir // 4
+        T.check(lib.t2x_proposal_init(dev, n_bins, 0), "proposal init")
+        T.check(lib.t2x_proposal_set_params(dev, flat.ctypes.data_as(C.POINTER(C.c_float)), flat.size), "params")
+        T.check(lib.t2x_proposal_mark_trained(dev, C.byref(cfg)), "mark trained")
+        log(f"[learned] proposal parameters loaded from {args.load_proposal}")
+    elif "learned" in paths:
         t = time.time()
         train = train_proposal(lib, dev, cfg, sz, ds, w, steps=args.train_steps)
         log(f"[learned] proposal trained: {train.steps} steps in {train.ms_total:.0f} ms "
             f"(setup {train.ms_setup:.0f} ms), loss {train.loss_first:.3f} -> {train.loss_last:.3f}, "
             f"train top-1 {train.top1_last:.3f} ({time.time() - t:.1f} s wall)")
+        if args.save_proposal:
+            nparam = C.c_longlong()
+            T.check(lib.t2x_proposal_num_params(dev, C.byref(nparam)))
+            flat = np.zeros(nparam.value, np.float32)
+            T.check(lib.t2x_proposal_get_params(dev, flat.ctypes.data_as(C.POINTER(C.c_float)), nparam.value))
+            np.save(args.save_proposal, flat)
     nw = min(n, args.warmup_particles)
     runs = {}
     for pname in paths:
@@ -314,9 +330,20 @@ def main():
     avg_ms = st.ms_coarse_gemm / launches
     peak = MFMA_PEAK_TFLOPS[head]
     achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
+    # HBM bytes per launch of the same stage from the committed rocprofv3 --pmc measurement
+    # (profiles/coarse_gemm_traffic.json: FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)
     traffic = os.environ.get("T2X_GEMM_TRAFFIC_BYTES")
+    traffic_src = "env T2X_GEMM_TRAFFIC_BYTES" if traffic else None
+    tj = os.path.join(REPO, "profiles", "coarse_gemm_traffic.json")
+    if traffic is None and os.path.exists(tj):
+        with open(tj) as f:
+            tr = json.load(f).get(head)
+        if tr:
+            traffic = tr["bytes_per_launch"]
+            traffic_src = "profiles/coarse_gemm_traffic.json (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, 800-particle batches)"
     roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
             "frac": achieved / peak, "traffic": float(traffic) if traffic else None,
+            "traffic_unit": "bytes per launch", "traffic_source": traffic_src,
             "kernel": "coarse diff2 cross term, " + KERNELS[head],
             "avg_launch_ms": avg_ms, "launches": launches, "flop_per_launch": flop_per_launch}
     stages = {k: getattr(st, k) for k in ("ms_preprocess", "ms_coarse_project", "ms_coarse_diff2",

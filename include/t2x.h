@@ -334,6 +334,9 @@ int t2x_proposal_init(t2x_device* dev, int n_bins, uint64_t seed);
 int t2x_proposal_num_params(t2x_device* dev, long long* n);
 int t2x_proposal_get_params(t2x_device* dev, float* host, long long n);
 int t2x_proposal_set_params(t2x_device* dev, const float* host, long long n);
+/* Declare the current parameters (e.g. set from a network trained earlier) usable for the
+ * geometry of cfg (box, classes, HEALPix order), as a finished t2x_proposal_train does. */
+int t2x_proposal_mark_trained(t2x_device* dev, const t2x_estep_config* cfg);
 /* One forward + backward pass on host data (tests): x [B][64][64], label [B], shift_target
  * [B][2]; writes the mean loss, the gradient (n_params floats, optional) and the outputs
  * [B][n_bins+2] (optional). bf16 selects the operand precision of the layer GEMMs. */

@@ -322,6 +322,24 @@ extern "C" int t2x_proposal_set_params(t2x_device* h, const float* host, long lo
     return T2X_OK;
 }
 
+extern "C" int t2x_proposal_mark_trained(t2x_device* h, const t2x_estep_config* cfg)
+{
+    if (!h || !cfg) { set_error("null argument"); return T2X_EINVAL; }
+    Device* d = reinterpret_cast<Device*>(h);
+    std::lock_guard<std::mutex> lk(d->mtx);
+    Net* net = d->net.get();
+    if (!net) { set_error("proposal network not initialised"); return T2X_ESTATE; }
+    HostSampling S;
+    T2X_TRY(build_sampling(cfg, S));
+    const int n_bins = (int)(cfg->nr_classes * ((long)S.rot.size() / 4));
+    if (n_bins != net->n_bins) { set_error("t2x_proposal_mark_trained: bins do not match the configuration"); return T2X_EINVAL; }
+    net->trained = true;
+    net->train_ori = cfg->ori_size;
+    net->train_K = cfg->nr_classes;
+    net->train_order = cfg->healpix_order;
+    return T2X_OK;
+}
+
 extern "C" int t2x_proposal_loss_grad(t2x_device* h, int B, const float* x, const int* label,
                                       const float* shift_target, double shift_weight, int bf16, double* loss,
                                       float* grad, float* outputs)

@@ -220,6 +220,7 @@ def load(path=None):
         "t2x_proposal_num_params": (C.c_int, [vp, C.POINTER(C.c_longlong)]),
         "t2x_proposal_get_params": (C.c_int, [vp, C.POINTER(C.c_float), C.c_longlong]),
         "t2x_proposal_set_params": (C.c_int, [vp, C.POINTER(C.c_float), C.c_longlong]),
+        "t2x_proposal_mark_trained": (C.c_int, [vp, C.POINTER(EstepConfig)]),
         "t2x_proposal_loss_grad": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int),
                                              C.POINTER(C.c_float), C.c_double, C.c_int, PD,
                                              C.POINTER(C.c_float), C.POINTER(C.c_float)]),
