@@ -43,9 +43,10 @@ DTYPES = {"fp32": "fp32",
           "fp16x3": "fp16x3 (fp16 hi+lo MFMA, fp32 accumulate, fp64 normalisation)",
           "learned": "fp16x3 cross term + bf16 proposal network (fp32 accumulate), fp64 normalisation"}
 KERNELS = {"fp32": "k_gemm_tn<EPI_DIFF2> (FP32 MFMA 16x16x4)",
-           "bf16x3": "k_gemm_x3w<bf16, EPI_DIFF2> (BF16 MFMA 16x16x32, 3 products)",
-           "fp16x3": "k_gemm_x3w<f16, EPI_DIFF2> (FP16 MFMA 16x16x32, 3 products)",
-           "learned": "k_gemm_x3_bins<f16> (FP16 MFMA 16x16x32, 3 products, top-k bins)"}
+           "bf16x3": "k_gemm_x3p<bf16, EPI_DIFF2> (BF16 MFMA 16x16x32, 3 products, 256x256 pipelined)",
+           "fp16x3": "k_gemm_x3p<f16, EPI_DIFF2> (FP16 MFMA 16x16x32, 3 products, 256x256 pipelined)",
+           "learned": "k_gemm_x3p<f16> on exhaustive-fallback rows + k_gemm_x3_bins<f16> on top-k bins "
+                      "(FP16 MFMA 16x16x32, 3 products)"}
 TOPK, MIN_CONF = 16, 0.3
 
 

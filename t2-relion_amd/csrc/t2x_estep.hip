@@ -38,8 +38,9 @@ int proposal_infer(Device* d, const float2* Fimg, int cs, int B, int topk, doubl
                    std::vector<float>& maxp, std::vector<int>& fallback);
 }
 int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const float* As, long long lda,
-                   const void* Bh, const void* Bl, const float* Bs, long long ldb, int K, int Mrows_pad,
-                   int Nrows_pad, const EpiParams& ep, hipStream_t st, int tile, const int* rowmap);
+                   long long a_rows, const void* Bh, const void* Bl, const float* Bs, long long ldb, int K,
+                   int Mrows_pad, int Nrows_pad, const EpiParams& ep, hipStream_t st, int tile,
+                   const int* rowmap);
 
 __global__ void k_pass_min(WeightSrc s, int B, SelResult* res);
 __global__ void k_select(WeightSrc s, SelParams prm, int B, SelResult* res);
@@ -1175,8 +1176,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                         EpiParams ef = ep;
                         ef.M = n_fb_rows;
                         T2X_TRY(launch_gemm_x3(prec, EPI_DIFF2, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc,
-                                               d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc,
-                                               (int)round_up(n_fb_rows, 256), (int)O_pad, ef, st, 256,
+                                               M_pad, d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(),
+                                               Kc, Kc, (int)round_up(n_fb_rows, 256), (int)O_pad, ef, st, 0,
                                                d->prop_rows.as<int>()));
                     }
                     T2X_TRY(launch_gemm_x3_bins(prec, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc, d->coarse_Bh.ptr,
@@ -1185,8 +1186,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                                                 O_pad - 1, ep, st));
                 } else
                     T2X_TRY(launch_gemm_x3(prec, EPI_DIFF2, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc,
-                                           d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc,
-                                           (int)M_pad, (int)O_pad, ep, st, 0, nullptr));
+                                           M_pad, d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc,
+                                           Kc, (int)M_pad, (int)O_pad, ep, st, 0, nullptr));
             }
         }
         (void)hipEventRecord(d->ev[ev++], st);   // 3: coarse gemm end
