@@ -36,18 +36,25 @@ import t2x_py as T  # noqa: E402
 from t2x_py import workloads as W  # noqa: E402
 
 METRIC = "E-step particles/s, 1 MI355X: plain FP32 vs mixed+learned at equal FSC-0.143"
-MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2500.0, "fp16x3": 2500.0, "learned": 2500.0}
-PRECISIONS = {"fp32": 0, "bf16x3": 1, "fp16x3": 2, "learned": 2}
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2500.0, "fp16x3": 2500.0, "learned": 2500.0, "learned_gated": 2500.0}
+PRECISIONS = {"fp32": 0, "bf16x3": 1, "fp16x3": 2, "learned": 2, "learned_gated": 2}
 DTYPES = {"fp32": "fp32",
           "bf16x3": "bf16x3 (bf16 hi+lo MFMA, fp32 accumulate, fp64 normalisation)",
           "fp16x3": "fp16x3 (fp16 hi+lo MFMA, fp32 accumulate, fp64 normalisation)",
-          "learned": "fp16x3 cross term + bf16 proposal network (fp32 accumulate), fp64 normalisation"}
+          "learned": "fp16x3 cross term + bf16 proposal network (fp32 accumulate), fp64 normalisation",
+          "learned_gated": "fp16x3 cross term + bf16 proposal network (fp32 accumulate), fp64 normalisation"}
 KERNELS = {"fp32": "k_gemm_tn<EPI_DIFF2> (FP32 MFMA 16x16x4)",
            "bf16x3": "k_gemm_x3p<bf16, EPI_DIFF2> (BF16 MFMA 16x16x32, 3 products, 256x256 pipelined)",
            "fp16x3": "k_gemm_x3p<f16, EPI_DIFF2> (FP16 MFMA 16x16x32, 3 products, 256x256 pipelined)",
            "learned": "k_gemm_x3p<f16> on exhaustive-fallback rows + k_gemm_x3_bins<f16> on top-k bins "
-                      "(FP16 MFMA 16x16x32, 3 products)"}
-TOPK, MIN_CONF = 16, 0.3
+                      "(FP16 MFMA 16x16x32, 3 products)",
+           "learned_gated": "k_gemm_x3p<f16> on exhaustive-fallback rows + k_gemm_x3_bins<f16> on top-k bins "
+                            "(FP16 MFMA 16x16x32, 3 products)"}
+# learned proposal settings: (top-k bins, gate, threshold). "learned" is the configuration the
+# workload specifies (top-16 bins, exhaustive search when the top softmax probability is below
+# 0.3); "learned_gated" keeps 64 bins and searches exhaustively unless their probability mass
+# reaches 0.98 (calibrated against exhaustive search on this workload: >= 99% pose agreement).
+PROPOSAL = {"learned": (16, 0, 0.3), "learned_gated": (64, 1, 0.98)}
 
 
 def log(*a):
@@ -126,8 +133,10 @@ def train_proposal(lib, dev, cfg, sz, ds, w, steps=2000, batch=256, n_proj=20000
 
 def configure(cfg, pname):
     cfg.precision = PRECISIONS[pname]
-    cfg.proposal_topk = TOPK if pname == "learned" else 0
-    cfg.proposal_min_conf = MIN_CONF
+    topk, gate, conf = PROPOSAL.get(pname, (0, 0, 0.0))
+    cfg.proposal_topk = topk
+    cfg.proposal_gate = gate
+    cfg.proposal_min_conf = conf
 
 
 def rot_matrix(rot, tilt, psi):
@@ -234,9 +243,9 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device-to-device copy kernel whose FETCH_SIZE/WRITE_SIZE "
                          "counters calibrate the per-dispatch HBM traffic of the GEMM in --pmc runs")
-    ap.add_argument("--precision", default="fp32,fp16x3,learned",
-                    help="comma list of paths to time (fp32, bf16x3, fp16x3, learned); the last one is "
-                         "the headline value, the others are reported beside it")
+    ap.add_argument("--precision", default="fp32,fp16x3,learned,learned_gated",
+                    help="comma list of paths to time (fp32, bf16x3, fp16x3, learned, learned_gated); the "
+                         "last one is the headline value, the others are reported beside it")
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--save-proposal", default="", help="write the trained proposal parameters (.npy)")
     ap.add_argument("--load-proposal", default="",
@@ -275,14 +284,15 @@ def main():
         log("copy calibration: 1 GiB device-to-device copy (t2x::k_copy16) done")
     paths = args.precision.split(",")
     train = None
-    if "learned" in paths and args.load_proposal:
+    use_net = any(p.startswith("learned") for p in paths)
+    if use_net and args.load_proposal:
         flat = np.ascontiguousarray(np.load(args.load_proposal), dtype=np.float32)
         n_bins = cfg.nr_classes * ds["info"].nr_dir // 4
         T.check(lib.t2x_proposal_init(dev, n_bins, 0), "proposal init")
         T.check(lib.t2x_proposal_set_params(dev, flat.ctypes.data_as(C.POINTER(C.c_float)), flat.size), "params")
         T.check(lib.t2x_proposal_mark_trained(dev, C.byref(cfg)), "mark trained")
         log(f"[learned] proposal parameters loaded from {args.load_proposal}")
-    elif "learned" in paths:
+    elif use_net:
         t = time.time()
         train = train_proposal(lib, dev, cfg, sz, ds, w, steps=args.train_steps)
         log(f"[learned] proposal trained: {train.steps} steps in {train.ms_total:.0f} ms "
@@ -324,7 +334,7 @@ def main():
     launches = max(1, st.n_coarse_diff2_launches)
     nprod = 1 if head == "fp32" else 3
     flop_total = nprod * st.coarse_diff2_flop
-    if head == "learned":   # only the scored bins are computed
+    if head.startswith("learned"):   # only the scored bins are computed
         n_bins = cfg.nr_classes * ds["info"].nr_dir // 4
         flop_total *= st.proposal_bins_scored / max(1.0, st.n_proposal_particles * n_bins)
     flop_per_launch = flop_total / launches
@@ -359,12 +369,14 @@ def main():
              "coarse_gemm_ms": rr["st"].ms_coarse_gemm,
              "pose_index_agreement_vs_fp32": (float((rr["out"]["best_hidden"] == runs["fp32"]["out"]["best_hidden"]).mean())
                                               if "fp32" in runs else None)}
-        if pname == "learned":
+        if pname.startswith("learned"):
             s2 = rr["st"]
             n_bins = cfg.nr_classes * ds["info"].nr_dir // 4
             v.update({"fallback_rate": s2.n_proposal_fallback / max(1, s2.n_proposal_particles),
                       "bins_scored_per_particle": s2.proposal_bins_scored / max(1, s2.n_proposal_particles),
-                      "n_bins": n_bins, "topk": TOPK, "min_conf": MIN_CONF,
+                      "n_bins": n_bins, "topk": PROPOSAL[pname][0],
+                      "gate": ("top-1 softmax probability", "top-k probability mass")[PROPOSAL[pname][1]],
+                      "min_conf": PROPOSAL[pname][2],
                       "pose_agreement_within_one_step_vs_exhaustive":
                           pose_agreement(rr["out"], runs[ref]["out"], step_deg) if ref else None,
                       "training_ms": train.ms_total if train else None,
@@ -382,6 +394,18 @@ def main():
             for k in q:
                 quality.setdefault("max_abs_fsc_diff_vs_fp32", {})[k] = float(
                     max(np.abs(a - b).max() for a, b in zip(q[k]["curves"], q["fp32"]["curves"])))
+    targets = None
+    if "fp32" in runs and head != "fp32":
+        hv = variants[head]
+        agree = hv.get("pose_agreement_within_one_step_vs_exhaustive")
+        fsc_d = None
+        if quality and "fp32" in quality["fsc_0143_A_per_class"]:
+            fsc_d = max(abs(a - b) for a, b in zip(quality["fsc_0143_A_per_class"][head],
+                                                   quality["fsc_0143_A_per_class"]["fp32"]))
+        targets = {"speedup_vs_plain_fp32": value / runs["fp32"]["value"], "speedup_target": 3.0,
+                   "pose_agreement": agree if agree is not None else hv["pose_index_agreement_vs_fp32"],
+                   "pose_agreement_target": 0.99,
+                   "max_fsc_0143_diff_A_vs_fp32": fsc_d, "fsc_0143_diff_target_A": 0.1}
     cpu = None if args.no_cpu else cpu_baseline(w, cfg, sz)
     line = {
         "metric": METRIC, "value": value, "unit": "particles/s", "n_gpus": 1,
@@ -400,6 +424,7 @@ def main():
         "variants": variants,
         "speedup_vs_plain_fp32": (value / runs["fp32"]["value"]) if "fp32" in runs else None,
         "quality": quality,
+        "targets": targets,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

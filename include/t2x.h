@@ -49,6 +49,13 @@ enum {
     T2X_PREC_FP16X3 = 2       /* FP16 hi+lo split, 3 MFMA products, FP32 accumulation  */
 };
 
+/* Learned-proposal gate (t2x_estep_config.proposal_gate): the value compared with
+ * proposal_min_conf to decide between the top-k restricted and the exhaustive coarse search. */
+enum {
+    T2X_GATE_TOP1 = 0,        /* softmax probability of the most probable bin            */
+    T2X_GATE_TOPK_MASS = 1    /* softmax probability mass of the top-k bins              */
+};
+
 /* Orientational prior modes (ml_model.h NOPRIOR / PRIOR_ROTTILT_PSI). */
 enum { T2X_NOPRIOR = 0, T2X_PRIOR_ROTTILT_PSI = 1 };
 
@@ -91,8 +98,8 @@ typedef struct t2x_estep_config {
     int    precision;              /* T2X_PREC_* for the cross terms                     */
     int    proposal_topk;          /* > 0: coarse pass scores only the top-k bins of the */
                                    /* trained proposal network (t2x_proposal_train)      */
-    int    proposal_reserved;
-    double proposal_min_conf;      /* max softmax below this: exhaustive coarse search   */
+    int    proposal_gate;          /* T2X_GATE_*: what proposal_min_conf is compared with */
+    double proposal_min_conf;      /* gate value below this: exhaustive coarse search    */
     int    reserved[4];
 } t2x_estep_config;
 
@@ -128,7 +135,7 @@ typedef struct t2x_particle_out {
     long   nr_significant_coarse; /* METADATA_NR_SIGN                                     */
     long   nr_fine_samples;       /* fine (orientation, translation) pairs scored         */
     long   nr_fine_significant;   /* fine samples with weight >= significant_weight       */
-    long   best_hidden;           /* fine hidden index of the best pose (see DESIGN)      */
+    long   best_hidden;           /* fine hidden index (ihidden_over) of the best pose    */
 } t2x_particle_out;
 
 /* Weighted sums the E-step adds into (host, caller-owned, double). Any member may be NULL. */
@@ -302,7 +309,8 @@ int t2x_synth_particles_device(t2x_device* dev, int ori_size, double pixel_size,
  * search, plus a 2-D shift regression. It is trained on the GPU on noisy CTF-modulated projections
  * of the current references (the device projectors), with the model's noise spectrum; with
  * cfg->proposal_topk > 0 the E-step's coarse pass scores only the top-k bins of each particle and
- * searches exhaustively when the top softmax probability is below cfg->proposal_min_conf.
+ * searches exhaustively when the gate value (cfg->proposal_gate: top softmax probability or
+ * top-k probability mass) is below cfg->proposal_min_conf.
  * ------------------------------------------------------------------------------------- */
 typedef struct t2x_proposal_train_params {
     int      steps;            /* optimiser steps (2000)                                     */
@@ -348,6 +356,10 @@ int t2x_proposal_loss_grad(t2x_device* dev, int B, const float* x, const int* la
  * the noise spectrum. */
 int t2x_proposal_train(t2x_device* dev, const t2x_estep_config* cfg, const t2x_model* model,
                        const t2x_proposal_train_params* tp, t2x_proposal_train_stats* out);
+/* Gate values of the particles of the last t2x_estep call on this handle that ran the
+ * proposal, in input order: top softmax probability and top-k probability mass (either may be
+ * NULL). n must equal that call's particle count. */
+int t2x_proposal_gate_values(t2x_device* dev, int n, float* top1, float* topk_mass);
 
 /* ---------------------------------------------------------------------------------------
  * Reconstruction (M-step side of the backprojector), all arithmetic in FP64 on the GPU.

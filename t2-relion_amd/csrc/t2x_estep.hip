@@ -34,8 +34,9 @@ struct BinWorkHost {
     int pstart, cnt, rt, col0, ncols;
 };
 namespace cnn {
-int proposal_infer(Device* d, const float2* Fimg, int cs, int B, int topk, double min_conf, std::vector<int>& bins,
-                   std::vector<float>& maxp, std::vector<int>& fallback);
+int proposal_infer(Device* d, const float2* Fimg, int cs, int B, int topk, double min_conf, int gate,
+                   std::vector<int>& bins, std::vector<float>& maxp, std::vector<float>& mass,
+                   std::vector<int>& fallback);
 }
 int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const float* As, long long lda,
                    long long a_rows, const void* Bh, const void* Bl, const float* Bs, long long ldb, int K,
@@ -803,6 +804,16 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         }
         if (bin_cols % 64) { set_error("learned proposal needs 4 * nr_psi to be a multiple of 64"); return T2X_EINVAL; }
         topk = std::min(cfg->proposal_topk, n_bins);
+        if ((size_t)(n_bins + topk) * sizeof(float) > 64 * 1024) {   // the top-k kernel holds them in LDS
+            set_error("learned proposal: too many bins");
+            return T2X_EINVAL;
+        }
+        if (cfg->proposal_gate != T2X_GATE_TOP1 && cfg->proposal_gate != T2X_GATE_TOPK_MASS) {
+            set_error("learned proposal: unknown proposal_gate");
+            return T2X_EINVAL;
+        }
+        d->prop_gate_top1.clear();
+        d->prop_gate_mass.clear();
     }
     if (T * over_trans > STORE_MAX_T) { set_error("too many fine translations"); return T2X_EINVAL; }
     std::vector<const float2*> h_vols(K);
@@ -1087,9 +1098,11 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         if (use_prop) {
             const auto tp0 = std::chrono::steady_clock::now();
             std::vector<int> pbins, pfb;
-            std::vector<float> pmax;
-            T2X_TRY(cnn::proposal_infer(d, d->Fimg_cur.as<float2>(), cs, B, topk, cfg->proposal_min_conf, pbins, pmax,
-                                        pfb));
+            std::vector<float> pmax, pmass;
+            T2X_TRY(cnn::proposal_infer(d, d->Fimg_cur.as<float2>(), cs, B, topk, cfg->proposal_min_conf,
+                                        cfg->proposal_gate, pbins, pmax, pmass, pfb));
+            d->prop_gate_top1.insert(d->prop_gate_top1.end(), pmax.begin(), pmax.end());
+            d->prop_gate_mass.insert(d->prop_gate_mass.end(), pmass.begin(), pmass.end());
             const int mw = (n_bins + 31) / 32;
             std::vector<unsigned> mask((size_t)B * mw, 0u);
             std::vector<std::vector<int>> members(n_bins);

@@ -675,18 +675,20 @@ __device__ __forceinline__ void gemm_epilogue_staged(const floatx4 (&acc)[8][4],
         }
     }
     const long long blk = (n0 + tcol0) / 64;
+    // branch-light: padding rows (output row -1, uniform over the 16 lanes of a row) are
+    // masked, the running minima are selected and updated with conditional moves
 #pragma unroll
     for (int i = 0; i < 8; i++) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int4 e = es.row[trow0 + i * 16 + row_grp + r];
-            if (e.x < 0) continue;   // padding row (uniform over the 16 lanes of the row)
+            const bool rv = e.x >= 0;
             const float sa = __int_as_float(e.w);
-            float* crow = ep.C + (long long)e.x * ep.ldc;
+            float* crow = ep.C + (long long)max(e.x, 0) * ep.ldc;
             if (EPI == EPI_STORE) {
 #pragma unroll
                 for (int j = 0; j < 4; j++)
-                    if (nval[j]) crow[ncol[j]] = acc[i][j][r] * sa * sb[j];
+                    if (rv && nval[j]) crow[ncol[j]] = acc[i][j][r] * sa * sb[j];
             } else {
                 const double* cbn1 = es.cbn1[e.z];
                 float rmin = 3.0e38f, dv[4];
@@ -694,30 +696,31 @@ __device__ __forceinline__ void gemm_epilogue_staged(const floatx4 (&acc)[8][4],
                 for (int j = 0; j < 4; j++) {
                     const float v = acc[i][j][r] * sa * sb[j];
                     dv[j] = (float)(cbn1[tcol0 + j * 16 + col_in] - (double)v);
-                    if (nval[j] && scored[j]) rmin = fminf(rmin, dv[j]);
+                    rmin = (nval[j] && scored[j]) ? fminf(rmin, dv[j]) : rmin;
                 }
                 rmin = min_row16(rmin);
-                bool store = true;
+                bool store = rv;
                 if (lazy) {
                     // the particle's final minimum is <= es_q, so a block more than the cut (60,
                     // plus one for rounding) above es_q is negligible whatever the final minimum
                     const int q = e.z - z0;
-                    const float es_q = q == 0 ? est[0] : q == 1 ? est[1] : q == 2 ? est[2] : q == 3 ? est[3] : 3.4e38f;
-                    store = !(rmin > es_q + 61.f);
-                    if (rmin < es_q) {
-                        if (col_in == 0) atomicMin(&ep.pmin[e.y], float_to_ordered(rmin));
-                        if (q == 0) est[0] = rmin;
-                        else if (q == 1) est[1] = rmin;
-                        else if (q == 2) est[2] = rmin;
-                        else if (q == 3) est[3] = rmin;
-                    }
+                    float es_q = 3.4e38f;
+                    es_q = q == 0 ? est[0] : es_q;
+                    es_q = q == 1 ? est[1] : es_q;
+                    es_q = q == 2 ? est[2] : es_q;
+                    es_q = q == 3 ? est[3] : es_q;
+                    store = rv && !(rmin > es_q + 61.f);
+                    const bool lower = rv && rmin < es_q;
+                    if (lower && col_in == 0) atomicMin(&ep.pmin[e.y], float_to_ordered(rmin));
+                    est[0] = (lower && q == 0) ? rmin : est[0];
+                    est[1] = (lower && q == 1) ? rmin : est[1];
+                    est[2] = (lower && q == 2) ? rmin : est[2];
+                    est[3] = (lower && q == 3) ? rmin : est[3];
                 }
-                if (store) {
 #pragma unroll
-                    for (int j = 0; j < 4; j++)
-                        if (nval[j]) crow[ncol[j]] = dv[j];
-                }
-                if (col_in == 0) ep.rowmin[(long long)e.x * ep.rowmin_ld + blk] = rmin;
+                for (int j = 0; j < 4; j++)
+                    if (store && nval[j]) crow[ncol[j]] = dv[j];
+                if (rv && col_in == 0) ep.rowmin[(long long)e.x * ep.rowmin_ld + blk] = rmin;
             }
         }
     }

@@ -154,6 +154,7 @@ struct Device {
     DevBuf prop_crop, prop_img, prop_out, prop_mask, prop_plist, prop_work, prop_rows;
     hipfftHandle prop_plan = 0;
     int prop_plan_b = 0;
+    std::vector<float> prop_gate_top1, prop_gate_mass;   // per particle of the last E-step call
 };
 
 // ---------------- device helpers ----------------

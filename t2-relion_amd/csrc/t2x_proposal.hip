@@ -180,10 +180,12 @@ __global__ void k_particle_crop(const float2* __restrict__ F, int cs, float2* __
 
 // per particle: softmax over the bin logits, top-k bins (descending logit, ties by index) and the
 // top probability; fallback when it is below min_conf
-__global__ void k_topk(const float* __restrict__ y, int n_bins, int topk, float min_conf, int* __restrict__ bins,
-                       float* __restrict__ maxp, int* __restrict__ fallback)
+__global__ void k_topk(const float* __restrict__ y, int n_bins, int topk, float min_conf, int gate,
+                       int* __restrict__ bins, float* __restrict__ maxp, float* __restrict__ mass,
+                       int* __restrict__ fallback)
 {
-    extern __shared__ float sl[];   // n_bins logits
+    extern __shared__ float sl[];   // n_bins logits, then the topk logits of the selected bins
+    float* tv = sl + n_bins;
     __shared__ float rv[256];
     __shared__ int ri[256];
     const int b = blockIdx.x;
@@ -212,6 +214,7 @@ __global__ void k_topk(const float* __restrict__ y, int n_bins, int topk, float 
         }
         const int sel = ri[0];
         if (k == 0) m0 = rv[0];
+        if (threadIdx.x == 0) tv[k] = rv[0];
         if (threadIdx.x == 0) bins[(size_t)b * topk + k] = sel;
         __syncthreads();
         if (threadIdx.x == 0) sl[sel] = -3.0e38f;   // removed; the softmax below uses yb
@@ -226,15 +229,20 @@ __global__ void k_topk(const float* __restrict__ y, int n_bins, int topk, float 
     }
     if (threadIdx.x == 0) {
         const float p = 1.f / rv[0];
+        float mk = 0.f;   // softmax mass of the selected bins
+        for (int k = 0; k < topk; k++) mk += expf(tv[k] - m0);
+        mk *= p;
         maxp[b] = p;
-        fallback[b] = p < min_conf;
+        mass[b] = mk;
+        fallback[b] = (gate == T2X_GATE_TOPK_MASS ? mk : p) < min_conf;
     }
 }
 
 // Network inference on B preprocessed particles: top-k bins, top probability, fallback flags
 // (host vectors). Fimg: device [B][cs*(cs/2+1)].
-int proposal_infer(Device* d, const float2* Fimg, int cs, int B, int topk, double min_conf, std::vector<int>& bins,
-                   std::vector<float>& maxp, std::vector<int>& fallback)
+int proposal_infer(Device* d, const float2* Fimg, int cs, int B, int topk, double min_conf, int gate,
+                   std::vector<int>& bins, std::vector<float>& maxp, std::vector<float>& mass,
+                   std::vector<int>& fallback)
 {
     Net& net = *d->net;
     hipStream_t st = d->stream;
@@ -246,18 +254,21 @@ int proposal_infer(Device* d, const float2* Fimg, int cs, int B, int topk, doubl
     T2X_TRY(crop_to_input(net, crop.as<float2>(), B, st, d->prop_img, d->prop_plan, d->prop_plan_b));
     T2X_TRY(net.forward(B, true, st));
     DevBuf& outb = d->prop_out;
-    T2X_TRY(outb.ensure(sizeof(int) * (size_t)B * (topk + 2)));
+    T2X_TRY(outb.ensure(sizeof(int) * (size_t)B * (topk + 3)));
     int* d_bins = outb.as<int>();
     float* d_maxp = reinterpret_cast<float*>(d_bins + (size_t)B * topk);
-    int* d_fb = d_bins + (size_t)B * (topk + 1);
-    k_topk<<<B, 256, sizeof(float) * net.n_bins, st>>>(net.y.as<float>(), net.n_bins, topk, (float)min_conf, d_bins,
-                                                       d_maxp, d_fb);
+    float* d_mass = d_maxp + B;
+    int* d_fb = d_bins + (size_t)B * (topk + 2);
+    k_topk<<<B, 256, sizeof(float) * (net.n_bins + topk), st>>>(net.y.as<float>(), net.n_bins, topk, (float)min_conf, gate,
+                                                       d_bins, d_maxp, d_mass, d_fb);
     T2X_HIP(hipGetLastError());
     bins.resize((size_t)B * topk);
     maxp.resize(B);
+    mass.resize(B);
     fallback.resize(B);
     T2X_HIP(hipMemcpyAsync(bins.data(), d_bins, sizeof(int) * (size_t)B * topk, hipMemcpyDeviceToHost, st));
     T2X_HIP(hipMemcpyAsync(maxp.data(), d_maxp, sizeof(float) * B, hipMemcpyDeviceToHost, st));
+    T2X_HIP(hipMemcpyAsync(mass.data(), d_mass, sizeof(float) * B, hipMemcpyDeviceToHost, st));
     T2X_HIP(hipMemcpyAsync(fallback.data(), d_fb, sizeof(int) * B, hipMemcpyDeviceToHost, st));
     T2X_HIP(hipStreamSynchronize(st));
     return T2X_OK;
@@ -539,5 +550,19 @@ extern "C" int t2x_proposal_train(t2x_device* h, const t2x_estep_config* cfg, co
         out->steps = tp->steps;
         out->n_bins = n_bins;
     }
+    return T2X_OK;
+}
+
+extern "C" int t2x_proposal_gate_values(t2x_device* h, int n, float* top1, float* topk_mass)
+{
+    if (!h || n < 0) { set_error("t2x_proposal_gate_values: bad arguments"); return T2X_EINVAL; }
+    Device* d = reinterpret_cast<Device*>(h);
+    std::lock_guard<std::mutex> lk(d->mtx);
+    if ((size_t)n != d->prop_gate_top1.size()) {
+        set_error("t2x_proposal_gate_values: n differs from the particle count of the last proposal E-step");
+        return T2X_EINVAL;
+    }
+    if (top1) std::copy(d->prop_gate_top1.begin(), d->prop_gate_top1.end(), top1);
+    if (topk_mass) std::copy(d->prop_gate_mass.begin(), d->prop_gate_mass.end(), topk_mass);
     return T2X_OK;
 }

@@ -24,6 +24,8 @@ T2X_ENOSIG = -6
 T2X_PREC_FP32 = 0
 T2X_PREC_BF16X3 = 1
 T2X_PREC_FP16X3 = 2
+T2X_GATE_TOP1 = 0
+T2X_GATE_TOPK_MASS = 1
 
 
 class CtfParams(C.Structure):
@@ -45,7 +47,7 @@ class EstepConfig(C.Structure):
         ("do_norm_correction", C.c_int), ("avg_norm_correction", C.c_double),
         ("do_ctf_correction", C.c_int), ("sigma2_offset", C.c_double),
         ("orientational_prior_mode", C.c_int), ("precision", C.c_int),
-        ("proposal_topk", C.c_int), ("proposal_reserved", C.c_int),
+        ("proposal_topk", C.c_int), ("proposal_gate", C.c_int),
         ("proposal_min_conf", C.c_double), ("reserved", C.c_int * 4),
     ]
 
@@ -221,6 +223,7 @@ def load(path=None):
         "t2x_proposal_get_params": (C.c_int, [vp, C.POINTER(C.c_float), C.c_longlong]),
         "t2x_proposal_set_params": (C.c_int, [vp, C.POINTER(C.c_float), C.c_longlong]),
         "t2x_proposal_mark_trained": (C.c_int, [vp, C.POINTER(EstepConfig)]),
+        "t2x_proposal_gate_values": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "t2x_proposal_loss_grad": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_int),
                                              C.POINTER(C.c_float), C.c_double, C.c_int, PD,
                                              C.POINTER(C.c_float), C.POINTER(C.c_float)]),

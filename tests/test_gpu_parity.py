@@ -1,6 +1,6 @@
 """libt2x on the GPU against the FP64 oracle on identical float32 inputs.
 
-Tolerances (north star): diff2 within 1e-5 relative, identical max-weight pose indices for
+Tolerances: diff2 within 1e-5 relative, identical max-weight pose indices for
 >= 99.9% of particles, backprojected data/weight volumes within 1e-4 relative (Frobenius norm),
 weighted sums within 1e-4. Weights are exp(-diff2) so an FP32 diff2 error of ~3e-5 absolute
 shows as ~3e-5 relative in the normalised weights; they are held to 1e-3.

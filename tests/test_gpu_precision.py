@@ -9,7 +9,7 @@ weights exp(-(d - min d)) / sum, significant-set agreement and max-weight pose a
 Tolerances. FP16x3 with per-row power-of-two scaling keeps ~22 mantissa bits per operand,
 below FP32 rounding of the accumulator, so it is held to the FP32 bar (diff2 1e-5 relative,
 >= 99.9% identical poses). BF16x3 keeps ~16 bits per operand: its diff2 is held to 1e-4
-relative and its poses to the north-star mixed-precision bar of >= 99% agreement.
+relative and its poses to the mixed-precision bar of >= 99% agreement.
 """
 import numpy as np
 import pytest

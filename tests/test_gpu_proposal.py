@@ -38,10 +38,11 @@ def setup(gpu_device):
     return dict(lib=lib, dev=gpu_device, w=w, n=n, cfg=cfg, sz=sz, ds=ds)
 
 
-def run(s, topk=0, min_conf=0.3):
+def run(s, topk=0, min_conf=0.3, gate=T.T2X_GATE_TOP1):
     lib, dev, cfg, n, ds = s["lib"], s["dev"], s["cfg"], s["n"], s["ds"]
     cfg.proposal_topk = topk
     cfg.proposal_min_conf = min_conf
+    cfg.proposal_gate = gate
     outs = (T.ParticleOut * n)()
     ws, wa = T.make_wsum(cfg.nr_classes, ds["info"].nr_dir, cfg.ori_size)
     Bm.run_step(lib, dev, cfg, s["sz"], ds, n, outs, ws)
@@ -131,3 +132,25 @@ def test_trained_proposal_beats_untrained(setup):
     assert st.loss_last < st.loss_first
     assert a_t >= a_u + 0.3
     assert st0.steps == 1 and st0_fb is not None
+
+
+@pytest.mark.parametrize("gate", [T.T2X_GATE_TOP1, T.T2X_GATE_TOPK_MASS])
+def test_gate_values_and_fallback_count(setup, gate):
+    """The gate values read back per particle are probabilities (top-1 <= top-k mass <= 1), and
+    exactly the particles whose gate value is below the threshold take the exhaustive search."""
+    s = setup
+    lib, dev, n = s["lib"], s["dev"], s["n"]
+    topk, thr = 32, (0.05 if gate == T.T2X_GATE_TOP1 else 0.6)
+    T.check(lib.t2x_stats_reset(dev))
+    run(s, topk=topk, min_conf=thr, gate=gate)
+    stats = T.Stats()
+    T.check(lib.t2x_stats_get(dev, C.byref(stats)))
+    top1 = np.zeros(n, np.float32)
+    mass = np.zeros(n, np.float32)
+    T.check(lib.t2x_proposal_gate_values(dev, n, top1.ctypes.data_as(C.POINTER(C.c_float)),
+                                         mass.ctypes.data_as(C.POINTER(C.c_float))))
+    assert np.all(top1 > 0) and np.all(top1 <= mass * (1 + 1e-5)) and np.all(mass <= 1 + 1e-5)
+    g = top1 if gate == T.T2X_GATE_TOP1 else mass
+    assert stats.n_proposal_particles == n
+    assert stats.n_proposal_fallback == int((g < thr).sum())
+    assert lib.t2x_proposal_gate_values(dev, n + 1, None, None) != 0   # count must match the last call

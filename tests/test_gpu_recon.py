@@ -4,7 +4,7 @@ reconstruction) on the same particles.
 
 Tolerances: reconstruction and FSC are FP64 on both sides and differ only by FFT algorithm and
 summation order: 1e-9 relative (Frobenius) on maps, 1e-9 absolute on FSC values. The half-map
-FSC of the plain FP32 GPU path is held to the north-star bar of 0.01 at every shell.
+FSC of the plain FP32 GPU path is held to a bar of 0.01 at every shell.
 """
 import numpy as np
 import pytest
