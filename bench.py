@@ -367,6 +367,8 @@ def main():
     for pname, rr in runs.items():
         v = {"particles_per_s": rr["value"], "ms_per_step": rr["dt"] * 1e3 / args.steps,
              "coarse_gemm_ms": rr["st"].ms_coarse_gemm,
+             "gemm_variant_autotuned": ([rr["st"].gemm_tile_used, rr["st"].gemm_group_used]
+                                        if rr["st"].gemm_tile_used else None),
              "pose_index_agreement_vs_fp32": (float((rr["out"]["best_hidden"] == runs["fp32"]["out"]["best_hidden"]).mean())
                                               if "fp32" in runs else None)}
         if pname.startswith("learned"):

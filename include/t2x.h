@@ -100,7 +100,10 @@ typedef struct t2x_estep_config {
                                    /* trained proposal network (t2x_proposal_train)      */
     int    proposal_gate;          /* T2X_GATE_*: what proposal_min_conf is compared with */
     double proposal_min_conf;      /* gate value below this: exhaustive coarse search    */
-    int    reserved[4];
+    int    gemm_tile;              /* split-precision GEMM variant: 0 autotuned (fastest  */
+                                   /* on first use per shape), 128, 256, 257 or 258      */
+    int    gemm_group;             /* its tile-walk group in tile rows (0: default)      */
+    int    reserved[2];
 } t2x_estep_config;
 
 /* Model arrays of the current iteration (host, caller-owned, read-only). */
@@ -194,6 +197,8 @@ typedef struct t2x_stats {
     long   n_proposal_particles; /* particles that went through the proposal network         */
     long   n_proposal_fallback;  /* of those, searched exhaustively (low confidence)          */
     double proposal_bins_scored; /* sum over particles of the bins scored in the coarse pass  */
+    int    gemm_tile_used;       /* split-precision GEMM variant of the last launch (tile, group) */
+    int    gemm_group_used;
 } t2x_stats;
 
 /* ---------------------------------------------------------------------------------------

@@ -40,8 +40,8 @@ int proposal_infer(Device* d, const float2* Fimg, int cs, int B, int topk, doubl
 }
 int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const float* As, long long lda,
                    long long a_rows, const void* Bh, const void* Bl, const float* Bs, long long ldb, int K,
-                   int Mrows_pad, int Nrows_pad, const EpiParams& ep, hipStream_t st, int tile,
-                   const int* rowmap);
+                   int Mrows_pad, int Nrows_pad, const EpiParams& ep, hipStream_t st, GemmChoice want,
+                   const int* rowmap, GemmChoice* used);
 
 __global__ void k_pass_min(WeightSrc s, int B, SelResult* res);
 __global__ void k_select(WeightSrc s, SelParams prm, int B, SelResult* res);
@@ -787,6 +787,11 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     const int over_rot = S.over_rot, over_trans = S.over_trans, nover = over_rot * over_trans;
     hipStream_t st = d->stream;
     if (T > 128) { set_error("more than 128 coarse translations"); return T2X_EINVAL; }
+    if (!(cfg->gemm_tile == 0 || cfg->gemm_tile == 128 || cfg->gemm_tile == 256 || cfg->gemm_tile == 257 ||
+          cfg->gemm_tile == 258) || cfg->gemm_group < 0) {
+        set_error("gemm_tile must be 0 (autotuned), 128, 256, 257 or 258 and gemm_group >= 0");
+        return T2X_EINVAL;
+    }
     // learned proposal: the coarse pass scores only the top-k bins (classes x HEALPix directions
     // one order down = blocks of 4 directions x Npsi orientations) of each particle
     const bool use_prop = cfg->proposal_topk > 0;
@@ -1179,6 +1184,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                 T2X_TRY(launch_gemm(EPI_DIFF2, d->A.as<float>(), Kc, d->coarse_B.as<float>(), Kc, Kc,
                                     (int)M_pad, (int)O_pad, ep, st));
             } else {
+                const GemmChoice gwant{cfg->gemm_tile, cfg->gemm_group};
+                GemmChoice gused{0, 0};
                 T2X_TRY(d->Ah.ensure(sizeof(uint16_t) * M_pad * Kc));
                 T2X_TRY(d->Al.ensure(sizeof(uint16_t) * M_pad * Kc));
                 T2X_TRY(d->As.ensure(sizeof(float) * M_pad));
@@ -1190,8 +1197,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                         ef.M = n_fb_rows;
                         T2X_TRY(launch_gemm_x3(prec, EPI_DIFF2, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc,
                                                M_pad, d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(),
-                                               Kc, Kc, (int)round_up(n_fb_rows, 256), (int)O_pad, ef, st, 0,
-                                               d->prop_rows.as<int>()));
+                                               Kc, Kc, (int)round_up(n_fb_rows, 256), (int)O_pad, ef, st, gwant,
+                                               d->prop_rows.as<int>(), &gused));
                     }
                     T2X_TRY(launch_gemm_x3_bins(prec, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc, d->coarse_Bh.ptr,
                                                 d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc,
@@ -1200,7 +1207,11 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                 } else
                     T2X_TRY(launch_gemm_x3(prec, EPI_DIFF2, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc,
                                            M_pad, d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc,
-                                           Kc, (int)M_pad, (int)O_pad, ep, st, 0, nullptr));
+                                           Kc, (int)M_pad, (int)O_pad, ep, st, gwant, nullptr, &gused));
+                if (gused.tile) {
+                    d->stats.s.gemm_tile_used = gused.tile;
+                    d->stats.s.gemm_group_used = gused.group;
+                }
             }
         }
         (void)hipEventRecord(d->ev[ev++], st);   // 3: coarse gemm end

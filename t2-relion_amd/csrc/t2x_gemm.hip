@@ -8,6 +8,9 @@
 // K step 32, v_mfma_f32_16x16x32_{bf16,f16} on split hi/lo operands (split precision).
 #include "t2x_kernels.h"
 #include <algorithm>
+#include <map>
+#include <mutex>
+#include <tuple>
 
 namespace t2x {
 
@@ -1045,28 +1048,15 @@ int launch_split(int prec, const float* X, long long ldx, int K, int rows, int v
     return T2X_OK;
 }
 
-// Tile variant of the split-precision GEMM: 128 (k_gemm_x3), 256 (k_gemm_x3w, 256x256), 257
-// (k_gemm_x3d, 256x128 with direct-to-LDS loads) or 258 (k_gemm_x3p, 256x256 pipelined, the
-// default: 18% faster than k_gemm_x3w on the class256 shapes, identical results); 0 = default.
+// Variants of the split-precision GEMM (all give identical results: every output element sees
+// the same MFMA chain and the same epilogue arithmetic): tile 128 (k_gemm_x3), 256 (k_gemm_x3w,
+// 256x256), 257 (k_gemm_x3d, 256x128 with direct-to-LDS loads) or 258 (k_gemm_x3p, 256x256
+// pipelined), each with a tile-walk group of `group` tile rows.
 static bool valid_tile(int t) { return t == 128 || t == 256 || t == 257 || t == 258; }
-int gemm_x3_tile(int requested)
-{
-    if (valid_tile(requested)) return requested;
-    const char* e = getenv("T2X_GEMM_TILE");
-    const int v = e ? atoi(e) : 0;
-    return valid_tile(v) ? v : 258;
-}
 
-int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const float* As, long long lda,
-                   long long a_rows, const void* Bh, const void* Bl, const float* Bs, long long ldb, int K,
-                   int Mrows_pad, int Nrows_pad, const EpiParams& ep, hipStream_t st, int tile,
-                   const int* rowmap)
+static int launch_x3_variant(int prec, int epi, const SplitOperand& A, const SplitOperand& B, int K, int Mrows_pad,
+                             int Nrows_pad, const EpiParams& ep, hipStream_t st, int tile, int g, const int* rowmap)
 {
-    tile = gemm_x3_tile(tile);
-    // the pipelined kernel addresses operands with 32-bit byte offsets
-    const long long lim = 1LL << 32;
-    if (tile == 258 && (a_rows * lda * 2 >= lim || (long long)Nrows_pad * ldb * 2 >= lim)) tile = 256;
-    if (rowmap && tile != 258) tile = 256;   // row-mapped launches use a 256x256 kernel
     const int tile_m = tile == 257 ? DBM : (tile == 258 ? 256 : tile);
     const int tile_n = tile == 257 ? DBN : (tile == 258 ? 256 : tile);
     if (K % XK != 0 || Mrows_pad % tile_m != 0 || Nrows_pad % tile_n != 0) {
@@ -1076,44 +1066,126 @@ int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const floa
     const int tm = Mrows_pad / tile_m, tn = Nrows_pad / tile_n;
     const long long nwg = (long long)tm * tn;
     if (nwg > 0x7fffffffLL) { set_error("gemm: grid too large"); return T2X_EINVAL; }
-    SplitOperand A{Ah, Al, As, lda}, B{Bh, Bl, Bs, ldb};
-    const int g = gemm_group_m();
+    const bool bf = prec == T2X_PREC_BF16X3, store = epi == EPI_STORE;
+    const unsigned grid = (unsigned)nwg;
     if (tile == 257) {
-        const int gw = std::max(1, g / 2);
-        if (prec == T2X_PREC_BF16X3) {
-            if (epi == EPI_STORE) k_gemm_x3d<__bf16, EPI_STORE><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, gw, ep);
-            else k_gemm_x3d<__bf16, EPI_DIFF2><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, gw, ep);
-        } else {
-            if (epi == EPI_STORE) k_gemm_x3d<_Float16, EPI_STORE><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, gw, ep);
-            else k_gemm_x3d<_Float16, EPI_DIFF2><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, gw, ep);
-        }
+        if (bf) { if (store) k_gemm_x3d<__bf16, EPI_STORE><<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep);
+                  else k_gemm_x3d<__bf16, EPI_DIFF2><<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep); }
+        else    { if (store) k_gemm_x3d<_Float16, EPI_STORE><<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep);
+                  else k_gemm_x3d<_Float16, EPI_DIFF2><<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep); }
     } else if (tile == 258) {
-        const int gw = std::max(1, g / 2);
-        if (prec == T2X_PREC_BF16X3) {
-            if (epi == EPI_STORE) k_gemm_x3p<__bf16, EPI_STORE><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep, rowmap);
-            else k_gemm_x3p<__bf16, EPI_DIFF2><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep, rowmap);
-        } else {
-            if (epi == EPI_STORE) k_gemm_x3p<_Float16, EPI_STORE><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep, rowmap);
-            else k_gemm_x3p<_Float16, EPI_DIFF2><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep, rowmap);
-        }
+        if (bf) { if (store) k_gemm_x3p<__bf16, EPI_STORE><<<grid, 512, 0, st>>>(A, B, K, tm, tn, g, ep, rowmap);
+                  else k_gemm_x3p<__bf16, EPI_DIFF2><<<grid, 512, 0, st>>>(A, B, K, tm, tn, g, ep, rowmap); }
+        else    { if (store) k_gemm_x3p<_Float16, EPI_STORE><<<grid, 512, 0, st>>>(A, B, K, tm, tn, g, ep, rowmap);
+                  else k_gemm_x3p<_Float16, EPI_DIFF2><<<grid, 512, 0, st>>>(A, B, K, tm, tn, g, ep, rowmap); }
     } else if (tile == 256) {
-        const int gw = std::max(1, g / 2);
-        if (prec == T2X_PREC_BF16X3) {
-            if (epi == EPI_STORE) k_gemm_x3w<__bf16, EPI_STORE><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep, rowmap);
-            else k_gemm_x3w<__bf16, EPI_DIFF2><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep, rowmap);
-        } else {
-            if (epi == EPI_STORE) k_gemm_x3w<_Float16, EPI_STORE><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep, rowmap);
-            else k_gemm_x3w<_Float16, EPI_DIFF2><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, gw, ep, rowmap);
-        }
-    } else if (prec == T2X_PREC_BF16X3) {
-        if (epi == EPI_STORE) k_gemm_x3<__bf16, EPI_STORE><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, g, ep);
-        else k_gemm_x3<__bf16, EPI_DIFF2><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, g, ep);
+        if (bf) { if (store) k_gemm_x3w<__bf16, EPI_STORE><<<grid, 512, 0, st>>>(A, B, K, tm, tn, g, ep, rowmap);
+                  else k_gemm_x3w<__bf16, EPI_DIFF2><<<grid, 512, 0, st>>>(A, B, K, tm, tn, g, ep, rowmap); }
+        else    { if (store) k_gemm_x3w<_Float16, EPI_STORE><<<grid, 512, 0, st>>>(A, B, K, tm, tn, g, ep, rowmap);
+                  else k_gemm_x3w<_Float16, EPI_DIFF2><<<grid, 512, 0, st>>>(A, B, K, tm, tn, g, ep, rowmap); }
     } else {
-        if (epi == EPI_STORE) k_gemm_x3<_Float16, EPI_STORE><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, g, ep);
-        else k_gemm_x3<_Float16, EPI_DIFF2><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, g, ep);
+        if (bf) { if (store) k_gemm_x3<__bf16, EPI_STORE><<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep);
+                  else k_gemm_x3<__bf16, EPI_DIFF2><<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep); }
+        else    { if (store) k_gemm_x3<_Float16, EPI_STORE><<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep);
+                  else k_gemm_x3<_Float16, EPI_DIFF2><<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep); }
     }
     T2X_HIP(hipGetLastError());
     return T2X_OK;
+}
+
+// Autotuner. For each (device, precision, epilogue, N, K, row map or not) the first launch runs
+// once untimed (so that lazy-store state has settled) and then times every eligible candidate
+// once with HIP events on the caller's stream, on the real operands (each run writes the same
+// outputs); the fastest is kept for the rest of the process. N and K fix the per-tile work; M
+// only scales the number of tiles, so batches of other particle counts reuse the choice.
+struct TuneKey {
+    int dev, prec, epi, N, K, mapped;
+    bool operator<(const TuneKey& o) const
+    {
+        return std::tie(dev, prec, epi, N, K, mapped) < std::tie(o.dev, o.prec, o.epi, o.N, o.K, o.mapped);
+    }
+};
+static std::mutex g_tune_mtx;
+static std::map<TuneKey, GemmChoice> g_tune;
+
+static const GemmChoice kCandidates[] = {{258, 32}, {258, 16}, {258, 64}, {256, 32}, {257, 32}, {128, 64}};
+
+static bool eligible(const GemmChoice& c, bool mapped, bool off32)
+{
+    if (mapped && c.tile != 258 && c.tile != 256) return false;   // row maps: 256x256 kernels only
+    if (c.tile == 258 && !off32) return false;                      // 32-bit operand offsets
+    return true;
+}
+
+int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const float* As, long long lda,
+                   long long a_rows, const void* Bh, const void* Bl, const float* Bs, long long ldb, int K,
+                   int Mrows_pad, int Nrows_pad, const EpiParams& ep, hipStream_t st, GemmChoice want,
+                   const int* rowmap, GemmChoice* used)
+{
+    const long long lim = 1LL << 32;   // the pipelined kernel addresses operands with 32-bit byte offsets
+    const bool off32 = a_rows * lda * 2 < lim && (long long)Nrows_pad * ldb * 2 < lim;
+    const bool mapped = rowmap != nullptr;
+    SplitOperand A{Ah, Al, As, lda}, B{Bh, Bl, Bs, ldb};
+    GemmChoice c = want;
+    if (c.tile == 0) {   // environment override (tile sweeps), else the autotuner
+        const char* e = getenv("T2X_GEMM_TILE");
+        const int v = e ? atoi(e) : 0;
+        if (valid_tile(v)) c = {v, 0};
+    }
+    if (c.tile != 0) {
+        if (!valid_tile(c.tile)) { set_error("gemm_x3: unknown tile variant"); return T2X_EINVAL; }
+        if (!eligible(c, mapped, off32)) c.tile = 256;
+        if (c.group <= 0) c.group = c.tile == 128 ? 64 : 32;
+        if (used) *used = c;
+        return launch_x3_variant(prec, epi, A, B, K, Mrows_pad, Nrows_pad, ep, st, c.tile, c.group, rowmap);
+    }
+    int dev = 0;
+    T2X_HIP(hipGetDevice(&dev));
+    const TuneKey key{dev, prec, epi, Nrows_pad, K, mapped ? 1 : 0};
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mtx);
+        auto it = g_tune.find(key);
+        if (it != g_tune.end()) {
+            if (used) *used = it->second;
+            return launch_x3_variant(prec, epi, A, B, K, Mrows_pad, Nrows_pad, ep, st, it->second.tile,
+                                     it->second.group, rowmap);
+        }
+    }
+    GemmChoice best{0, 0};
+    float best_ms = 3.0e38f;
+    hipEvent_t e0, e1;
+    T2X_HIP(hipEventCreate(&e0));
+    T2X_HIP(hipEventCreate(&e1));
+    int rc = launch_x3_variant(prec, epi, A, B, K, Mrows_pad, Nrows_pad, ep, st, 258 * off32 + 256 * !off32,
+                               32, rowmap);
+    for (const GemmChoice& cand : kCandidates) {
+        if (rc != T2X_OK) break;
+        if (!eligible(cand, mapped, off32)) continue;
+        const int tile_m = cand.tile == 257 ? DBM : (cand.tile == 258 ? 256 : cand.tile);
+        const int tile_n = cand.tile == 257 ? DBN : (cand.tile == 258 ? 256 : cand.tile);
+        if (Mrows_pad % tile_m || Nrows_pad % tile_n) continue;
+        if (hipEventRecord(e0, st) != hipSuccess) { set_error("gemm_x3 autotune: event record failed"); rc = T2X_EDEVICE; break; }
+        rc = launch_x3_variant(prec, epi, A, B, K, Mrows_pad, Nrows_pad, ep, st, cand.tile, cand.group, rowmap);
+        if (rc != T2X_OK) break;
+        float ms = 0.f;
+        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+            set_error("gemm_x3 autotune: event timing failed");
+            rc = T2X_EDEVICE;
+            break;
+        }
+        if (ms < best_ms) { best_ms = ms; best = cand; }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc != T2X_OK) return rc;
+    if (best.tile == 0) { set_error("gemm_x3: no eligible tile variant"); return T2X_EINVAL; }
+    {
+        std::lock_guard<std::mutex> lk(g_tune_mtx);
+        g_tune[key] = best;
+    }
+    if (used) *used = best;
+    return T2X_OK;   // the candidates' runs have written the outputs
 }
 
 }  // namespace t2x

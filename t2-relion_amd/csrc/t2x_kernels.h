@@ -63,6 +63,12 @@ __device__ __forceinline__ bool coarse_allowed(const WeightSrc& s, int b, long l
 }
 
 // Epilogue parameters of the coarse cross-term GEMMs.
+// Variant of the split-precision GEMM: tile 128 / 256 / 257 / 258 (0: autotuned) and tile-walk
+// group in tile rows (0: the variant's default).
+struct GemmChoice {
+    int tile, group;
+};
+
 struct EpiParams {
     float* C;            // output
     long long ldc;

@@ -48,7 +48,8 @@ class EstepConfig(C.Structure):
         ("do_ctf_correction", C.c_int), ("sigma2_offset", C.c_double),
         ("orientational_prior_mode", C.c_int), ("precision", C.c_int),
         ("proposal_topk", C.c_int), ("proposal_gate", C.c_int),
-        ("proposal_min_conf", C.c_double), ("reserved", C.c_int * 4),
+        ("proposal_min_conf", C.c_double), ("gemm_tile", C.c_int), ("gemm_group", C.c_int),
+        ("reserved", C.c_int * 2),
     ]
 
 
@@ -113,7 +114,7 @@ class Stats(C.Structure):
                                    "fine_flop", "bp_orientations", "sig_coarse")] + \
         [(n, C.c_int) for n in ("coarse_size", "current_size", "P_coarse", "P_fine")] + \
         [("ms_proposal", C.c_double), ("n_proposal_particles", C.c_long), ("n_proposal_fallback", C.c_long),
-         ("proposal_bins_scored", C.c_double)]
+         ("proposal_bins_scored", C.c_double), ("gemm_tile_used", C.c_int), ("gemm_group_used", C.c_int)]
 
 
 class ReconParams(C.Structure):

@@ -1,0 +1,61 @@
+"""Tile variants of the split-precision coarse GEMM, and the autotuner that picks among them.
+
+Every variant (128x128 k_gemm_x3, 256x256 k_gemm_x3w, 256x128 direct-to-LDS k_gemm_x3d, 256x256
+pipelined k_gemm_x3p, each with any tile-walk group) feeds every output element the same chain
+of MFMAs (lo.hi, hi.lo, hi.hi per K step of 32, K steps in order) and the same FP64 epilogue
+arithmetic, so the coarse diff2 matrix (all of it stored when diagnostics are requested), the
+selected poses and dLL must be bit-identical between variants; the autotuner, which times the
+eligible variants on the real operands at the first launch of a shape, must report one of them
+and change nothing in the results.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import estep_harness as H
+import t2x_py as T
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module", params=[T.T2X_PREC_FP16X3, T.T2X_PREC_BF16X3], ids=["fp16x3", "bf16x3"])
+def case(request, gpu_device):
+    wl = H.Workload("refine64", n=24, seed_offset=5, precision=request.param)
+    wl.cfg.gemm_tile, wl.cfg.gemm_group = 258, 0
+    ref = wl.run_gpu(dev=gpu_device, n_debug=24, fine_cap=16384)
+    return wl, ref
+
+
+@pytest.mark.parametrize("tile,group", [(256, 0), (257, 0), (128, 0), (258, 16), (258, 64), (0, 0)])
+def test_variant_bit_identical(case, gpu_device, tile, group):
+    wl, ref = case
+    lib = T.load()
+    wl.cfg.gemm_tile, wl.cfg.gemm_group = tile, group
+    T.check(lib.t2x_stats_reset(gpu_device))
+    g = wl.run_gpu(dev=gpu_device, n_debug=24, fine_cap=16384)
+    st = T.Stats()
+    T.check(lib.t2x_stats_get(gpu_device, C.byref(st)))
+    if tile:
+        assert st.gemm_tile_used == tile and (group == 0 or st.gemm_group_used == group)
+    else:
+        assert st.gemm_tile_used in (128, 256, 257, 258) and st.gemm_group_used > 0
+    print(f"\n[gemm variant] requested ({tile}, {group}) used ({st.gemm_tile_used}, {st.gemm_group_used})")
+    assert np.array_equal(g["dbg"]["coarse_diff2"], ref["dbg"]["coarse_diff2"])
+    assert np.array_equal(g["out"]["best_hidden"], ref["out"]["best_hidden"])
+    assert np.array_equal(g["out"]["dLL"], ref["out"]["dLL"])
+    wl.cfg.gemm_tile, wl.cfg.gemm_group = 258, 0
+
+
+def test_bad_variant_rejected(case, gpu_device):
+    wl, _ = case
+    lib = T.load()
+    wl.cfg.gemm_tile = 300
+    try:
+        outs = (T.ParticleOut * wl.n)()
+        ws, _ = T.make_wsum(wl.cfg.nr_classes, wl.info.nr_dir, wl.N)
+        st = lib.t2x_estep(gpu_device, C.byref(wl.cfg), C.byref(wl.model), wl.n,
+                           wl.images.ctypes.data_as(C.c_void_p), 0, wl.pin, outs, C.byref(ws), None)
+        assert st == T.T2X_EINVAL
+    finally:
+        wl.cfg.gemm_tile = 258
