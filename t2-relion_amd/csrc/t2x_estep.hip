@@ -680,7 +680,7 @@ static void build_table(int size, int cur, const double* sigma2, int nsig2, doub
         }
     }
     t.npix = (int)t.h_x.size();
-    t.npix_pad = (int)round_up(t.npix, 16);
+    t.npix_pad = (int)round_up(t.npix, 32);   // K of the N1 GEMM: a multiple of the split GEMM's K step
 }
 
 static int upload_table(PassTables& t, hipStream_t st)
@@ -936,6 +936,12 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             T2X_TRY(d->coarse_Bs.ensure(sizeof(float) * O_pad));
             T2X_TRY(launch_split(prec, d->coarse_B.as<float>(), Kc, Kc, (int)O_pad, (int)O_all, d->coarse_Bh.ptr,
                                  d->coarse_Bl.ptr, Kc, d->coarse_Bs.as<float>(), st));
+            // |F|^2 of the N1 term, split the same way
+            T2X_TRY(d->coarse_B2h.ensure(sizeof(uint16_t) * O_pad * Kn));
+            T2X_TRY(d->coarse_B2l.ensure(sizeof(uint16_t) * O_pad * Kn));
+            T2X_TRY(d->coarse_B2s.ensure(sizeof(float) * O_pad));
+            T2X_TRY(launch_split(prec, d->coarse_B2.as<float>(), Kn, Kn, (int)O_pad, (int)O_all, d->coarse_B2h.ptr,
+                                 d->coarse_B2l.ptr, Kn, d->coarse_B2s.as<float>(), st));
         }
         T2X_HIP(hipStreamSynchronize(st));
         d_Ai.release();
@@ -1071,7 +1077,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         // ---- coarse pass ----
         const long long M = (long long)B * T;
         const long long M_pad = round_up(M, 256);
-        const long long B_pad = round_up(B, 128);
+        const long long B_pad = round_up(B, prec == T2X_PREC_FP32 ? 128 : 256);   // N1 GEMM rows
         T2X_TRY(d->A.ensure(sizeof(float) * M_pad * Kc));
         T2X_TRY(d->A1.ensure(sizeof(float) * B_pad * Kn));
         T2X_TRY(d->N1.ensure(sizeof(float) * B_pad * O_pad));
@@ -1095,8 +1101,20 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             ep.ldc = O_pad;
             ep.M = B;
             ep.N = (int)O_all;
-            T2X_TRY(launch_gemm(EPI_STORE, d->A1.as<float>(), Kn, d->coarse_B2.as<float>(), Kn, Kn,
-                                (int)B_pad, (int)O_pad, ep, st));
+            if (prec == T2X_PREC_FP32) {
+                T2X_TRY(launch_gemm(EPI_STORE, d->A1.as<float>(), Kn, d->coarse_B2.as<float>(), Kn, Kn,
+                                    (int)B_pad, (int)O_pad, ep, st));
+            } else {   // N1 = sum_p (minv ctf^2 / 2) |F|^2 in the same split precision as the cross term
+                T2X_TRY(d->A1h.ensure(sizeof(uint16_t) * B_pad * Kn));
+                T2X_TRY(d->A1l.ensure(sizeof(uint16_t) * B_pad * Kn));
+                T2X_TRY(d->A1s.ensure(sizeof(float) * B_pad));
+                T2X_TRY(launch_split(prec, d->A1.as<float>(), Kn, Kn, (int)B_pad, (int)B, d->A1h.ptr, d->A1l.ptr, Kn,
+                                     d->A1s.as<float>(), st));
+                T2X_TRY(launch_gemm_x3(prec, EPI_STORE, d->A1h.ptr, d->A1l.ptr, d->A1s.as<float>(), Kn, B_pad,
+                                       d->coarse_B2h.ptr, d->coarse_B2l.ptr, d->coarse_B2s.as<float>(), Kn, Kn,
+                                       (int)B_pad, (int)O_pad, ep, st,
+                                       GemmChoice{cfg->gemm_tile, cfg->gemm_group}, nullptr, nullptr));
+            }
         }
         // ---- learned proposal: bins per particle, work lists of the restricted coarse GEMM ----
         int n_work = 0, n_fb_rows = 0;

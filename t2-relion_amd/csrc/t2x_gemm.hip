@@ -1095,9 +1095,10 @@ static int launch_x3_variant(int prec, int epi, const SplitOperand& A, const Spl
 
 // Autotuner. For each (device, precision, epilogue, N, K, row map or not) the first launch runs
 // once untimed (so that lazy-store state has settled) and then times every eligible candidate
-// once with HIP events on the caller's stream, on the real operands (each run writes the same
-// outputs); the fastest is kept for the rest of the process. N and K fix the per-tile work; M
-// only scales the number of tiles, so batches of other particle counts reuse the choice.
+// twice (keeping the better run) with HIP events on the caller's stream, on the real operands
+// (each run writes the same outputs); the fastest is kept for the rest of the process. N and K
+// fix the per-tile work; M only scales the number of tiles, so batches of other particle counts
+// reuse the choice.
 struct TuneKey {
     int dev, prec, epi, N, K, mapped;
     bool operator<(const TuneKey& o) const
@@ -1164,17 +1165,23 @@ int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const floa
         const int tile_m = cand.tile == 257 ? DBM : (cand.tile == 258 ? 256 : cand.tile);
         const int tile_n = cand.tile == 257 ? DBN : (cand.tile == 258 ? 256 : cand.tile);
         if (Mrows_pad % tile_m || Nrows_pad % tile_n) continue;
-        if (hipEventRecord(e0, st) != hipSuccess) { set_error("gemm_x3 autotune: event record failed"); rc = T2X_EDEVICE; break; }
-        rc = launch_x3_variant(prec, epi, A, B, K, Mrows_pad, Nrows_pad, ep, st, cand.tile, cand.group, rowmap);
-        if (rc != T2X_OK) break;
-        float ms = 0.f;
-        if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
-            hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
-            set_error("gemm_x3 autotune: event timing failed");
-            rc = T2X_EDEVICE;
-            break;
+        for (int rep = 0; rep < 2 && rc == T2X_OK; rep++) {   // best of two runs per candidate
+            if (hipEventRecord(e0, st) != hipSuccess) {
+                set_error("gemm_x3 autotune: event record failed");
+                rc = T2X_EDEVICE;
+                break;
+            }
+            rc = launch_x3_variant(prec, epi, A, B, K, Mrows_pad, Nrows_pad, ep, st, cand.tile, cand.group, rowmap);
+            if (rc != T2X_OK) break;
+            float ms = 0.f;
+            if (hipEventRecord(e1, st) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                set_error("gemm_x3 autotune: event timing failed");
+                rc = T2X_EDEVICE;
+                break;
+            }
+            if (ms < best_ms) { best_ms = ms; best = cand; }
         }
-        if (ms < best_ms) { best_ms = ms; best = cand; }
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);

@@ -137,6 +137,8 @@ struct Device {
     DevBuf coarse_B;      // [O_all_pad][K_pad] float: re/im of the coarse slices
     DevBuf coarse_B2;     // [O_all_pad][Kn_pad] float: |F|^2 of the coarse slices
     DevBuf coarse_Bh, coarse_Bl, coarse_Bs;   // 16-bit hi/lo split of coarse_B + row scales
+    DevBuf coarse_B2h, coarse_B2l, coarse_B2s;   // the same for coarse_B2 (|F|^2 of the N1 term)
+    DevBuf A1h, A1l, A1s;                        // the same for A1 (per batch)
     int coarse_O_all = 0, coarse_O_pad = 0;
 
     // scratch
