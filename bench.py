@@ -160,6 +160,51 @@ def pose_agreement(a, b, step_deg):
     return ok / max(1, n)
 
 
+def stage_rooflines(st, cfg, info, out, n_steps):
+    """Achieved rate of every E-step stage against its MI355X bound, from the stage's GPU event
+    time and the algorithmic work per unit of the counting convention (HBM bytes or flops):
+    projector O*P_c*(8*8 + 8) bytes; coarse / fine exp-weight and significance S*(4 + 8 + 32)
+    bytes on the dense sample counts (the selection skips negligible blocks, so its rate can
+    exceed the bandwidth: work avoided, not bandwidth); fine diff2 4*S_f*P_f flops; wavg
+    20*S_sig*P_f + 40*O_sig*P_f flops and backprojection 192*O_sig*P_f bytes (one stage,
+    k_store, bounded by its atomics: the rates are reported against both peaks)."""
+    K, O = cfg.nr_classes, cfg.nr_classes * info.nr_dir * info.nr_psi
+    n = st.n_particles
+    s_coarse = float(n) * O * info.nr_trans
+    s_sig = float(np.sum(out["nr_fine_significant"])) * n_steps
+    o_sig = st.bp_orientations
+    hbm, fp32 = 8000.0, 157.3
+
+    def gbs(b, ms):
+        return b / (ms * 1e-3) / 1e9 if ms > 0 else None
+
+    def tfs(f, ms):
+        return f / (ms * 1e-3) / 1e12 if ms > 0 else None
+
+    res = {}
+    b = O * st.P_coarse * 72.0 * max(1, st.n_calls) if st.ms_coarse_project > 0 else 0.0
+    res["projector (coarse slices, k_project_coarse)"] = {
+        "ms": st.ms_coarse_project, "GB/s": gbs(b, st.ms_coarse_project),
+        "frac_hbm": (gbs(b, st.ms_coarse_project) or 0) / hbm, "note": "once per call; cached across calls"}
+    b = s_coarse * 44.0
+    res["coarse weights + significance (FP64)"] = {
+        "ms": st.ms_coarse_weights, "GB/s": gbs(b, st.ms_coarse_weights),
+        "frac_hbm": (gbs(b, st.ms_coarse_weights) or 0) / hbm}
+    res["fine diff2 (k_fine_diff2)"] = {
+        "ms": st.ms_fine_diff2, "TFLOP/s": tfs(st.fine_flop, st.ms_fine_diff2),
+        "frac_fp32": (tfs(st.fine_flop, st.ms_fine_diff2) or 0) / fp32}
+    b = st.fine_samples * 44.0
+    res["fine weights + significance (FP64)"] = {
+        "ms": st.ms_fine_weights, "GB/s": gbs(b, st.ms_fine_weights),
+        "frac_hbm": (gbs(b, st.ms_fine_weights) or 0) / hbm}
+    f = 20.0 * s_sig * st.P_fine + 40.0 * o_sig * st.P_fine
+    b = 192.0 * o_sig * st.P_fine
+    res["wavg + backprojection (k_store)"] = {
+        "ms": st.ms_store, "TFLOP/s": tfs(f, st.ms_store), "frac_fp32": (tfs(f, st.ms_store) or 0) / fp32,
+        "GB/s": gbs(b, st.ms_store), "frac_hbm": (gbs(b, st.ms_store) or 0) / hbm}
+    return res
+
+
 def fsc_resolution(fsc, ori, angpix):
     """resolution (A) of the first shell below 0.143 (shell 1 onwards)"""
     for s in range(1, len(fsc)):
@@ -357,6 +402,7 @@ def main():
             "traffic_unit": "bytes per launch", "traffic_source": traffic_src,
             "kernel": "coarse diff2 cross term, " + KERNELS[head],
             "avg_launch_ms": avg_ms, "launches": launches, "flop_per_launch": flop_per_launch}
+    stage_roof = stage_rooflines(st, cfg, ds["info"], r["out"], args.steps)
     stages = {k: getattr(st, k) for k in ("ms_preprocess", "ms_coarse_project", "ms_coarse_diff2",
                                            "ms_coarse_weights", "ms_fine_diff2", "ms_fine_weights",
                                            "ms_store", "ms_proposal")}
@@ -428,6 +474,7 @@ def main():
         "quality": quality,
         "targets": targets,
         "roofline": roof,
+        "stage_roofline": stage_roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
