@@ -16,7 +16,7 @@ LIB = os.path.join(OUT_DIR, "libt2x.so")
 
 SOURCES = ["t2x_device.hip", "t2x_sampling.cpp", "t2x_gemm.hip", "t2x_select.hip",
            "t2x_estep.hip", "t2x_synth.hip", "t2x_recon.hip",
-           "t2x_cnn.hip", "t2x_proposal.hip"]
+           "t2x_cnn.hip", "t2x_proposal.hip", "t2x_io.cpp"]
 
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-munsafe-fp-atomics",

@@ -117,6 +117,12 @@ class Stats(C.Structure):
          ("proposal_bins_scored", C.c_double), ("gemm_tile_used", C.c_int), ("gemm_group_used", C.c_int)]
 
 
+class MrcInfo(C.Structure):
+    _fields_ = [("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int), ("mode", C.c_int),
+                ("angpix", C.c_double), ("data_offset", C.c_long), ("swapped", C.c_int),
+                ("dmin", C.c_double), ("dmax", C.c_double), ("dmean", C.c_double), ("rms", C.c_double)]
+
+
 class ReconParams(C.Structure):
     _fields_ = [("ori_size", C.c_int), ("pad_size", C.c_int), ("r_max", C.c_int),
                 ("padding_factor", C.c_double), ("ref_dim", C.c_int), ("data_dim", C.c_int),
@@ -231,6 +237,20 @@ def load(path=None):
         "t2x_proposal_train": (C.c_int, [vp, C.POINTER(EstepConfig), C.POINTER(Model),
                                          C.POINTER(ProposalTrainParams), C.POINTER(ProposalTrainStats)]),
         "t2x_reconstruct": (C.c_int, [vp, C.POINTER(ReconParams), PD, PD, PD, PD, PD]),
+        "t2x_mrc_info_read": (C.c_int, [C.c_char_p, C.POINTER(MrcInfo)]),
+        "t2x_mrc_read_sections": (C.c_int, [C.c_char_p, C.c_long, C.c_long, C.POINTER(C.c_float)]),
+        "t2x_mrc_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_float)]),
+        "t2x_star_read": (C.c_int, [C.c_char_p, C.POINTER(vp)]),
+        "t2x_star_free": (C.c_int, [vp]),
+        "t2x_star_write": (C.c_int, [vp, C.c_char_p]),
+        "t2x_star_num_rows": (C.c_int, [vp, C.c_char_p, C.POINTER(C.c_long)]),
+        "t2x_star_get_double": (C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_long, PD]),
+        "t2x_star_get_string": (C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_long, C.c_char_p, C.c_long]),
+        "t2x_star_set_double": (C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_long, C.c_double]),
+        "t2x_star_set_int": (C.c_int, [vp, C.c_char_p, C.c_char_p, C.c_long, C.c_long]),
+        "t2x_star_particles": (C.c_int, [vp, C.c_long, C.c_long, C.POINTER(ParticleIn), PD]),
+        "t2x_star_image_ref": (C.c_int, [vp, C.c_long, C.POINTER(C.c_long), C.c_char_p, C.c_long]),
+        "t2x_star_store_results": (C.c_int, [vp, C.c_long, C.c_long, C.POINTER(ParticleOut), C.c_int]),
         "t2x_fsc": (C.c_int, [vp, C.c_int, PD, PD, PD]),
         "t2x_synth_reference_device": (C.c_int, [vp, C.c_int, C.c_double, C.c_int, C.c_int,
                                                  C.c_double, C.POINTER(Blob), C.c_int, C.c_double,

@@ -1,7 +1,8 @@
-"""The C-ABI library without a GPU: it loads, exports every entry point include/t2x.h declares,
+"""The C-ABI library without a GPU: it loads, exports every entry point include/*.h declares,
 rejects bad arguments with status codes, and its host-side sampling grids equal the oracle's
 bit for bit."""
 import ctypes as C
+import glob
 import os
 import re
 
@@ -15,9 +16,11 @@ REPO = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def declared_functions():
-    src = open(os.path.join(REPO, "include", "t2x.h")).read()
-    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(t2x_\w+)\s*\(", src, flags=re.M)))
+    names = set()
+    for h in sorted(glob.glob(os.path.join(REPO, "include", "*.h"))):
+        src = re.sub(r"/\*.*?\*/", "", open(h).read(), flags=re.S)
+        names |= set(re.findall(r"^\s*(?:int|const char\*)\s+(t2x_\w+)\s*\(", src, flags=re.M))
+    return sorted(names)
 
 
 def test_library_exports_every_declared_symbol():
