@@ -236,7 +236,7 @@ static int check_config(const t2x_estep_config* c)
     if (c->ori_size < 4 || (c->ori_size & 1)) return T2X_EINVAL;
     if (c->current_size < 2 || (c->current_size & 1) || c->current_size > c->ori_size) return T2X_EINVAL;
     if (c->coarse_size < 2 || (c->coarse_size & 1) || c->coarse_size > c->current_size) return T2X_EINVAL;
-    if (c->ref_dim != 3) return T2X_EINVAL;
+    if (c->ref_dim != 3 && c->ref_dim != 2) return T2X_EINVAL;
     if (c->nr_classes < 1) return T2X_EINVAL;
     if (c->adaptive_oversampling < 0 || c->adaptive_oversampling > 1) return T2X_EINVAL;
     if (c->offset_step <= 0 || c->offset_range < 0 || c->psi_step <= 0) return T2X_EINVAL;
@@ -516,6 +516,45 @@ static inline bool project_pixel(const orc_volume* v, const double* Ai, int x, i
     return true;
 }
 
+/* Projector::rotate2D (projector.cpp:904-1016), bilinear branch, for 2D references
+ * (pad x (pad/2+1), y centred). Its radius test truncates xp^2 + yp^2 to an integer (the
+ * reference declares `const int r_ref_2`); the clean form applies the same truncation to the
+ * rotation-invariant pf^2 (x^2 + y^2). */
+static inline bool project_pixel_2d(const orc_volume* v, const double* Ai, int x, int y, bool clean,
+                                    cplx* res)
+{
+    const int pad = v->pad_size, xdim = pad / 2 + 1, start = -(pad / 2);
+    double xp = Ai[0] * x + Ai[1] * y;
+    double yp = Ai[3] * x + Ai[4] * y;
+    const int r_max_ref = (int)(v->r_max * v->padding_factor);
+    const int r_max_ref_2 = r_max_ref * r_max_ref;
+    if (clean) {
+        double r2 = v->padding_factor * v->padding_factor * (double)(x * x + y * y);
+        if ((int)r2 > r_max_ref_2) return false;
+    } else {
+        const int r_ref_2 = (int)(xp * xp + yp * yp);
+        if (r_ref_2 > r_max_ref_2) return false;
+    }
+    bool is_neg_x = xp < 0;
+    if (is_neg_x) { xp = -xp; yp = -yp; }
+    int x0 = R_FLOOR(xp);
+    double fx = xp - x0;
+    int x1 = x0 + 1;
+    int y0 = R_FLOOR(yp);
+    double fy = yp - y0;
+    y0 -= start;
+    int y1 = y0 + 1;
+    if (x0 < 0 || x1 >= xdim || y0 < 0 || y1 >= pad) return false;   /* never within r_max */
+    const cplx* d = reinterpret_cast<const cplx*>(v->data);
+    cplx d00 = d[(size_t)y0 * xdim + x0], d01 = d[(size_t)y0 * xdim + x1];
+    cplx d10 = d[(size_t)y1 * xdim + x0], d11 = d[(size_t)y1 * xdim + x1];
+    cplx dx0 = d00 + (d01 - d00) * fx;
+    cplx dx1 = d10 + (d11 - d10) * fx;
+    cplx val = dx0 + (dx1 - dx0) * fy;
+    *res = is_neg_x ? std::conj(val) : val;
+    return true;
+}
+
 static void project_into(const orc_volume* v, const double* A, int size, cplx* f2d, bool clean)
 {
     double Ai[9];
@@ -529,7 +568,9 @@ static void project_into(const orc_volume* v, const double* A, int size, cplx* f
         const int x_max = R_FLOOR(std::sqrt((double)(r_max_out_2 - y * y)));
         for (int x = 0; x <= x_max; x++) {
             cplx val;
-            if (project_pixel(v, Ai, x, y, clean, &val)) f2d[i * xs + x] = val;
+            const bool in = v->ref_dim == 2 ? project_pixel_2d(v, Ai, x, y, clean, &val)
+                                             : project_pixel(v, Ai, x, y, clean, &val);
+            if (in) f2d[i * xs + x] = val;
         }
     }
 }
@@ -611,10 +652,71 @@ static void backproject_from(orc_volume* v, const cplx* f2d, const double* fw, i
     }
 }
 
+/* BackProjector::backrotate2D (backprojector.cpp:471-668), bilinear, no magnification matrix:
+ * the ellipse x-range of every row (integer-truncated r_max * pf), no further radius test. */
+static void backrotate_from(orc_volume* v, const cplx* f2d, const double* fw, int s, const double* A)
+{
+    const double pf = v->padding_factor;
+    double Ai[9];
+    inv3(A, Ai);
+    for (int i = 0; i < 9; i++) Ai[i] *= pf;
+    const int r_max_ref = (int)(v->r_max * pf);
+    const int r_max_ref_2 = r_max_ref * r_max_ref;
+    const double Am_Xx = Ai[0] * 1.0 + Ai[1] * 0.0, Am_Xy = Ai[0] * 0.0 + Ai[1] * 1.0;
+    const double Am_Yx = Ai[3] * 1.0 + Ai[4] * 0.0, Am_Yy = Ai[3] * 0.0 + Ai[4] * 1.0;
+    const double AtA_xx = Am_Xx * Am_Xx + Am_Yx * Am_Yx;
+    const double AtA_xy = Am_Xx * Am_Xy + Am_Yx * Am_Yy;
+    const double AtA_yy = Am_Xy * Am_Xy + Am_Yy * Am_Yy;
+    const double AtA_xy2 = AtA_xy * AtA_xy;
+    const int sh = s / 2 + 1;
+    const int pad = v->pad_size, xdim = pad / 2 + 1, start = -(pad / 2);
+    cplx* d = reinterpret_cast<cplx*>(v->data);
+    double* w = v->weight;
+    for (int i = 0; i < s; i++) {
+        int y, first_allowed_x;
+        if (i < sh) { y = i; first_allowed_x = 0; }
+        else { y = i - s; first_allowed_x = 1; }
+        double discr = AtA_xy2 * y * y - AtA_xx * (AtA_yy * y * y - r_max_ref_2);
+        if (discr < 0.0) continue;
+        double dd = std::sqrt(discr) / AtA_xx;
+        double q = -AtA_xy * y / AtA_xx;
+        int first_x = R_CEIL(q - dd);
+        int last_x = R_FLOOR(q + dd);
+        if (first_x < first_allowed_x) first_x = first_allowed_x;
+        if (last_x > sh - 1) last_x = sh - 1;
+        for (int x = first_x; x <= last_x; x++) {
+            double my_weight = fw ? fw[i * sh + x] : 1.0;
+            if (my_weight <= 0.) continue;
+            cplx my_val = f2d[i * sh + x];
+            double xu = 1.0 * x + 0.0 * y, yu = 0.0 * x + 1.0 * y;
+            double xp = Ai[0] * xu + Ai[1] * yu;
+            double yp = Ai[3] * xu + Ai[4] * yu;
+            bool is_neg_x = xp < 0;
+            if (is_neg_x) { xp = -xp; yp = -yp; }
+            int x0 = R_FLOOR(xp);
+            double fx = xp - x0;
+            int x1 = x0 + 1;
+            int y0 = R_FLOOR(yp);
+            double fy = yp - y0;
+            y0 -= start;
+            int y1 = y0 + 1;
+            if (x0 < 0 || x1 >= xdim || y0 < 0 || y1 >= pad) continue;   /* never within r_max */
+            double mfx = 1. - fx, mfy = 1. - fy;
+            double dd00 = mfy * mfx, dd01 = mfy * fx, dd10 = fy * mfx, dd11 = fy * fx;
+            if (is_neg_x) my_val = std::conj(my_val);
+            d[(size_t)y0 * xdim + x0] += dd00 * my_val; d[(size_t)y0 * xdim + x1] += dd01 * my_val;
+            d[(size_t)y1 * xdim + x0] += dd10 * my_val; d[(size_t)y1 * xdim + x1] += dd11 * my_val;
+            w[(size_t)y0 * xdim + x0] += dd00 * my_weight; w[(size_t)y0 * xdim + x1] += dd01 * my_weight;
+            w[(size_t)y1 * xdim + x0] += dd10 * my_weight; w[(size_t)y1 * xdim + x1] += dd11 * my_weight;
+        }
+    }
+}
+
 extern "C" void orc_backproject(orc_volume* v, const double* f2d, const double* fweight, int size,
                                 const double* A9)
 {
-    backproject_from(v, reinterpret_cast<const cplx*>(f2d), fweight, size, A9);
+    if (v->ref_dim == 2) backrotate_from(v, reinterpret_cast<const cplx*>(f2d), fweight, size, A9);
+    else backproject_from(v, reinterpret_cast<const cplx*>(f2d), fweight, size, A9);
 }
 
 /* =====================================================================================
@@ -1014,6 +1116,12 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
 
     Prep P;
     preprocess(cfg, image, pin, P);
+    if (cfg->ref_dim == 2) {
+        /* 2D references: the offset prior is the class's (ml_optimiser.cpp:7627-7631, 8283-8287);
+         * the same for every class here (checked by the caller) */
+        P.prior_x = sh.model->prior_offset_class ? sh.model->prior_offset_class[0] : 0.;
+        P.prior_y = sh.model->prior_offset_class ? sh.model->prior_offset_class[1] : 0.;
+    }
 
     /* orientation prior (NOPRIOR): pdf_direction[class][idir]; offset prior per translation */
     auto pdf_orient = [&](int c, long idir) { return sh.model->pdf_direction[(size_t)c * Nd + idir]; };
@@ -1236,6 +1344,7 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
     std::vector<double> thr_sigma2(nsig2, 0.);
     std::vector<double> thr_pdf_dir((size_t)K * Nd, 0.), thr_pdf_class(K, 0.);
     double thr_sumw_group = 0, thr_sigma2_offset = 0, wsum_norm = 0;
+    std::vector<double> thr_prior_off((size_t)2 * K, 0.);   /* 2D references (ml_optimiser.cpp:8504-8508) */
     double max_weight = -1.;
     long best_hidden = -1;
     std::vector<cplx> Fref(np, 0.), Frefctf(np), Facc(np);
@@ -1287,6 +1396,10 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
                             }
                             thr_sumw_group += weight;
                             thr_pdf_class[c] += weight;
+                            if (cfg->ref_dim == 2) {
+                                thr_prior_off[2 * c] += weight * pix * (P.old_x + trx[q]);
+                                thr_prior_off[2 * c + 1] += weight * pix * (P.old_y + try_[q]);
+                            }
                             double dx = P.prior_x - P.old_x - trx[q];
                             thr_sigma2_offset += weight * pix * pix * dx * dx;
                             double dy = P.prior_y - P.old_y - try_[q];
@@ -1311,7 +1424,8 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
                         }
                     {
                         std::lock_guard<std::mutex> lk((*sh.class_mutex)[c]);
-                        backproject_from(&sh.bp[c], Facc.data(), Fw.data(), s, A);
+                        if (sh.bp[c].ref_dim == 2) backrotate_from(&sh.bp[c], Facc.data(), Fw.data(), s, A);
+                        else backproject_from(&sh.bp[c], Facc.data(), Fw.data(), s, A);
                     }
                 }
             }
@@ -1349,6 +1463,8 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
         if (wsum->pdf_direction)
             for (size_t i = 0; i < thr_pdf_dir.size(); i++) wsum->pdf_direction[i] += thr_pdf_dir[i];
         if (wsum->sigma2_offset) wsum->sigma2_offset[0] += thr_sigma2_offset;
+        if (wsum->prior_offset_class && cfg->ref_dim == 2)
+            for (int c = 0; c < 2 * K; c++) wsum->prior_offset_class[c] += thr_prior_off[c];
         if (wsum->avg_norm_correction && cfg->do_norm_correction) wsum->avg_norm_correction[0] += normcorr;
         if (wsum->LL) wsum->LL[0] += dLL;
         if (wsum->ave_Pmax) wsum->ave_Pmax[0] += out.pmax;
@@ -1362,6 +1478,13 @@ extern "C" int orc_estep(const t2x_estep_config* cfg, const t2x_model* model,
                          t2x_estep_debug* dbg, int proj_mode, int nthreads)
 {
     if (check_config(cfg) != T2X_OK || !model || !proj || !bp || n < 0) return T2X_EINVAL;
+    if (cfg->ref_dim == 2 && model->prior_offset_class)
+        for (int c = 1; c < cfg->nr_classes; c++)
+            if (model->prior_offset_class[2 * c] != model->prior_offset_class[0] ||
+                model->prior_offset_class[2 * c + 1] != model->prior_offset_class[1])
+                return T2X_EINVAL;   /* class-dependent offset priors are not supported */
+    for (int c = 0; c < cfg->nr_classes; c++)
+        if (proj[c].ref_dim != cfg->ref_dim || bp[c].ref_dim != cfg->ref_dim) return T2X_EINVAL;
     Shared sh;
     sh.cfg = cfg;
     sh.model = model;

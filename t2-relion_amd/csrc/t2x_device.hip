@@ -91,7 +91,7 @@ static size_t vol_elems(int ref_dim, int pad)
 static int check_dims(int iclass, int ref_dim, int pad, int r_max, double pf)
 {
     if (iclass < 0) { set_error("negative class index"); return T2X_EINVAL; }
-    if (ref_dim != 3) { set_error("only 3D references are supported"); return T2X_EINVAL; }
+    if (ref_dim != 3 && ref_dim != 2) { set_error("ref_dim must be 3 or 2"); return T2X_EINVAL; }
     if (pad < 3 || !(pad & 1)) { set_error("pad_size must be odd"); return T2X_EINVAL; }
     if (r_max < 1 || pf <= 0) { set_error("bad r_max or padding factor"); return T2X_EINVAL; }
     long need = 2L * ((long)std::lround(pf * r_max) + 1) + 1;

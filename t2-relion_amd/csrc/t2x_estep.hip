@@ -219,7 +219,7 @@ __global__ __launch_bounds__(256) void k_project_coarse(const float2* const* __r
                                                         int Norient, const int2* __restrict__ pix,
                                                         const unsigned char* __restrict__ inproj,
                                                         int npix, float* __restrict__ Bm, long long ldb,
-                                                        float* __restrict__ B2, long long ldb2)
+                                                        float* __restrict__ B2, long long ldb2, int ref2d)
 {
     const int n = blockIdx.x;
     const int c = n / Norient, o = n - c * Norient;
@@ -233,7 +233,7 @@ __global__ __launch_bounds__(256) void k_project_coarse(const float2* const* __r
             float x = (float)pix[p].x, y = (float)pix[p].y;
             float xp = a[0] * x + a[1] * y, yp = a[2] * x + a[3] * y, zp = a[4] * x + a[5] * y;
             float2 v;
-            if (trilinear_sample(vol, pad, xp, yp, zp, &v)) f = v;
+            if (ref_sample(vol, pad, ref2d, xp, yp, zp, &v)) f = v;
         }
         Bm[(size_t)n * ldb + 2 * p] = f.x;
         Bm[(size_t)n * ldb + 2 * p + 1] = f.y;
@@ -271,6 +271,7 @@ __global__ __launch_bounds__(256) void k_coarse_imgside(const float2* __restrict
 
 struct FineGeom {
     int Norient, Npsi, n_subdir, n_subpsi, over_rot, over_trans, T;
+    int ref2d;              // 2D references (bilinear slices, backrotate2D insertion)
     const double* dir_sc;   // [Nd*n_subdir][4] sin/cos of rot and tilt
     const double* psi_sc;   // [Npsi*n_subpsi][2] sin/cos of psi
     double pf;
@@ -342,7 +343,7 @@ __global__ __launch_bounds__(256) void k_fine_diff2(
         if (inproj[p]) {
             float x = (float)pix[p].x, y = (float)pix[p].y;
             float2 v;
-            if (trilinear_sample(vol, pad, a0 * x + a1 * y, a2 * x + a3 * y, a4 * x + a5 * y, &v)) f = v;
+            if (ref_sample(vol, pad, g.ref2d, a0 * x + a1 * y, a2 * x + a3 * y, a4 * x + a5 * y, &v)) f = v;
         }
         const size_t w = (size_t)b * Pw + win[p];
         float2 X = Fimg[w];
@@ -389,7 +390,8 @@ __global__ __launch_bounds__(256) void k_fine_diff2(
 // contraction so that pixels on the r_max sphere are kept or dropped exactly as there.
 __device__ __noinline__ bool bp_pixel_inside(double AtA_xx, double AtA_xy, double AtA_yy,
                                              double AtA_xy2, int max_r2, double a0, double a1,
-                                             double a2, double a3, double a4, double a5, int x, int y)
+                                             double a2, double a3, double a4, double a5, int x, int y,
+                                             int ref2d)
 {
 #pragma clang fp contract(off)
     double discr = AtA_xy2 * y * y - AtA_xx * (AtA_yy * y * y - max_r2);
@@ -400,6 +402,7 @@ __device__ __noinline__ bool bp_pixel_inside(double AtA_xx, double AtA_xy, doubl
     int first_x = (lo == (int)lo) ? (int)lo : ((lo > 0) ? (int)(lo + 1) : (int)lo);
     int last_x = (hi == (int)hi) ? (int)hi : ((hi > 0) ? (int)hi : (int)(hi - 1));
     if (x < first_x || x > last_x) return false;
+    if (ref2d) return true;   // backrotate2D has no further radius test (backprojector.cpp:568-596)
     double xp = a0 * x + a1 * y, yp = a2 * x + a3 * y, zp = a4 * x + a5 * y;
     return !(xp * xp + yp * yp + zp * zp > max_r2);
 }
@@ -454,6 +457,7 @@ struct StoreParams {
     double* w_pdf_class;       // [K]
     double* w_pdf_dir;         // [K*Nd]
     double* w_sigma2_offset;   // [1]
+    double* w_prior_offset;    // [2K] 2D references: sum w * pixel * (old offset + translation)
     double* w_norm;            // [B] per particle sum of wdiff2
     int Nd;
 };
@@ -515,7 +519,7 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
     inv3(A, Ai);
     // scalar sums (ml_optimiser.cpp:8505-8552), exact weights recomputed in double
     if (threadIdx.x == 0) {
-        double sw = 0., soff = 0.;
+        double sw = 0., soff = 0., pox = 0., poy = 0.;
         const double px2 = P.pixel_size * P.pixel_size;
         const PartScalars s = P.ps[b];
         for (int q = 0; q < nt; q++) {
@@ -533,11 +537,17 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
             double dx = s.prior_x - s.old_x - P.ftx[ft];
             double dy = s.prior_y - s.old_y - P.fty[ft];
             soff += w * px2 * dx * dx + w * px2 * dy * dy;
+            pox += w * P.pixel_size * (s.old_x + P.ftx[ft]);   // ml_optimiser.cpp:8504-8508
+            poy += w * P.pixel_size * (s.old_y + P.fty[ft]);
         }
         atomicAdd(P.w_sumw, sw);
         atomicAdd(&P.w_pdf_class[c], sw);
         atomicAdd(&P.w_pdf_dir[(size_t)c * P.Nd + idir], sw);
         atomicAdd(P.w_sigma2_offset, soff);
+        if (g.ref2d) {
+            atomicAdd(&P.w_prior_offset[2 * c], pox);
+            atomicAdd(&P.w_prior_offset[2 * c + 1], poy);
+        }
     }
     // projection frame: Ainv = A.inv() * pf (float for sampling, double for the scatter geometry)
     const double pf = g.pf;
@@ -550,7 +560,8 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
     float* bpw = P.bp_weight[c];
     const int bpad = P.bp_pads[c];
     const double bpf = pf;
-    const int rr = relion_round(P.bp_rmax[c] * bpf);
+    // backproject2Dto3D rounds r_max * pf, backrotate2D truncates it (`const int r_max_ref`)
+    const int rr = g.ref2d ? (int)(P.bp_rmax[c] * bpf) : relion_round(P.bp_rmax[c] * bpf);
     const int max_r2 = rr * rr;
     const int bxdim = bpad / 2 + 1, bh = bpad / 2;
     // ellipse row bounds of backproject2Dto3D (backprojector.cpp:80-135) in double
@@ -562,7 +573,7 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
         float2 f = make_float2(0.f, 0.f);
         if (P.inproj[p]) {
             float2 v;
-            if (trilinear_sample(vol, pad, fa0 * x + fa1 * y, fa2 * x + fa3 * y, fa4 * x + fa5 * y, &v)) f = v;
+            if (ref_sample(vol, pad, g.ref2d, fa0 * x + fa1 * y, fa2 * x + fa3 * y, fa4 * x + fa5 * y, &v)) f = v;
         }
         const size_t wi = (size_t)b * P.Pw + P.win[p];
         const float ctf = P.Fctf[wi];
@@ -588,7 +599,36 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
             norm_acc += d2acc;
         }
         if (!(wacc > 0.f)) continue;
-        if (!bp_pixel_inside(AtA_xx, AtA_xy, AtA_yy, AtA_xy2, max_r2, da0, da1, da2, da3, da4, da5, x, y)) continue;
+        if (!bp_pixel_inside(AtA_xx, AtA_xy, AtA_yy, AtA_xy2, max_r2, da0, da1, da2, da3, da4, da5, x, y, g.ref2d))
+            continue;
+        if (g.ref2d) {
+            // BackProjector::backrotate2D (backprojector.cpp:568-643): bilinear insertion
+            double xp, yp;
+            {
+#pragma clang fp contract(off)
+                xp = da0 * x + da1 * y;
+                yp = da2 * x + da3 * y;
+            }
+            bool neg = xp < 0;
+            if (neg) { xp = -xp; yp = -yp; }
+            double fx0 = floor(xp), fy0 = floor(yp);
+            int x0 = (int)fx0, y0 = (int)fy0 + bh;
+            if (x0 < 0 || x0 + 1 >= bxdim || y0 < 0 || y0 + 1 >= bpad) continue;
+            double fx = xp - fx0, fy = yp - fy0;
+            double mfx = 1. - fx, mfy = 1. - fy;
+            float vr = accr, vi = neg ? -acci : acci;
+            const size_t sy = bxdim, base = (size_t)y0 * sy + x0;
+            const double wts[4] = {mfy * mfx, mfy * fx, fy * mfx, fy * fx};
+            const size_t offs[4] = {0, 1, sy, sy + 1};
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const float wk = (float)wts[k];
+                atomicAdd(&bpd[base + offs[k]].x, wk * vr);
+                atomicAdd(&bpd[base + offs[k]].y, wk * vi);
+                atomicAdd(&bpw[base + offs[k]], wk * wacc);
+            }
+            continue;
+        }
         double xp, yp, zp;
         {
 #pragma clang fp contract(off)
@@ -802,6 +842,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             set_error("the learned proposal runs with a split-precision cross term (BF16X3 or FP16X3)");
             return T2X_EINVAL;
         }
+        if (cfg->ref_dim != 3) { set_error("the learned proposal needs 3D references"); return T2X_EINVAL; }
         if (!d->net || !d->net->trained) { set_error("proposal network not trained on this handle"); return T2X_ESTATE; }
         if (Nd % 4 || d->net->n_bins != n_bins) {
             set_error("proposal network bins do not match classes x directions / 4");
@@ -834,6 +875,11 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             return T2X_ESTATE;
         }
         ClassVolumes& cv = it->second;
+        if (cv.p_ref_dim != cfg->ref_dim || cv.b_ref_dim != cfg->ref_dim) {
+            set_error("projector/backprojector of class " + std::to_string(c) + " do not have ref_dim " +
+                      std::to_string(cfg->ref_dim));
+            return T2X_EINVAL;
+        }
         h_vols[c] = cv.proj;
         h_pads[c] = cv.p_pad;
         h_bpd[c] = cv.bp_data;
@@ -845,6 +891,19 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         if (cv.p_rmax * 2 < cs) {
             // the CPU projector only fills r <= r_max; smaller r_max than current_size/2 is allowed
         }
+    }
+    const int ref2d = cfg->ref_dim == 2 ? 1 : 0;
+    // 2D references: the offset prior is the class's (ml_optimiser.cpp:7627-7631, 8283-8287);
+    // class-dependent priors would make the translation prior class-dependent, not supported
+    double class_prior_x = 0., class_prior_y = 0.;
+    if (ref2d && model->prior_offset_class) {
+        class_prior_x = model->prior_offset_class[0];
+        class_prior_y = model->prior_offset_class[1];
+        for (int c = 1; c < K; c++)
+            if (model->prior_offset_class[2 * c] != class_prior_x || model->prior_offset_class[2 * c + 1] != class_prior_y) {
+                set_error("2D references: per-class offset priors must be equal");
+                return T2X_EINVAL;
+            }
     }
     // ---- per-configuration tables ----
     const double* sigma2 = model->sigma2_noise;
@@ -928,7 +987,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         k_project_coarse<<<(unsigned)O_all, 256, 0, st>>>(
             d_vols.as<const float2*>(), d_pads.as<int>(), d_Ai.as<float>(), (int)Norient,
             tc.d_pix.as<int2>(), tc.d_inproj.as<unsigned char>(), tc.npix, d->coarse_B.as<float>(), Kc,
-            d->coarse_B2.as<float>(), Kn);
+            d->coarse_B2.as<float>(), Kn, ref2d);
         T2X_HIP(hipGetLastError());
         if (prec != T2X_PREC_FP32) {
             T2X_TRY(d->coarse_Bh.ensure(sizeof(uint16_t) * O_pad * Kc));
@@ -956,7 +1015,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     }
 
     // ---- device accumulators ----
-    const size_t nws = xN + 1 + K + K * Nd + 1;
+    const size_t nws = xN + 1 + K + K * Nd + 1 + 2 * K;
     T2X_TRY(d->wsum_dev.ensure(sizeof(double) * nws));
     T2X_HIP(hipMemsetAsync(d->wsum_dev.ptr, 0, sizeof(double) * nws, st));
     double* w_sigma2 = d->wsum_dev.as<double>();
@@ -964,6 +1023,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     double* w_pdfc = w_sumw + 1;
     double* w_pdfd = w_pdfc + K;
     double* w_off = w_pdfd + K * Nd;
+    double* w_poff = w_off + 1;   // [2K] per-class prior offsets, 2D references
 
     // ---- batch size: coarse diff2 dominates memory (B*T*O_pad floats) ----
     const size_t per_particle = (size_t)T * O_pad * 4 + (size_t)T * Kc * (prec == T2X_PREC_FP32 ? 4 : 8) +
@@ -1024,8 +1084,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             double px_ = p.prior_offset_x, py_ = p.prior_offset_y;
             if (px_ > 998.99 && px_ < 999.01) px_ = 0.;
             if (py_ > 998.99 && py_ < 999.01) py_ = 0.;
-            s.prior_x = px_;
-            s.prior_y = py_;
+            s.prior_x = ref2d ? class_prior_x : px_;
+            s.prior_y = ref2d ? class_prior_y : py_;
             s.norm_scale = cfg->do_norm_correction ? cfg->avg_norm_correction / p.norm_correction : 1.0;
             hctf[i] = ctf_const(p.ctf);
             // pdf_offset / mean (ml_optimiser.cpp:7638-7667, 7701-7725); offsets mix px and A as there
@@ -1371,6 +1431,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         g.dir_sc = d_dirsc.as<double>();
         g.psi_sc = d_psisc.as<double>();
         g.pf = pf;
+        g.ref2d = ref2d;
         (void)hipEventRecord(d->ev[ev++], st);   // 5: fine start
         if (nitems > 0) {
             k_fine_diff2<<<(unsigned)nitems, 256, 0, st>>>(
@@ -1464,6 +1525,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         P.w_pdf_class = w_pdfc;
         P.w_pdf_dir = w_pdfd;
         P.w_sigma2_offset = w_off;
+        P.w_prior_offset = w_poff;
         P.w_norm = d->best.as<double>();
         P.Nd = (int)Nd;
         if (xN > 520) { set_error("box too large for the shell accumulator"); return T2X_EINVAL; }
@@ -1566,6 +1628,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         if (wsum->pdf_direction)
             for (long i = 0; i < K * Nd; i++) wsum->pdf_direction[i] += hw[xN + 1 + K + i];
         if (wsum->sigma2_offset) wsum->sigma2_offset[0] += hw[xN + 1 + K + K * Nd];
+        if (wsum->prior_offset_class && ref2d)
+            for (int c = 0; c < 2 * K; c++) wsum->prior_offset_class[c] += hw[xN + 2 + K + K * Nd + c];
         if (wsum->avg_norm_correction && cfg->do_norm_correction) wsum->avg_norm_correction[0] += avg_norm;
         if (wsum->LL) wsum->LL[0] += LL;
         if (wsum->ave_Pmax) wsum->ave_Pmax[0] += Pmax;

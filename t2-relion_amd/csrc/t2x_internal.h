@@ -243,6 +243,36 @@ __device__ __forceinline__ bool trilinear_sample(const float2* __restrict__ vol,
     return true;
 }
 
+// Bilinear sample of a half-complex 2D reference ([pad][pad/2+1], y centred) at logical
+// (xp, yp) (already multiplied by the padding factor), with the Friedel flip of
+// Projector::rotate2D (projector.cpp:955-995).
+__device__ __forceinline__ bool bilinear_sample(const float2* __restrict__ ref, int pad, float xp, float yp,
+                                                float2* out)
+{
+    const int xdim = pad / 2 + 1, h = pad / 2;
+    bool neg = xp < 0.f;
+    if (neg) { xp = -xp; yp = -yp; }
+    float fx0 = floorf(xp), fy0 = floorf(yp);
+    int x0 = (int)fx0, y0 = (int)fy0 + h;
+    float fx = xp - fx0, fy = yp - fy0;
+    if (x0 < 0 || x0 + 1 >= xdim || y0 < 0 || y0 + 1 >= pad) return false;
+    const float2* p = ref + (size_t)y0 * xdim + x0;
+    float2 d00 = p[0], d01 = p[1], d10 = p[xdim], d11 = p[xdim + 1];
+    float2 dx0 = make_float2(d00.x + (d01.x - d00.x) * fx, d00.y + (d01.y - d00.y) * fx);
+    float2 dx1 = make_float2(d10.x + (d11.x - d10.x) * fx, d10.y + (d11.y - d10.y) * fx);
+    float2 v = make_float2(dx0.x + (dx1.x - dx0.x) * fy, dx0.y + (dx1.y - dx0.y) * fy);
+    if (neg) v.y = -v.y;
+    *out = v;
+    return true;
+}
+
+// Slice value of a 3D reference (trilinear) or of a 2D reference (bilinear, zp unused).
+__device__ __forceinline__ bool ref_sample(const float2* __restrict__ vol, int pad, int ref2d, float xp, float yp,
+                                           float zp, float2* out)
+{
+    return ref2d ? bilinear_sample(vol, pad, xp, yp, out) : trilinear_sample(vol, pad, xp, yp, zp, out);
+}
+
 // CTF::getCTF (ctf.h:181-230) with precomputed constants; double precision.
 struct CtfConst {
     double K1, K2, K3, K4, K5, Axx, Axy, Ayy, scale;

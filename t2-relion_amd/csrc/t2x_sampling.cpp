@@ -97,7 +97,7 @@ int check_config(const t2x_estep_config* c)
         set_error("coarse_size must be even and <= current_size");
         return T2X_EINVAL;
     }
-    if (c->ref_dim != 3) { set_error("only 3D references are supported"); return T2X_EINVAL; }
+    if (c->ref_dim != 3 && c->ref_dim != 2) { set_error("ref_dim must be 3 or 2"); return T2X_EINVAL; }
     if (c->nr_classes < 1) { set_error("nr_classes < 1"); return T2X_EINVAL; }
     if (c->adaptive_oversampling != 1) {
         set_error("adaptive_oversampling must be 1 (two-pass E-step)");
@@ -115,24 +115,34 @@ int check_config(const t2x_estep_config* c)
         set_error("precision mode not supported");
         return T2X_EINVAL;
     }
-    if (c->healpix_order < 0 || c->healpix_order > 8) { set_error("healpix_order out of range"); return T2X_EINVAL; }
+    if (c->ref_dim == 3 && (c->healpix_order < 0 || c->healpix_order > 8)) {
+        set_error("healpix_order out of range");
+        return T2X_EINVAL;
+    }
     return T2X_OK;
 }
 
 int build_sampling(const t2x_estep_config* cfg, HostSampling& s)
 {
     T2X_TRY(check_config(cfg));
+    const bool is3d = cfg->ref_dim == 3;
     const int order = cfg->healpix_order;
-    const long npix = 12L << (2 * order);
+    const long npix = is3d ? (12L << (2 * order)) : 1;
     s = HostSampling();
-    for (long p = 0; p < npix; p++) {
-        double z, phi;
-        pixel_centre(order, p, &z, &phi);
-        double r = phi * 180.0 / kPi, t = std::acos(z) * 180.0 / kPi;
-        normalise_direction(r, t);
-        s.rot.push_back(r);
-        s.tilt.push_back(t);
-        s.ipix.push_back(p);
+    if (is3d) {
+        for (long p = 0; p < npix; p++) {
+            double z, phi;
+            pixel_centre(order, p, &z, &phi);
+            double r = phi * 180.0 / kPi, t = std::acos(z) * 180.0 / kPi;
+            normalise_direction(r, t);
+            s.rot.push_back(r);
+            s.tilt.push_back(t);
+            s.ipix.push_back(p);
+        }
+    } else {   // 2D references: in-plane rotations only (healpix_sampling.cpp:590-595)
+        s.rot.push_back(0.);
+        s.tilt.push_back(0.);
+        s.ipix.push_back(-1);
     }
     s.nr_dir = npix;
     int nr_psi = ceil_relion(360. / cfg->psi_step);
@@ -153,15 +163,17 @@ int build_sampling(const t2x_estep_config* cfg, HostSampling& s)
     }
     s.nr_trans = (long)s.tx.size();
     const int os = cfg->adaptive_oversampling;
-    s.over_rot = 1 << (3 * os);
+    // oversamplingFactorOrientations (healpix_sampling.cpp:1720-1735): 2D references oversample psi only
+    s.over_rot = is3d ? 1 << (3 * os) : 1 << os;
     s.over_trans = 1 << (2 * os);
-    s.n_subdir = 1 << (2 * os);
+    s.n_subdir = is3d ? 1 << (2 * os) : 1;
     s.n_subpsi = 1 << os;
     // sub-directions: children of each coarse pixel at order+os, j (y) outer, i (x) inner
     const int fact = 1 << os;
     s.sub_rot.resize(npix * s.n_subdir);
     s.sub_tilt.resize(npix * s.n_subdir);
-    for (long d = 0; d < npix; d++) {
+    if (!is3d) s.sub_rot[0] = s.sub_tilt[0] = 0.;
+    for (long d = 0; is3d && d < npix; d++) {
         long x, y;
         int face;
         nest_to_face(order, s.ipix[d], &x, &y, &face);
@@ -180,7 +192,12 @@ int build_sampling(const t2x_estep_config* cfg, HostSampling& s)
     s.sub_psi.resize(nr_psi * s.n_subpsi);
     for (int i = 0; i < nr_psi; i++)
         for (int k = 0; k < s.n_subpsi; k++)
-            s.sub_psi[i * s.n_subpsi + k] = s.psi[i] - 0.5 * psi_step + (0.5 + k) * psi_step / s.n_subpsi;
+        {
+            // pushbackOversampledPsiAngles (healpix_sampling.cpp:2023-2050): 2D wraps to (-180, 180]
+            double over = s.psi[i] - 0.5 * psi_step + (0.5 + k) * psi_step / s.n_subpsi;
+            if (!is3d && over > 180.) over -= 360.;
+            s.sub_psi[i * s.n_subpsi + k] = over;
+        }
     // sines and cosines of the fine angles (host libm, as the CPU path's Euler_angles2matrix)
     s.sub_dir_sc.resize(s.sub_rot.size() * 4);
     for (size_t k = 0; k < s.sub_rot.size(); k++) {

@@ -30,6 +30,21 @@ WORKLOADS = {
                      max_shift=6.0, snr=0.03, healpix_order=3, offset_range_px=6.0,
                      offset_step_px=1.5, particle_diameter=240.0, resolution=10.0,
                      seed=20241),
+    # 2D classification (parity size): 4 class averages = z-projections of 64^3 blob volumes
+    # @2 A, psi step 6 deg (60 in-plane angles), offsets 5/1 px, SNR 0.05.
+    "class2d64": dict(ori_size=64, pixel_size=2.0, n_blobs=150, blob_radius=24.0,
+                      blob_sigma=(1.5, 3.0), blob_amp=(0.5, 1.0), K=4, n_particles=2000,
+                      max_shift=4.0, snr=0.05, ref_dim=2, psi_step=6.0, healpix_order=0,
+                      offset_range_px=5.0, offset_step_px=1.0, particle_diameter=110.0,
+                      resolution=20.0, seed=20242),
+    # 2D classification with uneven workloads: 100 class averages 128^2 @2 A (z-projections of
+    # 300-blob volumes), 100,000 particles with Zipf(1.2) class frequencies, psi step 6 deg,
+    # offsets 5/1 px, SNR log-uniform 0.01-0.1.
+    "class2d128": dict(ori_size=128, pixel_size=2.0, n_blobs=300, blob_radius=48.0,
+                       blob_sigma=(1.5, 3.0), blob_amp=(0.5, 1.0), K=100, n_particles=100000,
+                       max_shift=5.0, snr=(0.01, 0.1), zipf=1.2, ref_dim=2, psi_step=6.0,
+                       healpix_order=0, offset_range_px=5.0, offset_step_px=1.0,
+                       particle_diameter=200.0, resolution=12.0, seed=20243),
 }
 
 
@@ -58,8 +73,10 @@ def derived_sizes(N, angpix, healpix_order, particle_diameter, resolution, incr_
 def make_config(w, *, current_size=None, coarse_size=None, precision=T2X_PREC_FP32,
                 maximum_significants=0, K=None):
     N, angpix = w["ori_size"], w["pixel_size"]
+    ref_dim = w.get("ref_dim", 3)
     cur, coarse, r_max, pad = derived_sizes(N, angpix, w["healpix_order"],
-                                            w["particle_diameter"], w["resolution"])
+                                            w["particle_diameter"], w["resolution"],
+                                            ref_dim=ref_dim, psi_step=w.get("psi_step"))
     if current_size:
         cur = current_size
         r_max = cur // 2
@@ -67,14 +84,14 @@ def make_config(w, *, current_size=None, coarse_size=None, precision=T2X_PREC_FP
         coarse = min(coarse, cur)
     if coarse_size:
         coarse = coarse_size
-    ang = 360.0 / (6 * 2 ** w["healpix_order"])
+    ang = 360.0 / (6 * 2 ** w["healpix_order"]) if ref_dim == 3 else w["psi_step"]
     cfg = EstepConfig()
     cfg.ori_size = N
     cfg.pixel_size = angpix
     cfg.current_size = cur
     cfg.coarse_size = coarse
     cfg.nr_classes = K or w["K"]
-    cfg.ref_dim = 3
+    cfg.ref_dim = ref_dim
     cfg.healpix_order = w["healpix_order"]
     cfg.psi_step = ang
     cfg.offset_range = w["offset_range_px"] * angpix
