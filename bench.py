@@ -70,11 +70,10 @@ def build_dataset(lib, dev, w, n, cfg, sz, seed):
         for k in range(nb):
             allb[c * nb + k] = blob_sets[c][k]
     boff = (C.c_int * (K + 1))(*[c * nb for c in range(K + 1)])
-    pin, tr = W.gen_particles(rng, n, K, w["max_shift"])
+    pin, tr, snr = W.workload_particles(rng, w, n, K)
     N = cfg.ori_size
     dimg = C.c_void_p()
     T.check(lib.t2x_device_alloc(dev, n * N * N * 4, C.byref(dimg)), "alloc images")
-    snr = np.full(n, w["snr"])
     noise_var = np.zeros(n)
     T.check(lib.t2x_synth_particles_device(dev, N, cfg.pixel_size, n, allb, boff, pin,
                                            T.dptr(tr["rot"]), T.dptr(tr["tilt"]), T.dptr(tr["psi"]),
@@ -101,17 +100,21 @@ def build_dataset(lib, dev, w, n, cfg, sz, seed):
                 noise_var=noise_var)
 
 
-def set_projectors(lib, dev, sz, ds):
+def set_projectors(lib, dev, sz, ds, ref_dim=3):
     pad, rmax = sz["pad_size"], sz["r_max"]
+    # a 2D class average is the z-projection of its blob volume: the central (kz = 0) plane of
+    # the volume's transform, which starts pad/2 planes into the [z][y][x] array
+    off = (pad // 2) * pad * (pad // 2 + 1) * 4 if ref_dim == 2 else 0
     for c, (dre, dim) in enumerate(ds["refs"]):
         # AccProjector::initMdl of every iteration (device-resident model -> projector volume)
-        T.check(lib.t2x_projector_set(dev, c, 3, pad, rmax, 2.0, C.cast(dre, C.POINTER(C.c_float)),
-                                      C.cast(dim, C.POINTER(C.c_float))), "projector")
+        T.check(lib.t2x_projector_set(dev, c, ref_dim, pad, rmax, 2.0,
+                                      C.cast(C.c_void_p(dre.value + off), C.POINTER(C.c_float)),
+                                      C.cast(C.c_void_p(dim.value + off), C.POINTER(C.c_float))), "projector")
 
 
 def run_step(lib, dev, cfg, sz, ds, n, outs, ws, first=0):
     """One E-step over particles [first, first+n): projectors set, backprojectors cleared."""
-    set_projectors(lib, dev, sz, ds)
+    set_projectors(lib, dev, sz, ds, cfg.ref_dim)
     for c in range(cfg.nr_classes):
         T.check(lib.t2x_backprojector_clear(dev, c), "bp clear")
     model = T.Model(T.dptr(ds["sigma2"]), T.dptr(ds["pdf_class"]), T.dptr(ds["pdf_dir"]), None)
@@ -288,9 +291,10 @@ def main():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device-to-device copy kernel whose FETCH_SIZE/WRITE_SIZE "
                          "counters calibrate the per-dispatch HBM traffic of the GEMM in --pmc runs")
-    ap.add_argument("--precision", default="fp32,fp16x3,learned,learned_gated",
+    ap.add_argument("--precision", default="",
                     help="comma list of paths to time (fp32, bf16x3, fp16x3, learned, learned_gated); the "
-                         "last one is the headline value, the others are reported beside it")
+                         "last one is the headline value, the others are reported beside it (default: "
+                         "fp32,fp16x3,learned,learned_gated for 3D references, fp32,fp16x3 for 2D)")
     ap.add_argument("--train-steps", type=int, default=2000)
     ap.add_argument("--save-proposal", default="", help="write the trained proposal parameters (.npy)")
     ap.add_argument("--load-proposal", default="",
@@ -315,7 +319,7 @@ def main():
     ds = build_dataset(lib, dev, w, n, cfg, sz, w["seed"])
     log(f"dataset synthesised on the GPU in {time.time() - t0:.1f} s")
     for c in range(cfg.nr_classes):
-        T.check(lib.t2x_backprojector_init(dev, c, 3, sz["pad_size"], sz["r_max"], 2.0), "bp init")
+        T.check(lib.t2x_backprojector_init(dev, c, cfg.ref_dim, sz["pad_size"], sz["r_max"], 2.0), "bp init")
     outs = (T.ParticleOut * n)()
     ws, wa = T.make_wsum(cfg.nr_classes, ds["info"].nr_dir, cfg.ori_size)
     if args.copy_calibration:
@@ -327,7 +331,8 @@ def main():
         T.check(lib.t2x_device_free(dev, src))
         T.check(lib.t2x_device_free(dev, dst))
         log("copy calibration: 1 GiB device-to-device copy (t2x::k_copy16) done")
-    paths = args.precision.split(",")
+    paths = (args.precision or ("fp32,fp16x3,learned,learned_gated" if cfg.ref_dim == 3 else
+                                "fp32,fp16x3")).split(",")
     train = None
     use_net = any(p.startswith("learned") for p in paths)
     if use_net and args.load_proposal:
@@ -407,7 +412,7 @@ def main():
                                            "ms_coarse_weights", "ms_fine_diff2", "ms_fine_weights",
                                            "ms_store", "ms_proposal")}
     nsig = st.sig_coarse / max(1, st.n_particles)
-    step_deg = 360.0 / (6 * 2 ** cfg.healpix_order)
+    step_deg = 360.0 / (6 * 2 ** cfg.healpix_order) if cfg.ref_dim == 3 else cfg.psi_step
     ref = "fp16x3" if "fp16x3" in runs else ("fp32" if "fp32" in runs else None)
     variants = {}
     for pname, rr in runs.items():
@@ -434,7 +439,7 @@ def main():
                       "train_top1_last": train.top1_last if train else None})
         variants[pname] = v
     quality = None
-    if args.quality_particles > 0:
+    if args.quality_particles > 0 and cfg.ref_dim == 3:
         nq = min(n, args.quality_particles)
         q = quality_phase(lib, dev, cfg, sz, ds, list(runs.keys()), nq)
         quality = {"particles": nq, "fsc_0143_A_per_class": {k: v["fsc_res"] for k, v in q.items()}}
@@ -461,7 +466,10 @@ def main():
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": DTYPES[head],
         "data": "synthetic (GPU-generated blob volumes and CTF-modulated noisy projections)",
         "config": {"workload": f"{args.workload}: {n} particles {cfg.ori_size}^2 @ {cfg.pixel_size} A/px, "
-                               f"K={cfg.nr_classes}, HEALPix order {cfg.healpix_order}, offsets "
+                               f"K={cfg.nr_classes}, "
+                               + (f"HEALPix order {cfg.healpix_order}" if cfg.ref_dim == 3 else
+                                  f"2D references, psi step {cfg.psi_step} deg")
+                               + f", offsets "
                                f"{w['offset_range_px']}/{w['offset_step_px']} px, current_size "
                                f"{cfg.current_size}, coarse_size {cfg.coarse_size}, path {head}",
                    "global_batch": n, "parallelism": "single GPU",

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define T2X_ABI_VERSION 1
+#define T2X_ABI_VERSION 2
 
 enum {
     T2X_OK = 0,
@@ -94,7 +94,9 @@ typedef struct t2x_estep_config {
     double avg_norm_correction;    /* mymodel.avg_norm_correction                        */
     int    do_ctf_correction;      /* 0/1                                                */
     double sigma2_offset;          /* mymodel.sigma2_offset (A^2)                        */
-    int    orientational_prior_mode; /* T2X_NOPRIOR only in this version                 */
+    int    orientational_prior_mode; /* T2X_NOPRIOR: global search over pdf_direction;   */
+                                   /* T2X_PRIOR_ROTTILT_PSI: local searches around each  */
+                                   /* particle's prior angles (sigma_rot/tilt/psi)       */
     int    precision;              /* T2X_PREC_* for the cross terms                     */
     int    proposal_topk;          /* > 0: coarse pass scores only the top-k bins of the */
                                    /* trained proposal network (t2x_proposal_train)      */
@@ -104,6 +106,10 @@ typedef struct t2x_estep_config {
                                    /* on first use per shape), 128, 256, 257 or 258      */
     int    gemm_group;             /* its tile-walk group in tile rows (0: default)      */
     int    reserved[2];
+    /* local searches: sqrt(mymodel.sigma2_rot / sigma2_tilt / sigma2_psi) in degrees;
+     * sigma_rot and sigma_tilt both > 0 (a Gaussian on the angle between the directions) or
+     * both 0 (every direction); sigma_psi 0: every psi angle */
+    double sigma_rot, sigma_tilt, sigma_psi;
 } t2x_estep_config;
 
 /* Model arrays of the current iteration (host, caller-owned, read-only). */
@@ -121,6 +127,8 @@ typedef struct t2x_particle_in {
     double prior_offset_x, prior_offset_y;   /* px, 999 = unset (METADATA_XOFF_PRIOR)     */
     double norm_correction;                  /* METADATA_NORM                              */
     double rot, tilt, psi;                   /* current angles (METADATA_ROT/TILT/PSI)     */
+    double prior_rot, prior_tilt, prior_psi; /* METADATA_ROT/TILT/PSI_PRIOR, 999 = unset:  */
+                                             /* the local search centres on rot/tilt/psi  */
 } t2x_particle_in;
 
 /* Per-particle results (the exp_metadata row written by storeWeightedSums). */

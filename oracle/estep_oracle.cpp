@@ -240,8 +240,100 @@ static int check_config(const t2x_estep_config* c)
     if (c->nr_classes < 1) return T2X_EINVAL;
     if (c->adaptive_oversampling < 0 || c->adaptive_oversampling > 1) return T2X_EINVAL;
     if (c->offset_step <= 0 || c->offset_range < 0 || c->psi_step <= 0) return T2X_EINVAL;
-    if (c->orientational_prior_mode != T2X_NOPRIOR) return T2X_EINVAL;
+    if (c->orientational_prior_mode == T2X_PRIOR_ROTTILT_PSI) {
+        /* local searches with one Gaussian on the angle between directions (sigma_rot and
+         * sigma_tilt both set) or none (both 0); the rot-only / tilt-only branches are not restated */
+        if (c->sigma_rot < 0 || c->sigma_tilt < 0 || c->sigma_psi < 0) return T2X_EINVAL;
+        if ((c->sigma_rot > 0.) != (c->sigma_tilt > 0.)) return T2X_EINVAL;
+    } else if (c->orientational_prior_mode != T2X_NOPRIOR) {
+        return T2X_EINVAL;
+    }
     return T2X_OK;
+}
+
+/* gaussian1D (funcs.cpp:356-360), evaluated left to right */
+static double gaussian1D(double x, double sigma, double mu)
+{
+    x -= mu;
+    return 1 / std::sqrt(2 * PI * sigma * sigma) * std::exp(-0.5 * ((x / sigma) * (x / sigma)));
+}
+
+/* Euler_angles2direction (euler.cpp:94-114) */
+static void angles2direction(double rot, double tilt, double v[3])
+{
+    double a = rot * PI / 180, b = tilt * PI / 180;
+    double ca = std::cos(a), cb = std::cos(b), sa = std::sin(a), sb = std::sin(b);
+    v[0] = sb * ca;
+    v[1] = sb * sa;
+    v[2] = cb;
+}
+
+/* HealpixSampling::selectOrientationsWithNonZeroPriorProbability (healpix_sampling.cpp:769-1145)
+ * for C1 (no symmetry mates), no bimodal psi, no multi-body terms, sigma_cutoff 3: the normalised
+ * prior of every direction and psi angle, 0 outside the search range; the nearest one alone when
+ * none is within range. Prior angles of 999 fall back to the current angles
+ * (ml_optimiser.cpp:5851-5857). Returns the numbers of selected directions and psi angles. */
+static void local_priors(const Sampling& S, const t2x_estep_config* cfg, const t2x_particle_in& pin,
+                         std::vector<double>& dpr, std::vector<double>& ppr, long* nd, long* np)
+{
+    auto unset = [](double v) { return v > 998.99 && v < 999.01; };
+    const double prot = unset(pin.prior_rot) ? pin.rot : pin.prior_rot;
+    const double ptilt = unset(pin.prior_tilt) ? pin.tilt : pin.prior_tilt;
+    const double ppsi = unset(pin.prior_psi) ? pin.psi : pin.prior_psi;
+    const double cutoff = 3.;
+    const long Nd = S.nr_dir(), Npsi = S.nr_psi();
+    dpr.assign(Nd, 0.);
+    ppr.assign(Npsi, 0.);
+    *nd = *np = 0;
+    if (S.is_3D) {
+        double sum = 0., best_ang = 9999.;
+        long best = -999;
+        if (cfg->sigma_rot > 0. && cfg->sigma_tilt > 0.) {
+            double pv[3], v[3];
+            angles2direction(prot, ptilt, pv);
+            const double sig = std::max(cfg->sigma_rot, cfg->sigma_tilt);
+            for (long d = 0; d < Nd; d++) {
+                angles2direction(S.rot[d], S.tilt[d], v);
+                double dot = 0.;
+                for (int i = 0; i < 3; i++) dot += v[i] * pv[i];   /* dotProduct (matrix1d.h) */
+                double diffang = std::acos(dot) * 180. / PI;        /* ACOSD (macros.h:301) */
+                if (diffang > 180.) diffang = std::fabs(diffang - 360.);
+                if (diffang < cutoff * sig) {
+                    dpr[d] = gaussian1D(diffang, sig, 0.);
+                    sum += dpr[d];
+                    (*nd)++;
+                }
+                if (diffang < best_ang) { best = d; best_ang = diffang; }
+            }
+        } else {
+            for (long d = 0; d < Nd; d++) { dpr[d] = 1.; sum += 1.; (*nd)++; }
+        }
+        for (long d = 0; d < Nd; d++) if (dpr[d] > 0.) dpr[d] /= sum;
+        if (*nd == 0) { dpr[best] = 1.; *nd = 1; }
+    } else {
+        dpr[0] = 1.;
+        *nd = 1;
+    }
+    double sum = 0., best_diff = 9999.;
+    long best = -999;
+    for (long p = 0; p < Npsi; p++) {
+        if (cfg->sigma_psi > 0.) {
+            double diffpsi = std::fabs(S.psi[p] - ppsi);
+            if (diffpsi > 180.) diffpsi = std::fabs(diffpsi - 360.);
+            if (diffpsi < cutoff * cfg->sigma_psi) {
+                ppr[p] = gaussian1D(diffpsi, cfg->sigma_psi, 0.);
+                sum += ppr[p];
+                (*np)++;
+            }
+            if (diffpsi < best_diff) { best = p; best_diff = diffpsi; }
+        } else {
+            ppr[p] = 1.;
+            sum += 1.;
+            (*np)++;
+        }
+    }
+    for (long p = 0; p < Npsi; p++) if (ppr[p] > 0.) ppr[p] /= sum;
+    if (*np == 0) { ppr[best] = 1.; *np = 1; }
 }
 
 extern "C" int orc_sampling_info(const t2x_estep_config* cfg, t2x_sampling_info* info)
@@ -294,6 +386,20 @@ extern "C" int orc_sampling_oversampled_translations(const t2x_estep_config* cfg
     s.translations_px(itrans, os, cfg->pixel_size, a, b);
     for (size_t i = 0; i < a.size(); i++) { x[i] = a[i]; y[i] = b[i]; }
     return (int)a.size();
+}
+
+/* Local-search priors of one particle: dpr[nr_dir], ppr[nr_psi] (0 outside the range). */
+extern "C" int orc_local_priors(const t2x_estep_config* cfg, const t2x_particle_in* pin, double* dpr,
+                                double* ppr, long* nd, long* np)
+{
+    if (check_config(cfg) != T2X_OK || !pin) return T2X_EINVAL;
+    Sampling s;
+    s.setup(cfg);
+    std::vector<double> d, p;
+    local_priors(s, cfg, *pin, d, p, nd, np);
+    std::copy(d.begin(), d.end(), dpr);
+    std::copy(p.begin(), p.end(), ppr);
+    return T2X_OK;
 }
 
 /* =====================================================================================
@@ -1123,8 +1229,15 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
         P.prior_y = sh.model->prior_offset_class ? sh.model->prior_offset_class[1] : 0.;
     }
 
-    /* orientation prior (NOPRIOR): pdf_direction[class][idir]; offset prior per translation */
-    auto pdf_orient = [&](int c, long idir) { return sh.model->pdf_direction[(size_t)c * Nd + idir]; };
+    /* orientation prior: NOPRIOR pdf_direction[class][idir]; local searches the product of the
+     * particle's direction and psi priors (ml_optimiser.cpp:7031-7045, 7700-7713) */
+    const bool local = cfg->orientational_prior_mode == T2X_PRIOR_ROTTILT_PSI;
+    std::vector<double> dpr, ppr;
+    long nd_loc = 0, np_loc = 0;
+    if (local) local_priors(S, cfg, pin, dpr, ppr, &nd_loc, &np_loc);
+    auto pdf_orient = [&](int c, long idir, long ipsi) {
+        return local ? dpr[idir] * ppr[ipsi] : sh.model->pdf_direction[(size_t)c * Nd + idir];
+    };
     const double sig2off = cfg->sigma2_offset;
     auto pdf_offset = [&](long itrans) {
         double ox = P.old_x + S.tx[itrans], oy = P.old_y + S.ty[itrans];
@@ -1183,7 +1296,7 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
             for (long idir = 0; idir < Nd; idir++)
                 for (long ipsi = 0; ipsi < Npsi; ipsi++) {
                     long iorientclass = (long)c * Norient + idir * Npsi + ipsi;
-                    double pdf_o = pdf_orient(c, idir);
+                    double pdf_o = pdf_orient(c, idir, ipsi);
                     bool proceed = true;
                     if (ipass > 0) {
                         proceed = false;
@@ -1234,12 +1347,19 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
         if (ipass == 0 && dbg && ipart < dbg->n_debug && dbg->coarse_diff2)
             for (long h = 0; h < Nhid; h++) dbg->coarse_diff2[(size_t)ipart * Nhid + h] = (float)Mw_coarse[h];
 
-        /* convertAllSquaredDifferencesToWeights (ml_optimiser.cpp:7510-8041), NOPRIOR */
+        /* convertAllSquaredDifferencesToWeights (ml_optimiser.cpp:7510-8041); the means run
+         * over the particle's orientations (all of them, or the local ones) and translations */
         double pdf_orientation_mean = 0, pdf_offset_mean = 0;
         long cnt_o = 0, cnt_t = 0;
         for (int c = 0; c < K; c++) {
-            for (long idir = 0; idir < Nd; idir++)
-                for (long ipsi = 0; ipsi < Npsi; ipsi++) { pdf_orientation_mean += pdf_orient(c, idir); cnt_o++; }
+            for (long idir = 0; idir < Nd; idir++) {
+                if (local && !(dpr[idir] > 0.)) continue;
+                for (long ipsi = 0; ipsi < Npsi; ipsi++) {
+                    if (local && !(ppr[ipsi] > 0.)) continue;
+                    pdf_orientation_mean += pdf_orient(c, idir, ipsi);
+                    cnt_o++;
+                }
+            }
             for (long it = 0; it < Nt; it++) { pdf_offset_mean += pdf_offset(it); cnt_t++; }
         }
         pdf_orientation_mean /= (double)cnt_o;
@@ -1264,7 +1384,7 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
             for (long idir = 0; idir < Nd; idir++)
                 for (long ipsi = 0; ipsi < Npsi; ipsi++) {
                     long iorientclass = (long)c * Norient + idir * Npsi + ipsi;
-                    double pdf_o = pdf_orient(c, idir);
+                    double pdf_o = pdf_orient(c, idir, ipsi);
                     if (pdf_orientation_mean != 0.) pdf_o /= pdf_orientation_mean;
                     for (long it = 0; it < Nt; it++) {
                         long ihidden = iorientclass * Nt + it;

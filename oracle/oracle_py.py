@@ -47,6 +47,8 @@ def load():
                                                             C.c_long, C.c_int, PD, PD, PD]),
         "orc_sampling_oversampled_translations": (C.c_int, [C.POINTER(T.EstepConfig), C.c_long,
                                                             C.c_int, PD, PD]),
+        "orc_local_priors": (C.c_int, [C.POINTER(T.EstepConfig), C.POINTER(T.ParticleIn), PD, PD,
+                                       C.POINTER(C.c_long), C.POINTER(C.c_long)]),
         "orc_euler_angles2matrix": (None, [C.c_double, C.c_double, C.c_double, PD]),
         "orc_ctf_value": (C.c_double, [C.POINTER(T.CtfParams), C.c_double, C.c_double]),
         "orc_ctf_image": (None, [C.POINTER(T.CtfParams), C.c_int, C.c_double, C.c_int, PD]),

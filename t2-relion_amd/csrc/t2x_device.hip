@@ -67,7 +67,8 @@ extern "C" int t2x_device_close(t2x_device* h)
                       &d->fine_diff2, &d->fine_min, &d->fine_sel, &d->ab_fine, &d->ab_coarse,
                       &d->wsum_dev, &d->store_items, &d->best, &d->tables, &d->Y_fine,
                       &d->dbg_tmp, &d->coarse_B, &d->coarse_B2, &d->cand_w, &d->cand_e,
-                      &d->cand_n, &d->pairs_strided};
+                      &d->cand_n, &d->pairs_strided, &d->loc_dir, &d->loc_psi, &d->loc_mean,
+                      &d->loc_ang, &d->loc_tab, &d->loc_list};
     for (DevBuf* b : bufs) b->release();
     PassTables* tabs[] = {&d->coarse_tab, &d->fine_tab};
     for (PassTables* t : tabs) {

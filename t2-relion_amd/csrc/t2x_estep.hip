@@ -460,6 +460,7 @@ struct StoreParams {
     double* w_prior_offset;    // [2K] 2D references: sum w * pixel * (old offset + translation)
     double* w_norm;            // [B] per particle sum of wdiff2
     int Nd;
+    LocalPrior lp;             // local searches (lp.dir null: global)
 };
 
 #define STORE_MAX_T 1024
@@ -496,7 +497,7 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
         const long long k = p0 + q / g.over_trans;
         const int iot = q % g.over_trans;
         const int2 nt2 = pairs[k];
-        double prior = P.prior_o[nt2.x] * P.prior_t[(size_t)b * g.T + nt2.y];
+        double prior = orient_prior(P.prior_o, P.lp, b, nt2.x) * P.prior_t[(size_t)b * g.T + nt2.y];
         double w = 0.;
         if (prior > 0.) {
             double dd = (double)P.fine_d2[k * nover + ir * g.over_trans + iot] - mind2;
@@ -526,7 +527,7 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
             const long long k = p0 + q / g.over_trans;
             const int iot = q % g.over_trans;
             const int2 nt2 = pairs[k];
-            double prior = P.prior_o[nt2.x] * P.prior_t[(size_t)b * g.T + nt2.y];
+            double prior = orient_prior(P.prior_o, P.lp, b, nt2.x) * P.prior_t[(size_t)b * g.T + nt2.y];
             if (!(prior > 0.)) continue;
             double dd = (double)P.fine_d2[k * nover + ir * g.over_trans + iot] - mind2;
             double w = (dd > 700.) ? 0. : prior * exp(-dd);
@@ -675,6 +676,172 @@ __global__ void k_fill_f32(float* p, float v, long long n)
 {
     long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = v;
+}
+
+// ------------------------------------------------------------------------------------------
+// Local angular searches: HealpixSampling::selectOrientationsWithNonZeroPriorProbability
+// (healpix_sampling.cpp:769-1145) for C1, without bimodal psi or multi-body terms, sigma_cutoff 3,
+// and the mean prior of convertAllSquaredDifferencesToWeights (ml_optimiser.cpp:7600-7622).
+// One workgroup per particle, FP64 with the reference's operation order and no contraction.
+// ------------------------------------------------------------------------------------------
+struct LocalSelParams {
+    const double* dir_rot;   // [Nd] degrees
+    const double* dir_tilt;  // [Nd]
+    const double* psi;       // [Npsi]
+    int Nd, Npsi, K;
+    double sig_dir;          // max(sigma_rot, sigma_tilt); 0: every direction
+    double sig_psi;          // 0: every psi angle
+};
+
+__device__ __noinline__ double gaussian1d(double x, double sigma)
+{
+#pragma clang fp contract(off)
+    return 1 / sqrt(2 * kPi * sigma * sigma) * exp(-0.5 * ((x / sigma) * (x / sigma)));   // funcs.cpp:356-360
+}
+
+__device__ __noinline__ void angles_to_direction(double rot, double tilt, double* v)
+{
+#pragma clang fp contract(off)
+    const double a = rot * kPi / 180, b = tilt * kPi / 180;   // euler.cpp:94-114
+    const double ca = cos(a), cb = cos(b), sa = sin(a), sb = sin(b);
+    v[0] = sb * ca;
+    v[1] = sb * sa;
+    v[2] = cb;
+}
+
+// angle (degrees) between direction d and the prior direction pv (ACOSD of the dot product)
+__device__ __noinline__ double direction_angle(const LocalSelParams& q, int d, const double* pv)
+{
+#pragma clang fp contract(off)
+    double v[3];
+    angles_to_direction(q.dir_rot[d], q.dir_tilt[d], v);
+    double dot = 0.;
+    dot += v[0] * pv[0];
+    dot += v[1] * pv[1];
+    dot += v[2] * pv[2];
+    double a = acos(dot) * 180. / kPi;
+    if (a > 180.) a = fabs(a - 360.);
+    return a;
+}
+
+__device__ __noinline__ double psi_distance(double psi, double prior)
+{
+#pragma clang fp contract(off)
+    double a = fabs(psi - prior);
+    if (a > 180.) a = fabs(a - 360.);
+    return a;
+}
+
+// Lowest-index minimum over the workgroup (the reference keeps the first strict minimum).
+__device__ void block_argmin(double& v, int& i, double* sv, int* si)
+{
+    sv[threadIdx.x] = v;
+    si[threadIdx.x] = i;
+    __syncthreads();
+    for (int s = blockDim.x / 2; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) {
+            const double ov = sv[threadIdx.x + s];
+            const int oi = si[threadIdx.x + s];
+            if (oi >= 0 && (si[threadIdx.x] < 0 || ov < sv[threadIdx.x] ||
+                            (ov == sv[threadIdx.x] && oi < si[threadIdx.x]))) {
+                sv[threadIdx.x] = ov;
+                si[threadIdx.x] = oi;
+            }
+        }
+        __syncthreads();
+    }
+    v = sv[0];
+    i = si[0];
+    __syncthreads();
+}
+
+// pang: [B][3] prior rot, tilt, psi (already resolved from the current angles where unset).
+// Outputs: dir [B][Nd], psi [B][Npsi] normalised priors (0 outside the range), mean [B];
+// list [B][Nd + Npsi] scratch (compacted selected indices).
+__global__ __launch_bounds__(256) void k_local_prior(const double* __restrict__ pang, int B, LocalSelParams q,
+                                                     double* __restrict__ dir, double* __restrict__ psi,
+                                                     double* __restrict__ mean, int* __restrict__ list)
+{
+#pragma clang fp contract(off)
+    __shared__ double sv[256];
+    __shared__ int si[256];
+    __shared__ int s_cnt[2];
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    double* db = dir + (size_t)b * q.Nd;
+    double* pb = psi + (size_t)b * q.Npsi;
+    int* dl = list + (size_t)b * (q.Nd + q.Npsi);
+    int* pl = dl + q.Nd;
+    const double prot = pang[3 * b], ptilt = pang[3 * b + 1], ppsi = pang[3 * b + 2];
+    // directions: Gaussian on the angle to the prior direction within 3 sigma
+    double best = 9999.;
+    int bi = -1;
+    if (q.sig_dir > 0.) {
+        double pv[3];
+        angles_to_direction(prot, ptilt, pv);
+        for (int d = threadIdx.x; d < q.Nd; d += blockDim.x) {
+            const double a = direction_angle(q, d, pv);
+            db[d] = (a < 3. * q.sig_dir) ? gaussian1d(a, q.sig_dir) : 0.;
+            if (a < best) { best = a; bi = d; }
+        }
+    } else {
+        for (int d = threadIdx.x; d < q.Nd; d += blockDim.x) db[d] = 1.;
+    }
+    block_argmin(best, bi, sv, si);
+    // psi angles
+    double bestp = 9999.;
+    int bp = -1;
+    for (int p = threadIdx.x; p < q.Npsi; p += blockDim.x) {
+        if (q.sig_psi > 0.) {
+            const double a = psi_distance(q.psi[p], ppsi);
+            pb[p] = (a < 3. * q.sig_psi) ? gaussian1d(a, q.sig_psi) : 0.;
+            if (a < bestp) { bestp = a; bp = p; }
+        } else {
+            pb[p] = 1.;
+        }
+    }
+    block_argmin(bestp, bp, sv, si);
+    __threadfence_block();
+    __syncthreads();
+    // sums in index order (sumprior of the reference), compaction, normalisation
+    if (threadIdx.x == 0) {
+        double sd = 0.;
+        int nd = 0;
+        for (int d = 0; d < q.Nd; d++) {
+            const double x = db[d];
+            if (x > 0.) { sd += x; dl[nd++] = d; }
+        }
+        for (int k = 0; k < nd; k++) db[dl[k]] /= sd;
+        if (nd == 0) { const int d0 = bi >= 0 ? bi : 0; db[d0] = 1.; dl[nd++] = d0; }
+        double sp = 0.;
+        int np = 0;
+        for (int p = 0; p < q.Npsi; p++) {
+            const double x = pb[p];
+            if (x > 0.) { sp += x; pl[np++] = p; }
+        }
+        for (int k = 0; k < np; k++) pb[pl[k]] /= sp;
+        if (np == 0) { const int p0 = bp >= 0 ? bp : 0; pb[p0] = 1.; pl[np++] = p0; }
+        // pdf_orientation_mean over classes x local directions x local psi angles
+        double m = 0.;
+        for (int c = 0; c < q.K; c++)
+            for (int k = 0; k < nd; k++) {
+                const double x = db[dl[k]];
+                for (int j = 0; j < np; j++) m += x * pb[pl[j]];
+            }
+        mean[b] = m / (double)((long long)q.K * nd * np);
+    }
+}
+
+// Local searches: N1[b][n] = +inf for orientations outside particle b's range, so that the
+// coarse GEMM epilogue (diff2 = C_b + N1 - cross) gives them +inf: never a minimum, never a
+// candidate, and their prior is 0.
+__global__ void k_local_mask_n1(float* __restrict__ N1, long long ldn, int B, long long O_all, LocalPrior lp)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (long long)B * O_all) return;
+    const int b = (int)(i / O_all);
+    const long long n = i - (long long)b * O_all;
+    if (!local_in_range(lp, b, n)) N1[(long long)b * ldn + n] = __builtin_inff();
 }
 
 __global__ void k_items_from_orients(const long long* orient_off, int B, int over_rot, long long* item_off)
@@ -835,9 +1002,14 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     // learned proposal: the coarse pass scores only the top-k bins (classes x HEALPix directions
     // one order down = blocks of 4 directions x Npsi orientations) of each particle
     const bool use_prop = cfg->proposal_topk > 0;
+    const bool local = cfg->orientational_prior_mode == T2X_PRIOR_ROTTILT_PSI;
     const int n_bins = (int)(K * (Nd / 4)), bin_cols = (int)(4 * Npsi);
     int topk = 0;
     if (use_prop) {
+        if (local) {
+            set_error("local searches and the learned proposal are exclusive");
+            return T2X_EINVAL;
+        }
         if (cfg->precision == T2X_PREC_FP32) {
             set_error("the learned proposal runs with a split-precision cross term (BF16X3 or FP16X3)");
             return T2X_EINVAL;
@@ -926,7 +1098,9 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     T2X_TRY(upload(d->ab_coarse, abc, st));
     T2X_TRY(upload(d->ab_fine, abf, st));
     // orientation prior (NOPRIOR): pdf_direction / mean over all classes and orientations;
-    // 0 where the class or the direction has zero probability (never scored)
+    // 0 where the class or the direction has zero probability (never scored). Local searches:
+    // the class gate (1, or 0 for a class of zero probability); the per-particle direction and
+    // psi priors come from k_local_prior (ml_optimiser.cpp:7031-7045, 7700-7716)
     double mean_o = 0.;
     for (int c = 0; c < K; c++)
         for (long dd = 0; dd < Nd; dd++) mean_o += model->pdf_direction[(size_t)c * Nd + dd] * Npsi;
@@ -934,9 +1108,9 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     std::vector<double> prior_o(O_all);
     for (int c = 0; c < K; c++)
         for (long o = 0; o < Norient; o++) {
-            double p = model->pdf_direction[(size_t)c * Nd + o / Npsi];
+            double p = local ? 1. : model->pdf_direction[(size_t)c * Nd + o / Npsi];
             bool scored = model->pdf_class[c] > 0. && p > 0.;
-            if (mean_o != 0.) p /= mean_o;
+            if (mean_o != 0. && !local) p /= mean_o;
             prior_o[(size_t)c * Norient + o] = scored ? p : 0.;
         }
     T2X_TRY(upload(d->prior_o, prior_o, st));
@@ -956,6 +1130,21 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     T2X_TRY(upload(d_bpw, h_bpw, st));
     T2X_TRY(upload(d_bpads, h_bpads, st));
     T2X_TRY(upload(d_bprmax, h_bprmax, st));
+    LocalSelParams lsp{};
+    if (local) {
+        std::vector<double> tab(2 * Nd + Npsi);
+        for (long dd = 0; dd < Nd; dd++) { tab[dd] = S.rot[dd]; tab[Nd + dd] = S.tilt[dd]; }
+        for (long p = 0; p < Npsi; p++) tab[2 * Nd + p] = S.psi[p];
+        T2X_TRY(upload(d->loc_tab, tab, st));
+        lsp.dir_rot = d->loc_tab.as<double>();
+        lsp.dir_tilt = lsp.dir_rot + Nd;
+        lsp.psi = lsp.dir_rot + 2 * Nd;
+        lsp.Nd = (int)Nd;
+        lsp.Npsi = (int)Npsi;
+        lsp.K = K;
+        lsp.sig_dir = ref2d ? 0. : std::max(cfg->sigma_rot, cfg->sigma_tilt);
+        lsp.sig_psi = cfg->sigma_psi;
+    }
 
     // ---- coarse projections, cached across calls ----
     const int Kc = tc.npix_pad * 2;            // GEMM K (re/im)
@@ -1107,6 +1296,37 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         T2X_TRY(upload(d_ctf, hctf, st));
         T2X_TRY(upload(d->prior_t, prior_t, st));
         PartScalars* d_ps = d->pscal.as<PartScalars>();
+        // local searches: each particle's direction / psi priors around its prior angles (the
+        // current angles where the priors are unset, ml_optimiser.cpp:5851-5857)
+        LocalPrior lp{};
+        if (local) {
+            std::vector<double> pang((size_t)3 * B);
+            auto unset = [](double v) { return v > 998.99 && v < 999.01; };
+            for (int i = 0; i < B; i++) {
+                const t2x_particle_in& p = pin[b0 + i];
+                pang[3 * i] = unset(p.prior_rot) ? p.rot : p.prior_rot;
+                pang[3 * i + 1] = unset(p.prior_tilt) ? p.tilt : p.prior_tilt;
+                pang[3 * i + 2] = unset(p.prior_psi) ? p.psi : p.prior_psi;
+                if (!std::isfinite(pang[3 * i]) || !std::isfinite(pang[3 * i + 1]) || !std::isfinite(pang[3 * i + 2])) {
+                    set_error("local searches: non-finite prior angles for particle " + std::to_string(b0 + i));
+                    return T2X_EINVAL;
+                }
+            }
+            T2X_TRY(upload(d->loc_ang, pang, st));
+            T2X_TRY(d->loc_dir.ensure(sizeof(double) * B * Nd));
+            T2X_TRY(d->loc_psi.ensure(sizeof(double) * B * Npsi));
+            T2X_TRY(d->loc_mean.ensure(sizeof(double) * B));
+            T2X_TRY(d->loc_list.ensure(sizeof(int) * B * (Nd + Npsi)));
+            k_local_prior<<<B, 256, 0, st>>>(d->loc_ang.as<double>(), B, lsp, d->loc_dir.as<double>(),
+                                            d->loc_psi.as<double>(), d->loc_mean.as<double>(),
+                                            d->loc_list.as<int>());
+            T2X_HIP(hipGetLastError());
+            lp.dir = d->loc_dir.as<double>();
+            lp.psi = d->loc_psi.as<double>();
+            lp.mean = d->loc_mean.as<double>();
+            lp.Nd = (int)Nd;
+            lp.Npsi = (int)Npsi;
+        }
         // ---- preprocessing ----
         T2X_TRY(d->img2.ensure(sizeof(float) * 2 * B * NN));
         T2X_TRY(d->fft_out.ensure(sizeof(float2) * 2 * B * N * xN));
@@ -1174,6 +1394,11 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                                        d->coarse_B2h.ptr, d->coarse_B2l.ptr, d->coarse_B2s.as<float>(), Kn, Kn,
                                        (int)B_pad, (int)O_pad, ep, st,
                                        GemmChoice{cfg->gemm_tile, cfg->gemm_group}, nullptr, nullptr));
+            }
+            if (local) {
+                const long long ne = (long long)B * O_all;
+                k_local_mask_n1<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(d->N1.as<float>(), O_pad, B, O_all, lp);
+                T2X_HIP(hipGetLastError());
             }
         }
         // ---- learned proposal: bins per particle, work lists of the restricted coarse GEMM ----
@@ -1315,6 +1540,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         wc.rowmin_ld = rowmin_ld;
         wc.nover = nover;
         wc.over_trans = over_trans;
+        wc.lp = lp;
         k_pass_min<<<B, 1024, 0, st>>>(wc, B, d_selc);
         SelParams sp{cfg->adaptive_fraction, (long long)cfg->maximum_significants, 1};
         // candidate path: sum pass, compaction of the weights that can be significant, exact
@@ -1328,25 +1554,30 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         int* d_overflow = d_cand_count + B;
         T2X_HIP(hipMemsetAsync(d_cand_count, 0, sizeof(int) * 2 * B, st));   // counts and overflow flags
         T2X_TRY(d->sum_part.ensure(sum_part_bytes() * B * T));
-        {
-            const dim3 grid((unsigned)T, (unsigned)B);
-            k_coarse_sum_rows<<<grid, 256, 0, st>>>(wc, B, d_selc, d->sum_part.as<SumPart>());
-            k_coarse_sum_final<<<B, 64, 0, st>>>(wc, B, d->sum_part.as<SumPart>(), d_selc);
-            k_coarse_candidates<<<grid, 256, 0, st>>>(wc, B, sp, d_selc, d->cand_w.as<double>(),
-                                                      d->cand_e.as<int>(), d_cand_count);
-        }
-        k_cand_select<<<B, 1024, 0, st>>>(wc, B, sp, d_selc, d->cand_w.as<double>(), d->cand_e.as<int>(),
-                                          d_cand_count, d->pairs_strided.as<int2>(),
-                                          d->sig_count.as<long long>(), d_overflow);
-        T2X_HIP(hipGetLastError());
         std::vector<long long> counts(2 * B);
         std::vector<SelResult> hselc(B);
-        std::vector<int> hover(B);
-        T2X_HIP(hipMemcpyAsync(hover.data(), d_overflow, sizeof(int) * B, hipMemcpyDeviceToHost, st));
-        T2X_HIP(hipStreamSynchronize(st));
-        bool fallback = false;
-        for (int i = 0; i < B; i++) fallback |= hover[i] != 0;
-        if (getenv("T2X_FORCE_RADIX_SELECT")) fallback = true;   // exercises the fallback in tests
+        bool fallback = true;
+        // local searches take the radix-select path directly (the candidate filter works from the
+        // global log prior)
+        if (!local) {
+            {
+                const dim3 grid((unsigned)T, (unsigned)B);
+                k_coarse_sum_rows<<<grid, 256, 0, st>>>(wc, B, d_selc, d->sum_part.as<SumPart>());
+                k_coarse_sum_final<<<B, 64, 0, st>>>(wc, B, d->sum_part.as<SumPart>(), d_selc);
+                k_coarse_candidates<<<grid, 256, 0, st>>>(wc, B, sp, d_selc, d->cand_w.as<double>(),
+                                                          d->cand_e.as<int>(), d_cand_count);
+            }
+            k_cand_select<<<B, 1024, 0, st>>>(wc, B, sp, d_selc, d->cand_w.as<double>(), d->cand_e.as<int>(),
+                                              d_cand_count, d->pairs_strided.as<int2>(),
+                                              d->sig_count.as<long long>(), d_overflow);
+            T2X_HIP(hipGetLastError());
+            std::vector<int> hover(B);
+            T2X_HIP(hipMemcpyAsync(hover.data(), d_overflow, sizeof(int) * B, hipMemcpyDeviceToHost, st));
+            T2X_HIP(hipStreamSynchronize(st));
+            fallback = false;
+            for (int i = 0; i < B; i++) fallback |= hover[i] != 0;
+            if (getenv("T2X_FORCE_RADIX_SELECT")) fallback = true;   // exercises the fallback in tests
+        }
         if (fallback) {
             k_select<<<B, 1024, 0, st>>>(wc, sp, B, d_selc);
             k_sig_count<<<B, 1024, 0, st>>>(wc, B, d_selc, d->sig_count.as<long long>());
@@ -1395,7 +1626,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                 for (long o = 0; o < O_all; o++)
                     for (long t = 0; t < T; t++) {
                         float v = row[(size_t)t * O_pad + o];
-                        if (!(prior_o[o] > 0.)) v = -999.f;
+                        if (!(prior_o[o] > 0.) || std::isinf(v)) v = -999.f;   // not scored / out of range
                         if (dbg->coarse_diff2) dbg->coarse_diff2[(size_t)(b0 + i) * Nh + o * T + t] = v;
                     }
                 if (dbg->coarse_min_diff2) dbg->coarse_min_diff2[b0 + i] = hselc[i].min_d2;
@@ -1528,6 +1759,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         P.w_prior_offset = w_poff;
         P.w_norm = d->best.as<double>();
         P.Nd = (int)Nd;
+        P.lp = lp;
         if (xN > 520) { set_error("box too large for the shell accumulator"); return T2X_EINVAL; }
         if (nitems > 0) {
             k_store<<<(unsigned)nitems, 256, 0, st>>>(d_item_off.as<long long>(), B, d_orient_off.as<long long>(),

@@ -147,6 +147,7 @@ struct Device {
         sig_orient_count, orient_pairs_off, fine_items, fine_diff2, fine_min, fine_sel, ab_fine,
         ab_coarse, wsum_dev, store_items, best, tables, Y_fine, dbg_tmp, cand_w, cand_e, cand_n,
         pairs_strided, Ah, Al, As, sum_part, log_prior_o, pmin;
+    DevBuf loc_dir, loc_psi, loc_mean, loc_ang, loc_tab, loc_list;   // local searches
     PassTables coarse_tab, fine_tab;
     std::string tab_key;
 

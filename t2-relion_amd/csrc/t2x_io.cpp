@@ -569,6 +569,9 @@ extern "C" int t2x_star_particles(const t2x_star* s, long first, long n, t2x_par
         p.rot = opt("rlnAngleRot", 0.0);
         p.tilt = opt("rlnAngleTilt", 0.0);
         p.psi = opt("rlnAnglePsi", 0.0);
+        p.prior_rot = opt("rlnAngleRotPrior", 999.0);
+        p.prior_tilt = opt("rlnAngleTiltPrior", 999.0);
+        p.prior_psi = opt("rlnAnglePsiPrior", 999.0);
         if (angpix) angpix[i] = px;
     }
     return T2X_OK;

@@ -30,6 +30,42 @@ struct SelResult {
 
 enum { SRC_COARSE = 0, SRC_FINE = 1 };
 
+// Local angular searches (selectOrientationsWithNonZeroPriorProbability, healpix_sampling.cpp:
+// 769-1145): per particle, the normalised prior of every direction and psi angle (0 outside the
+// search range) and the mean of their products over the particle's orientations. dir == null:
+// global search.
+struct LocalPrior {
+    const double* dir;    // [B][Nd]
+    const double* psi;    // [B][Npsi]
+    const double* mean;   // [B]
+    int Nd, Npsi;
+};
+
+// pdf_orientation / mean of orientation n (all classes) for particle b (ml_optimiser.cpp:
+// 7700-7716); prior_o holds pdf_direction / mean (global search) or the class gate 1 / 0 (local).
+__device__ __forceinline__ double orient_prior(const double* prior_o, const LocalPrior& lp, int b, long long n)
+{
+    double p = prior_o[n];
+    if (lp.dir && p > 0.) {
+        const long long no = (long long)lp.Nd * lp.Npsi;
+        const long long o = n % no;
+        const int d = (int)(o / lp.Npsi), ps = (int)(o - (long long)d * lp.Npsi);
+        const double q = lp.dir[(size_t)b * lp.Nd + d] * lp.psi[(size_t)b * lp.Npsi + ps];
+        const double m = lp.mean[b];
+        p = (m != 0.) ? q / m : q;
+    }
+    return p;
+}
+
+// Whether orientation n is in particle b's local search range.
+__device__ __forceinline__ bool local_in_range(const LocalPrior& lp, int b, long long n)
+{
+    const long long no = (long long)lp.Nd * lp.Npsi;
+    const long long o = n % no;
+    const int d = (int)(o / lp.Npsi), ps = (int)(o - (long long)d * lp.Npsi);
+    return lp.dir[(size_t)b * lp.Nd + d] > 0. && lp.psi[(size_t)b * lp.Npsi + ps] > 0.;
+}
+
 // Description of the weights of one pass, read by the selection kernels.
 struct WeightSrc {
     int kind;                  // SRC_COARSE / SRC_FINE
@@ -53,6 +89,7 @@ struct WeightSrc {
     const unsigned int* bin_mask;
     int mask_words;            // 32-bit words per particle
     int bin_cols;              // orientations per bin
+    LocalPrior lp;             // local searches (lp.dir null: global)
 };
 
 __device__ __forceinline__ bool coarse_allowed(const WeightSrc& s, int b, long long n)

@@ -107,8 +107,18 @@ int check_config(const t2x_estep_config* c)
         set_error("bad offset or psi sampling");
         return T2X_EINVAL;
     }
-    if (c->orientational_prior_mode != T2X_NOPRIOR) {
-        set_error("only global searches (NOPRIOR) are supported");
+    if (c->orientational_prior_mode == T2X_PRIOR_ROTTILT_PSI) {
+        if (!(c->sigma_rot >= 0.) || !(c->sigma_tilt >= 0.) || !(c->sigma_psi >= 0.)) {
+            set_error("local searches: negative angular sigma");
+            return T2X_EINVAL;
+        }
+        if ((c->sigma_rot > 0.) != (c->sigma_tilt > 0.)) {
+            set_error("local searches: sigma_rot and sigma_tilt must both be set or both be 0 "
+                      "(rot-only and tilt-only priors are not supported)");
+            return T2X_EINVAL;
+        }
+    } else if (c->orientational_prior_mode != T2X_NOPRIOR) {
+        set_error("orientational_prior_mode must be T2X_NOPRIOR or T2X_PRIOR_ROTTILT_PSI");
         return T2X_EINVAL;
     }
     if (c->precision != T2X_PREC_FP32 && c->precision != T2X_PREC_BF16X3 && c->precision != T2X_PREC_FP16X3) {

@@ -27,14 +27,14 @@ __device__ __forceinline__ double pass_weight(const WeightSrc& s, int b, long lo
         if (!coarse_allowed(s, b, n)) return 0.;
         // blocks beyond the negligible-weight cut (their values may not have been stored)
         if ((double)s.rowmin[((size_t)b * s.T + t) * s.rowmin_ld + n / 64] - mind2 > 60.0) return 0.;
-        prior = s.prior_o[n] * s.prior_t[(size_t)b * s.T + t];
+        prior = orient_prior(s.prior_o, s.lp, b, n) * s.prior_t[(size_t)b * s.T + t];
         if (!(prior > 0.)) return 0.;
         d = s.d2[((size_t)b * s.T + t) * s.ld + n];
     } else {
         long long base = pair_off_b_cache[0];
         long long k = base + e / s.nover;
         int2 nt = s.pairs[k];
-        prior = s.prior_o[nt.x] * s.prior_t[(size_t)b * s.T + nt.y];
+        prior = orient_prior(s.prior_o, s.lp, b, nt.x) * s.prior_t[(size_t)b * s.T + nt.y];
         if (!(prior > 0.)) return 0.;
         d = s.d2[base * s.nover + e];
     }

@@ -24,6 +24,8 @@ T2X_ENOSIG = -6
 T2X_PREC_FP32 = 0
 T2X_PREC_BF16X3 = 1
 T2X_PREC_FP16X3 = 2
+T2X_NOPRIOR = 0
+T2X_PRIOR_ROTTILT_PSI = 1
 T2X_GATE_TOP1 = 0
 T2X_GATE_TOPK_MASS = 1
 
@@ -50,6 +52,7 @@ class EstepConfig(C.Structure):
         ("proposal_topk", C.c_int), ("proposal_gate", C.c_int),
         ("proposal_min_conf", C.c_double), ("gemm_tile", C.c_int), ("gemm_group", C.c_int),
         ("reserved", C.c_int * 2),
+        ("sigma_rot", C.c_double), ("sigma_tilt", C.c_double), ("sigma_psi", C.c_double),
     ]
 
 
@@ -64,7 +67,8 @@ class ParticleIn(C.Structure):
                 ("old_offset_x", C.c_double), ("old_offset_y", C.c_double),
                 ("prior_offset_x", C.c_double), ("prior_offset_y", C.c_double),
                 ("norm_correction", C.c_double),
-                ("rot", C.c_double), ("tilt", C.c_double), ("psi", C.c_double)]
+                ("rot", C.c_double), ("tilt", C.c_double), ("psi", C.c_double),
+                ("prior_rot", C.c_double), ("prior_tilt", C.c_double), ("prior_psi", C.c_double)]
 
 
 class ParticleOut(C.Structure):

@@ -152,8 +152,28 @@ def gen_particles(rng, n, K, max_shift, def_range=(10000.0, 30000.0), voltage=30
         pin[i].prior_offset_y = 999.0
         pin[i].norm_correction = 1.0
         pin[i].rot = pin[i].tilt = pin[i].psi = 0.0
+        pin[i].prior_rot = pin[i].prior_tilt = pin[i].prior_psi = 999.0
     truth = dict(rot=rot, tilt=tilt, psi=psi, x=sx, y=sy, cls=cls)
     return pin, truth
+
+
+def workload_particles(rng, w, n, K):
+    """Particles of a workload: poses and CTFs (gen_particles), class labels with Zipf(s) class
+    frequencies when the workload names one, per-particle SNR (a constant, or log-uniform over a
+    (lo, hi) range), in-plane poses only (rot = tilt = 0) for 2D references."""
+    cw = None
+    if w.get("zipf"):
+        cw = 1.0 / np.arange(1, K + 1) ** float(w["zipf"])
+    pin, tr = gen_particles(rng, n, K, w["max_shift"], class_weights=cw)
+    if w.get("ref_dim", 3) == 2:
+        tr["rot"][:] = 0.0
+        tr["tilt"][:] = 0.0
+    s = w["snr"]
+    if isinstance(s, (tuple, list)):
+        snr = np.exp(rng.uniform(np.log(s[0]), np.log(s[1]), n))
+    else:
+        snr = np.full(n, float(s))
+    return pin, tr, snr
 
 
 def mask_power_fraction(N, angpix, particle_diameter, width=5):

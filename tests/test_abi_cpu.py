@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 20
     for n in names:
         assert hasattr(lib, n), n
-    assert lib.t2x_abi_version() == 1
+    assert lib.t2x_abi_version() == 2
 
 
 def test_bad_arguments_return_status():
@@ -43,8 +43,13 @@ def test_bad_arguments_return_status():
     info = T.SamplingInfo()
     assert lib.t2x_sampling_info_get(C.byref(cfg), C.byref(info)) == T.T2X_EINVAL
     cfg2, _ = W.make_config(W.WORKLOADS["refine64"])
-    cfg2.orientational_prior_mode = 1
+    cfg2.orientational_prior_mode = 2
     assert lib.t2x_sampling_info_get(C.byref(cfg2), C.byref(info)) == T.T2X_EINVAL
+    cfg2.orientational_prior_mode = T.T2X_PRIOR_ROTTILT_PSI
+    cfg2.sigma_rot, cfg2.sigma_tilt = 10.0, 0.0   # rot-only prior: not supported
+    assert lib.t2x_sampling_info_get(C.byref(cfg2), C.byref(info)) == T.T2X_EINVAL
+    cfg2.sigma_tilt = 10.0
+    assert lib.t2x_sampling_info_get(C.byref(cfg2), C.byref(info)) == 0
 
 
 @pytest.mark.parametrize("name", ["refine64", "class256"])
