@@ -166,9 +166,9 @@ def pose_agreement(a, b, step_deg):
 def stage_rooflines(st, cfg, info, out, n_steps):
     """Achieved rate of every E-step stage against its MI355X bound, from the stage's GPU event
     time and the algorithmic work per unit of the counting convention (HBM bytes or flops):
-    projector O*P_c*(8*8 + 8) bytes; coarse / fine exp-weight and significance S*(4 + 8 + 32)
-    bytes on the dense sample counts (the selection skips negligible blocks, so its rate can
-    exceed the bandwidth: work avoided, not bandwidth); fine diff2 4*S_f*P_f flops; wavg
+    projector O*P_c*(8*8 + 8) bytes; coarse weights and significance in dense samples per second
+    (negligible blocks are skipped unread); fine exp-weight and significance S*(4 + 8 + 32) bytes;
+    fine diff2 4*S_f*P_f flops; wavg
     20*S_sig*P_f + 40*O_sig*P_f flops and backprojection 192*O_sig*P_f bytes (one stage,
     k_store, bounded by its atomics: the rates are reported against both peaks)."""
     K, O = cfg.nr_classes, cfg.nr_classes * info.nr_dir * info.nr_psi
@@ -189,10 +189,13 @@ def stage_rooflines(st, cfg, info, out, n_steps):
     res["projector (coarse slices, k_project_coarse)"] = {
         "ms": st.ms_coarse_project, "GB/s": gbs(b, st.ms_coarse_project),
         "frac_hbm": (gbs(b, st.ms_coarse_project) or 0) / hbm, "note": "once per call; cached across calls"}
-    b = s_coarse * 44.0
+    # the selection reads only the 64-column blocks whose row minimum is within the negligible
+    # cut (most of the dense samples are skipped unread), so its rate is reported in dense
+    # samples per second, not against the bandwidth
     res["coarse weights + significance (FP64)"] = {
-        "ms": st.ms_coarse_weights, "GB/s": gbs(b, st.ms_coarse_weights),
-        "frac_hbm": (gbs(b, st.ms_coarse_weights) or 0) / hbm}
+        "ms": st.ms_coarse_weights,
+        "dense_Gsamples_per_s": s_coarse / (st.ms_coarse_weights * 1e-3) / 1e9 if st.ms_coarse_weights > 0 else None,
+        "note": "negligible blocks skipped unread: not a bandwidth figure"}
     res["fine diff2 (k_fine_diff2)"] = {
         "ms": st.ms_fine_diff2, "TFLOP/s": tfs(st.fine_flop, st.ms_fine_diff2),
         "frac_fp32": (tfs(st.fine_flop, st.ms_fine_diff2) or 0) / fp32}
@@ -206,6 +209,31 @@ def stage_rooflines(st, cfg, info, out, n_steps):
         "ms": st.ms_store, "TFLOP/s": tfs(f, st.ms_store), "frac_fp32": (tfs(f, st.ms_store) or 0) / fp32,
         "GB/s": gbs(b, st.ms_store), "frac_hbm": (gbs(b, st.ms_store) or 0) / hbm}
     return res
+
+
+def gemm_library_reference(M, N, K, reps=10):
+    """The vendor library's FP16 GEMM (hipBLASLt through torch.matmul, FP32 accumulate, FP16
+    output, no epilogue) at the coarse GEMM's shape: C[M][N] = A[M][K] B[N][K]^T, the layout of
+    the cross term (rows = particle translations, columns = orientations, K = re/im pixels)."""
+    import torch
+    dev = torch.device("cuda:0")
+    a = torch.randn(M, K, device=dev, dtype=torch.float16)
+    b = torch.randn(N, K, device=dev, dtype=torch.float16)
+    for _ in range(3):
+        c = a @ b.t()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        c = a @ b.t()
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b, c
+    torch.cuda.empty_cache()
+    tf = 2.0 * M * N * K / (ms * 1e-3) / 1e12
+    return {"what": "torch.matmul fp16 (hipBLASLt), C = A B^T, no epilogue", "M": M, "N": N, "K": K,
+            "ms": ms, "TFLOP/s": tf, "frac_fp16_peak": tf / 2500.0}
 
 
 def fsc_resolution(fsc, ori, angpix):
@@ -301,7 +329,15 @@ def main():
                     help="use proposal parameters saved by an earlier run instead of training")
     ap.add_argument("--quality-particles", type=int, default=2000,
                     help="particles of the half-map FSC phase (0: skip)")
+    ap.add_argument("--gemm-library-reference", action="store_true",
+                    help="also time the vendor library's FP16 GEMM at the coarse GEMM's shape "
+                         "(800-particle batch rows x all orientations x 2 P_c) for comparison")
+    ap.add_argument("--gemm-library-shape", default="", help=argparse.SUPPRESS)
     args = ap.parse_args()
+    if args.gemm_library_shape:   # child of --gemm-library-reference
+        M, N, K = (int(x) for x in args.gemm_library_shape.split(","))
+        print(json.dumps(gemm_library_reference(M, N, K)), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the E-step "
               "shards over particles with one process per GPU, which is not part of this build.",
@@ -396,7 +432,7 @@ def main():
     traffic = os.environ.get("T2X_GEMM_TRAFFIC_BYTES")
     traffic_src = "env T2X_GEMM_TRAFFIC_BYTES" if traffic else None
     tj = os.path.join(REPO, "profiles", "coarse_gemm_traffic.json")
-    if traffic is None and os.path.exists(tj):
+    if traffic is None and os.path.exists(tj) and args.workload == "class256":
         with open(tj) as f:
             tr = json.load(f).get(head)
         if tr:
@@ -460,6 +496,22 @@ def main():
                    "pose_agreement_target": 0.99,
                    "max_fsc_0143_diff_A_vs_fp32": fsc_d, "fsc_0143_diff_target_A": 0.1}
     cpu = None if args.no_cpu else cpu_baseline(w, cfg, sz)
+    libref = None
+    if args.gemm_library_reference:
+        kc = 2 * ((st.P_coarse + 31) // 32 * 32)
+        o_all = cfg.nr_classes * ds["info"].nr_dir * ds["info"].nr_psi
+        # in a child process: torch brings its own HIP runtime, which cannot share this process
+        # with the one libt2x is linked against
+        import subprocess
+        shape = f"{min(n, 800) * ds['info'].nr_trans},{o_all},{kc}"
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--gemm-library-shape", shape],
+                           capture_output=True, text=True, timeout=300)
+        if r.returncode == 0 and r.stdout.strip():
+            libref = json.loads(r.stdout.strip().splitlines()[-1])
+            log(f"[library reference] {libref['TFLOP/s']:.0f} TFLOP/s ({libref['frac_fp16_peak']:.3f} of the FP16 peak)")
+        else:
+            libref = {"error": (r.stderr or "")[-400:]}
+            log("[library reference] failed: " + libref["error"])
     line = {
         "metric": METRIC, "value": value, "unit": "particles/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps,
@@ -482,6 +534,7 @@ def main():
         "quality": quality,
         "targets": targets,
         "roofline": roof,
+        "gemm_library_reference": libref,
         "stage_roofline": stage_roof,
         "cpu_baseline": cpu,
     }

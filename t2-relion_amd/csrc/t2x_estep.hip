@@ -265,6 +265,96 @@ __global__ __launch_bounds__(256) void k_coarse_imgside(const float2* __restrict
     }
 }
 
+// pdf_offset / mean of every coarse translation of each particle (ml_optimiser.cpp:7638-7667,
+// 7701-7725): one workgroup per particle, the mean summed in translation order. The offsets mix
+// pixels (old offset, prior) and the translation grid's Angstrom as the reference does.
+__global__ __launch_bounds__(128) void k_prior_t(const PartScalars* __restrict__ ps, int B,
+                                                 const double* __restrict__ tx, const double* __restrict__ ty,
+                                                 int T, double pix, double sigma2_offset,
+                                                 double* __restrict__ prior_t)
+{
+    __shared__ double pt[128];
+    __shared__ double s_mean;
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    const PartScalars s = ps[b];
+    for (int t = threadIdx.x; t < T; t += blockDim.x) {
+        const double ox = s.old_x + tx[t], oy = s.old_y + ty[t];
+        const double td2 = ((ox - s.prior_x) * (ox - s.prior_x) + (oy - s.prior_y) * (oy - s.prior_y)) * pix * pix;
+        pt[t] = (sigma2_offset < 0.0001) ? ((td2 > 0.) ? 0. : 1.)
+                                         : exp(td2 / (-2. * sigma2_offset)) / (2. * kPi * sigma2_offset);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double m = 0.;
+        for (int t = 0; t < T; t++) m += pt[t];
+        s_mean = m / (double)T;
+    }
+    __syncthreads();
+    const double m = s_mean;
+    for (int t = threadIdx.x; t < T; t += blockDim.x) prior_t[(size_t)b * T + t] = (m > 0.) ? pt[t] / m : pt[t];
+}
+
+// k_coarse_imgside followed by the split of each row into hi + lo halves (k_split_rows), fused:
+// one workgroup per GEMM row (particle b, translation t; rows >= M are zero), the row's values
+// kept in LDS between the max-abs reduction and the split, so the FP32 A matrix is neither
+// cleared, written nor read back. Same arithmetic as the two kernels, so the same bits.
+template <class H>
+__global__ __launch_bounds__(256) void k_coarse_imgside_split(const float2* __restrict__ Fimg,
+                                                              const float* __restrict__ Fctf, int Pw,
+                                                              const int* __restrict__ win,
+                                                              const float* __restrict__ minv, int npix,
+                                                              const float2* __restrict__ ab, int T, int M, int Kc,
+                                                              H* __restrict__ hi, H* __restrict__ lo,
+                                                              float* __restrict__ inv_scale, int use_scale,
+                                                              float* __restrict__ A1, long long lda1)
+{
+    extern __shared__ float row[];   // [Kc]
+    __shared__ float red[4];
+    const int r = blockIdx.x;
+    const bool valid = r < M;
+    const int b = valid ? r / T : 0, t = valid ? r - b * T : 0;
+    float mx = 0.f;
+    for (int p = threadIdx.x; p < Kc / 2; p += blockDim.x) {
+        float vr = 0.f, vi = 0.f;
+        if (valid && p < npix) {
+            const size_t w = (size_t)b * Pw + win[p];
+            const float2 x = Fimg[w];
+            const float ctf = Fctf[w], m = minv[p];
+            const float s = m * ctf;
+            const float2 sh = ab[(size_t)t * npix + p];
+            const float xr = x.x * sh.x - x.y * sh.y, xi = x.x * sh.y + x.y * sh.x;
+            vr = s * xr;
+            vi = s * xi;
+            if (t == 0) A1[(size_t)b * lda1 + p] = 0.5f * m * ctf * ctf;
+        }
+        row[2 * p] = vr;
+        row[2 * p + 1] = vi;
+        mx = fmaxf(mx, fmaxf(fabsf(vr), fabsf(vi)));
+    }
+    float sc = 1.f;
+    if (use_scale) {
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+        __syncthreads();
+        mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+        int e = 0;
+        if (mx > 0.f) (void)frexpf(mx, &e);
+        sc = ldexpf(1.f, -e);
+        if (threadIdx.x == 0) inv_scale[r] = ldexpf(1.f, e);
+    } else {
+        __syncthreads();
+        if (threadIdx.x == 0) inv_scale[r] = 1.f;
+    }
+    for (int k = threadIdx.x; k < Kc; k += blockDim.x) {
+        const float v = row[k] * sc;
+        const H h = (H)v;
+        const H l = (H)(v - (float)h);
+        hi[(size_t)r * Kc + k] = h;
+        lo[(size_t)r * Kc + k] = l;
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // Fine pass
 // ------------------------------------------------------------------------------------------
@@ -1130,6 +1220,9 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     T2X_TRY(upload(d_bpw, h_bpw, st));
     T2X_TRY(upload(d_bpads, h_bpads, st));
     T2X_TRY(upload(d_bprmax, h_bprmax, st));
+    DevBuf d_ctx, d_cty;   // coarse translations (A) for k_prior_t
+    T2X_TRY(upload(d_ctx, S.tx, st));
+    T2X_TRY(upload(d_cty, S.ty, st));
     LocalSelParams lsp{};
     if (local) {
         std::vector<double> tab(2 * Nd + Npsi);
@@ -1263,7 +1356,6 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         }
         std::vector<PartScalars> hps(B);
         std::vector<CtfConst> hctf(B);
-        std::vector<double> prior_t((size_t)B * T);
         for (int i = 0; i < B; i++) {
             const t2x_particle_in& p = pin[b0 + i];
             PartScalars& s = hps[i];
@@ -1277,25 +1369,15 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             s.prior_y = ref2d ? class_prior_y : py_;
             s.norm_scale = cfg->do_norm_correction ? cfg->avg_norm_correction / p.norm_correction : 1.0;
             hctf[i] = ctf_const(p.ctf);
-            // pdf_offset / mean (ml_optimiser.cpp:7638-7667, 7701-7725); offsets mix px and A as there
-            std::vector<double> pt(T);
-            double mean_t = 0.;
-            for (long t = 0; t < T; t++) {
-                double ox = s.old_x + S.tx[t], oy = s.old_y + S.ty[t];
-                double td2 = ((ox - s.prior_x) * (ox - s.prior_x) + (oy - s.prior_y) * (oy - s.prior_y)) * pix * pix;
-                double v = (cfg->sigma2_offset < 0.0001) ? ((td2 > 0.) ? 0. : 1.)
-                                                         : std::exp(td2 / (-2. * cfg->sigma2_offset)) / (2. * kPi * cfg->sigma2_offset);
-                pt[t] = v;
-                mean_t += v;
-            }
-            mean_t /= (double)T;
-            for (long t = 0; t < T; t++) prior_t[(size_t)i * T + t] = (mean_t > 0.) ? pt[t] / mean_t : pt[t];
         }
         T2X_TRY(upload(d->pscal, hps, st));
         DevBuf d_ctf;
         T2X_TRY(upload(d_ctf, hctf, st));
-        T2X_TRY(upload(d->prior_t, prior_t, st));
         PartScalars* d_ps = d->pscal.as<PartScalars>();
+        T2X_TRY(d->prior_t.ensure(sizeof(double) * B * T));
+        k_prior_t<<<B, 128, 0, st>>>(d_ps, B, d_ctx.as<double>(), d_cty.as<double>(), (int)T, pix,
+                                     cfg->sigma2_offset, d->prior_t.as<double>());
+        T2X_HIP(hipGetLastError());
         // local searches: each particle's direction / psi priors around its prior angles (the
         // current angles where the priors are unset, ml_optimiser.cpp:5851-5857)
         LocalPrior lp{};
@@ -1358,21 +1440,42 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         const long long M = (long long)B * T;
         const long long M_pad = round_up(M, 256);
         const long long B_pad = round_up(B, prec == T2X_PREC_FP32 ? 128 : 256);   // N1 GEMM rows
-        T2X_TRY(d->A.ensure(sizeof(float) * M_pad * Kc));
+        if (prec == T2X_PREC_FP32) T2X_TRY(d->A.ensure(sizeof(float) * M_pad * Kc));
         T2X_TRY(d->A1.ensure(sizeof(float) * B_pad * Kn));
         T2X_TRY(d->N1.ensure(sizeof(float) * B_pad * O_pad));
         T2X_TRY(d->diff2.ensure(sizeof(float) * M_pad * O_pad));
         const int rowmin_ld = (int)(O_pad / 64);
         DevBuf& d_rowmin = d->minb;
         T2X_TRY(d_rowmin.ensure(sizeof(float) * M_pad * rowmin_ld));
-        T2X_HIP(hipMemsetAsync(d->A.ptr, 0, sizeof(float) * M_pad * Kc, st));
         T2X_HIP(hipMemsetAsync(d->A1.ptr, 0, sizeof(float) * B_pad * Kn, st));
-        {
+        if (prec == T2X_PREC_FP32) {
+            T2X_HIP(hipMemsetAsync(d->A.ptr, 0, sizeof(float) * M_pad * Kc, st));
             dim3 grid((unsigned)T, (unsigned)B);
             k_coarse_imgside<<<grid, 256, 0, st>>>(d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw,
                                                    tc.d_win.as<int>(), tc.d_minv.as<float>(), tc.npix,
                                                    d->ab_coarse.as<float2>(), (int)T, d->A.as<float>(), Kc,
                                                    d->A1.as<float>(), Kn);
+            T2X_HIP(hipGetLastError());
+        } else {
+            // the image side straight into the split operand (hi, lo, per-row scale)
+            T2X_TRY(d->Ah.ensure(sizeof(uint16_t) * M_pad * Kc));
+            T2X_TRY(d->Al.ensure(sizeof(uint16_t) * M_pad * Kc));
+            T2X_TRY(d->As.ensure(sizeof(float) * M_pad));
+            const size_t lds = sizeof(float) * Kc;
+            if (lds > 65536) {
+                set_error("coarse image too large for the split operand kernel (2 P_c > 16384)");
+                return T2X_EINVAL;
+            }
+            if (prec == T2X_PREC_BF16X3)
+                k_coarse_imgside_split<__bf16><<<(unsigned)M_pad, 256, lds, st>>>(
+                    d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, tc.d_win.as<int>(),
+                    tc.d_minv.as<float>(), tc.npix, d->ab_coarse.as<float2>(), (int)T, (int)M, Kc,
+                    (__bf16*)d->Ah.ptr, (__bf16*)d->Al.ptr, d->As.as<float>(), 0, d->A1.as<float>(), Kn);
+            else
+                k_coarse_imgside_split<_Float16><<<(unsigned)M_pad, 256, lds, st>>>(
+                    d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, tc.d_win.as<int>(),
+                    tc.d_minv.as<float>(), tc.npix, d->ab_coarse.as<float2>(), (int)T, (int)M, Kc,
+                    (_Float16*)d->Ah.ptr, (_Float16*)d->Al.ptr, d->As.as<float>(), 1, d->A1.as<float>(), Kn);
             T2X_HIP(hipGetLastError());
         }
         {
@@ -1489,11 +1592,6 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             } else {
                 const GemmChoice gwant{cfg->gemm_tile, cfg->gemm_group};
                 GemmChoice gused{0, 0};
-                T2X_TRY(d->Ah.ensure(sizeof(uint16_t) * M_pad * Kc));
-                T2X_TRY(d->Al.ensure(sizeof(uint16_t) * M_pad * Kc));
-                T2X_TRY(d->As.ensure(sizeof(float) * M_pad));
-                T2X_TRY(launch_split(prec, d->A.as<float>(), Kc, Kc, (int)M_pad, (int)M, d->Ah.ptr, d->Al.ptr, Kc,
-                                     d->As.as<float>(), st));
                 if (use_prop) {
                     if (n_fb_rows > 0) {
                         EpiParams ef = ep;
