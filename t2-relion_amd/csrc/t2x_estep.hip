@@ -1180,7 +1180,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     // the diff2 kernel reads minv of the diff2 definition)
     std::vector<float> minv_diff2(tf.npix);
     for (int p = 0; p < tf.npix; p++) minv_diff2[p] = (tf.h_ires[p] > 0) ? tf.h_minv[p] : 0.f;
-    DevBuf d_minv_fd;
+    DevBuf& d_minv_fd = d->minv_fd;
     T2X_TRY(upload(d_minv_fd, minv_diff2, st));
     // phase tables
     std::vector<float2> abc = phase_table(tc, S.coarse_tx_px, S.coarse_ty_px, N);
@@ -1209,7 +1209,9 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                                                                                   d->log_prior_o.as<double>());
     T2X_HIP(hipGetLastError());
     // device copies of angle tables and class pointer arrays
-    DevBuf d_dirsc, d_psisc, d_ftx, d_fty, d_vols, d_pads, d_bpd, d_bpw, d_bpads, d_bprmax;
+    DevBuf &d_dirsc = d->dirsc, &d_psisc = d->psisc, &d_ftx = d->ftx, &d_fty = d->fty, &d_vols = d->vols_tab,
+           &d_pads = d->pads_tab, &d_bpd = d->bpd_tab, &d_bpw = d->bpw_tab, &d_bpads = d->bpads_tab,
+           &d_bprmax = d->bprmax_tab;
     T2X_TRY(upload(d_dirsc, S.sub_dir_sc, st));
     T2X_TRY(upload(d_psisc, S.sub_psi_sc, st));
     T2X_TRY(upload(d_ftx, S.fine_tx_px, st));
@@ -1220,7 +1222,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     T2X_TRY(upload(d_bpw, h_bpw, st));
     T2X_TRY(upload(d_bpads, h_bpads, st));
     T2X_TRY(upload(d_bprmax, h_bprmax, st));
-    DevBuf d_ctx, d_cty;   // coarse translations (A) for k_prior_t
+    DevBuf &d_ctx = d->ctx, &d_cty = d->cty;   // coarse translations (A) for k_prior_t
     T2X_TRY(upload(d_ctx, S.tx, st));
     T2X_TRY(upload(d_cty, S.ty, st));
     LocalSelParams lsp{};
@@ -1371,7 +1373,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             hctf[i] = ctf_const(p.ctf);
         }
         T2X_TRY(upload(d->pscal, hps, st));
-        DevBuf d_ctf;
+        DevBuf& d_ctf = d->ctfc;
         T2X_TRY(upload(d_ctf, hctf, st));
         PartScalars* d_ps = d->pscal.as<PartScalars>();
         T2X_TRY(d->prior_t.ensure(sizeof(double) * B * T));
@@ -1669,8 +1671,12 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                                               d_cand_count, d->pairs_strided.as<int2>(),
                                               d->sig_count.as<long long>(), d_overflow);
             T2X_HIP(hipGetLastError());
+            // overflow flags, counts and results in one synchronisation; the counts and results are
+            // read again only when a particle overflowed
             std::vector<int> hover(B);
             T2X_HIP(hipMemcpyAsync(hover.data(), d_overflow, sizeof(int) * B, hipMemcpyDeviceToHost, st));
+            T2X_HIP(hipMemcpyAsync(counts.data(), d->sig_count.ptr, sizeof(long long) * 2 * B, hipMemcpyDeviceToHost, st));
+            T2X_HIP(hipMemcpyAsync(hselc.data(), d_selc, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
             T2X_HIP(hipStreamSynchronize(st));
             fallback = false;
             for (int i = 0; i < B; i++) fallback |= hover[i] != 0;
@@ -1681,10 +1687,10 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             k_sig_count<<<B, 1024, 0, st>>>(wc, B, d_selc, d->sig_count.as<long long>());
             T2X_HIP(hipGetLastError());
             stt.n_select_fallbacks += 1;
+            T2X_HIP(hipMemcpyAsync(counts.data(), d->sig_count.ptr, sizeof(long long) * 2 * B, hipMemcpyDeviceToHost, st));
+            T2X_HIP(hipMemcpyAsync(hselc.data(), d_selc, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
+            T2X_HIP(hipStreamSynchronize(st));
         }
-        T2X_HIP(hipMemcpyAsync(counts.data(), d->sig_count.ptr, sizeof(long long) * 2 * B, hipMemcpyDeviceToHost, st));
-        T2X_HIP(hipMemcpyAsync(hselc.data(), d_selc, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
-        T2X_HIP(hipStreamSynchronize(st));
         (void)hipEventRecord(d->ev[ev++], st);   // 4: coarse weights done
         std::vector<long long> pair_off(B + 1, 0), orient_off(B + 1, 0);
         for (int i = 0; i < B; i++) {
@@ -1697,7 +1703,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             orient_off[i + 1] = orient_off[i] + counts[2 * i + 1];
         }
         const long long npairs = pair_off[B], norients = orient_off[B];
-        DevBuf d_pair_off, d_orient_off, d_item_off;
+        DevBuf &d_pair_off = d->pair_off_d, &d_orient_off = d->orient_off_d, &d_item_off = d->item_off_d;
         T2X_TRY(upload(d_pair_off, pair_off, st));
         T2X_TRY(upload(d_orient_off, orient_off, st));
         T2X_TRY(d->sig_pairs.ensure(sizeof(int2) * std::max<long long>(npairs, 1)));

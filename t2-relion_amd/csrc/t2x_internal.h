@@ -148,6 +148,9 @@ struct Device {
         ab_coarse, wsum_dev, store_items, best, tables, Y_fine, dbg_tmp, cand_w, cand_e, cand_n,
         pairs_strided, Ah, Al, As, sum_part, log_prior_o, pmin;
     DevBuf loc_dir, loc_psi, loc_mean, loc_ang, loc_tab, loc_list;   // local searches
+    // per-call tables and per-batch offsets (grow-only, so no allocation per call or batch)
+    DevBuf minv_fd, dirsc, psisc, ftx, fty, vols_tab, pads_tab, bpd_tab, bpw_tab, bpads_tab, bprmax_tab,
+        ctx, cty, ctfc, pair_off_d, orient_off_d, item_off_d;
     PassTables coarse_tab, fine_tab;
     std::string tab_key;
 
