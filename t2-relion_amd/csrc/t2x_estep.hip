@@ -1318,7 +1318,9 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         size_t fr = 0, tot = 0;
         if (hipMemGetInfo(&fr, &tot) == hipSuccess && fr / 3 < budget) budget = fr / 3;
     }
-    int Bmax = (int)std::max<size_t>(1, std::min<size_t>(1024, budget / per_particle));
+    // up to 4,096 particles per batch (small references, e.g. 2D classification, amortise the
+    // per-batch launches and synchronisations over more particles)
+    int Bmax = (int)std::max<size_t>(1, std::min<size_t>(4096, budget / per_particle));
     const char* env_b = getenv("T2X_BATCH");
     if (env_b && atoi(env_b) > 0) Bmax = std::min(Bmax, atoi(env_b));
 
