@@ -881,6 +881,33 @@ extern "C" int orc_synth_reference(int N, double angpix, int pad, int r_max, dou
     return T2X_OK;
 }
 
+/* The central (kz = 0) plane of orc_synth_reference's volume alone: the transform of the blobs'
+ * z-projection, i.e. a 2D class average (pad x (pad/2+1) complex). */
+extern "C" int orc_synth_reference_plane(int N, double angpix, int pad, int r_max, double pf,
+                                         const t2x_blob* blobs, int nb, double lowpass_A, double* out)
+{
+    int xdim = pad / 2 + 1, h = pad / 2;
+    int max_r2 = R_ROUND(r_max * pf) * R_ROUND(r_max * pf);
+    double cutoff_shell = lowpass_A > 0 ? N * angpix / lowpass_A : 0;
+    double inv = 1.0 / ((double)N * N);
+#pragma omp parallel for schedule(dynamic)
+    for (int ky = 0; ky < pad; ky++)
+        for (int kx = 0; kx < xdim; kx++) {
+            int y = ky - h, x = kx;
+            size_t idx = (size_t)ky * xdim + kx;
+            int r2 = x * x + y * y;
+            cplx v = 0;
+            if (r2 <= max_r2) {
+                double f = 1.0 / (pf * N);
+                v = blob_ft(blobs, nb, x * f, y * f, 0 * f) * inv;
+                v *= lowpass_weight(std::sqrt((double)r2) / pf, cutoff_shell);
+            }
+            out[2 * idx] = v.real();
+            out[2 * idx + 1] = v.imag();
+        }
+    return T2X_OK;
+}
+
 /* splitmix64 finaliser */
 static inline uint64_t mix64(uint64_t z)
 {

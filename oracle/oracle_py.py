@@ -58,6 +58,8 @@ def load():
         "orc_backproject": (None, [C.POINTER(Volume), PD, PD, C.c_int, PD]),
         "orc_synth_reference": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
                                           C.POINTER(T.Blob), C.c_int, C.c_double, PD]),
+        "orc_synth_reference_plane": (C.c_int, [C.c_int, C.c_double, C.c_int, C.c_int, C.c_double,
+                                                C.POINTER(T.Blob), C.c_int, C.c_double, PD]),
         "orc_synth_particles": (C.c_int, [C.c_int, C.c_double, C.c_int, C.POINTER(T.Blob), PI,
                                           C.POINTER(T.ParticleIn), PD, PD, PD, PD, PD, PI, PD,
                                           C.c_uint64, C.POINTER(C.c_float), PD]),

@@ -126,3 +126,52 @@ def test_fp16x3_2d(class2d_pair, gpu_device):
     for c in range(wl.cfg.nr_classes):
         if np.abs(o["bp"][c][1]).sum() > 0:
             assert H.rel_norm(g["bp"][c][1], o["bp"][c][1]) < 1e-4
+
+
+@pytest.fixture(scope="module")
+def class2d128_case():
+    """The 2D-classification configuration's own shapes: 128^2, 100 class averages, Zipf(1.2)
+    class frequencies, per-particle SNR log-uniform in [0.01, 0.1]; 16 particles."""
+    wl = H.Workload("class2d128", n=16)
+    return wl, wl.run_oracle()
+
+
+@pytest.mark.parametrize("prec", [0, 2])
+def test_class2d128_shapes(class2d128_case, gpu_device, prec):
+    wl, o = class2d128_case
+    wl.cfg.precision = prec
+    g = wl.run_gpu(dev=gpu_device)
+    wl.cfg.precision = 0
+    assert (o["out"]["best_hidden"] == g["out"]["best_hidden"]).mean() >= 0.999
+    assert H.rel(g["out"]["dLL"], o["out"]["dLL"]) < 1e-5
+    for k, v in o["wsum"].items():
+        if np.abs(v).sum() > 0:
+            assert H.rel_norm(g["wsum"][k], v) < 1e-4, k
+    for c in range(wl.cfg.nr_classes):
+        if np.abs(o["bp"][c][1]).sum() > 0:
+            assert H.rel_norm(g["bp"][c][1], o["bp"][c][1]) < 1e-4, c
+            assert H.rel_norm(g["bp"][c][0], o["bp"][c][0]) < 1e-4, c
+
+
+@pytest.fixture(scope="module")
+def class2d64_many():
+    wl = H.Workload("class2d64", n=1500, seed_offset=6)
+    return wl, wl.run_oracle()
+
+
+@pytest.mark.parametrize("batch", [0, 700])
+def test_large_and_ragged_batches(class2d64_many, gpu_device, monkeypatch, batch):
+    """1,500 particles in one batch (above 1,024), or in batches of 700, 700 and 100."""
+    wl, o = class2d64_many
+    if batch:
+        monkeypatch.setenv("T2X_BATCH", str(batch))
+    g = wl.run_gpu(dev=gpu_device)
+    agree = (o["out"]["best_hidden"] == g["out"]["best_hidden"]).mean()
+    print("1500-particle pose agreement", agree)
+    assert agree >= 0.999
+    assert H.rel(g["out"]["dLL"], o["out"]["dLL"]) < 1e-5
+    for k, v in o["wsum"].items():
+        if np.abs(v).sum() > 0:
+            assert H.rel_norm(g["wsum"][k], v) < 1e-4, k
+    for c in range(wl.cfg.nr_classes):
+        assert H.rel_norm(g["bp"][c][1], o["bp"][c][1]) < 1e-4

@@ -135,3 +135,15 @@ def test_sampling_2d(orc, name):
         want = ps * 6.0 + np.array([-1.5, 1.5])
         want = np.where(want > 180, want - 360, want)
         np.testing.assert_allclose(p, want, atol=1e-12)
+
+
+def test_reference_plane_is_central_plane_of_volume(orc):
+    rng = np.random.default_rng(14)
+    blobs = W.gen_blobs(rng, 12, 8.0, (1.5, 3.0), (0.5, 1.0))
+    N, pad, rmax = 32, 2 * (2 * 8 + 1) + 1, 8
+    xdim = pad // 2 + 1
+    vol = np.zeros(2 * pad * pad * xdim)
+    plane = np.zeros(2 * pad * xdim)
+    assert orc.orc_synth_reference(N, 2.0, pad, rmax, 2.0, blobs, 12, 12.0, T.dptr(vol)) == 0
+    assert orc.orc_synth_reference_plane(N, 2.0, pad, rmax, 2.0, blobs, 12, 12.0, T.dptr(plane)) == 0
+    assert np.array_equal(vol.reshape(pad, -1)[pad // 2], plane)
