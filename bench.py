@@ -367,8 +367,9 @@ def main():
         T.check(lib.t2x_device_free(dev, src))
         T.check(lib.t2x_device_free(dev, dst))
         log("copy calibration: 1 GiB device-to-device copy (t2x::k_copy16) done")
-    paths = (args.precision or ("fp32,fp16x3,learned,learned_gated" if cfg.ref_dim == 3 else
-                                "fp32,fp16x3")).split(",")
+    local = cfg.orientational_prior_mode == T.T2X_PRIOR_ROTTILT_PSI
+    paths = (args.precision or ("fp32,fp16x3" if local or cfg.ref_dim == 2 else
+                                "fp32,fp16x3,learned,learned_gated")).split(",")
     train = None
     use_net = any(p.startswith("learned") for p in paths)
     if use_net and args.load_proposal:
@@ -441,7 +442,9 @@ def main():
     roof = {"bound": "mfma", "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
             "frac": achieved / peak, "traffic": float(traffic) if traffic else None,
             "traffic_unit": "bytes per launch", "traffic_source": traffic_src,
-            "kernel": "coarse diff2 cross term, " + KERNELS[head],
+            "kernel": ("coarse diff2 of local searches: (particle, orientation) rows x translations, k_local_zrows + "
+                       + ("k_gemm_tn (FP32 MFMA)" if head == "fp32" else "k_gemm_x3 128x128 (split MFMA)") + " + k_local_diff2" if local
+                       else "coarse diff2 cross term, " + KERNELS[head]),
             "avg_launch_ms": avg_ms, "launches": launches, "flop_per_launch": flop_per_launch}
     stage_roof = stage_rooflines(st, cfg, ds["info"], r["out"], args.steps)
     stages = {k: getattr(st, k) for k in ("ms_preprocess", "ms_coarse_project", "ms_coarse_diff2",
@@ -475,7 +478,7 @@ def main():
                       "train_top1_last": train.top1_last if train else None})
         variants[pname] = v
     quality = None
-    if args.quality_particles > 0 and cfg.ref_dim == 3:
+    if args.quality_particles > 0 and cfg.ref_dim == 3 and not local:
         nq = min(n, args.quality_particles)
         q = quality_phase(lib, dev, cfg, sz, ds, list(runs.keys()), nq)
         quality = {"particles": nq, "fsc_0143_A_per_class": {k: v["fsc_res"] for k, v in q.items()}}
@@ -521,6 +524,7 @@ def main():
                                f"K={cfg.nr_classes}, "
                                + (f"HEALPix order {cfg.healpix_order}" if cfg.ref_dim == 3 else
                                   f"2D references, psi step {cfg.psi_step} deg")
+                               + (f", local searches sigma_ang {cfg.sigma_rot} deg" if local else "")
                                + f", offsets "
                                f"{w['offset_range_px']}/{w['offset_step_px']} px, current_size "
                                f"{cfg.current_size}, coarse_size {cfg.coarse_size}, path {head}",

@@ -395,7 +395,9 @@ __device__ __forceinline__ void fine_matrix(const FineGeom& g, int n, int ir, do
 // Fine diff2 of every significant coarse pair: one workgroup per (particle, significant
 // orientation, oversampled rotation). The slice at current size is projected into LDS as
 // Z = F conj(minv ctf X); each wave then scores one oversampled translation at a time,
-// G_t = sum_p Re(Z conj(AB_t)), diff2 = C_b + 1/2 sum minv ctf^2 |F|^2 - G_t.
+// G_t = sum_p Re(Z conj(AB_t)), diff2 = C_b + 1/2 sum minv ctf^2 |F|^2 - G_t. With coarse_C (and
+// the coarse pixel set, phase table and orientation grid, no oversampling) the same kernel scores
+// the coarse pass of a local search on each particle's own sample list.
 __global__ __launch_bounds__(256) void k_fine_diff2(
     const long long* __restrict__ item_off, int B, const long long* __restrict__ orient_off,
     const int* __restrict__ orients, const long long* __restrict__ orient_pair_start,
@@ -405,7 +407,7 @@ __global__ __launch_bounds__(256) void k_fine_diff2(
     const float* __restrict__ minv, const int* __restrict__ win, int npix,
     const float2* __restrict__ Fimg, const float* __restrict__ Fctf, int Pw,
     const float2* __restrict__ ab_fine, const PartScalars* __restrict__ ps,
-    float* __restrict__ fine_d2, unsigned int* __restrict__ fine_min)
+    float* __restrict__ fine_d2, unsigned int* __restrict__ fine_min, int coarse_C)
 {
     __shared__ float2 Z[FINE_MAX_PIX];
     __shared__ double red[4];
@@ -446,7 +448,7 @@ __global__ __launch_bounds__(256) void k_fine_diff2(
     for (int o = 32; o > 0; o >>= 1) n1 += __shfl_xor(n1, o);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n1;
     __syncthreads();
-    const double base = ps[b].C_fine + red[0] + red[1] + red[2] + red[3];
+    const double base = (coarse_C ? ps[b].C_coarse : ps[b].C_fine) + red[0] + red[1] + red[2] + red[3];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nt = (int)(p1 - p0) * g.over_trans;
     float lmin = 3.0e38f;
@@ -472,6 +474,140 @@ __global__ __launch_bounds__(256) void k_fine_diff2(
     if (threadIdx.x == 0) {
         float m = fminf(fminf(wmin[0], wmin[1]), fminf(wmin[2], wmin[3]));
         if (m < 3.0e38f) atomicMin(&fine_min[b], float_to_ordered(m));
+    }
+}
+
+// Local searches, coarse pass as a GEMM over the batch's (particle, orientation) rows:
+// G[(b,o)][t] = sum_p Re(Z_bo,p conj(AB_t,p)) with Z_bo = F_o conj(minv ctf X_b), i.e. the real
+// dot product of [Z.re, Z.im] with [AB.re, AB.im] over K = 2 P_c; the phase table AB is the
+// shared operand. k_local_zrows projects each row's slice and forms Z (FP32, or straight into the
+// hi/lo split operand with its row scale) and n1 = 1/2 sum minv ctf^2 |F|^2; k_local_diff2 forms
+// diff2 = C_coarse + n1 - G into the sample list (orientation-major, T per orientation).
+template <class H, int SPLIT>
+__global__ __launch_bounds__(256) void k_local_zrows(int B, long long nrows, const long long* __restrict__ orient_off,
+                                                     const int* __restrict__ orients, FineGeom g,
+                                                     const float2* const* __restrict__ vols,
+                                                     const int* __restrict__ pads, const int2* __restrict__ pix,
+                                                     const unsigned char* __restrict__ inproj,
+                                                     const float* __restrict__ minv, const int* __restrict__ win,
+                                                     int npix, const float2* __restrict__ Fimg,
+                                                     const float* __restrict__ Fctf, int Pw, int Kc,
+                                                     float* __restrict__ zf, H* __restrict__ zh, H* __restrict__ zl,
+                                                     float* __restrict__ zs, double* __restrict__ n1out)
+{
+    extern __shared__ float zrow[];   // [Kc] (split operand only)
+    __shared__ double red[4];
+    __shared__ float redf[4];
+    // XCD-aware row order: workgroups are dealt round-robin to the 8 XCDs, so XCD k is given the
+    // contiguous rows [k R/8, (k+1) R/8) (the grid is a multiple of 8). Consecutive rows of a
+    // particle are psi rotations of one direction, i.e. the same central plane of the volume,
+    // whose voxels then stay in that XCD's L2.
+    const long long per = (long long)gridDim.x / 8;
+    const long long j = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    const bool valid = j < nrows;
+    float mx = 0.f;
+    double n1 = 0.;
+    if (valid) {
+        const int b = find_particle(orient_off, B, j);
+        const int n = orients[j];
+        double A[9], Ai[9];
+        int c, idir;
+        fine_matrix(g, n, 0, A, &c, &idir);
+        inv3(A, Ai);
+        const float a0 = (float)(Ai[0] * g.pf), a1 = (float)(Ai[1] * g.pf);
+        const float a2 = (float)(Ai[3] * g.pf), a3 = (float)(Ai[4] * g.pf);
+        const float a4 = (float)(Ai[6] * g.pf), a5 = (float)(Ai[7] * g.pf);
+        const float2* vol = vols[c];
+        const int pad = pads[c];
+        for (int p = threadIdx.x; p < Kc / 2; p += blockDim.x) {
+            float zr = 0.f, zi = 0.f;
+            if (p < npix) {
+                float2 f = make_float2(0.f, 0.f);
+                if (inproj[p]) {
+                    const float x = (float)pix[p].x, y = (float)pix[p].y;
+                    float2 v;
+                    if (ref_sample(vol, pad, g.ref2d, a0 * x + a1 * y, a2 * x + a3 * y, a4 * x + a5 * y, &v)) f = v;
+                }
+                const size_t w = (size_t)b * Pw + win[p];
+                const float2 X = Fimg[w];
+                const float ctf = Fctf[w], m = minv[p];
+                const float s = m * ctf;
+                const float yr = s * X.x, yi = s * X.y;
+                zr = f.x * yr + f.y * yi;
+                zi = f.y * yr - f.x * yi;
+                n1 += 0.5 * (double)m * (double)ctf * ctf * ((double)f.x * f.x + (double)f.y * f.y);
+            }
+            if (SPLIT) {
+                zrow[2 * p] = zr;
+                zrow[2 * p + 1] = zi;
+                mx = fmaxf(mx, fmaxf(fabsf(zr), fabsf(zi)));
+            } else {
+                zf[j * Kc + 2 * p] = zr;
+                zf[j * Kc + 2 * p + 1] = zi;
+            }
+        }
+    } else if (!SPLIT) {
+        for (int k = threadIdx.x; k < Kc; k += blockDim.x) zf[j * Kc + k] = 0.f;
+    }
+    for (int o = 32; o > 0; o >>= 1) n1 += __shfl_xor(n1, o);
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if ((threadIdx.x & 63) == 0) { red[threadIdx.x >> 6] = n1; redf[threadIdx.x >> 6] = mx; }
+    __syncthreads();
+    if (valid && threadIdx.x == 0) n1out[j] = red[0] + red[1] + red[2] + red[3];
+    if (SPLIT) {
+        // rows beyond nrows are zero (their LDS row is never written: write zeros directly)
+        mx = fmaxf(fmaxf(redf[0], redf[1]), fmaxf(redf[2], redf[3]));
+        int e = 0;
+        if (mx > 0.f) (void)frexpf(mx, &e);
+        const float sc = ldexpf(1.f, -e);
+        if (threadIdx.x == 0) zs[j] = ldexpf(1.f, e);
+        for (int k = threadIdx.x; k < Kc; k += blockDim.x) {
+            const float v = valid ? zrow[k] * sc : 0.f;
+            const H h = (H)v;
+            const H l = (H)(v - (float)h);
+            zh[j * Kc + k] = h;
+            zl[j * Kc + k] = l;
+        }
+    }
+}
+
+// The phase table as the GEMM's shared operand: row t = [re, im] of AB_t over the coarse pixels,
+// zero beyond 2 P_c and for rows T .. Tpad-1.
+__global__ void k_ab_rows(const float2* __restrict__ ab, int T, int npix, int Kc, float* __restrict__ out)
+{
+    const int t = blockIdx.x;
+    for (int p = threadIdx.x; p < Kc / 2; p += blockDim.x) {
+        float2 v = make_float2(0.f, 0.f);
+        if (t < T && p < npix) v = ab[(size_t)t * npix + p];
+        out[(size_t)t * Kc + 2 * p] = v.x;
+        out[(size_t)t * Kc + 2 * p + 1] = v.y;
+    }
+}
+
+// diff2 = C_coarse + n1 - G for every (row, translation); per-particle minimum.
+__global__ __launch_bounds__(128) void k_local_diff2(int B, long long nrows, const long long* __restrict__ orient_off,
+                                                     const PartScalars* __restrict__ ps,
+                                                     const double* __restrict__ n1, const float* __restrict__ G,
+                                                     int ldg, int T, float* __restrict__ d2,
+                                                     unsigned int* __restrict__ dmin)
+{
+    __shared__ float wm[2];
+    const long long j = blockIdx.x;
+    if (j >= nrows) return;
+    const int b = find_particle(orient_off, B, j);
+    const double base = ps[b].C_coarse + n1[j];
+    float m = 3.0e38f;
+    for (int t = threadIdx.x; t < T; t += blockDim.x) {
+        const float d = (float)(base - (double)G[j * ldg + t]);
+        d2[j * T + t] = d;
+        m = fminf(m, d);
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fminf(m, __shfl_xor(m, o));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x >> 6] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const float mm = (blockDim.x > 64) ? fminf(wm[0], wm[1]) : wm[0];
+        if (mm < 3.0e38f) atomicMin(&dmin[b], float_to_ordered(mm));
     }
 }
 
@@ -850,7 +986,8 @@ __device__ void block_argmin(double& v, int& i, double* sv, int* si)
 // list [B][Nd + Npsi] scratch (compacted selected indices).
 __global__ __launch_bounds__(256) void k_local_prior(const double* __restrict__ pang, int B, LocalSelParams q,
                                                      double* __restrict__ dir, double* __restrict__ psi,
-                                                     double* __restrict__ mean, int* __restrict__ list)
+                                                     double* __restrict__ mean, int* __restrict__ list,
+                                                     int* __restrict__ nsel)
 {
 #pragma clang fp contract(off)
     __shared__ double sv[256];
@@ -919,6 +1056,35 @@ __global__ __launch_bounds__(256) void k_local_prior(const double* __restrict__ 
                 for (int j = 0; j < np; j++) m += x * pb[pl[j]];
             }
         mean[b] = m / (double)((long long)q.K * nd * np);
+        nsel[2 * b] = nd;
+        nsel[2 * b + 1] = np;
+    }
+}
+
+// The coarse sample list of a local search: for each particle, its orientations (classes with
+// non-zero probability x its directions x its psi angles, in the global order) each followed by
+// all T translations. orient_off: [B+1] offsets of the particles' orientations; pair k of
+// orientation j is (j * T + t).
+__global__ void k_local_pairs(int B, const int* __restrict__ list, int Nd, int Npsi, const int* __restrict__ nsel,
+                              const int* __restrict__ cls, int Kact, long long Norient, int T,
+                              const long long* __restrict__ orient_off, int* __restrict__ orients,
+                              long long* __restrict__ orient_pair_start, int2* __restrict__ pairs)
+{
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    const int* dl = list + (size_t)b * (Nd + Npsi);
+    const int* pl = dl + Nd;
+    const int nd = nsel[2 * b], np = nsel[2 * b + 1];
+    const long long per_c = (long long)nd * np, no = (long long)Kact * per_c, o0 = orient_off[b];
+    for (long long j = threadIdx.x; j < no; j += blockDim.x) {
+        const int ci = (int)(j / per_c);
+        const long long r = j - ci * per_c;
+        const int di = (int)(r / np), pi = (int)(r - (long long)di * np);
+        const int n = (int)(cls[ci] * Norient + (long long)dl[di] * Npsi + pl[pi]);
+        const long long p0 = (o0 + j) * T;
+        orients[o0 + j] = n;
+        orient_pair_start[o0 + j] = p0;
+        for (int t = 0; t < T; t++) pairs[p0 + t] = make_int2(n, t);
     }
 }
 
@@ -1187,6 +1353,26 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     std::vector<float2> abf = phase_table(tf, S.fine_tx_px, S.fine_ty_px, N);
     T2X_TRY(upload(d->ab_coarse, abc, st));
     T2X_TRY(upload(d->ab_fine, abf, st));
+    // local searches: the phase table as the coarse GEMM's shared operand (Tl rows, Kc columns)
+    const bool local_rows = cfg->orientational_prior_mode == T2X_PRIOR_ROTTILT_PSI && !getenv("T2X_LOCAL_DENSE");
+    const int Tl = (int)round_up(T, 128);
+    if (local_rows) {
+        const int Kcl = tc.npix_pad * 2;
+        if (sizeof(float) * Kcl > 65536) {
+            set_error("coarse image too large for the local-search operand kernel (2 P_c > 16384)");
+            return T2X_EINVAL;
+        }
+        T2X_TRY(d->lc_ab.ensure(sizeof(float) * Tl * Kcl));
+        k_ab_rows<<<Tl, 256, 0, st>>>(d->ab_coarse.as<float2>(), (int)T, tc.npix, Kcl, d->lc_ab.as<float>());
+        T2X_HIP(hipGetLastError());
+        if (cfg->precision != T2X_PREC_FP32) {
+            T2X_TRY(d->lc_abh.ensure(sizeof(uint16_t) * Tl * Kcl));
+            T2X_TRY(d->lc_abl.ensure(sizeof(uint16_t) * Tl * Kcl));
+            T2X_TRY(d->lc_abs.ensure(sizeof(float) * Tl));
+            T2X_TRY(launch_split(cfg->precision, d->lc_ab.as<float>(), Kcl, Kcl, Tl, (int)T, d->lc_abh.ptr,
+                                 d->lc_abl.ptr, Kcl, d->lc_abs.as<float>(), st));
+        }
+    }
     // orientation prior (NOPRIOR): pdf_direction / mean over all classes and orientations;
     // 0 where the class or the direction has zero probability (never scored). Local searches:
     // the class gate (1, or 0 for a class of zero probability); the per-particle direction and
@@ -1226,6 +1412,28 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     T2X_TRY(upload(d_ctx, S.tx, st));
     T2X_TRY(upload(d_cty, S.ty, st));
     LocalSelParams lsp{};
+    // local searches score each particle's own sample list (T2X_LOCAL_DENSE=1: the dense pass over
+    // every orientation with out-of-range ones masked, kept for comparison)
+    const bool local_pairs = local && !getenv("T2X_LOCAL_DENSE");
+    int Kact = 0;
+    if (local_pairs) {
+        std::vector<int> cls;
+        for (int c = 0; c < K; c++) if (model->pdf_class[c] > 0.) cls.push_back(c);
+        Kact = (int)cls.size();
+        T2X_TRY(upload(d->lc_cls, cls, st));
+        std::vector<double> dsc(4 * Nd), psc(2 * Npsi);
+        for (long dd = 0; dd < Nd; dd++) {
+            const double a = S.rot[dd] * kPi / 180, b = S.tilt[dd] * kPi / 180;
+            dsc[4 * dd] = std::sin(a); dsc[4 * dd + 1] = std::cos(a);
+            dsc[4 * dd + 2] = std::sin(b); dsc[4 * dd + 3] = std::cos(b);
+        }
+        for (long p = 0; p < Npsi; p++) {
+            const double g = S.psi[p] * kPi / 180;
+            psc[2 * p] = std::sin(g); psc[2 * p + 1] = std::cos(g);
+        }
+        T2X_TRY(upload(d->lc_dirsc, dsc, st));
+        T2X_TRY(upload(d->lc_psisc, psc, st));
+    }
     if (local) {
         std::vector<double> tab(2 * Nd + Npsi);
         for (long dd = 0; dd < Nd; dd++) { tab[dd] = S.rot[dd]; tab[Nd + dd] = S.tilt[dd]; }
@@ -1252,7 +1460,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     for (int c = 0; c < K; c++) key += "/" + std::to_string(d->classes[c].proj_version);
     EventTimer et{d, true};
     (void)hipEventRecord(d->ev[0], st);
-    if (key != d->coarse_key) {
+    if (!local_pairs && key != d->coarse_key) {
         T2X_TRY(d->coarse_B.ensure(sizeof(float) * O_pad * Kc));
         T2X_TRY(d->coarse_B2.ensure(sizeof(float) * O_pad * Kn));
         T2X_HIP(hipMemsetAsync(d->coarse_B.ptr, 0, sizeof(float) * O_pad * Kc, st));
@@ -1310,9 +1518,23 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     double* w_poff = w_off + 1;   // [2K] per-class prior offsets, 2D references
 
     // ---- batch size: coarse diff2 dominates memory (B*T*O_pad floats) ----
-    const size_t per_particle = (size_t)T * O_pad * 4 + (size_t)T * Kc * (prec == T2X_PREC_FP32 ? 4 : 8) +
-                                (size_t)4 * N * N * 4 +
-                                (size_t)2 * N * xN * 8 * 2;
+    size_t per_particle = (size_t)T * O_pad * 4 + (size_t)T * Kc * (prec == T2X_PREC_FP32 ? 4 : 8) +
+                          (size_t)4 * N * N * 4 + (size_t)2 * N * xN * 8 * 2;
+    if (local_pairs) {
+        // expected sample list: directions within 3 sigma (+ one step) of the prior, psi angles
+        // within 3 sigma_psi (+ one step) on either side; 16 bytes per sample (pair, diff2)
+        const double step_d = ref2d ? 0. : 360. / (6 << cfg->healpix_order), step_p = 360. / (double)Npsi;
+        const double rd = std::min(180., 3. * std::max(cfg->sigma_rot, cfg->sigma_tilt) + step_d);
+        const double nd_est = (ref2d || !(cfg->sigma_rot > 0.)) ? (double)Nd
+                              : std::min((double)Nd, Nd * (1. - std::cos(rd * kPi / 180.)) / 2. + 16.);
+        const double np_est = (cfg->sigma_psi > 0.) ? std::min((double)Npsi, Npsi * (6. * cfg->sigma_psi + 2. * step_p) / 360. + 2.)
+                                                    : (double)Npsi;
+        // per sample list entry 16 bytes; per (particle, orientation) row its Z row (4 bytes per K
+        // element), G row (Tl floats) and n1
+        const double rows_est = Kact * nd_est * np_est;
+        per_particle = (size_t)(16. * rows_est * T + rows_est * (4. * Kc + 4. * Tl + 8.)) + (size_t)4 * N * N * 4 +
+                       (size_t)2 * N * xN * 8 * 2;
+    }
     size_t budget = (size_t)24 << 30;   // 24 GiB of scratch for the batch
     {
         size_t fr = 0, tot = 0;
@@ -1403,9 +1625,10 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             T2X_TRY(d->loc_psi.ensure(sizeof(double) * B * Npsi));
             T2X_TRY(d->loc_mean.ensure(sizeof(double) * B));
             T2X_TRY(d->loc_list.ensure(sizeof(int) * B * (Nd + Npsi)));
+            T2X_TRY(d->loc_nsel.ensure(sizeof(int) * 2 * B));
             k_local_prior<<<B, 256, 0, st>>>(d->loc_ang.as<double>(), B, lsp, d->loc_dir.as<double>(),
                                             d->loc_psi.as<double>(), d->loc_mean.as<double>(),
-                                            d->loc_list.as<int>());
+                                            d->loc_list.as<int>(), d->loc_nsel.as<int>());
             T2X_HIP(hipGetLastError());
             lp.dir = d->loc_dir.as<double>();
             lp.psi = d->loc_psi.as<double>();
@@ -1442,258 +1665,375 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
 
         // ---- coarse pass ----
         const long long M = (long long)B * T;
-        const long long M_pad = round_up(M, 256);
-        const long long B_pad = round_up(B, prec == T2X_PREC_FP32 ? 128 : 256);   // N1 GEMM rows
-        if (prec == T2X_PREC_FP32) T2X_TRY(d->A.ensure(sizeof(float) * M_pad * Kc));
-        T2X_TRY(d->A1.ensure(sizeof(float) * B_pad * Kn));
-        T2X_TRY(d->N1.ensure(sizeof(float) * B_pad * O_pad));
-        T2X_TRY(d->diff2.ensure(sizeof(float) * M_pad * O_pad));
-        const int rowmin_ld = (int)(O_pad / 64);
-        DevBuf& d_rowmin = d->minb;
-        T2X_TRY(d_rowmin.ensure(sizeof(float) * M_pad * rowmin_ld));
-        T2X_HIP(hipMemsetAsync(d->A1.ptr, 0, sizeof(float) * B_pad * Kn, st));
-        if (prec == T2X_PREC_FP32) {
-            T2X_HIP(hipMemsetAsync(d->A.ptr, 0, sizeof(float) * M_pad * Kc, st));
-            dim3 grid((unsigned)T, (unsigned)B);
-            k_coarse_imgside<<<grid, 256, 0, st>>>(d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw,
-                                                   tc.d_win.as<int>(), tc.d_minv.as<float>(), tc.npix,
-                                                   d->ab_coarse.as<float2>(), (int)T, d->A.as<float>(), Kc,
-                                                   d->A1.as<float>(), Kn);
-            T2X_HIP(hipGetLastError());
-        } else {
-            // the image side straight into the split operand (hi, lo, per-row scale)
-            T2X_TRY(d->Ah.ensure(sizeof(uint16_t) * M_pad * Kc));
-            T2X_TRY(d->Al.ensure(sizeof(uint16_t) * M_pad * Kc));
-            T2X_TRY(d->As.ensure(sizeof(float) * M_pad));
-            const size_t lds = sizeof(float) * Kc;
-            if (lds > 65536) {
-                set_error("coarse image too large for the split operand kernel (2 P_c > 16384)");
-                return T2X_EINVAL;
-            }
-            if (prec == T2X_PREC_BF16X3)
-                k_coarse_imgside_split<__bf16><<<(unsigned)M_pad, 256, lds, st>>>(
-                    d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, tc.d_win.as<int>(),
-                    tc.d_minv.as<float>(), tc.npix, d->ab_coarse.as<float2>(), (int)T, (int)M, Kc,
-                    (__bf16*)d->Ah.ptr, (__bf16*)d->Al.ptr, d->As.as<float>(), 0, d->A1.as<float>(), Kn);
-            else
-                k_coarse_imgside_split<_Float16><<<(unsigned)M_pad, 256, lds, st>>>(
-                    d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, tc.d_win.as<int>(),
-                    tc.d_minv.as<float>(), tc.npix, d->ab_coarse.as<float2>(), (int)T, (int)M, Kc,
-                    (_Float16*)d->Ah.ptr, (_Float16*)d->Al.ptr, d->As.as<float>(), 1, d->A1.as<float>(), Kn);
-            T2X_HIP(hipGetLastError());
-        }
-        {
-            EpiParams ep{};
-            ep.C = d->N1.as<float>();
-            ep.ldc = O_pad;
-            ep.M = B;
-            ep.N = (int)O_all;
-            if (prec == T2X_PREC_FP32) {
-                T2X_TRY(launch_gemm(EPI_STORE, d->A1.as<float>(), Kn, d->coarse_B2.as<float>(), Kn, Kn,
-                                    (int)B_pad, (int)O_pad, ep, st));
-            } else {   // N1 = sum_p (minv ctf^2 / 2) |F|^2 in the same split precision as the cross term
-                T2X_TRY(d->A1h.ensure(sizeof(uint16_t) * B_pad * Kn));
-                T2X_TRY(d->A1l.ensure(sizeof(uint16_t) * B_pad * Kn));
-                T2X_TRY(d->A1s.ensure(sizeof(float) * B_pad));
-                T2X_TRY(launch_split(prec, d->A1.as<float>(), Kn, Kn, (int)B_pad, (int)B, d->A1h.ptr, d->A1l.ptr, Kn,
-                                     d->A1s.as<float>(), st));
-                T2X_TRY(launch_gemm_x3(prec, EPI_STORE, d->A1h.ptr, d->A1l.ptr, d->A1s.as<float>(), Kn, B_pad,
-                                       d->coarse_B2h.ptr, d->coarse_B2l.ptr, d->coarse_B2s.as<float>(), Kn, Kn,
-                                       (int)B_pad, (int)O_pad, ep, st,
-                                       GemmChoice{cfg->gemm_tile, cfg->gemm_group}, nullptr, nullptr));
-            }
-            if (local) {
-                const long long ne = (long long)B * O_all;
-                k_local_mask_n1<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(d->N1.as<float>(), O_pad, B, O_all, lp);
-                T2X_HIP(hipGetLastError());
-            }
-        }
-        // ---- learned proposal: bins per particle, work lists of the restricted coarse GEMM ----
-        int n_work = 0, n_fb_rows = 0;
-        if (use_prop) {
-            const auto tp0 = std::chrono::steady_clock::now();
-            std::vector<int> pbins, pfb;
-            std::vector<float> pmax, pmass;
-            T2X_TRY(cnn::proposal_infer(d, d->Fimg_cur.as<float2>(), cs, B, topk, cfg->proposal_min_conf,
-                                        cfg->proposal_gate, pbins, pmax, pmass, pfb));
-            d->prop_gate_top1.insert(d->prop_gate_top1.end(), pmax.begin(), pmax.end());
-            d->prop_gate_mass.insert(d->prop_gate_mass.end(), pmass.begin(), pmass.end());
-            const int mw = (n_bins + 31) / 32;
-            std::vector<unsigned> mask((size_t)B * mw, 0u);
-            std::vector<std::vector<int>> members(n_bins);
-            std::vector<int> fb_rows;   // exhaustive particles: all their rows through the dense kernel
-            long long scored = 0;
-            for (int i = 0; i < B; i++) {
-                if (pfb[i]) {
-                    for (int j = 0; j < mw; j++) mask[(size_t)i * mw + j] = ~0u;
-                    for (long t = 0; t < T; t++) fb_rows.push_back((int)(i * T + t));
-                    scored += n_bins;
-                    d->stats.s.n_proposal_fallback++;
-                } else {
-                    for (int k = 0; k < topk; k++) {
-                        const int j = pbins[(size_t)i * topk + k];
-                        mask[(size_t)i * mw + (j >> 5)] |= 1u << (j & 31);
-                        members[j].push_back(i);
-                    }
-                    scored += topk;
-                }
-            }
-            std::vector<int> plist;
-            std::vector<BinWorkHost> work;
-            for (int j = 0; j < n_bins; j++) {
-                if (members[j].empty()) continue;
-                const int pstart = (int)plist.size(), cnt = (int)members[j].size();
-                plist.insert(plist.end(), members[j].begin(), members[j].end());
-                const long long rows = (long long)cnt * T;
-                for (int rt = 0; (long long)rt * 128 < rows; rt++)
-                    for (int c0 = 0; c0 < bin_cols; c0 += 128)
-                        work.push_back(BinWorkHost{pstart, cnt, rt, j * bin_cols + c0, std::min(128, bin_cols - c0)});
-            }
-            T2X_TRY(upload(d->prop_mask, mask, st));
-            T2X_TRY(upload(d->prop_plist, plist, st));
-            T2X_TRY(upload(d->prop_work, work, st));
-            T2X_TRY(upload(d->prop_rows, fb_rows, st));
-            n_work = (int)work.size();
-            n_fb_rows = (int)fb_rows.size();
-            d->stats.s.n_proposal_particles += B;
-            d->stats.s.proposal_bins_scored += (double)scored;
-            d->stats.s.ms_proposal +=
-                std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp0).count();
-        }
-        (void)hipEventRecord(d->ev[ev++], st);   // 2: coarse gemm start
-        {
-            EpiParams ep{};
-            ep.C = d->diff2.as<float>();
-            ep.ldc = O_pad;
-            ep.M = (int)M;
-            ep.N = (int)O_all;
-            ep.T = (int)T;
-            // C_b is read from the PartScalars array: pass a strided view via a packed copy
-            ep.Cb = nullptr;
-            DevBuf& d_cb = d->tables;
-            std::vector<double> dummy;   // filled on device below
-            T2X_TRY(d_cb.ensure(sizeof(double) * B));
-            // copy C_coarse out of the scalars (device to device, strided)
-            T2X_HIP(hipMemcpy2DAsync(d_cb.ptr, sizeof(double), (char*)d_ps + offsetof(PartScalars, C_coarse),
-                                     sizeof(PartScalars), sizeof(double), B, hipMemcpyDeviceToDevice, st));
-            ep.Cb = d_cb.as<double>();
-            ep.N1 = d->N1.as<float>();
-            ep.ldn = O_pad;
-            ep.prior_o = d->prior_o.as<double>();
-            ep.rowmin = d_rowmin.as<float>();
-            ep.rowmin_ld = rowmin_ld;
-            // running per-particle minima for the lazy stores of negligible blocks (all values are
-            // stored when diagnostic copies of the coarse diff2 are requested)
-            T2X_TRY(d->pmin.ensure(sizeof(unsigned) * B));
-            k_fill_u32<<<(unsigned)((B + 255) / 256), 256, 0, st>>>(d->pmin.as<unsigned>(), 0xFF800000u, B);  // +inf
-            ep.pmin = (dbg && dbg->n_debug > 0) ? nullptr : d->pmin.as<unsigned>();
-            // rows beyond the valid columns of the last tile keep +inf
-            k_fill_f32<<<(unsigned)((M_pad * rowmin_ld + 255) / 256), 256, 0, st>>>(d_rowmin.as<float>(), 3.0e38f,
-                                                                                    M_pad * rowmin_ld);
-            if (prec == T2X_PREC_FP32) {
-                T2X_TRY(launch_gemm(EPI_DIFF2, d->A.as<float>(), Kc, d->coarse_B.as<float>(), Kc, Kc,
-                                    (int)M_pad, (int)O_pad, ep, st));
-            } else {
-                const GemmChoice gwant{cfg->gemm_tile, cfg->gemm_group};
-                GemmChoice gused{0, 0};
-                if (use_prop) {
-                    if (n_fb_rows > 0) {
-                        EpiParams ef = ep;
-                        ef.M = n_fb_rows;
-                        T2X_TRY(launch_gemm_x3(prec, EPI_DIFF2, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc,
-                                               M_pad, d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(),
-                                               Kc, Kc, (int)round_up(n_fb_rows, 256), (int)O_pad, ef, st, gwant,
-                                               d->prop_rows.as<int>(), &gused));
-                    }
-                    T2X_TRY(launch_gemm_x3_bins(prec, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc, d->coarse_Bh.ptr,
-                                                d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc,
-                                                d->prop_plist.as<int>(), d->prop_work.ptr, n_work, (int)T,
-                                                O_pad - 1, ep, st));
-                } else
-                    T2X_TRY(launch_gemm_x3(prec, EPI_DIFF2, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc,
-                                           M_pad, d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc,
-                                           Kc, (int)M_pad, (int)O_pad, ep, st, gwant, nullptr, &gused));
-                if (gused.tile) {
-                    d->stats.s.gemm_tile_used = gused.tile;
-                    d->stats.s.gemm_group_used = gused.group;
-                }
-            }
-        }
-        (void)hipEventRecord(d->ev[ev++], st);   // 3: coarse gemm end
-        // ---- coarse weights and significance ----
         T2X_TRY(d->sel_out.ensure(sizeof(SelResult) * 2 * B));
         SelResult* d_selc = d->sel_out.as<SelResult>();
         SelResult* d_self = d_selc + B;
-        WeightSrc wc{};
-        wc.kind = SRC_COARSE;
-        if (use_prop) {
-            wc.bin_mask = d->prop_mask.as<unsigned>();
-            wc.mask_words = (n_bins + 31) / 32;
-            wc.bin_cols = bin_cols;
-        }
-        wc.d2 = d->diff2.as<float>();
-        wc.ld = O_pad;
-        wc.T = (int)T;
-        wc.O_all = (int)O_all;
-        wc.prior_o = d->prior_o.as<double>();
-        wc.log_prior_o = d->log_prior_o.as<double>();
-        wc.prior_t = d->prior_t.as<double>();
-        wc.rowmin = d_rowmin.as<float>();
-        wc.rowmin_ld = rowmin_ld;
-        wc.nover = nover;
-        wc.over_trans = over_trans;
-        wc.lp = lp;
-        k_pass_min<<<B, 1024, 0, st>>>(wc, B, d_selc);
-        SelParams sp{cfg->adaptive_fraction, (long long)cfg->maximum_significants, 1};
-        // candidate path: sum pass, compaction of the weights that can be significant, exact
-        // sorted walk per particle; the radix-select path handles batches that overflow
         T2X_TRY(d->sig_count.ensure(sizeof(long long) * 2 * B));
-        T2X_TRY(d->cand_w.ensure(sizeof(double) * B * kCandCap));
-        T2X_TRY(d->cand_e.ensure(sizeof(int) * B * kCandCap));
-        T2X_TRY(d->cand_n.ensure(sizeof(int) * 2 * B));
-        T2X_TRY(d->pairs_strided.ensure(sizeof(int2) * B * kCandCap));
-        int* d_cand_count = d->cand_n.as<int>();
-        int* d_overflow = d_cand_count + B;
-        T2X_HIP(hipMemsetAsync(d_cand_count, 0, sizeof(int) * 2 * B, st));   // counts and overflow flags
-        T2X_TRY(d->sum_part.ensure(sum_part_bytes() * B * T));
         std::vector<long long> counts(2 * B);
         std::vector<SelResult> hselc(B);
         bool fallback = true;
-        // local searches take the radix-select path directly (the candidate filter works from the
-        // global log prior)
-        if (!local) {
-            {
-                const dim3 grid((unsigned)T, (unsigned)B);
-                k_coarse_sum_rows<<<grid, 256, 0, st>>>(wc, B, d_selc, d->sum_part.as<SumPart>());
-                k_coarse_sum_final<<<B, 64, 0, st>>>(wc, B, d->sum_part.as<SumPart>(), d_selc);
-                k_coarse_candidates<<<grid, 256, 0, st>>>(wc, B, sp, d_selc, d->cand_w.as<double>(),
-                                                          d->cand_e.as<int>(), d_cand_count);
-            }
-            k_cand_select<<<B, 1024, 0, st>>>(wc, B, sp, d_selc, d->cand_w.as<double>(), d->cand_e.as<int>(),
-                                              d_cand_count, d->pairs_strided.as<int2>(),
-                                              d->sig_count.as<long long>(), d_overflow);
-            T2X_HIP(hipGetLastError());
-            // overflow flags, counts and results in one synchronisation; the counts and results are
-            // read again only when a particle overflowed
-            std::vector<int> hover(B);
-            T2X_HIP(hipMemcpyAsync(hover.data(), d_overflow, sizeof(int) * B, hipMemcpyDeviceToHost, st));
-            T2X_HIP(hipMemcpyAsync(counts.data(), d->sig_count.ptr, sizeof(long long) * 2 * B, hipMemcpyDeviceToHost, st));
-            T2X_HIP(hipMemcpyAsync(hselc.data(), d_selc, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
+        WeightSrc wc{};
+        long long npairs_c = 0;
+        if (local_pairs) {
+            // local search on per-particle sample lists: each particle's orientations in range x all
+            // translations, scored at coarse size by the fine-pass kernel (no dense pass over every
+            // orientation); weights, significance and the significant pairs as in the dense pass
+            std::vector<int> hnsel(2 * B);
+            T2X_HIP(hipMemcpyAsync(hnsel.data(), d->loc_nsel.ptr, sizeof(int) * 2 * B, hipMemcpyDeviceToHost, st));
             T2X_HIP(hipStreamSynchronize(st));
-            fallback = false;
-            for (int i = 0; i < B; i++) fallback |= hover[i] != 0;
-            if (getenv("T2X_FORCE_RADIX_SELECT")) fallback = true;   // exercises the fallback in tests
-        }
-        if (fallback) {
+            std::vector<long long> coff(B + 1, 0), cpoff(B + 1, 0);
+            for (int i = 0; i < B; i++) coff[i + 1] = coff[i] + (long long)Kact * hnsel[2 * i] * hnsel[2 * i + 1];
+            for (int i = 0; i <= B; i++) cpoff[i] = coff[i] * T;
+            const long long norients_c = coff[B];
+            npairs_c = cpoff[B];
+            if (norients_c > 0x7fffffffLL) {
+                set_error("local search: more than 2^31 orientations in one batch");
+                return T2X_EINVAL;
+            }
+            T2X_TRY(upload(d->lc_orient_off, coff, st));
+            T2X_TRY(upload(d->lc_pair_off, cpoff, st));
+            T2X_TRY(d->lc_orients.ensure(sizeof(int) * std::max<long long>(norients_c, 1)));
+            T2X_TRY(d->lc_ops.ensure(sizeof(long long) * std::max<long long>(norients_c, 1)));
+            T2X_TRY(d->lc_pairs.ensure(sizeof(int2) * std::max<long long>(npairs_c, 1)));
+            T2X_TRY(d->lc_d2.ensure(sizeof(float) * std::max<long long>(npairs_c, 1)));
+            T2X_TRY(d->lc_min.ensure(sizeof(unsigned) * B));
+            k_local_pairs<<<B, 256, 0, st>>>(B, d->loc_list.as<int>(), (int)Nd, (int)Npsi, d->loc_nsel.as<int>(),
+                                            d->lc_cls.as<int>(), Kact, (long long)Norient, (int)T,
+                                            d->lc_orient_off.as<long long>(), d->lc_orients.as<int>(),
+                                            d->lc_ops.as<long long>(), d->lc_pairs.as<int2>());
+            k_fill_u32<<<(B + 255) / 256, 256, 0, st>>>(d->lc_min.as<unsigned int>(), 0xffffffffu, B);
+            T2X_HIP(hipGetLastError());
+            FineGeom gc{};
+            gc.Norient = (int)Norient;
+            gc.Npsi = (int)Npsi;
+            gc.n_subdir = 1;
+            gc.n_subpsi = 1;
+            gc.over_rot = 1;
+            gc.over_trans = 1;
+            gc.T = (int)T;
+            gc.dir_sc = d->lc_dirsc.as<double>();
+            gc.psi_sc = d->lc_psisc.as<double>();
+            gc.pf = pf;
+            gc.ref2d = ref2d;
+            (void)hipEventRecord(d->ev[ev++], st);   // 2: coarse diff2 start
+            if (norients_c > 0) {
+                // rows = the batch's (particle, orientation) pairs, columns = translations
+                const long long rows = norients_c, rows_pad = round_up(rows, 128);
+                EpiParams eg{};
+                T2X_TRY(d->lc_n1.ensure(sizeof(double) * rows));
+                T2X_TRY(d->lc_G.ensure(sizeof(float) * rows_pad * Tl));
+                eg.C = d->lc_G.as<float>();
+                eg.ldc = Tl;
+                eg.M = (int)rows;
+                eg.N = (int)T;
+                if (prec == T2X_PREC_FP32) {
+                    T2X_TRY(d->lc_z.ensure(sizeof(float) * rows_pad * Kc));
+                    k_local_zrows<float, 0><<<(unsigned)rows_pad, 256, 0, st>>>(
+                        B, rows, d->lc_orient_off.as<long long>(), d->lc_orients.as<int>(), gc,
+                        d_vols.as<const float2*>(), d_pads.as<int>(), tc.d_pix.as<int2>(),
+                        tc.d_inproj.as<unsigned char>(), tc.d_minv.as<float>(), tc.d_win.as<int>(), tc.npix,
+                        d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, Kc, d->lc_z.as<float>(), nullptr,
+                        nullptr, nullptr, d->lc_n1.as<double>());
+                    T2X_HIP(hipGetLastError());
+                    T2X_TRY(launch_gemm(EPI_STORE, d->lc_z.as<float>(), Kc, d->lc_ab.as<float>(), Kc, Kc,
+                                        (int)rows_pad, Tl, eg, st));
+                } else {
+                    T2X_TRY(d->lc_zh.ensure(sizeof(uint16_t) * rows_pad * Kc));
+                    T2X_TRY(d->lc_zl.ensure(sizeof(uint16_t) * rows_pad * Kc));
+                    T2X_TRY(d->lc_zs.ensure(sizeof(float) * rows_pad));
+                    const size_t lds = sizeof(float) * Kc;
+                    if (prec == T2X_PREC_BF16X3)
+                        k_local_zrows<__bf16, 1><<<(unsigned)rows_pad, 256, lds, st>>>(
+                            B, rows, d->lc_orient_off.as<long long>(), d->lc_orients.as<int>(), gc,
+                            d_vols.as<const float2*>(), d_pads.as<int>(), tc.d_pix.as<int2>(),
+                            tc.d_inproj.as<unsigned char>(), tc.d_minv.as<float>(), tc.d_win.as<int>(), tc.npix,
+                            d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, Kc, nullptr,
+                            (__bf16*)d->lc_zh.ptr, (__bf16*)d->lc_zl.ptr, d->lc_zs.as<float>(), d->lc_n1.as<double>());
+                    else
+                        k_local_zrows<_Float16, 1><<<(unsigned)rows_pad, 256, lds, st>>>(
+                            B, rows, d->lc_orient_off.as<long long>(), d->lc_orients.as<int>(), gc,
+                            d_vols.as<const float2*>(), d_pads.as<int>(), tc.d_pix.as<int2>(),
+                            tc.d_inproj.as<unsigned char>(), tc.d_minv.as<float>(), tc.d_win.as<int>(), tc.npix,
+                            d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, Kc, nullptr,
+                            (_Float16*)d->lc_zh.ptr, (_Float16*)d->lc_zl.ptr, d->lc_zs.as<float>(), d->lc_n1.as<double>());
+                    T2X_HIP(hipGetLastError());
+                    T2X_TRY(launch_gemm_x3(prec, EPI_STORE, d->lc_zh.ptr, d->lc_zl.ptr, d->lc_zs.as<float>(), Kc, rows_pad,
+                                           d->lc_abh.ptr, d->lc_abl.ptr, d->lc_abs.as<float>(), Kc, Kc, (int)rows_pad, Tl,
+                                           eg, st, GemmChoice{128, 64}, nullptr, nullptr));
+                }
+                k_local_diff2<<<(unsigned)rows, 128, 0, st>>>(B, rows, d->lc_orient_off.as<long long>(), d_ps,
+                                                             d->lc_n1.as<double>(), d->lc_G.as<float>(), Tl, (int)T,
+                                                             d->lc_d2.as<float>(), d->lc_min.as<unsigned int>());
+                T2X_HIP(hipGetLastError());
+            }
+            (void)hipEventRecord(d->ev[ev++], st);   // 3: coarse diff2 end
+            wc.kind = SRC_FINE;
+            wc.d2 = d->lc_d2.as<float>();
+            wc.pair_off = d->lc_pair_off.as<long long>();
+            wc.pairs = d->lc_pairs.as<int2>();
+            wc.fine_min = d->lc_min.as<unsigned int>();
+            wc.nover = 1;
+            wc.over_trans = 1;
+            wc.T = (int)T;
+            wc.O_all = (int)O_all;
+            wc.prior_o = d->prior_o.as<double>();
+            wc.prior_t = d->prior_t.as<double>();
+            wc.lp = lp;
+            k_pass_min<<<B, 1024, 0, st>>>(wc, B, d_selc);
+            SelParams sp{cfg->adaptive_fraction, (long long)cfg->maximum_significants, 1};
             k_select<<<B, 1024, 0, st>>>(wc, sp, B, d_selc);
             k_sig_count<<<B, 1024, 0, st>>>(wc, B, d_selc, d->sig_count.as<long long>());
             T2X_HIP(hipGetLastError());
-            stt.n_select_fallbacks += 1;
             T2X_HIP(hipMemcpyAsync(counts.data(), d->sig_count.ptr, sizeof(long long) * 2 * B, hipMemcpyDeviceToHost, st));
             T2X_HIP(hipMemcpyAsync(hselc.data(), d_selc, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
             T2X_HIP(hipStreamSynchronize(st));
+            (void)hipEventRecord(d->ev[ev++], st);   // 4: coarse weights done
+        } else {
+            const long long M_pad = round_up(M, 256);
+            const long long B_pad = round_up(B, prec == T2X_PREC_FP32 ? 128 : 256);   // N1 GEMM rows
+            if (prec == T2X_PREC_FP32) T2X_TRY(d->A.ensure(sizeof(float) * M_pad * Kc));
+            T2X_TRY(d->A1.ensure(sizeof(float) * B_pad * Kn));
+            T2X_TRY(d->N1.ensure(sizeof(float) * B_pad * O_pad));
+            T2X_TRY(d->diff2.ensure(sizeof(float) * M_pad * O_pad));
+            const int rowmin_ld = (int)(O_pad / 64);
+            DevBuf& d_rowmin = d->minb;
+            T2X_TRY(d_rowmin.ensure(sizeof(float) * M_pad * rowmin_ld));
+            T2X_HIP(hipMemsetAsync(d->A1.ptr, 0, sizeof(float) * B_pad * Kn, st));
+            if (prec == T2X_PREC_FP32) {
+                T2X_HIP(hipMemsetAsync(d->A.ptr, 0, sizeof(float) * M_pad * Kc, st));
+                dim3 grid((unsigned)T, (unsigned)B);
+                k_coarse_imgside<<<grid, 256, 0, st>>>(d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw,
+                                                       tc.d_win.as<int>(), tc.d_minv.as<float>(), tc.npix,
+                                                       d->ab_coarse.as<float2>(), (int)T, d->A.as<float>(), Kc,
+                                                       d->A1.as<float>(), Kn);
+                T2X_HIP(hipGetLastError());
+            } else {
+                // the image side straight into the split operand (hi, lo, per-row scale)
+                T2X_TRY(d->Ah.ensure(sizeof(uint16_t) * M_pad * Kc));
+                T2X_TRY(d->Al.ensure(sizeof(uint16_t) * M_pad * Kc));
+                T2X_TRY(d->As.ensure(sizeof(float) * M_pad));
+                const size_t lds = sizeof(float) * Kc;
+                if (lds > 65536) {
+                    set_error("coarse image too large for the split operand kernel (2 P_c > 16384)");
+                    return T2X_EINVAL;
+                }
+                if (prec == T2X_PREC_BF16X3)
+                    k_coarse_imgside_split<__bf16><<<(unsigned)M_pad, 256, lds, st>>>(
+                        d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, tc.d_win.as<int>(),
+                        tc.d_minv.as<float>(), tc.npix, d->ab_coarse.as<float2>(), (int)T, (int)M, Kc,
+                        (__bf16*)d->Ah.ptr, (__bf16*)d->Al.ptr, d->As.as<float>(), 0, d->A1.as<float>(), Kn);
+                else
+                    k_coarse_imgside_split<_Float16><<<(unsigned)M_pad, 256, lds, st>>>(
+                        d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, tc.d_win.as<int>(),
+                        tc.d_minv.as<float>(), tc.npix, d->ab_coarse.as<float2>(), (int)T, (int)M, Kc,
+                        (_Float16*)d->Ah.ptr, (_Float16*)d->Al.ptr, d->As.as<float>(), 1, d->A1.as<float>(), Kn);
+                T2X_HIP(hipGetLastError());
+            }
+            {
+                EpiParams ep{};
+                ep.C = d->N1.as<float>();
+                ep.ldc = O_pad;
+                ep.M = B;
+                ep.N = (int)O_all;
+                if (prec == T2X_PREC_FP32) {
+                    T2X_TRY(launch_gemm(EPI_STORE, d->A1.as<float>(), Kn, d->coarse_B2.as<float>(), Kn, Kn,
+                                        (int)B_pad, (int)O_pad, ep, st));
+                } else {   // N1 = sum_p (minv ctf^2 / 2) |F|^2 in the same split precision as the cross term
+                    T2X_TRY(d->A1h.ensure(sizeof(uint16_t) * B_pad * Kn));
+                    T2X_TRY(d->A1l.ensure(sizeof(uint16_t) * B_pad * Kn));
+                    T2X_TRY(d->A1s.ensure(sizeof(float) * B_pad));
+                    T2X_TRY(launch_split(prec, d->A1.as<float>(), Kn, Kn, (int)B_pad, (int)B, d->A1h.ptr, d->A1l.ptr, Kn,
+                                         d->A1s.as<float>(), st));
+                    T2X_TRY(launch_gemm_x3(prec, EPI_STORE, d->A1h.ptr, d->A1l.ptr, d->A1s.as<float>(), Kn, B_pad,
+                                           d->coarse_B2h.ptr, d->coarse_B2l.ptr, d->coarse_B2s.as<float>(), Kn, Kn,
+                                           (int)B_pad, (int)O_pad, ep, st,
+                                           GemmChoice{cfg->gemm_tile, cfg->gemm_group}, nullptr, nullptr));
+                }
+                if (local) {
+                    const long long ne = (long long)B * O_all;
+                    k_local_mask_n1<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(d->N1.as<float>(), O_pad, B, O_all, lp);
+                    T2X_HIP(hipGetLastError());
+                }
+            }
+            // ---- learned proposal: bins per particle, work lists of the restricted coarse GEMM ----
+            int n_work = 0, n_fb_rows = 0;
+            if (use_prop) {
+                const auto tp0 = std::chrono::steady_clock::now();
+                std::vector<int> pbins, pfb;
+                std::vector<float> pmax, pmass;
+                T2X_TRY(cnn::proposal_infer(d, d->Fimg_cur.as<float2>(), cs, B, topk, cfg->proposal_min_conf,
+                                            cfg->proposal_gate, pbins, pmax, pmass, pfb));
+                d->prop_gate_top1.insert(d->prop_gate_top1.end(), pmax.begin(), pmax.end());
+                d->prop_gate_mass.insert(d->prop_gate_mass.end(), pmass.begin(), pmass.end());
+                const int mw = (n_bins + 31) / 32;
+                std::vector<unsigned> mask((size_t)B * mw, 0u);
+                std::vector<std::vector<int>> members(n_bins);
+                std::vector<int> fb_rows;   // exhaustive particles: all their rows through the dense kernel
+                long long scored = 0;
+                for (int i = 0; i < B; i++) {
+                    if (pfb[i]) {
+                        for (int j = 0; j < mw; j++) mask[(size_t)i * mw + j] = ~0u;
+                        for (long t = 0; t < T; t++) fb_rows.push_back((int)(i * T + t));
+                        scored += n_bins;
+                        d->stats.s.n_proposal_fallback++;
+                    } else {
+                        for (int k = 0; k < topk; k++) {
+                            const int j = pbins[(size_t)i * topk + k];
+                            mask[(size_t)i * mw + (j >> 5)] |= 1u << (j & 31);
+                            members[j].push_back(i);
+                        }
+                        scored += topk;
+                    }
+                }
+                std::vector<int> plist;
+                std::vector<BinWorkHost> work;
+                for (int j = 0; j < n_bins; j++) {
+                    if (members[j].empty()) continue;
+                    const int pstart = (int)plist.size(), cnt = (int)members[j].size();
+                    plist.insert(plist.end(), members[j].begin(), members[j].end());
+                    const long long rows = (long long)cnt * T;
+                    for (int rt = 0; (long long)rt * 128 < rows; rt++)
+                        for (int c0 = 0; c0 < bin_cols; c0 += 128)
+                            work.push_back(BinWorkHost{pstart, cnt, rt, j * bin_cols + c0, std::min(128, bin_cols - c0)});
+                }
+                T2X_TRY(upload(d->prop_mask, mask, st));
+                T2X_TRY(upload(d->prop_plist, plist, st));
+                T2X_TRY(upload(d->prop_work, work, st));
+                T2X_TRY(upload(d->prop_rows, fb_rows, st));
+                n_work = (int)work.size();
+                n_fb_rows = (int)fb_rows.size();
+                d->stats.s.n_proposal_particles += B;
+                d->stats.s.proposal_bins_scored += (double)scored;
+                d->stats.s.ms_proposal +=
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp0).count();
+            }
+            (void)hipEventRecord(d->ev[ev++], st);   // 2: coarse gemm start
+            {
+                EpiParams ep{};
+                ep.C = d->diff2.as<float>();
+                ep.ldc = O_pad;
+                ep.M = (int)M;
+                ep.N = (int)O_all;
+                ep.T = (int)T;
+                // C_b is read from the PartScalars array: pass a strided view via a packed copy
+                ep.Cb = nullptr;
+                DevBuf& d_cb = d->tables;
+                std::vector<double> dummy;   // filled on device below
+                T2X_TRY(d_cb.ensure(sizeof(double) * B));
+                // copy C_coarse out of the scalars (device to device, strided)
+                T2X_HIP(hipMemcpy2DAsync(d_cb.ptr, sizeof(double), (char*)d_ps + offsetof(PartScalars, C_coarse),
+                                         sizeof(PartScalars), sizeof(double), B, hipMemcpyDeviceToDevice, st));
+                ep.Cb = d_cb.as<double>();
+                ep.N1 = d->N1.as<float>();
+                ep.ldn = O_pad;
+                ep.prior_o = d->prior_o.as<double>();
+                ep.rowmin = d_rowmin.as<float>();
+                ep.rowmin_ld = rowmin_ld;
+                // running per-particle minima for the lazy stores of negligible blocks (all values are
+                // stored when diagnostic copies of the coarse diff2 are requested)
+                T2X_TRY(d->pmin.ensure(sizeof(unsigned) * B));
+                k_fill_u32<<<(unsigned)((B + 255) / 256), 256, 0, st>>>(d->pmin.as<unsigned>(), 0xFF800000u, B);  // +inf
+                ep.pmin = (dbg && dbg->n_debug > 0) ? nullptr : d->pmin.as<unsigned>();
+                // rows beyond the valid columns of the last tile keep +inf
+                k_fill_f32<<<(unsigned)((M_pad * rowmin_ld + 255) / 256), 256, 0, st>>>(d_rowmin.as<float>(), 3.0e38f,
+                                                                                        M_pad * rowmin_ld);
+                if (prec == T2X_PREC_FP32) {
+                    T2X_TRY(launch_gemm(EPI_DIFF2, d->A.as<float>(), Kc, d->coarse_B.as<float>(), Kc, Kc,
+                                        (int)M_pad, (int)O_pad, ep, st));
+                } else {
+                    const GemmChoice gwant{cfg->gemm_tile, cfg->gemm_group};
+                    GemmChoice gused{0, 0};
+                    if (use_prop) {
+                        if (n_fb_rows > 0) {
+                            EpiParams ef = ep;
+                            ef.M = n_fb_rows;
+                            T2X_TRY(launch_gemm_x3(prec, EPI_DIFF2, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc,
+                                                   M_pad, d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(),
+                                                   Kc, Kc, (int)round_up(n_fb_rows, 256), (int)O_pad, ef, st, gwant,
+                                                   d->prop_rows.as<int>(), &gused));
+                        }
+                        T2X_TRY(launch_gemm_x3_bins(prec, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc, d->coarse_Bh.ptr,
+                                                    d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc,
+                                                    d->prop_plist.as<int>(), d->prop_work.ptr, n_work, (int)T,
+                                                    O_pad - 1, ep, st));
+                    } else
+                        T2X_TRY(launch_gemm_x3(prec, EPI_DIFF2, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc,
+                                               M_pad, d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc,
+                                               Kc, (int)M_pad, (int)O_pad, ep, st, gwant, nullptr, &gused));
+                    if (gused.tile) {
+                        d->stats.s.gemm_tile_used = gused.tile;
+                        d->stats.s.gemm_group_used = gused.group;
+                    }
+                }
+            }
+            (void)hipEventRecord(d->ev[ev++], st);   // 3: coarse gemm end
+            // ---- coarse weights and significance ----
+            wc.kind = SRC_COARSE;
+            if (use_prop) {
+                wc.bin_mask = d->prop_mask.as<unsigned>();
+                wc.mask_words = (n_bins + 31) / 32;
+                wc.bin_cols = bin_cols;
+            }
+            wc.d2 = d->diff2.as<float>();
+            wc.ld = O_pad;
+            wc.T = (int)T;
+            wc.O_all = (int)O_all;
+            wc.prior_o = d->prior_o.as<double>();
+            wc.log_prior_o = d->log_prior_o.as<double>();
+            wc.prior_t = d->prior_t.as<double>();
+            wc.rowmin = d_rowmin.as<float>();
+            wc.rowmin_ld = rowmin_ld;
+            wc.nover = nover;
+            wc.over_trans = over_trans;
+            wc.lp = lp;
+            k_pass_min<<<B, 1024, 0, st>>>(wc, B, d_selc);
+            SelParams sp{cfg->adaptive_fraction, (long long)cfg->maximum_significants, 1};
+            // candidate path: sum pass, compaction of the weights that can be significant, exact
+            // sorted walk per particle; the radix-select path handles batches that overflow
+            T2X_TRY(d->cand_w.ensure(sizeof(double) * B * kCandCap));
+            T2X_TRY(d->cand_e.ensure(sizeof(int) * B * kCandCap));
+            T2X_TRY(d->cand_n.ensure(sizeof(int) * 2 * B));
+            T2X_TRY(d->pairs_strided.ensure(sizeof(int2) * B * kCandCap));
+            int* d_cand_count = d->cand_n.as<int>();
+            int* d_overflow = d_cand_count + B;
+            T2X_HIP(hipMemsetAsync(d_cand_count, 0, sizeof(int) * 2 * B, st));   // counts and overflow flags
+            T2X_TRY(d->sum_part.ensure(sum_part_bytes() * B * T));
+            // local searches take the radix-select path directly (the candidate filter works from the
+            // global log prior)
+            if (!local) {
+                {
+                    const dim3 grid((unsigned)T, (unsigned)B);
+                    k_coarse_sum_rows<<<grid, 256, 0, st>>>(wc, B, d_selc, d->sum_part.as<SumPart>());
+                    k_coarse_sum_final<<<B, 64, 0, st>>>(wc, B, d->sum_part.as<SumPart>(), d_selc);
+                    k_coarse_candidates<<<grid, 256, 0, st>>>(wc, B, sp, d_selc, d->cand_w.as<double>(),
+                                                              d->cand_e.as<int>(), d_cand_count);
+                }
+                k_cand_select<<<B, 1024, 0, st>>>(wc, B, sp, d_selc, d->cand_w.as<double>(), d->cand_e.as<int>(),
+                                                  d_cand_count, d->pairs_strided.as<int2>(),
+                                                  d->sig_count.as<long long>(), d_overflow);
+                T2X_HIP(hipGetLastError());
+                // overflow flags, counts and results in one synchronisation; the counts and results are
+                // read again only when a particle overflowed
+                std::vector<int> hover(B);
+                T2X_HIP(hipMemcpyAsync(hover.data(), d_overflow, sizeof(int) * B, hipMemcpyDeviceToHost, st));
+                T2X_HIP(hipMemcpyAsync(counts.data(), d->sig_count.ptr, sizeof(long long) * 2 * B, hipMemcpyDeviceToHost, st));
+                T2X_HIP(hipMemcpyAsync(hselc.data(), d_selc, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
+                T2X_HIP(hipStreamSynchronize(st));
+                fallback = false;
+                for (int i = 0; i < B; i++) fallback |= hover[i] != 0;
+                if (getenv("T2X_FORCE_RADIX_SELECT")) fallback = true;   // exercises the fallback in tests
+            }
+            if (fallback) {
+                k_select<<<B, 1024, 0, st>>>(wc, sp, B, d_selc);
+                k_sig_count<<<B, 1024, 0, st>>>(wc, B, d_selc, d->sig_count.as<long long>());
+                T2X_HIP(hipGetLastError());
+                stt.n_select_fallbacks += 1;
+                T2X_HIP(hipMemcpyAsync(counts.data(), d->sig_count.ptr, sizeof(long long) * 2 * B, hipMemcpyDeviceToHost, st));
+                T2X_HIP(hipMemcpyAsync(hselc.data(), d_selc, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
+                T2X_HIP(hipStreamSynchronize(st));
+            }
+            (void)hipEventRecord(d->ev[ev++], st);   // 4: coarse weights done
         }
-        (void)hipEventRecord(d->ev[ev++], st);   // 4: coarse weights done
         std::vector<long long> pair_off(B + 1, 0), orient_off(B + 1, 0);
         for (int i = 0; i < B; i++) {
             if (hselc[i].npos == 0 || !(hselc[i].sum_w > 0.) || std::isnan(hselc[i].sum_w)) {
@@ -1721,7 +2061,36 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                                               d->sig_orient.as<int>(), d->orient_pairs_off.as<long long>());
         T2X_HIP(hipGetLastError());
         // debug: coarse diff2 / significance in ihidden order (class, dir, psi, trans)
-        if (dbg && dbg->n_debug > 0 && b0 < dbg->n_debug) {
+        if (dbg && dbg->n_debug > 0 && b0 < dbg->n_debug && local_pairs) {
+            const int nd = std::min(B, dbg->n_debug - b0);
+            const long Nh = O_all * T;
+            std::vector<float> cd(std::max<long long>(npairs_c, 1));
+            std::vector<int2> cp(std::max<long long>(npairs_c, 1));
+            std::vector<long long> cpo(B + 1);
+            T2X_HIP(hipStreamSynchronize(st));
+            T2X_HIP(hipMemcpy(cd.data(), d->lc_d2.ptr, sizeof(float) * cd.size(), hipMemcpyDeviceToHost));
+            T2X_HIP(hipMemcpy(cp.data(), d->lc_pairs.ptr, sizeof(int2) * cp.size(), hipMemcpyDeviceToHost));
+            T2X_HIP(hipMemcpy(cpo.data(), d->lc_pair_off.ptr, sizeof(long long) * (B + 1), hipMemcpyDeviceToHost));
+            for (int i = 0; i < nd; i++) {
+                if (dbg->coarse_diff2) {
+                    float* row = dbg->coarse_diff2 + (size_t)(b0 + i) * Nh;
+                    std::fill(row, row + Nh, -999.f);
+                    for (long long k = cpo[i]; k < cpo[i + 1]; k++) row[(long)cp[k].x * T + cp[k].y] = cd[k];
+                }
+                if (dbg->coarse_min_diff2) dbg->coarse_min_diff2[b0 + i] = hselc[i].min_d2;
+                if (dbg->coarse_sum_weight) dbg->coarse_sum_weight[b0 + i] = hselc[i].sum_w;
+                if (dbg->coarse_sig_weight) dbg->coarse_sig_weight[b0 + i] = hselc[i].sig_w;
+            }
+            if (dbg->coarse_significant) {
+                std::vector<int2> hp(std::max<long long>(npairs, 1));
+                T2X_HIP(hipMemcpy(hp.data(), d->sig_pairs.ptr, sizeof(int2) * hp.size(), hipMemcpyDeviceToHost));
+                for (int i = 0; i < nd; i++) {
+                    unsigned char* m = dbg->coarse_significant + (size_t)(b0 + i) * Nh;
+                    std::memset(m, 0, Nh);
+                    for (long long k = pair_off[i]; k < pair_off[i + 1]; k++) m[(long)hp[k].x * T + hp[k].y] = 1;
+                }
+            }
+        } else if (dbg && dbg->n_debug > 0 && b0 < dbg->n_debug) {
             int nd = std::min(B, dbg->n_debug - b0);
             const long Nh = O_all * T;
             std::vector<float> row((size_t)T * O_pad);
@@ -1777,12 +2146,14 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                 d_vols.as<const float2*>(), d_pads.as<int>(), tf.d_pix.as<int2>(), tf.d_inproj.as<unsigned char>(),
                 d_minv_fd.as<float>(), tf.d_win.as<int>(), tf.npix, d->Fimg_cur.as<float2>(),
                 d->Fctf_cur.as<float>(), Pw, d->ab_fine.as<float2>(), d_ps, d->fine_diff2.as<float>(),
-                d->fine_min.as<unsigned int>());
+                d->fine_min.as<unsigned int>(), 0);
             T2X_HIP(hipGetLastError());
         }
         (void)hipEventRecord(d->ev[ev++], st);   // 6: fine diff2 end
         WeightSrc wf = wc;
         wf.kind = SRC_FINE;
+        wf.nover = nover;
+        wf.over_trans = over_trans;
         wf.d2 = d->fine_diff2.as<float>();
         wf.pair_off = d_pair_off.as<long long>();
         wf.pairs = d->sig_pairs.as<int2>();
@@ -1898,7 +2269,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             stt.ms_store += ms[7];
         }
         stt.n_coarse_diff2_launches += 1;
-        stt.coarse_diff2_flop += 4.0 * (double)T * (double)O_all * (double)tc.npix * B;
+        stt.coarse_diff2_flop += local_pairs ? 4.0 * (double)npairs_c * tc.npix
+                                             : 4.0 * (double)T * (double)O_all * (double)tc.npix * B;
         stt.coarse_diff2_bytes += 4.0 * ((double)M * Kc + (double)O_pad * Kc + (double)M * O_pad);
         stt.fine_samples += (double)npairs * nover;
         stt.fine_flop += 4.0 * (double)npairs * nover * tf.npix;

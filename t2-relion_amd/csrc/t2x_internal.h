@@ -147,7 +147,10 @@ struct Device {
         sig_orient_count, orient_pairs_off, fine_items, fine_diff2, fine_min, fine_sel, ab_fine,
         ab_coarse, wsum_dev, store_items, best, tables, Y_fine, dbg_tmp, cand_w, cand_e, cand_n,
         pairs_strided, Ah, Al, As, sum_part, log_prior_o, pmin;
-    DevBuf loc_dir, loc_psi, loc_mean, loc_ang, loc_tab, loc_list;   // local searches
+    DevBuf loc_dir, loc_psi, loc_mean, loc_ang, loc_tab, loc_list, loc_nsel;   // local searches
+    // coarse pass of a local search on per-particle sample lists
+    DevBuf lc_orients, lc_ops, lc_pairs, lc_d2, lc_min, lc_pair_off, lc_orient_off, lc_cls, lc_dirsc, lc_psisc,
+        lc_n1, lc_G, lc_z, lc_zh, lc_zl, lc_zs, lc_ab, lc_abh, lc_abl, lc_abs;
     // per-call tables and per-batch offsets (grow-only, so no allocation per call or batch)
     DevBuf minv_fd, dirsc, psisc, ftx, fty, vols_tab, pads_tab, bpd_tab, bpw_tab, bpads_tab, bprmax_tab,
         ctx, cty, ctfc, pair_off_d, orient_off_d, item_off_d;

@@ -399,11 +399,17 @@ __global__ __launch_bounds__(1024) void k_sig_count(WeightSrc s, int B, const Se
     const int b = blockIdx.x;
     if (b >= B) return;
     const double mind2 = res[b].min_d2, sig = res[b].sig_w;
+    // dense coarse layout (columns = orientations), or the pair list of a local search's coarse
+    // pass (SRC_FINE with one sample per pair: every orientation followed by its T translations)
+    const bool plist = s.kind == SRC_FINE;
+    const long long* pob = plist ? s.pair_off + b : nullptr;
+    const int ncol = plist ? (int)((s.pair_off[b + 1] - s.pair_off[b]) / s.T) : s.O_all;
     unsigned long long np = 0, no = 0;
-    for (int n = threadIdx.x; n < s.O_all; n += blockDim.x) {
+    for (int n = threadIdx.x; n < ncol; n += blockDim.x) {
         int c = 0;
         for (int t = 0; t < s.T; t++) {
-            double w = pass_weight(s, b, (long long)t * s.O_all + n, mind2, nullptr);
+            const long long e = plist ? (long long)n * s.T + t : (long long)t * s.O_all + n;
+            double w = pass_weight(s, b, e, mind2, pob);
             if (w >= sig && w > 0.) c++;
         }
         np += c;
@@ -430,17 +436,22 @@ __global__ __launch_bounds__(1024) void k_sig_write(WeightSrc s, int B, const Se
     const int b = blockIdx.x;
     if (b >= B) return;
     const double mind2 = res[b].min_d2, sig = res[b].sig_w;
+    const bool plist = s.kind == SRC_FINE;   // as in k_sig_count
+    const long long* pob = plist ? s.pair_off + b : nullptr;
+    const int ncol = plist ? (int)((s.pair_off[b + 1] - s.pair_off[b]) / s.T) : s.O_all;
     __shared__ int scan_p[1024], scan_o[1024];
     __shared__ long long run_p, run_o;
     if (threadIdx.x == 0) { run_p = pair_off[b]; run_o = orient_off[b]; }
     __syncthreads();
-    for (int base = 0; base < s.O_all; base += blockDim.x) {
-        int n = base + threadIdx.x;
+    for (int base = 0; base < ncol; base += blockDim.x) {
+        const int col = base + threadIdx.x;
         unsigned long long mask[2] = {0ull, 0ull};   // up to 128 translations
-        int c = 0;
-        if (n < s.O_all) {
+        int c = 0, n = col;
+        if (col < ncol) {
+            if (plist) n = s.pairs[s.pair_off[b] + (long long)col * s.T].x;
             for (int t = 0; t < s.T; t++) {
-                double w = pass_weight(s, b, (long long)t * s.O_all + n, mind2, nullptr);
+                const long long e = plist ? (long long)col * s.T + t : (long long)t * s.O_all + col;
+                double w = pass_weight(s, b, e, mind2, pob);
                 if (w >= sig && w > 0.) { mask[t >> 6] |= 1ull << (t & 63); c++; }
             }
         }

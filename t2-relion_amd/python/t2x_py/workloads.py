@@ -30,6 +30,14 @@ WORKLOADS = {
                      max_shift=6.0, snr=0.03, healpix_order=3, offset_range_px=6.0,
                      offset_step_px=1.5, particle_diameter=240.0, resolution=10.0,
                      seed=20241),
+    # Local refinement: 360^3 @0.9 A (1500 blobs, r<=150 px), 50000 particles 360^2, SNR 0.02,
+    # local searches at HEALPix 5 (12288 directions x 192 psi) with sigma_ang 3 deg around prior
+    # angles 1 deg (rms) from the true poses, offsets 3/0.5 px; resolution 7 A (current size 112).
+    "local360": dict(ori_size=360, pixel_size=0.9, n_blobs=1500, blob_radius=150.0,
+                     blob_sigma=(1.5, 3.0), blob_amp=(0.5, 1.0), K=1, n_particles=50000,
+                     max_shift=3.0, snr=0.02, healpix_order=5, offset_range_px=3.0,
+                     offset_step_px=0.5, particle_diameter=300.0, resolution=7.0,
+                     sigma_ang=3.0, prior_noise=1.0, seed=20244),
     # 2D classification (parity size): 4 class averages = z-projections of 64^3 blob volumes
     # @2 A, psi step 6 deg (60 in-plane angles), offsets 5/1 px, SNR 0.05.
     "class2d64": dict(ori_size=64, pixel_size=2.0, n_blobs=150, blob_radius=24.0,
@@ -107,6 +115,9 @@ def make_config(w, *, current_size=None, coarse_size=None, precision=T2X_PREC_FP
     cfg.do_ctf_correction = 1
     cfg.sigma2_offset = 10.0 ** 2          # relion_refine --offset 10 (A)
     cfg.orientational_prior_mode = 0
+    if w.get("sigma_ang"):   # local searches (relion_refine --sigma_ang)
+        cfg.orientational_prior_mode = 1
+        cfg.sigma_rot = cfg.sigma_tilt = cfg.sigma_psi = float(w["sigma_ang"])
     cfg.precision = precision
     return cfg, dict(current_size=cur, coarse_size=coarse, r_max=r_max, pad_size=pad)
 
@@ -168,6 +179,12 @@ def workload_particles(rng, w, n, K):
     if w.get("ref_dim", 3) == 2:
         tr["rot"][:] = 0.0
         tr["tilt"][:] = 0.0
+    if w.get("sigma_ang"):   # prior angles near the true poses
+        e = float(w.get("prior_noise", 1.0))
+        for i in range(n):
+            pin[i].prior_rot = tr["rot"][i] + rng.normal(0.0, e)
+            pin[i].prior_tilt = min(max(tr["tilt"][i] + rng.normal(0.0, e), 0.0), 180.0)
+            pin[i].prior_psi = (tr["psi"][i] + rng.normal(0.0, e) + 180.0) % 360.0 - 180.0
     s = w["snr"]
     if isinstance(s, (tuple, list)):
         snr = np.exp(rng.uniform(np.log(s[0]), np.log(s[1]), n))

@@ -104,3 +104,20 @@ def test_local_with_proposal_rejected(gpu_device):
     wl.cfg.proposal_topk = 16
     with pytest.raises(RuntimeError, match="exclusive"):
         wl.run_gpu(dev=gpu_device)
+
+
+def test_sample_lists_equal_dense_masked_pass(gpu_device, monkeypatch):
+    """The local search's coarse pass on per-particle sample lists (default) and the dense pass
+    over every orientation with the out-of-range ones masked (T2X_LOCAL_DENSE=1) score the same
+    samples: identical scored sets and poses, diff2 within FP32 summation-order differences."""
+    wl = localise(H.Workload("refine64", n=16, seed_offset=37, K=2), 9.0, 11.0)
+    a = wl.run_gpu(dev=gpu_device, n_debug=16, fine_cap=16384)
+    monkeypatch.setenv("T2X_LOCAL_DENSE", "1")
+    b = wl.run_gpu(dev=gpu_device, n_debug=16, fine_cap=16384)
+    va, vb = a["dbg"]["coarse_diff2"] > -998, b["dbg"]["coarse_diff2"] > -998
+    assert np.array_equal(va, vb)
+    assert H.rel(a["dbg"]["coarse_diff2"][va], b["dbg"]["coarse_diff2"][vb]) < 2e-6
+    assert np.array_equal(a["out"]["best_hidden"], b["out"]["best_hidden"])
+    assert H.rel(a["out"]["dLL"], b["out"]["dLL"]) < 2e-6
+    for c in range(2):
+        assert H.rel_norm(a["bp"][c][1], b["bp"][c][1]) < 1e-5
