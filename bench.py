@@ -71,6 +71,16 @@ def build_dataset(lib, dev, w, n, cfg, sz, seed):
             allb[c * nb + k] = blob_sets[c][k]
     boff = (C.c_int * (K + 1))(*[c * nb for c in range(K + 1)])
     pin, tr, snr = W.workload_particles(rng, w, n, K)
+    if cfg.orientational_prior_mode == T.T2X_PRIOR_ROTTILT_PSI:
+        # local searches: process particles in order of prior orientation (cells of the prior
+        # direction, then psi), so that a batch's particles share in-range slices
+        pr = np.array([[p.prior_rot, p.prior_tilt, p.prior_psi] for p in pin])
+        zb = np.clip(((np.cos(np.radians(pr[:, 1])) + 1.0) * 18.0).astype(int), 0, 35)
+        fb = ((pr[:, 0] % 360.0) / 10.0).astype(int)
+        order = np.lexsort((pr[:, 2], fb, zb))
+        pin = (T.ParticleIn * n)(*[pin[i] for i in order])
+        tr = {k: v[order] for k, v in tr.items()}
+        snr = snr[order]
     N = cfg.ori_size
     dimg = C.c_void_p()
     T.check(lib.t2x_device_alloc(dev, n * N * N * 4, C.byref(dimg)), "alloc images")
@@ -476,6 +486,8 @@ def main():
                           n * args.steps / (rr["dt"] + (train.ms_total / 1e3 if train else 0.0)),
                       "train_loss_first_last": [train.loss_first, train.loss_last] if train else None,
                       "train_top1_last": train.top1_last if train else None})
+        if rr["st"].local_rows > 0:
+            v["local_rows_per_slice"] = rr["st"].local_rows / max(1.0, rr["st"].local_slices)
         variants[pname] = v
     quality = None
     if args.quality_particles > 0 and cfg.ref_dim == 3 and not local:

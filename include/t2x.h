@@ -207,6 +207,8 @@ typedef struct t2x_stats {
     double proposal_bins_scored; /* sum over particles of the bins scored in the coarse pass  */
     int    gemm_tile_used;       /* split-precision GEMM variant of the last launch (tile, group) */
     int    gemm_group_used;
+    double local_rows;           /* local searches: (particle, orientation) rows of the coarse pass */
+    double local_slices;         /* of those, distinct slices projected (shared within a batch)    */
 } t2x_stats;
 
 /* ---------------------------------------------------------------------------------------

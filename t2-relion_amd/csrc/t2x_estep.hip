@@ -480,17 +480,47 @@ __global__ __launch_bounds__(256) void k_fine_diff2(
 // Local searches, coarse pass as a GEMM over the batch's (particle, orientation) rows:
 // G[(b,o)][t] = sum_p Re(Z_bo,p conj(AB_t,p)) with Z_bo = F_o conj(minv ctf X_b), i.e. the real
 // dot product of [Z.re, Z.im] with [AB.re, AB.im] over K = 2 P_c; the phase table AB is the
-// shared operand. k_local_zrows projects each row's slice and forms Z (FP32, or straight into the
-// hi/lo split operand with its row scale) and n1 = 1/2 sum minv ctf^2 |F|^2; k_local_diff2 forms
+// shared operand. k_local_zrows forms Z from the row's slice (projected once per batch by
+// k_project_list) as FP32 or straight into the hi/lo split operand with its row scale, and
+// n1 = 1/2 sum minv ctf^2 |F|^2; k_local_diff2 forms
 // diff2 = C_coarse + n1 - G into the sample list (orientation-major, T per orientation).
+// The batch's distinct in-range orientations projected once (exact FP32 slices at coarse size,
+// XCD-aware order as k_local_zrows), shared by every particle whose range holds them.
+__global__ __launch_bounds__(256) void k_project_list(long long nu, const int* __restrict__ ulist, FineGeom g,
+                                                      const float2* const* __restrict__ vols,
+                                                      const int* __restrict__ pads, const int2* __restrict__ pix,
+                                                      const unsigned char* __restrict__ inproj, int npix, int ldf,
+                                                      float2* __restrict__ F)
+{
+    const long long per = (long long)gridDim.x / 8;
+    const long long u = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
+    if (u >= nu) return;
+    double A[9], Ai[9];
+    int c, idir;
+    fine_matrix(g, ulist[u], 0, A, &c, &idir);
+    inv3(A, Ai);
+    const float a0 = (float)(Ai[0] * g.pf), a1 = (float)(Ai[1] * g.pf);
+    const float a2 = (float)(Ai[3] * g.pf), a3 = (float)(Ai[4] * g.pf);
+    const float a4 = (float)(Ai[6] * g.pf), a5 = (float)(Ai[7] * g.pf);
+    const float2* vol = vols[c];
+    const int pad = pads[c];
+    for (int p = threadIdx.x; p < npix; p += blockDim.x) {
+        float2 f = make_float2(0.f, 0.f);
+        if (inproj[p]) {
+            const float x = (float)pix[p].x, y = (float)pix[p].y;
+            float2 v;
+            if (ref_sample(vol, pad, g.ref2d, a0 * x + a1 * y, a2 * x + a3 * y, a4 * x + a5 * y, &v)) f = v;
+        }
+        F[u * ldf + p] = f;
+    }
+}
+
 template <class H, int SPLIT>
 __global__ __launch_bounds__(256) void k_local_zrows(int B, long long nrows, const long long* __restrict__ orient_off,
-                                                     const int* __restrict__ orients, FineGeom g,
-                                                     const float2* const* __restrict__ vols,
-                                                     const int* __restrict__ pads, const int2* __restrict__ pix,
-                                                     const unsigned char* __restrict__ inproj,
-                                                     const float* __restrict__ minv, const int* __restrict__ win,
-                                                     int npix, const float2* __restrict__ Fimg,
+                                                     const int* __restrict__ rowu, const float2* __restrict__ F,
+                                                     int ldf, const float* __restrict__ minv,
+                                                     const int* __restrict__ win, int npix,
+                                                     const float2* __restrict__ Fimg,
                                                      const float* __restrict__ Fctf, int Pw, int Kc,
                                                      float* __restrict__ zf, H* __restrict__ zh, H* __restrict__ zl,
                                                      float* __restrict__ zs, double* __restrict__ n1out)
@@ -509,25 +539,11 @@ __global__ __launch_bounds__(256) void k_local_zrows(int B, long long nrows, con
     double n1 = 0.;
     if (valid) {
         const int b = find_particle(orient_off, B, j);
-        const int n = orients[j];
-        double A[9], Ai[9];
-        int c, idir;
-        fine_matrix(g, n, 0, A, &c, &idir);
-        inv3(A, Ai);
-        const float a0 = (float)(Ai[0] * g.pf), a1 = (float)(Ai[1] * g.pf);
-        const float a2 = (float)(Ai[3] * g.pf), a3 = (float)(Ai[4] * g.pf);
-        const float a4 = (float)(Ai[6] * g.pf), a5 = (float)(Ai[7] * g.pf);
-        const float2* vol = vols[c];
-        const int pad = pads[c];
+        const float2* Fu = F + (long long)rowu[j] * ldf;
         for (int p = threadIdx.x; p < Kc / 2; p += blockDim.x) {
             float zr = 0.f, zi = 0.f;
             if (p < npix) {
-                float2 f = make_float2(0.f, 0.f);
-                if (inproj[p]) {
-                    const float x = (float)pix[p].x, y = (float)pix[p].y;
-                    float2 v;
-                    if (ref_sample(vol, pad, g.ref2d, a0 * x + a1 * y, a2 * x + a3 * y, a4 * x + a5 * y, &v)) f = v;
-                }
+                const float2 f = Fu[p];
                 const size_t w = (size_t)b * Pw + win[p];
                 const float2 X = Fimg[w];
                 const float ctf = Fctf[w], m = minv[p];
@@ -1715,8 +1731,31 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             gc.psi_sc = d->lc_psisc.as<double>();
             gc.pf = pf;
             gc.ref2d = ref2d;
+            // the batch's distinct orientations (particles sorted by prior orientation share many)
+            std::vector<int> horients(std::max<long long>(norients_c, 1));
+            T2X_HIP(hipMemcpyAsync(horients.data(), d->lc_orients.ptr, sizeof(int) * norients_c, hipMemcpyDeviceToHost, st));
+            T2X_HIP(hipStreamSynchronize(st));
+            if ((long long)d->lc_umap.size() < O_all) d->lc_umap.assign(O_all, -1);
+            std::vector<int> ulist, rowu(std::max<long long>(norients_c, 1));
+            for (long long r = 0; r < norients_c; r++) {
+                int& u = d->lc_umap[horients[r]];
+                if (u < 0) { u = (int)ulist.size(); ulist.push_back(horients[r]); }
+                rowu[r] = u;
+            }
+            for (int n : ulist) d->lc_umap[n] = -1;
+            const long long nu = (long long)ulist.size();
+            stt.local_slices += (double)nu;
+            stt.local_rows += (double)norients_c;
+            T2X_TRY(upload(d->lc_ulist, ulist, st));
+            T2X_TRY(upload(d->lc_rowu, rowu, st));
+            const int ldf = tc.npix_pad;
+            T2X_TRY(d->lc_F.ensure(sizeof(float2) * std::max<long long>(nu, 1) * ldf));
             (void)hipEventRecord(d->ev[ev++], st);   // 2: coarse diff2 start
             if (norients_c > 0) {
+                k_project_list<<<(unsigned)round_up(nu, 8), 256, 0, st>>>(
+                    nu, d->lc_ulist.as<int>(), gc, d_vols.as<const float2*>(), d_pads.as<int>(), tc.d_pix.as<int2>(),
+                    tc.d_inproj.as<unsigned char>(), tc.npix, ldf, d->lc_F.as<float2>());
+                T2X_HIP(hipGetLastError());
                 // rows = the batch's (particle, orientation) pairs, columns = translations
                 const long long rows = norients_c, rows_pad = round_up(rows, 128);
                 EpiParams eg{};
@@ -1729,9 +1768,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                 if (prec == T2X_PREC_FP32) {
                     T2X_TRY(d->lc_z.ensure(sizeof(float) * rows_pad * Kc));
                     k_local_zrows<float, 0><<<(unsigned)rows_pad, 256, 0, st>>>(
-                        B, rows, d->lc_orient_off.as<long long>(), d->lc_orients.as<int>(), gc,
-                        d_vols.as<const float2*>(), d_pads.as<int>(), tc.d_pix.as<int2>(),
-                        tc.d_inproj.as<unsigned char>(), tc.d_minv.as<float>(), tc.d_win.as<int>(), tc.npix,
+                        B, rows, d->lc_orient_off.as<long long>(), d->lc_rowu.as<int>(), d->lc_F.as<float2>(), ldf,
+                        tc.d_minv.as<float>(), tc.d_win.as<int>(), tc.npix,
                         d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, Kc, d->lc_z.as<float>(), nullptr,
                         nullptr, nullptr, d->lc_n1.as<double>());
                     T2X_HIP(hipGetLastError());
@@ -1744,16 +1782,14 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                     const size_t lds = sizeof(float) * Kc;
                     if (prec == T2X_PREC_BF16X3)
                         k_local_zrows<__bf16, 1><<<(unsigned)rows_pad, 256, lds, st>>>(
-                            B, rows, d->lc_orient_off.as<long long>(), d->lc_orients.as<int>(), gc,
-                            d_vols.as<const float2*>(), d_pads.as<int>(), tc.d_pix.as<int2>(),
-                            tc.d_inproj.as<unsigned char>(), tc.d_minv.as<float>(), tc.d_win.as<int>(), tc.npix,
+                            B, rows, d->lc_orient_off.as<long long>(), d->lc_rowu.as<int>(), d->lc_F.as<float2>(),
+                            ldf, tc.d_minv.as<float>(), tc.d_win.as<int>(), tc.npix,
                             d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, Kc, nullptr,
                             (__bf16*)d->lc_zh.ptr, (__bf16*)d->lc_zl.ptr, d->lc_zs.as<float>(), d->lc_n1.as<double>());
                     else
                         k_local_zrows<_Float16, 1><<<(unsigned)rows_pad, 256, lds, st>>>(
-                            B, rows, d->lc_orient_off.as<long long>(), d->lc_orients.as<int>(), gc,
-                            d_vols.as<const float2*>(), d_pads.as<int>(), tc.d_pix.as<int2>(),
-                            tc.d_inproj.as<unsigned char>(), tc.d_minv.as<float>(), tc.d_win.as<int>(), tc.npix,
+                            B, rows, d->lc_orient_off.as<long long>(), d->lc_rowu.as<int>(), d->lc_F.as<float2>(),
+                            ldf, tc.d_minv.as<float>(), tc.d_win.as<int>(), tc.npix,
                             d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, Kc, nullptr,
                             (_Float16*)d->lc_zh.ptr, (_Float16*)d->lc_zl.ptr, d->lc_zs.as<float>(), d->lc_n1.as<double>());
                     T2X_HIP(hipGetLastError());

@@ -118,7 +118,8 @@ class Stats(C.Structure):
                                    "fine_flop", "bp_orientations", "sig_coarse")] + \
         [(n, C.c_int) for n in ("coarse_size", "current_size", "P_coarse", "P_fine")] + \
         [("ms_proposal", C.c_double), ("n_proposal_particles", C.c_long), ("n_proposal_fallback", C.c_long),
-         ("proposal_bins_scored", C.c_double), ("gemm_tile_used", C.c_int), ("gemm_group_used", C.c_int)]
+         ("proposal_bins_scored", C.c_double), ("gemm_tile_used", C.c_int), ("gemm_group_used", C.c_int),
+                ("local_rows", C.c_double), ("local_slices", C.c_double)]
 
 
 class MrcInfo(C.Structure):
