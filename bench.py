@@ -20,6 +20,11 @@ resolution per path, and the pose agreement of each path with exhaustive search.
 
 The CPU baseline times the FP64 oracle (the restatement of relion_refine's CPU E-step) on a
 bounded sample of the same workload with one particle per thread.
+
+Other workloads (--workload): class2d128 (2D classification, 100,000 particles 128^2, K=100;
+paths fp32, fp16x3) and local360 (local searches, 50,000 particles 360^2, HEALPix 5, sigma_ang
+3 deg, particles processed in order of prior orientation; paths fp32, fp16x3). The default
+(class256) is the headline line.
 """
 import argparse
 import ctypes as C

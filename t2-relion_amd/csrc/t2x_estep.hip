@@ -395,9 +395,7 @@ __device__ __forceinline__ void fine_matrix(const FineGeom& g, int n, int ir, do
 // Fine diff2 of every significant coarse pair: one workgroup per (particle, significant
 // orientation, oversampled rotation). The slice at current size is projected into LDS as
 // Z = F conj(minv ctf X); each wave then scores one oversampled translation at a time,
-// G_t = sum_p Re(Z conj(AB_t)), diff2 = C_b + 1/2 sum minv ctf^2 |F|^2 - G_t. With coarse_C (and
-// the coarse pixel set, phase table and orientation grid, no oversampling) the same kernel scores
-// the coarse pass of a local search on each particle's own sample list.
+// G_t = sum_p Re(Z conj(AB_t)), diff2 = C_b + 1/2 sum minv ctf^2 |F|^2 - G_t.
 __global__ __launch_bounds__(256) void k_fine_diff2(
     const long long* __restrict__ item_off, int B, const long long* __restrict__ orient_off,
     const int* __restrict__ orients, const long long* __restrict__ orient_pair_start,
@@ -407,7 +405,7 @@ __global__ __launch_bounds__(256) void k_fine_diff2(
     const float* __restrict__ minv, const int* __restrict__ win, int npix,
     const float2* __restrict__ Fimg, const float* __restrict__ Fctf, int Pw,
     const float2* __restrict__ ab_fine, const PartScalars* __restrict__ ps,
-    float* __restrict__ fine_d2, unsigned int* __restrict__ fine_min, int coarse_C)
+    float* __restrict__ fine_d2, unsigned int* __restrict__ fine_min)
 {
     __shared__ float2 Z[FINE_MAX_PIX];
     __shared__ double red[4];
@@ -448,7 +446,7 @@ __global__ __launch_bounds__(256) void k_fine_diff2(
     for (int o = 32; o > 0; o >>= 1) n1 += __shfl_xor(n1, o);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n1;
     __syncthreads();
-    const double base = (coarse_C ? ps[b].C_coarse : ps[b].C_fine) + red[0] + red[1] + red[2] + red[3];
+    const double base = ps[b].C_fine + red[0] + red[1] + red[2] + red[3];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int nt = (int)(p1 - p0) * g.over_trans;
     float lmin = 3.0e38f;
@@ -1079,12 +1077,12 @@ __global__ __launch_bounds__(256) void k_local_prior(const double* __restrict__ 
 
 // The coarse sample list of a local search: for each particle, its orientations (classes with
 // non-zero probability x its directions x its psi angles, in the global order) each followed by
-// all T translations. orient_off: [B+1] offsets of the particles' orientations; pair k of
-// orientation j is (j * T + t).
+// all T translations. orient_off: [B+1] offsets of the particles' orientations; the pairs of
+// orientation j are (j * T + t).
 __global__ void k_local_pairs(int B, const int* __restrict__ list, int Nd, int Npsi, const int* __restrict__ nsel,
                               const int* __restrict__ cls, int Kact, long long Norient, int T,
                               const long long* __restrict__ orient_off, int* __restrict__ orients,
-                              long long* __restrict__ orient_pair_start, int2* __restrict__ pairs)
+                              int2* __restrict__ pairs)
 {
     const int b = blockIdx.x;
     if (b >= B) return;
@@ -1099,7 +1097,6 @@ __global__ void k_local_pairs(int B, const int* __restrict__ list, int Nd, int N
         const int n = (int)(cls[ci] * Norient + (long long)dl[di] * Npsi + pl[pi]);
         const long long p0 = (o0 + j) * T;
         orients[o0 + j] = n;
-        orient_pair_start[o0 + j] = p0;
         for (int t = 0; t < T; t++) pairs[p0 + t] = make_int2(n, t);
     }
 }
@@ -1692,8 +1689,9 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         long long npairs_c = 0;
         if (local_pairs) {
             // local search on per-particle sample lists: each particle's orientations in range x all
-            // translations, scored at coarse size by the fine-pass kernel (no dense pass over every
-            // orientation); weights, significance and the significant pairs as in the dense pass
+            // translations, scored by a GEMM of the batch's (particle, orientation) rows against
+            // the phase table (no dense pass over every orientation); weights, significance and the
+            // significant pairs as in the dense pass, read from the lists
             std::vector<int> hnsel(2 * B);
             T2X_HIP(hipMemcpyAsync(hnsel.data(), d->loc_nsel.ptr, sizeof(int) * 2 * B, hipMemcpyDeviceToHost, st));
             T2X_HIP(hipStreamSynchronize(st));
@@ -1709,14 +1707,13 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             T2X_TRY(upload(d->lc_orient_off, coff, st));
             T2X_TRY(upload(d->lc_pair_off, cpoff, st));
             T2X_TRY(d->lc_orients.ensure(sizeof(int) * std::max<long long>(norients_c, 1)));
-            T2X_TRY(d->lc_ops.ensure(sizeof(long long) * std::max<long long>(norients_c, 1)));
             T2X_TRY(d->lc_pairs.ensure(sizeof(int2) * std::max<long long>(npairs_c, 1)));
             T2X_TRY(d->lc_d2.ensure(sizeof(float) * std::max<long long>(npairs_c, 1)));
             T2X_TRY(d->lc_min.ensure(sizeof(unsigned) * B));
             k_local_pairs<<<B, 256, 0, st>>>(B, d->loc_list.as<int>(), (int)Nd, (int)Npsi, d->loc_nsel.as<int>(),
                                             d->lc_cls.as<int>(), Kact, (long long)Norient, (int)T,
                                             d->lc_orient_off.as<long long>(), d->lc_orients.as<int>(),
-                                            d->lc_ops.as<long long>(), d->lc_pairs.as<int2>());
+                                            d->lc_pairs.as<int2>());
             k_fill_u32<<<(B + 255) / 256, 256, 0, st>>>(d->lc_min.as<unsigned int>(), 0xffffffffu, B);
             T2X_HIP(hipGetLastError());
             FineGeom gc{};
@@ -2182,7 +2179,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                 d_vols.as<const float2*>(), d_pads.as<int>(), tf.d_pix.as<int2>(), tf.d_inproj.as<unsigned char>(),
                 d_minv_fd.as<float>(), tf.d_win.as<int>(), tf.npix, d->Fimg_cur.as<float2>(),
                 d->Fctf_cur.as<float>(), Pw, d->ab_fine.as<float2>(), d_ps, d->fine_diff2.as<float>(),
-                d->fine_min.as<unsigned int>(), 0);
+                d->fine_min.as<unsigned int>());
             T2X_HIP(hipGetLastError());
         }
         (void)hipEventRecord(d->ev[ev++], st);   // 6: fine diff2 end

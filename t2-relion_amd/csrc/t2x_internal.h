@@ -149,7 +149,7 @@ struct Device {
         pairs_strided, Ah, Al, As, sum_part, log_prior_o, pmin;
     DevBuf loc_dir, loc_psi, loc_mean, loc_ang, loc_tab, loc_list, loc_nsel;   // local searches
     // coarse pass of a local search on per-particle sample lists
-    DevBuf lc_orients, lc_ops, lc_pairs, lc_d2, lc_min, lc_pair_off, lc_orient_off, lc_cls, lc_dirsc, lc_psisc,
+    DevBuf lc_orients, lc_pairs, lc_d2, lc_min, lc_pair_off, lc_orient_off, lc_cls, lc_dirsc, lc_psisc,
         lc_n1, lc_G, lc_z, lc_zh, lc_zl, lc_zs, lc_ab, lc_abh, lc_abl, lc_abs, lc_ulist, lc_rowu, lc_F;
     std::vector<int> lc_umap;   // orientation -> slice of the batch (-1 between batches)
     // per-call tables and per-batch offsets (grow-only, so no allocation per call or batch)
