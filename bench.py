@@ -139,10 +139,11 @@ def run_step(lib, dev, cfg, sz, ds, n, outs, ws, first=0):
     T.check(lib.t2x_estep(dev, C.byref(cfg), C.byref(model), n, img, 1, pin, outs, C.byref(ws), None), "estep")
 
 
-def train_proposal(lib, dev, cfg, sz, ds, w, steps=2000, batch=256, n_proj=200000, seed=2024):
+def train_proposal(lib, dev, cfg, sz, ds, w, steps=2000, batch=256, n_proj=200000, seed=2024, lr=3e-3,
+                   shift_weight=1.0):
     set_projectors(lib, dev, sz, ds)
     model = T.Model(T.dptr(ds["sigma2"]), T.dptr(ds["pdf_class"]), T.dptr(ds["pdf_dir"]), None)
-    tp = T.ProposalTrainParams(steps=steps, batch=batch, n_projections=n_proj, bf16=1, lr=3e-3, shift_weight=1.0,
+    tp = T.ProposalTrainParams(steps=steps, batch=batch, n_projections=n_proj, bf16=1, lr=lr, shift_weight=shift_weight,
                                max_shift_px=float(w["max_shift"]), seed=seed)
     st = T.ProposalTrainStats()
     T.check(lib.t2x_proposal_train(dev, C.byref(cfg), C.byref(model), C.byref(tp), C.byref(st)), "train")
@@ -339,6 +340,8 @@ def main():
                          "last one is the headline value, the others are reported beside it (default: "
                          "fp32,fp16x3,learned,learned_gated for 3D references, fp32,fp16x3 for 2D)")
     ap.add_argument("--train-steps", type=int, default=2000)
+    ap.add_argument("--train-lr", type=float, default=3e-3, help="peak Adam learning rate of the proposal network")
+    ap.add_argument("--train-shift-weight", type=float, default=1.0, help="weight of the shift regression loss")
     ap.add_argument("--save-proposal", default="", help="write the trained proposal parameters (.npy)")
     ap.add_argument("--load-proposal", default="",
                     help="use proposal parameters saved by an earlier run instead of training")
@@ -396,10 +399,11 @@ def main():
         log(f"[learned] proposal parameters loaded from {args.load_proposal}")
     elif use_net:
         t = time.time()
-        train = train_proposal(lib, dev, cfg, sz, ds, w, steps=args.train_steps)
+        train = train_proposal(lib, dev, cfg, sz, ds, w, steps=args.train_steps, lr=args.train_lr,
+                               shift_weight=args.train_shift_weight)
         log(f"[learned] proposal trained: {train.steps} steps in {train.ms_total:.0f} ms "
             f"(setup {train.ms_setup:.0f} ms), loss {train.loss_first:.3f} -> {train.loss_last:.3f}, "
-            f"train top-1 {train.top1_last:.3f} ({time.time() - t:.1f} s wall)")
+            f"train top-1 {train.top1_last:.3f} top-16 {train.top16_last:.3f} ({time.time() - t:.1f} s wall)")
         if args.save_proposal:
             nparam = C.c_longlong()
             T.check(lib.t2x_proposal_num_params(dev, C.byref(nparam)))

@@ -147,3 +147,16 @@ def test_lazy_stores_leave_results_unchanged(gpu_device):
     assert np.array_equal(a["out"]["best_hidden"], b["out"]["best_hidden"])
     assert np.array_equal(a["out"]["dLL"], b["out"]["dLL"])
     assert H.rel_norm(a["bp"][0][1], b["bp"][0][1]) < 1e-6
+
+
+def test_empty_and_single_particle(gpu_device):
+    """No particles: the call succeeds and adds nothing; one particle: parity with the oracle."""
+    wl = H.Workload("refine64", n=1, seed_offset=14)
+    o = wl.run_oracle()
+    g = wl.run_gpu(dev=gpu_device)
+    assert g["out"]["best_hidden"][0] == o["out"]["best_hidden"][0]
+    assert H.rel(g["out"]["dLL"], o["out"]["dLL"]) < 1e-5
+    wl.n = 0
+    e = wl.run_gpu(dev=gpu_device)
+    assert all(np.abs(v).sum() == 0 for v in e["wsum"].values())
+    assert np.abs(e["bp"][0][1]).sum() == 0
