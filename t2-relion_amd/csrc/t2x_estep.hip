@@ -995,6 +995,32 @@ __device__ void block_argmin(double& v, int& i, double* sv, int* si)
     __syncthreads();
 }
 
+// Indices i < n with v[i] > 0 written to out in increasing order by the whole workgroup (256
+// threads, 4 waves; scratch: 5 ints of LDS); returns their number to every thread.
+__device__ int block_compact_positive(const double* v, int n, int* out, int* scratch)
+{
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (threadIdx.x == 0) scratch[4] = 0;
+    __syncthreads();
+    for (int base = 0; base < n; base += blockDim.x) {
+        const int i = base + threadIdx.x;
+        const bool f = i < n && v[i] > 0.;
+        const unsigned long long m = __ballot(f);
+        const int before = __popcll(m & ((1ull << lane) - 1ull));
+        if (lane == 0) scratch[w] = __popcll(m);
+        __syncthreads();
+        int off = scratch[4];
+        for (int k = 0; k < w; k++) off += scratch[k];
+        if (f) out[off + before] = i;
+        __syncthreads();
+        if (threadIdx.x == 0) scratch[4] += scratch[0] + scratch[1] + scratch[2] + scratch[3];
+        __syncthreads();
+    }
+    const int total = scratch[4];
+    __syncthreads();
+    return total;
+}
+
 // pang: [B][3] prior rot, tilt, psi (already resolved from the current angles where unset).
 // Outputs: dir [B][Nd], psi [B][Npsi] normalised priors (0 outside the range), mean [B];
 // list [B][Nd + Npsi] scratch (compacted selected indices).
@@ -1044,22 +1070,17 @@ __global__ __launch_bounds__(256) void k_local_prior(const double* __restrict__ 
     block_argmin(bestp, bp, sv, si);
     __threadfence_block();
     __syncthreads();
-    // sums in index order (sumprior of the reference), compaction, normalisation
+    // the selected indices in increasing order (all threads), then the sums in that order
+    // (sumprior of the reference) and the normalisation
+    int nd = block_compact_positive(db, q.Nd, dl, si);
+    int np = block_compact_positive(pb, q.Npsi, pl, si);
     if (threadIdx.x == 0) {
         double sd = 0.;
-        int nd = 0;
-        for (int d = 0; d < q.Nd; d++) {
-            const double x = db[d];
-            if (x > 0.) { sd += x; dl[nd++] = d; }
-        }
+        for (int k = 0; k < nd; k++) sd += db[dl[k]];
         for (int k = 0; k < nd; k++) db[dl[k]] /= sd;
         if (nd == 0) { const int d0 = bi >= 0 ? bi : 0; db[d0] = 1.; dl[nd++] = d0; }
         double sp = 0.;
-        int np = 0;
-        for (int p = 0; p < q.Npsi; p++) {
-            const double x = pb[p];
-            if (x > 0.) { sp += x; pl[np++] = p; }
-        }
+        for (int k = 0; k < np; k++) sp += pb[pl[k]];
         for (int k = 0; k < np; k++) pb[pl[k]] /= sp;
         if (np == 0) { const int p0 = bp >= 0 ? bp : 0; pb[p0] = 1.; pl[np++] = p0; }
         // pdf_orientation_mean over classes x local directions x local psi angles
