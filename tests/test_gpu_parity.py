@@ -160,3 +160,33 @@ def test_empty_and_single_particle(gpu_device):
     e = wl.run_gpu(dev=gpu_device)
     assert all(np.abs(v).sum() == 0 for v in e["wsum"].values())
     assert np.abs(e["bp"][0][1]).sum() == 0
+
+
+@pytest.fixture(scope="module")
+def class256_case():
+    """The benchmark workload's own shapes (256^2, K=4, HEALPix 3, 7.2 M coarse samples per
+    particle), 4 particles: small enough for the FP64 oracle."""
+    wl = H.Workload("class256", n=4)
+    return wl, wl.run_oracle(n_debug=4, fine_cap=1 << 16)
+
+
+@pytest.mark.parametrize("prec", [0, 2])
+def test_class256_shapes(class256_case, gpu_device, prec):
+    wl, o = class256_case
+    wl.cfg.precision = prec
+    g = wl.run_gpu(dev=gpu_device, n_debug=4, fine_cap=1 << 16)
+    wl.cfg.precision = 0
+    od, gd = o["dbg"], g["dbg"]
+    valid = od["coarse_diff2"] > -998
+    assert np.array_equal(valid, gd["coarse_diff2"] > -998)
+    e = H.rel(gd["coarse_diff2"][valid], od["coarse_diff2"][valid])
+    print(f"class256 shapes precision {prec}: coarse diff2 max rel {e:.2e}")
+    assert e < 1e-5
+    assert np.array_equal(o["out"]["best_hidden"], g["out"]["best_hidden"])
+    assert H.rel(g["out"]["dLL"], o["out"]["dLL"]) < 1e-5
+    for k, v in o["wsum"].items():
+        if np.abs(v).sum() > 0:
+            assert H.rel_norm(g["wsum"][k], v) < 1e-4, k
+    for c in range(wl.cfg.nr_classes):
+        if np.abs(o["bp"][c][1]).sum() > 0:
+            assert H.rel_norm(g["bp"][c][1], o["bp"][c][1]) < 1e-4, c
