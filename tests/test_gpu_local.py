@@ -121,3 +121,18 @@ def test_sample_lists_equal_dense_masked_pass(gpu_device, monkeypatch):
     assert H.rel(a["out"]["dLL"], b["out"]["dLL"]) < 2e-6
     for c in range(2):
         assert H.rel_norm(a["bp"][c][1], b["bp"][c][1]) < 1e-5
+
+
+def test_local360_shapes(gpu_device):
+    """The local-refinement configuration's own shapes (360^2, HEALPix 5 = 2.36 M orientations,
+    sigma_ang 3 deg, 113 translations, P_c 5,007) on 3 particles; 60-blob references keep the
+    oracle's reference synthesis short (the shapes do not depend on the blob count)."""
+    wl = H.Workload("local360", n=3, overrides=dict(n_blobs=60))
+    o = wl.run_oracle()
+    g = wl.run_gpu(dev=gpu_device)
+    assert np.array_equal(o["out"]["best_hidden"], g["out"]["best_hidden"])
+    assert H.rel(g["out"]["dLL"], o["out"]["dLL"]) < 1e-5
+    for k, v in o["wsum"].items():
+        if np.abs(v).sum() > 0:
+            assert H.rel_norm(g["wsum"][k], v) < 1e-4, k
+    assert H.rel_norm(g["bp"][0][1], o["bp"][0][1]) < 1e-4
