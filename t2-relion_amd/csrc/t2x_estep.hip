@@ -42,6 +42,10 @@ int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const floa
                    long long a_rows, const void* Bh, const void* Bl, const float* Bs, long long ldb, int K,
                    int Mrows_pad, int Nrows_pad, const EpiParams& ep, hipStream_t st, GemmChoice want,
                    const int* rowmap, GemmChoice* used);
+int launch_gemm_local_x3(int prec, const void* Sh, const void* Sl, const float* Ss, const void* Wh, const void* Wl,
+                         const float* Ws, long long ld, int K, const int2* items, int n_items, int T,
+                         const long long* orient_off, const int* rowu, const double* cb, const double* n1,
+                         float* d2, unsigned int* dmin, hipStream_t st);
 
 __global__ void k_pass_min(WeightSrc s, int B, SelResult* res);
 __global__ void k_select(WeightSrc s, SelParams prm, int B, SelResult* res);
@@ -475,21 +479,26 @@ __global__ __launch_bounds__(256) void k_fine_diff2(
     }
 }
 
-// Local searches, coarse pass as a GEMM over the batch's (particle, orientation) rows:
+// Local searches, coarse pass (FP32) as a GEMM over the batch's (particle, orientation) rows:
 // G[(b,o)][t] = sum_p Re(Z_bo,p conj(AB_t,p)) with Z_bo = F_o conj(minv ctf X_b), i.e. the real
 // dot product of [Z.re, Z.im] with [AB.re, AB.im] over K = 2 P_c; the phase table AB is the
 // shared operand. k_local_zrows forms Z from the row's slice (projected once per batch by
-// k_project_list) as FP32 or straight into the hi/lo split operand with its row scale, and
-// n1 = 1/2 sum minv ctf^2 |F|^2; k_local_diff2 forms
+// k_project_list) and n1 = 1/2 sum minv ctf^2 |F|^2; k_local_diff2 forms
 // diff2 = C_coarse + n1 - G into the sample list (orientation-major, T per orientation).
 // The batch's distinct in-range orientations projected once (exact FP32 slices at coarse size,
-// XCD-aware order as k_local_zrows), shared by every particle whose range holds them.
+// XCD-aware order as k_local_zrows), shared by every particle whose range holds them. SPLIT 0:
+// float2 rows F (the FP32 path); SPLIT 1: the rows straight into the split operand (hi, lo and a
+// power-of-two row scale for FP16, as k_split_rows) plus |F|^2 for the n1 term.
+template <class H, int SPLIT>
 __global__ __launch_bounds__(256) void k_project_list(long long nu, const int* __restrict__ ulist, FineGeom g,
                                                       const float2* const* __restrict__ vols,
                                                       const int* __restrict__ pads, const int2* __restrict__ pix,
                                                       const unsigned char* __restrict__ inproj, int npix, int ldf,
-                                                      float2* __restrict__ F)
+                                                      float2* __restrict__ F, H* __restrict__ fh, H* __restrict__ fl,
+                                                      float* __restrict__ fs, int use_scale, float* __restrict__ F2)
 {
+    extern __shared__ float frow[];   // [2 ldf] (split only)
+    __shared__ float redf[4];
     const long long per = (long long)gridDim.x / 8;
     const long long u = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
     if (u >= nu) return;
@@ -502,38 +511,81 @@ __global__ __launch_bounds__(256) void k_project_list(long long nu, const int* _
     const float a4 = (float)(Ai[6] * g.pf), a5 = (float)(Ai[7] * g.pf);
     const float2* vol = vols[c];
     const int pad = pads[c];
-    for (int p = threadIdx.x; p < npix; p += blockDim.x) {
+    float mx = 0.f;
+    for (int p = threadIdx.x; p < ldf; p += blockDim.x) {
         float2 f = make_float2(0.f, 0.f);
-        if (inproj[p]) {
+        if (p < npix && inproj[p]) {
             const float x = (float)pix[p].x, y = (float)pix[p].y;
             float2 v;
             if (ref_sample(vol, pad, g.ref2d, a0 * x + a1 * y, a2 * x + a3 * y, a4 * x + a5 * y, &v)) f = v;
         }
-        F[u * ldf + p] = f;
+        if (SPLIT) {
+            frow[2 * p] = f.x;
+            frow[2 * p + 1] = f.y;
+            mx = fmaxf(mx, fmaxf(fabsf(f.x), fabsf(f.y)));
+            F2[u * ldf + p] = f.x * f.x + f.y * f.y;   // |F|^2 for the n1 term
+        } else if (p < npix) {
+            F[u * ldf + p] = f;
+        }
+    }
+    if (SPLIT) {
+        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+        if ((threadIdx.x & 63) == 0) redf[threadIdx.x >> 6] = mx;
+        __syncthreads();
+        float sc = 1.f;
+        if (use_scale) {
+            mx = fmaxf(fmaxf(redf[0], redf[1]), fmaxf(redf[2], redf[3]));
+            int e = 0;
+            if (mx > 0.f) (void)frexpf(mx, &e);
+            sc = ldexpf(1.f, -e);
+            if (threadIdx.x == 0) fs[u] = ldexpf(1.f, e);
+        } else if (threadIdx.x == 0) {
+            fs[u] = 1.f;
+        }
+        const int Kc = 2 * ldf;
+        for (int k = threadIdx.x; k < Kc; k += blockDim.x) {
+            const float v = frow[k] * sc;
+            const H h = (H)v;
+            const H l = (H)(v - (float)h);
+            fh[u * Kc + k] = h;
+            fl[u * Kc + k] = l;
+        }
     }
 }
 
-template <class H, int SPLIT>
+// n1 of every (particle, orientation) row: 1/2 sum_p minv ctf^2 |F|^2 = sum_p A1_b,p |F_u,p|^2 in
+// FP64 (one wave per row).
+__global__ __launch_bounds__(256) void k_local_n1(int B, long long nrows, const long long* __restrict__ orient_off,
+                                                  const int* __restrict__ rowu, const float* __restrict__ F2, int ldf,
+                                                  const float* __restrict__ A1, long long lda1, int npix,
+                                                  double* __restrict__ n1out)
+{
+    const long long j = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (j >= nrows) return;
+    const int b = find_particle(orient_off, B, j);
+    const float* f2 = F2 + (long long)rowu[j] * ldf;
+    const float* a1 = A1 + (long long)b * lda1;
+    double acc = 0.;
+    for (int p = lane; p < npix; p += 64) acc += (double)a1[p] * (double)f2[p];
+    for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
+    if (lane == 0) n1out[j] = acc;
+}
+
 __global__ __launch_bounds__(256) void k_local_zrows(int B, long long nrows, const long long* __restrict__ orient_off,
                                                      const int* __restrict__ rowu, const float2* __restrict__ F,
                                                      int ldf, const float* __restrict__ minv,
                                                      const int* __restrict__ win, int npix,
                                                      const float2* __restrict__ Fimg,
                                                      const float* __restrict__ Fctf, int Pw, int Kc,
-                                                     float* __restrict__ zf, H* __restrict__ zh, H* __restrict__ zl,
-                                                     float* __restrict__ zs, double* __restrict__ n1out)
+                                                     float* __restrict__ zf, double* __restrict__ n1out)
 {
-    extern __shared__ float zrow[];   // [Kc] (split operand only)
     __shared__ double red[4];
-    __shared__ float redf[4];
     // XCD-aware row order: workgroups are dealt round-robin to the 8 XCDs, so XCD k is given the
-    // contiguous rows [k R/8, (k+1) R/8) (the grid is a multiple of 8). Consecutive rows of a
-    // particle are psi rotations of one direction, i.e. the same central plane of the volume,
-    // whose voxels then stay in that XCD's L2.
+    // contiguous rows [k R/8, (k+1) R/8) (the grid is a multiple of 8)
     const long long per = (long long)gridDim.x / 8;
     const long long j = (long long)(blockIdx.x % 8) * per + blockIdx.x / 8;
     const bool valid = j < nrows;
-    float mx = 0.f;
     double n1 = 0.;
     if (valid) {
         const int b = find_particle(orient_off, B, j);
@@ -551,38 +603,16 @@ __global__ __launch_bounds__(256) void k_local_zrows(int B, long long nrows, con
                 zi = f.y * yr - f.x * yi;
                 n1 += 0.5 * (double)m * (double)ctf * ctf * ((double)f.x * f.x + (double)f.y * f.y);
             }
-            if (SPLIT) {
-                zrow[2 * p] = zr;
-                zrow[2 * p + 1] = zi;
-                mx = fmaxf(mx, fmaxf(fabsf(zr), fabsf(zi)));
-            } else {
-                zf[j * Kc + 2 * p] = zr;
-                zf[j * Kc + 2 * p + 1] = zi;
-            }
+            zf[j * Kc + 2 * p] = zr;
+            zf[j * Kc + 2 * p + 1] = zi;
         }
-    } else if (!SPLIT) {
+    } else {
         for (int k = threadIdx.x; k < Kc; k += blockDim.x) zf[j * Kc + k] = 0.f;
     }
     for (int o = 32; o > 0; o >>= 1) n1 += __shfl_xor(n1, o);
-    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-    if ((threadIdx.x & 63) == 0) { red[threadIdx.x >> 6] = n1; redf[threadIdx.x >> 6] = mx; }
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n1;
     __syncthreads();
     if (valid && threadIdx.x == 0) n1out[j] = red[0] + red[1] + red[2] + red[3];
-    if (SPLIT) {
-        // rows beyond nrows are zero (their LDS row is never written: write zeros directly)
-        mx = fmaxf(fmaxf(redf[0], redf[1]), fmaxf(redf[2], redf[3]));
-        int e = 0;
-        if (mx > 0.f) (void)frexpf(mx, &e);
-        const float sc = ldexpf(1.f, -e);
-        if (threadIdx.x == 0) zs[j] = ldexpf(1.f, e);
-        for (int k = threadIdx.x; k < Kc; k += blockDim.x) {
-            const float v = valid ? zrow[k] * sc : 0.f;
-            const H h = (H)v;
-            const H l = (H)(v - (float)h);
-            zh[j * Kc + k] = h;
-            zl[j * Kc + k] = l;
-        }
-    }
 }
 
 // The phase table as the GEMM's shared operand: row t = [re, im] of AB_t over the coarse pixels,
@@ -1387,10 +1417,10 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     std::vector<float2> abf = phase_table(tf, S.fine_tx_px, S.fine_ty_px, N);
     T2X_TRY(upload(d->ab_coarse, abc, st));
     T2X_TRY(upload(d->ab_fine, abf, st));
-    // local searches: the phase table as the coarse GEMM's shared operand (Tl rows, Kc columns)
+    // local searches in FP32: the phase table as the coarse GEMM's shared operand (Tl rows, Kc columns)
     const bool local_rows = cfg->orientational_prior_mode == T2X_PRIOR_ROTTILT_PSI && !getenv("T2X_LOCAL_DENSE");
     const int Tl = (int)round_up(T, 128);
-    if (local_rows) {
+    if (local_rows && cfg->precision == T2X_PREC_FP32) {   // the split modes take the image side instead
         const int Kcl = tc.npix_pad * 2;
         if (sizeof(float) * Kcl > 65536) {
             set_error("coarse image too large for the local-search operand kernel (2 P_c > 16384)");
@@ -1399,13 +1429,6 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         T2X_TRY(d->lc_ab.ensure(sizeof(float) * Tl * Kcl));
         k_ab_rows<<<Tl, 256, 0, st>>>(d->ab_coarse.as<float2>(), (int)T, tc.npix, Kcl, d->lc_ab.as<float>());
         T2X_HIP(hipGetLastError());
-        if (cfg->precision != T2X_PREC_FP32) {
-            T2X_TRY(d->lc_abh.ensure(sizeof(uint16_t) * Tl * Kcl));
-            T2X_TRY(d->lc_abl.ensure(sizeof(uint16_t) * Tl * Kcl));
-            T2X_TRY(d->lc_abs.ensure(sizeof(float) * Tl));
-            T2X_TRY(launch_split(cfg->precision, d->lc_ab.as<float>(), Kcl, Kcl, Tl, (int)T, d->lc_abh.ptr,
-                                 d->lc_abl.ptr, Kcl, d->lc_abs.as<float>(), st));
-        }
     }
     // orientation prior (NOPRIOR): pdf_direction / mean over all classes and orientations;
     // 0 where the class or the direction has zero probability (never scored). Local searches:
@@ -1767,58 +1790,94 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             T2X_TRY(upload(d->lc_ulist, ulist, st));
             T2X_TRY(upload(d->lc_rowu, rowu, st));
             const int ldf = tc.npix_pad;
-            T2X_TRY(d->lc_F.ensure(sizeof(float2) * std::max<long long>(nu, 1) * ldf));
+            if (prec == T2X_PREC_FP32) T2X_TRY(d->lc_F.ensure(sizeof(float2) * std::max<long long>(nu, 1) * ldf));
+            else T2X_TRY(d->lc_F2.ensure(sizeof(float) * std::max<long long>(nu, 1) * ldf));
             (void)hipEventRecord(d->ev[ev++], st);   // 2: coarse diff2 start
             if (norients_c > 0) {
-                k_project_list<<<(unsigned)round_up(nu, 8), 256, 0, st>>>(
-                    nu, d->lc_ulist.as<int>(), gc, d_vols.as<const float2*>(), d_pads.as<int>(), tc.d_pix.as<int2>(),
-                    tc.d_inproj.as<unsigned char>(), tc.npix, ldf, d->lc_F.as<float2>());
+                const unsigned ugrid = (unsigned)round_up(nu, 8);
+                if (prec == T2X_PREC_FP32) {
+                    k_project_list<_Float16, 0><<<ugrid, 256, 0, st>>>(
+                        nu, d->lc_ulist.as<int>(), gc, d_vols.as<const float2*>(), d_pads.as<int>(), tc.d_pix.as<int2>(),
+                        tc.d_inproj.as<unsigned char>(), tc.npix, ldf, d->lc_F.as<float2>(), nullptr, nullptr, nullptr,
+                        0, nullptr);
+                } else {
+                    T2X_TRY(d->lc_fh.ensure(sizeof(uint16_t) * nu * Kc));
+                    T2X_TRY(d->lc_fl.ensure(sizeof(uint16_t) * nu * Kc));
+                    T2X_TRY(d->lc_fs.ensure(sizeof(float) * nu));
+                    const size_t lds = sizeof(float) * Kc;
+                    if (prec == T2X_PREC_BF16X3)
+                        k_project_list<__bf16, 1><<<ugrid, 256, lds, st>>>(
+                            nu, d->lc_ulist.as<int>(), gc, d_vols.as<const float2*>(), d_pads.as<int>(),
+                            tc.d_pix.as<int2>(), tc.d_inproj.as<unsigned char>(), tc.npix, ldf, nullptr,
+                            (__bf16*)d->lc_fh.ptr, (__bf16*)d->lc_fl.ptr, d->lc_fs.as<float>(), 0, d->lc_F2.as<float>());
+                    else
+                        k_project_list<_Float16, 1><<<ugrid, 256, lds, st>>>(
+                            nu, d->lc_ulist.as<int>(), gc, d_vols.as<const float2*>(), d_pads.as<int>(),
+                            tc.d_pix.as<int2>(), tc.d_inproj.as<unsigned char>(), tc.npix, ldf, nullptr,
+                            (_Float16*)d->lc_fh.ptr, (_Float16*)d->lc_fl.ptr, d->lc_fs.as<float>(), 1, d->lc_F2.as<float>());
+                }
                 T2X_HIP(hipGetLastError());
                 // rows = the batch's (particle, orientation) pairs, columns = translations
                 const long long rows = norients_c, rows_pad = round_up(rows, 128);
-                EpiParams eg{};
                 T2X_TRY(d->lc_n1.ensure(sizeof(double) * rows));
-                T2X_TRY(d->lc_G.ensure(sizeof(float) * rows_pad * Tl));
-                eg.C = d->lc_G.as<float>();
-                eg.ldc = Tl;
-                eg.M = (int)rows;
-                eg.N = (int)T;
                 if (prec == T2X_PREC_FP32) {
+                    // FP32: (particle, orientation) operand rows against the phase table
+                    EpiParams eg{};
+                    T2X_TRY(d->lc_G.ensure(sizeof(float) * rows_pad * Tl));
+                    eg.C = d->lc_G.as<float>();
+                    eg.ldc = Tl;
+                    eg.M = (int)rows;
+                    eg.N = (int)T;
                     T2X_TRY(d->lc_z.ensure(sizeof(float) * rows_pad * Kc));
-                    k_local_zrows<float, 0><<<(unsigned)rows_pad, 256, 0, st>>>(
+                    k_local_zrows<<<(unsigned)rows_pad, 256, 0, st>>>(
                         B, rows, d->lc_orient_off.as<long long>(), d->lc_rowu.as<int>(), d->lc_F.as<float2>(), ldf,
                         tc.d_minv.as<float>(), tc.d_win.as<int>(), tc.npix,
-                        d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, Kc, d->lc_z.as<float>(), nullptr,
-                        nullptr, nullptr, d->lc_n1.as<double>());
+                        d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, Kc, d->lc_z.as<float>(),
+                        d->lc_n1.as<double>());
                     T2X_HIP(hipGetLastError());
                     T2X_TRY(launch_gemm(EPI_STORE, d->lc_z.as<float>(), Kc, d->lc_ab.as<float>(), Kc, Kc,
                                         (int)rows_pad, Tl, eg, st));
+                    k_local_diff2<<<(unsigned)rows, 128, 0, st>>>(B, rows, d->lc_orient_off.as<long long>(), d_ps,
+                                                                 d->lc_n1.as<double>(), d->lc_G.as<float>(), Tl, (int)T,
+                                                                 d->lc_d2.as<float>(), d->lc_min.as<unsigned int>());
+                    T2X_HIP(hipGetLastError());
                 } else {
-                    T2X_TRY(d->lc_zh.ensure(sizeof(uint16_t) * rows_pad * Kc));
-                    T2X_TRY(d->lc_zl.ensure(sizeof(uint16_t) * rows_pad * Kc));
-                    T2X_TRY(d->lc_zs.ensure(sizeof(float) * rows_pad));
+                    // grouped split GEMM: each particle's slices (gathered) x its translation rows
+                    // the image side, 128 readable rows past the last particle
+                    const long long wrows = round_up(M + 128, 128);
+                    T2X_TRY(d->Ah.ensure(sizeof(uint16_t) * wrows * Kc));
+                    T2X_TRY(d->Al.ensure(sizeof(uint16_t) * wrows * Kc));
+                    T2X_TRY(d->As.ensure(sizeof(float) * wrows));
+                    T2X_TRY(d->A1.ensure(sizeof(float) * B * Kn));
                     const size_t lds = sizeof(float) * Kc;
                     if (prec == T2X_PREC_BF16X3)
-                        k_local_zrows<__bf16, 1><<<(unsigned)rows_pad, 256, lds, st>>>(
-                            B, rows, d->lc_orient_off.as<long long>(), d->lc_rowu.as<int>(), d->lc_F.as<float2>(),
-                            ldf, tc.d_minv.as<float>(), tc.d_win.as<int>(), tc.npix,
-                            d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, Kc, nullptr,
-                            (__bf16*)d->lc_zh.ptr, (__bf16*)d->lc_zl.ptr, d->lc_zs.as<float>(), d->lc_n1.as<double>());
+                        k_coarse_imgside_split<__bf16><<<(unsigned)wrows, 256, lds, st>>>(
+                            d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, tc.d_win.as<int>(),
+                            tc.d_minv.as<float>(), tc.npix, d->ab_coarse.as<float2>(), (int)T, (int)M, Kc,
+                            (__bf16*)d->Ah.ptr, (__bf16*)d->Al.ptr, d->As.as<float>(), 0, d->A1.as<float>(), Kn);
                     else
-                        k_local_zrows<_Float16, 1><<<(unsigned)rows_pad, 256, lds, st>>>(
-                            B, rows, d->lc_orient_off.as<long long>(), d->lc_rowu.as<int>(), d->lc_F.as<float2>(),
-                            ldf, tc.d_minv.as<float>(), tc.d_win.as<int>(), tc.npix,
-                            d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, Kc, nullptr,
-                            (_Float16*)d->lc_zh.ptr, (_Float16*)d->lc_zl.ptr, d->lc_zs.as<float>(), d->lc_n1.as<double>());
+                        k_coarse_imgside_split<_Float16><<<(unsigned)wrows, 256, lds, st>>>(
+                            d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, tc.d_win.as<int>(),
+                            tc.d_minv.as<float>(), tc.npix, d->ab_coarse.as<float2>(), (int)T, (int)M, Kc,
+                            (_Float16*)d->Ah.ptr, (_Float16*)d->Al.ptr, d->As.as<float>(), 1, d->A1.as<float>(), Kn);
                     T2X_HIP(hipGetLastError());
-                    T2X_TRY(launch_gemm_x3(prec, EPI_STORE, d->lc_zh.ptr, d->lc_zl.ptr, d->lc_zs.as<float>(), Kc, rows_pad,
-                                           d->lc_abh.ptr, d->lc_abl.ptr, d->lc_abs.as<float>(), Kc, Kc, (int)rows_pad, Tl,
-                                           eg, st, GemmChoice{128, 64}, nullptr, nullptr));
+                    k_local_n1<<<(unsigned)((rows + 3) / 4), 256, 0, st>>>(
+                        B, rows, d->lc_orient_off.as<long long>(), d->lc_rowu.as<int>(), d->lc_F2.as<float>(), ldf,
+                        d->A1.as<float>(), Kn, tc.npix, d->lc_n1.as<double>());
+                    T2X_HIP(hipGetLastError());
+                    std::vector<int2> items;
+                    for (int i = 0; i < B; i++)
+                        for (long long s0 = 0; s0 < coff[i + 1] - coff[i]; s0 += 128) items.push_back(make_int2(i, (int)s0));
+                    T2X_TRY(upload(d->lc_items, items, st));
+                    T2X_TRY(d->tables.ensure(sizeof(double) * B));
+                    T2X_HIP(hipMemcpy2DAsync(d->tables.ptr, sizeof(double), (char*)d_ps + offsetof(PartScalars, C_coarse),
+                                             sizeof(PartScalars), sizeof(double), B, hipMemcpyDeviceToDevice, st));
+                    T2X_TRY(launch_gemm_local_x3(prec, d->lc_fh.ptr, d->lc_fl.ptr, d->lc_fs.as<float>(), d->Ah.ptr,
+                                                 d->Al.ptr, d->As.as<float>(), Kc, Kc, d->lc_items.as<int2>(),
+                                                 (int)items.size(), (int)T, d->lc_orient_off.as<long long>(),
+                                                 d->lc_rowu.as<int>(), d->tables.as<double>(), d->lc_n1.as<double>(),
+                                                 d->lc_d2.as<float>(), d->lc_min.as<unsigned int>(), st));
                 }
-                k_local_diff2<<<(unsigned)rows, 128, 0, st>>>(B, rows, d->lc_orient_off.as<long long>(), d_ps,
-                                                             d->lc_n1.as<double>(), d->lc_G.as<float>(), Tl, (int)T,
-                                                             d->lc_d2.as<float>(), d->lc_min.as<unsigned int>());
-                T2X_HIP(hipGetLastError());
             }
             (void)hipEventRecord(d->ev[ev++], st);   // 3: coarse diff2 end
             wc.kind = SRC_FINE;

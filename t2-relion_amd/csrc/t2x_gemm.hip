@@ -986,6 +986,148 @@ __global__ __launch_bounds__(256) void k_gemm_x3_bins(SplitOperand A, SplitOpera
     }
 }
 
+// Local searches: grouped split-precision GEMM. Work item (b, s0) multiplies 128 of particle b's
+// in-range slices (rows s0.., gathered from the batch's slice operand through rowu) by the
+// particle's T translation rows of the image-side operand (columns; T <= 128, rows b*T..b*T+127,
+// the operand holding at least 128 readable rows past the last particle), so no per-(particle,
+// orientation) operand is formed. Epilogue: diff2 = C_b + n1 - G in FP64 rounded once, stored in
+// the sample list (orientation-major, T per orientation), and the per-particle minimum.
+template <class H>
+__global__ __launch_bounds__(256) void k_gemm_local_x3(SplitOperand S, SplitOperand W, int K, const int2* __restrict__ items,
+                                                       int T, const long long* __restrict__ orient_off,
+                                                       const int* __restrict__ rowu, const double* __restrict__ cb,
+                                                       const double* __restrict__ n1, float* __restrict__ d2,
+                                                       unsigned int* __restrict__ dmin)
+{
+    typedef typename Vec8<H>::type v8;
+    __shared__ __attribute__((aligned(16))) H Ash[GBM][XK + XPAD];
+    __shared__ __attribute__((aligned(16))) H Asl[GBM][XK + XPAD];
+    __shared__ __attribute__((aligned(16))) H Bsh[GBN][XK + XPAD];
+    __shared__ __attribute__((aligned(16))) H Bsl[GBN][XK + XPAD];
+    __shared__ float wmin[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const int2 it = items[blockIdx.x];
+    const int b = it.x, s0 = it.y;
+    const long long o0 = orient_off[b];
+    const int ncols = (int)(orient_off[b + 1] - o0);
+    const long long w0 = (long long)b * T;
+
+    floatx4 acc[4][4];
+    for (int i = 0; i < 4; i++)
+        for (int j = 0; j < 4; j++) acc[i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    const int lrow = tid >> 2, lc = (tid & 3) * 8;
+    const H* Sh = reinterpret_cast<const H*>(S.hi);
+    const H* Sl = reinterpret_cast<const H*>(S.lo);
+    const H* Wh = reinterpret_cast<const H*>(W.hi);
+    const H* Wl = reinterpret_cast<const H*>(W.lo);
+    const int sa = s0 + lrow, sb = s0 + lrow + 64;
+    const long long ua = sa < ncols ? rowu[o0 + sa] : 0, ub = sb < ncols ? rowu[o0 + sb] : 0;
+    const size_t ra0 = (size_t)ua * S.ld + lc, ra1 = (size_t)ub * S.ld + lc;
+    const size_t rb0 = (size_t)(w0 + lrow) * W.ld + lc, rb1 = (size_t)(w0 + lrow + 64) * W.ld + lc;
+    uint4 pa0h = *reinterpret_cast<const uint4*>(Sh + ra0), pa1h = *reinterpret_cast<const uint4*>(Sh + ra1);
+    uint4 pa0l = *reinterpret_cast<const uint4*>(Sl + ra0), pa1l = *reinterpret_cast<const uint4*>(Sl + ra1);
+    uint4 pb0h = *reinterpret_cast<const uint4*>(Wh + rb0), pb1h = *reinterpret_cast<const uint4*>(Wh + rb1);
+    uint4 pb0l = *reinterpret_cast<const uint4*>(Wl + rb0), pb1l = *reinterpret_cast<const uint4*>(Wl + rb1);
+
+    for (int k0 = 0; k0 < K; k0 += XK) {
+        __syncthreads();
+        *reinterpret_cast<uint4*>(&Ash[lrow][lc]) = pa0h;
+        *reinterpret_cast<uint4*>(&Ash[lrow + 64][lc]) = pa1h;
+        *reinterpret_cast<uint4*>(&Asl[lrow][lc]) = pa0l;
+        *reinterpret_cast<uint4*>(&Asl[lrow + 64][lc]) = pa1l;
+        *reinterpret_cast<uint4*>(&Bsh[lrow][lc]) = pb0h;
+        *reinterpret_cast<uint4*>(&Bsh[lrow + 64][lc]) = pb1h;
+        *reinterpret_cast<uint4*>(&Bsl[lrow][lc]) = pb0l;
+        *reinterpret_cast<uint4*>(&Bsl[lrow + 64][lc]) = pb1l;
+        __syncthreads();
+        if (k0 + XK < K) {
+            const int kn = k0 + XK;
+            pa0h = *reinterpret_cast<const uint4*>(Sh + ra0 + kn);
+            pa1h = *reinterpret_cast<const uint4*>(Sh + ra1 + kn);
+            pa0l = *reinterpret_cast<const uint4*>(Sl + ra0 + kn);
+            pa1l = *reinterpret_cast<const uint4*>(Sl + ra1 + kn);
+            pb0h = *reinterpret_cast<const uint4*>(Wh + rb0 + kn);
+            pb1h = *reinterpret_cast<const uint4*>(Wh + rb1 + kn);
+            pb0l = *reinterpret_cast<const uint4*>(Wl + rb0 + kn);
+            pb1l = *reinterpret_cast<const uint4*>(Wl + rb1 + kn);
+        }
+        const int kc = (lane >> 4) * 8;
+        v8 ah[4], al[4], bh[4], bl[4];
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            ah[i] = *reinterpret_cast<const v8*>(&Ash[wm * 64 + i * 16 + (lane & 15)][kc]);
+            al[i] = *reinterpret_cast<const v8*>(&Asl[wm * 64 + i * 16 + (lane & 15)][kc]);
+        }
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            bh[j] = *reinterpret_cast<const v8*>(&Bsh[wn * 64 + j * 16 + (lane & 15)][kc]);
+            bl[j] = *reinterpret_cast<const v8*>(&Bsl[wn * 64 + j * 16 + (lane & 15)][kc]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = mfma16x16x32<H>(al[i], bh[j], acc[i][j]);
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = mfma16x16x32<H>(ah[i], bl[j], acc[i][j]);
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[i][j] = mfma16x16x32<H>(ah[i], bh[j], acc[i][j]);
+        }
+    }
+
+    // epilogue: lane holds C[4*(lane/16)+r][lane%16] of each 16x16 tile; rows = slices,
+    // columns = translations (16 consecutive translations per 16 lanes: coalesced stores)
+    const double c_b = cb[b];
+    float m = 3.0e38f;
+#pragma unroll
+    for (int i = 0; i < 4; i++) {
+#pragma unroll
+        for (int r = 0; r < 4; r++) {
+            const int s = s0 + wm * 64 + i * 16 + (lane >> 4) * 4 + r;
+            if (s >= ncols) continue;
+            const long long row = o0 + s;
+            const float ss = S.inv_scale ? S.inv_scale[rowu[row]] : 1.f;
+            const double base = c_b + n1[row];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const int t = wn * 64 + j * 16 + (lane & 15);
+                if (t >= T) continue;
+                const float ws = W.inv_scale ? W.inv_scale[w0 + t] : 1.f;
+                const float g = acc[i][j][r] * ss * ws;
+                const float d = (float)(base - (double)g);
+                d2[row * T + t] = d;
+                m = fminf(m, d);
+            }
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) m = fminf(m, __shfl_xor(m, o));
+    if (lane == 0) wmin[wave] = m;
+    __syncthreads();
+    if (tid == 0) {
+        const float mm = fminf(fminf(wmin[0], wmin[1]), fminf(wmin[2], wmin[3]));
+        if (mm < 3.0e38f) atomicMin(&dmin[b], float_to_ordered(mm));
+    }
+}
+
+int launch_gemm_local_x3(int prec, const void* Sh, const void* Sl, const float* Ss, const void* Wh, const void* Wl,
+                         const float* Ws, long long ld, int K, const int2* items, int n_items, int T,
+                         const long long* orient_off, const int* rowu, const double* cb, const double* n1,
+                         float* d2, unsigned int* dmin, hipStream_t st)
+{
+    if (K % XK != 0 || T > GBN || n_items <= 0) {
+        set_error("gemm_local: K not a multiple of 32, more than 128 translations, or no work");
+        return T2X_EINVAL;
+    }
+    SplitOperand S{Sh, Sl, Ss, ld}, W{Wh, Wl, Ws, ld};
+    if (prec == T2X_PREC_BF16X3)
+        k_gemm_local_x3<__bf16><<<n_items, 256, 0, st>>>(S, W, K, items, T, orient_off, rowu, cb, n1, d2, dmin);
+    else
+        k_gemm_local_x3<_Float16><<<n_items, 256, 0, st>>>(S, W, K, items, T, orient_off, rowu, cb, n1, d2, dmin);
+    T2X_HIP(hipGetLastError());
+    return T2X_OK;
+}
+
 int launch_gemm_x3_bins(int prec, const void* Ah, const void* Al, const float* As, long long lda, const void* Bh,
                         const void* Bl, const float* Bs, long long ldb, int K, const int* plist, const void* work,
                         int n_work, int T, long long n_clamp, const EpiParams& ep, hipStream_t st)

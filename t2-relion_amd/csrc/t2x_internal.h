@@ -150,7 +150,8 @@ struct Device {
     DevBuf loc_dir, loc_psi, loc_mean, loc_ang, loc_tab, loc_list, loc_nsel;   // local searches
     // coarse pass of a local search on per-particle sample lists
     DevBuf lc_orients, lc_pairs, lc_d2, lc_min, lc_pair_off, lc_orient_off, lc_cls, lc_dirsc, lc_psisc,
-        lc_n1, lc_G, lc_z, lc_zh, lc_zl, lc_zs, lc_ab, lc_abh, lc_abl, lc_abs, lc_ulist, lc_rowu, lc_F;
+        lc_n1, lc_G, lc_z, lc_ab, lc_ulist, lc_rowu, lc_F, lc_F2,
+        lc_fh, lc_fl, lc_fs, lc_items;
     std::vector<int> lc_umap;   // orientation -> slice of the batch (-1 between batches)
     // per-call tables and per-batch offsets (grow-only, so no allocation per call or batch)
     DevBuf minv_fd, dirsc, psisc, ftx, fty, vols_tab, pads_tab, bpd_tab, bpw_tab, bpads_tab, bprmax_tab,

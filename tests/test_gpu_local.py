@@ -123,13 +123,22 @@ def test_sample_lists_equal_dense_masked_pass(gpu_device, monkeypatch):
         assert H.rel_norm(a["bp"][c][1], b["bp"][c][1]) < 1e-5
 
 
-def test_local360_shapes(gpu_device):
+@pytest.fixture(scope="module")
+def local360_case():
+    wl = H.Workload("local360", n=3, overrides=dict(n_blobs=60))
+    return wl, wl.run_oracle()
+
+
+@pytest.mark.parametrize("prec", [0, 2])
+def test_local360_shapes(local360_case, gpu_device, prec):
     """The local-refinement configuration's own shapes (360^2, HEALPix 5 = 2.36 M orientations,
     sigma_ang 3 deg, 113 translations, P_c 5,007) on 3 particles; 60-blob references keep the
-    oracle's reference synthesis short (the shapes do not depend on the blob count)."""
-    wl = H.Workload("local360", n=3, overrides=dict(n_blobs=60))
-    o = wl.run_oracle()
+    oracle's reference synthesis short (the shapes do not depend on the blob count). FP32: the
+    operand-row GEMM; FP16x3: the grouped GEMM of gathered slices."""
+    wl, o = local360_case
+    wl.cfg.precision = prec
     g = wl.run_gpu(dev=gpu_device)
+    wl.cfg.precision = 0
     assert np.array_equal(o["out"]["best_hidden"], g["out"]["best_hidden"])
     assert H.rel(g["out"]["dLL"], o["out"]["dLL"]) < 1e-5
     for k, v in o["wsum"].items():
