@@ -19,6 +19,7 @@
 #include "t2x_cnn.h"
 
 #include <chrono>
+#include <hipcub/hipcub.hpp>
 
 namespace t2x {
 
@@ -1164,6 +1165,28 @@ __global__ void k_local_mask_n1(float* __restrict__ N1, long long ldn, int B, lo
     if (!local_in_range(lp, b, n)) N1[(long long)b * ldn + n] = __builtin_inff();
 }
 
+// The batch's distinct orientations in increasing order: flags over every orientation, an
+// exclusive scan (hipCUB), the compacted list and each row's slice index.
+__global__ void k_mark_orients(long long nrows, const int* __restrict__ orients, int* __restrict__ flags)
+{
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nrows) flags[orients[j]] = 1;
+}
+
+__global__ void k_compact_orients(long long n, const int* __restrict__ flags, const int* __restrict__ pos,
+                                  int* __restrict__ ulist)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n && flags[i]) ulist[pos[i]] = (int)i;
+}
+
+__global__ void k_row_slices(long long nrows, const int* __restrict__ orients, const int* __restrict__ pos,
+                             int* __restrict__ rowu)
+{
+    const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j < nrows) rowu[j] = pos[orients[j]];
+}
+
 __global__ void k_items_from_orients(const long long* orient_off, int B, int over_rot, long long* item_off)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1773,22 +1796,41 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             gc.pf = pf;
             gc.ref2d = ref2d;
             // the batch's distinct orientations (particles sorted by prior orientation share many)
-            std::vector<int> horients(std::max<long long>(norients_c, 1));
-            T2X_HIP(hipMemcpyAsync(horients.data(), d->lc_orients.ptr, sizeof(int) * norients_c, hipMemcpyDeviceToHost, st));
-            T2X_HIP(hipStreamSynchronize(st));
-            if ((long long)d->lc_umap.size() < O_all) d->lc_umap.assign(O_all, -1);
-            std::vector<int> ulist, rowu(std::max<long long>(norients_c, 1));
-            for (long long r = 0; r < norients_c; r++) {
-                int& u = d->lc_umap[horients[r]];
-                if (u < 0) { u = (int)ulist.size(); ulist.push_back(horients[r]); }
-                rowu[r] = u;
+            // distinct orientations of the batch (increasing order: a direction's psi rotations,
+            // which share the volume's central plane, are adjacent) and each row's slice
+            long long nu = 0;
+            if (norients_c > 0) {
+                T2X_TRY(d->lc_flags.ensure(sizeof(int) * (O_all + 1)));
+                T2X_TRY(d->lc_pos.ensure(sizeof(int) * (O_all + 1)));
+                k_fill_u32<<<(unsigned)((O_all + 1 + 255) / 256), 256, 0, st>>>(d->lc_flags.as<unsigned>(), 0u, (int)(O_all + 1));
+                k_mark_orients<<<(unsigned)((norients_c + 255) / 256), 256, 0, st>>>(norients_c, d->lc_orients.as<int>(),
+                                                                                   d->lc_flags.as<int>());
+                size_t tmp = 0;
+                if (hipcub::DeviceScan::ExclusiveSum(nullptr, tmp, d->lc_flags.as<int>(), d->lc_pos.as<int>(),
+                                                     (int)(O_all + 1), st) != hipSuccess) {
+                    set_error("hipcub scan size query failed");
+                    return T2X_EDEVICE;
+                }
+                T2X_TRY(d->lc_scan_tmp.ensure(std::max<size_t>(tmp, 1)));
+                if (hipcub::DeviceScan::ExclusiveSum(d->lc_scan_tmp.ptr, tmp, d->lc_flags.as<int>(), d->lc_pos.as<int>(),
+                                                     (int)(O_all + 1), st) != hipSuccess) {
+                    set_error("hipcub scan failed");
+                    return T2X_EDEVICE;
+                }
+                int hnu = 0;   // pos[O_all]: the number of distinct orientations (flags[O_all] = 0)
+                T2X_HIP(hipMemcpyAsync(&hnu, d->lc_pos.as<int>() + O_all, sizeof(int), hipMemcpyDeviceToHost, st));
+                T2X_HIP(hipStreamSynchronize(st));
+                nu = hnu;
+                T2X_TRY(d->lc_ulist.ensure(sizeof(int) * std::max<long long>(nu, 1)));
+                T2X_TRY(d->lc_rowu.ensure(sizeof(int) * norients_c));
+                k_compact_orients<<<(unsigned)((O_all + 255) / 256), 256, 0, st>>>(O_all, d->lc_flags.as<int>(),
+                                                                                 d->lc_pos.as<int>(), d->lc_ulist.as<int>());
+                k_row_slices<<<(unsigned)((norients_c + 255) / 256), 256, 0, st>>>(norients_c, d->lc_orients.as<int>(),
+                                                                                 d->lc_pos.as<int>(), d->lc_rowu.as<int>());
+                T2X_HIP(hipGetLastError());
             }
-            for (int n : ulist) d->lc_umap[n] = -1;
-            const long long nu = (long long)ulist.size();
             stt.local_slices += (double)nu;
             stt.local_rows += (double)norients_c;
-            T2X_TRY(upload(d->lc_ulist, ulist, st));
-            T2X_TRY(upload(d->lc_rowu, rowu, st));
             const int ldf = tc.npix_pad;
             if (prec == T2X_PREC_FP32) T2X_TRY(d->lc_F.ensure(sizeof(float2) * std::max<long long>(nu, 1) * ldf));
             else T2X_TRY(d->lc_F2.ensure(sizeof(float) * std::max<long long>(nu, 1) * ldf));

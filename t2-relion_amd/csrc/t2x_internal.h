@@ -151,8 +151,7 @@ struct Device {
     // coarse pass of a local search on per-particle sample lists
     DevBuf lc_orients, lc_pairs, lc_d2, lc_min, lc_pair_off, lc_orient_off, lc_cls, lc_dirsc, lc_psisc,
         lc_n1, lc_G, lc_z, lc_ab, lc_ulist, lc_rowu, lc_F, lc_F2,
-        lc_fh, lc_fl, lc_fs, lc_items;
-    std::vector<int> lc_umap;   // orientation -> slice of the batch (-1 between batches)
+        lc_fh, lc_fl, lc_fs, lc_items, lc_flags, lc_pos, lc_scan_tmp;
     // per-call tables and per-batch offsets (grow-only, so no allocation per call or batch)
     DevBuf minv_fd, dirsc, psisc, ftx, fty, vols_tab, pads_tab, bpd_tab, bpw_tab, bpads_tab, bprmax_tab,
         ctx, cty, ctfc, pair_off_d, orient_off_d, item_off_d;
