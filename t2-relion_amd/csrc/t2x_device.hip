@@ -100,11 +100,16 @@ static int check_dims(int iclass, int ref_dim, int pad, int r_max, double pf)
     return T2X_OK;
 }
 
+// Interleaves re/im into the projector layout: a 2D plane row-major, a 3D volume in 4x4x4 bricks.
 __global__ void k_interleave(const float* __restrict__ re, const float* __restrict__ im,
-                             float2* __restrict__ out, size_t n)
+                             float2* __restrict__ out, size_t n, int ref_dim, int pad)
 {
     size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) out[i] = make_float2(re[i], im[i]);
+    if (i >= n) return;
+    if (ref_dim == 2) { out[i] = make_float2(re[i], im[i]); return; }
+    const int xdim = pad / 2 + 1;
+    const int x = (int)(i % xdim), y = (int)((i / xdim) % pad), z = (int)(i / ((size_t)xdim * pad));
+    out[vol_brick_index(x, y, z, (xdim + 3) / 4, (pad + 3) / 4)] = make_float2(re[i], im[i]);
 }
 
 extern "C" int t2x_projector_set(t2x_device* h, int iclass, int ref_dim, int pad_size, int r_max,
@@ -122,7 +127,8 @@ extern "C" int t2x_projector_set(t2x_device* h, int iclass, int ref_dim, int pad
         c.proj = nullptr;
     }
     if (!c.proj) {
-        if (hipMalloc(&c.proj, n * sizeof(float2)) != hipSuccess) {
+        const size_t ne = ref_dim == 3 ? vol_brick_elems(pad_size) : n;
+        if (hipMalloc(&c.proj, ne * sizeof(float2)) != hipSuccess) {
             set_error("hipMalloc projector volume failed");
             return T2X_ENOMEM;
         }
@@ -142,7 +148,7 @@ extern "C" int t2x_projector_set(t2x_device* h, int iclass, int ref_dim, int pad
     }
     T2X_HIP(hipMemcpyAsync(dre, re, n * sizeof(float), hipMemcpyDefault, d->stream));
     T2X_HIP(hipMemcpyAsync(dim, im, n * sizeof(float), hipMemcpyDefault, d->stream));
-    k_interleave<<<(unsigned)((n + 255) / 256), 256, 0, d->stream>>>(dre, dim, c.proj, n);
+    k_interleave<<<(unsigned)((n + 255) / 256), 256, 0, d->stream>>>(dre, dim, c.proj, n, ref_dim, pad_size);
     T2X_HIP(hipGetLastError());
     T2X_HIP(hipStreamSynchronize(d->stream));
     (void)hipFree(dre);

@@ -71,7 +71,9 @@ struct ClassVolumes {
     // projector
     int p_ref_dim = 0, p_pad = 0, p_rmax = 0;
     double p_pf = 0;
-    float2* proj = nullptr;   // pad*pad*(pad/2+1) complex
+    // 3D references: pad x pad x (pad/2+1) complex values in 4x4x4 bricks (vol_brick_index);
+    // 2D references: the pad x (pad/2+1) plane, row-major
+    float2* proj = nullptr;
     uint64_t proj_version = 0;
     // backprojector
     int b_ref_dim = 0, b_pad = 0, b_rmax = 0;
@@ -225,6 +227,20 @@ __host__ __device__ inline void inv3(const double* A, double* R)
 // Trilinear sample of the half-complex projector volume at logical (xp, yp, zp) (already
 // multiplied by the padding factor), with the Friedel flip of projector.cpp:583-640.
 // Returns false where the reference skips the pixel (outside the box).
+// Element (x, y, z) of a projector volume stored in 4x4x4 bricks (x fastest inside a brick,
+// bricks x fastest): the 8 corners of a trilinear sample lie in at most 8 bricks of 512 bytes
+// whatever the slice orientation, instead of 4 rows and 2 planes far apart.
+__host__ __device__ __forceinline__ size_t vol_brick_index(int x, int y, int z, int nbx, int nby)
+{
+    return ((((size_t)(z >> 2) * nby + (y >> 2)) * nbx + (x >> 2)) << 6) + (((z & 3) * 4 + (y & 3)) * 4 + (x & 3));
+}
+
+__host__ __device__ __forceinline__ size_t vol_brick_elems(int pad)
+{
+    const size_t nbx = (pad / 2 + 1 + 3) / 4, nb = (pad + 3) / 4;
+    return nbx * nb * nb * 64;
+}
+
 __device__ __forceinline__ bool trilinear_sample(const float2* __restrict__ vol, int pad, float xp,
                                                  float yp, float zp, float2* out)
 {
@@ -235,10 +251,14 @@ __device__ __forceinline__ bool trilinear_sample(const float2* __restrict__ vol,
     int x0 = (int)fx0, y0 = (int)fy0 + h, z0 = (int)fz0 + h;
     float fx = xp - fx0, fy = yp - fy0, fz = zp - fz0;
     if (x0 < 0 || x0 + 1 >= xdim || y0 < 0 || y0 + 1 >= pad || z0 < 0 || z0 + 1 >= pad) return false;
-    const size_t sy = xdim, sz = (size_t)pad * xdim;
-    const float2* p = vol + (size_t)z0 * sz + (size_t)y0 * sy + x0;
-    float2 d000 = p[0], d001 = p[1], d010 = p[sy], d011 = p[sy + 1];
-    float2 d100 = p[sz], d101 = p[sz + 1], d110 = p[sz + sy], d111 = p[sz + sy + 1];
+    const int nbx = (xdim + 3) / 4, nby = (pad + 3) / 4;
+    float2 d000 = vol[vol_brick_index(x0, y0, z0, nbx, nby)], d001 = vol[vol_brick_index(x0 + 1, y0, z0, nbx, nby)];
+    float2 d010 = vol[vol_brick_index(x0, y0 + 1, z0, nbx, nby)];
+    float2 d011 = vol[vol_brick_index(x0 + 1, y0 + 1, z0, nbx, nby)];
+    float2 d100 = vol[vol_brick_index(x0, y0, z0 + 1, nbx, nby)];
+    float2 d101 = vol[vol_brick_index(x0 + 1, y0, z0 + 1, nbx, nby)];
+    float2 d110 = vol[vol_brick_index(x0, y0 + 1, z0 + 1, nbx, nby)];
+    float2 d111 = vol[vol_brick_index(x0 + 1, y0 + 1, z0 + 1, nbx, nby)];
     float2 dx00 = make_float2(d000.x + (d001.x - d000.x) * fx, d000.y + (d001.y - d000.y) * fx);
     float2 dx01 = make_float2(d100.x + (d101.x - d100.x) * fx, d100.y + (d101.y - d100.y) * fx);
     float2 dx10 = make_float2(d010.x + (d011.x - d010.x) * fx, d010.y + (d011.y - d010.y) * fx);
