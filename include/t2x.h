@@ -406,6 +406,17 @@ typedef struct t2x_recon_params {
 int t2x_reconstruct(t2x_device* dev, const t2x_recon_params* p, const double* data_re,
                     const double* data_im, const double* weight, const double* tau2, double* vol_out);
 
+/* The same reconstruction with its inputs taken from class iclass's device backprojector
+ * (t2x_backprojector_init / the E-step's accumulation) instead of host arrays: replaces the
+ * AccBackprojector::getMdlData copy (acc_backprojector_impl.h:84-98) and its XFLOAT->RFLOAT
+ * widening into wsum_model.BPref[j].data/weight (ml_optimiser.cpp:3483-3503, one GPU, BPref
+ * starting at zero) followed by BackProjector::reconstruct (backprojector.cpp:1344-1770),
+ * without the volumes crossing PCIe. The backprojector's
+ * ref_dim, pad_size, r_max and padding_factor must equal p's. Results equal t2x_reconstruct on
+ * the volumes t2x_backprojector_get returns. */
+int t2x_reconstruct_class(t2x_device* dev, int iclass, const t2x_recon_params* p, const double* tau2,
+                          double* vol_out);
+
 /* Replaces getFSC(MultidimArray<RFLOAT>&, MultidimArray<RFLOAT>&, fsc) (fftw.cpp:481-523): Fourier
  * shell correlation of two n^3 real maps ([z][y][x]); fsc has n/2+1 shells, shell = ROUND(|k|). */
 int t2x_fsc(t2x_device* dev, int n, const double* map1, const double* map2, double* fsc);

@@ -254,6 +254,20 @@ __global__ void k_fsc_final(long long n, const double* part, int nblocks, double
     fsc[s] = num / sqrt(d1 * d2);
 }
 
+// The device backprojector's FP32 complex data and weight (projector layout, as accumulated by the
+// E-step) widened to the FP64 inputs of the reconstruction: the same values the host path reads
+// back into BackProjector::data and ::weight.
+__global__ void k_bp_widen(size_t n, const float2* __restrict__ d, const float* __restrict__ w,
+                           double* __restrict__ re, double* __restrict__ im, double* __restrict__ wo)
+{
+    for (size_t e = (size_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (size_t)gridDim.x * blockDim.x) {
+        const float2 v = d[e];
+        re[e] = v.x;
+        im[e] = v.y;
+        wo[e] = w[e];
+    }
+}
+
 struct FftPlan {
     hipfftHandle h = 0;
     ~FftPlan() { if (h) hipfftDestroy(h); }
@@ -265,10 +279,11 @@ struct FftPlan {
 
 using namespace t2x;
 
-extern "C" int t2x_reconstruct(t2x_device* hdev, const t2x_recon_params* p, const double* data_re,
-                               const double* data_im, const double* weight, const double* tau2, double* vol_out)
+// Shared body of t2x_reconstruct (inputs from host arrays) and t2x_reconstruct_class (inputs from
+// the device backprojector `bp`); the caller holds the device lock.
+static int reconstruct_run(Device* d, const t2x_recon_params* p, const double* data_re, const double* data_im,
+                           const double* weight, const ClassVolumes* bp, const double* tau2, double* vol_out)
 {
-    if (!hdev || !p || !data_re || !data_im || !weight || !vol_out) { set_error("null argument"); return T2X_EINVAL; }
     if (p->ref_dim != 3) { set_error("t2x_reconstruct: only 3D references are supported"); return T2X_EINVAL; }
     if (!p->skip_gridding) {
         set_error("t2x_reconstruct: the iterative gridding branch (--dont_skip_gridding) is not supported");
@@ -280,8 +295,11 @@ extern "C" int t2x_reconstruct(t2x_device* hdev, const t2x_recon_params* p, cons
         return T2X_EINVAL;
     }
     if (p->do_map && !tau2) { set_error("t2x_reconstruct: do_map needs tau2"); return T2X_EINVAL; }
-    Device* d = reinterpret_cast<Device*>(hdev);
-    std::lock_guard<std::mutex> lk(d->mtx);
+    if (bp && (bp->b_ref_dim != p->ref_dim || bp->b_pad != p->pad_size || bp->b_rmax != p->r_max ||
+               bp->b_pf != p->padding_factor)) {
+        set_error("t2x_reconstruct_class: the backprojector's ref_dim/pad_size/r_max/padding_factor differ from the params");
+        return T2X_EINVAL;
+    }
     T2X_HIP(hipSetDevice(d->id));
     hipStream_t st = d->stream;
 
@@ -324,9 +342,15 @@ extern "C" int t2x_reconstruct(t2x_device* hdev, const t2x_recon_params* p, cons
     T2X_TRY(b_vol.ensure(no * 8));
     T2X_TRY(b_bg.ensure(kShellBlocks * 8));
     T2X_TRY(b_bw.ensure(kShellBlocks * 8));
-    T2X_HIP(hipMemcpyAsync(b_re.ptr, data_re, nv * 8, hipMemcpyHostToDevice, st));
-    T2X_HIP(hipMemcpyAsync(b_im.ptr, data_im, nv * 8, hipMemcpyHostToDevice, st));
-    T2X_HIP(hipMemcpyAsync(b_w.ptr, weight, nv * 8, hipMemcpyHostToDevice, st));
+    if (bp) {
+        k_bp_widen<<<(unsigned)std::min<size_t>((nv + 255) / 256, 65536), 256, 0, st>>>(
+            nv, bp->bp_data, bp->bp_weight, b_re.as<double>(), b_im.as<double>(), b_w.as<double>());
+        T2X_HIP(hipGetLastError());
+    } else {
+        T2X_HIP(hipMemcpyAsync(b_re.ptr, data_re, nv * 8, hipMemcpyHostToDevice, st));
+        T2X_HIP(hipMemcpyAsync(b_im.ptr, data_im, nv * 8, hipMemcpyHostToDevice, st));
+        T2X_HIP(hipMemcpyAsync(b_w.ptr, weight, nv * 8, hipMemcpyHostToDevice, st));
+    }
     if (p->do_map) T2X_HIP(hipMemcpyAsync(b_tau.ptr, tau2, (size_t)nshell_tau * 8, hipMemcpyHostToDevice, st));
     else T2X_HIP(hipMemsetAsync(b_tau.ptr, 0, (size_t)nshell_tau * 8, st));
     T2X_HIP(hipMemsetAsync(b_err.ptr, 0, sizeof(int), st));
@@ -371,6 +395,29 @@ extern "C" int t2x_reconstruct(t2x_device* hdev, const t2x_recon_params* p, cons
     T2X_HIP(hipMemcpyAsync(vol_out, b_vol.ptr, no * 8, hipMemcpyDeviceToHost, st));
     T2X_HIP(hipStreamSynchronize(st));
     return T2X_OK;
+}
+
+extern "C" int t2x_reconstruct(t2x_device* hdev, const t2x_recon_params* p, const double* data_re,
+                               const double* data_im, const double* weight, const double* tau2, double* vol_out)
+{
+    if (!hdev || !p || !data_re || !data_im || !weight || !vol_out) { set_error("null argument"); return T2X_EINVAL; }
+    Device* d = reinterpret_cast<Device*>(hdev);
+    std::lock_guard<std::mutex> lk(d->mtx);
+    return reconstruct_run(d, p, data_re, data_im, weight, nullptr, tau2, vol_out);
+}
+
+extern "C" int t2x_reconstruct_class(t2x_device* hdev, int iclass, const t2x_recon_params* p, const double* tau2,
+                                     double* vol_out)
+{
+    if (!hdev || !p || !vol_out) { set_error("null argument"); return T2X_EINVAL; }
+    Device* d = reinterpret_cast<Device*>(hdev);
+    std::lock_guard<std::mutex> lk(d->mtx);
+    auto it = d->classes.find(iclass);
+    if (it == d->classes.end() || !it->second.bp_data) {
+        set_error("t2x_reconstruct_class: backprojector not initialised");
+        return T2X_ESTATE;
+    }
+    return reconstruct_run(d, p, nullptr, nullptr, nullptr, &it->second, tau2, vol_out);
 }
 
 extern "C" int t2x_fsc(t2x_device* hdev, int n, const double* map1, const double* map2, double* fsc)

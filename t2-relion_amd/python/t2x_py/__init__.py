@@ -242,6 +242,7 @@ def load(path=None):
         "t2x_proposal_train": (C.c_int, [vp, C.POINTER(EstepConfig), C.POINTER(Model),
                                          C.POINTER(ProposalTrainParams), C.POINTER(ProposalTrainStats)]),
         "t2x_reconstruct": (C.c_int, [vp, C.POINTER(ReconParams), PD, PD, PD, PD, PD]),
+        "t2x_reconstruct_class": (C.c_int, [vp, C.c_int, C.POINTER(ReconParams), PD, PD]),
         "t2x_mrc_info_read": (C.c_int, [C.c_char_p, C.POINTER(MrcInfo)]),
         "t2x_mrc_read_sections": (C.c_int, [C.c_char_p, C.c_long, C.c_long, C.POINTER(C.c_float)]),
         "t2x_mrc_write": (C.c_int, [C.c_char_p, C.c_int, C.c_int, C.c_int, C.c_double, C.POINTER(C.c_float)]),

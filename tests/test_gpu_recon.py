@@ -35,6 +35,13 @@ def gpu_recon(dev, p, re, im, w, tau2=None):
     return out
 
 
+def gpu_recon_class(dev, iclass, p, tau2=None):
+    out = np.zeros((p.ori_size,) * 3)
+    t = None if tau2 is None else np.ascontiguousarray(tau2, dtype=np.float64)
+    T.check(T.load().t2x_reconstruct_class(dev, iclass, p, T.dptr(t), T.dptr(out)), "reconstruct_class")
+    return out
+
+
 def orc_recon(p, re, im, w, tau2=None):
     out = np.zeros((p.ori_size,) * 3)
     t = None if tau2 is None else np.ascontiguousarray(tau2, dtype=np.float64)
@@ -119,8 +126,32 @@ def test_half_map_fsc_gpu_vs_cpu_path(gpu_device):
         maps["gpu" + side] = gpu_recon(gpu_device, p, dg.real.reshape(shape), dg.imag.reshape(shape),
                                        wg.reshape(shape))
         maps["cpu" + side] = orc_recon(p, do.real.reshape(shape), do.imag.reshape(shape), wo.reshape(shape))
+        # the same reconstruction read straight from the device backprojector the E-step filled
+        dm = gpu_recon_class(gpu_device, 0, p)
+        print(f"\n[reconstruct_class {side}] rel vs host-array path {H.rel_norm(dm, maps['gpu' + side]):.2e}")
+        assert H.rel_norm(dm, maps["gpu" + side]) < 1e-12
     f_gpu = gpu_fsc(gpu_device, maps["gpua"], maps["gpub"])
     f_cpu = orc_fsc(maps["cpua"], maps["cpub"])
     d = np.abs(f_gpu - f_cpu)
     print("\n[half-map FSC] max |FSC_gpu - FSC_cpu|", d.max(), "map rel", H.rel_norm(maps["gpua"], maps["cpua"]))
     assert d.max() <= 0.01
+
+
+def test_reconstruct_class_from_backprojector(gpu_device):
+    """t2x_reconstruct_class on a backprojector holding known volumes (written by a refine64
+    E-step, MAP term on) equals t2x_reconstruct on the volumes t2x_backprojector_get returns;
+    parameters that disagree with the backprojector and an uninitialised class are rejected."""
+    lib = T.load()
+    wl = H.Workload("refine64", n=8, seed_offset=5)
+    g = wl.run_gpu(dev=gpu_device)
+    p = T.recon_params(wl.cfg.ori_size, wl.rmax, do_map=True, tau2_fudge=1.0, minres_map=2)
+    tau2 = np.random.default_rng(9).uniform(0.05, 1.0, wl.cfg.ori_size // 2 + 1)
+    dg, wg = g["bp"][0]
+    shape = (p.pad_size, p.pad_size, p.pad_size // 2 + 1)
+    ref = gpu_recon(gpu_device, p, dg.real.reshape(shape), dg.imag.reshape(shape), wg.reshape(shape), tau2)
+    out = gpu_recon_class(gpu_device, 0, p, tau2)
+    assert H.rel_norm(out, ref) < 1e-12
+    bad = T.recon_params(wl.cfg.ori_size, wl.rmax - 1)
+    o = np.zeros((p.ori_size,) * 3)
+    assert lib.t2x_reconstruct_class(gpu_device, 0, bad, None, T.dptr(o)) == T.T2X_EINVAL
+    assert lib.t2x_reconstruct_class(gpu_device, 57, p, T.dptr(tau2), T.dptr(o)) == T.T2X_ESTATE
