@@ -117,7 +117,9 @@ typedef struct t2x_model {
     const double* sigma2_noise;    /* [ori_size/2+1]            mymodel.sigma2_noise[0]  */
     const double* pdf_class;       /* [K]                        mymodel.pdf_class       */
     const double* pdf_direction;   /* [K * n_dir]                mymodel.pdf_direction   */
-    const double* prior_offset_class; /* [2K] x,y in px for 2D refs, may be NULL         */
+    const double* prior_offset_class; /* [2K] x,y per class for 2D refs (may differ per   */
+                                   /* class), compared with old offset + translation as  */
+                                   /* ml_optimiser.cpp:7627-7653 does; may be NULL       */
 } t2x_model;
 
 /* Per-particle input metadata (one image per particle). */

@@ -1249,12 +1249,14 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
 
     Prep P;
     preprocess(cfg, image, pin, P);
-    if (cfg->ref_dim == 2) {
-        /* 2D references: the offset prior is the class's (ml_optimiser.cpp:7627-7631, 8283-8287);
-         * the same for every class here (checked by the caller) */
-        P.prior_x = sh.model->prior_offset_class ? sh.model->prior_offset_class[0] : 0.;
-        P.prior_y = sh.model->prior_offset_class ? sh.model->prior_offset_class[1] : 0.;
-    }
+    /* the offset prior per class: the particle's own (3D references), or for 2D references the
+     * class's prior_offset_class (ml_optimiser.cpp:7627-7631, 7678-7682, 8283-8287) */
+    std::vector<double> qx(sh.cfg->nr_classes, P.prior_x), qy(sh.cfg->nr_classes, P.prior_y);
+    if (cfg->ref_dim == 2)
+        for (int c = 0; c < cfg->nr_classes; c++) {
+            qx[c] = sh.model->prior_offset_class ? sh.model->prior_offset_class[2 * c] : 0.;
+            qy[c] = sh.model->prior_offset_class ? sh.model->prior_offset_class[2 * c + 1] : 0.;
+        }
 
     /* orientation prior: NOPRIOR pdf_direction[class][idir]; local searches the product of the
      * particle's direction and psi priors (ml_optimiser.cpp:7031-7045, 7700-7713) */
@@ -1266,9 +1268,9 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
         return local ? dpr[idir] * ppr[ipsi] : sh.model->pdf_direction[(size_t)c * Nd + idir];
     };
     const double sig2off = cfg->sigma2_offset;
-    auto pdf_offset = [&](long itrans) {
+    auto pdf_offset = [&](int c, long itrans) {
         double ox = P.old_x + S.tx[itrans], oy = P.old_y + S.ty[itrans];
-        double tdiff2 = (ox - P.prior_x) * (ox - P.prior_x) + (oy - P.prior_y) * (oy - P.prior_y);
+        double tdiff2 = (ox - qx[c]) * (ox - qx[c]) + (oy - qy[c]) * (oy - qy[c]);
         tdiff2 *= pix * pix;
         if (sig2off < 0.0001) return (tdiff2 > 0.) ? 0. : 1.;
         return std::exp(tdiff2 / (-2. * sig2off)) / (2. * PI * sig2off);
@@ -1387,16 +1389,17 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
                     cnt_o++;
                 }
             }
-            for (long it = 0; it < Nt; it++) { pdf_offset_mean += pdf_offset(it); cnt_t++; }
+            for (long it = 0; it < Nt; it++) { pdf_offset_mean += pdf_offset(c, it); cnt_t++; }
         }
         pdf_orientation_mean /= (double)cnt_o;
         pdf_offset_mean /= (double)cnt_t;
-        std::vector<double> pdf_t(Nt);
-        for (long it = 0; it < Nt; it++) {
-            double v = pdf_offset(it);
-            if (pdf_offset_mean > 0.) v /= pdf_offset_mean;
-            pdf_t[it] = v;
-        }
+        std::vector<double> pdf_t((size_t)K * Nt);   /* [class][translation] */
+        for (int c = 0; c < K; c++)
+            for (long it = 0; it < Nt; it++) {
+                double v = pdf_offset(c, it);
+                if (pdf_offset_mean > 0.) v /= pdf_offset_mean;
+                pdf_t[(size_t)c * Nt + it] = v;
+            }
         double sumw = 0;
         std::vector<double> positive;
         auto weigh = [&](double d2, double prior) {
@@ -1415,7 +1418,7 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
                     if (pdf_orientation_mean != 0.) pdf_o /= pdf_orientation_mean;
                     for (long it = 0; it < Nt; it++) {
                         long ihidden = iorientclass * Nt + it;
-                        double prior = pdf_o * pdf_t[it];
+                        double prior = pdf_o * pdf_t[(size_t)c * Nt + it];
                         if (ipass == 0) {
                             double w = weigh(Mw_coarse[ihidden], prior);
                             Mw_coarse[ihidden] = w;
@@ -1547,9 +1550,9 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
                                 thr_prior_off[2 * c] += weight * pix * (P.old_x + trx[q]);
                                 thr_prior_off[2 * c + 1] += weight * pix * (P.old_y + try_[q]);
                             }
-                            double dx = P.prior_x - P.old_x - trx[q];
+                            double dx = qx[c] - P.old_x - trx[q];
                             thr_sigma2_offset += weight * pix * pix * dx * dx;
-                            double dy = P.prior_y - P.old_y - try_[q];
+                            double dy = qy[c] - P.old_y - try_[q];
                             thr_sigma2_offset += weight * pix * pix * dy * dy;
                             thr_pdf_dir[(size_t)c * Nd + idir] += weight;
                             const cplx* Fn = Fsh_nm[q].data();
@@ -1625,11 +1628,6 @@ extern "C" int orc_estep(const t2x_estep_config* cfg, const t2x_model* model,
                          t2x_estep_debug* dbg, int proj_mode, int nthreads)
 {
     if (check_config(cfg) != T2X_OK || !model || !proj || !bp || n < 0) return T2X_EINVAL;
-    if (cfg->ref_dim == 2 && model->prior_offset_class)
-        for (int c = 1; c < cfg->nr_classes; c++)
-            if (model->prior_offset_class[2 * c] != model->prior_offset_class[0] ||
-                model->prior_offset_class[2 * c + 1] != model->prior_offset_class[1])
-                return T2X_EINVAL;   /* class-dependent offset priors are not supported */
     for (int c = 0; c < cfg->nr_classes; c++)
         if (proj[c].ref_dim != cfg->ref_dim || bp[c].ref_dim != cfg->ref_dim) return T2X_EINVAL;
     Shared sh;

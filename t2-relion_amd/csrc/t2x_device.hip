@@ -62,7 +62,7 @@ extern "C" int t2x_device_close(t2x_device* h)
     for (auto& kv : d->classes) release_class(kv.second);
     DevBuf* bufs[] = {&d->img, &d->img2, &d->fft_out, &d->Fimg_cur, &d->Fimg_nm_cur, &d->Fctf_cur,
                       &d->pscal, &d->power, &d->A, &d->A1, &d->N1, &d->diff2, &d->minb,
-                      &d->sel_out, &d->prior_o, &d->prior_t, &d->sig_pairs, &d->sig_count,
+                      &d->sel_out, &d->prior_o, &d->prior_t, &d->cls_prior, &d->sig_pairs, &d->sig_count,
                       &d->sig_orient, &d->sig_orient_count, &d->orient_pairs_off, &d->fine_items,
                       &d->fine_diff2, &d->fine_min, &d->fine_sel, &d->ab_fine, &d->ab_coarse,
                       &d->wsum_dev, &d->store_items, &d->best, &d->tables, &d->Y_fine,

@@ -276,6 +276,7 @@ __global__ __launch_bounds__(256) void k_coarse_imgside(const float2* __restrict
 __global__ __launch_bounds__(128) void k_prior_t(const PartScalars* __restrict__ ps, int B,
                                                  const double* __restrict__ tx, const double* __restrict__ ty,
                                                  int T, double pix, double sigma2_offset,
+                                                 const double* __restrict__ cls_prior, int K,
                                                  double* __restrict__ prior_t)
 {
     __shared__ double pt[128];
@@ -283,6 +284,33 @@ __global__ __launch_bounds__(128) void k_prior_t(const PartScalars* __restrict__
     const int b = blockIdx.x;
     if (b >= B) return;
     const PartScalars s = ps[b];
+    if (cls_prior) {
+        // 2D references with class-dependent offset priors: one table per class, normalised by
+        // the mean over every (class, translation) (ml_optimiser.cpp:7607-7660), [b][c][t]
+        const int KT = K * T;
+        double* out = prior_t + (size_t)b * KT;
+        double part = 0.;
+        for (int e = threadIdx.x; e < KT; e += blockDim.x) {
+            const int c = e / T, t = e - c * T;
+            const double ox = s.old_x + tx[t], oy = s.old_y + ty[t];
+            const double qx = cls_prior[2 * c], qy = cls_prior[2 * c + 1];
+            const double td2 = ((ox - qx) * (ox - qx) + (oy - qy) * (oy - qy)) * pix * pix;
+            const double v = (sigma2_offset < 0.0001) ? ((td2 > 0.) ? 0. : 1.)
+                                                      : exp(td2 / (-2. * sigma2_offset)) / (2. * kPi * sigma2_offset);
+            out[e] = v;
+            part += v;
+        }
+        pt[threadIdx.x] = part;
+        __syncthreads();
+        for (int h = blockDim.x / 2; h > 0; h >>= 1) {
+            if ((int)threadIdx.x < h) pt[threadIdx.x] += pt[threadIdx.x + h];
+            __syncthreads();
+        }
+        const double m = pt[0] / (double)KT;
+        // each thread rescales the entries it wrote
+        for (int e = threadIdx.x; e < KT; e += blockDim.x) out[e] = (m > 0.) ? out[e] / m : out[e];
+        return;
+    }
     for (int t = threadIdx.x; t < T; t += blockDim.x) {
         const double ox = s.old_x + tx[t], oy = s.old_y + ty[t];
         const double td2 = ((ox - s.prior_x) * (ox - s.prior_x) + (oy - s.prior_y) * (oy - s.prior_y)) * pix * pix;
@@ -703,6 +731,8 @@ struct StoreParams {
     const int* bp_rmax;
     const double* prior_o;
     const double* prior_t;
+    int tK;                    // > 0: prior_t per class (trans_prior), cls_prior holds the classes' offsets
+    const double* cls_prior;   // [2K] px, 2D references with class-dependent offset priors
     const int2* pix;
     const unsigned char* inproj;
     const float* minv_store;   // Minvsigma2 with the origin term (ml_optimiser.cpp:8099-8104)
@@ -768,7 +798,8 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
         const long long k = p0 + q / g.over_trans;
         const int iot = q % g.over_trans;
         const int2 nt2 = pairs[k];
-        double prior = orient_prior(P.prior_o, P.lp, b, nt2.x) * P.prior_t[(size_t)b * g.T + nt2.y];
+        double prior = orient_prior(P.prior_o, P.lp, b, nt2.x) *
+                       trans_prior(P.prior_t, g.T, P.tK, g.Norient, b, nt2.x, nt2.y);
         double w = 0.;
         if (prior > 0.) {
             double dd = (double)P.fine_d2[k * nover + ir * g.over_trans + iot] - mind2;
@@ -794,11 +825,14 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
         double sw = 0., soff = 0., pox = 0., poy = 0.;
         const double px2 = P.pixel_size * P.pixel_size;
         const PartScalars s = P.ps[b];
+        const double qx = P.cls_prior ? P.cls_prior[2 * c] : s.prior_x;
+        const double qy = P.cls_prior ? P.cls_prior[2 * c + 1] : s.prior_y;
         for (int q = 0; q < nt; q++) {
             const long long k = p0 + q / g.over_trans;
             const int iot = q % g.over_trans;
             const int2 nt2 = pairs[k];
-            double prior = orient_prior(P.prior_o, P.lp, b, nt2.x) * P.prior_t[(size_t)b * g.T + nt2.y];
+            double prior = orient_prior(P.prior_o, P.lp, b, nt2.x) *
+                           trans_prior(P.prior_t, g.T, P.tK, g.Norient, b, nt2.x, nt2.y);
             if (!(prior > 0.)) continue;
             double dd = (double)P.fine_d2[k * nover + ir * g.over_trans + iot] - mind2;
             double w = (dd > 700.) ? 0. : prior * exp(-dd);
@@ -806,8 +840,8 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
             w /= sumw;
             const int ft = nt2.y * g.over_trans + iot;
             sw += w;
-            double dx = s.prior_x - s.old_x - P.ftx[ft];
-            double dy = s.prior_y - s.old_y - P.fty[ft];
+            double dx = qx - s.old_x - P.ftx[ft];
+            double dy = qy - s.old_y - P.fty[ft];
             soff += w * px2 * dx * dx + w * px2 * dy * dy;
             pox += w * P.pixel_size * (s.old_x + P.ftx[ft]);   // ml_optimiser.cpp:8504-8508
             poy += w * P.pixel_size * (s.old_y + P.fty[ft]);
@@ -1408,17 +1442,26 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         }
     }
     const int ref2d = cfg->ref_dim == 2 ? 1 : 0;
-    // 2D references: the offset prior is the class's (ml_optimiser.cpp:7627-7631, 8283-8287);
-    // class-dependent priors would make the translation prior class-dependent, not supported
+    // 2D references: the offset prior is the class's (ml_optimiser.cpp:7627-7631, 8283-8287).
+    // Equal for every class (the first iteration): one translation prior table per particle;
+    // otherwise one per (particle, class), tK = K (k_prior_t, trans_prior).
     double class_prior_x = 0., class_prior_y = 0.;
+    int tK = 0;
     if (ref2d && model->prior_offset_class) {
         class_prior_x = model->prior_offset_class[0];
         class_prior_y = model->prior_offset_class[1];
-        for (int c = 1; c < K; c++)
-            if (model->prior_offset_class[2 * c] != class_prior_x || model->prior_offset_class[2 * c + 1] != class_prior_y) {
-                set_error("2D references: per-class offset priors must be equal");
+        for (int c = 0; c < 2 * K; c++)
+            if (!std::isfinite(model->prior_offset_class[c])) {
+                set_error("2D references: non-finite class offset prior");
                 return T2X_EINVAL;
             }
+        for (int c = 1; c < K; c++)
+            if (model->prior_offset_class[2 * c] != class_prior_x || model->prior_offset_class[2 * c + 1] != class_prior_y)
+                tK = K;
+        if (tK) {
+            std::vector<double> cp(model->prior_offset_class, model->prior_offset_class + 2 * K);
+            T2X_TRY(upload(d->cls_prior, cp, st));
+        }
     }
     // ---- per-configuration tables ----
     const double* sigma2 = model->sigma2_noise;
@@ -1680,9 +1723,10 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         DevBuf& d_ctf = d->ctfc;
         T2X_TRY(upload(d_ctf, hctf, st));
         PartScalars* d_ps = d->pscal.as<PartScalars>();
-        T2X_TRY(d->prior_t.ensure(sizeof(double) * B * T));
+        T2X_TRY(d->prior_t.ensure(sizeof(double) * B * T * (tK ? tK : 1)));
         k_prior_t<<<B, 128, 0, st>>>(d_ps, B, d_ctx.as<double>(), d_cty.as<double>(), (int)T, pix,
-                                     cfg->sigma2_offset, d->prior_t.as<double>());
+                                     cfg->sigma2_offset, tK ? d->cls_prior.as<double>() : nullptr, tK,
+                                     d->prior_t.as<double>());
         T2X_HIP(hipGetLastError());
         // local searches: each particle's direction / psi priors around its prior angles (the
         // current angles where the priors are unset, ml_optimiser.cpp:5851-5857)
@@ -1933,6 +1977,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             wc.O_all = (int)O_all;
             wc.prior_o = d->prior_o.as<double>();
             wc.prior_t = d->prior_t.as<double>();
+            wc.tK = tK;
+            wc.tcols = (int)Norient;
             wc.lp = lp;
             k_pass_min<<<B, 1024, 0, st>>>(wc, B, d_selc);
             SelParams sp{cfg->adaptive_fraction, (long long)cfg->maximum_significants, 1};
@@ -2136,6 +2182,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             wc.prior_o = d->prior_o.as<double>();
             wc.log_prior_o = d->log_prior_o.as<double>();
             wc.prior_t = d->prior_t.as<double>();
+            wc.tK = tK;
+            wc.tcols = (int)Norient;
             wc.rowmin = d_rowmin.as<float>();
             wc.rowmin_ld = rowmin_ld;
             wc.nover = nover;
@@ -2364,6 +2412,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         P.bp_rmax = d_bprmax.as<int>();
         P.prior_o = d->prior_o.as<double>();
         P.prior_t = d->prior_t.as<double>();
+        P.tK = tK;
+        P.cls_prior = tK ? d->cls_prior.as<double>() : nullptr;
         P.pix = tf.d_pix.as<int2>();
         P.inproj = tf.d_inproj.as<unsigned char>();
         P.minv_store = tf.d_minv.as<float>();

@@ -145,7 +145,7 @@ struct Device {
 
     // scratch
     DevBuf img, img2, fft_out, Fimg_cur, Fimg_nm_cur, Fctf_cur, pscal, power;
-    DevBuf A, A1, N1, diff2, minb, sel_out, prior_o, prior_t, sig_pairs, sig_count, sig_orient,
+    DevBuf A, A1, N1, diff2, minb, sel_out, prior_o, prior_t, cls_prior, sig_pairs, sig_count, sig_orient,
         sig_orient_count, orient_pairs_off, fine_items, fine_diff2, fine_min, fine_sel, ab_fine,
         ab_coarse, wsum_dev, store_items, best, tables, Y_fine, dbg_tmp, cand_w, cand_e, cand_n,
         pairs_strided, Ah, Al, As, sum_part, log_prior_o, pmin;

@@ -27,14 +27,14 @@ __device__ __forceinline__ double pass_weight(const WeightSrc& s, int b, long lo
         if (!coarse_allowed(s, b, n)) return 0.;
         // blocks beyond the negligible-weight cut (their values may not have been stored)
         if ((double)s.rowmin[((size_t)b * s.T + t) * s.rowmin_ld + n / 64] - mind2 > 60.0) return 0.;
-        prior = orient_prior(s.prior_o, s.lp, b, n) * s.prior_t[(size_t)b * s.T + t];
+        prior = orient_prior(s.prior_o, s.lp, b, n) * trans_prior(s.prior_t, s.T, s.tK, s.tcols, b, n, (int)t);
         if (!(prior > 0.)) return 0.;
         d = s.d2[((size_t)b * s.T + t) * s.ld + n];
     } else {
         long long base = pair_off_b_cache[0];
         long long k = base + e / s.nover;
         int2 nt = s.pairs[k];
-        prior = orient_prior(s.prior_o, s.lp, b, nt.x) * s.prior_t[(size_t)b * s.T + nt.y];
+        prior = orient_prior(s.prior_o, s.lp, b, nt.x) * trans_prior(s.prior_t, s.T, s.tK, s.tcols, b, nt.x, nt.y);
         if (!(prior > 0.)) return 0.;
         d = s.d2[base * s.nover + e];
     }
@@ -575,12 +575,22 @@ __global__ void k_log_prior(const double* __restrict__ p, long long n, double* _
 // minimum holds only negligible weights and is skipped without reading it. Each wave takes 64
 // blocks at a time, one lane per block minimum, and walks the surviving blocks with one lane
 // per column.
+// The translation prior of row (b, t): its value, or with per-class priors the largest over
+// the classes (a bound used to skip rows; every weight uses its own class's value).
+__device__ __forceinline__ double row_prior_max(const WeightSrc& s, int b, int t)
+{
+    if (s.tK == 0) return s.prior_t[(size_t)b * s.T + t];
+    double m = 0.;
+    for (int c = 0; c < s.tK; c++) m = fmax(m, s.prior_t[((size_t)b * s.tK + c) * s.T + t]);
+    return m;
+}
+
 __global__ __launch_bounds__(256) void k_coarse_sum_rows(WeightSrc s, int B, const SelResult* res, SumPart* part)
 {
     const int t = blockIdx.x, b = blockIdx.y;
     if (b >= B || t >= s.T) return;
     const double mind2 = res[b].min_d2;
-    const double pt = s.prior_t[(size_t)b * s.T + t];
+    const double pt = row_prior_max(s, b, t);
     const size_t row_id = (size_t)b * s.T + t;
     const float* row = s.d2 + row_id * s.ld;
     const float* rmin = s.rowmin + row_id * s.rowmin_ld;
@@ -601,7 +611,8 @@ __global__ __launch_bounds__(256) void k_coarse_sum_rows(WeightSrc s, int B, con
                     const double po = s.prior_o[n];
                     const double dd = (double)row[n] - mind2;
                     if (po > 0. && dd <= kTinyDD && coarse_allowed(s, b, n)) {
-                        const double w = (po * pt) * exp(-dd);
+                        const double ptn = s.tK ? trans_prior(s.prior_t, s.T, s.tK, s.tcols, b, n, t) : pt;
+                        const double w = (po * ptn) * exp(-dd);
                         if (w > 0.) {
                             npos++;
                             sum += w;
@@ -663,14 +674,15 @@ __global__ __launch_bounds__(256) void k_coarse_candidates(WeightSrc s, int B, S
     const double mind2 = res[b].min_d2;
     const long long nb = res[b].nsig;
     const double thr = (nb > 0) ? 0.5 * (1.0 - prm.fraction) * res[b].sum_w / (double)nb : 0.;
-    const double pt = s.prior_t[(size_t)b * s.T + t];
+    const double pt = row_prior_max(s, b, t);
     const size_t row_id = (size_t)b * s.T + t;
     const float* row = s.d2 + row_id * s.ld;
     const float* rmin = s.rowmin + row_id * s.rowmin_ld;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nw = blockDim.x >> 6;
     if (!(pt > 0.) || !(thr > 0.)) return;
     // w >= thr  <=>  dd <= log(prior_o) + log(pt) - log(thr): compared in the log domain (relaxed by
-    // 1e-9, far below the factor-2 margin of thr), exact test on the few that pass
+    // 1e-9, far below the factor-2 margin of thr; with per-class translation priors pt is the
+    // row's largest, so the filter only relaxes), exact test on the few that pass
     const double lbias = log(pt) - log(thr) + 1e-9;
     const int nblk = (s.O_all + 63) / 64;
     for (int j0 = wave * 64; j0 < nblk; j0 += nw * 64) {
@@ -687,7 +699,8 @@ __global__ __launch_bounds__(256) void k_coarse_candidates(WeightSrc s, int B, S
                 const double dd = (double)row[n] - mind2;
                 if (dd <= kTinyDD && dd <= s.log_prior_o[n] + lbias && coarse_allowed(s, b, n)) {
                     const double po = s.prior_o[n];
-                    w = (po * pt) * exp(-dd);
+                    const double ptn = s.tK ? trans_prior(s.prior_t, s.T, s.tK, s.tcols, b, n, t) : pt;
+                    w = (po * ptn) * exp(-dd);
                     keep = (po > 0.) && (w > 0.) && (w >= thr);
                 }
             }

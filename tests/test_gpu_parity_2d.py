@@ -105,14 +105,23 @@ def test_class_offset_prior_2d(gpu_device):
     assert not np.allclose(base["wsum"]["sigma2_offset"], o["wsum"]["sigma2_offset"])
 
 
-def test_unequal_class_offset_priors_rejected(gpu_device):
-    wl = H.Workload("class2d64", n=2, seed_offset=5)
-    pc = np.zeros(2 * wl.cfg.nr_classes)
-    pc[2] = 1.0
+@pytest.mark.parametrize("prec", [T.T2X_PREC_FP32, T.T2X_PREC_FP16X3])
+def test_unequal_class_offset_priors_2d(gpu_device, prec):
+    """Class-dependent offset priors (2D classification after its first iteration): one
+    translation prior table per class, normalised over every (class, translation)."""
+    wl = H.Workload("class2d64", n=12, seed_offset=5, precision=prec)
+    pc = np.array([0.0, 0.0, 1.0, -2.0, -1.5, 0.5, 2.0, 2.0])[:2 * wl.cfg.nr_classes]
     wl.prior_class = pc
     wl.model = T.Model(T.dptr(wl.sigma2), T.dptr(wl.pdf_class), T.dptr(wl.pdf_dir), T.dptr(pc))
-    with pytest.raises(RuntimeError, match="offset priors"):
-        wl.run_gpu(dev=gpu_device)
+    o = wl.run_oracle()
+    g = wl.run_gpu(dev=gpu_device)
+    assert (o["out"]["best_hidden"] == g["out"]["best_hidden"]).mean() >= 0.999
+    assert H.rel(g["out"]["dLL"], o["out"]["dLL"]) < 1e-5
+    assert H.rel(g["out"]["sum_weight"], o["out"]["sum_weight"]) < 1e-4
+    for k in ("prior_offset_class", "sigma2_offset", "pdf_class"):
+        assert H.rel_norm(g["wsum"][k], o["wsum"][k]) < 1e-4, k
+    eq = with_class_prior(H.Workload("class2d64", n=12, seed_offset=5, precision=prec), (0.0, 0.0)).run_oracle()
+    assert not np.allclose(eq["wsum"]["sigma2_offset"], o["wsum"]["sigma2_offset"])
 
 
 def test_fp16x3_2d(class2d_pair, gpu_device):

@@ -147,3 +147,51 @@ def test_reference_plane_is_central_plane_of_volume(orc):
     assert orc.orc_synth_reference(N, 2.0, pad, rmax, 2.0, blobs, 12, 12.0, T.dptr(vol)) == 0
     assert orc.orc_synth_reference_plane(N, 2.0, pad, rmax, 2.0, blobs, 12, 12.0, T.dptr(plane)) == 0
     assert np.array_equal(vol.reshape(pad, -1)[pad // 2], plane)
+
+
+def _set_class_priors(wl, xy):
+    wl.prior_class = np.ascontiguousarray(np.asarray(xy, float).ravel())
+    wl.model = T.Model(T.dptr(wl.sigma2), T.dptr(wl.pdf_class), T.dptr(wl.pdf_dir), T.dptr(wl.prior_class))
+
+
+def test_class_offset_prior_normalisation():
+    """The translation prior is normalised by its mean over every (class, translation)
+    (ml_optimiser.cpp:7607-7660). Class 1 is excluded by its direction prior (pdf_direction 0) in
+    both runs; moving its offset prior 1000 px away makes its translation prior exactly 0, which
+    halves that mean: every weight of class 0 doubles (sum_weight x 2) and nothing else changes."""
+    import estep_harness as H
+    wl = H.Workload("class2d64", n=5, K=2, seed_offset=7)
+    wl.pdf_dir[1] = 0.0
+    _set_class_priors(wl, [0.0, 0.0, 0.0, 0.0])
+    a = wl.run_oracle()
+    _set_class_priors(wl, [0.0, 0.0, 1000.0, 0.0])
+    b = wl.run_oracle()
+    assert np.all(a["out"]["iclass"] == 1) and np.array_equal(a["out"]["best_hidden"], b["out"]["best_hidden"])
+    assert np.allclose(b["out"]["sum_weight"] / a["out"]["sum_weight"], 2.0, rtol=1e-12, atol=0)
+    for k in ("sigma2_offset", "prior_offset_class", "pdf_class", "sigma2_noise"):
+        assert np.allclose(a["wsum"][k], b["wsum"][k], rtol=1e-12, atol=1e-300), k
+
+
+def test_class_offset_prior_picks_own_centre():
+    """sigma2_offset below 1e-4 makes the translation prior an indicator of offset == prior
+    (ml_optimiser.cpp:7655-7658): each particle's offset must then sit on the prior of the class
+    it is assigned to (within the fine pass's +-1/4-step oversampling), for both classes. The
+    particles are made with their true class's prior as offset (an image shifted by s px has
+    offset -s px). The priors are compared with old offset + coarse translation, the translation
+    in the grid's units (A, ml_optimiser.cpp:7634-7635): q = (-4, 2) is the grid point of -2, +1 px
+    at 2 A/px."""
+    import estep_harness as H
+    wl = H.Workload("class2d64", n=10, K=2, seed_offset=3)
+    pix = wl.cfg.pixel_size
+    q = np.array([[0.0, 0.0], [-2.0 * pix, 1.0 * pix]])
+
+    def tie(tr):
+        tr["x"][:] = -q[tr["cls"], 0] / pix
+        tr["y"][:] = -q[tr["cls"], 1] / pix
+    wl = H.Workload("class2d64", n=10, K=2, seed_offset=3, truth_hook=tie)
+    wl.cfg.sigma2_offset = 0.0
+    _set_class_priors(wl, q)
+    o = wl.run_oracle()["out"]
+    c = o["iclass"] - 1                      # 1-based class numbers
+    assert set(c.tolist()) == {0, 1}
+    assert np.all(np.abs(o["xoff"] - q[c, 0] / pix) <= 0.5) and np.all(np.abs(o["yoff"] - q[c, 1] / pix) <= 0.5)
