@@ -132,7 +132,9 @@ def run_step(lib, dev, cfg, sz, ds, n, outs, ws, first=0):
     set_projectors(lib, dev, sz, ds, cfg.ref_dim)
     for c in range(cfg.nr_classes):
         T.check(lib.t2x_backprojector_clear(dev, c), "bp clear")
-    model = T.Model(T.dptr(ds["sigma2"]), T.dptr(ds["pdf_class"]), T.dptr(ds["pdf_dir"]), None)
+    pc = ds.get("prior_class")
+    model = T.Model(T.dptr(ds["sigma2"]), T.dptr(ds["pdf_class"]), T.dptr(ds["pdf_dir"]),
+                    T.dptr(pc) if pc is not None else None)
     N = cfg.ori_size
     img = C.c_void_p(ds["dimg"].value + first * N * N * 4)
     pin = C.cast(C.c_void_p(C.addressof(ds["pin"]) + first * C.sizeof(T.ParticleIn)), C.POINTER(T.ParticleIn))
@@ -347,6 +349,9 @@ def main():
                     help="use proposal parameters saved by an earlier run instead of training")
     ap.add_argument("--quality-particles", type=int, default=2000,
                     help="particles of the half-map FSC phase (0: skip)")
+    ap.add_argument("--class-offset-priors", action="store_true",
+                    help="2D references: a different offset prior per class (uniform within one "
+                         "offset step), as in every iteration after the first")
     ap.add_argument("--gemm-library-reference", action="store_true",
                     help="also time the vendor library's FP16 GEMM at the coarse GEMM's shape "
                          "(800-particle batch rows x all orientations x 2 P_c) for comparison")
@@ -372,6 +377,11 @@ def main():
     t0 = time.time()
     ds = build_dataset(lib, dev, w, n, cfg, sz, w["seed"])
     log(f"dataset synthesised on the GPU in {time.time() - t0:.1f} s")
+    if args.class_offset_priors:
+        if cfg.ref_dim != 2:
+            sys.exit("--class-offset-priors applies to 2D references only")
+        step = w["offset_step_px"] * cfg.pixel_size
+        ds["prior_class"] = np.random.default_rng(w["seed"] + 1).uniform(-step, step, 2 * cfg.nr_classes)
     for c in range(cfg.nr_classes):
         T.check(lib.t2x_backprojector_init(dev, c, cfg.ref_dim, sz["pad_size"], sz["r_max"], 2.0), "bp init")
     outs = (T.ParticleOut * n)()
@@ -546,6 +556,7 @@ def main():
                                + (f"HEALPix order {cfg.healpix_order}" if cfg.ref_dim == 3 else
                                   f"2D references, psi step {cfg.psi_step} deg")
                                + (f", local searches sigma_ang {cfg.sigma_rot} deg" if local else "")
+                               + (", class-dependent offset priors" if args.class_offset_priors else "")
                                + f", offsets "
                                f"{w['offset_range_px']}/{w['offset_step_px']} px, current_size "
                                f"{cfg.current_size}, coarse_size {cfg.coarse_size}, path {head}",

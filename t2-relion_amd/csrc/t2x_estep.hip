@@ -286,12 +286,12 @@ __global__ __launch_bounds__(128) void k_prior_t(const PartScalars* __restrict__
     const PartScalars s = ps[b];
     if (cls_prior) {
         // 2D references with class-dependent offset priors: one table per class, normalised by
-        // the mean over every (class, translation) (ml_optimiser.cpp:7607-7660), [b][c][t]
+        // the mean over every (class, translation) (ml_optimiser.cpp:7607-7660), [b][t][c]
         const int KT = K * T;
         double* out = prior_t + (size_t)b * KT;
         double part = 0.;
         for (int e = threadIdx.x; e < KT; e += blockDim.x) {
-            const int c = e / T, t = e - c * T;
+            const int t = e / K, c = e - t * K;
             const double ox = s.old_x + tx[t], oy = s.old_y + ty[t];
             const double qx = cls_prior[2 * c], qy = cls_prior[2 * c + 1];
             const double td2 = ((ox - qx) * (ox - qx) + (oy - qy) * (oy - qy)) * pix * pix;

@@ -75,7 +75,7 @@ struct WeightSrc {
     int O_all;                 // coarse orientations (all classes)
     const double* prior_o;     // [O_all]
     const double* log_prior_o; // [O_all] log(prior_o) (-inf where 0), coarse candidates only
-    const double* prior_t;     // [B][T], or [B][tK][T] (see trans_prior)
+    const double* prior_t;     // [B][T], or [B][T][tK] (see trans_prior)
     int tK;                    // 0: one translation prior per particle; > 0: per class
     int tcols;                 // with tK > 0: orientation columns per class
     const float* rowmin;       // coarse: [B*T][ld/64] per-row minima from the GEMM
@@ -95,12 +95,12 @@ struct WeightSrc {
 };
 
 // Translation prior of particle b, coarse translation t, for orientation column n. 2D references
-// with class-dependent offset priors (ml_optimiser.cpp:7627-7631, 7678-7682) hold one table per
-// class, the class of column n being n / tcols; otherwise one table per particle.
+// with class-dependent offset priors (ml_optimiser.cpp:7627-7631, 7678-7682) hold [B][T][tK] (a
+// row's classes contiguous), the class of column n being n / tcols; otherwise [B][T].
 __device__ __forceinline__ double trans_prior(const double* pt, int T, int tK, int tcols, int b, long long n, int t)
 {
     if (tK == 0) return pt[(size_t)b * T + t];
-    return pt[((size_t)b * tK + (size_t)(n / tcols)) * T + t];
+    return pt[((size_t)b * T + t) * tK + (unsigned)n / (unsigned)tcols];
 }
 
 __device__ __forceinline__ bool coarse_allowed(const WeightSrc& s, int b, long long n)

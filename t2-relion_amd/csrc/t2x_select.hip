@@ -581,8 +581,30 @@ __device__ __forceinline__ double row_prior_max(const WeightSrc& s, int b, int t
 {
     if (s.tK == 0) return s.prior_t[(size_t)b * s.T + t];
     double m = 0.;
-    for (int c = 0; c < s.tK; c++) m = fmax(m, s.prior_t[((size_t)b * s.tK + c) * s.T + t]);
+    const double* r = s.prior_t + ((size_t)b * s.T + t) * s.tK;
+    for (int c = 0; c < s.tK; c++) m = fmax(m, r[c]);
     return m;
+}
+
+// Per-class translation priors of row (b, t) staged in LDS when there are at most kRowPriorLds
+// classes (every column of the row reads one of them).
+constexpr int kRowPriorLds = 1024;
+__device__ __forceinline__ bool stage_row_priors(const WeightSrc& s, int b, int t, double* sp)
+{
+    const bool use = s.tK > 0 && s.tK <= kRowPriorLds;
+    if (use) {
+        const double* r = s.prior_t + ((size_t)b * s.T + t) * s.tK;
+        for (int c = threadIdx.x; c < s.tK; c += blockDim.x) sp[c] = r[c];
+    }
+    __syncthreads();
+    return use;
+}
+__device__ __forceinline__ double col_prior(const WeightSrc& s, const double* sp, bool staged, int b, int n, int t,
+                                            double pt_row)
+{
+    if (s.tK == 0) return pt_row;
+    if (staged) return sp[(unsigned)n / (unsigned)s.tcols];
+    return trans_prior(s.prior_t, s.T, s.tK, s.tcols, b, n, t);
 }
 
 __global__ __launch_bounds__(256) void k_coarse_sum_rows(WeightSrc s, int B, const SelResult* res, SumPart* part)
@@ -591,6 +613,8 @@ __global__ __launch_bounds__(256) void k_coarse_sum_rows(WeightSrc s, int B, con
     if (b >= B || t >= s.T) return;
     const double mind2 = res[b].min_d2;
     const double pt = row_prior_max(s, b, t);
+    __shared__ double s_pt[kRowPriorLds];
+    const bool staged = stage_row_priors(s, b, t, s_pt);
     const size_t row_id = (size_t)b * s.T + t;
     const float* row = s.d2 + row_id * s.ld;
     const float* rmin = s.rowmin + row_id * s.rowmin_ld;
@@ -611,7 +635,7 @@ __global__ __launch_bounds__(256) void k_coarse_sum_rows(WeightSrc s, int B, con
                     const double po = s.prior_o[n];
                     const double dd = (double)row[n] - mind2;
                     if (po > 0. && dd <= kTinyDD && coarse_allowed(s, b, n)) {
-                        const double ptn = s.tK ? trans_prior(s.prior_t, s.T, s.tK, s.tcols, b, n, t) : pt;
+                        const double ptn = col_prior(s, s_pt, staged, b, n, t, pt);
                         const double w = (po * ptn) * exp(-dd);
                         if (w > 0.) {
                             npos++;
@@ -675,6 +699,8 @@ __global__ __launch_bounds__(256) void k_coarse_candidates(WeightSrc s, int B, S
     const long long nb = res[b].nsig;
     const double thr = (nb > 0) ? 0.5 * (1.0 - prm.fraction) * res[b].sum_w / (double)nb : 0.;
     const double pt = row_prior_max(s, b, t);
+    __shared__ double s_pt[kRowPriorLds];
+    const bool staged = stage_row_priors(s, b, t, s_pt);
     const size_t row_id = (size_t)b * s.T + t;
     const float* row = s.d2 + row_id * s.ld;
     const float* rmin = s.rowmin + row_id * s.rowmin_ld;
@@ -699,7 +725,7 @@ __global__ __launch_bounds__(256) void k_coarse_candidates(WeightSrc s, int B, S
                 const double dd = (double)row[n] - mind2;
                 if (dd <= kTinyDD && dd <= s.log_prior_o[n] + lbias && coarse_allowed(s, b, n)) {
                     const double po = s.prior_o[n];
-                    const double ptn = s.tK ? trans_prior(s.prior_t, s.T, s.tK, s.tcols, b, n, t) : pt;
+                    const double ptn = col_prior(s, s_pt, staged, b, n, t, pt);
                     w = (po * ptn) * exp(-dd);
                     keep = (po > 0.) && (w > 0.) && (w >= thr);
                 }
