@@ -332,7 +332,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="class256")
     ap.add_argument("--particles", type=int, default=0, help="default: the workload's count")
-    ap.add_argument("--warmup-particles", type=int, default=512)
+    ap.add_argument("--warmup-particles", type=int, default=0,
+                    help="particles per warm-up step; 0: all of them, so that the warm-up launches every "
+                         "batch shape of the timed step and the GEMM autotuner never runs while timed")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="also run one 1 GiB device-to-device copy kernel whose FETCH_SIZE/WRITE_SIZE "
@@ -420,7 +422,7 @@ def main():
             flat = np.zeros(nparam.value, np.float32)
             T.check(lib.t2x_proposal_get_params(dev, flat.ctypes.data_as(C.POINTER(C.c_float)), nparam.value))
             np.save(args.save_proposal, flat)
-    nw = min(n, args.warmup_particles)
+    nw = n if args.warmup_particles <= 0 else min(n, args.warmup_particles)
     runs = {}
     for pname in paths:
         configure(cfg, pname)

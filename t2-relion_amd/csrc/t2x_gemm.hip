@@ -8,6 +8,7 @@
 // K step 32, v_mfma_f32_16x16x32_{bf16,f16} on split hi/lo operands (split precision).
 #include "t2x_kernels.h"
 #include <algorithm>
+#include <cmath>
 #include <map>
 #include <mutex>
 #include <tuple>
@@ -1241,13 +1242,17 @@ static int launch_x3_variant(int prec, int epi, const SplitOperand& A, const Spl
 // (each run writes the same outputs); the fastest is kept for the rest of the process. N and K
 // fix the per-tile work; M only scales the number of tiles, so batches of other particle counts
 // reuse the choice.
+// The best tile walk depends on the number of tile rows as well, so the key holds the row count
+// rounded to the nearest power of two (row-mapped launches cover a varying subset of a batch).
 struct TuneKey {
-    int dev, prec, epi, N, K, mapped;
+    int dev, prec, epi, N, K, mapped, mlog2;
     bool operator<(const TuneKey& o) const
     {
-        return std::tie(dev, prec, epi, N, K, mapped) < std::tie(o.dev, o.prec, o.epi, o.N, o.K, o.mapped);
+        return std::tie(dev, prec, epi, N, K, mapped, mlog2) <
+               std::tie(o.dev, o.prec, o.epi, o.N, o.K, o.mapped, o.mlog2);
     }
 };
+static int rows_log2(long long m) { return m > 0 ? (int)std::lround(std::log2((double)m)) : 0; }
 static std::mutex g_tune_mtx;
 static std::map<TuneKey, GemmChoice> g_tune;
 
@@ -1284,7 +1289,7 @@ int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const floa
     }
     int dev = 0;
     T2X_HIP(hipGetDevice(&dev));
-    const TuneKey key{dev, prec, epi, Nrows_pad, K, mapped ? 1 : 0};
+    const TuneKey key{dev, prec, epi, Nrows_pad, K, mapped ? 1 : 0, rows_log2(ep.M)};
     {
         std::lock_guard<std::mutex> lk(g_tune_mtx);
         auto it = g_tune.find(key);

@@ -3,6 +3,7 @@
     python -m t2x_py.rocprof_summary trace <run_results.db>
     python -m t2x_py.rocprof_summary pmc <run_results.db> <COUNTER> [kernel-substring|...]
     python -m t2x_py.rocprof_summary traffic <fetch.db> <write.db> <kernel-substring|...> <n_last>
+    python -m t2x_py.rocprof_summary last <run_results.db> <kernel-substring|...> <n_last>
 
 `traffic` calibrates FETCH_SIZE and WRITE_SIZE (KiB per dispatch) against the 1 GiB
 device-to-device copy kernel k_copy16 in the same run (1 GiB read + 1 GiB written), sums the
@@ -25,6 +26,23 @@ def trace(db, top=25):
         short = name.split("(")[0].replace("void ", "")
         out.append(f"| `{short}` | {calls} | {tot / 1e3:.3f} | {avg / 1e3:.3f} | {pct:.2f} |")
     return "\n".join(out)
+
+
+def last(db, sub, n_last):
+    """Mean duration (ms) of the last n_last dispatches of each kernel matching one of the
+    '|'-separated substrings, in dispatch order (kernel-trace database)."""
+    con = sqlite3.connect(db)
+    cols = [r[1] for r in con.execute("pragma table_info(kernels)")]
+    name = "name" if "name" in cols else "kernel_name"
+    dur = "duration" if "duration" in cols else "(end - start)"
+    order = "start" if "start" in cols else "rowid"
+    rows = con.execute(f"select {name}, {dur} from kernels order by {order}").fetchall()
+    out = {}
+    for s in sub.split("|"):
+        d = [r[1] for r in rows if s in r[0]]
+        tail = d[-n_last:]
+        out[s] = {"dispatches": len(d), "mean_ms_last": sum(tail) / len(tail) / 1e6 if tail else None}
+    return out
 
 
 def dispatches(db, counter):
@@ -72,7 +90,9 @@ def traffic(fetch_db, write_db, sub, n_last):
 
 
 if __name__ == "__main__":
-    if sys.argv[1] == "trace":
+    if sys.argv[1] == "last":
+        print(json.dumps(last(sys.argv[2], sys.argv[3], int(sys.argv[4])), indent=1))
+    elif sys.argv[1] == "trace":
         print(trace(sys.argv[2]))
     elif sys.argv[1] == "pmc":
         print(pmc(sys.argv[2], sys.argv[3], sys.argv[4] if len(sys.argv) > 4 else None))
