@@ -1,6 +1,6 @@
 #!/usr/bin/env python3
 """E-step throughput on one MI355X on the 3D-classification workload (20,000 particles of 256^2,
-K=4 references of 256^3 at 1.1 A/px, HEALPix order 3, offsets 6/1.5 px), three paths timed in the
+K=4 references of 256^3 at 1.1 A/px, HEALPix order 3, offsets 6/1.5 px), four paths timed in the
 same run on the same resident particles:
 
   fp32     plain FP32 cross term (the parity path)
@@ -10,6 +10,11 @@ same run on the same resident particles:
            particles whose top softmax probability is below 0.3. The network is trained first
            (2000 steps of batch 256 on 200,000 projections of the current references made on the
            fly by the GPU projector); its training time is reported beside the E-step rate.
+  learned_gated  the same network, top-64 bins, exhaustive search below a top-64 probability
+           mass of 0.98 (the headline value: it meets the pose-agreement bar).
+
+Each path runs one untimed warm-up step over the whole workload first (every batch shape, so the
+GEMM autotuner has finished before the timed step).
 
 A step is one full E-step over all particles (synthesised once on the GPU, resident in HBM):
 reference upload + coarse projections, preprocessing, both sampling passes, significance,
@@ -23,7 +28,8 @@ bounded sample of the same workload with one particle per thread.
 
 Other workloads (--workload): class2d128 (2D classification, 100,000 particles 128^2, K=100;
 paths fp32, fp16x3) and local360 (local searches, 50,000 particles 360^2, HEALPix 5, sigma_ang
-3 deg, particles processed in order of prior orientation; paths fp32, fp16x3). The default
+3 deg, particles processed in order of prior orientation; paths fp32, fp16x3);
+--class-offset-priors gives the 2D references different offset priors per class. The default
 (class256) is the headline line.
 """
 import argparse
