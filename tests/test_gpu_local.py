@@ -129,7 +129,12 @@ def local360_case():
     return wl, wl.run_oracle()
 
 
-@pytest.mark.parametrize("prec", [0, 2])
+# precision -> (dLL, weighted sums / backprojection) relative bars: FP32 and FP16x3 at the FP32
+# bars; BF16x3 (16 mantissa bits per operand, diff2 ~1e-5 relative) one decade looser
+LOCAL360_BARS = {0: (1e-5, 1e-4), 1: (1e-4, 1e-3), 2: (1e-5, 1e-4)}
+
+
+@pytest.mark.parametrize("prec", [0, 1, 2])
 def test_local360_shapes(local360_case, gpu_device, prec):
     """The local-refinement configuration's own shapes (360^2, HEALPix 5 = 2.36 M orientations,
     sigma_ang 3 deg, 113 translations, P_c 5,007) on 3 particles; 60-blob references keep the
@@ -139,9 +144,13 @@ def test_local360_shapes(local360_case, gpu_device, prec):
     wl.cfg.precision = prec
     g = wl.run_gpu(dev=gpu_device)
     wl.cfg.precision = 0
+    tol_dll, tol_sum = LOCAL360_BARS[prec]
     assert np.array_equal(o["out"]["best_hidden"], g["out"]["best_hidden"])
-    assert H.rel(g["out"]["dLL"], o["out"]["dLL"]) < 1e-5
+    e_dll = H.rel(g["out"]["dLL"], o["out"]["dLL"])
+    e_bd, e_bw = H.rel_norm(g["bp"][0][0], o["bp"][0][0]), H.rel_norm(g["bp"][0][1], o["bp"][0][1])
+    print(f"local360 precision {prec}: dLL rel {e_dll:.2e} BP data {e_bd:.2e} weight {e_bw:.2e}")
+    assert e_dll < tol_dll
     for k, v in o["wsum"].items():
         if np.abs(v).sum() > 0:
-            assert H.rel_norm(g["wsum"][k], v) < 1e-4, k
-    assert H.rel_norm(g["bp"][0][1], o["bp"][0][1]) < 1e-4
+            assert H.rel_norm(g["wsum"][k], v) < tol_sum, k
+    assert e_bd < tol_sum and e_bw < tol_sum

@@ -91,6 +91,7 @@ def test_multiclass_and_offsets(gpu_device):
     assert agree >= 0.999
     assert H.rel(g["out"]["dLL"], o["out"]["dLL"]) < 1e-5
     for c in range(2):
+        assert H.rel_norm(g["bp"][c][0], o["bp"][c][0]) < 1e-4
         assert H.rel_norm(g["bp"][c][1], o["bp"][c][1]) < 1e-4
     for k in ("pdf_class", "sigma2_offset", "sigma2_noise"):
         assert H.rel_norm(g["wsum"][k], o["wsum"][k]) < 1e-4
@@ -189,4 +190,31 @@ def test_class256_shapes(class256_case, gpu_device, prec):
             assert H.rel_norm(g["wsum"][k], v) < 1e-4, k
     for c in range(wl.cfg.nr_classes):
         if np.abs(o["bp"][c][1]).sum() > 0:
+            assert H.rel_norm(g["bp"][c][0], o["bp"][c][0]) < 1e-4, c
             assert H.rel_norm(g["bp"][c][1], o["bp"][c][1]) < 1e-4, c
+
+
+def test_norm_correction(gpu_device):
+    """--norm: images scaled by avg_norm_correction / norm_correction before masking
+    (ml_optimiser.cpp:5960-5966) and the updated norm_correction = old / avg * sqrt(2 wsum_norm)
+    (ml_optimiser.cpp:8829-8841) with its average summed into wsum_model.avg_norm_correction."""
+    wl = H.Workload("refine64", n=12, seed_offset=21, old_offsets=True)
+    wl.cfg.do_norm_correction = 1
+    wl.cfg.avg_norm_correction = 0.93
+    rng = np.random.default_rng(7)
+    for i in range(wl.n):
+        wl.pin[i].norm_correction = float(rng.uniform(0.8, 1.25))
+    o = wl.run_oracle()
+    g = wl.run_gpu(dev=gpu_device)
+    assert np.array_equal(o["out"]["best_hidden"], g["out"]["best_hidden"])
+    nc_in = np.array([wl.pin[i].norm_correction for i in range(wl.n)])
+    assert np.all(np.abs(o["out"]["norm_correction"] - nc_in) > 1e-3)   # the update ran
+    assert H.rel(g["out"]["norm_correction"], o["out"]["norm_correction"]) < 1e-4
+    assert H.rel(g["out"]["dLL"], o["out"]["dLL"]) < 1e-5
+    assert g["wsum"]["avg_norm_correction"][0] == pytest.approx(o["wsum"]["avg_norm_correction"][0], rel=1e-4)
+    assert o["wsum"]["avg_norm_correction"][0] == pytest.approx(o["out"]["norm_correction"].sum(), rel=1e-12)
+    for k, v in o["wsum"].items():
+        if np.abs(v).sum() > 0:
+            assert H.rel_norm(g["wsum"][k], v) < 1e-4, k
+    assert H.rel_norm(g["bp"][0][0], o["bp"][0][0]) < 1e-4
+    assert H.rel_norm(g["bp"][0][1], o["bp"][0][1]) < 1e-4

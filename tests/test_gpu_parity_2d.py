@@ -134,6 +134,7 @@ def test_fp16x3_2d(class2d_pair, gpu_device):
     assert (o["out"]["best_hidden"] == g["out"]["best_hidden"]).mean() >= 0.999
     for c in range(wl.cfg.nr_classes):
         if np.abs(o["bp"][c][1]).sum() > 0:
+            assert H.rel_norm(g["bp"][c][0], o["bp"][c][0]) < 1e-4
             assert H.rel_norm(g["bp"][c][1], o["bp"][c][1]) < 1e-4
 
 
@@ -183,4 +184,5 @@ def test_large_and_ragged_batches(class2d64_many, gpu_device, monkeypatch, batch
         if np.abs(v).sum() > 0:
             assert H.rel_norm(g["wsum"][k], v) < 1e-4, k
     for c in range(wl.cfg.nr_classes):
+        assert H.rel_norm(g["bp"][c][0], o["bp"][c][0]) < 1e-4
         assert H.rel_norm(g["bp"][c][1], o["bp"][c][1]) < 1e-4

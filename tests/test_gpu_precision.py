@@ -9,7 +9,9 @@ weights exp(-(d - min d)) / sum, significant-set agreement and max-weight pose a
 Tolerances. FP16x3 with per-row power-of-two scaling keeps ~22 mantissa bits per operand,
 below FP32 rounding of the accumulator, so it is held to the FP32 bar (diff2 1e-5 relative,
 >= 99.9% identical poses). BF16x3 keeps ~16 bits per operand: its diff2 is held to 1e-4
-relative and its poses to the mixed-precision bar of >= 99% agreement.
+relative and its poses to the mixed-precision bar of >= 99% agreement. The same bars, one
+decade apart, hold the normalised weights (absolute), dLL and the backprojected data and weight
+volumes (relative, Frobenius).
 """
 import numpy as np
 import pytest
@@ -18,10 +20,11 @@ import estep_harness as H
 
 pytestmark = pytest.mark.gpu
 
+# name -> (precision, diff2 rel, pose agreement, normalised weight abs, dLL rel, BP rel)
 VARIANTS = {
-    "fp32": (0, 1e-5, 0.999),
-    "bf16x3": (1, 1e-4, 0.99),
-    "fp16x3": (2, 1e-5, 0.999),
+    "fp32": (0, 1e-5, 0.999, 1e-4, 1e-5, 1e-4),
+    "bf16x3": (1, 1e-4, 0.99, 1e-3, 1e-4, 1e-3),
+    "fp16x3": (2, 1e-5, 0.999, 1e-4, 1e-5, 1e-4),
 }
 
 
@@ -43,7 +46,7 @@ def oracle_ref():
 
 @pytest.mark.parametrize("name", list(VARIANTS))
 def test_split_precision_vs_oracle(name, oracle_ref, gpu_device):
-    prec, tol_d2, tol_pose = VARIANTS[name]
+    prec, tol_d2, tol_pose, tol_w, tol_dll, tol_bp = VARIANTS[name]
     wl = H.Workload("refine64", n=24, seed_offset=3, precision=prec)
     o = oracle_ref
     g = wl.run_gpu(dev=gpu_device, n_debug=24, fine_cap=16384)
@@ -57,7 +60,12 @@ def test_split_precision_vs_oracle(name, oracle_ref, gpu_device):
     pose = (o["out"]["best_hidden"] == g["out"]["best_hidden"]).mean()
     e_dll = H.rel(g["out"]["dLL"], o["out"]["dLL"])
     e_bp = H.rel_norm(g["bp"][0][1], o["bp"][0][1])
+    e_bd = H.rel_norm(g["bp"][0][0], o["bp"][0][0])
     print(f"\n[precision {name}] coarse diff2 max rel {e_d2:.3e}  norm weight max abs {e_w:.3e}  "
-          f"sig sets {sig:.4f}  pose {pose:.4f}  dLL rel {e_dll:.3e}  BP weight rel {e_bp:.3e}")
+          f"sig sets {sig:.4f}  pose {pose:.4f}  dLL rel {e_dll:.3e}  BP data rel {e_bd:.3e}  "
+          f"BP weight rel {e_bp:.3e}")
     assert e_d2 < tol_d2
     assert pose >= tol_pose
+    assert e_w < tol_w
+    assert e_dll < tol_dll
+    assert e_bd < tol_bp and e_bp < tol_bp
