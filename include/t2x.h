@@ -234,6 +234,21 @@ int t2x_abi_version(void);
 int t2x_projector_set(t2x_device* dev, int iclass, int ref_dim, int pad_size, int r_max,
                       double padding_factor, const float* re, const float* im);
 
+/* No counterpart in the reference. t2x_projector_set for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc):
+ * it calls no hipMalloc, hipFree or allocating library routine. Class iclass must already hold a
+ * projector volume of the same ref_dim and pad_size (an earlier t2x_projector_set allocated it;
+ * otherwise T2X_ESTATE, or T2X_EINVAL for another shape). The re/im planes (host or device
+ * memory) are staged through a static device array of 2 x 8 Mi floats (64 MiB, held by each GPU
+ * the library runs on until the process ends) in chunks, so every volume size is accepted.
+ * Every call on one GPU shares that array: calls that overlap on one GPU (several handles or
+ * threads) race. The resulting volume equals t2x_projector_set's bit for bit. */
+int t2x_projector_set_noalloc(t2x_device* dev, int iclass, int ref_dim, int pad_size, int r_max,
+                              double padding_factor, const float* re, const float* im);
+/* Diagnostic copy of class iclass's projector volume in the layout t2x_projector_set takes
+ * (tests; no reference counterpart). */
+int t2x_debug_projector_get(t2x_device* dev, int iclass, float* re, float* im);
+
 /* ---------------------------------------------------------------------------------------
  * Backprojector (AccBackprojector, src/acc/acc_backprojector.h:15-87)
  * ------------------------------------------------------------------------------------- */

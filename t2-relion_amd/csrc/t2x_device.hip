@@ -17,6 +17,41 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 
 using namespace t2x;
 
+namespace t2x {
+
+// Staging array of t2x_projector_set_noalloc: the re and im planes of one chunk of
+// kNoallocChunk elements each (2 x 32 MiB per GPU, held by every GPU the library runs on until
+// the process ends). Host code never passes the symbol itself to a kernel or a copy (on the host
+// it names a stand-in, not device memory): each device's address comes from
+// hipGetSymbolAddress, taken once per device with that device current.
+constexpr size_t kNoallocChunk = size_t(8) << 20;
+static __device__ __align__(256) float scratch[2 * kNoallocChunk];
+
+static std::mutex g_scratch_mtx;
+static std::map<int, float*> g_scratch_addr;   // device id -> address of that device's copy
+
+// Called with device_id current (t2x_device_open).
+static int noalloc_scratch_address(int device_id, float** out)
+{
+    std::lock_guard<std::mutex> lk(g_scratch_mtx);
+    auto it = g_scratch_addr.find(device_id);
+    if (it != g_scratch_addr.end()) {
+        *out = it->second;
+        return T2X_OK;
+    }
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) {
+        set_error(std::string("hipGetSymbolAddress of the staging array failed: ") + hipGetErrorString(e));
+        return T2X_EDEVICE;
+    }
+    g_scratch_addr[device_id] = static_cast<float*>(p);
+    *out = static_cast<float*>(p);
+    return T2X_OK;
+}
+
+}  // namespace t2x
+
 extern "C" const char* t2x_last_error(void) { return g_last_error.c_str(); }
 extern "C" int t2x_abi_version(void) { return T2X_ABI_VERSION; }
 
@@ -31,8 +66,11 @@ extern "C" int t2x_device_open(int device_id, t2x_device** out)
         return T2X_EINVAL;
     }
     T2X_HIP(hipSetDevice(device_id));
+    float* scratch_ptr = nullptr;
+    T2X_TRY(noalloc_scratch_address(device_id, &scratch_ptr));
     auto* d = new Device();
     d->id = device_id;
+    d->noalloc_scratch = scratch_ptr;
     if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
         delete d;
         set_error("hipStreamCreate failed");
@@ -101,15 +139,17 @@ static int check_dims(int iclass, int ref_dim, int pad, int r_max, double pf)
 }
 
 // Interleaves re/im into the projector layout: a 2D plane row-major, a 3D volume in 4x4x4 bricks.
+// Elements [base, base + n) of the caller's layout, re[k]/im[k] holding element base + k.
 __global__ void k_interleave(const float* __restrict__ re, const float* __restrict__ im,
-                             float2* __restrict__ out, size_t n, int ref_dim, int pad)
+                             float2* __restrict__ out, size_t base, size_t n, int ref_dim, int pad)
 {
-    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
-    if (ref_dim == 2) { out[i] = make_float2(re[i], im[i]); return; }
+    const size_t k = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const size_t i = base + k;
+    if (ref_dim == 2) { out[i] = make_float2(re[k], im[k]); return; }
     const int xdim = pad / 2 + 1;
     const int x = (int)(i % xdim), y = (int)((i / xdim) % pad), z = (int)(i / ((size_t)xdim * pad));
-    out[vol_brick_index(x, y, z, (xdim + 3) / 4, (pad + 3) / 4)] = make_float2(re[i], im[i]);
+    out[vol_brick_index(x, y, z, (xdim + 3) / 4, (pad + 3) / 4)] = make_float2(re[k], im[k]);
 }
 
 extern "C" int t2x_projector_set(t2x_device* h, int iclass, int ref_dim, int pad_size, int r_max,
@@ -148,11 +188,83 @@ extern "C" int t2x_projector_set(t2x_device* h, int iclass, int ref_dim, int pad
     }
     T2X_HIP(hipMemcpyAsync(dre, re, n * sizeof(float), hipMemcpyDefault, d->stream));
     T2X_HIP(hipMemcpyAsync(dim, im, n * sizeof(float), hipMemcpyDefault, d->stream));
-    k_interleave<<<(unsigned)((n + 255) / 256), 256, 0, d->stream>>>(dre, dim, c.proj, n, ref_dim, pad_size);
+    k_interleave<<<(unsigned)((n + 255) / 256), 256, 0, d->stream>>>(dre, dim, c.proj, 0, n, ref_dim, pad_size);
     T2X_HIP(hipGetLastError());
     T2X_HIP(hipStreamSynchronize(d->stream));
     (void)hipFree(dre);
     (void)hipFree(dim);
+    return T2X_OK;
+}
+
+// The same upload without any allocation: the volume must exist (an earlier t2x_projector_set of
+// this class with the same ref_dim and pad_size) and the planes pass through the device's static
+// staging array in chunks of kNoallocChunk elements, so every volume size is served.
+extern "C" int t2x_projector_set_noalloc(t2x_device* h, int iclass, int ref_dim, int pad_size, int r_max,
+                                         double pf, const float* re, const float* im)
+{
+    if (!h || !re || !im) { set_error("null argument"); return T2X_EINVAL; }
+    T2X_TRY(check_dims(iclass, ref_dim, pad_size, r_max, pf));
+    Device* d = reinterpret_cast<Device*>(h);
+    std::lock_guard<std::mutex> lk(d->mtx);
+    auto it = d->classes.find(iclass);
+    if (it == d->classes.end() || !it->second.proj) {
+        set_error("t2x_projector_set_noalloc: class " + std::to_string(iclass) +
+                  " has no projector volume (it would need an allocation: call t2x_projector_set once first)");
+        return T2X_ESTATE;
+    }
+    ClassVolumes& c = it->second;
+    if (c.p_pad != pad_size || c.p_ref_dim != ref_dim) {
+        set_error("t2x_projector_set_noalloc: class " + std::to_string(iclass) + " holds a volume of ref_dim " +
+                  std::to_string(c.p_ref_dim) + " and pad_size " + std::to_string(c.p_pad) +
+                  "; another shape would need a new allocation");
+        return T2X_EINVAL;
+    }
+    if (!d->noalloc_scratch) { set_error("staging array address not set on this handle"); return T2X_ESTATE; }
+    T2X_HIP(hipSetDevice(d->id));
+    c.p_rmax = r_max;
+    c.p_pf = pf;
+    c.proj_version = ++d->proj_counter;
+    d->coarse_key.clear();
+    const size_t n = vol_elems(ref_dim, pad_size);
+    float* sre = d->noalloc_scratch;
+    float* sim = d->noalloc_scratch + kNoallocChunk;
+    // stream order makes the next chunk's copies wait for the previous chunk's kernel
+    for (size_t i0 = 0; i0 < n; i0 += kNoallocChunk) {
+        const size_t m = std::min(kNoallocChunk, n - i0);
+        T2X_HIP(hipMemcpyAsync(sre, re + i0, m * sizeof(float), hipMemcpyDefault, d->stream));
+        T2X_HIP(hipMemcpyAsync(sim, im + i0, m * sizeof(float), hipMemcpyDefault, d->stream));
+        k_interleave<<<(unsigned)((m + 255) / 256), 256, 0, d->stream>>>(sre, sim, c.proj, i0, m, ref_dim, pad_size);
+        T2X_HIP(hipGetLastError());
+    }
+    T2X_HIP(hipStreamSynchronize(d->stream));
+    return T2X_OK;
+}
+
+// Projector volume of class iclass in the caller's layout (tests only).
+extern "C" int t2x_debug_projector_get(t2x_device* h, int iclass, float* re, float* im)
+{
+    if (!h || !re || !im) { set_error("null argument"); return T2X_EINVAL; }
+    Device* d = reinterpret_cast<Device*>(h);
+    std::lock_guard<std::mutex> lk(d->mtx);
+    auto it = d->classes.find(iclass);
+    if (it == d->classes.end() || !it->second.proj) { set_error("projector not set"); return T2X_ESTATE; }
+    const ClassVolumes& c = it->second;
+    const size_t n = vol_elems(c.p_ref_dim, c.p_pad);
+    const size_t ne = c.p_ref_dim == 3 ? vol_brick_elems(c.p_pad) : n;
+    T2X_HIP(hipSetDevice(d->id));
+    T2X_HIP(hipStreamSynchronize(d->stream));
+    std::vector<float2> v(ne);
+    T2X_HIP(hipMemcpy(v.data(), c.proj, ne * sizeof(float2), hipMemcpyDeviceToHost));
+    const int pad = c.p_pad, xdim = pad / 2 + 1;
+    for (size_t i = 0; i < n; i++) {
+        size_t src = i;
+        if (c.p_ref_dim == 3) {
+            const int x = (int)(i % xdim), y = (int)((i / xdim) % pad), z = (int)(i / ((size_t)xdim * pad));
+            src = vol_brick_index(x, y, z, (xdim + 3) / 4, (pad + 3) / 4);
+        }
+        re[i] = v[src].x;
+        im[i] = v[src].y;
+    }
     return T2X_OK;
 }
 

@@ -126,6 +126,9 @@ namespace cnn { struct Net; }
 struct Device {
     int id = 0;
     hipStream_t stream = nullptr;
+    // this device's copy of the static staging array of t2x_projector_set_noalloc (its address
+    // from hipGetSymbolAddress with the device current; the symbol itself is not a device address)
+    float* noalloc_scratch = nullptr;
     hipEvent_t ev[16];
     std::map<int, ClassVolumes> classes;
     uint64_t proj_counter = 0;

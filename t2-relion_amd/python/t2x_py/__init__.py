@@ -212,6 +212,9 @@ def load(path=None):
         "t2x_device_close": (C.c_int, [vp]),
         "t2x_projector_set": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
                                         C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "t2x_projector_set_noalloc": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double,
+                                                C.POINTER(C.c_float), C.POINTER(C.c_float)]),
+        "t2x_debug_projector_get": (C.c_int, [vp, C.c_int, C.POINTER(C.c_float), C.POINTER(C.c_float)]),
         "t2x_backprojector_init": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_double]),
         "t2x_backprojector_get": (C.c_int, [vp, C.c_int, PD, PD, PD]),
         "t2x_backprojector_clear": (C.c_int, [vp, C.c_int]),
