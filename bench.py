@@ -133,8 +133,10 @@ def set_projectors(lib, dev, sz, ds, ref_dim=3):
                                       C.cast(C.c_void_p(dim.value + off), C.POINTER(C.c_float))), "projector")
 
 
-def run_step(lib, dev, cfg, sz, ds, n, outs, ws, first=0):
-    """One E-step over particles [first, first+n): projectors set, backprojectors cleared."""
+def run_step(lib, dev, cfg, sz, ds, n, outs, ws, first=0, readback=False):
+    """One E-step over particles [first, first+n): projectors set, backprojectors cleared; with
+    readback, the backprojected volumes of every class copied to host FP64 arrays at the end
+    (AccBackprojector::getMdlData into wsum_model.BPref)."""
     set_projectors(lib, dev, sz, ds, cfg.ref_dim)
     for c in range(cfg.nr_classes):
         T.check(lib.t2x_backprojector_clear(dev, c), "bp clear")
@@ -145,6 +147,15 @@ def run_step(lib, dev, cfg, sz, ds, n, outs, ws, first=0):
     img = C.c_void_p(ds["dimg"].value + first * N * N * 4)
     pin = C.cast(C.c_void_p(C.addressof(ds["pin"]) + first * C.sizeof(T.ParticleIn)), C.POINTER(T.ParticleIn))
     T.check(lib.t2x_estep(dev, C.byref(cfg), C.byref(model), n, img, 1, pin, outs, C.byref(ws), None), "estep")
+    if readback:
+        pad = sz["pad_size"]
+        nvox = (pad * pad if cfg.ref_dim == 3 else pad) * (pad // 2 + 1)
+        bufs = ds.setdefault("bp_host", {})
+        for c in range(cfg.nr_classes):
+            if c not in bufs:
+                bufs[c] = (np.zeros(nvox), np.zeros(nvox), np.zeros(nvox))
+            re, im, wt = bufs[c]
+            T.check(lib.t2x_backprojector_get(dev, c, T.dptr(re), T.dptr(im), T.dptr(wt)), "bp readback")
 
 
 def train_proposal(lib, dev, cfg, sz, ds, w, steps=2000, batch=256, n_proj=200000, seed=2024, lr=3e-3,
@@ -303,8 +314,21 @@ def quality_phase(lib, dev, cfg, sz, ds, paths, nq):
     return res
 
 
-def cpu_baseline(w, cfg, sz, seconds_target=20.0):
-    """FP64 oracle on a bounded sample (same workload shape), one particle per thread."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, cfg, sz, n_particles=64):
+    """FP64 oracle on a bounded sample of the same workload (64 particles), one particle per
+    OpenMP thread as relion_refine's thread pool does, on every core of this process's CPU
+    share (OMP_NUM_THREADS, else the affinity mask)."""
     sys.path.insert(0, os.path.join(REPO, "oracle"))
     sys.path.insert(0, os.path.join(REPO, "tests"))
     try:
@@ -312,16 +336,19 @@ def cpu_baseline(w, cfg, sz, seconds_target=20.0):
     except Exception as e:  # oracle not built
         return {"value": None, "unit": "particles/s", "cores": 0, "kind": "port",
                 "sample": f"unavailable: {e}"}
-    ncpu = len(os.sched_getaffinity(0))
-    threads = max(1, min(16, ncpu))
-    wl = H.Workload(w_name(w), n=threads, seed_offset=99, current_size=cfg.current_size,
+    affinity = len(os.sched_getaffinity(0))
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = max(1, min(share, affinity) if share > 0 else affinity)
+    wl = H.Workload(w_name(w), n=n_particles, seed_offset=99, current_size=cfg.current_size,
                     coarse_size=cfg.coarse_size)
     t0 = time.time()
     wl.run_oracle(nthreads=threads)
     dt = time.time() - t0
-    return {"value": threads / dt, "unit": "particles/s", "cores": threads, "kind": "port",
-            "sample": f"{threads} particles of the same workload (FP64 oracle, {threads} OpenMP "
-                      f"threads, one particle each), {dt:.1f} s"}
+    return {"value": n_particles / dt, "unit": "particles/s", "cores": threads, "kind": "port",
+            "nproc": os.cpu_count(), "affinity_cpus": affinity, "omp_num_threads": share or None,
+            "cpu_model": cpu_model(),
+            "sample": f"{n_particles} particles of the same workload (FP64 oracle, {threads} OpenMP "
+                      f"threads = this process's CPU share, one particle per thread), {dt:.1f} s"}
 
 
 def w_name(w):
@@ -355,8 +382,8 @@ def main():
     ap.add_argument("--save-proposal", default="", help="write the trained proposal parameters (.npy)")
     ap.add_argument("--load-proposal", default="",
                     help="use proposal parameters saved by an earlier run instead of training")
-    ap.add_argument("--quality-particles", type=int, default=2000,
-                    help="particles of the half-map FSC phase (0: skip)")
+    ap.add_argument("--quality-particles", type=int, default=-1,
+                    help="particles of the half-map FSC phase (-1: all of them; 0: skip)")
     ap.add_argument("--class-offset-priors", action="store_true",
                     help="2D references: a different offset prior per class (uniform within one "
                          "offset step), as in every iteration after the first")
@@ -434,14 +461,14 @@ def main():
         configure(cfg, pname)
         for i in range(args.warmup):
             t = time.time()
-            run_step(lib, dev, cfg, sz, ds, nw, outs, ws)
+            run_step(lib, dev, cfg, sz, ds, nw, outs, ws, readback=True)
             T.check(lib.t2x_device_synchronize(dev))
             log(f"[{pname}] warmup {i}: {nw} particles in {time.time() - t:.2f} s")
         T.check(lib.t2x_stats_reset(dev))
         T.check(lib.t2x_device_synchronize(dev))
         t0 = time.time()
         for i in range(args.steps):
-            run_step(lib, dev, cfg, sz, ds, n, outs, ws)
+            run_step(lib, dev, cfg, sz, ds, n, outs, ws, readback=True)
             log(f"[{pname}] step {i} done at {time.time() - t0:.2f} s")
         T.check(lib.t2x_device_synchronize(dev))
         dt = time.time() - t0
@@ -454,10 +481,11 @@ def main():
     r = runs[head]
     st, dt, value = r["st"], r["dt"], r["value"]
     # dominant kernel: the coarse cross-term GEMM; algorithmic 4*T*O*P flops per scored particle
-    # (x3 MFMA products for the split-precision forms)
+    # (the counting convention: one complex inner product per (translation, orientation) pair);
+    # the split-precision forms issue 3 MFMA products for each, reported as their own figure
     launches = max(1, st.n_coarse_diff2_launches)
     nprod = 1 if head == "fp32" else 3
-    flop_total = nprod * st.coarse_diff2_flop
+    flop_total = st.coarse_diff2_flop
     if head.startswith("learned"):   # only the scored bins are computed
         n_bins = cfg.nr_classes * ds["info"].nr_dir // 4
         flop_total *= st.proposal_bins_scored / max(1.0, st.n_proposal_particles * n_bins)
@@ -482,11 +510,19 @@ def main():
             "kernel": ("coarse diff2 of local searches: (particle, orientation) rows x translations, k_local_zrows + "
                        + ("k_gemm_tn (FP32 MFMA)" if head == "fp32" else "k_gemm_x3 128x128 (split MFMA)") + " + k_local_diff2" if local
                        else "coarse diff2 cross term, " + KERNELS[head]),
-            "avg_launch_ms": avg_ms, "launches": launches, "flop_per_launch": flop_per_launch}
+            "avg_launch_ms": avg_ms, "launches": launches, "flop_per_launch": flop_per_launch,
+            "flop_convention": "algorithmic 4*T*O*P_c per scored particle (one complex product per pair)",
+            "mfma_products_per_pair": nprod,
+            "mfma_product_utilisation": nprod * achieved / peak}
     stage_roof = stage_rooflines(st, cfg, ds["info"], r["out"], args.steps)
-    stages = {k: getattr(st, k) for k in ("ms_preprocess", "ms_coarse_project", "ms_coarse_diff2",
-                                           "ms_coarse_weights", "ms_fine_diff2", "ms_fine_weights",
-                                           "ms_store", "ms_proposal")}
+    # per step, a partition of the step's wall time: the GPU stages (HIP events on the library's
+    # stream) and the rest (host work between launches, synchronisations, uploads, the
+    # backprojector readback); the proposal's host time lies inside the coarse diff2 interval
+    gpu_keys = ("ms_preprocess", "ms_coarse_project", "ms_coarse_diff2", "ms_coarse_weights",
+                "ms_fine_diff2", "ms_fine_weights", "ms_store")
+    stages = {k: getattr(st, k) / args.steps for k in gpu_keys}
+    stages["other_host_sync_readback"] = dt * 1e3 / args.steps - sum(stages.values())
+    stages["of_which_in_coarse_diff2_proposal_host"] = st.ms_proposal / args.steps
     nsig = st.sig_coarse / max(1, st.n_particles)
     step_deg = 360.0 / (6 * 2 ** cfg.healpix_order) if cfg.ref_dim == 3 else cfg.psi_step
     ref = "fp16x3" if "fp16x3" in runs else ("fp32" if "fp32" in runs else None)
@@ -517,8 +553,8 @@ def main():
             v["local_rows_per_slice"] = rr["st"].local_rows / max(1.0, rr["st"].local_slices)
         variants[pname] = v
     quality = None
-    if args.quality_particles > 0 and cfg.ref_dim == 3 and not local:
-        nq = min(n, args.quality_particles)
+    if args.quality_particles != 0 and cfg.ref_dim == 3 and not local:
+        nq = n if args.quality_particles < 0 else min(n, args.quality_particles)
         q = quality_phase(lib, dev, cfg, sz, ds, list(runs.keys()), nq)
         quality = {"particles": nq, "fsc_0143_A_per_class": {k: v["fsc_res"] for k, v in q.items()}}
         if "fp32" in q:

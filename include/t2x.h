@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define T2X_ABI_VERSION 2
+#define T2X_ABI_VERSION 3
 
 enum {
     T2X_OK = 0,
@@ -89,9 +89,18 @@ typedef struct t2x_estep_config {
     long   maximum_significants;   /* <= 0: unlimited                                    */
     double particle_diameter;      /* A, circular soft mask                              */
     int    width_mask_edge;        /* px                                                 */
+    int    do_zero_mask;           /* 1 (--zero_mask): the mask's background is its      */
+                                   /* average value; 0 (relion_refine's default): noise  */
+                                   /* with the optics group's sigma2_noise spectrum,     */
+                                   /* drawn per particle from (random_seed, part_id)     */
+    uint64_t random_seed;          /* seed of that background noise                      */
     double sigma2_fudge;           /* 1                                                  */
     int    do_norm_correction;     /* 0/1                                                */
     double avg_norm_correction;    /* mymodel.avg_norm_correction                        */
+    int    do_scale_correction;    /* 0/1 (--scale): references scaled per image group   */
+    int    nr_groups;              /* image groups (scale correction); 0 means 1         */
+    int    nr_optics_groups;       /* optics groups (sigma2_noise); 0 means 1. Every     */
+                                   /* group has the box ori_size and pixel_size          */
     int    do_ctf_correction;      /* 0/1                                                */
     double sigma2_offset;          /* mymodel.sigma2_offset (A^2)                        */
     int    orientational_prior_mode; /* T2X_NOPRIOR: global search over pdf_direction;   */
@@ -105,7 +114,6 @@ typedef struct t2x_estep_config {
     int    gemm_tile;              /* split-precision GEMM variant: 0 autotuned (fastest  */
                                    /* on first use per shape), 128, 256, 257 or 258      */
     int    gemm_group;             /* its tile-walk group in tile rows (0: default)      */
-    int    reserved[2];
     /* local searches: sqrt(mymodel.sigma2_rot / sigma2_tilt / sigma2_psi) in degrees;
      * sigma_rot and sigma_tilt both > 0 (a Gaussian on the angle between the directions) or
      * both 0 (every direction); sigma_psi 0: every psi angle */
@@ -114,12 +122,18 @@ typedef struct t2x_estep_config {
 
 /* Model arrays of the current iteration (host, caller-owned, read-only). */
 typedef struct t2x_model {
-    const double* sigma2_noise;    /* [ori_size/2+1]            mymodel.sigma2_noise[0]  */
+    const double* sigma2_noise;    /* [nr_optics_groups][ori_size/2+1]                   */
+                                   /*                            mymodel.sigma2_noise    */
     const double* pdf_class;       /* [K]                        mymodel.pdf_class       */
     const double* pdf_direction;   /* [K * n_dir]                mymodel.pdf_direction   */
     const double* prior_offset_class; /* [2K] x,y per class for 2D refs (may differ per   */
                                    /* class), compared with old offset + translation as  */
                                    /* ml_optimiser.cpp:7627-7653 does; may be NULL       */
+    const double* scale_correction;   /* [nr_groups] mymodel.scale_correction (used with   */
+                                   /* do_scale_correction; each in (0, 10000])           */
+    const double* data_vs_prior_class; /* [K][ori_size/2+1] mymodel.data_vs_prior_class: */
+                                   /* the scale sums run over shells where it is > 3     */
+                                   /* (do_scale_correction; NULL: no shell qualifies)    */
 } t2x_model;
 
 /* Per-particle input metadata (one image per particle). */
@@ -131,6 +145,10 @@ typedef struct t2x_particle_in {
     double rot, tilt, psi;                   /* current angles (METADATA_ROT/TILT/PSI)     */
     double prior_rot, prior_tilt, prior_psi; /* METADATA_ROT/TILT/PSI_PRIOR, 999 = unset:  */
                                              /* the local search centres on rot/tilt/psi  */
+    int    group_id;                         /* image group, 0-based (getGroupId)          */
+    int    optics_group;                     /* optics group, 0-based (getOpticsGroup)     */
+    long   part_id;                          /* particle number in the data set: the key   */
+                                             /* of its mask-background noise               */
 } t2x_particle_in;
 
 /* Per-particle results (the exp_metadata row written by storeWeightedSums). */
@@ -153,8 +171,8 @@ typedef struct t2x_particle_out {
 
 /* Weighted sums the E-step adds into (host, caller-owned, double). Any member may be NULL. */
 typedef struct t2x_wsum {
-    double* sigma2_noise;         /* [ori_size/2+1] wsum_model.sigma2_noise[0]            */
-    double* sumw_group;           /* [1]                                                  */
+    double* sigma2_noise;         /* [nr_optics_groups][ori_size/2+1] wsum_model.sigma2_noise */
+    double* sumw_group;           /* [nr_optics_groups] wsum_model.sumw_group             */
     double* pdf_class;            /* [K]                                                  */
     double* pdf_direction;        /* [K * n_dir]                                          */
     double* prior_offset_class;   /* [2K]  (2D references only)                           */
@@ -162,6 +180,8 @@ typedef struct t2x_wsum {
     double* avg_norm_correction;  /* [1]                                                  */
     double* LL;                   /* [1]                                                  */
     double* ave_Pmax;             /* [1]                                                  */
+    double* wsum_signal_product;  /* [nr_groups] XA sums of the scale correction          */
+    double* wsum_reference_power; /* [nr_groups] AA sums of the scale correction          */
 } t2x_wsum;
 
 /* Optional inspection buffers for parity tests (host, caller-owned; NULL = skip).
@@ -177,6 +197,8 @@ typedef struct t2x_estep_debug {
     double* coarse_min_diff2; /* [n_debug]                                                                */
     double* coarse_sum_weight;/* [n_debug]                                                                */
     double* coarse_sig_weight;/* [n_debug]                                                                */
+    double* fine_weight;      /* [n_debug][fine_capacity] fine weights (the CPU oracle fills it; libt2x  */
+                              /* leaves it untouched)                                                    */
 } t2x_estep_debug;
 
 /* Sampling sizes implied by a configuration (HealpixSampling::NrDirections etc.). */

@@ -60,7 +60,8 @@ int t2x_star_set_int(t2x_star* s, const char* block, const char* label, long row
  * getMetaAndImageDataSubset reads them (ml_optimiser.cpp:10295-10400): CTF from the row and its
  * optics group (data_optics), offsets and priors converted from Angstrom to pixels with the
  * group's pixel size, angles and angle priors (rlnAngleRot/Tilt/PsiPrior), norm correction (1
- * when absent), priors 999 when absent.
+ * when absent), priors 999 when absent, group_id / optics_group from rlnGroupNumber /
+ * rlnOpticsGroup (1-based in the file, 0-based here; 0 when absent), part_id = the row.
  * angpix[i] receives the pixel size of each row (may be NULL). */
 int t2x_star_particles(const t2x_star* s, long first, long n, t2x_particle_in* pin, double* angpix);
 /* rlnImageName of a row, "index@path": the 1-based index and the stack path. */

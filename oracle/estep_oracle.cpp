@@ -240,6 +240,7 @@ static int check_config(const t2x_estep_config* c)
     if (c->nr_classes < 1) return T2X_EINVAL;
     if (c->adaptive_oversampling < 0 || c->adaptive_oversampling > 1) return T2X_EINVAL;
     if (c->offset_step <= 0 || c->offset_range < 0 || c->psi_step <= 0) return T2X_EINVAL;
+    if (c->nr_groups < 0 || c->nr_optics_groups < 0) return T2X_EINVAL;
     if (c->orientational_prior_mode == T2X_PRIOR_ROTTILT_PSI) {
         /* local searches with one Gaussian on the angle between directions (sigma_rot and
          * sigma_tilt both set) or none (both 0); the rot-only / tilt-only branches are not restated */
@@ -871,9 +872,10 @@ extern "C" int orc_synth_reference(int N, double angpix, int pad, int r_max, dou
                 int r2 = x * x + y * y + z * z;
                 cplx v = 0;
                 if (r2 <= max_r2) {
+                    /* voxels beyond the low-pass edge are zero: their blob sum is not evaluated */
+                    const double lw = lowpass_weight(std::sqrt((double)r2) / pf, cutoff_shell);
                     double f = 1.0 / (pf * N);
-                    v = blob_ft(blobs, nb, x * f, y * f, z * f) * inv;
-                    v *= lowpass_weight(std::sqrt((double)r2) / pf, cutoff_shell);
+                    if (lw > 0.) v = blob_ft(blobs, nb, x * f, y * f, z * f) * inv * lw;
                 }
                 out[2 * idx] = v.real();
                 out[2 * idx + 1] = v.imag();
@@ -1014,8 +1016,54 @@ static void fft_real(const std::vector<double>& img, int N, std::vector<cplx>& F
     orc_fft2_r2c(N, img.data(), reinterpret_cast<double*>(F.data()));
 }
 
+/* The background of the soft mask when do_zero_mask is off (ml_optimiser.cpp:6173-6226): a noise
+ * image whose Fourier components (FFTW half layout, N x (N/2+1)) within shell N/2 are complex
+ * Gaussians of standard deviation sqrt(sigma2_fudge * sigma2_noise[shell]) per part, transformed
+ * back without normalisation. The reference draws them from the process-wide rand() stream,
+ * reseeded per particle (init_random_generator(random_seed + part_id)); here each component is
+ * the counter-based sample keyed by (random_seed, part_id, 2 q + part) with q the component's
+ * index, and the x = 0 and x = N/2 columns are made Hermitian (row -y the conjugate of row y, real
+ * at the self-conjugate frequencies) so that the real inverse transform is unambiguous. */
+static void background_noise(const t2x_estep_config* cfg, const double* sigma2, long part_id,
+                             std::vector<double>& noise)
+{
+    const int N = cfg->ori_size, xN = N / 2 + 1;
+    const uint64_t seed = cfg->random_seed ^ 0x6d61736b6e6f6973ULL;
+    std::vector<double> F((size_t)2 * N * xN, 0.);
+    for (int i = 0; i < N; i++) {
+        const int ip = (i < xN) ? i : i - N;
+        for (int j = 0; j < xN; j++) {
+            const int ires = R_ROUND(std::sqrt((double)(ip * ip + j * j)));
+            if (ires >= xN) continue;
+            const bool edge = (j == 0 || j == N / 2);
+            int src = i;                        /* row whose draws this component uses */
+            bool conj = false;
+            if (edge && ip < 0) { src = N - i; conj = true; }
+            const long q = (long)src * xN + j;
+            const double sd = std::sqrt(cfg->sigma2_fudge * sigma2[ires]);
+            double re = sd * orc_noise_sample(seed, part_id, 2 * q);
+            double im = sd * orc_noise_sample(seed, part_id, 2 * q + 1);
+            if (edge && (i == 0 || i == N / 2)) im = 0.;
+            F[2 * ((size_t)i * xN + j)] = re;
+            F[2 * ((size_t)i * xN + j) + 1] = conj ? -im : im;
+        }
+    }
+    noise.assign((size_t)N * N, 0.);
+    orc_fft2_c2r(N, F.data(), noise.data());
+}
+
+extern "C" int orc_background_noise(const t2x_estep_config* cfg, const double* sigma2_noise, long part_id,
+                                    double* noise)
+{
+    if (check_config(cfg) != T2X_OK || !sigma2_noise || !noise) return T2X_EINVAL;
+    std::vector<double> v;
+    background_noise(cfg, sigma2_noise, part_id, v);
+    std::copy(v.begin(), v.end(), noise);
+    return T2X_OK;
+}
+
 static void preprocess(const t2x_estep_config* cfg, const float* image, const t2x_particle_in& pin,
-                       Prep& P)
+                       const double* sigma2, Prep& P)
 {
     const int N = cfg->ori_size;
     const int cs = cfg->current_size;
@@ -1052,12 +1100,15 @@ static void preprocess(const t2x_estep_config* cfg, const float* image, const t2
     window_ft(Faux.data(), N, P.Fimg_nomask.data(), cs);
     center_by_sign(P.Fimg_nomask.data(), cs);
 
-    /* softMaskOutsideMap(img, radius, width) with the average-background fill */
+    /* softMaskOutsideMap(img, radius, width) with the average-background fill (--zero_mask),
+     * or with the noise image Mnoise (mask.cpp:43-97) */
     double radius = cfg->particle_diameter / (2. * cfg->pixel_size);
     double cw = (double)cfg->width_mask_edge;
     double radius_p = radius + cw;
+    std::vector<double> Mnoise;
+    if (!cfg->do_zero_mask) background_noise(cfg, sigma2, pin.part_id, Mnoise);
     double sum_bg = 0, sum = 0;
-    for (int i = 0; i < N; i++)
+    for (int i = 0; i < N && cfg->do_zero_mask; i++)
         for (int j = 0; j < N; j++) {
             int y = i - N / 2, x = j - N / 2;
             double r = std::sqrt((double)(x * x + y * y));
@@ -1070,17 +1121,18 @@ static void preprocess(const t2x_estep_config* cfg, const float* image, const t2
                 sum_bg += rc * v;
             }
         }
-    sum_bg /= sum;
+    if (cfg->do_zero_mask) sum_bg /= sum;
     for (int i = 0; i < N; i++)
         for (int j = 0; j < N; j++) {
             int y = i - N / 2, x = j - N / 2;
             double r = std::sqrt((double)(x * x + y * y));
             double& v = img[(size_t)i * N + j];
             if (r < radius) continue;
-            if (r > radius_p) v = sum_bg;
+            const double add = cfg->do_zero_mask ? sum_bg : Mnoise[(size_t)i * N + j];
+            if (r > radius_p) v = add;
             else {
                 double rc = 0.5 + 0.5 * std::cos(PI * (radius_p - r) / cw);
-                v = (1 - rc) * v + rc * sum_bg;
+                v = (1 - rc) * v + rc * add;
             }
         }
 
@@ -1110,12 +1162,14 @@ static void preprocess(const t2x_estep_config* cfg, const float* image, const t2
 }
 
 extern "C" int orc_preprocess(const t2x_estep_config* cfg, const float* image,
-                              const t2x_particle_in* pin, double* Fimg, double* Fimg_nomask,
-                              double* Fctf, double* power, double* highres, double* old_xy)
+                              const t2x_particle_in* pin, const double* sigma2_noise, double* Fimg,
+                              double* Fimg_nomask, double* Fctf, double* power, double* highres,
+                              double* old_xy)
 {
     if (check_config(cfg) != T2X_OK) return T2X_EINVAL;
+    if (!cfg->do_zero_mask && !sigma2_noise) return T2X_EINVAL;
     Prep P;
-    preprocess(cfg, image, *pin, P);
+    preprocess(cfg, image, *pin, sigma2_noise, P);
     size_t nc = P.Fimg.size();
     if (Fimg) std::memcpy(Fimg, P.Fimg.data(), nc * sizeof(cplx));
     if (Fimg_nomask) std::memcpy(Fimg_nomask, P.Fimg_nomask.data(), nc * sizeof(cplx));
@@ -1244,11 +1298,24 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
     const int nr_passes = AO > 0 ? 2 : 1;
     const double pix = cfg->pixel_size;
     const int N = cfg->ori_size;
-    const double* sigma2 = sh.model->sigma2_noise;
     const int nsig2 = N / 2 + 1;
+    /* the particle's optics group selects its noise spectrum (ml_optimiser.cpp:6703-6720) and
+     * its image group the scale of the references (:7119-7125, 8326-8350) */
+    const int n_og = cfg->nr_optics_groups > 0 ? cfg->nr_optics_groups : 1;
+    const int n_grp = cfg->nr_groups > 0 ? cfg->nr_groups : 1;
+    if (pin.optics_group < 0 || pin.optics_group >= n_og || pin.group_id < 0 || pin.group_id >= n_grp)
+        return T2X_EINVAL;
+    const double* sigma2 = sh.model->sigma2_noise + (size_t)pin.optics_group * nsig2;
+    double scale_d2 = 1., scale_st = 1., scale_raw = 1.;
+    if (cfg->do_scale_correction) {
+        scale_raw = sh.model->scale_correction[pin.group_id];
+        if (!(scale_raw > 0.) || scale_raw > 10000.) return T2X_EINVAL;
+        scale_d2 = scale_raw;
+        scale_st = std::max(0.001, scale_raw);   /* the store clamps small scales (:8333-8343) */
+    }
 
     Prep P;
-    preprocess(cfg, image, pin, P);
+    preprocess(cfg, image, pin, sigma2, P);
     /* the offset prior per class: the particle's own (3D references), or for 2D references the
      * class's prior_offset_class (ml_optimiser.cpp:7627-7631, 7678-7682, 8283-8287) */
     std::vector<double> qx(sh.cfg->nr_classes, P.prior_x), qy(sh.cfg->nr_classes, P.prior_y);
@@ -1317,60 +1384,67 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
         }
         const int nov_t = os ? over_t : 1, nov_r = os ? over_r : 1;
         double mymin = 99e99;
-        std::vector<cplx> Fref(np, 0.), Frefctf(np);
-        std::vector<double> orot, otilt, opsi;
-        double A[9];
-        for (int c = 0; c < K; c++) {
-            if (!(sh.model->pdf_class[c] > 0.)) continue;
-            for (long idir = 0; idir < Nd; idir++)
-                for (long ipsi = 0; ipsi < Npsi; ipsi++) {
-                    long iorientclass = (long)c * Norient + idir * Npsi + ipsi;
-                    double pdf_o = pdf_orient(c, idir, ipsi);
-                    bool proceed = true;
-                    if (ipass > 0) {
-                        proceed = false;
-                        for (long it = 0; it < Nt; it++)
-                            if (Msig[iorientclass * Nt + it]) { proceed = true; break; }
-                    }
-                    if (!(proceed && pdf_o > 0.)) continue;
-                    S.orientations(idir, ipsi, os, orot, otilt, opsi);
-                    for (int ir = 0; ir < nov_r; ir++) {
-                        orc_euler_angles2matrix(orot[ir], otilt[ir], opsi[ir], A);
-                        if (sh.clean) std::fill(Fref.begin(), Fref.end(), cplx(0, 0));
-                        project_into(&sh.proj[c], A, s, Fref.data(), sh.clean);
-                        for (size_t n = 0; n < np; n++) Frefctf[n] = Fref[n] * Fctf[n];
-                        long ihidden = iorientclass * Nt;
-                        for (long it = 0; it < Nt; it++, ihidden++) {
-                            if (ipass > 0 && !Msig[ihidden]) continue;
-                            for (int iot = 0; iot < nov_t; iot++) {
-                                const cplx* Fs = Fshift[it * nov_t + iot].data();
-                                double diff2 = P.highres_Xi2 / 2.;
-                                for (size_t n = 0; n < np; n++) {
-                                    double dr = Frefctf[n].real() - Fs[n].real();
-                                    double di = Frefctf[n].imag() - Fs[n].imag();
-                                    diff2 += (dr * dr + di * di) * 0.5 * Minv[n];
-                                }
-                                if (ipass == 0) {
-                                    Mw_coarse[ihidden] = diff2;
-                                } else {
-                                    auto itb = fine_index.find(ihidden);
-                                    int bi;
-                                    if (itb == fine_index.end()) {
-                                        bi = (int)fine_blocks.size();
-                                        fine_index[ihidden] = bi;
-                                        FineBlock fb;
-                                        for (int q = 0; q < 64; q++) fb.v[q] = -999.;
-                                        fine_blocks.push_back(fb);
-                                    } else bi = itb->second;
-                                    fine_blocks[bi].v[ir * over_t + iot] = diff2;
-                                    fine_dbg_d2.push_back(diff2);
-                                    fine_dbg_h.push_back(ihidden * nover + ir * over_t + iot);
-                                }
-                                if (diff2 < mymin) mymin = diff2;
-                            }
+        /* the diff2 of every (oversampled) sample of one (class, direction, psi) */
+        auto score = [&](int c, long idir, long ipsi, std::vector<cplx>& Fref, std::vector<cplx>& Frefctf,
+                         double& mn) {
+            std::vector<double> orot, otilt, opsi;
+            double A[9];
+            long iorientclass = (long)c * Norient + idir * Npsi + ipsi;
+            double pdf_o = pdf_orient(c, idir, ipsi);
+            bool proceed = true;
+            if (ipass > 0) {
+                proceed = false;
+                for (long it = 0; it < Nt; it++)
+                    if (Msig[iorientclass * Nt + it]) { proceed = true; break; }
+            }
+            if (!(proceed && pdf_o > 0.)) return;
+            S.orientations(idir, ipsi, os, orot, otilt, opsi);
+            for (int ir = 0; ir < nov_r; ir++) {
+                orc_euler_angles2matrix(orot[ir], otilt[ir], opsi[ir], A);
+                if (sh.clean) std::fill(Fref.begin(), Fref.end(), cplx(0, 0));
+                project_into(&sh.proj[c], A, s, Fref.data(), sh.clean);
+                for (size_t n = 0; n < np; n++) Frefctf[n] = Fref[n] * Fctf[n];
+                if (cfg->do_scale_correction)
+                    for (size_t n = 0; n < np; n++) Frefctf[n] *= scale_d2;
+                long ihidden = iorientclass * Nt;
+                for (long it = 0; it < Nt; it++, ihidden++) {
+                    if (ipass > 0 && !Msig[ihidden]) continue;
+                    for (int iot = 0; iot < nov_t; iot++) {
+                        const cplx* Fs = Fshift[it * nov_t + iot].data();
+                        double diff2 = P.highres_Xi2 / 2.;
+                        for (size_t n = 0; n < np; n++) {
+                            double dr = Frefctf[n].real() - Fs[n].real();
+                            double di = Frefctf[n].imag() - Fs[n].imag();
+                            diff2 += (dr * dr + di * di) * 0.5 * Minv[n];
                         }
+                        if (ipass == 0) {
+                            Mw_coarse[ihidden] = diff2;
+                        } else {
+                            auto itb = fine_index.find(ihidden);
+                            int bi;
+                            if (itb == fine_index.end()) {
+                                bi = (int)fine_blocks.size();
+                                fine_index[ihidden] = bi;
+                                FineBlock fb;
+                                for (int q = 0; q < 64; q++) fb.v[q] = -999.;
+                                fine_blocks.push_back(fb);
+                            } else bi = itb->second;
+                            fine_blocks[bi].v[ir * over_t + iot] = diff2;
+                            fine_dbg_d2.push_back(diff2);
+                            fine_dbg_h.push_back(ihidden * nover + ir * over_t + iot);
+                        }
+                        if (diff2 < mn) mn = diff2;
                     }
                 }
+            }
+        };
+        {
+            std::vector<cplx> Fref(np, 0.), Frefctf(np);
+            for (int c = 0; c < K; c++) {
+                if (!(sh.model->pdf_class[c] > 0.)) continue;
+                for (long idir = 0; idir < Nd; idir++)
+                    for (long ipsi = 0; ipsi < Npsi; ipsi++) score(c, idir, ipsi, Fref, Frefctf, mymin);
+            }
         }
         min_diff2 = mymin;
         if (ipass == 0 && dbg && ipart < dbg->n_debug && dbg->coarse_diff2)
@@ -1461,6 +1535,12 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
         for (long q = 0; q < nf && q < dbg->fine_capacity; q++) {
             if (dbg->fine_diff2) dbg->fine_diff2[(size_t)ipart * dbg->fine_capacity + q] = (float)fine_dbg_d2[q];
             if (dbg->fine_hidden) dbg->fine_hidden[(size_t)ipart * dbg->fine_capacity + q] = fine_dbg_h[q];
+            if (dbg->fine_weight) {   /* the fine pass's weight of the sample (priors included) */
+                const long h = fine_dbg_h[q];
+                const auto itb = fine_index.find(h / nover);
+                dbg->fine_weight[(size_t)ipart * dbg->fine_capacity + q] =
+                    itb == fine_index.end() ? 0. : fine_blocks[itb->second].v[h % nover];
+            }
         }
     }
 
@@ -1494,6 +1574,8 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
     std::vector<double> thr_sigma2(nsig2, 0.);
     std::vector<double> thr_pdf_dir((size_t)K * Nd, 0.), thr_pdf_class(K, 0.);
     double thr_sumw_group = 0, thr_sigma2_offset = 0, wsum_norm = 0;
+    double wsum_XA = 0, wsum_AA = 0;   /* scale correction (ml_optimiser.cpp:8477-8486) */
+    const double* dvp = sh.model->data_vs_prior_class;
     std::vector<double> thr_prior_off((size_t)2 * K, 0.);   /* 2D references (ml_optimiser.cpp:8504-8508) */
     double max_weight = -1.;
     long best_hidden = -1;
@@ -1522,7 +1604,8 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
                     orc_euler_angles2matrix(orot[ir], otilt[ir], opsi[ir], A);
                     if (sh.clean) std::fill(Fref.begin(), Fref.end(), cplx(0, 0));
                     project_into(&sh.proj[c], A, s, Fref.data(), sh.clean);
-                    for (size_t n = 0; n < np; n++) Frefctf[n] = Fref[n] * P.Fctf[n];
+                    /* Frefctf and the CTF of the backprojection both carry the group's scale */
+                    for (size_t n = 0; n < np; n++) Frefctf[n] = Fref[n] * (P.Fctf[n] * scale_st);
                     std::fill(Facc.begin(), Facc.end(), cplx(0, 0));
                     std::fill(Fw.begin(), Fw.end(), 0.);
                     long ihidden = iorientclass * Nt;
@@ -1542,6 +1625,11 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
                                     double wd = weight * (dr * dr + di * di);
                                     thr_sigma2[ires] += wd;
                                     wsum_norm += wd;
+                                    if (cfg->do_scale_correction && dvp && dvp[(size_t)c * nsig2 + ires] > 3.) {
+                                        wsum_XA += weight * (Frefctf[n].real() * Fs[n].real() +
+                                                             Frefctf[n].imag() * Fs[n].imag());
+                                        wsum_AA += weight * std::norm(Frefctf[n]);
+                                    }
                                 }
                             }
                             thr_sumw_group += weight;
@@ -1557,7 +1645,7 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
                             thr_pdf_dir[(size_t)c * Nd + idir] += weight;
                             const cplx* Fn = Fsh_nm[q].data();
                             for (size_t n = 0; n < np; n++) {
-                                double myctf = P.Fctf[n];
+                                double myctf = P.Fctf[n] * scale_st;
                                 double wxs = weight * myctf * Minv[n];
                                 Facc[n] += Fn[n] * wxs;
                                 Fw[n] += wxs * myctf;
@@ -1605,9 +1693,16 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
     out.best_hidden = best_hidden;
     if (wsum) {
         std::lock_guard<std::mutex> lk(*sh.global_mutex);
+        const size_t og = (size_t)pin.optics_group;
         if (wsum->sigma2_noise)
-            for (int i = 0; i < nsig2; i++) wsum->sigma2_noise[i] += thr_sigma2[i];
-        if (wsum->sumw_group) wsum->sumw_group[0] += thr_sumw_group;
+            for (int i = 0; i < nsig2; i++) wsum->sigma2_noise[og * nsig2 + i] += thr_sigma2[i];
+        if (wsum->sumw_group) wsum->sumw_group[og] += thr_sumw_group;
+        if (cfg->do_scale_correction) {
+            /* XA / scale and AA / scale^2 with the group's unclamped scale (:8855-8862) */
+            if (wsum->wsum_signal_product) wsum->wsum_signal_product[pin.group_id] += wsum_XA / scale_raw;
+            if (wsum->wsum_reference_power)
+                wsum->wsum_reference_power[pin.group_id] += wsum_AA / (scale_raw * scale_raw);
+        }
         if (wsum->pdf_class)
             for (int c = 0; c < K; c++) wsum->pdf_class[c] += thr_pdf_class[c];
         if (wsum->pdf_direction)
@@ -1628,6 +1723,7 @@ extern "C" int orc_estep(const t2x_estep_config* cfg, const t2x_model* model,
                          t2x_estep_debug* dbg, int proj_mode, int nthreads)
 {
     if (check_config(cfg) != T2X_OK || !model || !proj || !bp || n < 0) return T2X_EINVAL;
+    if (cfg->do_scale_correction && !model->scale_correction) return T2X_EINVAL;
     for (int c = 0; c < cfg->nr_classes; c++)
         if (proj[c].ref_dim != cfg->ref_dim || bp[c].ref_dim != cfg->ref_dim) return T2X_EINVAL;
     Shared sh;
@@ -1645,17 +1741,20 @@ extern "C" int orc_estep(const t2x_estep_config* cfg, const t2x_model* model,
     sh.clean = (proj_mode == ORC_PROJ_CLEAN);
     int status = T2X_OK;
     const size_t NN = (size_t)cfg->ori_size * cfg->ori_size;
+    auto one = [&](int p) {
+        std::memset(&out[p], 0, sizeof(t2x_particle_out));
+        int st = estep_particle(sh, images + p * NN, pin[p], out[p], wsum, dbg, p);
+        if (st != T2X_OK) {
+#ifdef _OPENMP
+#pragma omp critical
+#endif
+            status = st;
+        }
+    };
 #ifdef _OPENMP
     int nt = nthreads > 0 ? nthreads : omp_get_max_threads();
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nt)
 #endif
-    for (int p = 0; p < n; p++) {
-        std::memset(&out[p], 0, sizeof(t2x_particle_out));
-        int st = estep_particle(sh, images + p * NN, pin[p], out[p], wsum, dbg, p);
-        if (st != T2X_OK) {
-#pragma omp critical
-            status = st;
-        }
-    }
+    for (int p = 0; p < n; p++) one(p);
     return status;
 }

@@ -93,10 +93,14 @@ int orc_estep(const t2x_estep_config* cfg, const t2x_model* model, const orc_vol
 
 /* Preprocessing only (getFourierTransformsAndCtfs, ml_optimiser.cpp:5689-6593) for one image:
  * Fimg / Fimg_nomask (complex, current_size x (current_size/2+1)), Fctf (same, real),
- * power spectrum [ori_size/2+1], highres_Xi2. */
+ * power spectrum [ori_size/2+1], highres_Xi2. sigma2_noise [ori_size/2+1] is the particle's
+ * optics group's spectrum, read for the noise background (do_zero_mask 0; may be NULL otherwise). */
 int orc_preprocess(const t2x_estep_config* cfg, const float* image, const t2x_particle_in* pin,
-                   double* Fimg, double* Fimg_nomask, double* Fctf, double* power,
-                   double* highres_Xi2, double* old_offset_xy);
+                   const double* sigma2_noise, double* Fimg, double* Fimg_nomask, double* Fctf,
+                   double* power, double* highres_Xi2, double* old_offset_xy);
+/* The noise image of the mask background (do_zero_mask 0): ori_size^2 doubles, row-major. */
+int orc_background_noise(const t2x_estep_config* cfg, const double* sigma2_noise, long part_id,
+                         double* noise);
 
 /* Weight conversion of a dense diff2 row (convertAllSquaredDifferencesToWeights core,
  * ml_optimiser.cpp:7760-8031) with per-sample prior factors: weights out, sum, threshold. */

@@ -69,7 +69,8 @@ def load():
                                 C.POINTER(T.ParticleIn), C.POINTER(T.ParticleOut),
                                 C.POINTER(T.Wsum), C.POINTER(T.EstepDebug), C.c_int, C.c_int]),
         "orc_preprocess": (C.c_int, [C.POINTER(T.EstepConfig), C.POINTER(C.c_float),
-                                     C.POINTER(T.ParticleIn), PD, PD, PD, PD, PD, PD]),
+                                     C.POINTER(T.ParticleIn), PD, PD, PD, PD, PD, PD, PD]),
+        "orc_background_noise": (C.c_int, [C.POINTER(T.EstepConfig), PD, C.c_long, PD]),
         "orc_reconstruct": (C.c_int, [C.POINTER(T.ReconParams), PD, PD, PD, PD, PD]),
         "orc_fsc": (C.c_int, [C.c_int, PD, PD, PD]),
         "orc_weights_from_diff2": (C.c_int, [C.c_long, PD, PD, C.c_double, C.c_double, C.c_long,

@@ -113,6 +113,7 @@ extern "C" int t2x_device_close(t2x_device* h)
         t->d_pix.release(); t->d_win.release(); t->d_minv.release(); t->d_ires.release(); t->d_inproj.release();
     }
     if (d->fft_plan) (void)hipfftDestroy(d->fft_plan);
+    if (d->noise_plan) (void)hipfftDestroy(d->noise_plan);
     if (d->prop_plan) (void)hipfftDestroy(d->prop_plan);
     d->net.reset();
     for (auto& e : d->ev) (void)hipEventDestroy(e);

@@ -69,6 +69,9 @@ __global__ void k_cand_scatter(int B, const int2* pairs_strided, const long long
                                const long long* orient_off, int2* pairs, int* orients,
                                long long* orient_pair_start);
 static const int kCandCap = 4096;   // CAND_CAP of t2x_select.hip
+// Dynamic LDS of the kernels that hold one operand row while they find its maximum; longer rows
+// take the path through HBM (FP32 row, then k_split_rows).
+static const size_t kRowLdsMax = 65536;
 
 static const double kPi = 3.14159265358979323846;
 
@@ -80,12 +83,14 @@ __host__ __device__ static inline int relion_round(double x) { return (x > 0) ? 
 // ------------------------------------------------------------------------------------------
 
 // Norm correction, integer re-centring with zero fill (selfTranslate DONT_WRAP on the rounded
-// old offset, ml_optimiser.cpp:6097-6098) and the soft circular mask with the average
-// background value (softMaskOutsideMap, mask.cpp:43-95). Writes the unmasked image to
-// out[b] and the masked image to out[B + b].
+// old offset, ml_optimiser.cpp:6097-6098) and the soft circular mask (softMaskOutsideMap,
+// mask.cpp:43-97) whose background is the average value (--zero_mask, noise == null) or the
+// particle's noise image noise[b]. Writes the unmasked image to out[b] and the masked image to
+// out[B + b].
 __global__ __launch_bounds__(256) void k_prep_real(const float* __restrict__ img, int N, int B,
                                                    const PartScalars* __restrict__ ps,
                                                    double radius, double cosine_width,
+                                                   const float* __restrict__ noise,
                                                    float* __restrict__ out)
 {
     const int b = blockIdx.x;
@@ -103,6 +108,7 @@ __global__ __launch_bounds__(256) void k_prep_real(const float* __restrict__ img
         float v = 0.f;
         if (si >= 0 && si < N && sj >= 0 && sj < N) v = src[(size_t)si * N + sj] * scale;
         um[q] = v;
+        if (noise) continue;
         double y = i - N / 2, x = j - N / 2;
         double r = sqrt(x * x + y * y);
         if (r < radius) continue;
@@ -118,19 +124,53 @@ __global__ __launch_bounds__(256) void k_prep_real(const float* __restrict__ img
     if ((threadIdx.x & 63) == 0) { r1[threadIdx.x >> 6] = sbg; r2[threadIdx.x >> 6] = sw; }
     __syncthreads();
     __shared__ double bg;
-    if (threadIdx.x == 0) bg = (r1[0] + r1[1] + r1[2] + r1[3]) / (r2[0] + r2[1] + r2[2] + r2[3]);
+    if (threadIdx.x == 0) bg = noise ? 0. : (r1[0] + r1[1] + r1[2] + r1[3]) / (r2[0] + r2[1] + r2[2] + r2[3]);
     __syncthreads();
+    const float* nz = noise ? noise + (size_t)b * NN : nullptr;
     for (size_t q = threadIdx.x; q < NN; q += blockDim.x) {
         int i = (int)(q / N), j = (int)(q % N);
         double y = i - N / 2, x = j - N / 2;
         double r = sqrt(x * x + y * y);
         float v = um[q];
+        const double add = nz ? (double)nz[q] : bg;
         if (r < radius) mk[q] = v;
-        else if (r > radius_p) mk[q] = (float)bg;
+        else if (r > radius_p) mk[q] = (float)add;
         else {
             double rc = 0.5 + 0.5 * cos(kPi * (radius_p - r) / cosine_width);
-            mk[q] = (float)((1 - rc) * v + rc * bg);
+            mk[q] = (float)((1 - rc) * v + rc * add);
         }
+    }
+}
+
+// Fourier components of each particle's mask-background noise (ml_optimiser.cpp:6173-6212) in
+// the half layout N x (N/2+1): within shell N/2, complex Gaussians of standard deviation
+// sqrt(fudge * sigma2_noise[og][shell]) per part, keyed by (seed, part_id, 2 q + part) with q
+// the component's index; the x = 0 and x = N/2 columns Hermitian (row -y the conjugate of row
+// y, real at the self-conjugate frequencies) so that the real inverse transform is unambiguous.
+__global__ __launch_bounds__(256) void k_noise_spectrum(int N, const PartScalars* __restrict__ ps,
+                                                        const double* __restrict__ sigma2, double fudge,
+                                                        uint64_t seed, float2* __restrict__ F)
+{
+    const int b = blockIdx.x, xN = N / 2 + 1;
+    const PartScalars s = ps[b];
+    const double* s2 = sigma2 + (size_t)s.og * xN;
+    float2* out = F + (size_t)b * N * xN;
+    for (int q = threadIdx.x; q < N * xN; q += blockDim.x) {
+        const int i = q / xN, j = q - i * xN;
+        const int ip = (i < xN) ? i : i - N;
+        const int ires = relion_round(sqrt((double)(ip * ip + j * j)));
+        float2 v = make_float2(0.f, 0.f);
+        if (ires < xN) {
+            const bool edge = (j == 0 || j == N / 2);
+            const bool conj = edge && ip < 0;
+            const long src = (long)(conj ? N - i : i) * xN + j;
+            const double sd = sqrt(fudge * s2[ires]);
+            const double re = sd * noise_sample(seed, (long)s.part_id, 2 * src);
+            double im = sd * noise_sample(seed, (long)s.part_id, 2 * src + 1);
+            if (edge && (i == 0 || i == N / 2)) im = 0.;
+            v = make_float2((float)re, (float)(conj ? -im : im));
+        }
+        out[q] = v;
     }
 }
 
@@ -164,7 +204,7 @@ __global__ __launch_bounds__(256) void k_prep_fourier(const float2* __restrict__
         Fimg[(size_t)b * cs * xc + q] = make_float2((float)(m.x * inv) * sgn, (float)(m.y * inv) * sgn);
         double v = 1.0;
         if (do_ctf) v = ctf_value(c, j / (N * angpix), ip / (N * angpix));
-        Fctf[(size_t)b * cs * xc + q] = (float)v;
+        Fctf[(size_t)b * cs * xc + q] = (float)(v * ps[b].scale);   // Frefctf *= myscale (ml_optimiser.cpp:7119-7125)
     }
     double hres = 0.;
     if (cs < N) {
@@ -195,10 +235,11 @@ __global__ __launch_bounds__(256) void k_const_term(const float2* __restrict__ F
                                                     PartScalars* __restrict__ ps, int which)
 {
     const int b = blockIdx.x;
+    const float* mv = minv + (size_t)ps[b].og * npix;   // the optics group's Minvsigma2
     double s = 0.;
     for (int p = threadIdx.x; p < npix; p += blockDim.x) {
         float2 x = Fimg[(size_t)b * Pw + win[p]];
-        s += 0.5 * (double)minv[p] * ((double)x.x * x.x + (double)x.y * x.y);
+        s += 0.5 * (double)mv[p] * ((double)x.x * x.x + (double)x.y * x.y);
     }
     for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
     __shared__ double red[4];
@@ -252,15 +293,17 @@ __global__ __launch_bounds__(256) void k_coarse_imgside(const float2* __restrict
                                                         const float* __restrict__ Fctf, int Pw,
                                                         const int* __restrict__ win,
                                                         const float* __restrict__ minv, int npix,
+                                                        const PartScalars* __restrict__ ps,
                                                         const float2* __restrict__ ab, int T,
                                                         float* __restrict__ A, long long lda,
                                                         float* __restrict__ A1, long long lda1)
 {
     const int b = blockIdx.y, t = blockIdx.x;
+    const float* mv = minv + (size_t)ps[b].og * npix;
     for (int p = threadIdx.x; p < npix; p += blockDim.x) {
         const size_t w = (size_t)b * Pw + win[p];
         float2 x = Fimg[w];
-        float ctf = Fctf[w], m = minv[p];
+        float ctf = Fctf[w], m = mv[p];
         float s = m * ctf;
         float2 sh = ab[(size_t)t * npix + p];
         float xr = x.x * sh.x - x.y * sh.y, xi = x.x * sh.y + x.y * sh.x;
@@ -337,6 +380,7 @@ __global__ __launch_bounds__(256) void k_coarse_imgside_split(const float2* __re
                                                               const float* __restrict__ Fctf, int Pw,
                                                               const int* __restrict__ win,
                                                               const float* __restrict__ minv, int npix,
+                                                              const PartScalars* __restrict__ ps,
                                                               const float2* __restrict__ ab, int T, int M, int Kc,
                                                               H* __restrict__ hi, H* __restrict__ lo,
                                                               float* __restrict__ inv_scale, int use_scale,
@@ -347,13 +391,14 @@ __global__ __launch_bounds__(256) void k_coarse_imgside_split(const float2* __re
     const int r = blockIdx.x;
     const bool valid = r < M;
     const int b = valid ? r / T : 0, t = valid ? r - b * T : 0;
+    const float* mv = minv + (valid ? (size_t)ps[b].og * npix : 0);
     float mx = 0.f;
     for (int p = threadIdx.x; p < Kc / 2; p += blockDim.x) {
         float vr = 0.f, vi = 0.f;
         if (valid && p < npix) {
             const size_t w = (size_t)b * Pw + win[p];
             const float2 x = Fimg[w];
-            const float ctf = Fctf[w], m = minv[p];
+            const float ctf = Fctf[w], m = mv[p];
             const float s = m * ctf;
             const float2 sh = ab[(size_t)t * npix + p];
             const float xr = x.x * sh.x - x.y * sh.y, xi = x.x * sh.y + x.y * sh.x;
@@ -423,88 +468,402 @@ __device__ __forceinline__ void fine_matrix(const FineGeom& g, int n, int ir, do
     *idir = d;
 }
 
-#define FINE_MAX_PIX 6144
+// Fine diff2 of every significant coarse pair as a batched complex GEMM on MFMA
+// (getAllSquaredDifferences ipass 1, ml_optimiser.cpp:6922-7507; the reference GPU path runs it as
+// a block-sparse split-TF32 GEMM, fine_splitImg_BCF_PMO.cuh:139-649):
+//   diff2[(item), (ft)] = C_fine + n1(item) - G(item, ft),  G = sum_p Re(Z_p conj(AB_ft,p)),
+// item = (particle, significant orientation, oversampled rotation), Z = F conj(minv ctf X) the
+// item's slice times the weighted image, AB_ft the phase row of fine translation ft, n1 = 1/2
+// sum minv ctf^2 |F|^2. One workgroup per tile of 16 items of one particle x up to 128 columns
+// (the particle's significant coarse translations x their oversampled translations, dense over
+// the tile): the pixels stream through LDS in chunks of 256, each chunk's Z rows projected by the
+// 256 threads (one pixel each, all 16 rows), then the 4 waves each take a quarter of the chunk's
+// K on MFMA (split-K inside the workgroup, summed in LDS at the end). MODE 0: FP32 operands on
+// v_mfma_f32_16x16x4f32; MODE 1 / 2: BF16 / FP16 hi+lo operands, Al Bh + Ah Bl + Ah Bh on
+// v_mfma_f32_16x16x32, each chunk's FP16 rows scaled by the power of two that maps their
+// maximum into [0.5, 1) and the chunk's products scaled back (exactly) before they are summed.
+// Only the significant pairs' diff2 are written; there is no limit on the current size.
+#define FT_ROWS 16
+#define FT_COLS 128
+#define FT_KC 256
+#define FT_MAX_T 256
 
-// Fine diff2 of every significant coarse pair: one workgroup per (particle, significant
-// orientation, oversampled rotation). The slice at current size is projected into LDS as
-// Z = F conj(minv ctf X); each wave then scores one oversampled translation at a time,
-// G_t = sum_p Re(Z conj(AB_t)), diff2 = C_b + 1/2 sum minv ctf^2 |F|^2 - G_t.
-__global__ __launch_bounds__(256) void k_fine_diff2(
-    const long long* __restrict__ item_off, int B, const long long* __restrict__ orient_off,
-    const int* __restrict__ orients, const long long* __restrict__ orient_pair_start,
-    const long long* __restrict__ pair_off, const int2* __restrict__ pairs, FineGeom g,
-    const float2* const* __restrict__ vols, const int* __restrict__ pads,
-    const int2* __restrict__ pix, const unsigned char* __restrict__ inproj,
-    const float* __restrict__ minv, const int* __restrict__ win, int npix,
-    const float2* __restrict__ Fimg, const float* __restrict__ Fctf, int Pw,
-    const float2* __restrict__ ab_fine, const PartScalars* __restrict__ ps,
-    float* __restrict__ fine_d2, unsigned int* __restrict__ fine_min)
+struct FineTileParams {
+    const long long* tile_off;          // [B+1] first tile of each particle
+    int B;
+    int ncb;                            // column blocks per particle: ceil(T * over_trans / FT_COLS)
+    const long long* orient_off;        // [B+1]
+    const int* orients;                 // significant (class, orientation) of each row
+    const long long* orient_pair_start; // first pair of each significant orientation
+    const long long* pair_off;          // [B+1]
+    const int2* pairs;                  // (class orientation, coarse translation), CPU order
+    const unsigned* tmask;              // [B][TW] significant coarse translations
+    int TW;
+    FineGeom g;
+    const float2* const* vols;
+    const int* pads;
+    const int2* pix;
+    const unsigned char* inproj;
+    const float* minv;                  // [og][npix] diff2 Minvsigma2 (0 at the origin)
+    const int* win;
+    int npix;
+    const float2* Fimg;
+    const float* Fctf;
+    int Pw;
+    const float* abf;                   // MODE 0: [T_f][ldk] phase rows (re, im), 0 beyond npix
+    const void* abh;                    // MODE 1/2: the same rows as hi / lo halves
+    const void* abl;
+    int ldk;
+    const PartScalars* ps;
+    float* fine_d2;
+    unsigned int* fine_min;
+};
+
+template <int MODE> struct FineHalf { typedef _Float16 type; };
+template <> struct FineHalf<1> { typedef __bf16 type; };
+
+template <class H>
+__device__ __forceinline__ floatx4 fine_mfma32(typename Vec8<H>::type a, typename Vec8<H>::type b, floatx4 c);
+template <>
+__device__ __forceinline__ floatx4 fine_mfma32<__bf16>(Vec8<__bf16>::type a, Vec8<__bf16>::type b, floatx4 c)
 {
-    __shared__ float2 Z[FINE_MAX_PIX];
-    __shared__ double red[4];
-    __shared__ float wmin[4];
-    const long long item = blockIdx.x;
-    const int b = find_particle(item_off, B, item);
-    const long long local = item - item_off[b];
-    const long long j = orient_off[b] + local / g.over_rot;
-    const int ir = (int)(local % g.over_rot);
-    const int n = orients[j];
-    const long long p0 = orient_pair_start[j];
-    const long long p1 = (j + 1 < orient_off[b + 1]) ? orient_pair_start[j + 1] : pair_off[b + 1];
-    double A[9], Ai[9];
-    int c, idir;
-    fine_matrix(g, n, ir, A, &c, &idir);
-    inv3(A, Ai);   // Ainv = A.inv() * pf (projector.cpp:530-531)
-    const float a0 = (float)(Ai[0] * g.pf), a1 = (float)(Ai[1] * g.pf);
-    const float a2 = (float)(Ai[3] * g.pf), a3 = (float)(Ai[4] * g.pf);
-    const float a4 = (float)(Ai[6] * g.pf), a5 = (float)(Ai[7] * g.pf);
-    const float2* vol = vols[c];
-    const int pad = pads[c];
-    double n1 = 0.;
-    for (int p = threadIdx.x; p < npix; p += blockDim.x) {
-        float2 f = make_float2(0.f, 0.f);
-        if (inproj[p]) {
-            float x = (float)pix[p].x, y = (float)pix[p].y;
-            float2 v;
-            if (ref_sample(vol, pad, g.ref2d, a0 * x + a1 * y, a2 * x + a3 * y, a4 * x + a5 * y, &v)) f = v;
-        }
-        const size_t w = (size_t)b * Pw + win[p];
-        float2 X = Fimg[w];
-        float ctf = Fctf[w], m = minv[p];
-        float s = m * ctf;
-        float yr = s * X.x, yi = s * X.y;       // Y = minv ctf X
-        Z[p] = make_float2(f.x * yr + f.y * yi, f.y * yr - f.x * yi);   // F conj(Y)
-        n1 += 0.5 * (double)m * (double)ctf * ctf * ((double)f.x * f.x + (double)f.y * f.y);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
+template <>
+__device__ __forceinline__ floatx4 fine_mfma32<_Float16>(Vec8<_Float16>::type a, Vec8<_Float16>::type b, floatx4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_f16(a, b, c, 0, 0, 0);
+}
+
+// LDS of one tile: the chunk's Z rows (FP32 [16][2 KC + 4], or hi and lo halves [16][2 KC + 8]
+// each), reused after the K loop for the 4 waves' partial products [4][16][128].
+#define FT_ZLD32 (2 * FT_KC + 4)
+#define FT_ZLD16 (2 * FT_KC + 8)
+#define FT_SMEM_BYTES (4 * FT_ROWS * FT_COLS * 4 > 2 * FT_ROWS * FT_ZLD16 * 2 ? 4 * FT_ROWS * FT_COLS * 4 \
+                                                                               : 2 * FT_ROWS * FT_ZLD16 * 2)
+
+template <int MODE>
+__global__ __launch_bounds__(256) void k_fine_mfma(FineTileParams P)
+{
+    typedef typename FineHalf<MODE>::type H;
+    typedef typename Vec8<H>::type v8;
+    __shared__ __attribute__((aligned(16))) unsigned char smem[FT_SMEM_BYTES];
+    __shared__ float s_geo[FT_ROWS][6];
+    __shared__ const float2* s_vol[FT_ROWS];
+    __shared__ int s_pad[FT_ROWS];
+    __shared__ unsigned s_cmax[2][FT_ROWS];   // FP16: |Z| maxima of each row in the current / next chunk
+    __shared__ double s_n1[FT_ROWS];
+    __shared__ int s_rank[FT_MAX_T];
+    __shared__ int s_slot_t[FT_MAX_T];
+    __shared__ int s_nt;
+    __shared__ float s_wmin[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const FineGeom& g = P.g;
+    const long long tile = blockIdx.x;
+    const int b = find_particle(P.tile_off, P.B, tile);
+    const long long local = tile - P.tile_off[b];
+    const int rt = (int)(local / P.ncb), cb = (int)(local - (long long)rt * P.ncb);
+
+    // the particle's significant coarse translations: rank of each, and the translation of each rank
+    const unsigned* tm = P.tmask + (size_t)b * P.TW;
+    if (tid < g.T) {
+        const int w = tid >> 5, bit = tid & 31;
+        int r = __popc(tm[w] & ((1u << bit) - 1u));
+        for (int q = 0; q < w; q++) r += __popc(tm[q]);
+        const bool set = (tm[w] >> bit) & 1u;
+        s_rank[tid] = set ? r : -1;
+        if (set) s_slot_t[r] = tid;
     }
-    for (int o = 32; o > 0; o >>= 1) n1 += __shfl_xor(n1, o);
-    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = n1;
-    __syncthreads();
-    const double base = ps[b].C_fine + red[0] + red[1] + red[2] + red[3];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    const int nt = (int)(p1 - p0) * g.over_trans;
-    float lmin = 3.0e38f;
-    for (int q = wave; q < nt; q += 4) {
-        const long long k = p0 + q / g.over_trans;
-        const int iot = q % g.over_trans;
-        const int ft = pairs[k].y * g.over_trans + iot;
-        const float2* abt = ab_fine + (size_t)ft * npix;
-        float acc = 0.f;
-        for (int p = lane; p < npix; p += 64) {
-            float2 z = Z[p], s = abt[p];
-            acc += z.x * s.x + z.y * s.y;
+    if (tid == 0) {
+        int nt = 0;
+        for (int q = 0; q < P.TW; q++) nt += __popc(tm[q]);
+        s_nt = nt;
+    }
+    const long long o0 = P.orient_off[b];
+    const int n_items = (int)(P.orient_off[b + 1] - o0) * g.over_rot;
+    const int r0 = rt * FT_ROWS;
+    const int nrow = min(FT_ROWS, n_items - r0);
+    if (tid < FT_ROWS) {
+        s_n1[tid] = 0.;
+        s_vol[tid] = nullptr;
+        if (tid < nrow) {
+            const int li = r0 + tid;
+            const long long j = o0 + li / g.over_rot;
+            const int ir = li % g.over_rot;
+            double A[9], Ai[9];
+            int c, idir;
+            fine_matrix(g, P.orients[j], ir, A, &c, &idir);
+            inv3(A, Ai);   // Ainv = A.inv() * pf (projector.cpp:530-531)
+            s_geo[tid][0] = (float)(Ai[0] * g.pf);
+            s_geo[tid][1] = (float)(Ai[1] * g.pf);
+            s_geo[tid][2] = (float)(Ai[3] * g.pf);
+            s_geo[tid][3] = (float)(Ai[4] * g.pf);
+            s_geo[tid][4] = (float)(Ai[6] * g.pf);
+            s_geo[tid][5] = (float)(Ai[7] * g.pf);
+            s_vol[tid] = P.vols[c];
+            s_pad[tid] = P.pads[c];
         }
-        for (int o = 32; o > 0; o >>= 1) acc += __shfl_xor(acc, o);
-        if (lane == 0) {
-            float d = (float)(base - (double)acc);
-            fine_d2[k * (g.over_rot * g.over_trans) + ir * g.over_trans + iot] = d;
+    }
+    __syncthreads();
+    const int ncols = s_nt * g.over_trans;
+    const int c0 = cb * FT_COLS;
+    if (c0 >= ncols || nrow <= 0) return;   // uniform over the workgroup
+    const int ncol_tile = min(FT_COLS, ncols - c0);
+    const int ntj = (ncol_tile + 15) >> 4;
+
+    // the phase row of each of this lane's columns (column tiles j: column c0 + 16 j + lane % 16;
+    // columns past the tile read row 0 and are never stored)
+    int colrow[FT_COLS / 16];   // element offsets (T_f ldk < 2^31, checked by the host)
+#pragma unroll
+    for (int j = 0; j < FT_COLS / 16; j++) {
+        const int col = c0 + j * 16 + (lane & 15);
+        int ft = 0;
+        if (col < ncols) ft = s_slot_t[col / g.over_trans] * g.over_trans + col % g.over_trans;
+        colrow[j] = ft * P.ldk;
+    }
+    // two-level FP32 accumulation: acc holds one chunk's products (128 K values of this wave),
+    // tot the running sum of the chunks, so no accumulator adds more than ~128 + npix/256 terms
+    floatx4 acc[FT_COLS / 16], tot[FT_COLS / 16];
+#pragma unroll
+    for (int j = 0; j < FT_COLS / 16; j++) tot[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+    // per thread (one pixel per chunk) and row: FP32 partial sums of 1/2 minv ctf^2 |F|^2 over
+    // the thread's npix / 256 pixels, summed over the workgroup in FP64
+    float n1[FT_ROWS];
+#pragma unroll
+    for (int r = 0; r < FT_ROWS; r++) n1[r] = 0.f;
+    const float* mv = P.minv + (size_t)P.ps[b].og * P.npix;
+    float* Zs = reinterpret_cast<float*>(smem);
+    H* Zh = reinterpret_cast<H*>(smem);
+    H* Zl = Zh + FT_ROWS * FT_ZLD16;
+
+    if (tid < 2 * FT_ROWS) s_cmax[tid >> 4][tid & 15] = 0u;
+    __syncthreads();
+    int chunk = 0;
+    for (int k0 = 0; k0 < P.npix; k0 += FT_KC, chunk++) {
+        const int p = k0 + tid;
+        float yr = 0.f, yi = 0.f, half_mc2 = 0.f;
+        float x = 0.f, y = 0.f;
+        bool inp = false;
+        if (p < P.npix) {
+            const int2 xy = P.pix[p];
+            x = (float)xy.x;
+            y = (float)xy.y;
+            inp = P.inproj[p] != 0;
+            const size_t w = (size_t)b * P.Pw + P.win[p];
+            const float2 X = P.Fimg[w];
+            const float ctf = P.Fctf[w], m = mv[p];
+            const float sy = m * ctf;
+            yr = sy * X.x;
+            yi = sy * X.y;
+            half_mc2 = 0.5f * m * ctf * ctf;
+        }
+        float zrr[MODE == 2 ? FT_ROWS : 1], zir[MODE == 2 ? FT_ROWS : 1];
+        unsigned* cmax = s_cmax[chunk & 1];
+#pragma unroll
+        for (int r = 0; r < FT_ROWS; r++) {
+            float2 f = make_float2(0.f, 0.f);
+            const float2* vol = s_vol[r];
+            if (vol && inp) {
+                float2 v;
+                const float* a = s_geo[r];
+                if (ref_sample(vol, s_pad[r], g.ref2d, a[0] * x + a[1] * y, a[2] * x + a[3] * y, a[4] * x + a[5] * y, &v))
+                    f = v;
+            }
+            const float zr = f.x * yr + f.y * yi, zi = f.y * yr - f.x * yi;   // F conj(Y)
+            n1[r] += half_mc2 * (f.x * f.x + f.y * f.y);
+            if (MODE == 0) {
+                *reinterpret_cast<float2*>(&Zs[r * FT_ZLD32 + 2 * tid]) = make_float2(zr, zi);
+            } else if (MODE == 1) {   // BF16 keeps FP32's exponent range: no scale
+                const H hr = (H)zr, hi_ = (H)zi;
+                Zh[r * FT_ZLD16 + 2 * tid] = hr;
+                Zh[r * FT_ZLD16 + 2 * tid + 1] = hi_;
+                Zl[r * FT_ZLD16 + 2 * tid] = (H)(zr - (float)hr);
+                Zl[r * FT_ZLD16 + 2 * tid + 1] = (H)(zi - (float)hi_);
+            } else {   // FP16: the chunk's row maximum first (wave maxima, then LDS)
+                zrr[r] = zr;
+                zir[r] = zi;
+                float mx = fmaxf(fabsf(zr), fabsf(zi));
+                for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+                if (lane == 0) atomicMax(&cmax[r], __float_as_uint(mx));
+            }
+        }
+        if (MODE == 2) {
+            __syncthreads();
+            if (tid < FT_ROWS) s_cmax[(chunk + 1) & 1][tid] = 0u;   // the next chunk's maxima
+            // each row of the chunk scaled by 2^-e so that its maximum lies in [0.5, 1)
+#pragma unroll
+            for (int r = 0; r < FT_ROWS; r++) {
+                int e = 0;
+                const float m = __uint_as_float(cmax[r]);
+                if (m > 0.f) (void)frexpf(m, &e);
+                const float sc = ldexpf(1.f, -e);
+                const float vr = zrr[r] * sc, vi = zir[r] * sc;
+                const H hr = (H)vr, hi_ = (H)vi;
+                Zh[r * FT_ZLD16 + 2 * tid] = hr;
+                Zh[r * FT_ZLD16 + 2 * tid + 1] = hi_;
+                Zl[r * FT_ZLD16 + 2 * tid] = (H)(vr - (float)hr);
+                Zl[r * FT_ZLD16 + 2 * tid + 1] = (H)(vi - (float)hi_);
+            }
+        }
+        __syncthreads();
+        // MFMA over this wave's quarter of the chunk's 2 KC real K values
+#pragma unroll
+        for (int j = 0; j < FT_COLS / 16; j++) acc[j] = floatx4{0.f, 0.f, 0.f, 0.f};
+        if (MODE == 0) {
+#pragma unroll 1
+            for (int s8 = 0; s8 < (2 * FT_KC) / 4 / 16; s8++) {
+                const int kk = wave * (2 * FT_KC / 4) + s8 * 16 + 4 * (lane >> 4);
+                const float4 a4 = *reinterpret_cast<const float4*>(&Zs[(lane & 15) * FT_ZLD32 + kk]);
+#pragma unroll
+                for (int j = 0; j < FT_COLS / 16; j++) {
+                    if (j < ntj) {
+                        const float4 b4 = *reinterpret_cast<const float4*>(P.abf + colrow[j] + 2 * k0 + kk);
+                        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.x, b4.x, acc[j], 0, 0, 0);
+                        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.y, b4.y, acc[j], 0, 0, 0);
+                        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.z, b4.z, acc[j], 0, 0, 0);
+                        acc[j] = __builtin_amdgcn_mfma_f32_16x16x4f32(a4.w, b4.w, acc[j], 0, 0, 0);
+                    }
+                }
+            }
+        } else {
+            const H* bh = reinterpret_cast<const H*>(P.abh);
+            const H* bl = reinterpret_cast<const H*>(P.abl);
+#pragma unroll 1
+            for (int s32 = 0; s32 < (2 * FT_KC) / 4 / 32; s32++) {
+                const int kk = wave * (2 * FT_KC / 4) + s32 * 32 + 8 * (lane >> 4);
+                const v8 ah = *reinterpret_cast<const v8*>(&Zh[(lane & 15) * FT_ZLD16 + kk]);
+                const v8 al = *reinterpret_cast<const v8*>(&Zl[(lane & 15) * FT_ZLD16 + kk]);
+#pragma unroll
+                for (int j = 0; j < FT_COLS / 16; j++) {
+                    if (j < ntj) {
+                        const v8 vbh = *reinterpret_cast<const v8*>(bh + colrow[j] + 2 * k0 + kk);
+                        const v8 vbl = *reinterpret_cast<const v8*>(bl + colrow[j] + 2 * k0 + kk);
+                        acc[j] = fine_mfma32<H>(al, vbh, acc[j]);
+                        acc[j] = fine_mfma32<H>(ah, vbl, acc[j]);
+                        acc[j] = fine_mfma32<H>(ah, vbh, acc[j]);
+                    }
+                }
+            }
+        }
+        if (MODE == 2) {   // the chunk's products back to the rows' scale (powers of two: exact)
+            float us[4];
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                int e = 0;
+                const float m = __uint_as_float(cmax[4 * (lane >> 4) + i]);
+                if (m > 0.f) (void)frexpf(m, &e);
+                us[i] = ldexpf(1.f, e);
+            }
+#pragma unroll
+            for (int j = 0; j < FT_COLS / 16; j++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) tot[j][i] += acc[j][i] * us[i];
+        } else {
+#pragma unroll
+            for (int j = 0; j < FT_COLS / 16; j++) tot[j] += acc[j];
+        }
+        __syncthreads();
+    }
+    // n1 per row: workgroup sums
+#pragma unroll
+    for (int r = 0; r < FT_ROWS; r++) {
+        double v = (double)n1[r];
+        for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+        if (lane == 0) atomicAdd(&s_n1[r], v);
+    }
+    // the 4 waves' partial products (lane holds rows 4 (lane / 16) + i of column 16 j + lane % 16)
+    float* red = reinterpret_cast<float*>(smem);
+#pragma unroll
+    for (int j = 0; j < FT_COLS / 16; j++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+            red[(wave * FT_ROWS + 4 * (lane >> 4) + i) * FT_COLS + j * 16 + (lane & 15)] = tot[j][i];
+    __syncthreads();
+    for (int e = tid; e < FT_ROWS * FT_COLS; e += 256)
+        red[e] = red[e] + red[FT_ROWS * FT_COLS + e] + red[2 * FT_ROWS * FT_COLS + e] + red[3 * FT_ROWS * FT_COLS + e];
+    __syncthreads();
+    // diff2 of the significant pairs of this tile's rows and columns
+    float lmin = 3.0e38f;
+    const int nover = g.over_rot * g.over_trans;
+    const double Cf = P.ps[b].C_fine;
+    for (int r = 0; r < nrow; r++) {
+        const int li = r0 + r;
+        const long long j = o0 + li / g.over_rot;
+        const int ir = li % g.over_rot;
+        const long long p0 = P.orient_pair_start[j];
+        const long long p1 = (j + 1 < P.orient_off[b + 1]) ? P.orient_pair_start[j + 1] : P.pair_off[b + 1];
+        const double base = Cf + s_n1[r];
+        const int nq = (int)(p1 - p0) * g.over_trans;
+        for (int q = tid; q < nq; q += 256) {
+            const long long k = p0 + q / g.over_trans;
+            const int iot = q % g.over_trans;
+            const int col = s_rank[P.pairs[k].y] * g.over_trans + iot - c0;
+            if (col < 0 || col >= ncol_tile) continue;
+            const float d = (float)(base - (double)red[r * FT_COLS + col]);
+            P.fine_d2[k * nover + ir * g.over_trans + iot] = d;
             lmin = fminf(lmin, d);
         }
     }
-    if (lane == 0) wmin[wave] = lmin;
+    for (int o = 32; o > 0; o >>= 1) lmin = fminf(lmin, __shfl_xor(lmin, o));
+    if (lane == 0) s_wmin[wave] = lmin;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        float m = fminf(fminf(wmin[0], wmin[1]), fminf(wmin[2], wmin[3]));
-        if (m < 3.0e38f) atomicMin(&fine_min[b], float_to_ordered(m));
+    if (tid == 0) {
+        const float m = fminf(fminf(s_wmin[0], s_wmin[1]), fminf(s_wmin[2], s_wmin[3]));
+        if (m < 3.0e38f) atomicMin(&P.fine_min[b], float_to_ordered(m));
+    }
+}
+
+// Significant coarse translations of each particle (bit t of tmask[b]) from its pair list.
+__global__ void k_fine_tmask(const long long* __restrict__ pair_off, const int2* __restrict__ pairs, int B, int TW,
+                             unsigned* __restrict__ tmask)
+{
+    __shared__ unsigned m[FT_MAX_T / 32];
+    const int b = blockIdx.x;
+    for (int w = threadIdx.x; w < TW; w += blockDim.x) m[w] = 0u;
+    __syncthreads();
+    for (long long k = pair_off[b] + threadIdx.x; k < pair_off[b + 1]; k += blockDim.x) {
+        const int t = pairs[k].y;
+        atomicOr(&m[t >> 5], 1u << (t & 31));
+    }
+    __syncthreads();
+    for (int w = threadIdx.x; w < TW; w += blockDim.x) tmask[(size_t)b * TW + w] = m[w];
+}
+
+// Zero columns [n, ld) of each float2 row.
+__global__ void k_zero_row_tails(float2* __restrict__ F, int n, int ld)
+{
+    for (int p = n + threadIdx.x; p < ld; p += blockDim.x) F[(size_t)blockIdx.x * ld + p] = make_float2(0.f, 0.f);
+}
+
+// |F|^2 of float2 rows (the n1 term's operand when the split rows are made from FP32 rows).
+__global__ void k_abs2(const float2* __restrict__ F, long long n, float* __restrict__ F2)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x) {
+        const float2 f = F[i];
+        F2[i] = f.x * f.x + f.y * f.y;
+    }
+}
+
+// Phase rows of the fine translations with K padded to ldk (multiple of 2 FT_KC): FP32 rows
+// (MODE 0) or hi / lo halves (split modes; |values| <= 1, no scale).
+template <class H>
+__global__ void k_fine_ab_rows(const float2* __restrict__ ab, int npix, int ldk, float* __restrict__ out32,
+                               H* __restrict__ hi, H* __restrict__ lo)
+{
+    const int t = blockIdx.x;
+    for (int p = threadIdx.x; p < ldk / 2; p += blockDim.x) {
+        const float2 v = p < npix ? ab[(size_t)t * npix + p] : make_float2(0.f, 0.f);
+        const size_t o = (size_t)t * ldk + 2 * p;
+        if (out32) {
+            out32[o] = v.x;
+            out32[o + 1] = v.y;
+        } else {
+            const H hr = (H)v.x, hi_ = (H)v.y;
+            hi[o] = hr;
+            hi[o + 1] = hi_;
+            lo[o] = (H)(v.x - (float)hr);
+            lo[o + 1] = (H)(v.y - (float)hi_);
+        }
     }
 }
 
@@ -605,6 +964,7 @@ __global__ __launch_bounds__(256) void k_local_zrows(int B, long long nrows, con
                                                      const int* __restrict__ rowu, const float2* __restrict__ F,
                                                      int ldf, const float* __restrict__ minv,
                                                      const int* __restrict__ win, int npix,
+                                                     const PartScalars* __restrict__ ps,
                                                      const float2* __restrict__ Fimg,
                                                      const float* __restrict__ Fctf, int Pw, int Kc,
                                                      float* __restrict__ zf, double* __restrict__ n1out)
@@ -619,13 +979,14 @@ __global__ __launch_bounds__(256) void k_local_zrows(int B, long long nrows, con
     if (valid) {
         const int b = find_particle(orient_off, B, j);
         const float2* Fu = F + (long long)rowu[j] * ldf;
+        const float* mv = minv + (size_t)ps[b].og * npix;
         for (int p = threadIdx.x; p < Kc / 2; p += blockDim.x) {
             float zr = 0.f, zi = 0.f;
             if (p < npix) {
                 const float2 f = Fu[p];
                 const size_t w = (size_t)b * Pw + win[p];
                 const float2 X = Fimg[w];
-                const float ctf = Fctf[w], m = minv[p];
+                const float ctf = Fctf[w], m = mv[p];
                 const float s = m * ctf;
                 const float yr = s * X.x, yi = s * X.y;
                 zr = f.x * yr + f.y * yi;
@@ -735,7 +1096,7 @@ struct StoreParams {
     const double* cls_prior;   // [2K] px, 2D references with class-dependent offset priors
     const int2* pix;
     const unsigned char* inproj;
-    const float* minv_store;   // Minvsigma2 with the origin term (ml_optimiser.cpp:8099-8104)
+    const float* minv_store;   // [og][npix] Minvsigma2 with the origin term (ml_optimiser.cpp:8099-8104)
     const int* win;
     const int* ires;
     int npix;                  // store pixel set: every pixel with ires >= 0
@@ -752,14 +1113,17 @@ struct StoreParams {
     double pixel_size;
     int nshell;                // N/2+1
     int cs;
+    const double* dvp;         // [K][nshell] data_vs_prior_class (scale correction), or null
     // accumulators
-    double* w_sigma2;          // [nshell]
-    double* w_sumw;            // [1]
+    double* w_sigma2;          // [og][nshell]
+    double* w_sumw;            // [og]
     double* w_pdf_class;       // [K]
     double* w_pdf_dir;         // [K*Nd]
     double* w_sigma2_offset;   // [1]
     double* w_prior_offset;    // [2K] 2D references: sum w * pixel * (old offset + translation)
     double* w_norm;            // [B] per particle sum of wdiff2
+    double* w_xa;              // [B] per particle scale sums XA and AA (ml_optimiser.cpp:8477-8486)
+    double* w_aa;
     int Nd;
     LocalPrior lp;             // local searches (lp.dir null: global)
 };
@@ -789,6 +1153,7 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
     const long long p1 = (j + 1 < orient_off[b + 1]) ? orient_pair_start[j + 1] : pair_off[b + 1];
     const double sumw = P.fine_res[b].sum_w, sig = P.fine_res[b].sig_w, mind2 = P.fine_res[b].min_d2;
     const int nover = g.over_rot * g.over_trans;
+    const int og = P.ps[b].og;
     if (threadIdx.x == 0) s_cnt = 0;
     for (int i = threadIdx.x; i < P.nshell; i += blockDim.x) shell[i] = 0.;
     __syncthreads();
@@ -846,7 +1211,7 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
             pox += w * P.pixel_size * (s.old_x + P.ftx[ft]);   // ml_optimiser.cpp:8504-8508
             poy += w * P.pixel_size * (s.old_y + P.fty[ft]);
         }
-        atomicAdd(P.w_sumw, sw);
+        atomicAdd(&P.w_sumw[og], sw);
         atomicAdd(&P.w_pdf_class[c], sw);
         atomicAdd(&P.w_pdf_dir[(size_t)c * P.Nd + idir], sw);
         atomicAdd(P.w_sigma2_offset, soff);
@@ -873,7 +1238,10 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
     // ellipse row bounds of backproject2Dto3D (backprojector.cpp:80-135) in double
     double AtA_xx, AtA_xy, AtA_yy, AtA_xy2;
     bp_ata(da0, da1, da2, da3, da4, da5, &AtA_xx, &AtA_xy, &AtA_yy, &AtA_xy2);
-    double norm_acc = 0.;
+    double norm_acc = 0., xa_acc = 0., aa_acc = 0.;
+    const float ctf_f = (float)P.ps[b].ctf_store;
+    const float* minv_og = P.minv_store + (size_t)og * P.npix;
+    const double* dvp_c = P.dvp ? P.dvp + (size_t)c * P.nshell : nullptr;
     for (int p = threadIdx.x; p < P.npix; p += blockDim.x) {
         const int x = P.pix[p].x, y = P.pix[p].y;
         float2 f = make_float2(0.f, 0.f);
@@ -882,27 +1250,36 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
             if (ref_sample(vol, pad, g.ref2d, fa0 * x + fa1 * y, fa2 * x + fa3 * y, fa4 * x + fa5 * y, &v)) f = v;
         }
         const size_t wi = (size_t)b * P.Pw + P.win[p];
-        const float ctf = P.Fctf[wi];
+        const float ctf = P.Fctf[wi] * ctf_f;            // CTF x the group's (clamped) scale
         const float2 X = P.Fimg[wi], Xn = P.Fimg_nm[wi];
         const float fr = f.x * ctf, fi = f.y * ctf;     // Frefctf
-        const float ms = P.minv_store[p];
-        float d2acc = 0.f, accr = 0.f, acci = 0.f, wacc = 0.f;
+        const float ms = minv_og[p];
+        const int ir_ = P.ires[p];
+        const bool scale_sums = dvp_c && ir_ >= 0 && dvp_c[ir_] > 3.;
+        float d2acc = 0.f, accr = 0.f, acci = 0.f, wacc = 0.f, xacc = 0.f, wsum_q = 0.f;
         for (int q = 0; q < cnt; q++) {
             const float w = s_w[q];
             const float2 s = P.ab_fine[(size_t)s_ft[q] * P.npix + p];
             const float xr = X.x * s.x - X.y * s.y, xi = X.x * s.y + X.y * s.x;
             const float dr = fr - xr, di = fi - xi;
             d2acc += w * (dr * dr + di * di);
+            if (scale_sums) {
+                xacc += w * (fr * xr + fi * xi);
+                wsum_q += w;
+            }
             const float nr = Xn.x * s.x - Xn.y * s.y, ni = Xn.x * s.y + Xn.y * s.x;
             const float wx = w * ctf * ms;
             accr += nr * wx;
             acci += ni * wx;
             wacc += wx * ctf;
         }
-        const int ir_ = P.ires[p];
         if (ir_ >= 0) {
             atomicAdd(&shell[ir_], (double)d2acc);
             norm_acc += d2acc;
+        }
+        if (scale_sums) {
+            xa_acc += (double)xacc;
+            aa_acc += (double)wsum_q * ((double)fr * fr + (double)fi * fi);
         }
         if (!(wacc > 0.f)) continue;
         if (!bp_pixel_inside(AtA_xx, AtA_xy, AtA_yy, AtA_xy2, max_r2, da0, da1, da2, da3, da4, da5, x, y, g.ref2d))
@@ -967,8 +1344,20 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = norm_acc;
     __syncthreads();
     if (threadIdx.x == 0) atomicAdd(&P.w_norm[b], red[0] + red[1] + red[2] + red[3]);
+    if (dvp_c) {
+        __shared__ double red2[8];
+        __syncthreads();
+        for (int o = 32; o > 0; o >>= 1) { xa_acc += __shfl_xor(xa_acc, o); aa_acc += __shfl_xor(aa_acc, o); }
+        if ((threadIdx.x & 63) == 0) { red2[threadIdx.x >> 6] = xa_acc; red2[4 + (threadIdx.x >> 6)] = aa_acc; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            atomicAdd(&P.w_xa[b], red2[0] + red2[1] + red2[2] + red2[3]);
+            atomicAdd(&P.w_aa[b], red2[4] + red2[5] + red2[6] + red2[7]);
+        }
+    }
+    double* w_s2 = P.w_sigma2 + (size_t)og * P.nshell;
     for (int i = threadIdx.x; i < P.nshell; i += blockDim.x)
-        if (shell[i] != 0.) atomicAdd(&P.w_sigma2[i], shell[i]);
+        if (shell[i] != 0.) atomicAdd(&w_s2[i], shell[i]);
 }
 
 __global__ void k_fill_u32(unsigned int* p, unsigned int v, int n)
@@ -1233,28 +1622,28 @@ __global__ void k_items_from_orients(const long long* orient_off, int B, int ove
 
 // Compact pixel table of one pass: every window pixel with ires >= 0 (Mresol,
 // ml_optimiser.cpp:5596-5625), ordered by window index.
-static void build_table(int size, int cur, const double* sigma2, int nsig2, double fudge, bool store,
+// Compact pixel table of one pass with each optics group's Minvsigma2 (ml_optimiser.cpp:6703-6720):
+// h_minv holds n_og rows of npix values. The pixel set (every pixel of the Mresol map for the
+// store; those with ires > 0 for diff2) does not depend on the noise spectra.
+static void build_table(int size, int cur, const double* sigma2, int nsig2, int n_og, double fudge, bool store,
                         PassTables& t)
 {
     const int xs = size / 2 + 1, xc = cur / 2 + 1;
     t.size = size;
     t.h_x.clear(); t.h_y.clear(); t.h_win.clear(); t.h_ires.clear(); t.h_minv.clear(); t.h_inproj.clear();
     const int rmo = xs - 1;
+    std::vector<int> keep_ires;
     for (int i = 0; i < size; i++) {
         int ip = (i < xs) ? i : i - size;
         for (int j = 0; j < xs; j++) {
             int ires = relion_round(std::sqrt((double)(ip * ip + j * j)));
             if (!(ires < xs && !(j == 0 && ip < 0))) continue;   // Mresol == -1
-            double mv = 0.;
-            if (ires > 0 && ires < nsig2) mv = 1. / (fudge * sigma2[ires]);
-            if (store && ires == 0) mv = 1. / (fudge * sigma2[0]);
-            if (!store && mv == 0.) continue;   // no contribution to diff2
+            if (!store && !(ires > 0 && ires < nsig2)) continue;   // no contribution to diff2
             int ii = ip < 0 ? ip + cur : ip;
             t.h_x.push_back(j);
             t.h_y.push_back(ip);
             t.h_win.push_back(ii * xc + j);
             t.h_ires.push_back(ires);
-            t.h_minv.push_back((float)mv);
             // projected pixels: x <= FLOOR(sqrt(r_max_out^2 - y^2)), y = i for i <= r_max_out
             int y_proj = (i <= rmo) ? i : i - size;
             int xmax = (int)std::floor(std::sqrt((double)(rmo * rmo - y_proj * y_proj)));
@@ -1264,6 +1653,18 @@ static void build_table(int size, int cur, const double* sigma2, int nsig2, doub
         }
     }
     t.npix = (int)t.h_x.size();
+    t.n_og = n_og;
+    t.h_minv.resize((size_t)n_og * t.npix);
+    for (int g = 0; g < n_og; g++) {
+        const double* s2 = sigma2 + (size_t)g * nsig2;
+        for (int p = 0; p < t.npix; p++) {
+            const int ires = t.h_ires[p];
+            double mv = 0.;
+            if (ires > 0 && ires < nsig2) mv = 1. / (fudge * s2[ires]);
+            if (store && ires == 0) mv = 1. / (fudge * s2[0]);   // the origin term of the store
+            t.h_minv[(size_t)g * t.npix + p] = (float)mv;
+        }
+    }
     t.npix_pad = (int)round_up(t.npix, 32);   // K of the N1 GEMM: a multiple of the split GEMM's K step
 }
 
@@ -1271,14 +1672,14 @@ static int upload_table(PassTables& t, hipStream_t st)
 {
     T2X_TRY(t.d_pix.ensure(sizeof(int2) * t.npix));
     T2X_TRY(t.d_win.ensure(sizeof(int) * t.npix));
-    T2X_TRY(t.d_minv.ensure(sizeof(float) * t.npix));
+    T2X_TRY(t.d_minv.ensure(sizeof(float) * t.h_minv.size()));
     T2X_TRY(t.d_ires.ensure(sizeof(int) * t.npix));
     T2X_TRY(t.d_inproj.ensure(t.npix));
     std::vector<int2> xy(t.npix);
     for (int i = 0; i < t.npix; i++) xy[i] = make_int2(t.h_x[i], t.h_y[i]);
     T2X_HIP(hipMemcpyAsync(t.d_pix.ptr, xy.data(), sizeof(int2) * t.npix, hipMemcpyHostToDevice, st));
     T2X_HIP(hipMemcpyAsync(t.d_win.ptr, t.h_win.data(), sizeof(int) * t.npix, hipMemcpyHostToDevice, st));
-    T2X_HIP(hipMemcpyAsync(t.d_minv.ptr, t.h_minv.data(), sizeof(float) * t.npix, hipMemcpyHostToDevice, st));
+    T2X_HIP(hipMemcpyAsync(t.d_minv.ptr, t.h_minv.data(), sizeof(float) * t.h_minv.size(), hipMemcpyHostToDevice, st));
     T2X_HIP(hipMemcpyAsync(t.d_ires.ptr, t.h_ires.data(), sizeof(int) * t.npix, hipMemcpyHostToDevice, st));
     T2X_HIP(hipMemcpyAsync(t.d_inproj.ptr, t.h_inproj.data(), t.npix, hipMemcpyHostToDevice, st));
     T2X_HIP(hipStreamSynchronize(st));
@@ -1330,6 +1731,40 @@ static int ensure_fft(Device* d, int N, int batch)
     d->fft_n = N;
     d->fft_batch = batch;
     return T2X_OK;
+}
+
+// The image side of the split GEMMs: rows (b, t) < M of minv ctf X_t as hi / lo halves with their
+// row scales (rows M..rows-1 zero), and A1 = minv ctf^2 / 2 for the N1 term. Rows that fit the
+// LDS row buffer go through k_coarse_imgside_split; longer ones (coarse images beyond 8,192
+// pixels) through an FP32 matrix in HBM and k_split_rows, with the same bits.
+static int imgside_split(Device* d, int prec, const PassTables& tc, const PartScalars* d_ps, int Pw, int T, int B,
+                         long long rows, int Kc, int Kn, hipStream_t st)
+{
+    const long long M = (long long)B * T;
+    const size_t lds = sizeof(float) * Kc;
+    if (lds <= kRowLdsMax) {
+        if (prec == T2X_PREC_BF16X3)
+            k_coarse_imgside_split<__bf16><<<(unsigned)rows, 256, lds, st>>>(
+                d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, tc.d_win.as<int>(), tc.d_minv.as<float>(),
+                tc.npix, d_ps, d->ab_coarse.as<float2>(), T, (int)M, Kc, (__bf16*)d->Ah.ptr, (__bf16*)d->Al.ptr,
+                d->As.as<float>(), 0, d->A1.as<float>(), Kn);
+        else
+            k_coarse_imgside_split<_Float16><<<(unsigned)rows, 256, lds, st>>>(
+                d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, tc.d_win.as<int>(), tc.d_minv.as<float>(),
+                tc.npix, d_ps, d->ab_coarse.as<float2>(), T, (int)M, Kc, (_Float16*)d->Ah.ptr, (_Float16*)d->Al.ptr,
+                d->As.as<float>(), 1, d->A1.as<float>(), Kn);
+        T2X_HIP(hipGetLastError());
+        return T2X_OK;
+    }
+    T2X_TRY(d->A.ensure(sizeof(float) * rows * Kc));
+    T2X_HIP(hipMemsetAsync(d->A.ptr, 0, sizeof(float) * rows * Kc, st));
+    if (B > 0) {
+        k_coarse_imgside<<<dim3((unsigned)T, (unsigned)B), 256, 0, st>>>(
+            d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, tc.d_win.as<int>(), tc.d_minv.as<float>(), tc.npix,
+            d_ps, d->ab_coarse.as<float2>(), T, d->A.as<float>(), Kc, d->A1.as<float>(), Kn);
+        T2X_HIP(hipGetLastError());
+    }
+    return launch_split(prec, d->A.as<float>(), Kc, Kc, (int)rows, (int)M, d->Ah.ptr, d->Al.ptr, Kc, d->As.as<float>(), st);
 }
 
 struct EventTimer {
@@ -1411,6 +1846,26 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         d->prop_gate_mass.clear();
     }
     if (T * over_trans > STORE_MAX_T) { set_error("too many fine translations"); return T2X_EINVAL; }
+    // image groups (scale correction) and optics groups (noise spectra)
+    const int n_og = cfg->nr_optics_groups > 0 ? cfg->nr_optics_groups : 1;
+    const int n_grp = cfg->nr_groups > 0 ? cfg->nr_groups : 1;
+    if (cfg->nr_optics_groups < 0 || cfg->nr_groups < 0) { set_error("negative group count"); return T2X_EINVAL; }
+    if (cfg->do_scale_correction) {
+        if (!model->scale_correction) { set_error("scale correction without model->scale_correction"); return T2X_EINVAL; }
+        for (int g = 0; g < n_grp; g++) {
+            const double sc = model->scale_correction[g];
+            // relion_refine stops above 10000 (ml_optimiser.cpp:8329-8333); a scale <= 0 has no meaning
+            if (!(sc > 0.) || !(sc <= 10000.)) {
+                set_error("scale_correction of group " + std::to_string(g) + " outside (0, 10000]");
+                return T2X_EINVAL;
+            }
+        }
+    }
+    for (int i = 0; i < n; i++)
+        if (pin[i].optics_group < 0 || pin[i].optics_group >= n_og || pin[i].group_id < 0 || pin[i].group_id >= n_grp) {
+            set_error("particle " + std::to_string(i) + ": group_id or optics_group out of range");
+            return T2X_EINVAL;
+        }
     std::vector<const float2*> h_vols(K);
     std::vector<int> h_pads(K);
     std::vector<float2*> h_bpd(K);
@@ -1465,17 +1920,28 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     }
     // ---- per-configuration tables ----
     const double* sigma2 = model->sigma2_noise;
-    build_table(cc, cs, sigma2, xN, cfg->sigma2_fudge, false, d->coarse_tab);
-    build_table(cs, cs, sigma2, xN, cfg->sigma2_fudge, true, d->fine_tab);
+    build_table(cc, cs, sigma2, xN, n_og, cfg->sigma2_fudge, false, d->coarse_tab);
+    build_table(cs, cs, sigma2, xN, n_og, cfg->sigma2_fudge, true, d->fine_tab);
     PassTables& tc = d->coarse_tab;
     PassTables& tf = d->fine_tab;
-    if (tf.npix > FINE_MAX_PIX) { set_error("current_size too large for the fine-pass LDS tile"); return T2X_EINVAL; }
     T2X_TRY(upload_table(tc, st));
     T2X_TRY(upload_table(tf, st));
     // fine diff2 pixel subset: pixels with minv != 0 come first? (all store pixels are used;
     // the diff2 kernel reads minv of the diff2 definition)
-    std::vector<float> minv_diff2(tf.npix);
-    for (int p = 0; p < tf.npix; p++) minv_diff2[p] = (tf.h_ires[p] > 0) ? tf.h_minv[p] : 0.f;
+    std::vector<float> minv_diff2((size_t)n_og * tf.npix);
+    for (int g = 0; g < n_og; g++)
+        for (int p = 0; p < tf.npix; p++)
+            minv_diff2[(size_t)g * tf.npix + p] = (tf.h_ires[p] > 0) ? tf.h_minv[(size_t)g * tf.npix + p] : 0.f;
+    // noise spectra (mask background) and the scale sums' shell mask
+    {
+        std::vector<double> s2(sigma2, sigma2 + (size_t)n_og * xN);
+        T2X_TRY(upload(d->sigma2_tab, s2, st));
+        if (cfg->do_scale_correction && model->data_vs_prior_class) {
+            std::vector<double> dv(model->data_vs_prior_class, model->data_vs_prior_class + (size_t)K * xN);
+            T2X_TRY(upload(d->dvp_tab, dv, st));
+        }
+    }
+    const bool scale_sums = cfg->do_scale_correction && model->data_vs_prior_class;
     DevBuf& d_minv_fd = d->minv_fd;
     T2X_TRY(upload(d_minv_fd, minv_diff2, st));
     // phase tables
@@ -1483,15 +1949,36 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     std::vector<float2> abf = phase_table(tf, S.fine_tx_px, S.fine_ty_px, N);
     T2X_TRY(upload(d->ab_coarse, abc, st));
     T2X_TRY(upload(d->ab_fine, abf, st));
+    // the fine pass's GEMM operand: phase rows with K padded to whole pixel chunks
+    const int ldk_fine = (int)round_up(tf.npix, FT_KC) * 2;
+    {
+        const long long Tf = T * over_trans;
+        if (cfg->precision == T2X_PREC_FP32) {
+            T2X_TRY(d->ab_fk.ensure(sizeof(float) * Tf * ldk_fine));
+            k_fine_ab_rows<_Float16><<<(unsigned)Tf, 256, 0, st>>>(d->ab_fine.as<float2>(), tf.npix, ldk_fine,
+                                                                  d->ab_fk.as<float>(), nullptr, nullptr);
+        } else {
+            T2X_TRY(d->ab_fh.ensure(sizeof(uint16_t) * Tf * ldk_fine));
+            T2X_TRY(d->ab_fl.ensure(sizeof(uint16_t) * Tf * ldk_fine));
+            if (cfg->precision == T2X_PREC_BF16X3)
+                k_fine_ab_rows<__bf16><<<(unsigned)Tf, 256, 0, st>>>(d->ab_fine.as<float2>(), tf.npix, ldk_fine, nullptr,
+                                                                    (__bf16*)d->ab_fh.ptr, (__bf16*)d->ab_fl.ptr);
+            else
+                k_fine_ab_rows<_Float16><<<(unsigned)Tf, 256, 0, st>>>(d->ab_fine.as<float2>(), tf.npix, ldk_fine, nullptr,
+                                                                      (_Float16*)d->ab_fh.ptr, (_Float16*)d->ab_fl.ptr);
+        }
+        T2X_HIP(hipGetLastError());
+    }
+    if (T > FT_MAX_T) { set_error("more than 256 coarse translations"); return T2X_EINVAL; }
+    if ((long long)T * over_trans * ldk_fine >= 0x7fffffffLL) {
+        set_error("fine pass: phase table beyond 2^31 elements");
+        return T2X_EINVAL;
+    }
     // local searches in FP32: the phase table as the coarse GEMM's shared operand (Tl rows, Kc columns)
     const bool local_rows = cfg->orientational_prior_mode == T2X_PRIOR_ROTTILT_PSI && !getenv("T2X_LOCAL_DENSE");
     const int Tl = (int)round_up(T, 128);
     if (local_rows && cfg->precision == T2X_PREC_FP32) {   // the split modes take the image side instead
         const int Kcl = tc.npix_pad * 2;
-        if (sizeof(float) * Kcl > 65536) {
-            set_error("coarse image too large for the local-search operand kernel (2 P_c > 16384)");
-            return T2X_EINVAL;
-        }
         T2X_TRY(d->lc_ab.ensure(sizeof(float) * Tl * Kcl));
         k_ab_rows<<<Tl, 256, 0, st>>>(d->ab_coarse.as<float2>(), (int)T, tc.npix, Kcl, d->lc_ab.as<float>());
         T2X_HIP(hipGetLastError());
@@ -1630,19 +2117,20 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     }
 
     // ---- device accumulators ----
-    const size_t nws = xN + 1 + K + K * Nd + 1 + 2 * K;
+    const size_t nws = (size_t)n_og * xN + n_og + K + K * Nd + 1 + 2 * K;
     T2X_TRY(d->wsum_dev.ensure(sizeof(double) * nws));
     T2X_HIP(hipMemsetAsync(d->wsum_dev.ptr, 0, sizeof(double) * nws, st));
-    double* w_sigma2 = d->wsum_dev.as<double>();
-    double* w_sumw = w_sigma2 + xN;
-    double* w_pdfc = w_sumw + 1;
+    double* w_sigma2 = d->wsum_dev.as<double>();   // [og][xN]
+    double* w_sumw = w_sigma2 + (size_t)n_og * xN;   // [og]
+    double* w_pdfc = w_sumw + n_og;
     double* w_pdfd = w_pdfc + K;
     double* w_off = w_pdfd + K * Nd;
     double* w_poff = w_off + 1;   // [2K] per-class prior offsets, 2D references
 
     // ---- batch size: coarse diff2 dominates memory (B*T*O_pad floats) ----
+    const size_t noise_bytes = cfg->do_zero_mask ? 0 : (size_t)N * N * 4 + (size_t)N * xN * 8;
     size_t per_particle = (size_t)T * O_pad * 4 + (size_t)T * Kc * (prec == T2X_PREC_FP32 ? 4 : 8) +
-                          (size_t)4 * N * N * 4 + (size_t)2 * N * xN * 8 * 2;
+                          (size_t)4 * N * N * 4 + (size_t)2 * N * xN * 8 * 2 + noise_bytes;
     if (local_pairs) {
         // expected sample list: directions within 3 sigma (+ one step) of the prior, psi angles
         // within 3 sigma_psi (+ one step) on either side; 16 bytes per sample (pair, diff2)
@@ -1656,7 +2144,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         // element), G row (Tl floats) and n1
         const double rows_est = Kact * nd_est * np_est;
         per_particle = (size_t)(16. * rows_est * T + rows_est * (4. * Kc + 4. * Tl + 8.)) + (size_t)4 * N * N * 4 +
-                       (size_t)2 * N * xN * 8 * 2;
+                       (size_t)2 * N * xN * 8 * 2 + noise_bytes;
     }
     // scratch for the batch: 24 GiB; 64 GiB for local searches, whose per-particle estimate is
     // a bound (every in-range slice distinct) and whose batches share more slices the larger they
@@ -1675,18 +2163,19 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     if (env_b && atoi(env_b) > 0) Bmax = std::min(Bmax, atoi(env_b));
 
     const double pix = cfg->pixel_size;
-    // logsigma2 of dLL (ml_optimiser.cpp:8888-8896)
-    double logsigma2 = 0.;
-    for (int i = 0; i < cs; i++) {
-        int ip = (i < xc) ? i : i - cs;
-        for (int j = 0; j < xc; j++) {
-            int ires = relion_round(std::sqrt((double)(ip * ip + j * j)));
-            if (!(ires < xc && !(j == 0 && ip < 0))) continue;
-            if (ires > 0 && ires < xN) logsigma2 += std::log(2. * kPi * sigma2[ires]);
+    // logsigma2 of dLL per optics group (ml_optimiser.cpp:8888-8896)
+    std::vector<double> logsigma2(n_og, 0.);
+    for (int g = 0; g < n_og; g++)
+        for (int i = 0; i < cs; i++) {
+            int ip = (i < xc) ? i : i - cs;
+            for (int j = 0; j < xc; j++) {
+                int ires = relion_round(std::sqrt((double)(ip * ip + j * j)));
+                if (!(ires < xc && !(j == 0 && ip < 0))) continue;
+                if (ires > 0 && ires < xN) logsigma2[g] += std::log(2. * kPi * sigma2[(size_t)g * xN + ires]);
+            }
         }
-    }
-    std::vector<double> h_wnorm_all(n, 0.);
-    std::vector<double> h_power_sum(xN, 0.);
+    std::vector<double> h_power_sum((size_t)n_og * xN, 0.);
+    std::vector<double> h_xa(n_grp, 0.), h_aa(n_grp, 0.);
     double LL = 0., Pmax = 0., avg_norm = 0.;
     t2x_stats& stt = d->stats.s;
     stt.coarse_size = cc;
@@ -1722,6 +2211,11 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             s.prior_x = ref2d ? class_prior_x : px_;
             s.prior_y = ref2d ? class_prior_y : py_;
             s.norm_scale = cfg->do_norm_correction ? cfg->avg_norm_correction / p.norm_correction : 1.0;
+            s.og = p.optics_group;
+            s.grp = p.group_id;
+            s.part_id = p.part_id;
+            s.scale = cfg->do_scale_correction ? model->scale_correction[p.group_id] : 1.0;
+            s.ctf_store = cfg->do_scale_correction ? std::max(0.001, s.scale) / s.scale : 1.0;
             hctf[i] = ctf_const(p.ctf);
         }
         T2X_TRY(upload(d->pscal, hps, st));
@@ -1772,8 +2266,35 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         T2X_TRY(d->Fimg_nm_cur.ensure(sizeof(float2) * B * Pw));
         T2X_TRY(d->Fctf_cur.ensure(sizeof(float) * B * Pw));
         T2X_TRY(d->power.ensure(sizeof(double) * B * xN));
+        const float* d_noise = nullptr;
+        if (!cfg->do_zero_mask) {
+            // the mask's background: per particle a noise image with its optics group's spectrum
+            T2X_TRY(d->noise_spec.ensure(sizeof(float2) * B * N * xN));
+            T2X_TRY(d->noise_img.ensure(sizeof(float) * B * NN));
+            k_noise_spectrum<<<B, 256, 0, st>>>(N, d_ps, d->sigma2_tab.as<double>(), cfg->sigma2_fudge,
+                                                cfg->random_seed ^ 0x6d61736b6e6f6973ULL, d->noise_spec.as<float2>());
+            T2X_HIP(hipGetLastError());
+            if (!d->noise_plan || d->noise_n != N || d->noise_batch != B) {
+                if (d->noise_plan) { (void)hipfftDestroy(d->noise_plan); d->noise_plan = 0; }
+                int dims[2] = {N, N};
+                if (hipfftPlanMany(&d->noise_plan, 2, dims, nullptr, 1, N * xN, nullptr, 1, N * N, HIPFFT_C2R, B) !=
+                        HIPFFT_SUCCESS ||
+                    hipfftSetStream(d->noise_plan, st) != HIPFFT_SUCCESS) {
+                    set_error("hipfft C2R plan for the mask background failed");
+                    return T2X_EDEVICE;
+                }
+                d->noise_n = N;
+                d->noise_batch = B;
+            }
+            if (hipfftExecC2R(d->noise_plan, d->noise_spec.as<hipfftComplex>(), d->noise_img.as<hipfftReal>()) !=
+                HIPFFT_SUCCESS) {
+                set_error("hipfftExecC2R failed");
+                return T2X_EDEVICE;
+            }
+            d_noise = d->noise_img.as<float>();
+        }
         k_prep_real<<<B, 256, 0, st>>>(d_img, N, B, d_ps, cfg->particle_diameter / (2. * pix),
-                                      (double)cfg->width_mask_edge, d->img2.as<float>());
+                                      (double)cfg->width_mask_edge, d_noise, d->img2.as<float>());
         T2X_HIP(hipGetLastError());
         T2X_TRY(ensure_fft(d, N, 2 * B));
         if (hipfftExecR2C(d->fft_plan, d->img2.as<hipfftReal>(), d->fft_out.as<hipfftComplex>()) != HIPFFT_SUCCESS) {
@@ -1803,6 +2324,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         bool fallback = true;
         WeightSrc wc{};
         long long npairs_c = 0;
+        std::vector<unsigned> prop_mask_host;   // learned proposal: per particle, the bins scored
         if (local_pairs) {
             // local search on per-particle sample lists: each particle's orientations in range x all
             // translations, scored by a GEMM of the batch's (particle, orientation) rows against
@@ -1896,7 +2418,22 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                     T2X_TRY(d->lc_fl.ensure(sizeof(uint16_t) * nu * Kc));
                     T2X_TRY(d->lc_fs.ensure(sizeof(float) * nu));
                     const size_t lds = sizeof(float) * Kc;
-                    if (prec == T2X_PREC_BF16X3)
+                    if (lds > kRowLdsMax) {
+                        // rows beyond the LDS row buffer: FP32 slices in HBM, then the same split
+                        // (row maximum, power-of-two scale, hi / lo) and |F|^2 from them: the same bits
+                        T2X_TRY(d->lc_F.ensure(sizeof(float2) * nu * ldf));
+                        k_project_list<_Float16, 0><<<ugrid, 256, 0, st>>>(
+                            nu, d->lc_ulist.as<int>(), gc, d_vols.as<const float2*>(), d_pads.as<int>(),
+                            tc.d_pix.as<int2>(), tc.d_inproj.as<unsigned char>(), tc.npix, ldf, d->lc_F.as<float2>(),
+                            nullptr, nullptr, nullptr, 0, nullptr);
+                        // k_project_list leaves the padding columns npix..ldf-1 unwritten in FP32 rows
+                        k_zero_row_tails<<<(unsigned)nu, 64, 0, st>>>(d->lc_F.as<float2>(), tc.npix, ldf);
+                        T2X_HIP(hipGetLastError());
+                        T2X_TRY(launch_split(prec, d->lc_F.as<float>(), Kc, Kc, (int)nu, (int)nu, d->lc_fh.ptr,
+                                             d->lc_fl.ptr, Kc, d->lc_fs.as<float>(), st));
+                        k_abs2<<<(unsigned)std::min<long long>((nu * ldf + 255) / 256, 65536), 256, 0, st>>>(
+                            d->lc_F.as<float2>(), nu * ldf, d->lc_F2.as<float>());
+                    } else if (prec == T2X_PREC_BF16X3)
                         k_project_list<__bf16, 1><<<ugrid, 256, lds, st>>>(
                             nu, d->lc_ulist.as<int>(), gc, d_vols.as<const float2*>(), d_pads.as<int>(),
                             tc.d_pix.as<int2>(), tc.d_inproj.as<unsigned char>(), tc.npix, ldf, nullptr,
@@ -1922,7 +2459,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                     T2X_TRY(d->lc_z.ensure(sizeof(float) * rows_pad * Kc));
                     k_local_zrows<<<(unsigned)rows_pad, 256, 0, st>>>(
                         B, rows, d->lc_orient_off.as<long long>(), d->lc_rowu.as<int>(), d->lc_F.as<float2>(), ldf,
-                        tc.d_minv.as<float>(), tc.d_win.as<int>(), tc.npix,
+                        tc.d_minv.as<float>(), tc.d_win.as<int>(), tc.npix, d_ps,
                         d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, Kc, d->lc_z.as<float>(),
                         d->lc_n1.as<double>());
                     T2X_HIP(hipGetLastError());
@@ -1940,18 +2477,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                     T2X_TRY(d->Al.ensure(sizeof(uint16_t) * wrows * Kc));
                     T2X_TRY(d->As.ensure(sizeof(float) * wrows));
                     T2X_TRY(d->A1.ensure(sizeof(float) * B * Kn));
-                    const size_t lds = sizeof(float) * Kc;
-                    if (prec == T2X_PREC_BF16X3)
-                        k_coarse_imgside_split<__bf16><<<(unsigned)wrows, 256, lds, st>>>(
-                            d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, tc.d_win.as<int>(),
-                            tc.d_minv.as<float>(), tc.npix, d->ab_coarse.as<float2>(), (int)T, (int)M, Kc,
-                            (__bf16*)d->Ah.ptr, (__bf16*)d->Al.ptr, d->As.as<float>(), 0, d->A1.as<float>(), Kn);
-                    else
-                        k_coarse_imgside_split<_Float16><<<(unsigned)wrows, 256, lds, st>>>(
-                            d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, tc.d_win.as<int>(),
-                            tc.d_minv.as<float>(), tc.npix, d->ab_coarse.as<float2>(), (int)T, (int)M, Kc,
-                            (_Float16*)d->Ah.ptr, (_Float16*)d->Al.ptr, d->As.as<float>(), 1, d->A1.as<float>(), Kn);
-                    T2X_HIP(hipGetLastError());
+                    T2X_TRY(imgside_split(d, prec, tc, d_ps, Pw, (int)T, B, wrows, Kc, Kn, st));
                     k_local_n1<<<(unsigned)((rows + 3) / 4), 256, 0, st>>>(
                         B, rows, d->lc_orient_off.as<long long>(), d->lc_rowu.as<int>(), d->lc_F2.as<float>(), ldf,
                         d->A1.as<float>(), Kn, tc.npix, d->lc_n1.as<double>());
@@ -2009,7 +2535,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                 T2X_HIP(hipMemsetAsync(d->A.ptr, 0, sizeof(float) * M_pad * Kc, st));
                 dim3 grid((unsigned)T, (unsigned)B);
                 k_coarse_imgside<<<grid, 256, 0, st>>>(d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw,
-                                                       tc.d_win.as<int>(), tc.d_minv.as<float>(), tc.npix,
+                                                       tc.d_win.as<int>(), tc.d_minv.as<float>(), tc.npix, d_ps,
                                                        d->ab_coarse.as<float2>(), (int)T, d->A.as<float>(), Kc,
                                                        d->A1.as<float>(), Kn);
                 T2X_HIP(hipGetLastError());
@@ -2018,22 +2544,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                 T2X_TRY(d->Ah.ensure(sizeof(uint16_t) * M_pad * Kc));
                 T2X_TRY(d->Al.ensure(sizeof(uint16_t) * M_pad * Kc));
                 T2X_TRY(d->As.ensure(sizeof(float) * M_pad));
-                const size_t lds = sizeof(float) * Kc;
-                if (lds > 65536) {
-                    set_error("coarse image too large for the split operand kernel (2 P_c > 16384)");
-                    return T2X_EINVAL;
-                }
-                if (prec == T2X_PREC_BF16X3)
-                    k_coarse_imgside_split<__bf16><<<(unsigned)M_pad, 256, lds, st>>>(
-                        d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, tc.d_win.as<int>(),
-                        tc.d_minv.as<float>(), tc.npix, d->ab_coarse.as<float2>(), (int)T, (int)M, Kc,
-                        (__bf16*)d->Ah.ptr, (__bf16*)d->Al.ptr, d->As.as<float>(), 0, d->A1.as<float>(), Kn);
-                else
-                    k_coarse_imgside_split<_Float16><<<(unsigned)M_pad, 256, lds, st>>>(
-                        d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, tc.d_win.as<int>(),
-                        tc.d_minv.as<float>(), tc.npix, d->ab_coarse.as<float2>(), (int)T, (int)M, Kc,
-                        (_Float16*)d->Ah.ptr, (_Float16*)d->Al.ptr, d->As.as<float>(), 1, d->A1.as<float>(), Kn);
-                T2X_HIP(hipGetLastError());
+                T2X_TRY(imgside_split(d, prec, tc, d_ps, Pw, (int)T, B, M_pad, Kc, Kn, st));
             }
             {
                 EpiParams ep{};
@@ -2103,6 +2614,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                             work.push_back(BinWorkHost{pstart, cnt, rt, j * bin_cols + c0, std::min(128, bin_cols - c0)});
                 }
                 T2X_TRY(upload(d->prop_mask, mask, st));
+                prop_mask_host = mask;
                 T2X_TRY(upload(d->prop_plist, plist, st));
                 T2X_TRY(upload(d->prop_work, work, st));
                 T2X_TRY(upload(d->prop_rows, fb_rows, st));
@@ -2306,10 +2818,15 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             for (int i = 0; i < nd; i++) {
                 T2X_HIP(hipMemcpy(row.data(), d->diff2.as<float>() + (size_t)i * T * O_pad,
                                   sizeof(float) * T * O_pad, hipMemcpyDeviceToHost));
+                const int mw = (n_bins + 31) / 32;
                 for (long o = 0; o < O_all; o++)
                     for (long t = 0; t < T; t++) {
                         float v = row[(size_t)t * O_pad + o];
                         if (!(prior_o[o] > 0.) || std::isinf(v)) v = -999.f;   // not scored / out of range
+                        if (use_prop) {   // a bin outside the particle's scored set keeps stale values
+                            const long bin = o / bin_cols;
+                            if (!((prop_mask_host[(size_t)i * mw + bin / 32] >> (bin % 32)) & 1u)) v = -999.f;
+                        }
                         if (dbg->coarse_diff2) dbg->coarse_diff2[(size_t)(b0 + i) * Nh + o * T + t] = v;
                     }
                 if (dbg->coarse_min_diff2) dbg->coarse_min_diff2[b0 + i] = hselc[i].min_d2;
@@ -2348,14 +2865,55 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         g.ref2d = ref2d;
         (void)hipEventRecord(d->ev[ev++], st);   // 5: fine start
         if (nitems > 0) {
-            k_fine_diff2<<<(unsigned)nitems, 256, 0, st>>>(
-                d_item_off.as<long long>(), B, d_orient_off.as<long long>(), d->sig_orient.as<int>(),
-                d->orient_pairs_off.as<long long>(), d_pair_off.as<long long>(), d->sig_pairs.as<int2>(), g,
-                d_vols.as<const float2*>(), d_pads.as<int>(), tf.d_pix.as<int2>(), tf.d_inproj.as<unsigned char>(),
-                d_minv_fd.as<float>(), tf.d_win.as<int>(), tf.npix, d->Fimg_cur.as<float2>(),
-                d->Fctf_cur.as<float>(), Pw, d->ab_fine.as<float2>(), d_ps, d->fine_diff2.as<float>(),
-                d->fine_min.as<unsigned int>());
+            // tiles of 16 items x 128 columns per particle (items: its significant orientations x
+            // oversampled rotations; columns: its significant coarse translations x oversampled ones)
+            const int TW = (int)((T + 31) / 32);
+            const int ncb = (int)((T * over_trans + FT_COLS - 1) / FT_COLS);
+            std::vector<long long> tile_off(B + 1, 0);
+            for (int i = 0; i < B; i++)
+                tile_off[i + 1] = tile_off[i] + ((orient_off[i + 1] - orient_off[i]) * over_rot + FT_ROWS - 1) / FT_ROWS * ncb;
+            const long long ntiles = tile_off[B];
+            if (ntiles > 0x7fffffffLL) { set_error("fine pass: too many tiles in one batch"); return T2X_EINVAL; }
+            T2X_TRY(upload(d->fine_tile_off, tile_off, st));
+            T2X_TRY(d->fine_tmask.ensure(sizeof(unsigned) * B * TW));
+            k_fine_tmask<<<B, 256, 0, st>>>(d_pair_off.as<long long>(), d->sig_pairs.as<int2>(), B, TW,
+                                            d->fine_tmask.as<unsigned>());
             T2X_HIP(hipGetLastError());
+            FineTileParams fp{};
+            fp.tile_off = d->fine_tile_off.as<long long>();
+            fp.B = B;
+            fp.ncb = ncb;
+            fp.orient_off = d_orient_off.as<long long>();
+            fp.orients = d->sig_orient.as<int>();
+            fp.orient_pair_start = d->orient_pairs_off.as<long long>();
+            fp.pair_off = d_pair_off.as<long long>();
+            fp.pairs = d->sig_pairs.as<int2>();
+            fp.tmask = d->fine_tmask.as<unsigned>();
+            fp.TW = TW;
+            fp.g = g;
+            fp.vols = d_vols.as<const float2*>();
+            fp.pads = d_pads.as<int>();
+            fp.pix = tf.d_pix.as<int2>();
+            fp.inproj = tf.d_inproj.as<unsigned char>();
+            fp.minv = d_minv_fd.as<float>();
+            fp.win = tf.d_win.as<int>();
+            fp.npix = tf.npix;
+            fp.Fimg = d->Fimg_cur.as<float2>();
+            fp.Fctf = d->Fctf_cur.as<float>();
+            fp.Pw = Pw;
+            fp.abf = d->ab_fk.as<float>();
+            fp.abh = d->ab_fh.ptr;
+            fp.abl = d->ab_fl.ptr;
+            fp.ldk = ldk_fine;
+            fp.ps = d_ps;
+            fp.fine_d2 = d->fine_diff2.as<float>();
+            fp.fine_min = d->fine_min.as<unsigned int>();
+            if (ntiles > 0) {
+                if (prec == T2X_PREC_FP32) k_fine_mfma<0><<<(unsigned)ntiles, 256, 0, st>>>(fp);
+                else if (prec == T2X_PREC_BF16X3) k_fine_mfma<1><<<(unsigned)ntiles, 256, 0, st>>>(fp);
+                else k_fine_mfma<2><<<(unsigned)ntiles, 256, 0, st>>>(fp);
+                T2X_HIP(hipGetLastError());
+            }
         }
         (void)hipEventRecord(d->ev[ev++], st);   // 6: fine diff2 end
         WeightSrc wf = wc;
@@ -2405,8 +2963,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             }
         }
         // ---- weighted sums and backprojection ----
-        T2X_TRY(d->best.ensure(sizeof(double) * B));
-        T2X_HIP(hipMemsetAsync(d->best.ptr, 0, sizeof(double) * B, st));
+        T2X_TRY(d->best.ensure(sizeof(double) * 3 * B));   // per particle: wdiff2 sum, XA, AA
+        T2X_HIP(hipMemsetAsync(d->best.ptr, 0, sizeof(double) * 3 * B, st));
         StoreParams P{};
         P.g = g;
         P.vols = d_vols.as<const float2*>();
@@ -2445,6 +3003,9 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         P.w_sigma2_offset = w_off;
         P.w_prior_offset = w_poff;
         P.w_norm = d->best.as<double>();
+        P.w_xa = P.w_norm + B;
+        P.w_aa = P.w_norm + 2 * B;
+        P.dvp = scale_sums ? d->dvp_tab.as<double>() : nullptr;
         P.Nd = (int)Nd;
         P.lp = lp;
         if (xN > 520) { set_error("box too large for the shell accumulator"); return T2X_EINVAL; }
@@ -2458,12 +3019,12 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         // ---- per-particle results ----
         std::vector<SelResult> hself(B);
         std::vector<long long> hnsig(B);
-        std::vector<double> hnorm(B), hpower((size_t)B * xN);
+        std::vector<double> hnorm(3 * B), hpower((size_t)B * xN);
         std::vector<PartScalars> hps2(B);
         std::vector<int2> hpairs;
         T2X_HIP(hipMemcpyAsync(hself.data(), d_self, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
         T2X_HIP(hipMemcpyAsync(hnsig.data(), d->fine_sel.ptr, sizeof(long long) * B, hipMemcpyDeviceToHost, st));
-        T2X_HIP(hipMemcpyAsync(hnorm.data(), d->best.ptr, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+        T2X_HIP(hipMemcpyAsync(hnorm.data(), d->best.ptr, sizeof(double) * 3 * B, hipMemcpyDeviceToHost, st));
         T2X_HIP(hipMemcpyAsync(hpower.data(), d->power.ptr, sizeof(double) * B * xN, hipMemcpyDeviceToHost, st));
         T2X_HIP(hipMemcpyAsync(hps2.data(), d_ps, sizeof(PartScalars) * B, hipMemcpyDeviceToHost, st));
         T2X_HIP(hipStreamSynchronize(st));
@@ -2516,10 +3077,17 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             o.nr_significant_coarse = hselc[i].nsig;
             o.nr_fine_samples = (pair_off[i + 1] - pair_off[i]) * nover;
             o.nr_fine_significant = hnsig[i];
+            const int og = pin[gi].optics_group;
             double wnorm = hnorm[i];
             for (int ires = cs / 2 + 1; ires < xN; ires++) {
-                h_power_sum[ires] += hpower[(size_t)i * xN + ires];
+                h_power_sum[(size_t)og * xN + ires] += hpower[(size_t)i * xN + ires];
                 wnorm += hpower[(size_t)i * xN + ires];
+            }
+            if (cfg->do_scale_correction) {
+                // XA / scale and AA / scale^2 with the group's scale (ml_optimiser.cpp:8855-8862)
+                const double sc = hps[i].scale;
+                h_xa[pin[gi].group_id] += hnorm[B + i] / sc;
+                h_aa[pin[gi].group_id] += hnorm[2 * B + i] / (sc * sc);
             }
             double normcorr = pin[gi].norm_correction;
             if (cfg->do_norm_correction) {
@@ -2528,7 +3096,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                 avg_norm += normcorr;
             }
             o.norm_correction = normcorr;
-            o.dLL = std::log(r.sum_w) - r.min_d2 - logsigma2;
+            o.dLL = std::log(r.sum_w) - r.min_d2 - logsigma2[og];
             LL += o.dLL;
             Pmax += o.pmax;
         }
@@ -2540,16 +3108,25 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     std::vector<double> hw(nws);
     T2X_HIP(hipMemcpy(hw.data(), d->wsum_dev.ptr, sizeof(double) * nws, hipMemcpyDeviceToHost));
     if (wsum) {
+        const size_t o_sumw = (size_t)n_og * xN, o_pdfc = o_sumw + n_og, o_pdfd = o_pdfc + K,
+                     o_off = o_pdfd + (size_t)K * Nd, o_poff = o_off + 1;
         if (wsum->sigma2_noise)
-            for (int i = 0; i < xN; i++) wsum->sigma2_noise[i] += hw[i] + h_power_sum[i];
-        if (wsum->sumw_group) wsum->sumw_group[0] += hw[xN];
+            for (size_t i = 0; i < (size_t)n_og * xN; i++) wsum->sigma2_noise[i] += hw[i] + h_power_sum[i];
+        if (wsum->sumw_group)
+            for (int g = 0; g < n_og; g++) wsum->sumw_group[g] += hw[o_sumw + g];
         if (wsum->pdf_class)
-            for (int c = 0; c < K; c++) wsum->pdf_class[c] += hw[xN + 1 + c];
+            for (int c = 0; c < K; c++) wsum->pdf_class[c] += hw[o_pdfc + c];
         if (wsum->pdf_direction)
-            for (long i = 0; i < K * Nd; i++) wsum->pdf_direction[i] += hw[xN + 1 + K + i];
-        if (wsum->sigma2_offset) wsum->sigma2_offset[0] += hw[xN + 1 + K + K * Nd];
+            for (long i = 0; i < K * Nd; i++) wsum->pdf_direction[i] += hw[o_pdfd + i];
+        if (wsum->sigma2_offset) wsum->sigma2_offset[0] += hw[o_off];
         if (wsum->prior_offset_class && ref2d)
-            for (int c = 0; c < 2 * K; c++) wsum->prior_offset_class[c] += hw[xN + 2 + K + K * Nd + c];
+            for (int c = 0; c < 2 * K; c++) wsum->prior_offset_class[c] += hw[o_poff + c];
+        if (cfg->do_scale_correction) {
+            if (wsum->wsum_signal_product)
+                for (int g = 0; g < n_grp; g++) wsum->wsum_signal_product[g] += h_xa[g];
+            if (wsum->wsum_reference_power)
+                for (int g = 0; g < n_grp; g++) wsum->wsum_reference_power[g] += h_aa[g];
+        }
         if (wsum->avg_norm_correction && cfg->do_norm_correction) wsum->avg_norm_correction[0] += avg_norm;
         if (wsum->LL) wsum->LL[0] += LL;
         if (wsum->ave_Pmax) wsum->ave_Pmax[0] += Pmax;

@@ -233,9 +233,6 @@ namespace t2x {
 // scale per row (rowscale), applied back in the epilogue. Tiles 128x128, K step 32.
 // ------------------------------------------------------------------------------------------
 
-template <class H> struct Vec8;
-template <> struct Vec8<__bf16> { typedef __bf16 type __attribute__((ext_vector_type(8))); };
-template <> struct Vec8<_Float16> { typedef _Float16 type __attribute__((ext_vector_type(8))); };
 
 template <class H>
 __device__ __forceinline__ floatx4 mfma16x16x32(typename Vec8<H>::type a, typename Vec8<H>::type b, floatx4 c);
@@ -1244,12 +1241,15 @@ static int launch_x3_variant(int prec, int epi, const SplitOperand& A, const Spl
 // reuse the choice.
 // The best tile walk depends on the number of tile rows as well, so the key holds the row count
 // rounded to the nearest power of two (row-mapped launches cover a varying subset of a batch).
+// Whether the operands fit 32-bit byte offsets is part of the key: the row count of the key is
+// not the operand's (a row-mapped launch reads a whole batch), so a choice made for small operands
+// must never be reused for operands of 4 GiB or more.
 struct TuneKey {
-    int dev, prec, epi, N, K, mapped, mlog2;
+    int dev, prec, epi, N, K, mapped, mlog2, off32;
     bool operator<(const TuneKey& o) const
     {
-        return std::tie(dev, prec, epi, N, K, mapped, mlog2) <
-               std::tie(o.dev, o.prec, o.epi, o.N, o.K, o.mapped, o.mlog2);
+        return std::tie(dev, prec, epi, N, K, mapped, mlog2, off32) <
+               std::tie(o.dev, o.prec, o.epi, o.N, o.K, o.mapped, o.mlog2, o.off32);
     }
 };
 static int rows_log2(long long m) { return m > 0 ? (int)std::lround(std::log2((double)m)) : 0; }
@@ -1257,6 +1257,9 @@ static std::mutex g_tune_mtx;
 static std::map<TuneKey, GemmChoice> g_tune;
 
 static const GemmChoice kCandidates[] = {{258, 32}, {258, 16}, {258, 64}, {256, 32}, {257, 32}, {128, 64}};
+
+static int tile_rows(int tile) { return tile == 257 ? DBM : (tile == 258 ? 256 : tile); }
+static int tile_cols(int tile) { return tile == 257 ? DBN : (tile == 258 ? 256 : tile); }
 
 static bool eligible(const GemmChoice& c, bool mapped, bool off32)
 {
@@ -1289,14 +1292,19 @@ int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const floa
     }
     int dev = 0;
     T2X_HIP(hipGetDevice(&dev));
-    const TuneKey key{dev, prec, epi, Nrows_pad, K, mapped ? 1 : 0, rows_log2(ep.M)};
+    const TuneKey key{dev, prec, epi, Nrows_pad, K, mapped ? 1 : 0, rows_log2(ep.M), off32 ? 1 : 0};
     {
-        std::lock_guard<std::mutex> lk(g_tune_mtx);
-        auto it = g_tune.find(key);
-        if (it != g_tune.end()) {
-            if (used) *used = it->second;
-            return launch_x3_variant(prec, epi, A, B, K, Mrows_pad, Nrows_pad, ep, st, it->second.tile,
-                                     it->second.group, rowmap);
+        GemmChoice hit{0, 0};
+        {
+            std::lock_guard<std::mutex> lk(g_tune_mtx);
+            auto it = g_tune.find(key);
+            if (it != g_tune.end()) hit = it->second;
+        }
+        // a cached choice is re-checked against this launch (offsets, tile multiples) before use
+        if (hit.tile != 0 && eligible(hit, mapped, off32) && Mrows_pad % tile_rows(hit.tile) == 0 &&
+            Nrows_pad % tile_cols(hit.tile) == 0) {
+            if (used) *used = hit;
+            return launch_x3_variant(prec, epi, A, B, K, Mrows_pad, Nrows_pad, ep, st, hit.tile, hit.group, rowmap);
         }
     }
     GemmChoice best{0, 0};
@@ -1309,9 +1317,7 @@ int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const floa
     for (const GemmChoice& cand : kCandidates) {
         if (rc != T2X_OK) break;
         if (!eligible(cand, mapped, off32)) continue;
-        const int tile_m = cand.tile == 257 ? DBM : (cand.tile == 258 ? 256 : cand.tile);
-        const int tile_n = cand.tile == 257 ? DBN : (cand.tile == 258 ? 256 : cand.tile);
-        if (Mrows_pad % tile_m || Nrows_pad % tile_n) continue;
+        if (Mrows_pad % tile_rows(cand.tile) || Nrows_pad % tile_cols(cand.tile)) continue;
         for (int rep = 0; rep < 2 && rc == T2X_OK; rep++) {   // best of two runs per candidate
             if (hipEventRecord(e0, st) != hipSuccess) {
                 set_error("gemm_x3 autotune: event record failed");

@@ -107,9 +107,10 @@ struct PassTables {
     int size = 0;          // image size of the pass (coarse or current)
     int npix = 0;          // compact pixels with Minvsigma2 != 0
     int npix_pad = 0;      // padded to a multiple of 16
+    int n_og = 1;          // optics groups: rows of the Minvsigma2 table
     DevBuf d_pix;          // int2 (x, y) logical frequencies, y signed
     DevBuf d_win;          // int: index into the current-size window of each compact pixel
-    DevBuf d_minv;         // float Minvsigma2 of each compact pixel
+    DevBuf d_minv;         // float Minvsigma2 of each compact pixel, one row per optics group
     DevBuf d_ires;         // int shell index of each compact pixel
     DevBuf d_inproj;       // uchar: pixel inside the projected disc x^2+y^2 <= (size/2)^2
     std::vector<int> h_x, h_y, h_win, h_ires;
@@ -136,6 +137,13 @@ struct Device {
     // FFT plan for batched N x N R2C
     hipfftHandle fft_plan = 0;
     int fft_n = 0, fft_batch = 0;
+    // batched N x N C2R of the mask-background noise (do_zero_mask 0)
+    hipfftHandle noise_plan = 0;
+    int noise_n = 0, noise_batch = 0;
+    DevBuf noise_spec, noise_img, sigma2_tab, dvp_tab;
+    // fine pass: tiles per particle, significant-translation masks, the phase rows as the GEMM
+    // operand (FP32 or hi / lo halves)
+    DevBuf fine_tile_off, fine_tmask, ab_fk, ab_fh, ab_fl;
 
     // cached coarse projections
     std::string coarse_key;
@@ -173,6 +181,26 @@ struct Device {
 };
 
 // ---------------- device helpers ----------------
+
+__device__ __forceinline__ uint64_t mix64(uint64_t z)
+{
+    z += 0x9E3779B97F4A7C15ULL;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+    return z ^ (z >> 31);
+}
+
+// Counter-based Gaussian sample keyed by (seed, particle, index): Box-Muller on two 53-bit
+// uniforms from the splitmix64 finaliser.
+__device__ __forceinline__ double noise_sample(uint64_t seed, long particle, long index)
+{
+    uint64_t k = mix64(seed ^ mix64((uint64_t)particle * 0x100000001B3ULL + 0x51ED27ULL));
+    uint64_t a = mix64(k + 2 * (uint64_t)index);
+    uint64_t b = mix64(k + 2 * (uint64_t)index + 1);
+    double u1 = ((a >> 11) + 1) * (1.0 / 9007199254740992.0);
+    double u2 = (b >> 11) * (1.0 / 9007199254740992.0);
+    return sqrt(-2.0 * log(u1)) * cos(2.0 * 3.14159265358979323846 * u2);
+}
 
 // Order-preserving map of a float to uint32 for atomicMin.
 __device__ __forceinline__ unsigned int float_to_ordered(float f)

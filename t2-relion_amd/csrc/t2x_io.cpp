@@ -572,6 +572,12 @@ extern "C" int t2x_star_particles(const t2x_star* s, long first, long n, t2x_par
         p.prior_rot = opt("rlnAngleRotPrior", 999.0);
         p.prior_tilt = opt("rlnAngleTiltPrior", 999.0);
         p.prior_psi = opt("rlnAnglePsiPrior", 999.0);
+        // 1-based group numbers in the file, 0-based in the E-step
+        p.group_id = (int)std::lround(opt("rlnGroupNumber", 1.0)) - 1;
+        p.optics_group = (int)std::lround(opt("rlnOpticsGroup", 1.0)) - 1;
+        if (p.group_id < 0 || p.optics_group < 0)
+            return bad("STAR: group or optics group number below 1 in particle row " + std::to_string(r));
+        p.part_id = r;
         if (angpix) angpix[i] = px;
     }
     return T2X_OK;

@@ -5,6 +5,10 @@
 namespace t2x {
 
 typedef float floatx4 __attribute__((ext_vector_type(4)));
+// 8 x 16-bit operand fragment of v_mfma_f32_16x16x32_{bf16,f16}
+template <class H> struct Vec8;
+template <> struct Vec8<__bf16> { typedef __bf16 type __attribute__((ext_vector_type(8))); };
+template <> struct Vec8<_Float16> { typedef _Float16 type __attribute__((ext_vector_type(8))); };
 
 // Per-particle preprocessing scalars (device, double).
 struct PartScalars {
@@ -14,6 +18,12 @@ struct PartScalars {
     double norm_scale;    // avg_norm / normcorr (1 when norm correction is off)
     double C_coarse;      // highres/2 + 1/2 sum_Mc minv |X|^2
     double C_fine;        // highres/2 + 1/2 sum_Mf minv |X|^2
+    double scale;         // the image group's scale correction (1 without), folded into the CTF
+    double ctf_store;     // factor on that CTF in the weighted sums and the backprojection (the
+                          // store clamps the scale at 0.001, ml_optimiser.cpp:8333-8343)
+    long long part_id;    // key of the mask-background noise
+    int og;               // optics group: row of the Minvsigma2 tables and of the noise spectrum
+    int grp;              // image group
 };
 
 // Per-particle selection results (device).

@@ -94,24 +94,7 @@ __global__ void k_synth_fourier(const t2x_blob* bl, const int* boff, const Synth
     }
 }
 
-__device__ __forceinline__ uint64_t mix64(uint64_t z)
-{
-    z += 0x9E3779B97F4A7C15ULL;
-    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-    z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-    return z ^ (z >> 31);
-}
-
-// Counter-based Gaussian sample keyed by (seed, particle, pixel): Box-Muller on two 53-bit uniforms.
-__device__ __forceinline__ double noise_sample(uint64_t seed, long particle, long pixel)
-{
-    uint64_t k = mix64(seed ^ mix64((uint64_t)particle * 0x100000001B3ULL + 0x51ED27ULL));
-    uint64_t a = mix64(k + 2 * (uint64_t)pixel);
-    uint64_t b = mix64(k + 2 * (uint64_t)pixel + 1);
-    double u1 = ((a >> 11) + 1) * (1.0 / 9007199254740992.0);
-    double u2 = (b >> 11) * (1.0 / 9007199254740992.0);
-    return sqrt(-2.0 * log(u1)) * cos(2.0 * kPiS * u2);
-}
+// noise_sample (t2x_internal.h): the counter-based Gaussian of the particle noise.
 
 __global__ __launch_bounds__(256) void k_synth_noise(float* img, int N, const double* snr, uint64_t seed,
                                                      long p_offset, double* noise_var)

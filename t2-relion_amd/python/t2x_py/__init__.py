@@ -45,13 +45,14 @@ class EstepConfig(C.Structure):
         ("offset_range", C.c_double), ("offset_step", C.c_double),
         ("adaptive_oversampling", C.c_int), ("adaptive_fraction", C.c_double),
         ("maximum_significants", C.c_long), ("particle_diameter", C.c_double),
-        ("width_mask_edge", C.c_int), ("sigma2_fudge", C.c_double),
+        ("width_mask_edge", C.c_int), ("do_zero_mask", C.c_int), ("random_seed", C.c_uint64),
+        ("sigma2_fudge", C.c_double),
         ("do_norm_correction", C.c_int), ("avg_norm_correction", C.c_double),
+        ("do_scale_correction", C.c_int), ("nr_groups", C.c_int), ("nr_optics_groups", C.c_int),
         ("do_ctf_correction", C.c_int), ("sigma2_offset", C.c_double),
         ("orientational_prior_mode", C.c_int), ("precision", C.c_int),
         ("proposal_topk", C.c_int), ("proposal_gate", C.c_int),
         ("proposal_min_conf", C.c_double), ("gemm_tile", C.c_int), ("gemm_group", C.c_int),
-        ("reserved", C.c_int * 2),
         ("sigma_rot", C.c_double), ("sigma_tilt", C.c_double), ("sigma_psi", C.c_double),
     ]
 
@@ -59,7 +60,9 @@ class EstepConfig(C.Structure):
 class Model(C.Structure):
     _fields_ = [("sigma2_noise", C.POINTER(C.c_double)), ("pdf_class", C.POINTER(C.c_double)),
                 ("pdf_direction", C.POINTER(C.c_double)),
-                ("prior_offset_class", C.POINTER(C.c_double))]
+                ("prior_offset_class", C.POINTER(C.c_double)),
+                ("scale_correction", C.POINTER(C.c_double)),
+                ("data_vs_prior_class", C.POINTER(C.c_double))]
 
 
 class ParticleIn(C.Structure):
@@ -68,7 +71,8 @@ class ParticleIn(C.Structure):
                 ("prior_offset_x", C.c_double), ("prior_offset_y", C.c_double),
                 ("norm_correction", C.c_double),
                 ("rot", C.c_double), ("tilt", C.c_double), ("psi", C.c_double),
-                ("prior_rot", C.c_double), ("prior_tilt", C.c_double), ("prior_psi", C.c_double)]
+                ("prior_rot", C.c_double), ("prior_tilt", C.c_double), ("prior_psi", C.c_double),
+                ("group_id", C.c_int), ("optics_group", C.c_int), ("part_id", C.c_long)]
 
 
 class ParticleOut(C.Structure):
@@ -88,7 +92,8 @@ PD = C.POINTER(C.c_double)
 class Wsum(C.Structure):
     _fields_ = [(n, PD) for n in (
         "sigma2_noise", "sumw_group", "pdf_class", "pdf_direction", "prior_offset_class",
-        "sigma2_offset", "avg_norm_correction", "LL", "ave_Pmax")]
+        "sigma2_offset", "avg_norm_correction", "LL", "ave_Pmax",
+        "wsum_signal_product", "wsum_reference_power")]
 
 
 class EstepDebug(C.Structure):
@@ -99,7 +104,8 @@ class EstepDebug(C.Structure):
                 ("fine_hidden", C.POINTER(C.c_long)),
                 ("fine_diff2", C.POINTER(C.c_float)),
                 ("fine_count", C.POINTER(C.c_long)),
-                ("coarse_min_diff2", PD), ("coarse_sum_weight", PD), ("coarse_sig_weight", PD)]
+                ("coarse_min_diff2", PD), ("coarse_sum_weight", PD), ("coarse_sig_weight", PD),
+                ("fine_weight", PD)]
 
 
 class SamplingInfo(C.Structure):
@@ -176,13 +182,14 @@ def fptr(a):
     return a.ctypes.data_as(C.POINTER(C.c_float))
 
 
-def make_wsum(K, n_dir, ori_size):
+def make_wsum(K, n_dir, ori_size, n_optics_groups=1, n_groups=1):
     """numpy accumulators + a Wsum struct pointing at them."""
     arrs = {
-        "sigma2_noise": np.zeros(ori_size // 2 + 1), "sumw_group": np.zeros(1),
+        "sigma2_noise": np.zeros(n_optics_groups * (ori_size // 2 + 1)), "sumw_group": np.zeros(n_optics_groups),
         "pdf_class": np.zeros(K), "pdf_direction": np.zeros(K * n_dir),
         "prior_offset_class": np.zeros(2 * K), "sigma2_offset": np.zeros(1),
-        "avg_norm_correction": np.zeros(1), "LL": np.zeros(1), "ave_Pmax": np.zeros(1)}
+        "avg_norm_correction": np.zeros(1), "LL": np.zeros(1), "ave_Pmax": np.zeros(1),
+        "wsum_signal_product": np.zeros(n_groups), "wsum_reference_power": np.zeros(n_groups)}
     ws = Wsum(**{k: dptr(v) for k, v in arrs.items()})
     return ws, arrs
 

@@ -109,6 +109,7 @@ def make_config(w, *, current_size=None, coarse_size=None, precision=T2X_PREC_FP
     cfg.maximum_significants = maximum_significants
     cfg.particle_diameter = w["particle_diameter"]
     cfg.width_mask_edge = 5
+    cfg.do_zero_mask = 1                   # relion_refine --zero_mask (average background)
     cfg.sigma2_fudge = 1.0
     cfg.do_norm_correction = 0
     cfg.avg_norm_correction = 1.0

@@ -105,6 +105,7 @@ def base_config(cs=24, cc=16, ao=1, diameter=1.0e5):
     cfg.adaptive_fraction = 0.999
     cfg.particle_diameter = diameter
     cfg.width_mask_edge = 5
+    cfg.do_zero_mask = 1
     cfg.sigma2_fudge = 1.0
     cfg.avg_norm_correction = 1.0
     cfg.do_ctf_correction = 1
@@ -112,13 +113,14 @@ def base_config(cs=24, cc=16, ao=1, diameter=1.0e5):
     return cfg
 
 
-def preprocess(orc, cfg, img, pin):
+def preprocess(orc, cfg, img, pin, sigma2=None):
     cs = cfg.current_size
     nc = cs * (cs // 2 + 1)
     Fi, Fn, Fc = np.zeros(2 * nc), np.zeros(2 * nc), np.zeros(nc)
     power, hres, old = np.zeros(N // 2 + 1), C.c_double(), np.zeros(2)
     im = np.ascontiguousarray(img, dtype=np.float32)
-    assert orc.orc_preprocess(C.byref(cfg), T.fptr(im), C.byref(pin), T.dptr(Fi), T.dptr(Fn), T.dptr(Fc),
+    s2 = T.dptr(np.ascontiguousarray(sigma2, dtype=np.float64)) if sigma2 is not None else None
+    assert orc.orc_preprocess(C.byref(cfg), T.fptr(im), C.byref(pin), s2, T.dptr(Fi), T.dptr(Fn), T.dptr(Fc),
                               T.dptr(power), C.byref(hres), T.dptr(old)) == 0
     shape = (cs, cs // 2 + 1)
     return dict(Fimg=(Fi[0::2] + 1j * Fi[1::2]).reshape(shape), Fnm=(Fn[0::2] + 1j * Fn[1::2]).reshape(shape),
