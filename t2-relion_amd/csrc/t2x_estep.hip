@@ -2816,10 +2816,11 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             std::vector<float> row((size_t)T * O_pad);
             T2X_HIP(hipStreamSynchronize(st));
             for (int i = 0; i < nd; i++) {
-                T2X_HIP(hipMemcpy(row.data(), d->diff2.as<float>() + (size_t)i * T * O_pad,
-                                  sizeof(float) * T * O_pad, hipMemcpyDeviceToHost));
                 const int mw = (n_bins + 31) / 32;
-                for (long o = 0; o < O_all; o++)
+                if (dbg->coarse_diff2)
+                    T2X_HIP(hipMemcpy(row.data(), d->diff2.as<float>() + (size_t)i * T * O_pad,
+                                      sizeof(float) * T * O_pad, hipMemcpyDeviceToHost));
+                for (long o = 0; o < O_all && dbg->coarse_diff2; o++)
                     for (long t = 0; t < T; t++) {
                         float v = row[(size_t)t * O_pad + o];
                         if (!(prior_o[o] > 0.) || std::isinf(v)) v = -999.f;   // not scored / out of range
