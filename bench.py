@@ -1,10 +1,12 @@
 #!/usr/bin/env python3
 """E-step throughput on one MI355X on the 3D-classification workload (20,000 particles of 256^2,
-K=4 references of 256^3 at 1.1 A/px, HEALPix order 3, offsets 6/1.5 px), four paths timed in the
+K=4 references of 256^3 at 1.1 A/px, HEALPix order 3, offsets 6/1.5 px), five paths timed in the
 same run on the same resident particles:
 
   fp32     plain FP32 cross terms (the parity path)
   fp16x3   split-precision cross terms (FP16 hi+lo on MFMA, FP32 accumulate, FP64 normalisation)
+  fp16x2   the dense coarse cross term with 2 MFMA products (FP16 hi+lo images x FP16 hi
+           references); the fine pass and every other cross term as fp16x3
   learned  fp16x3 + learned pose proposal: the coarse pass scores only the top-16 bins
            (classes x HEALPix order-2 directions) of the proposal network, exhaustive search for
            particles whose top softmax probability is below 0.3. The network is trained on the
@@ -55,16 +57,20 @@ import t2x_py as T  # noqa: E402
 from t2x_py import workloads as W  # noqa: E402
 
 METRIC = "E-step particles/s, 1 MI355X: plain FP32 vs mixed+learned at equal FSC-0.143"
-MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2500.0, "fp16x3": 2500.0, "learned": 2500.0, "learned_gated": 2500.0}
-PRECISIONS = {"fp32": 0, "bf16x3": 1, "fp16x3": 2, "learned": 2, "learned_gated": 2}
+MFMA_PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2500.0, "fp16x3": 2500.0, "fp16x2": 2500.0, "learned": 2500.0,
+                    "learned_gated": 2500.0}
+PRECISIONS = {"fp32": 0, "bf16x3": 1, "fp16x3": 2, "fp16x2": 3, "learned": 2, "learned_gated": 2}
 DTYPES = {"fp32": "fp32",
           "bf16x3": "bf16x3 (bf16 hi+lo MFMA, fp32 accumulate, fp64 normalisation)",
           "fp16x3": "fp16x3 (fp16 hi+lo MFMA, fp32 accumulate, fp64 normalisation)",
+          "fp16x2": "fp16x2 (coarse: fp16 hi+lo image x fp16 references, 2 MFMA products; fine: fp16x3; "
+                    "fp32 accumulate, fp64 normalisation)",
           "learned": "fp16x3 cross term + bf16 proposal network (fp32 accumulate), fp64 normalisation",
           "learned_gated": "fp16x3 cross term + bf16 proposal network (fp32 accumulate), fp64 normalisation"}
 KERNELS = {"fp32": "k_gemm_tn<EPI_DIFF2> (FP32 MFMA 16x16x4)",
            "bf16x3": "k_gemm_x3p<bf16, EPI_DIFF2> (BF16 MFMA 16x16x32, 3 products, 256x256 pipelined)",
            "fp16x3": "k_gemm_x3p<f16, EPI_DIFF2> (FP16 MFMA 16x16x32, 3 products, 256x256 pipelined)",
+           "fp16x2": "k_gemm_x3p<f16, EPI_DIFF2, 2> (FP16 MFMA 16x16x32, 2 products, 256x256 pipelined)",
            "learned": "k_gemm_x3p<f16> on exhaustive-fallback rows + k_gemm_x3_bins<f16> on top-k bins "
                       "(FP16 MFMA 16x16x32, 3 products)",
            "learned_gated": "k_gemm_x3p<f16> on exhaustive-fallback rows + k_gemm_x3_bins<f16> on top-k bins "
@@ -370,7 +376,9 @@ def precision_vs_oracle(dev, wl, o, names):
     """Each cross-term precision on the oracle's sample (same float32 particles and references):
     fine-pass diff2 relative error, per-particle normalised fine weights (the oracle's weights
     with the diff2 difference applied: the priors are the same FP64 numbers in both paths),
-    max-weight pose agreement, dLL relative error."""
+    max-weight pose agreement, dLL relative error, and the fraction of particles whose
+    significant fine set (count and samples) equals the oracle's; the fine errors are taken
+    over those particles."""
     sys.path.insert(0, os.path.join(REPO, "tests"))
     import estep_harness as H
     res = {}
@@ -394,9 +402,10 @@ def precision_vs_oracle(dev, wl, o, names):
         res[name] = {"fine_diff2_max_rel": e_d2, "normalised_fine_weight_max_abs": e_w,
                      "pose_agreement": float((g["out"]["best_hidden"] == o["out"]["best_hidden"]).mean()),
                      "dLL_max_rel": H.rel(g["out"]["dLL"], o["out"]["dLL"]),
-                     "particles_compared": n_cmp, "particles": wl.n}
+                     "particles_compared": n_cmp, "particles": wl.n,
+                     "significant_set_agreement": n_cmp / max(1, wl.n)}
         log(f"[precision vs FP64 oracle] {name}: fine diff2 {e_d2:.2e}, weights {e_w:.2e}, "
-            f"poses {res[name]['pose_agreement']:.4f}")
+            f"poses {res[name]['pose_agreement']:.4f}, significant sets {n_cmp}/{wl.n}")
     wl.cfg.precision = 0
     return res
 
@@ -416,9 +425,9 @@ def main():
                     help="also run one 1 GiB device-to-device copy kernel whose FETCH_SIZE/WRITE_SIZE "
                          "counters calibrate the per-dispatch HBM traffic of the GEMM in --pmc runs")
     ap.add_argument("--precision", default="",
-                    help="comma list of paths to time (fp32, bf16x3, fp16x3, learned, learned_gated); the "
-                         "last one is the headline value, the others are reported beside it (default: "
-                         "fp32,fp16x3,learned,learned_gated for 3D references, fp32,fp16x3 for 2D)")
+                    help="comma list of paths to time (fp32, bf16x3, fp16x3, fp16x2, learned, learned_gated) "
+                         "(default: fp32,fp16x3,fp16x2,learned,learned_gated for 3D references, "
+                         "fp32,fp16x3,fp16x2 for 2D and local searches)")
     ap.add_argument("--headline", default="", help="path of the headline value (default: the fastest path, "
                     "per-iteration costs included, that meets the pose-agreement and FSC bars)")
     ap.add_argument("--resolution", type=float, default=0.0,
@@ -484,8 +493,8 @@ def main():
         T.check(lib.t2x_device_free(dev, dst))
         log("copy calibration: 1 GiB device-to-device copy (t2x::k_copy16) done")
     local = cfg.orientational_prior_mode == T.T2X_PRIOR_ROTTILT_PSI
-    paths = (args.precision or ("fp32,fp16x3" if local or cfg.ref_dim == 2 else
-                                "fp32,fp16x3,learned,learned_gated")).split(",")
+    paths = (args.precision or ("fp32,fp16x3,fp16x2" if local or cfg.ref_dim == 2 else
+                                "fp32,fp16x3,fp16x2,learned,learned_gated")).split(",")
     train = None
     use_net = any(p.startswith("learned") for p in paths)
     if use_net and args.load_proposal:
@@ -607,7 +616,7 @@ def main():
     # (the counting convention: one complex inner product per (translation, orientation) pair);
     # the split-precision forms issue 3 MFMA products for each, reported as their own figure
     launches = max(1, st.n_coarse_diff2_launches)
-    nprod = 1 if head == "fp32" else 3
+    nprod = {"fp32": 1, "fp16x2": 2}.get(head, 3)
     flop_total = st.coarse_diff2_flop
     if head.startswith("learned"):   # only the scored bins are computed
         n_bins = cfg.nr_classes * ds["info"].nr_dir // 4
@@ -678,7 +687,7 @@ def main():
     if not args.no_cpu:
         cpu, wl_cpu, o_cpu = cpu_baseline(args.workload, w, cfg, sz)
         if wl_cpu is not None:
-            names = [p for p in ("fp32", "bf16x3", "fp16x3") if p in runs or p == "bf16x3"]
+            names = [p for p in ("fp32", "bf16x3", "fp16x3", "fp16x2") if p in runs or p == "bf16x3"]
             prec_report = precision_vs_oracle(dev, wl_cpu, o_cpu, names)
     libref = None
     if args.gemm_library_reference:

@@ -46,7 +46,10 @@ enum {
 enum {
     T2X_PREC_FP32 = 0,        /* plain FP32 products, FP32 accumulation                */
     T2X_PREC_BF16X3 = 1,      /* BF16 hi+lo split, 3 MFMA products, FP32 accumulation  */
-    T2X_PREC_FP16X3 = 2       /* FP16 hi+lo split, 3 MFMA products, FP32 accumulation  */
+    T2X_PREC_FP16X3 = 2,      /* FP16 hi+lo split, 3 MFMA products, FP32 accumulation  */
+    T2X_PREC_FP16X2 = 3       /* the dense coarse cross term as FP16 hi+lo image rows x  */
+                              /* FP16 reference rows (2 MFMA products, FP32 accumulation); */
+                              /* every other cross term as FP16X3                          */
 };
 
 /* Learned-proposal gate (t2x_estep_config.proposal_gate): the value compared with

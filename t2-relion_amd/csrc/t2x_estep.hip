@@ -372,8 +372,9 @@ __global__ __launch_bounds__(128) void k_prior_t(const PartScalars* __restrict__
 }
 
 // k_coarse_imgside followed by the split of each row into hi + lo halves (k_split_rows), fused:
-// one workgroup per GEMM row (particle b, translation t; rows >= M are zero), the row's values
-// kept in LDS between the max-abs reduction and the split, so the FP32 A matrix is neither
+// one workgroup per GEMM row (particle b, translation t; rows >= M are zero). FP16 (use_scale):
+// the row's values kept in LDS between the max-abs reduction and the scaled split; BF16: no
+// scale, the halves written as the values are formed (no LDS). The FP32 A matrix is neither
 // cleared, written nor read back. Same arithmetic as the two kernels, so the same bits.
 template <class H>
 __global__ __launch_bounds__(256) void k_coarse_imgside_split(const float2* __restrict__ Fimg,
@@ -406,24 +407,30 @@ __global__ __launch_bounds__(256) void k_coarse_imgside_split(const float2* __re
             vi = s * xi;
             if (t == 0) A1[(size_t)b * lda1 + p] = 0.5f * m * ctf * ctf;
         }
+        if (!use_scale) {   // BF16: no row scale, so no row buffer: the halves straight out
+            const H hr = (H)vr, hi_ = (H)vi;
+            hi[(size_t)r * Kc + 2 * p] = hr;
+            hi[(size_t)r * Kc + 2 * p + 1] = hi_;
+            lo[(size_t)r * Kc + 2 * p] = (H)(vr - (float)hr);
+            lo[(size_t)r * Kc + 2 * p + 1] = (H)(vi - (float)hi_);
+            continue;
+        }
         row[2 * p] = vr;
         row[2 * p + 1] = vi;
         mx = fmaxf(mx, fmaxf(fabsf(vr), fabsf(vi)));
     }
-    float sc = 1.f;
-    if (use_scale) {
-        for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-        if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
-        __syncthreads();
-        mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
-        int e = 0;
-        if (mx > 0.f) (void)frexpf(mx, &e);
-        sc = ldexpf(1.f, -e);
-        if (threadIdx.x == 0) inv_scale[r] = ldexpf(1.f, e);
-    } else {
-        __syncthreads();
+    if (!use_scale) {
         if (threadIdx.x == 0) inv_scale[r] = 1.f;
+        return;
     }
+    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = mx;
+    __syncthreads();
+    mx = fmaxf(fmaxf(red[0], red[1]), fmaxf(red[2], red[3]));
+    int e = 0;
+    if (mx > 0.f) (void)frexpf(mx, &e);
+    const float sc = ldexpf(1.f, -e);
+    if (threadIdx.x == 0) inv_scale[r] = ldexpf(1.f, e);
     for (int k = threadIdx.x; k < Kc; k += blockDim.x) {
         const float v = row[k] * sc;
         const H h = (H)v;
@@ -474,7 +481,7 @@ __device__ __forceinline__ void fine_matrix(const FineGeom& g, int n, int ir, do
 //   diff2[(item), (ft)] = C_fine + n1(item) - G(item, ft),  G = sum_p Re(Z_p conj(AB_ft,p)),
 // item = (particle, significant orientation, oversampled rotation), Z = F conj(minv ctf X) the
 // item's slice times the weighted image, AB_ft the phase row of fine translation ft, n1 = 1/2
-// sum minv ctf^2 |F|^2. One workgroup per tile of 16 items of one particle x up to 128 columns
+// sum minv ctf^2 |F|^2. One workgroup per tile of 16 items of one particle x up to 64 columns
 // (the particle's significant coarse translations x their oversampled translations, dense over
 // the tile): the pixels stream through LDS in chunks of 256, each chunk's Z rows projected by the
 // 256 threads (one pixel each, all 16 rows), then the 4 waves each take a quarter of the chunk's
@@ -484,7 +491,7 @@ __device__ __forceinline__ void fine_matrix(const FineGeom& g, int n, int ir, do
 // maximum into [0.5, 1) and the chunk's products scaled back (exactly) before they are summed.
 // Only the significant pairs' diff2 are written; there is no limit on the current size.
 #define FT_ROWS 16
-#define FT_COLS 128
+#define FT_COLS 64
 #define FT_KC 256
 #define FT_MAX_T 256
 
@@ -579,6 +586,10 @@ __global__ __launch_bounds__(256) void k_fine_mfma(FineTileParams P)
         for (int q = 0; q < P.TW; q++) nt += __popc(tm[q]);
         s_nt = nt;
     }
+    __syncthreads();
+    const int ncols = s_nt * g.over_trans;
+    const int c0 = cb * FT_COLS;
+    if (c0 >= ncols) return;   // uniform over the workgroup: this column block is empty
     const long long o0 = P.orient_off[b];
     const int n_items = (int)(P.orient_off[b + 1] - o0) * g.over_rot;
     const int r0 = rt * FT_ROWS;
@@ -605,9 +616,7 @@ __global__ __launch_bounds__(256) void k_fine_mfma(FineTileParams P)
         }
     }
     __syncthreads();
-    const int ncols = s_nt * g.over_trans;
-    const int c0 = cb * FT_COLS;
-    if (c0 >= ncols || nrow <= 0) return;   // uniform over the workgroup
+    if (nrow <= 0) return;   // uniform over the workgroup
     const int ncol_tile = min(FT_COLS, ncols - c0);
     const int ntj = (ncol_tile + 15) >> 4;
 
@@ -682,9 +691,11 @@ __global__ __launch_bounds__(256) void k_fine_mfma(FineTileParams P)
             } else {   // FP16: the chunk's row maximum first (wave maxima, then LDS)
                 zrr[r] = zr;
                 zir[r] = zi;
-                float mx = fmaxf(fabsf(zr), fabsf(zi));
-                for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
-                if (lane == 0) atomicMax(&cmax[r], __float_as_uint(mx));
+                if (vol) {   // uniform: rows past the tile's items stay zero
+                    float mx = fmaxf(fabsf(zr), fabsf(zi));
+                    for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
+                    if (lane == 0) atomicMax(&cmax[r], __float_as_uint(mx));
+                }
             }
         }
         if (MODE == 2) {
@@ -907,7 +918,14 @@ __global__ __launch_bounds__(256) void k_project_list(long long nu, const int* _
             float2 v;
             if (ref_sample(vol, pad, g.ref2d, a0 * x + a1 * y, a2 * x + a3 * y, a4 * x + a5 * y, &v)) f = v;
         }
-        if (SPLIT) {
+        if (SPLIT == 2) {   // BF16: FP32's exponent range, no row scale: the halves straight out
+            const H hx = (H)f.x, hy = (H)f.y;
+            fh[u * 2 * ldf + 2 * p] = hx;
+            fh[u * 2 * ldf + 2 * p + 1] = hy;
+            fl[u * 2 * ldf + 2 * p] = (H)(f.x - (float)hx);
+            fl[u * 2 * ldf + 2 * p + 1] = (H)(f.y - (float)hy);
+            F2[u * ldf + p] = f.x * f.x + f.y * f.y;
+        } else if (SPLIT) {
             frow[2 * p] = f.x;
             frow[2 * p + 1] = f.y;
             mx = fmaxf(mx, fmaxf(fabsf(f.x), fabsf(f.y)));
@@ -916,7 +934,9 @@ __global__ __launch_bounds__(256) void k_project_list(long long nu, const int* _
             F[u * ldf + p] = f;
         }
     }
-    if (SPLIT) {
+    if (SPLIT == 2) {
+        if (threadIdx.x == 0) fs[u] = 1.f;
+    } else if (SPLIT) {
         for (int o = 32; o > 0; o >>= 1) mx = fmaxf(mx, __shfl_xor(mx, o));
         if ((threadIdx.x & 63) == 0) redf[threadIdx.x >> 6] = mx;
         __syncthreads();
@@ -1742,9 +1762,9 @@ static int imgside_split(Device* d, int prec, const PassTables& tc, const PartSc
 {
     const long long M = (long long)B * T;
     const size_t lds = sizeof(float) * Kc;
-    if (lds <= kRowLdsMax) {
-        if (prec == T2X_PREC_BF16X3)
-            k_coarse_imgside_split<__bf16><<<(unsigned)rows, 256, lds, st>>>(
+    if (lds <= kRowLdsMax || prec == T2X_PREC_BF16X3) {
+        if (prec == T2X_PREC_BF16X3)   // no row scale: the kernel needs no row buffer
+            k_coarse_imgside_split<__bf16><<<(unsigned)rows, 256, 0, st>>>(
                 d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, tc.d_win.as<int>(), tc.d_minv.as<float>(),
                 tc.npix, d_ps, d->ab_coarse.as<float2>(), T, (int)M, Kc, (__bf16*)d->Ah.ptr, (__bf16*)d->Al.ptr,
                 d->As.as<float>(), 0, d->A1.as<float>(), Kn);
@@ -2418,7 +2438,12 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                     T2X_TRY(d->lc_fl.ensure(sizeof(uint16_t) * nu * Kc));
                     T2X_TRY(d->lc_fs.ensure(sizeof(float) * nu));
                     const size_t lds = sizeof(float) * Kc;
-                    if (lds > kRowLdsMax) {
+                    if (prec == T2X_PREC_BF16X3) {
+                        k_project_list<__bf16, 2><<<ugrid, 256, 0, st>>>(
+                            nu, d->lc_ulist.as<int>(), gc, d_vols.as<const float2*>(), d_pads.as<int>(),
+                            tc.d_pix.as<int2>(), tc.d_inproj.as<unsigned char>(), tc.npix, ldf, nullptr,
+                            (__bf16*)d->lc_fh.ptr, (__bf16*)d->lc_fl.ptr, d->lc_fs.as<float>(), 0, d->lc_F2.as<float>());
+                    } else if (lds > kRowLdsMax) {
                         // rows beyond the LDS row buffer: FP32 slices in HBM, then the same split
                         // (row maximum, power-of-two scale, hi / lo) and |F|^2 from them: the same bits
                         T2X_TRY(d->lc_F.ensure(sizeof(float2) * nu * ldf));
@@ -2433,12 +2458,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                                              d->lc_fl.ptr, Kc, d->lc_fs.as<float>(), st));
                         k_abs2<<<(unsigned)std::min<long long>((nu * ldf + 255) / 256, 65536), 256, 0, st>>>(
                             d->lc_F.as<float2>(), nu * ldf, d->lc_F2.as<float>());
-                    } else if (prec == T2X_PREC_BF16X3)
-                        k_project_list<__bf16, 1><<<ugrid, 256, lds, st>>>(
-                            nu, d->lc_ulist.as<int>(), gc, d_vols.as<const float2*>(), d_pads.as<int>(),
-                            tc.d_pix.as<int2>(), tc.d_inproj.as<unsigned char>(), tc.npix, ldf, nullptr,
-                            (__bf16*)d->lc_fh.ptr, (__bf16*)d->lc_fl.ptr, d->lc_fs.as<float>(), 0, d->lc_F2.as<float>());
-                    else
+                    } else
                         k_project_list<_Float16, 1><<<ugrid, 256, lds, st>>>(
                             nu, d->lc_ulist.as<int>(), gc, d_vols.as<const float2*>(), d_pads.as<int>(),
                             tc.d_pix.as<int2>(), tc.d_inproj.as<unsigned char>(), tc.npix, ldf, nullptr,
@@ -2561,7 +2581,9 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                     T2X_TRY(d->A1s.ensure(sizeof(float) * B_pad));
                     T2X_TRY(launch_split(prec, d->A1.as<float>(), Kn, Kn, (int)B_pad, (int)B, d->A1h.ptr, d->A1l.ptr, Kn,
                                          d->A1s.as<float>(), st));
-                    T2X_TRY(launch_gemm_x3(prec, EPI_STORE, d->A1h.ptr, d->A1l.ptr, d->A1s.as<float>(), Kn, B_pad,
+                    // (3 products also under FP16X2: N1 is a sum of positive terms, large beside diff2)
+                    T2X_TRY(launch_gemm_x3(prec == T2X_PREC_FP16X2 ? T2X_PREC_FP16X3 : prec, EPI_STORE, d->A1h.ptr,
+                                           d->A1l.ptr, d->A1s.as<float>(), Kn, B_pad,
                                            d->coarse_B2h.ptr, d->coarse_B2l.ptr, d->coarse_B2s.as<float>(), Kn, Kn,
                                            (int)B_pad, (int)O_pad, ep, st,
                                            GemmChoice{cfg->gemm_tile, cfg->gemm_group}, nullptr, nullptr));
@@ -2866,7 +2888,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         g.ref2d = ref2d;
         (void)hipEventRecord(d->ev[ev++], st);   // 5: fine start
         if (nitems > 0) {
-            // tiles of 16 items x 128 columns per particle (items: its significant orientations x
+            // tiles of 16 items x 64 columns per particle (items: its significant orientations x
             // oversampled rotations; columns: its significant coarse translations x oversampled ones)
             const int TW = (int)((T + 31) / 32);
             const int ncb = (int)((T * over_trans + FT_COLS - 1) / FT_COLS);

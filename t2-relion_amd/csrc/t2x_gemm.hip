@@ -556,12 +556,15 @@ __device__ __forceinline__ void lds_dma16(const H* src, H* lds_dst)
 
 template <class H>
 __device__ __forceinline__ void x3_group(floatx4 (&acc)[4], typename Vec8<H>::type al, typename Vec8<H>::type ah,
-                                         const typename Vec8<H>::type (&bh)[4], const typename Vec8<H>::type (&bl)[4])
+                                         const typename Vec8<H>::type (&bh)[4], const typename Vec8<H>::type (&bl)[4],
+                                         bool with_bl = true)
 {
 #pragma unroll
     for (int j = 0; j < 4; j++) acc[j] = mfma16x16x32<H>(al, bh[j], acc[j]);
+    if (with_bl) {
 #pragma unroll
-    for (int j = 0; j < 4; j++) acc[j] = mfma16x16x32<H>(ah, bl[j], acc[j]);
+        for (int j = 0; j < 4; j++) acc[j] = mfma16x16x32<H>(ah, bl[j], acc[j]);
+    }
 #pragma unroll
     for (int j = 0; j < 4; j++) acc[j] = mfma16x16x32<H>(ah, bh[j], acc[j]);
 }
@@ -727,7 +730,9 @@ __device__ __forceinline__ void gemm_epilogue_staged(const floatx4 (&acc)[8][4],
     }
 }
 
-template <class H, int EPI>
+// NP = 3: Al Bh + Ah Bl + Ah Bh; NP = 2 (FP16X2): Al Bh + Ah Bh, the B operand's hi half alone
+// (its lo half neither loaded nor read).
+template <class H, int EPI, int NP = 3>
 __global__ __launch_bounds__(512) void k_gemm_x3p(SplitOperand A, SplitOperand B, int K, int tiles_m,
                                                   int tiles_n, int group_m, EpiParams ep,
                                                   const int* __restrict__ rowmap)
@@ -780,8 +785,10 @@ __global__ __launch_bounds__(512) void k_gemm_x3p(SplitOperand A, SplitOperand B
         lds_dma16<H>(reinterpret_cast<const H*>(Al + kb + oa1), &ST[1][rb1 * 32]);               \
         lds_dma16<H>(reinterpret_cast<const H*>(Bh + kb + ob0), &ST[2][rb0 * 32]);               \
         lds_dma16<H>(reinterpret_cast<const H*>(Bh + kb + ob1), &ST[2][rb1 * 32]);               \
-        lds_dma16<H>(reinterpret_cast<const H*>(Bl + kb + ob0), &ST[3][rb0 * 32]);               \
-        lds_dma16<H>(reinterpret_cast<const H*>(Bl + kb + ob1), &ST[3][rb1 * 32]);               \
+        if (NP == 3) {                                                                            \
+            lds_dma16<H>(reinterpret_cast<const H*>(Bl + kb + ob0), &ST[3][rb0 * 32]);           \
+            lds_dma16<H>(reinterpret_cast<const H*>(Bl + kb + ob1), &ST[3][rb1 * 32]);           \
+        }                                                                                         \
     } while (0)
 
     // fragment offsets: A block i of this wave at oa + i * 512, B block j at ob + j * 512 (the
@@ -792,7 +799,7 @@ __global__ __launch_bounds__(512) void k_gemm_x3p(SplitOperand A, SplitOperand B
     do {                                                                         \
         _Pragma("unroll") for (int j = 0; j < 4; j++) {                          \
             BH[j] = *reinterpret_cast<const v8*>(&ST[2][ob + j * 512]);          \
-            BL[j] = *reinterpret_cast<const v8*>(&ST[3][ob + j * 512]);          \
+            if (NP == 3) BL[j] = *reinterpret_cast<const v8*>(&ST[3][ob + j * 512]); \
         }                                                                        \
     } while (0)
 #define X3P_READ_A(ST, i, AH, AL)                                                \
@@ -816,7 +823,7 @@ __global__ __launch_bounds__(512) void k_gemm_x3p(SplitOperand A, SplitOperand B
                 X3P_READ_A(CUR, 6, a6h, a6l);                                    \
                 X3P_READ_A(CUR, 7, a7h, a7l);                                    \
             }                                                                    \
-            x3_group<H>(acc[i], al, ah, bh, bl);                                 \
+            x3_group<H>(acc[i], al, ah, bh, bl, NP == 3);                        \
             if (i < 5) { ah = nh; al = nl; }                                     \
         }                                                                        \
         __builtin_amdgcn_s_waitcnt(0);                                           \
@@ -828,14 +835,14 @@ __global__ __launch_bounds__(512) void k_gemm_x3p(SplitOperand A, SplitOperand B
         /* after column j, and fragment j of the next step takes its registers */ \
         _Pragma("unroll") for (int j = 0; j < 4; j++) {                          \
             acc[6][j] = mfma16x16x32<H>(a6l, bh[j], acc[6][j]);                  \
-            acc[6][j] = mfma16x16x32<H>(a6h, bl[j], acc[6][j]);                  \
+            if (NP == 3) acc[6][j] = mfma16x16x32<H>(a6h, bl[j], acc[6][j]);     \
             acc[6][j] = mfma16x16x32<H>(a6h, bh[j], acc[6][j]);                  \
             acc[7][j] = mfma16x16x32<H>(a7l, bh[j], acc[7][j]);                  \
-            acc[7][j] = mfma16x16x32<H>(a7h, bl[j], acc[7][j]);                  \
+            if (NP == 3) acc[7][j] = mfma16x16x32<H>(a7h, bl[j], acc[7][j]);     \
             acc[7][j] = mfma16x16x32<H>(a7h, bh[j], acc[7][j]);                  \
             if (nxt) {                                                           \
                 bh[j] = *reinterpret_cast<const v8*>(&NXT[2][ob + j * 512]);     \
-                bl[j] = *reinterpret_cast<const v8*>(&NXT[3][ob + j * 512]);     \
+                if (NP == 3) bl[j] = *reinterpret_cast<const v8*>(&NXT[3][ob + j * 512]); \
             }                                                                    \
         }                                                                        \
     } while (0)
@@ -1216,6 +1223,9 @@ static int launch_x3_variant(int prec, int epi, const SplitOperand& A, const Spl
     } else if (tile == 258) {
         if (bf) { if (store) k_gemm_x3p<__bf16, EPI_STORE><<<grid, 512, 0, st>>>(A, B, K, tm, tn, g, ep, rowmap);
                   else k_gemm_x3p<__bf16, EPI_DIFF2><<<grid, 512, 0, st>>>(A, B, K, tm, tn, g, ep, rowmap); }
+        else if (prec == T2X_PREC_FP16X2) {
+                  if (store) k_gemm_x3p<_Float16, EPI_STORE, 2><<<grid, 512, 0, st>>>(A, B, K, tm, tn, g, ep, rowmap);
+                  else k_gemm_x3p<_Float16, EPI_DIFF2, 2><<<grid, 512, 0, st>>>(A, B, K, tm, tn, g, ep, rowmap); }
         else    { if (store) k_gemm_x3p<_Float16, EPI_STORE><<<grid, 512, 0, st>>>(A, B, K, tm, tn, g, ep, rowmap);
                   else k_gemm_x3p<_Float16, EPI_DIFF2><<<grid, 512, 0, st>>>(A, B, K, tm, tn, g, ep, rowmap); }
     } else if (tile == 256) {
@@ -1261,8 +1271,9 @@ static const GemmChoice kCandidates[] = {{258, 32}, {258, 16}, {258, 64}, {256, 
 static int tile_rows(int tile) { return tile == 257 ? DBM : (tile == 258 ? 256 : tile); }
 static int tile_cols(int tile) { return tile == 257 ? DBN : (tile == 258 ? 256 : tile); }
 
-static bool eligible(const GemmChoice& c, bool mapped, bool off32)
+static bool eligible(const GemmChoice& c, bool mapped, bool off32, int prec = 0)
 {
+    if (prec == T2X_PREC_FP16X2 && c.tile != 258) return false;     // 2 products: the pipelined kernel only
     if (mapped && c.tile != 258 && c.tile != 256) return false;   // row maps: 256x256 kernels only
     if (c.tile == 258 && !off32) return false;                      // 32-bit operand offsets
     return true;
@@ -1276,6 +1287,10 @@ int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const floa
     const long long lim = 1LL << 32;   // the pipelined kernel addresses operands with 32-bit byte offsets
     const bool off32 = a_rows * lda * 2 < lim && (long long)Nrows_pad * ldb * 2 < lim;
     const bool mapped = rowmap != nullptr;
+    if (prec == T2X_PREC_FP16X2 && !off32) {
+        set_error("FP16X2: operands beyond 4 GiB (the 2-product kernel addresses them with 32-bit offsets)");
+        return T2X_EINVAL;
+    }
     SplitOperand A{Ah, Al, As, lda}, B{Bh, Bl, Bs, ldb};
     GemmChoice c = want;
     if (c.tile == 0) {   // environment override (tile sweeps), else the autotuner
@@ -1285,7 +1300,7 @@ int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const floa
     }
     if (c.tile != 0) {
         if (!valid_tile(c.tile)) { set_error("gemm_x3: unknown tile variant"); return T2X_EINVAL; }
-        if (!eligible(c, mapped, off32)) c.tile = 256;
+        if (!eligible(c, mapped, off32, prec)) c.tile = prec == T2X_PREC_FP16X2 ? 258 : 256;
         if (c.group <= 0) c.group = c.tile == 128 ? 64 : 32;
         if (used) *used = c;
         return launch_x3_variant(prec, epi, A, B, K, Mrows_pad, Nrows_pad, ep, st, c.tile, c.group, rowmap);
@@ -1301,7 +1316,7 @@ int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const floa
             if (it != g_tune.end()) hit = it->second;
         }
         // a cached choice is re-checked against this launch (offsets, tile multiples) before use
-        if (hit.tile != 0 && eligible(hit, mapped, off32) && Mrows_pad % tile_rows(hit.tile) == 0 &&
+        if (hit.tile != 0 && eligible(hit, mapped, off32, prec) && Mrows_pad % tile_rows(hit.tile) == 0 &&
             Nrows_pad % tile_cols(hit.tile) == 0) {
             if (used) *used = hit;
             return launch_x3_variant(prec, epi, A, B, K, Mrows_pad, Nrows_pad, ep, st, hit.tile, hit.group, rowmap);
@@ -1316,7 +1331,7 @@ int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const floa
                                32, rowmap);
     for (const GemmChoice& cand : kCandidates) {
         if (rc != T2X_OK) break;
-        if (!eligible(cand, mapped, off32)) continue;
+        if (!eligible(cand, mapped, off32, prec)) continue;
         if (Mrows_pad % tile_rows(cand.tile) || Nrows_pad % tile_cols(cand.tile)) continue;
         for (int rep = 0; rep < 2 && rc == T2X_OK; rep++) {   // best of two runs per candidate
             if (hipEventRecord(e0, st) != hipSuccess) {

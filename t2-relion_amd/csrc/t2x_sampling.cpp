@@ -121,7 +121,8 @@ int check_config(const t2x_estep_config* c)
         set_error("orientational_prior_mode must be T2X_NOPRIOR or T2X_PRIOR_ROTTILT_PSI");
         return T2X_EINVAL;
     }
-    if (c->precision != T2X_PREC_FP32 && c->precision != T2X_PREC_BF16X3 && c->precision != T2X_PREC_FP16X3) {
+    if (c->precision != T2X_PREC_FP32 && c->precision != T2X_PREC_BF16X3 && c->precision != T2X_PREC_FP16X3 &&
+        c->precision != T2X_PREC_FP16X2) {
         set_error("precision mode not supported");
         return T2X_EINVAL;
     }

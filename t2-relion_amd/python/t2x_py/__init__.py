@@ -24,6 +24,7 @@ T2X_ENOSIG = -6
 T2X_PREC_FP32 = 0
 T2X_PREC_BF16X3 = 1
 T2X_PREC_FP16X3 = 2
+T2X_PREC_FP16X2 = 3
 T2X_NOPRIOR = 0
 T2X_PRIOR_ROTTILT_PSI = 1
 T2X_GATE_TOP1 = 0

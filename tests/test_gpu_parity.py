@@ -186,7 +186,11 @@ def class256_case():
     return wl, wl.run_oracle(n_debug=4, fine_cap=1 << 16)
 
 
-@pytest.mark.parametrize("prec", [0, 2])
+# coarse diff2 bar per precision: FP16x2 rounds the reference rows to 11 significant bits
+CLASS256_D2_BAR = {0: 1e-5, 2: 1e-5, 3: 2e-4}
+
+
+@pytest.mark.parametrize("prec", [0, 2, 3])
 def test_class256_shapes(class256_case, gpu_device, prec):
     wl, o = class256_case
     wl.cfg.precision = prec
@@ -197,7 +201,7 @@ def test_class256_shapes(class256_case, gpu_device, prec):
     assert np.array_equal(valid, gd["coarse_diff2"] > -998)
     e = H.rel(gd["coarse_diff2"][valid], od["coarse_diff2"][valid])
     print(f"class256 shapes precision {prec}: coarse diff2 max rel {e:.2e}")
-    assert e < 1e-5
+    assert e < CLASS256_D2_BAR[prec]
     assert np.array_equal(o["out"]["best_hidden"], g["out"]["best_hidden"])
     assert H.rel(g["out"]["dLL"], o["out"]["dLL"]) < 1e-5
     for k, v in o["wsum"].items():
