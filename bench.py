@@ -447,9 +447,10 @@ def main():
     ap.add_argument("--class-offset-priors", action="store_true",
                     help="2D references: a different offset prior per class (uniform within one "
                          "offset step), as in every iteration after the first")
-    ap.add_argument("--gemm-library-reference", action="store_true",
+    ap.add_argument("--gemm-library-reference", action=argparse.BooleanOptionalAction, default=True,
                     help="also time the vendor library's FP16 GEMM at the coarse GEMM's shape "
-                         "(800-particle batch rows x all orientations x 2 P_c) for comparison")
+                         "(800-particle batch rows x all orientations x 2 P_c), in a child process, "
+                         "for comparison with the dominant kernel's MFMA-product rate (default: on)")
     ap.add_argument("--gemm-library-shape", default="", help=argparse.SUPPRESS)
     args = ap.parse_args()
     if args.gemm_library_shape:   # child of --gemm-library-reference

@@ -59,3 +59,31 @@ def test_bad_variant_rejected(case, gpu_device):
         assert st == T.T2X_EINVAL
     finally:
         wl.cfg.gemm_tile = 258
+
+
+def test_two_product_kernel_across_k_step_counts(gpu_device):
+    """The 2-product kernel (FP16X2) at coarse sizes whose K-step counts (2 ceil(P_c / 32)) fall
+    in each residue mod 3 (and so end the two-stage K loop on either stage, after an odd or even
+    number of step pairs) gives the oracle's coarse diff2 within the 2-product bar (2e-4
+    relative: the reference rows carry 11 significant bits) and its max-weight poses; a stage
+    read before its loads landed, or a wrong slice, misses by orders of magnitude."""
+    lib = T.load()
+    seen = {}
+    for cs in (14, 16, 18, 24):
+        wl = H.Workload("refine64", n=6, seed_offset=41, current_size=cs, coarse_size=cs,
+                        precision=T.T2X_PREC_FP16X2)
+        o = wl.run_oracle(n_debug=6, fine_cap=16384)
+        T.check(lib.t2x_stats_reset(gpu_device))
+        g = wl.run_gpu(dev=gpu_device, n_debug=6, fine_cap=16384)
+        st = T.Stats()
+        T.check(lib.t2x_stats_get(gpu_device, C.byref(st)))
+        nk = 2 * -(-st.P_coarse // 32)
+        valid = o["dbg"]["coarse_diff2"] > -998
+        assert np.array_equal(valid, g["dbg"]["coarse_diff2"] > -998)
+        e = H.rel(g["dbg"]["coarse_diff2"][valid], o["dbg"]["coarse_diff2"][valid])
+        print(f"\n[fp16x2] coarse size {cs}: P_c {st.P_coarse}, {nk} K steps, coarse diff2 max rel {e:.2e}")
+        assert st.gemm_tile_used == 258
+        assert e < 2e-4, cs
+        assert np.array_equal(g["out"]["best_hidden"], o["out"]["best_hidden"]), cs
+        seen.setdefault(nk % 3, cs)
+    assert set(seen) == {0, 1, 2}, seen
