@@ -68,9 +68,12 @@ extern "C" int t2x_device_open(int device_id, t2x_device** out)
     T2X_HIP(hipSetDevice(device_id));
     float* scratch_ptr = nullptr;
     T2X_TRY(noalloc_scratch_address(device_id, &scratch_ptr));
+    int n_cu = 0;
+    T2X_HIP(hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, device_id));
     auto* d = new Device();
     d->id = device_id;
     d->noalloc_scratch = scratch_ptr;
+    d->n_cu = n_cu > 0 ? n_cu : 256;
     if (hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking) != hipSuccess) {
         delete d;
         set_error("hipStreamCreate failed");

@@ -49,11 +49,14 @@ int launch_gemm_local_x3(int prec, const void* Sh, const void* Sl, const float* 
                          float* d2, unsigned int* dmin, hipStream_t st);
 
 __global__ void k_pass_min(WeightSrc s, int B, SelResult* res);
-__global__ void k_select(WeightSrc s, SelParams prm, int B, SelResult* res);
-__global__ void k_sig_count(WeightSrc s, int B, const SelResult* res, long long* counts);
+__global__ void k_select(WeightSrc s, SelParams prm, int B, SelResult* res, const int* only);
+__global__ void k_sig_count(WeightSrc s, int B, const SelResult* res, long long* counts, const int* only);
 __global__ void k_sig_write(WeightSrc s, int B, const SelResult* res, const long long* pair_off,
                             const long long* orient_off, int2* pairs, int* orients,
-                            long long* orient_pair_start);
+                            long long* orient_pair_start, const int* only);
+__global__ void k_batch_plan(const long long* counts, const SelResult* selc, int B, int over_rot, int rows_per_tile,
+                             int ncb, long long cap_pairs, long long cap_orients, long long* pair_off,
+                             long long* orient_off, long long* item_off, long long* tile_off, BatchPlan* plan);
 __global__ void k_fine_argmax(WeightSrc s, int B, SelResult* res, long long* nfine_sig);
 struct SumPart;
 __global__ void k_coarse_sum_rows(WeightSrc s, int B, const SelResult* res, SumPart* part);
@@ -64,10 +67,10 @@ size_t sum_part_bytes();
 __global__ void k_log_prior(const double* p, long long n, double* lp);
 __global__ void k_cand_select(WeightSrc s, int B, SelParams prm, SelResult* res, const double* cand_w,
                               const int* cand_e, const int* cand_count, int2* pairs_strided,
-                              long long* counts, int* overflow);
+                              long long* counts, int* overflow, int force);
 __global__ void k_cand_scatter(int B, const int2* pairs_strided, const long long* pair_off,
                                const long long* orient_off, int2* pairs, int* orients,
-                               long long* orient_pair_start);
+                               long long* orient_pair_start, const int* skip);
 static const int kCandCap = 4096;   // CAND_CAP of t2x_select.hip
 // Dynamic LDS of the kernels that hold one operand row while they find its maximum; longer rows
 // take the path through HBM (FP32 row, then k_split_rows).
@@ -524,6 +527,7 @@ struct FineTileParams {
     const PartScalars* ps;
     float* fine_d2;
     unsigned int* fine_min;
+    unsigned long long* queue;          // work-queue counter (zero at launch)
 };
 
 template <int MODE> struct FineHalf { typedef _Float16 type; };
@@ -550,7 +554,7 @@ __device__ __forceinline__ floatx4 fine_mfma32<_Float16>(Vec8<_Float16>::type a,
                                                                                : 2 * FT_ROWS * FT_ZLD16 * 2)
 
 template <int MODE>
-__global__ __launch_bounds__(256) void k_fine_mfma(FineTileParams P)
+__device__ __forceinline__ void fine_tile(const FineTileParams& P, const long long tile)
 {
     typedef typename FineHalf<MODE>::type H;
     typedef typename Vec8<H>::type v8;
@@ -566,7 +570,6 @@ __global__ __launch_bounds__(256) void k_fine_mfma(FineTileParams P)
     __shared__ float s_wmin[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const FineGeom& g = P.g;
-    const long long tile = blockIdx.x;
     const int b = find_particle(P.tile_off, P.B, tile);
     const long long local = tile - P.tile_off[b];
     const int rt = (int)(local / P.ncb), cb = (int)(local - (long long)rt * P.ncb);
@@ -821,6 +824,25 @@ __global__ __launch_bounds__(256) void k_fine_mfma(FineTileParams P)
     if (tid == 0) {
         const float m = fminf(fminf(s_wmin[0], s_wmin[1]), fminf(s_wmin[2], s_wmin[3]));
         if (m < 3.0e38f) atomicMin(&P.fine_min[b], float_to_ordered(m));
+    }
+}
+
+// The fine pass of a batch as one persistent launch: each workgroup takes the next tile from
+// the batch's work queue (P.queue, cleared by k_batch_plan) until the tiles, whose count the
+// plan wrote to tile_off[B], run out. Particles with many significant samples simply own more
+// tiles; the grid is sized to the device, not to the work.
+template <int MODE>
+__global__ __launch_bounds__(256) void k_fine_mfma(FineTileParams P)
+{
+    __shared__ long long s_tile;
+    const long long total = P.tile_off[P.B];
+    for (;;) {
+        __syncthreads();   // every thread is done with the previous tile's LDS
+        if (threadIdx.x == 0) s_tile = (long long)atomicAdd(P.queue, 1ull);
+        __syncthreads();
+        const long long tile = s_tile;
+        if (tile >= total) break;
+        fine_tile<MODE>(P, tile);
     }
 }
 
@@ -1150,12 +1172,13 @@ struct StoreParams {
 
 #define STORE_MAX_T 1024
 
-__global__ __launch_bounds__(256) void k_store(const long long* __restrict__ item_off, int B,
-                                               const long long* __restrict__ orient_off,
-                                               const int* __restrict__ orients,
-                                               const long long* __restrict__ orient_pair_start,
-                                               const long long* __restrict__ pair_off,
-                                               const int2* __restrict__ pairs, StoreParams P)
+__device__ __forceinline__ void store_item(const long long* __restrict__ item_off, int B,
+                                           const long long* __restrict__ orient_off,
+                                           const int* __restrict__ orients,
+                                           const long long* __restrict__ orient_pair_start,
+                                           const long long* __restrict__ pair_off,
+                                           const int2* __restrict__ pairs, const StoreParams& P,
+                                           const long long item)
 {
     __shared__ int s_ft[STORE_MAX_T];
     __shared__ float s_w[STORE_MAX_T];
@@ -1163,7 +1186,6 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
     __shared__ double shell[520];
     __shared__ double red[4];
     const FineGeom& g = P.g;
-    const long long item = blockIdx.x;
     const int b = find_particle(item_off, B, item);
     const long long local = item - item_off[b];
     const long long j = orient_off[b] + local / g.over_rot;
@@ -1378,6 +1400,29 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
     double* w_s2 = P.w_sigma2 + (size_t)og * P.nshell;
     for (int i = threadIdx.x; i < P.nshell; i += blockDim.x)
         if (shell[i] != 0.) atomicAdd(&w_s2[i], shell[i]);
+}
+
+// Weighted sums and backprojection of a batch as one persistent launch: each workgroup takes the
+// next (significant orientation, oversampled rotation) item from the work queue (cleared by
+// k_batch_plan) until the batch's items (item_off[B]) run out.
+__global__ __launch_bounds__(256) void k_store(const long long* __restrict__ item_off, int B,
+                                               const long long* __restrict__ orient_off,
+                                               const int* __restrict__ orients,
+                                               const long long* __restrict__ orient_pair_start,
+                                               const long long* __restrict__ pair_off,
+                                               const int2* __restrict__ pairs, StoreParams P,
+                                               unsigned long long* queue)
+{
+    __shared__ long long s_item;
+    const long long total = item_off[B];
+    for (;;) {
+        __syncthreads();   // every thread is done with the previous item's LDS
+        if (threadIdx.x == 0) s_item = (long long)atomicAdd(queue, 1ull);
+        __syncthreads();
+        const long long item = s_item;
+        if (item >= total) break;
+        store_item(item_off, B, orient_off, orients, orient_pair_start, pair_off, pairs, P, item);
+    }
 }
 
 __global__ void k_fill_u32(unsigned int* p, unsigned int v, int n)
@@ -1628,12 +1673,6 @@ __global__ void k_row_slices(long long nrows, const int* __restrict__ orients, c
 {
     const long long j = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (j < nrows) rowu[j] = pos[orients[j]];
-}
-
-__global__ void k_items_from_orients(const long long* orient_off, int B, int over_rot, long long* item_off)
-{
-    int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i <= B) item_off[i] = orient_off[i] * over_rot;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -2339,9 +2378,11 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         SelResult* d_selc = d->sel_out.as<SelResult>();
         SelResult* d_self = d_selc + B;
         T2X_TRY(d->sig_count.ensure(sizeof(long long) * 2 * B));
-        std::vector<long long> counts(2 * B);
         std::vector<SelResult> hselc(B);
-        bool fallback = true;
+        // particles sent to the radix select (candidate path: those whose candidates overflowed;
+        // null: every particle, as in local searches)
+        const int* d_radix = nullptr;
+        bool cand_path = false;
         WeightSrc wc{};
         long long npairs_c = 0;
         std::vector<unsigned> prop_mask_host;   // learned proposal: per particle, the bins scored
@@ -2533,12 +2574,9 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             wc.lp = lp;
             k_pass_min<<<B, 1024, 0, st>>>(wc, B, d_selc);
             SelParams sp{cfg->adaptive_fraction, (long long)cfg->maximum_significants, 1};
-            k_select<<<B, 1024, 0, st>>>(wc, sp, B, d_selc);
-            k_sig_count<<<B, 1024, 0, st>>>(wc, B, d_selc, d->sig_count.as<long long>());
+            k_select<<<B, 1024, 0, st>>>(wc, sp, B, d_selc, nullptr);
+            k_sig_count<<<B, 1024, 0, st>>>(wc, B, d_selc, d->sig_count.as<long long>(), nullptr);
             T2X_HIP(hipGetLastError());
-            T2X_HIP(hipMemcpyAsync(counts.data(), d->sig_count.ptr, sizeof(long long) * 2 * B, hipMemcpyDeviceToHost, st));
-            T2X_HIP(hipMemcpyAsync(hselc.data(), d_selc, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
-            T2X_HIP(hipStreamSynchronize(st));
             (void)hipEventRecord(d->ev[ev++], st);   // 4: coarse weights done
         } else {
             const long long M_pad = round_up(M, 256);
@@ -2750,58 +2788,85 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                     k_coarse_candidates<<<grid, 256, 0, st>>>(wc, B, sp, d_selc, d->cand_w.as<double>(),
                                                               d->cand_e.as<int>(), d_cand_count);
                 }
+                // T2X_FORCE_RADIX_SELECT (tests): "1" sends every particle to the radix select,
+                // "odd" every odd-numbered one
+                const char* force_env = getenv("T2X_FORCE_RADIX_SELECT");
+                const int force = !force_env ? 0 : (std::strcmp(force_env, "odd") == 0 ? 2 : 1);
                 k_cand_select<<<B, 1024, 0, st>>>(wc, B, sp, d_selc, d->cand_w.as<double>(), d->cand_e.as<int>(),
                                                   d_cand_count, d->pairs_strided.as<int2>(),
-                                                  d->sig_count.as<long long>(), d_overflow);
+                                                  d->sig_count.as<long long>(), d_overflow, force);
                 T2X_HIP(hipGetLastError());
-                // overflow flags, counts and results in one synchronisation; the counts and results are
-                // read again only when a particle overflowed
-                std::vector<int> hover(B);
-                T2X_HIP(hipMemcpyAsync(hover.data(), d_overflow, sizeof(int) * B, hipMemcpyDeviceToHost, st));
-                T2X_HIP(hipMemcpyAsync(counts.data(), d->sig_count.ptr, sizeof(long long) * 2 * B, hipMemcpyDeviceToHost, st));
-                T2X_HIP(hipMemcpyAsync(hselc.data(), d_selc, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
-                T2X_HIP(hipStreamSynchronize(st));
-                fallback = false;
-                for (int i = 0; i < B; i++) fallback |= hover[i] != 0;
-                if (getenv("T2X_FORCE_RADIX_SELECT")) fallback = true;   // exercises the fallback in tests
+                cand_path = true;
+                d_radix = d_overflow;
             }
-            if (fallback) {
-                k_select<<<B, 1024, 0, st>>>(wc, sp, B, d_selc);
-                k_sig_count<<<B, 1024, 0, st>>>(wc, B, d_selc, d->sig_count.as<long long>());
-                T2X_HIP(hipGetLastError());
-                stt.n_select_fallbacks += 1;
-                T2X_HIP(hipMemcpyAsync(counts.data(), d->sig_count.ptr, sizeof(long long) * 2 * B, hipMemcpyDeviceToHost, st));
-                T2X_HIP(hipMemcpyAsync(hselc.data(), d_selc, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
-                T2X_HIP(hipStreamSynchronize(st));
-            }
+            // the particles whose candidates overflowed (all of them without the candidate path)
+            // take the exact radix select over their whole row, on the device: no host decision
+            k_select<<<B, 1024, 0, st>>>(wc, sp, B, d_selc, d_radix);
+            k_sig_count<<<B, 1024, 0, st>>>(wc, B, d_selc, d->sig_count.as<long long>(), d_radix);
+            T2X_HIP(hipGetLastError());
             (void)hipEventRecord(d->ev[ev++], st);   // 4: coarse weights done
         }
-        std::vector<long long> pair_off(B + 1, 0), orient_off(B + 1, 0);
-        for (int i = 0; i < B; i++) {
-            if (hselc[i].npos == 0 || !(hselc[i].sum_w > 0.) || std::isnan(hselc[i].sum_w)) {
-                set_error("zero sum of weights in the coarse pass for particle " + std::to_string(b0 + i));
-                return T2X_EZEROSUM;
-            }
-            if (hselc[i].nsig == 0) { set_error("no significant coarse sample"); return T2X_ENOSIG; }
-            pair_off[i + 1] = pair_off[i] + counts[2 * i];
-            orient_off[i + 1] = orient_off[i] + counts[2 * i + 1];
-        }
-        const long long npairs = pair_off[B], norients = orient_off[B];
+        // ---- the batch's work lists, planned on the device ----
+        // Offsets of every particle's significant pairs, orientations, fine items and fine tiles
+        // come from k_batch_plan, and the fine pass and the store run as persistent work-queue
+        // launches over them: nothing is read back between the coarse selection and the end of
+        // the batch. The pair lists hold kCandCap pairs per particle (the candidate path's
+        // bound; a local search's scored pairs bound its significant ones); radix-selected
+        // particles can exceed that, and then the plan reports its totals, no kernel after it
+        // does any work, and the batch is planned again with lists of that size.
+        const int TW = (int)((T + 31) / 32);
+        const int ncb = (int)((T * over_trans + FT_COLS - 1) / FT_COLS);
         DevBuf &d_pair_off = d->pair_off_d, &d_orient_off = d->orient_off_d, &d_item_off = d->item_off_d;
-        T2X_TRY(upload(d_pair_off, pair_off, st));
-        T2X_TRY(upload(d_orient_off, orient_off, st));
-        T2X_TRY(d->sig_pairs.ensure(sizeof(int2) * std::max<long long>(npairs, 1)));
-        T2X_TRY(d->sig_orient.ensure(sizeof(int) * std::max<long long>(norients, 1)));
-        T2X_TRY(d->orient_pairs_off.ensure(sizeof(long long) * std::max<long long>(norients, 1)));
-        if (fallback)
-            k_sig_write<<<B, 1024, 0, st>>>(wc, B, d_selc, d_pair_off.as<long long>(), d_orient_off.as<long long>(),
-                                            d->sig_pairs.as<int2>(), d->sig_orient.as<int>(),
-                                            d->orient_pairs_off.as<long long>());
-        else
+        T2X_TRY(d_pair_off.ensure(sizeof(long long) * (B + 1)));
+        T2X_TRY(d_orient_off.ensure(sizeof(long long) * (B + 1)));
+        T2X_TRY(d_item_off.ensure(sizeof(long long) * (B + 1)));
+        T2X_TRY(d->fine_tile_off.ensure(sizeof(long long) * (B + 1)));
+        T2X_TRY(d->plan.ensure(sizeof(BatchPlan)));
+        BatchPlan* d_plan = d->plan.as<BatchPlan>();
+        long long cap = local_pairs ? std::max<long long>(npairs_c, 1) : (long long)B * kCandCap;
+        const char* cap_env = getenv("T2X_PLAN_CAP");   // tests: a small capacity forces a second plan
+        if (cap_env && atoll(cap_env) > 0) cap = atoll(cap_env);
+        else cap = std::max(cap, d->plan_cap);
+        BatchPlan hplan{};
+        std::vector<long long> pair_off(B + 1, 0);
+        long long npairs = 0, nitems = 0;
+        const int ev_plan = ev;
+        // per-particle results, read back at the end of the batch
+        std::vector<SelResult> hself(B);
+        std::vector<long long> hnsig(B);
+        std::vector<double> hnorm(3 * B), hpower((size_t)B * xN);
+        std::vector<PartScalars> hps2(B);
+        for (int attempt = 0;; attempt++) {
+        ev = ev_plan;
+        T2X_TRY(d->sig_pairs.ensure(sizeof(int2) * cap));
+        T2X_TRY(d->sig_orient.ensure(sizeof(int) * cap));
+        T2X_TRY(d->orient_pairs_off.ensure(sizeof(long long) * cap));
+        T2X_TRY(d->fine_diff2.ensure(sizeof(float) * cap * nover));
+        k_batch_plan<<<1, 1024, 0, st>>>(d->sig_count.as<long long>(), d_selc, B, over_rot, FT_ROWS, ncb, cap, cap,
+                                         d_pair_off.as<long long>(), d_orient_off.as<long long>(),
+                                         d_item_off.as<long long>(), d->fine_tile_off.as<long long>(), d_plan);
+        if (cand_path)
             k_cand_scatter<<<B, 256, 0, st>>>(B, d->pairs_strided.as<int2>(), d_pair_off.as<long long>(),
                                               d_orient_off.as<long long>(), d->sig_pairs.as<int2>(),
-                                              d->sig_orient.as<int>(), d->orient_pairs_off.as<long long>());
+                                              d->sig_orient.as<int>(), d->orient_pairs_off.as<long long>(), d_radix);
+        k_sig_write<<<B, 1024, 0, st>>>(wc, B, d_selc, d_pair_off.as<long long>(), d_orient_off.as<long long>(),
+                                        d->sig_pairs.as<int2>(), d->sig_orient.as<int>(),
+                                        d->orient_pairs_off.as<long long>(), d_radix);
         T2X_HIP(hipGetLastError());
+        const bool want_dbg = dbg && dbg->n_debug > 0 && b0 < dbg->n_debug;
+        if (want_dbg) {   // diagnostics read the plan and the coarse results on the host
+            T2X_HIP(hipMemcpyAsync(&hplan, d_plan, sizeof(BatchPlan), hipMemcpyDeviceToHost, st));
+            T2X_HIP(hipMemcpyAsync(pair_off.data(), d_pair_off.ptr, sizeof(long long) * (B + 1), hipMemcpyDeviceToHost, st));
+            T2X_HIP(hipMemcpyAsync(hselc.data(), d_selc, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
+            T2X_HIP(hipStreamSynchronize(st));
+            npairs = hplan.exceeded ? 0 : hplan.npairs;
+            if (hplan.err_code) {
+                set_error(std::string(hplan.err_code == T2X_EZEROSUM ? "zero sum of weights in the coarse pass"
+                                                                      : "no significant coarse sample") +
+                          " for particle " + std::to_string(b0 + hplan.err_particle));
+                return hplan.err_code;
+            }
+        }
         // debug: coarse diff2 / significance in ihidden order (class, dir, psi, trans)
         if (dbg && dbg->n_debug > 0 && b0 < dbg->n_debug && local_pairs) {
             const int nd = std::min(B, dbg->n_debug - b0);
@@ -2857,8 +2922,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                 if (dbg->coarse_sig_weight) dbg->coarse_sig_weight[b0 + i] = hselc[i].sig_w;
             }
             if (dbg->coarse_significant) {
-                std::vector<int2> hp(npairs);
-                T2X_HIP(hipMemcpy(hp.data(), d->sig_pairs.ptr, sizeof(int2) * npairs, hipMemcpyDeviceToHost));
+                std::vector<int2> hp(std::max<long long>(npairs, 1));
+                T2X_HIP(hipMemcpy(hp.data(), d->sig_pairs.ptr, sizeof(int2) * hp.size(), hipMemcpyDeviceToHost));
                 for (int i = 0; i < nd; i++) {
                     unsigned char* m = dbg->coarse_significant + (size_t)(b0 + i) * Nh;
                     std::memset(m, 0, Nh);
@@ -2867,11 +2932,6 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             }
         }
         // ---- fine pass ----
-        T2X_TRY(d_item_off.ensure(sizeof(long long) * (B + 1)));
-        k_items_from_orients<<<(B + 256) / 256, 256, 0, st>>>(d_orient_off.as<long long>(), B, over_rot,
-                                                              d_item_off.as<long long>());
-        const long long nitems = norients * over_rot;
-        T2X_TRY(d->fine_diff2.ensure(sizeof(float) * std::max<long long>(npairs * nover, 1)));
         T2X_TRY(d->fine_min.ensure(sizeof(unsigned int) * B));
         k_fill_u32<<<(B + 255) / 256, 256, 0, st>>>(d->fine_min.as<unsigned int>(), 0xffffffffu, B);
         FineGeom g{};
@@ -2887,17 +2947,10 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         g.pf = pf;
         g.ref2d = ref2d;
         (void)hipEventRecord(d->ev[ev++], st);   // 5: fine start
-        if (nitems > 0) {
+        {
             // tiles of 16 items x 64 columns per particle (items: its significant orientations x
-            // oversampled rotations; columns: its significant coarse translations x oversampled ones)
-            const int TW = (int)((T + 31) / 32);
-            const int ncb = (int)((T * over_trans + FT_COLS - 1) / FT_COLS);
-            std::vector<long long> tile_off(B + 1, 0);
-            for (int i = 0; i < B; i++)
-                tile_off[i + 1] = tile_off[i] + ((orient_off[i + 1] - orient_off[i]) * over_rot + FT_ROWS - 1) / FT_ROWS * ncb;
-            const long long ntiles = tile_off[B];
-            if (ntiles > 0x7fffffffLL) { set_error("fine pass: too many tiles in one batch"); return T2X_EINVAL; }
-            T2X_TRY(upload(d->fine_tile_off, tile_off, st));
+            // oversampled rotations; columns: its significant coarse translations x oversampled
+            // ones), planned by k_batch_plan, taken from a work queue by a persistent grid
             T2X_TRY(d->fine_tmask.ensure(sizeof(unsigned) * B * TW));
             k_fine_tmask<<<B, 256, 0, st>>>(d_pair_off.as<long long>(), d->sig_pairs.as<int2>(), B, TW,
                                             d->fine_tmask.as<unsigned>());
@@ -2931,12 +2984,14 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             fp.ps = d_ps;
             fp.fine_d2 = d->fine_diff2.as<float>();
             fp.fine_min = d->fine_min.as<unsigned int>();
-            if (ntiles > 0) {
-                if (prec == T2X_PREC_FP32) k_fine_mfma<0><<<(unsigned)ntiles, 256, 0, st>>>(fp);
-                else if (prec == T2X_PREC_BF16X3) k_fine_mfma<1><<<(unsigned)ntiles, 256, 0, st>>>(fp);
-                else k_fine_mfma<2><<<(unsigned)ntiles, 256, 0, st>>>(fp);
-                T2X_HIP(hipGetLastError());
-            }
+            fp.queue = &d_plan->queue[0];
+            // 4 workgroups per compute unit (the tile's LDS allows 4); those beyond the batch's
+            // tiles leave at their first queue read
+            const unsigned grid = (unsigned)(4 * d->n_cu);
+            if (prec == T2X_PREC_FP32) k_fine_mfma<0><<<grid, 256, 0, st>>>(fp);
+            else if (prec == T2X_PREC_BF16X3) k_fine_mfma<1><<<grid, 256, 0, st>>>(fp);
+            else k_fine_mfma<2><<<grid, 256, 0, st>>>(fp);
+            T2X_HIP(hipGetLastError());
         }
         (void)hipEventRecord(d->ev[ev++], st);   // 6: fine diff2 end
         WeightSrc wf = wc;
@@ -2949,18 +3004,18 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         wf.fine_min = d->fine_min.as<unsigned int>();
         k_pass_min<<<B, 1024, 0, st>>>(wf, B, d_self);
         SelParams spf{cfg->adaptive_fraction, (long long)cfg->maximum_significants, 0};
-        k_select<<<B, 1024, 0, st>>>(wf, spf, B, d_self);
+        k_select<<<B, 1024, 0, st>>>(wf, spf, B, d_self, nullptr);
         T2X_TRY(d->fine_sel.ensure(sizeof(long long) * B));
         k_fine_argmax<<<B, 1024, 0, st>>>(wf, B, d_self, d->fine_sel.as<long long>());
         T2X_HIP(hipGetLastError());
         (void)hipEventRecord(d->ev[ev++], st);   // 7: fine weights done
         if (dbg && dbg->n_debug > 0 && b0 < dbg->n_debug && dbg->fine_count) {
             int nd = std::min(B, dbg->n_debug - b0);
-            std::vector<float> fd(npairs * nover);
-            std::vector<int2> hp(npairs);
+            std::vector<float> fd(std::max<long long>(npairs * nover, 1));
+            std::vector<int2> hp(std::max<long long>(npairs, 1));
             T2X_HIP(hipStreamSynchronize(st));
-            T2X_HIP(hipMemcpy(fd.data(), d->fine_diff2.ptr, sizeof(float) * npairs * nover, hipMemcpyDeviceToHost));
-            T2X_HIP(hipMemcpy(hp.data(), d->sig_pairs.ptr, sizeof(int2) * npairs, hipMemcpyDeviceToHost));
+            T2X_HIP(hipMemcpy(fd.data(), d->fine_diff2.ptr, sizeof(float) * fd.size(), hipMemcpyDeviceToHost));
+            T2X_HIP(hipMemcpy(hp.data(), d->sig_pairs.ptr, sizeof(int2) * hp.size(), hipMemcpyDeviceToHost));
             for (int i = 0; i < nd; i++) {
                 long cnt = 0;
                 // CPU order: orientation, oversampled rotation, translation, oversampled translation
@@ -3032,25 +3087,46 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         P.Nd = (int)Nd;
         P.lp = lp;
         if (xN > 520) { set_error("box too large for the shell accumulator"); return T2X_EINVAL; }
-        if (nitems > 0) {
-            k_store<<<(unsigned)nitems, 256, 0, st>>>(d_item_off.as<long long>(), B, d_orient_off.as<long long>(),
-                                                      d->sig_orient.as<int>(), d->orient_pairs_off.as<long long>(),
-                                                      d_pair_off.as<long long>(), d->sig_pairs.as<int2>(), P);
-            T2X_HIP(hipGetLastError());
-        }
+        // one persistent launch: 8 workgroups per compute unit take the items from the queue
+        k_store<<<(unsigned)(8 * d->n_cu), 256, 0, st>>>(d_item_off.as<long long>(), B, d_orient_off.as<long long>(),
+                                                           d->sig_orient.as<int>(), d->orient_pairs_off.as<long long>(),
+                                                           d_pair_off.as<long long>(), d->sig_pairs.as<int2>(), P,
+                                                           &d_plan->queue[1]);
+        T2X_HIP(hipGetLastError());
         (void)hipEventRecord(d->ev[ev++], st);   // 8: store done
-        // ---- per-particle results ----
-        std::vector<SelResult> hself(B);
-        std::vector<long long> hnsig(B);
-        std::vector<double> hnorm(3 * B), hpower((size_t)B * xN);
-        std::vector<PartScalars> hps2(B);
-        std::vector<int2> hpairs;
+        // ---- per-particle results: the batch's one readback ----
+        T2X_HIP(hipMemcpyAsync(&hplan, d_plan, sizeof(BatchPlan), hipMemcpyDeviceToHost, st));
+        T2X_HIP(hipMemcpyAsync(pair_off.data(), d_pair_off.ptr, sizeof(long long) * (B + 1), hipMemcpyDeviceToHost, st));
+        T2X_HIP(hipMemcpyAsync(hselc.data(), d_selc, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
         T2X_HIP(hipMemcpyAsync(hself.data(), d_self, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
         T2X_HIP(hipMemcpyAsync(hnsig.data(), d->fine_sel.ptr, sizeof(long long) * B, hipMemcpyDeviceToHost, st));
         T2X_HIP(hipMemcpyAsync(hnorm.data(), d->best.ptr, sizeof(double) * 3 * B, hipMemcpyDeviceToHost, st));
         T2X_HIP(hipMemcpyAsync(hpower.data(), d->power.ptr, sizeof(double) * B * xN, hipMemcpyDeviceToHost, st));
         T2X_HIP(hipMemcpyAsync(hps2.data(), d_ps, sizeof(PartScalars) * B, hipMemcpyDeviceToHost, st));
         T2X_HIP(hipStreamSynchronize(st));
+        if (hplan.err_code) {
+            set_error(std::string(hplan.err_code == T2X_EZEROSUM ? "zero sum of weights in the coarse pass"
+                                                                  : "no significant coarse sample") +
+                      " for particle " + std::to_string(b0 + hplan.err_particle));
+            return hplan.err_code;
+        }
+        if (!hplan.exceeded) break;
+        // lists too small for the radix-selected particles: grow them to the plan's totals and
+        // run the batch's fine pass and store again (the first run did no work)
+        if (attempt > 0) { set_error("batch plan: work lists exceeded twice"); return T2X_EDEVICE; }
+        cap = std::max(hplan.npairs, hplan.norients);
+        d->plan_cap = std::max(d->plan_cap, cap);
+        }   // attempt
+        npairs = hplan.npairs;
+        nitems = hplan.nitems;
+        if (!cand_path) {
+            stt.n_select_fallbacks += 1;
+        } else {   // did any particle of the batch take the radix select?
+            std::vector<int> hflag(B);
+            T2X_HIP(hipMemcpy(hflag.data(), d_radix, sizeof(int) * B, hipMemcpyDeviceToHost));
+            for (int i = 0; i < B; i++)
+                if (hflag[i]) { stt.n_select_fallbacks += 1; break; }
+        }
         {
             float ms[8] = {0};
             for (int e = 1; e < ev; e++) (void)hipEventElapsedTime(&ms[e - 1], d->ev[e - 1], d->ev[e]);

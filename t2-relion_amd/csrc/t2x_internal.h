@@ -144,6 +144,12 @@ struct Device {
     // fine pass: tiles per particle, significant-translation masks, the phase rows as the GEMM
     // operand (FP32 or hi / lo halves)
     DevBuf fine_tile_off, fine_tmask, ab_fk, ab_fh, ab_fl;
+    // the batch plan (k_batch_plan: list totals, error status, work-queue counters), the
+    // capacity of the significant-pair lists grown by earlier batches, and the number of compute
+    // units (the persistent kernels' grids)
+    DevBuf plan;
+    long long plan_cap = 0;
+    int n_cu = 0;
 
     // cached coarse projections
     std::string coarse_key;

@@ -40,6 +40,17 @@ struct SelResult {
 
 enum { SRC_COARSE = 0, SRC_FINE = 1 };
 
+// Totals of a batch's work lists (k_batch_plan), the first particle with an error status, and
+// the work-queue counters of the persistent fine-pass and store kernels.
+struct BatchPlan {
+    long long npairs, norients, nitems, ntiles;
+    int exceeded;       // the lists' capacity was too small: every offset written as zero
+    int err_particle;   // first particle (in the batch) with an error status, or -1
+    int err_code;       // its status (T2X_EZEROSUM, T2X_ENOSIG)
+    int pad;
+    unsigned long long queue[2];   // [0] fine tiles, [1] store items
+};
+
 // Local angular searches (selectOrientationsWithNonZeroPriorProbability, healpix_sampling.cpp:
 // 769-1145): per particle, the normalised prior of every direction and psi angle (0 outside the
 // search range) and the mean of their products over the particle's orientations. dir == null:

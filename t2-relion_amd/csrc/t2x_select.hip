@@ -307,10 +307,11 @@ __device__ void select_run(const WeightSrc& s, int b, double mind2, long long n_
     __syncthreads();
 }
 
-__global__ __launch_bounds__(SEL_THREADS) void k_select(WeightSrc s, SelParams prm, int B, SelResult* res)
+__global__ __launch_bounds__(SEL_THREADS) void k_select(WeightSrc s, SelParams prm, int B, SelResult* res,
+                                                         const int* only)
 {
     const int b = blockIdx.x;
-    if (b >= B) return;
+    if (b >= B || (only && !only[b])) return;
     __shared__ SelShared S;
     __shared__ long long poff[2];
     if (threadIdx.x == 0 && s.kind == SRC_FINE) { poff[0] = s.pair_off[b]; poff[1] = s.pair_off[b + 1]; }
@@ -394,10 +395,10 @@ __global__ __launch_bounds__(SEL_THREADS) void k_select(WeightSrc s, SelParams p
 // orientations having at least one such pair (Mcoarse_significant, ml_optimiser.cpp:7996-8004;
 // isSignificantAnyImageAnyTranslation, :6898-6919).
 __global__ __launch_bounds__(1024) void k_sig_count(WeightSrc s, int B, const SelResult* res,
-                                                    long long* counts /* [B][2] */)
+                                                    long long* counts /* [B][2] */, const int* only)
 {
     const int b = blockIdx.x;
-    if (b >= B) return;
+    if (b >= B || (only && !only[b])) return;
     const double mind2 = res[b].min_d2, sig = res[b].sig_w;
     // dense coarse layout (columns = orientations), or the pair list of a local search's coarse
     // pass (SRC_FINE with one sample per pair: every orientation followed by its T translations)
@@ -429,12 +430,15 @@ __global__ __launch_bounds__(1024) void k_sig_count(WeightSrc s, int B, const Se
 }
 
 // Writes the ordered (orientation-major, then translation) significant pairs and orientations.
+// A particle given no room by the batch plan (an error, or a plan beyond the lists' capacity)
+// writes nothing.
 __global__ __launch_bounds__(1024) void k_sig_write(WeightSrc s, int B, const SelResult* res,
                                                     const long long* pair_off, const long long* orient_off,
-                                                    int2* pairs, int* orients, long long* orient_pair_start)
+                                                    int2* pairs, int* orients, long long* orient_pair_start,
+                                                    const int* only)
 {
     const int b = blockIdx.x;
-    if (b >= B) return;
+    if (b >= B || (only && !only[b]) || pair_off[b + 1] == pair_off[b]) return;
     const double mind2 = res[b].min_d2, sig = res[b].sig_w;
     const bool plist = s.kind == SRC_FINE;   // as in k_sig_count
     const long long* pob = plist ? s.pair_off + b : nullptr;
@@ -795,11 +799,12 @@ __device__ void sort_int_asc(int* a, int n)
 
 // Exact walk over the sorted candidates; the significant pairs, ordered by
 // (orientation, translation), go to a per-particle strided list.
+// force (tests): 1 sends every particle, 2 every odd-numbered one, to the radix select.
 __global__ __launch_bounds__(1024) void k_cand_select(WeightSrc s, int B, SelParams prm,
                                                       SelResult* res, const double* cand_w,
                                                       const int* cand_e, const int* cand_count,
                                                       int2* pairs_strided, long long* counts,
-                                                      int* overflow)
+                                                      int* overflow, int force)
 {
     const int b = blockIdx.x;
     if (b >= B) return;
@@ -807,7 +812,7 @@ __global__ __launch_bounds__(1024) void k_cand_select(WeightSrc s, int B, SelPar
     __shared__ int e[CAND_CAP];
     __shared__ int nsig_pairs;
     const int n = cand_count[b];
-    if (n > CAND_CAP || n <= 0) {
+    if (n > CAND_CAP || n <= 0 || force == 1 || (force == 2 && (b & 1))) {
         if (threadIdx.x == 0) overflow[b] = 1;
         return;
     }
@@ -864,13 +869,14 @@ __global__ __launch_bounds__(1024) void k_cand_select(WeightSrc s, int B, SelPar
     }
 }
 
-// Strided per-particle lists -> compact arrays at the host-computed offsets.
+// Strided per-particle lists -> compact arrays at the batch plan's offsets (particles sent to
+// the radix select, flagged in skip, are written by k_sig_write instead).
 __global__ void k_cand_scatter(int B, const int2* pairs_strided, const long long* pair_off,
                                const long long* orient_off, int2* pairs, int* orients,
-                               long long* orient_pair_start)
+                               long long* orient_pair_start, const int* skip)
 {
     const int b = blockIdx.x;
-    if (b >= B) return;
+    if (b >= B || (skip && skip[b])) return;
     const long long p0 = pair_off[b], m = pair_off[b + 1] - p0;
     for (long long i = threadIdx.x; i < m; i += blockDim.x)
         pairs[p0 + i] = pairs_strided[(size_t)b * CAND_CAP + i];
@@ -883,6 +889,98 @@ __global__ void k_cand_scatter(int B, const int2* pairs_strided, const long long
                 orient_pair_start[o] = p0 + i;
                 o++;
             }
+        }
+    }
+}
+
+
+// The batch's work lists, sized on the device (no host round trip between the coarse and the
+// fine pass). Per particle: its status from the coarse selection (no positive weight or a zero /
+// NaN sum: T2X_EZEROSUM; no significant sample: T2X_ENOSIG; such a particle gets no work), and
+// exclusive prefix sums of its significant pairs, orientations, fine items (orientations x
+// oversampled rotations) and fine tiles (ceil(items / rows_per_tile) x column blocks). When the
+// pairs or orientations exceed the lists' capacity every offset is written as zero (no kernel
+// after this one does any work) and the true totals are kept, so the host can grow the lists and
+// plan again. Clears the work-queue counters of the persistent kernels. One workgroup of 1024
+// threads; B <= 4 * 1024.
+__global__ __launch_bounds__(1024) void k_batch_plan(const long long* __restrict__ counts, const SelResult* __restrict__ selc,
+                                                     int B, int over_rot, int rows_per_tile, int ncb,
+                                                     long long cap_pairs, long long cap_orients,
+                                                     long long* pair_off, long long* orient_off,
+                                                     long long* item_off, long long* tile_off, BatchPlan* plan)
+{
+    __shared__ long long sc[4][1024];
+    __shared__ int s_err_b, s_err_code;
+    const int tid = threadIdx.x;
+    const int per = (B + 1023) / 1024;
+    const int i0 = min(B, tid * per), i1 = min(B, i0 + per);
+    if (tid == 0) { s_err_b = 0x7fffffff; s_err_code = 0; }
+    __syncthreads();
+    long long loc[4] = {0, 0, 0, 0};
+    for (int i = i0; i < i1; i++) {
+        const SelResult& r = selc[i];
+        int code = 0;
+        if (r.npos == 0 || !(r.sum_w > 0.) || isnan(r.sum_w)) code = T2X_EZEROSUM;
+        else if (r.nsig == 0) code = T2X_ENOSIG;
+        if (code) {
+            atomicMin(&s_err_b, i);
+            continue;
+        }
+        const long long np = counts[2 * i], no = counts[2 * i + 1];
+        loc[0] += np;
+        loc[1] += no;
+        loc[2] += no * over_rot;
+        loc[3] += (no * over_rot + rows_per_tile - 1) / rows_per_tile * ncb;
+    }
+    for (int q = 0; q < 4; q++) sc[q][tid] = loc[q];
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        long long v[4];
+        for (int q = 0; q < 4; q++) v[q] = tid >= o ? sc[q][tid - o] : 0;
+        __syncthreads();
+        for (int q = 0; q < 4; q++) sc[q][tid] += v[q];
+        __syncthreads();
+    }
+    const long long tot_p = sc[0][1023], tot_o = sc[1][1023];
+    const bool exceeded = tot_p > cap_pairs || tot_o > cap_orients;
+    long long run[4];
+    for (int q = 0; q < 4; q++) run[q] = sc[q][tid] - loc[q];
+    for (int i = i0; i < i1; i++) {
+        pair_off[i] = exceeded ? 0 : run[0];
+        orient_off[i] = exceeded ? 0 : run[1];
+        item_off[i] = exceeded ? 0 : run[2];
+        tile_off[i] = exceeded ? 0 : run[3];
+        const SelResult& r = selc[i];
+        const bool bad = r.npos == 0 || !(r.sum_w > 0.) || isnan(r.sum_w) || r.nsig == 0;
+        if (!bad) {
+            const long long np = counts[2 * i], no = counts[2 * i + 1];
+            run[0] += np;
+            run[1] += no;
+            run[2] += no * over_rot;
+            run[3] += (no * over_rot + rows_per_tile - 1) / rows_per_tile * ncb;
+        }
+    }
+    if (tid == 1023) {
+        pair_off[B] = exceeded ? 0 : sc[0][1023];
+        orient_off[B] = exceeded ? 0 : sc[1][1023];
+        item_off[B] = exceeded ? 0 : sc[2][1023];
+        tile_off[B] = exceeded ? 0 : sc[3][1023];
+        plan->npairs = tot_p;
+        plan->norients = tot_o;
+        plan->nitems = sc[2][1023];
+        plan->ntiles = sc[3][1023];
+        plan->exceeded = exceeded ? 1 : 0;
+        plan->queue[0] = 0;
+        plan->queue[1] = 0;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        plan->err_particle = s_err_b == 0x7fffffff ? -1 : s_err_b;
+        if (s_err_b != 0x7fffffff) {
+            const SelResult& r = selc[s_err_b];
+            plan->err_code = (r.npos == 0 || !(r.sum_w > 0.) || isnan(r.sum_w)) ? T2X_EZEROSUM : T2X_ENOSIG;
+        } else {
+            plan->err_code = 0;
         }
     }
 }

@@ -142,6 +142,45 @@ def test_radix_select_fallback_matches(gpu_device, monkeypatch):
     assert H.rel_norm(g["bp"][0][1], o["bp"][0][1]) < 1e-4
 
 
+def test_radix_select_per_particle(gpu_device, monkeypatch):
+    """The radix select is chosen per particle on the device (a particle whose candidates
+    overflow takes it, the others keep the candidate walk): with every odd-numbered particle
+    forced onto it, the batch's significant sets, poses and dLL equal the all-candidate run and
+    the oracle's."""
+    wl = H.Workload("refine64", n=10, seed_offset=14)
+    o = wl.run_oracle(n_debug=10, fine_cap=16384)
+    ref = wl.run_gpu(dev=gpu_device, n_debug=10, fine_cap=16384)
+    monkeypatch.setenv("T2X_FORCE_RADIX_SELECT", "odd")
+    g = wl.run_gpu(dev=gpu_device, n_debug=10, fine_cap=16384)
+    assert np.array_equal(g["dbg"]["coarse_significant"], ref["dbg"]["coarse_significant"])
+    assert np.array_equal(g["out"]["best_hidden"], ref["out"]["best_hidden"])
+    assert np.array_equal(g["out"]["dLL"], ref["out"]["dLL"])
+    assert (o["dbg"]["coarse_significant"] == g["dbg"]["coarse_significant"]).all(axis=1).mean() >= 0.99
+    assert (o["out"]["best_hidden"] == g["out"]["best_hidden"]).mean() >= 0.999
+    assert H.rel_norm(g["bp"][0][0], ref["bp"][0][0]) < 1e-6
+
+
+@pytest.mark.parametrize("n_debug", [0, 12])
+def test_batch_plan_grows_its_lists(gpu_device, monkeypatch, n_debug):
+    """The work lists of a batch are planned on the device; when they are too small for the
+    significant pairs (forced here with a capacity of one pair) the plan reports its totals, no
+    kernel after it does any work, and the batch runs again with lists of that size: the
+    results equal those of a run planned right the first time (poses, dLL, weighted sums;
+    backprojections up to the order of their FP32 atomic additions)."""
+    wl = H.Workload("refine64", n=12, seed_offset=15)
+    ref = wl.run_gpu(dev=gpu_device, n_debug=n_debug, fine_cap=16384)
+    monkeypatch.setenv("T2X_PLAN_CAP", "1")
+    g = wl.run_gpu(dev=gpu_device, n_debug=n_debug, fine_cap=16384)
+    assert np.array_equal(g["out"]["best_hidden"], ref["out"]["best_hidden"])
+    assert np.array_equal(g["out"]["dLL"], ref["out"]["dLL"])
+    assert np.array_equal(g["out"]["nr_fine_samples"], ref["out"]["nr_fine_samples"])
+    for k, v in ref["wsum"].items():
+        if np.abs(v).sum() > 0:
+            assert H.rel_norm(g["wsum"][k], v) < 1e-12, k
+    assert H.rel_norm(g["bp"][0][0], ref["bp"][0][0]) < 1e-6
+    assert H.rel_norm(g["bp"][0][1], ref["bp"][0][1]) < 1e-6
+
+
 def test_maximum_significants(gpu_device):
     """--maxsig caps the coarse significant samples (ml_optimiser.cpp:7950-7961)."""
     wl = H.Workload("refine64", n=8, seed_offset=12, maximum_significants=3)
