@@ -7,6 +7,8 @@ same run on the same resident particles:
   fp16x3   split-precision cross terms (FP16 hi+lo on MFMA, FP32 accumulate, FP64 normalisation)
   fp16x2   the dense coarse cross term with 2 MFMA products (FP16 hi+lo images x FP16 hi
            references); the fine pass and every other cross term as fp16x3
+  fp16x2+graph  the same, each batch's launches after its preprocessing captured once per batch
+           size as a hipGraph and replayed (reported beside the eager paths)
   learned  fp16x3 + learned pose proposal: the coarse pass scores only the top-16 bins
            (classes x HEALPix order-2 directions) of the proposal network, exhaustive search for
            particles whose top softmax probability is below 0.3. The network is trained on the
@@ -184,8 +186,12 @@ def train_proposal(lib, dev, cfg, sz, ds, w, steps=2000, batch=256, n_proj=20000
 
 
 def configure(cfg, pname):
-    cfg.precision = PRECISIONS[pname]
-    topk, gate, conf = PROPOSAL.get(pname, (0, 0, 0.0))
+    """A path name is a precision / proposal setting, optionally with "+graph": the same path
+    with each batch's launches after the preprocessing run as a replayed hipGraph."""
+    base, _, mode = pname.partition("+")
+    cfg.precision = PRECISIONS[base]
+    cfg.launch_graph = 1 if mode == "graph" else 0
+    topk, gate, conf = PROPOSAL.get(base, (0, 0, 0.0))
     cfg.proposal_topk = topk
     cfg.proposal_gate = gate
     cfg.proposal_min_conf = conf
@@ -425,9 +431,10 @@ def main():
                     help="also run one 1 GiB device-to-device copy kernel whose FETCH_SIZE/WRITE_SIZE "
                          "counters calibrate the per-dispatch HBM traffic of the GEMM in --pmc runs")
     ap.add_argument("--precision", default="",
-                    help="comma list of paths to time (fp32, bf16x3, fp16x3, fp16x2, learned, learned_gated) "
-                         "(default: fp32,fp16x3,fp16x2,learned,learned_gated for 3D references, "
-                         "fp32,fp16x3,fp16x2 for 2D and local searches)")
+                    help="comma list of paths to time (fp32, bf16x3, fp16x3, fp16x2, learned, learned_gated; "
+                         "a precision path + \"+graph\" runs its batches as replayed hipGraphs) (default: "
+                         "fp32,fp16x3,fp16x2,fp16x2+graph,learned,learned_gated for 3D references, "
+                         "fp32,fp16x3,fp16x2,fp16x2+graph for 2D, fp32,fp16x3,fp16x2 for local searches)")
     ap.add_argument("--headline", default="", help="path of the headline value (default: the fastest path, "
                     "per-iteration costs included, that meets the pose-agreement and FSC bars)")
     ap.add_argument("--resolution", type=float, default=0.0,
@@ -494,8 +501,9 @@ def main():
         T.check(lib.t2x_device_free(dev, dst))
         log("copy calibration: 1 GiB device-to-device copy (t2x::k_copy16) done")
     local = cfg.orientational_prior_mode == T.T2X_PRIOR_ROTTILT_PSI
-    paths = (args.precision or ("fp32,fp16x3,fp16x2" if local or cfg.ref_dim == 2 else
-                                "fp32,fp16x3,fp16x2,learned,learned_gated")).split(",")
+    paths = (args.precision or ("fp32,fp16x3,fp16x2" if local else
+                                "fp32,fp16x3,fp16x2,fp16x2+graph" if cfg.ref_dim == 2 else
+                                "fp32,fp16x3,fp16x2,fp16x2+graph,learned,learned_gated")).split(",")
     train = None
     use_net = any(p.startswith("learned") for p in paths)
     if use_net and args.load_proposal:
@@ -577,11 +585,18 @@ def main():
                       "train_top16_last": train.top16_last if train else None})
         if rr["st"].local_rows > 0:
             v["local_rows_per_slice"] = rr["st"].local_rows / max(1.0, rr["st"].local_slices)
+        if pname.endswith("+graph"):
+            base = pname.partition("+")[0]
+            v.update({"graph_launches_per_step": rr["st"].n_graph_launches / args.steps,
+                      "ms_in_graphs_per_step": rr["st"].ms_graph / args.steps,
+                      "vs_eager_launches": rr["value"] / runs[base]["value"] if base in runs else None,
+                      "note": "batches after the first of each size replay one captured hipGraph; their "
+                              "stages are timed together, so the per-stage fields cover the eager batches only"})
         variants[pname] = v
     quality = None
     if args.quality_particles != 0 and cfg.ref_dim == 3 and not local:
         nq = n if args.quality_particles < 0 else min(n, args.quality_particles)
-        q = quality_phase(lib, dev, cfg, sz, ds, list(runs.keys()), nq)
+        q = quality_phase(lib, dev, cfg, sz, ds, [k for k in runs if "+" not in k], nq)
         quality = {"particles": nq, "fsc_0143_A_per_class": {k: v["fsc_res"] for k, v in q.items()}}
         if "fp32" in q:
             for k in q:
@@ -608,7 +623,10 @@ def main():
     if args.headline:
         head = args.headline
     else:
-        ok = [p for p in runs if meets_bars(p)] or list(runs)
+        # graph-mode paths compute the same results as their eager path; they are reported as
+        # variants (their per-stage timings, which the roofline uses, cover eager batches only)
+        eager = [p for p in runs if "+" not in p]
+        ok = [p for p in eager if meets_bars(p)] or eager
         head = max(ok, key=lambda p: runs[p]["per_iter"])
     r = runs[head]
     st, dt, value = r["st"], r["iter_dt"], r["per_iter"]

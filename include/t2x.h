@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define T2X_ABI_VERSION 3
+#define T2X_ABI_VERSION 4
 
 enum {
     T2X_OK = 0,
@@ -121,6 +121,14 @@ typedef struct t2x_estep_config {
      * sigma_rot and sigma_tilt both > 0 (a Gaussian on the angle between the directions) or
      * both 0 (every direction); sigma_psi 0: every psi angle */
     double sigma_rot, sigma_tilt, sigma_psi;
+    int    launch_graph;           /* 1: from the second batch of a size on, each batch's */
+                                   /* coarse pass, selection, fine pass and store (all    */
+                                   /* launches after the preprocessing) run as one        */
+                                   /* captured hipGraph, replayed for the call's further  */
+                                   /* batches of that size; their stages are then timed   */
+                                   /* together (t2x_stats.ms_graph). Global searches      */
+                                   /* without the proposal and without diagnostics only;  */
+                                   /* elsewhere ignored                                   */
 } t2x_estep_config;
 
 /* Model arrays of the current iteration (host, caller-owned, read-only). */
@@ -236,6 +244,8 @@ typedef struct t2x_stats {
     int    gemm_group_used;
     double local_rows;           /* local searches: (particle, orientation) rows of the coarse pass */
     double local_slices;         /* of those, distinct slices projected (shared within a batch)    */
+    double ms_graph;             /* launch_graph: GPU time of the batches run as a graph           */
+    long   n_graph_launches;     /* batches run as a graph (captured, or replayed)                 */
 } t2x_stats;
 
 /* ---------------------------------------------------------------------------------------

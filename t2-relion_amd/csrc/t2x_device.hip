@@ -94,12 +94,22 @@ static void release_class(ClassVolumes& c)
     c.bp_weight = nullptr;
 }
 
+void t2x::destroy_graph(Device* d)
+{
+    if (d->graph_exec) (void)hipGraphExecDestroy(d->graph_exec);
+    if (d->graph) (void)hipGraphDestroy(d->graph);
+    d->graph_exec = nullptr;
+    d->graph = nullptr;
+    d->graph_B = 0;
+}
+
 extern "C" int t2x_device_close(t2x_device* h)
 {
     if (!h) return T2X_OK;
     Device* d = reinterpret_cast<Device*>(h);
     (void)hipSetDevice(d->id);
     (void)hipStreamSynchronize(d->stream);
+    destroy_graph(d);
     for (auto& kv : d->classes) release_class(kv.second);
     DevBuf* bufs[] = {&d->img, &d->img2, &d->fft_out, &d->Fimg_cur, &d->Fimg_nm_cur, &d->Fctf_cur,
                       &d->pscal, &d->power, &d->A, &d->A1, &d->N1, &d->diff2, &d->minb,

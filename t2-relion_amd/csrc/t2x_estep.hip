@@ -1437,6 +1437,32 @@ __global__ void k_fill_f32(float* p, float v, long long n)
     if (i < n) p[i] = v;
 }
 
+// Zeroes n 32-bit words. The batch's launches clear buffers with this kernel instead of
+// hipMemsetAsync, so that a captured batch graph holds kernel nodes only.
+__global__ void k_zero_words(unsigned* __restrict__ p, long long n)
+{
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
+static int zero_async(void* p, size_t bytes, hipStream_t st)
+{
+    if (bytes % 4) { set_error("zero_async: size not a multiple of 4 bytes"); return T2X_EINVAL; }
+    const long long n = (long long)(bytes / 4);
+    if (n == 0) return T2X_OK;
+    const unsigned grid = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+    k_zero_words<<<grid, 256, 0, st>>>(reinterpret_cast<unsigned*>(p), n);
+    T2X_HIP(hipGetLastError());
+    return T2X_OK;
+}
+
+// C_coarse of each particle's scalars into a dense array (the GEMM epilogue's C_b)
+__global__ void k_gather_ccoarse(const PartScalars* __restrict__ ps, int B, double* __restrict__ out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < B) out[i] = ps[i].C_coarse;
+}
+
 // ------------------------------------------------------------------------------------------
 // Local angular searches: HealpixSampling::selectOrientationsWithNonZeroPriorProbability
 // (healpix_sampling.cpp:769-1145) for C1, without bimodal psi or multi-body terms, sigma_cutoff 3,
@@ -1816,7 +1842,7 @@ static int imgside_split(Device* d, int prec, const PassTables& tc, const PartSc
         return T2X_OK;
     }
     T2X_TRY(d->A.ensure(sizeof(float) * rows * Kc));
-    T2X_HIP(hipMemsetAsync(d->A.ptr, 0, sizeof(float) * rows * Kc, st));
+    T2X_TRY(zero_async(d->A.ptr, sizeof(float) * rows * Kc, st));
     if (B > 0) {
         k_coarse_imgside<<<dim3((unsigned)T, (unsigned)B), 256, 0, st>>>(
             d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw, tc.d_win.as<int>(), tc.d_minv.as<float>(), tc.npix,
@@ -1852,6 +1878,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     }
     Device* d = reinterpret_cast<Device*>(h);
     std::lock_guard<std::mutex> lk(d->mtx);
+    const unsigned long long call_id = ++d->call_counter;   // a batch graph serves one call
+    d->eager_B.clear();
     T2X_HIP(hipSetDevice(d->id));
     HostSampling S;
     T2X_TRY(build_sampling(cfg, S));
@@ -2386,6 +2414,15 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         WeightSrc wc{};
         long long npairs_c = 0;
         std::vector<unsigned> prop_mask_host;   // learned proposal: per particle, the bins scored
+        // Launches of the batch after its preprocessing, in two parts: the coarse pass with its
+        // selection, then (from the batch plan on) the fine pass, weights and store. With
+        // launch_graph they are captured once per batch size as a hipGraph and replayed.
+        bool capturing = false;
+        auto mark = [&]() {   // stage events (not recorded into a graph)
+            if (!capturing) (void)hipEventRecord(d->ev[ev], st);
+            ev++;
+        };
+        auto coarse_part = [&]() -> int {
         if (local_pairs) {
             // local search on per-particle sample lists: each particle's orientations in range x all
             // translations, scored by a GEMM of the batch's (particle, orientation) rows against
@@ -2466,7 +2503,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             const int ldf = tc.npix_pad;
             if (prec == T2X_PREC_FP32) T2X_TRY(d->lc_F.ensure(sizeof(float2) * std::max<long long>(nu, 1) * ldf));
             else T2X_TRY(d->lc_F2.ensure(sizeof(float) * std::max<long long>(nu, 1) * ldf));
-            (void)hipEventRecord(d->ev[ev++], st);   // 2: coarse diff2 start
+            mark();   // 2: coarse diff2 start
             if (norients_c > 0) {
                 const unsigned ugrid = (unsigned)round_up(nu, 8);
                 if (prec == T2X_PREC_FP32) {
@@ -2557,7 +2594,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                                                  d->lc_d2.as<float>(), d->lc_min.as<unsigned int>(), st));
                 }
             }
-            (void)hipEventRecord(d->ev[ev++], st);   // 3: coarse diff2 end
+            mark();   // 3: coarse diff2 end
             wc.kind = SRC_FINE;
             wc.d2 = d->lc_d2.as<float>();
             wc.pair_off = d->lc_pair_off.as<long long>();
@@ -2577,7 +2614,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             k_select<<<B, 1024, 0, st>>>(wc, sp, B, d_selc, nullptr);
             k_sig_count<<<B, 1024, 0, st>>>(wc, B, d_selc, d->sig_count.as<long long>(), nullptr);
             T2X_HIP(hipGetLastError());
-            (void)hipEventRecord(d->ev[ev++], st);   // 4: coarse weights done
+            mark();   // 4: coarse weights done
         } else {
             const long long M_pad = round_up(M, 256);
             const long long B_pad = round_up(B, prec == T2X_PREC_FP32 ? 128 : 256);   // N1 GEMM rows
@@ -2588,9 +2625,9 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             const int rowmin_ld = (int)(O_pad / 64);
             DevBuf& d_rowmin = d->minb;
             T2X_TRY(d_rowmin.ensure(sizeof(float) * M_pad * rowmin_ld));
-            T2X_HIP(hipMemsetAsync(d->A1.ptr, 0, sizeof(float) * B_pad * Kn, st));
+            T2X_TRY(zero_async(d->A1.ptr, sizeof(float) * B_pad * Kn, st));
             if (prec == T2X_PREC_FP32) {
-                T2X_HIP(hipMemsetAsync(d->A.ptr, 0, sizeof(float) * M_pad * Kc, st));
+                T2X_TRY(zero_async(d->A.ptr, sizeof(float) * M_pad * Kc, st));
                 dim3 grid((unsigned)T, (unsigned)B);
                 k_coarse_imgside<<<grid, 256, 0, st>>>(d->Fimg_cur.as<float2>(), d->Fctf_cur.as<float>(), Pw,
                                                        tc.d_win.as<int>(), tc.d_minv.as<float>(), tc.npix, d_ps,
@@ -2685,7 +2722,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                 d->stats.s.ms_proposal +=
                     std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp0).count();
             }
-            (void)hipEventRecord(d->ev[ev++], st);   // 2: coarse gemm start
+            mark();   // 2: coarse gemm start
             {
                 EpiParams ep{};
                 ep.C = d->diff2.as<float>();
@@ -2696,11 +2733,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                 // C_b is read from the PartScalars array: pass a strided view via a packed copy
                 ep.Cb = nullptr;
                 DevBuf& d_cb = d->tables;
-                std::vector<double> dummy;   // filled on device below
                 T2X_TRY(d_cb.ensure(sizeof(double) * B));
-                // copy C_coarse out of the scalars (device to device, strided)
-                T2X_HIP(hipMemcpy2DAsync(d_cb.ptr, sizeof(double), (char*)d_ps + offsetof(PartScalars, C_coarse),
-                                         sizeof(PartScalars), sizeof(double), B, hipMemcpyDeviceToDevice, st));
+                k_gather_ccoarse<<<(unsigned)((B + 255) / 256), 256, 0, st>>>(d_ps, B, d_cb.as<double>());
                 ep.Cb = d_cb.as<double>();
                 ep.N1 = d->N1.as<float>();
                 ep.ldn = O_pad;
@@ -2744,7 +2778,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                     }
                 }
             }
-            (void)hipEventRecord(d->ev[ev++], st);   // 3: coarse gemm end
+            mark();   // 3: coarse gemm end
             // ---- coarse weights and significance ----
             wc.kind = SRC_COARSE;
             if (use_prop) {
@@ -2776,7 +2810,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             T2X_TRY(d->pairs_strided.ensure(sizeof(int2) * B * kCandCap));
             int* d_cand_count = d->cand_n.as<int>();
             int* d_overflow = d_cand_count + B;
-            T2X_HIP(hipMemsetAsync(d_cand_count, 0, sizeof(int) * 2 * B, st));   // counts and overflow flags
+            T2X_TRY(zero_async(d_cand_count, sizeof(int) * 2 * B, st));   // counts and overflow flags
             T2X_TRY(d->sum_part.ensure(sum_part_bytes() * B * T));
             // local searches take the radix-select path directly (the candidate filter works from the
             // global log prior)
@@ -2804,8 +2838,10 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             k_select<<<B, 1024, 0, st>>>(wc, sp, B, d_selc, d_radix);
             k_sig_count<<<B, 1024, 0, st>>>(wc, B, d_selc, d->sig_count.as<long long>(), d_radix);
             T2X_HIP(hipGetLastError());
-            (void)hipEventRecord(d->ev[ev++], st);   // 4: coarse weights done
+            mark();   // 4: coarse weights done
         }
+        return T2X_OK;
+        };
         // ---- the batch's work lists, planned on the device ----
         // Offsets of every particle's significant pairs, orientations, fine items and fine tiles
         // come from k_batch_plan, and the fine pass and the store run as persistent work-queue
@@ -2830,14 +2866,13 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         BatchPlan hplan{};
         std::vector<long long> pair_off(B + 1, 0);
         long long npairs = 0, nitems = 0;
-        const int ev_plan = ev;
+        int ev_plan = ev;
         // per-particle results, read back at the end of the batch
         std::vector<SelResult> hself(B);
         std::vector<long long> hnsig(B);
         std::vector<double> hnorm(3 * B), hpower((size_t)B * xN);
         std::vector<PartScalars> hps2(B);
-        for (int attempt = 0;; attempt++) {
-        ev = ev_plan;
+        auto post_part = [&]() -> int {
         T2X_TRY(d->sig_pairs.ensure(sizeof(int2) * cap));
         T2X_TRY(d->sig_orient.ensure(sizeof(int) * cap));
         T2X_TRY(d->orient_pairs_off.ensure(sizeof(long long) * cap));
@@ -2946,7 +2981,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         g.psi_sc = d_psisc.as<double>();
         g.pf = pf;
         g.ref2d = ref2d;
-        (void)hipEventRecord(d->ev[ev++], st);   // 5: fine start
+        mark();   // 5: fine start
         {
             // tiles of 16 items x 64 columns per particle (items: its significant orientations x
             // oversampled rotations; columns: its significant coarse translations x oversampled
@@ -2993,7 +3028,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             else k_fine_mfma<2><<<grid, 256, 0, st>>>(fp);
             T2X_HIP(hipGetLastError());
         }
-        (void)hipEventRecord(d->ev[ev++], st);   // 6: fine diff2 end
+        mark();   // 6: fine diff2 end
         WeightSrc wf = wc;
         wf.kind = SRC_FINE;
         wf.nover = nover;
@@ -3008,7 +3043,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         T2X_TRY(d->fine_sel.ensure(sizeof(long long) * B));
         k_fine_argmax<<<B, 1024, 0, st>>>(wf, B, d_self, d->fine_sel.as<long long>());
         T2X_HIP(hipGetLastError());
-        (void)hipEventRecord(d->ev[ev++], st);   // 7: fine weights done
+        mark();   // 7: fine weights done
         if (dbg && dbg->n_debug > 0 && b0 < dbg->n_debug && dbg->fine_count) {
             int nd = std::min(B, dbg->n_debug - b0);
             std::vector<float> fd(std::max<long long>(npairs * nover, 1));
@@ -3042,7 +3077,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         }
         // ---- weighted sums and backprojection ----
         T2X_TRY(d->best.ensure(sizeof(double) * 3 * B));   // per particle: wdiff2 sum, XA, AA
-        T2X_HIP(hipMemsetAsync(d->best.ptr, 0, sizeof(double) * 3 * B, st));
+        T2X_TRY(zero_async(d->best.ptr, sizeof(double) * 3 * B, st));
         StoreParams P{};
         P.g = g;
         P.vols = d_vols.as<const float2*>();
@@ -3093,7 +3128,61 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                                                            d_pair_off.as<long long>(), d->sig_pairs.as<int2>(), P,
                                                            &d_plan->queue[1]);
         T2X_HIP(hipGetLastError());
-        (void)hipEventRecord(d->ev[ev++], st);   // 8: store done
+        mark();   // 8: store done
+        return T2X_OK;
+        };
+        // graph mode: global searches without the proposal and without diagnostics; a batch
+        // size is captured only after it ran once eagerly in this call (the GEMM autotuner
+        // has timed its shapes), and a graph is replayed only while no device buffer has
+        // been reallocated since its capture
+        const bool gmode = cfg->launch_graph && !local && !use_prop && !(dbg && dbg->n_debug > 0);
+        bool graph_used = false;
+        for (int attempt = 0;; attempt++) {
+        if (attempt == 0) {
+            const bool replay = gmode && d->graph_exec && d->graph_call == call_id && d->graph_B == B &&
+                                d->graph_gen == alloc_generation();
+            if (replay || (gmode && d->eager_B.count(B))) {
+                if (!replay) {
+                    destroy_graph(d);
+                    T2X_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeRelaxed));
+                    capturing = true;
+                    int rc = coarse_part();
+                    ev_plan = ev;
+                    if (rc == T2X_OK) rc = post_part();
+                    hipGraph_t gr = nullptr;
+                    const hipError_t ec = hipStreamEndCapture(st, &gr);
+                    capturing = false;
+                    if (rc != T2X_OK || ec != hipSuccess) {
+                        if (gr) (void)hipGraphDestroy(gr);
+                        if (rc == T2X_OK) set_error(std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+                        return rc != T2X_OK ? rc : T2X_EDEVICE;
+                    }
+                    hipGraphExec_t ex = nullptr;
+                    if (hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) != hipSuccess) {
+                        (void)hipGraphDestroy(gr);
+                        set_error("hipGraphInstantiate of the batch graph failed");
+                        return T2X_EDEVICE;
+                    }
+                    d->graph = gr;
+                    d->graph_exec = ex;
+                    d->graph_B = B;
+                    d->graph_gen = alloc_generation();
+                    d->graph_call = call_id;
+                }
+                (void)hipEventRecord(d->ev[14], st);
+                T2X_HIP(hipGraphLaunch(d->graph_exec, st));
+                (void)hipEventRecord(d->ev[15], st);
+                graph_used = true;
+            } else {
+                T2X_TRY(coarse_part());
+                ev_plan = ev;
+                T2X_TRY(post_part());
+                if (gmode) d->eager_B.insert(B);
+            }
+        } else {
+            ev = ev_plan;
+            T2X_TRY(post_part());
+        }
         // ---- per-particle results: the batch's one readback ----
         T2X_HIP(hipMemcpyAsync(&hplan, d_plan, sizeof(BatchPlan), hipMemcpyDeviceToHost, st));
         T2X_HIP(hipMemcpyAsync(pair_off.data(), d_pair_off.ptr, sizeof(long long) * (B + 1), hipMemcpyDeviceToHost, st));
@@ -3129,7 +3218,15 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         }
         {
             float ms[8] = {0};
-            for (int e = 1; e < ev; e++) (void)hipEventElapsedTime(&ms[e - 1], d->ev[e - 1], d->ev[e]);
+            // a graph's stages are timed together (its launch between events 14 and 15)
+            const int ev_timed = graph_used ? 2 : ev;
+            for (int e = 1; e < ev_timed; e++) (void)hipEventElapsedTime(&ms[e - 1], d->ev[e - 1], d->ev[e]);
+            if (graph_used) {
+                float mg = 0.f;
+                (void)hipEventElapsedTime(&mg, d->ev[14], d->ev[15]);
+                stt.ms_graph += mg;
+                stt.n_graph_launches += 1;
+            }
             stt.ms_preprocess += ms[0];
             stt.ms_coarse_diff2 += ms[1] + ms[2];
             stt.ms_coarse_gemm += ms[2];

@@ -7,7 +7,9 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <atomic>
 #include <map>
+#include <set>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -36,6 +38,11 @@ void set_error(const std::string& msg);
     } while (0)
 
 // A device allocation that grows on demand and is reused across calls.
+// Counts every (re)allocation and release of a DevBuf: a captured graph holds raw device
+// pointers, so it is replayed only while this count is unchanged.
+inline std::atomic<unsigned long long> g_alloc_gen{0};
+inline unsigned long long alloc_generation() { return g_alloc_gen.load(); }
+
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
@@ -46,6 +53,7 @@ struct DevBuf {
     int ensure(size_t need)
     {
         if (need <= bytes) return T2X_OK;
+        g_alloc_gen++;
         if (ptr) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
@@ -60,6 +68,7 @@ struct DevBuf {
     template <class T> T* as() const { return reinterpret_cast<T*>(ptr); }
     void release()
     {
+        if (ptr) g_alloc_gen++;
         if (ptr) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
@@ -124,6 +133,9 @@ struct Stats {
 
 namespace cnn { struct Net; }
 
+struct Device;
+void destroy_graph(Device* d);
+
 struct Device {
     int id = 0;
     hipStream_t stream = nullptr;
@@ -150,6 +162,14 @@ struct Device {
     DevBuf plan;
     long long plan_cap = 0;
     int n_cu = 0;
+    // launch_graph: the batch graph captured in call graph_call for batches of graph_B
+    // particles (valid while alloc_generation() == graph_gen), and the batch sizes the current
+    // call has run eagerly
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    int graph_B = 0;
+    unsigned long long graph_gen = 0, graph_call = 0, call_counter = 0;
+    std::set<int> eager_B;
 
     // cached coarse projections
     std::string coarse_key;

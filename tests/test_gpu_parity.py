@@ -165,8 +165,8 @@ def test_batch_plan_grows_its_lists(gpu_device, monkeypatch, n_debug):
     """The work lists of a batch are planned on the device; when they are too small for the
     significant pairs (forced here with a capacity of one pair) the plan reports its totals, no
     kernel after it does any work, and the batch runs again with lists of that size: the
-    results equal those of a run planned right the first time (poses, dLL, weighted sums;
-    backprojections up to the order of their FP32 atomic additions)."""
+    results equal those of a run planned right the first time (poses, dLL; weighted sums and
+    backprojections up to the order of their atomic additions)."""
     wl = H.Workload("refine64", n=12, seed_offset=15)
     ref = wl.run_gpu(dev=gpu_device, n_debug=n_debug, fine_cap=16384)
     monkeypatch.setenv("T2X_PLAN_CAP", "1")
@@ -176,7 +176,7 @@ def test_batch_plan_grows_its_lists(gpu_device, monkeypatch, n_debug):
     assert np.array_equal(g["out"]["nr_fine_samples"], ref["out"]["nr_fine_samples"])
     for k, v in ref["wsum"].items():
         if np.abs(v).sum() > 0:
-            assert H.rel_norm(g["wsum"][k], v) < 1e-12, k
+            assert H.rel_norm(g["wsum"][k], v) < 1e-9, k
     assert H.rel_norm(g["bp"][0][0], ref["bp"][0][0]) < 1e-6
     assert H.rel_norm(g["bp"][0][1], ref["bp"][0][1]) < 1e-6
 
