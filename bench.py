@@ -352,7 +352,8 @@ FINE_CAP = 1 << 15   # fine samples recorded per particle in the precision repor
 
 
 def cpu_baseline(name, w, cfg, sz, n_particles=64):
-    """FP64 oracle on a bounded sample of the same workload (64 particles), one particle per
+    """FP64 oracle on a bounded sample of the same workload (64 particles, more when those take
+    under 8 s, for ~15 s of CPU work), one particle per
     OpenMP thread as relion_refine's thread pool does, on every core of this process's CPU
     share (OMP_NUM_THREADS, else the affinity mask). Returns the baseline and the sample with
     the oracle's results (its fine-pass diff2 and weights kept for the precision report)."""
@@ -371,10 +372,21 @@ def cpu_baseline(name, w, cfg, sz, n_particles=64):
     t0 = time.time()
     o = wl.run_oracle(nthreads=threads, n_debug=n_particles, fine_cap=FINE_CAP, coarse_debug=False)
     dt = time.time() - t0
-    return ({"value": n_particles / dt, "unit": "particles/s", "cores": threads, "kind": "port",
+    n_timed = n_particles
+    if dt < 8.0:
+        # a fast workload: a second, larger sample brings the timed CPU work to ~15 s (the first
+        # sample's results stay the precision report's reference)
+        n2 = int(min(16 * n_particles, max(n_particles, n_particles * (15.0 - dt) / max(dt, 1e-3))))
+        wl2 = H.Workload(name, n=n2, seed_offset=199, current_size=cfg.current_size,
+                         coarse_size=cfg.coarse_size, overrides=dict(resolution=w["resolution"]))
+        t1 = time.time()
+        wl2.run_oracle(nthreads=threads, n_debug=0, fine_cap=1, coarse_debug=False)
+        dt += time.time() - t1
+        n_timed += n2
+    return ({"value": n_timed / dt, "unit": "particles/s", "cores": threads, "kind": "port",
              "nproc": os.cpu_count(), "affinity_cpus": affinity, "omp_num_threads": share or None,
              "cpu_model": cpu_model(),
-             "sample": f"{n_particles} particles of the same workload (FP64 oracle, {threads} OpenMP "
+             "sample": f"{n_timed} particles of the same workload (FP64 oracle, {threads} OpenMP "
                        f"threads = this process's CPU share, one particle per thread), {dt:.1f} s"}, wl, o)
 
 
