@@ -238,7 +238,7 @@ static int check_config(const t2x_estep_config* c)
     if (c->coarse_size < 2 || (c->coarse_size & 1) || c->coarse_size > c->current_size) return T2X_EINVAL;
     if (c->ref_dim != 3 && c->ref_dim != 2) return T2X_EINVAL;
     if (c->nr_classes < 1) return T2X_EINVAL;
-    if (c->adaptive_oversampling < 0 || c->adaptive_oversampling > 1) return T2X_EINVAL;
+    if (c->adaptive_oversampling < 0 || c->adaptive_oversampling > 2) return T2X_EINVAL;
     if (c->offset_step <= 0 || c->offset_range < 0 || c->psi_step <= 0) return T2X_EINVAL;
     if (c->nr_groups < 0 || c->nr_optics_groups < 0) return T2X_EINVAL;
     if (c->orientational_prior_mode == T2X_PRIOR_ROTTILT_PSI) {
@@ -1283,7 +1283,7 @@ struct Shared {
     bool clean;
 };
 
-struct FineBlock { double v[64]; };
+struct FineBlock { std::vector<double> v; };   /* the nover fine samples of one coarse sample */
 
 static int estep_particle(Shared& sh, const float* image, const t2x_particle_in& pin,
                           t2x_particle_out& out, t2x_wsum* wsum, t2x_estep_debug* dbg, int ipart)
@@ -1350,7 +1350,7 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
     const int over_r = S.over_rot(AO), over_t = S.over_trans(AO);
     const int nover = over_r * over_t;
     double min_diff2 = 0, sum_weight = 0, sig_weight = 0;
-    long nr_sig_coarse = 0;
+    long nr_sig_coarse = 0, nr_sig_samples = 0;
     std::vector<double> fine_dbg_d2;
     std::vector<long> fine_dbg_h;
 
@@ -1426,7 +1426,7 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
                                 bi = (int)fine_blocks.size();
                                 fine_index[ihidden] = bi;
                                 FineBlock fb;
-                                for (int q = 0; q < 64; q++) fb.v[q] = -999.;
+                                fb.v.assign((size_t)nover, -999.);
                                 fine_blocks.push_back(fb);
                             } else bi = itb->second;
                             fine_blocks[bi].v[ir * over_t + iot] = diff2;
@@ -1519,7 +1519,10 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
         if (ipass == 0) {
             if (nsig == 0) return T2X_ENOSIG;
             nr_sig_coarse = nsig;
-            for (long h = 0; h < Nhid; h++) Msig[h] = (Mw_coarse[h] >= sig_weight) ? 1 : 0;
+            for (long h = 0; h < Nhid; h++) {
+                Msig[h] = (Mw_coarse[h] >= sig_weight) ? 1 : 0;
+                nr_sig_samples += Msig[h];
+            }
             if (dbg && ipart < dbg->n_debug) {
                 if (dbg->coarse_significant)
                     for (long h = 0; h < Nhid; h++) dbg->coarse_significant[(size_t)ipart * Nhid + h] = Msig[h];
@@ -1688,7 +1691,9 @@ static int estep_particle(Shared& sh, const float* image, const t2x_particle_in&
     out.min_diff2 = min_diff2;
     out.significant_weight = sig_weight;
     out.nr_significant_coarse = nr_sig_coarse;
-    out.nr_fine_samples = (AO > 0) ? (long)fine_dbg_d2.size() : 0;
+    /* samples whose weights the sums below read: the fine pass's, or with one pass the
+     * significant coarse samples */
+    out.nr_fine_samples = (AO > 0) ? (long)fine_dbg_d2.size() : nr_sig_samples;
     out.nr_fine_significant = nfine_sig;
     out.best_hidden = best_hidden;
     if (wsum) {

@@ -325,7 +325,7 @@ __global__ __launch_bounds__(128) void k_prior_t(const PartScalars* __restrict__
                                                  const double* __restrict__ cls_prior, int K,
                                                  double* __restrict__ prior_t)
 {
-    __shared__ double pt[128];
+    __shared__ double pt[MAX_COARSE_T];
     __shared__ double s_mean;
     const int b = blockIdx.x;
     if (b >= B) return;
@@ -496,7 +496,7 @@ __device__ __forceinline__ void fine_matrix(const FineGeom& g, int n, int ir, do
 #define FT_ROWS 16
 #define FT_COLS 64
 #define FT_KC 256
-#define FT_MAX_T 256
+#define FT_MAX_T MAX_COARSE_T
 
 struct FineTileParams {
     const long long* tile_off;          // [B+1] first tile of each particle
@@ -576,13 +576,13 @@ __device__ __forceinline__ void fine_tile(const FineTileParams& P, const long lo
 
     // the particle's significant coarse translations: rank of each, and the translation of each rank
     const unsigned* tm = P.tmask + (size_t)b * P.TW;
-    if (tid < g.T) {
-        const int w = tid >> 5, bit = tid & 31;
+    for (int t = tid; t < g.T; t += 256) {
+        const int w = t >> 5, bit = t & 31;
         int r = __popc(tm[w] & ((1u << bit) - 1u));
         for (int q = 0; q < w; q++) r += __popc(tm[q]);
         const bool set = (tm[w] >> bit) & 1u;
-        s_rank[tid] = set ? r : -1;
-        if (set) s_slot_t[r] = tid;
+        s_rank[t] = set ? r : -1;
+        if (set) s_slot_t[r] = t;
     }
     if (tid == 0) {
         int nt = 0;
@@ -1170,7 +1170,7 @@ struct StoreParams {
     LocalPrior lp;             // local searches (lp.dir null: global)
 };
 
-#define STORE_MAX_T 1024
+#define STORE_MAX_T 2048
 
 __device__ __forceinline__ void store_item(const long long* __restrict__ item_off, int B,
                                            const long long* __restrict__ orient_off,
@@ -1454,6 +1454,27 @@ static int zero_async(void* p, size_t bytes, hipStream_t st)
     k_zero_words<<<grid, 256, 0, st>>>(reinterpret_cast<unsigned*>(p), n);
     T2X_HIP(hipGetLastError());
     return T2X_OK;
+}
+
+// One pass (adaptive_oversampling 0): the significant pairs' diff2 straight from the coarse
+// matrix (rows (particle, translation), columns orientations), and the coarse pass's weight
+// normalisation as the stored one.
+__global__ void k_one_pass_d2(const long long* __restrict__ pair_off, const int2* __restrict__ pairs, int B,
+                              const float* __restrict__ diff2, long long ld, int T, float* __restrict__ d2_out)
+{
+    const int b = blockIdx.x;
+    if (b >= B) return;
+    const long long p0 = pair_off[b], p1 = pair_off[b + 1];
+    for (long long k = p0 + threadIdx.x; k < p1; k += blockDim.x) {
+        const int2 nt = pairs[k];
+        d2_out[k] = diff2[((long long)b * T + nt.y) * ld + nt.x];
+    }
+}
+
+__global__ void k_copy_selresult(const SelResult* __restrict__ src, int B, SelResult* __restrict__ dst)
+{
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b < B) dst[b] = src[b];
 }
 
 // C_coarse of each particle's scalars into a dense array (the GEMM epilogue's C_b)
@@ -1885,14 +1906,18 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     T2X_TRY(build_sampling(cfg, S));
     const int K = cfg->nr_classes;
     const int N = cfg->ori_size, xN = N / 2 + 1;
-    const int cs = cfg->current_size, cc = cfg->coarse_size, xc = cs / 2 + 1;
+    // adaptive_oversampling 0: one pass over the coarse sampling at the current size, whose
+    // weights (normalised over every sample) are stored directly (ml_optimiser.cpp:4373-4420,
+    // nr_sampling_passes = 1)
+    const bool one_pass = cfg->adaptive_oversampling == 0;
+    const int cs = cfg->current_size, cc = one_pass ? cs : cfg->coarse_size, xc = cs / 2 + 1;
     const int Pw = cs * xc;
     const long Nd = S.nr_dir, Npsi = S.nr_psi, T = S.nr_trans;
     const long Norient = Nd * Npsi;
     const long O_all = K * Norient;
     const int over_rot = S.over_rot, over_trans = S.over_trans, nover = over_rot * over_trans;
     hipStream_t st = d->stream;
-    if (T > 128) { set_error("more than 128 coarse translations"); return T2X_EINVAL; }
+    if (T > MAX_COARSE_T) { set_error("more than 512 coarse translations"); return T2X_EINVAL; }
     if (!(cfg->gemm_tile == 0 || cfg->gemm_tile == 128 || cfg->gemm_tile == 256 || cfg->gemm_tile == 257 ||
           cfg->gemm_tile == 258) || cfg->gemm_group < 0) {
         set_error("gemm_tile must be 0 (autotuned), 128, 256, 257 or 258 and gemm_group >= 0");
@@ -1902,6 +1927,10 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     // one order down = blocks of 4 directions x Npsi orientations) of each particle
     const bool use_prop = cfg->proposal_topk > 0;
     const bool local = cfg->orientational_prior_mode == T2X_PRIOR_ROTTILT_PSI;
+    if (local && one_pass) {
+        set_error("local searches need adaptive_oversampling >= 1 (two passes)");
+        return T2X_EINVAL;
+    }
     const int n_bins = (int)(K * (Nd / 4)), bin_cols = (int)(4 * Npsi);
     int topk = 0;
     if (use_prop) {
@@ -2056,7 +2085,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         }
         T2X_HIP(hipGetLastError());
     }
-    if (T > FT_MAX_T) { set_error("more than 256 coarse translations"); return T2X_EINVAL; }
+    if (T > FT_MAX_T) { set_error("more than 512 coarse translations"); return T2X_EINVAL; }
     if ((long long)T * over_trans * ldk_fine >= 0x7fffffffLL) {
         set_error("fine pass: phase table beyond 2^31 elements");
         return T2X_EINVAL;
@@ -2982,7 +3011,11 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         g.pf = pf;
         g.ref2d = ref2d;
         mark();   // 5: fine start
-        {
+        if (one_pass) {   // no second pass: the significant pairs keep their coarse diff2
+            k_one_pass_d2<<<B, 256, 0, st>>>(d_pair_off.as<long long>(), d->sig_pairs.as<int2>(), B,
+                                             d->diff2.as<float>(), O_pad, (int)T, d->fine_diff2.as<float>());
+            T2X_HIP(hipGetLastError());
+        } else {
             // tiles of 16 items x 64 columns per particle (items: its significant orientations x
             // oversampled rotations; columns: its significant coarse translations x oversampled
             // ones), planned by k_batch_plan, taken from a work queue by a persistent grid
@@ -3037,9 +3070,13 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         wf.pair_off = d_pair_off.as<long long>();
         wf.pairs = d->sig_pairs.as<int2>();
         wf.fine_min = d->fine_min.as<unsigned int>();
-        k_pass_min<<<B, 1024, 0, st>>>(wf, B, d_self);
-        SelParams spf{cfg->adaptive_fraction, (long long)cfg->maximum_significants, 0};
-        k_select<<<B, 1024, 0, st>>>(wf, spf, B, d_self, nullptr);
+        if (one_pass) {   // the coarse pass's minimum, sum and threshold over every sample
+            k_copy_selresult<<<(B + 255) / 256, 256, 0, st>>>(d_selc, B, d_self);
+        } else {
+            k_pass_min<<<B, 1024, 0, st>>>(wf, B, d_self);
+            SelParams spf{cfg->adaptive_fraction, (long long)cfg->maximum_significants, 0};
+            k_select<<<B, 1024, 0, st>>>(wf, spf, B, d_self, nullptr);
+        }
         T2X_TRY(d->fine_sel.ensure(sizeof(long long) * B));
         k_fine_argmax<<<B, 1024, 0, st>>>(wf, B, d_self, d->fine_sel.as<long long>());
         T2X_HIP(hipGetLastError());

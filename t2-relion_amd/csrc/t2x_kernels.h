@@ -40,6 +40,10 @@ struct SelResult {
 
 enum { SRC_COARSE = 0, SRC_FINE = 1 };
 
+// Most coarse translations of a pass (per-translation LDS tables and significance masks): e.g.
+// an offset range of 12 px at a step of 1 px (453 translations).
+#define MAX_COARSE_T 512
+
 // Totals of a batch's work lists (k_batch_plan), the first particle with an error status, and
 // the work-queue counters of the persistent fine-pass and store kernels.
 struct BatchPlan {

@@ -99,8 +99,8 @@ int check_config(const t2x_estep_config* c)
     }
     if (c->ref_dim != 3 && c->ref_dim != 2) { set_error("ref_dim must be 3 or 2"); return T2X_EINVAL; }
     if (c->nr_classes < 1) { set_error("nr_classes < 1"); return T2X_EINVAL; }
-    if (c->adaptive_oversampling != 1) {
-        set_error("adaptive_oversampling must be 1 (two-pass E-step)");
+    if (c->adaptive_oversampling < 0 || c->adaptive_oversampling > 2) {
+        set_error("adaptive_oversampling must be 0 (one pass), 1 or 2");
         return T2X_EINVAL;
     }
     if (c->offset_step <= 0 || c->offset_range < 0 || c->psi_step <= 0) {

@@ -449,7 +449,7 @@ __global__ __launch_bounds__(1024) void k_sig_write(WeightSrc s, int B, const Se
     __syncthreads();
     for (int base = 0; base < ncol; base += blockDim.x) {
         const int col = base + threadIdx.x;
-        unsigned long long mask[2] = {0ull, 0ull};   // up to 128 translations
+        unsigned long long mask[MAX_COARSE_T / 64] = {};   // bit t: translation t is significant
         int c = 0, n = col;
         if (col < ncol) {
             if (plist) n = s.pairs[s.pair_off[b] + (long long)col * s.T].x;

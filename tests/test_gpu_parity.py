@@ -181,6 +181,53 @@ def test_batch_plan_grows_its_lists(gpu_device, monkeypatch, n_debug):
     assert H.rel_norm(g["bp"][0][1], ref["bp"][0][1]) < 1e-6
 
 
+def compare_all(wl, o, g, tol_d2=1e-5):
+    """The parity bars of this file: coarse diff2, significant sets, poses, fine-sample counts,
+    dLL, weighted sums and both backprojected volumes."""
+    od, gd = o["dbg"], g["dbg"]
+    valid = od["coarse_diff2"] > -998
+    assert np.array_equal(valid, gd["coarse_diff2"] > -998)
+    assert H.rel(gd["coarse_diff2"][valid], od["coarse_diff2"][valid]) < tol_d2
+    assert (od["coarse_significant"] == gd["coarse_significant"]).all(axis=1).mean() >= 0.99
+    assert np.array_equal(o["out"]["best_hidden"], g["out"]["best_hidden"])
+    assert np.array_equal(o["out"]["nr_fine_samples"], g["out"]["nr_fine_samples"])
+    assert H.rel(g["out"]["dLL"], o["out"]["dLL"]) < 1e-5
+    for k, v in o["wsum"].items():
+        if np.abs(v).sum() > 0:
+            assert H.rel_norm(g["wsum"][k], v) < 1e-4, k
+    for c in range(wl.cfg.nr_classes):
+        if np.abs(o["bp"][c][1]).sum() > 0:
+            assert H.rel_norm(g["bp"][c][0], o["bp"][c][0]) < 1e-4, c
+            assert H.rel_norm(g["bp"][c][1], o["bp"][c][1]) < 1e-4, c
+
+
+@pytest.mark.parametrize("over", [0, 2])
+def test_adaptive_oversampling(gpu_device, over):
+    """adaptive_oversampling 0: one pass over the coarse sampling at the current size whose
+    weights, normalised over every sample, are stored directly (nr_sampling_passes = 1,
+    ml_optimiser.cpp:4373-4420); 2: a fine pass at HEALPix order + 2 and a quarter of the psi and
+    offset steps (64 rotations x 16 translations per significant coarse sample,
+    healpix_sampling.cpp:1720-1735)."""
+    wl = H.Workload("refine64", n=6, seed_offset=61)
+    wl.cfg.adaptive_oversampling = over
+    o = wl.run_oracle(n_debug=6, fine_cap=1 << 16)
+    g = wl.run_gpu(dev=gpu_device, n_debug=6, fine_cap=1 << 16)
+    nf = g["out"]["nr_fine_samples"]
+    print(f"\nadaptive_oversampling {over}: fine samples per particle {nf.min()}..{nf.max()}")
+    assert np.all(nf % (1 if over == 0 else 1024) == 0)
+    compare_all(wl, o, g)
+
+
+def test_wide_offset_range(gpu_device):
+    """More than 256 coarse translations (offset range 12 px at 1 px: 441 within the circle)."""
+    wl = H.Workload("refine64", n=4, seed_offset=62, overrides=dict(offset_range_px=12.0, offset_step_px=1.0))
+    assert wl.info.nr_trans > 256
+    o = wl.run_oracle(n_debug=4, fine_cap=1 << 16)
+    g = wl.run_gpu(dev=gpu_device, n_debug=4, fine_cap=1 << 16)
+    print(f"\n{wl.info.nr_trans} coarse translations")
+    compare_all(wl, o, g)
+
+
 def test_maximum_significants(gpu_device):
     """--maxsig caps the coarse significant samples (ml_optimiser.cpp:7950-7961)."""
     wl = H.Workload("refine64", n=8, seed_offset=12, maximum_significants=3)
