@@ -446,7 +446,8 @@ def main():
                     help="comma list of paths to time (fp32, bf16x3, fp16x3, fp16x2, learned, learned_gated; "
                          "a precision path + \"+graph\" runs its batches as replayed hipGraphs) (default: "
                          "fp32,fp16x3,fp16x2,fp16x2+graph,learned,learned_gated for 3D references, "
-                         "fp32,fp16x3,fp16x2,fp16x2+graph for 2D, fp32,fp16x3,fp16x2 for local searches)")
+                         "fp32,fp16x3,fp16x2,fp16x2+graph for 2D, fp32,bf16x3,fp16x3 for local searches, whose "
+                         "grouped GEMM has no 2-product form)")
     ap.add_argument("--headline", default="", help="path of the headline value (default: the fastest path, "
                     "per-iteration costs included, that meets the pose-agreement and FSC bars)")
     ap.add_argument("--resolution", type=float, default=0.0,
@@ -513,7 +514,7 @@ def main():
         T.check(lib.t2x_device_free(dev, dst))
         log("copy calibration: 1 GiB device-to-device copy (t2x::k_copy16) done")
     local = cfg.orientational_prior_mode == T.T2X_PRIOR_ROTTILT_PSI
-    paths = (args.precision or ("fp32,fp16x3,fp16x2" if local else
+    paths = (args.precision or ("fp32,bf16x3,fp16x3" if local else
                                 "fp32,fp16x3,fp16x2,fp16x2+graph" if cfg.ref_dim == 2 else
                                 "fp32,fp16x3,fp16x2,fp16x2+graph,learned,learned_gated")).split(",")
     train = None
