@@ -115,7 +115,8 @@ typedef struct t2x_estep_config {
     int    proposal_gate;          /* T2X_GATE_*: what proposal_min_conf is compared with */
     double proposal_min_conf;      /* gate value below this: exhaustive coarse search    */
     int    gemm_tile;              /* split-precision GEMM variant: 0 autotuned (fastest  */
-                                   /* on first use per shape), 128, 256, 257 or 258      */
+                                   /* on first use per shape), 128, 256, 257, 258, or    */
+                                   /* 259 (FP16X2 only: 128x256 tiles, 2 per CU)         */
     int    gemm_group;             /* its tile-walk group in tile rows (0: default)      */
     /* local searches: sqrt(mymodel.sigma2_rot / sigma2_tilt / sigma2_psi) in degrees;
      * sigma_rot and sigma_tilt both > 0 (a Gaussian on the angle between the directions) or

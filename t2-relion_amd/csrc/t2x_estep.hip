@@ -1919,8 +1919,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     hipStream_t st = d->stream;
     if (T > MAX_COARSE_T) { set_error("more than 512 coarse translations"); return T2X_EINVAL; }
     if (!(cfg->gemm_tile == 0 || cfg->gemm_tile == 128 || cfg->gemm_tile == 256 || cfg->gemm_tile == 257 ||
-          cfg->gemm_tile == 258) || cfg->gemm_group < 0) {
-        set_error("gemm_tile must be 0 (autotuned), 128, 256, 257 or 258 and gemm_group >= 0");
+          cfg->gemm_tile == 258 || cfg->gemm_tile == 259) || cfg->gemm_group < 0) {
+        set_error("gemm_tile must be 0 (autotuned), 128, 256, 257, 258 or 259 and gemm_group >= 0");
         return T2X_EINVAL;
     }
     // learned proposal: the coarse pass scores only the top-k bins (classes x HEALPix directions

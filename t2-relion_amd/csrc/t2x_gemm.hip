@@ -650,8 +650,11 @@ __device__ __forceinline__ float min_row16(float v)
 // tcol0..); same arithmetic, same outputs. The running minimum of each of the wave's particles
 // is read once, and lowered by the wave's own block minima as it goes (any value taken by the
 // running minimum bounds the final minimum from above, which is all the store test needs).
-template <int EPI>
-__device__ __forceinline__ void gemm_epilogue_staged(const floatx4 (&acc)[8][4], const EpiStage& es, int trow0,
+// C_b + N1 of slot z at tile column c, in FP64 (the order gemm_epilogue adds them in)
+__device__ __forceinline__ double es_base(const EpiStage& es, int z, int c) { return es.cbn1[z][c]; }
+
+template <int EPI, int MI, class ES>
+__device__ __forceinline__ void gemm_epilogue_staged(const floatx4 (&acc)[MI][4], const ES& es, int trow0,
                                                      int tcol0, long long n0, int lane, const EpiParams& ep)
 {
     const int col_in = lane & 15, row_grp = (lane >> 4) * 4;
@@ -682,7 +685,7 @@ __device__ __forceinline__ void gemm_epilogue_staged(const floatx4 (&acc)[8][4],
     // branch-light: padding rows (output row -1, uniform over the 16 lanes of a row) are
     // masked, the running minima are selected and updated with conditional moves
 #pragma unroll
-    for (int i = 0; i < 8; i++) {
+    for (int i = 0; i < MI; i++) {
 #pragma unroll
         for (int r = 0; r < 4; r++) {
             const int4 e = es.row[trow0 + i * 16 + row_grp + r];
@@ -694,12 +697,11 @@ __device__ __forceinline__ void gemm_epilogue_staged(const floatx4 (&acc)[8][4],
                 for (int j = 0; j < 4; j++)
                     if (rv && nval[j]) crow[ncol[j]] = acc[i][j][r] * sa * sb[j];
             } else {
-                const double* cbn1 = es.cbn1[e.z];
                 float rmin = 3.0e38f, dv[4];
 #pragma unroll
                 for (int j = 0; j < 4; j++) {
                     const float v = acc[i][j][r] * sa * sb[j];
-                    dv[j] = (float)(cbn1[tcol0 + j * 16 + col_in] - (double)v);
+                    dv[j] = (float)(es_base(es, e.z, tcol0 + j * 16 + col_in) - (double)v);
                     rmin = (nval[j] && scored[j]) ? fminf(rmin, dv[j]) : rmin;
                 }
                 rmin = min_row16(rmin);
@@ -868,6 +870,194 @@ __global__ __launch_bounds__(512) void k_gemm_x3p(SplitOperand A, SplitOperand B
         gemm_epilogue_staged<EPI>(acc, es, wm * 128, wn * 64, n0, lane, ep);
     else
         gemm_epilogue<8, EPI>(acc, m0 + wm * 128, n0 + wn * 64, lane, A.inv_scale, B.inv_scale, ep, rowmap);
+}
+
+// Two-product half-height variant (FP16X2 only): tile 128 (M) x 256 (N), 4 waves of 64 x 128,
+// 2 stages of 32 KB (A hi, A lo: 128 rows; B hi: 256 rows) + a 9 KB epilogue stage, so two
+// workgroups share a compute unit: one workgroup's K-step barriers and epilogue overlap the
+// other's MFMAs, where the 256x256 kernel's single workgroup stalls all 8 waves at each barrier.
+// Per K step a wave reads 8 A and 8 B fragments for 64 MFMAs (k_gemm_x3p<NP = 2>: 16 + 4 for 64).
+// Each element sees the MFMA chain of k_gemm_x3p<f16, EPI, 2> (lo.hi, hi.hi per K step, K steps
+// in order) and the same FP64 epilogue: the two give identical results.
+struct EpiStageH {
+    int4 row[128];
+    float sb[256];
+    int scored[256];
+    int pslot[EPI_SLOTS];
+    double cb[EPI_SLOTS];
+    float n1[EPI_SLOTS][256];
+};
+__device__ __forceinline__ double es_base(const EpiStageH& es, int z, int c) { return es.cb[z] + (double)es.n1[z][c]; }
+
+// stage_epilogue for a 128-row tile with 256 threads; false (uniformly) beyond EPI_SLOTS particles
+// per tile (T < 19)
+__device__ __forceinline__ bool stage_epilogue_h(EpiStageH& es, long long m0l, long long n0, const float* a_scale,
+                                                 const float* b_scale, const EpiParams& ep, const int* rowmap, int tid)
+{
+    const unsigned T = (unsigned)ep.T;
+    const unsigned m0 = (unsigned)m0l;
+    const unsigned p0 = m0 / T;
+    const int mlast = min((int)m0 + 127, ep.M - 1);
+    const int nslots = (mlast >= (int)m0) ? (int)((unsigned)mlast / T - p0) + 1 : 0;
+    if (nslots > EPI_SLOTS) return false;
+    if (tid < 128) {
+        const unsigned m = m0 + tid;
+        int4 e = make_int4(-1, 0, 0, __float_as_int(1.f));
+        if ((int)m < ep.M) {
+            const unsigned mm = rowmap ? (unsigned)rowmap[m] : m;
+            e.x = (int)mm;
+            e.y = (int)(mm / T);
+            e.z = (int)(m / T - p0);
+            if (a_scale) e.w = __float_as_int(a_scale[mm]);
+        }
+        es.row[tid] = e;
+    }
+    {
+        const int c = tid;
+        const long long n = n0 + c;
+        const bool nv = n < ep.N;
+        es.sb[c] = (b_scale && nv) ? b_scale[n] : 1.f;
+        es.scored[c] = nv && ep.prior_o[n] > 0.;
+    }
+    for (int q = tid; q < nslots * 256; q += 256) {
+        const int sl = q >> 8, c = q & 255;
+        const unsigned ms = max(m0, (p0 + sl) * T);   // a valid row of slot sl
+        const unsigned p = rowmap ? (unsigned)rowmap[ms] / T : p0 + sl;
+        const long long n = n0 + c;
+        es.n1[sl][c] = n < ep.N ? ep.N1[(long long)p * ep.ldn + n] : 0.f;
+    }
+    if (tid < EPI_SLOTS) {
+        int p = -1;
+        if (tid < nslots) {
+            const unsigned ms = max(m0, (p0 + tid) * T);
+            p = rowmap ? (int)((unsigned)rowmap[ms] / T) : (int)(p0 + tid);
+        }
+        es.pslot[tid] = p;
+        es.cb[tid] = p >= 0 ? ep.Cb[p] : 0.0;
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(256, 2) void k_gemm_x2h(SplitOperand A, SplitOperand B, int K, int tiles_m, int tiles_n,
+                                                     int group_m, EpiParams ep, const int* __restrict__ rowmap)
+{
+    typedef _Float16 H;
+    typedef Vec8<H>::type v8;
+    __shared__ __attribute__((aligned(16))) H sa0[2][128 * 32];   // stage 0: A hi, A lo
+    __shared__ __attribute__((aligned(16))) H sb0[256 * 32];      // stage 0: B hi
+    __shared__ __attribute__((aligned(16))) H sa1[2][128 * 32];
+    __shared__ __attribute__((aligned(16))) H sb1[256 * 32];
+    __shared__ EpiStageH es;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int wm = wave & 1, wn = wave >> 1;
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, group_m, tm, tn);
+    const long long m0 = (long long)tm * 128, n0 = (long long)tn * 256;
+
+    // acc[h][i][j]: rows 16 i of the wave's 64, columns 64 h + 16 j of its 128
+    floatx4 acc[2][4][4];
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) acc[h][i][j] = floatx4{0.f, 0.f, 0.f, 0.f};
+
+    // loader: wave w moves A row blocks [32w, 32w+16), [32w+16, 32w+32) of hi and lo, and B row
+    // blocks [64w + 16q, 64w + 16q + 16), q < 4; lane l fetches logical chunk (l & 3) ^ ((row >> 1) & 3)
+    // of row l >> 2 into physical chunk l & 3 (the swizzle of swz())
+    const int lr = lane >> 2;
+    const int lch = (lane & 3) ^ ((lr >> 1) & 3);
+    const int ra0 = wave * 32, ra1 = wave * 32 + 16, rb = wave * 64;
+    const long long ga0 = m0 + ra0 + lr, ga1 = m0 + ra1 + lr;
+    const long long am0 = rowmap ? (ga0 < ep.M ? rowmap[ga0] : 0) : ga0;
+    const long long am1 = rowmap ? (ga1 < ep.M ? rowmap[ga1] : 0) : ga1;
+    const char* Ah = reinterpret_cast<const char*>(A.hi);
+    const char* Al = reinterpret_cast<const char*>(A.lo);
+    const char* Bh = reinterpret_cast<const char*>(B.hi);
+    const unsigned oa0 = (unsigned)((am0 * A.ld + lch * 8) * (long long)sizeof(H));
+    const unsigned oa1 = (unsigned)((am1 * A.ld + lch * 8) * (long long)sizeof(H));
+    const unsigned obq = (unsigned)(((n0 + rb + lr) * B.ld + lch * 8) * (long long)sizeof(H));
+    const unsigned ob16 = (unsigned)(16 * B.ld * (long long)sizeof(H));   // 16 rows of B
+#define X2H_LOAD(SA, SB, k)                                                                  \
+    do {                                                                                     \
+        const long long kb = (long long)(k) * (long long)sizeof(H);                          \
+        lds_dma16<H>(reinterpret_cast<const H*>(Ah + kb + oa0), &SA[0][ra0 * 32]);          \
+        lds_dma16<H>(reinterpret_cast<const H*>(Ah + kb + oa1), &SA[0][ra1 * 32]);          \
+        lds_dma16<H>(reinterpret_cast<const H*>(Al + kb + oa0), &SA[1][ra0 * 32]);          \
+        lds_dma16<H>(reinterpret_cast<const H*>(Al + kb + oa1), &SA[1][ra1 * 32]);          \
+        _Pragma("unroll") for (int q = 0; q < 4; q++)                                        \
+            lds_dma16<H>(reinterpret_cast<const H*>(Bh + kb + obq + q * ob16), &SB[(rb + 16 * q) * 32]); \
+    } while (0)
+
+    const int fr = lane & 15, fc = lane >> 4;
+    const int oa = swz(wm * 64 + fr, fc), ob = swz(wn * 128 + fr, fc);
+#define X2H_READ_A(SA, i, AH, AL)                                   \
+    do {                                                            \
+        AH = *reinterpret_cast<const v8*>(&SA[0][oa + (i) * 512]);  \
+        AL = *reinterpret_cast<const v8*>(&SA[1][oa + (i) * 512]);  \
+    } while (0)
+    // one A block against the 8 B fragments: lo.hi then hi.hi per element
+#define X2H_GROUP(i, AL, AH)                                                                        \
+    do {                                                                                            \
+        _Pragma("unroll") for (int j = 0; j < 8; j++)                                               \
+            acc[j >> 2][i][j & 3] = mfma16x16x32<H>(AL, bh[j], acc[j >> 2][i][j & 3]);              \
+        _Pragma("unroll") for (int j = 0; j < 8; j++)                                               \
+            acc[j >> 2][i][j & 3] = mfma16x16x32<H>(AH, bh[j], acc[j >> 2][i][j & 3]);              \
+    } while (0)
+    // One K step on stage (CA, CB) (B fragments and A block 0 in registers): A blocks 1..3 read one
+    // group ahead, the barrier before the last group; after it stage s+2 loads into (CA, CB) and
+    // the next stage's first fragments are read while group 3 runs, B fragment j after column j.
+#define X2H_STEP(CA, CB, NA, NB, s)                                                \
+    do {                                                                           \
+        v8 nh, nl, a3h, a3l;                                                       \
+        X2H_READ_A(CA, 1, nh, nl);                                                 \
+        X2H_GROUP(0, al, ah);                                                      \
+        ah = nh; al = nl;                                                          \
+        X2H_READ_A(CA, 2, nh, nl);                                                 \
+        X2H_GROUP(1, al, ah);                                                      \
+        ah = nh; al = nl;                                                          \
+        X2H_READ_A(CA, 3, a3h, a3l);                                               \
+        X2H_GROUP(2, al, ah);                                                      \
+        __builtin_amdgcn_s_waitcnt(0);                                             \
+        __syncthreads();                                                           \
+        if ((s) + 2 < nk) X2H_LOAD(CA, CB, ((s) + 2) * XK);                        \
+        const bool nxt = (s) + 1 < nk;                                             \
+        if (nxt) X2H_READ_A(NA, 0, ah, al);                                        \
+        _Pragma("unroll") for (int j = 0; j < 8; j++) {                            \
+            acc[j >> 2][3][j & 3] = mfma16x16x32<H>(a3l, bh[j], acc[j >> 2][3][j & 3]); \
+            acc[j >> 2][3][j & 3] = mfma16x16x32<H>(a3h, bh[j], acc[j >> 2][3][j & 3]); \
+            if (nxt) bh[j] = *reinterpret_cast<const v8*>(&NB[ob + j * 512]);      \
+        }                                                                          \
+    } while (0)
+
+    const int nk = K / XK;
+    X2H_LOAD(sa0, sb0, 0);
+    const bool staged = stage_epilogue_h(es, m0, n0, A.inv_scale, B.inv_scale, ep, rowmap, tid);
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    if (nk > 1) X2H_LOAD(sa1, sb1, XK);
+    v8 bh[8], ah, al;
+#pragma unroll
+    for (int j = 0; j < 8; j++) bh[j] = *reinterpret_cast<const v8*>(&sb0[ob + j * 512]);
+    X2H_READ_A(sa0, 0, ah, al);
+    for (int s = 0; s < nk; s += 2) {
+        X2H_STEP(sa0, sb0, sa1, sb1, s);
+        if (s + 1 < nk) X2H_STEP(sa1, sb1, sa0, sb0, s + 1);
+    }
+#undef X2H_STEP
+#undef X2H_GROUP
+#undef X2H_READ_A
+#undef X2H_LOAD
+    if (staged) {
+        gemm_epilogue_staged<EPI_DIFF2>(acc[0], es, wm * 64, wn * 128, n0, lane, ep);
+        gemm_epilogue_staged<EPI_DIFF2>(acc[1], es, wm * 64, wn * 128 + 64, n0, lane, ep);
+    } else {
+        gemm_epilogue<4, EPI_DIFF2>(acc[0], m0 + wm * 64, n0 + wn * 128, lane, A.inv_scale, B.inv_scale, ep, rowmap);
+        gemm_epilogue<4, EPI_DIFF2>(acc[1], m0 + wm * 64, n0 + wn * 128 + 64, lane, A.inv_scale, B.inv_scale, ep,
+                                    rowmap);
+    }
 }
 
 // Learned-proposal variant: one workgroup per work item = (bin, row tile, column tile). The
@@ -1199,13 +1389,13 @@ int launch_split(int prec, const float* X, long long ldx, int K, int rows, int v
 // the same MFMA chain and the same epilogue arithmetic): tile 128 (k_gemm_x3), 256 (k_gemm_x3w,
 // 256x256), 257 (k_gemm_x3d, 256x128 with direct-to-LDS loads) or 258 (k_gemm_x3p, 256x256
 // pipelined), each with a tile-walk group of `group` tile rows.
-static bool valid_tile(int t) { return t == 128 || t == 256 || t == 257 || t == 258; }
+static bool valid_tile(int t) { return t == 128 || t == 256 || t == 257 || t == 258 || t == 259; }
 
 static int launch_x3_variant(int prec, int epi, const SplitOperand& A, const SplitOperand& B, int K, int Mrows_pad,
                              int Nrows_pad, const EpiParams& ep, hipStream_t st, int tile, int g, const int* rowmap)
 {
-    const int tile_m = tile == 257 ? DBM : (tile == 258 ? 256 : tile);
-    const int tile_n = tile == 257 ? DBN : (tile == 258 ? 256 : tile);
+    const int tile_m = tile == 257 ? DBM : (tile == 258 ? 256 : (tile == 259 ? 128 : tile));
+    const int tile_n = tile == 257 ? DBN : (tile == 258 || tile == 259 ? 256 : tile);
     if (K % XK != 0 || Mrows_pad % tile_m != 0 || Nrows_pad % tile_n != 0) {
         set_error("gemm_x3: operands not padded to tile multiples");
         return T2X_EINVAL;
@@ -1215,7 +1405,13 @@ static int launch_x3_variant(int prec, int epi, const SplitOperand& A, const Spl
     if (nwg > 0x7fffffffLL) { set_error("gemm: grid too large"); return T2X_EINVAL; }
     const bool bf = prec == T2X_PREC_BF16X3, store = epi == EPI_STORE;
     const unsigned grid = (unsigned)nwg;
-    if (tile == 257) {
+    if (tile == 259) {   // two-product diff2 kernel (FP16X2) of 128x256 tiles
+        if (prec != T2X_PREC_FP16X2 || store) {
+            set_error("gemm_x3: tile 259 serves the FP16X2 diff2 GEMM only");
+            return T2X_EINVAL;
+        }
+        k_gemm_x2h<<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep, rowmap);
+    } else if (tile == 257) {
         if (bf) { if (store) k_gemm_x3d<__bf16, EPI_STORE><<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep);
                   else k_gemm_x3d<__bf16, EPI_DIFF2><<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep); }
         else    { if (store) k_gemm_x3d<_Float16, EPI_STORE><<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep);
@@ -1266,15 +1462,18 @@ static int rows_log2(long long m) { return m > 0 ? (int)std::lround(std::log2((d
 static std::mutex g_tune_mtx;
 static std::map<TuneKey, GemmChoice> g_tune;
 
-static const GemmChoice kCandidates[] = {{258, 32}, {258, 16}, {258, 64}, {256, 32}, {257, 32}, {128, 64}};
+static const GemmChoice kCandidates[] = {{258, 32}, {258, 16}, {258, 64}, {259, 32}, {259, 16}, {259, 64},
+                                         {256, 32}, {257, 32}, {128, 64}};
 
-static int tile_rows(int tile) { return tile == 257 ? DBM : (tile == 258 ? 256 : tile); }
-static int tile_cols(int tile) { return tile == 257 ? DBN : (tile == 258 ? 256 : tile); }
+static int tile_rows(int tile) { return tile == 257 ? DBM : (tile == 258 ? 256 : (tile == 259 ? 128 : tile)); }
+static int tile_cols(int tile) { return tile == 257 ? DBN : (tile == 258 || tile == 259 ? 256 : tile); }
 
-static bool eligible(const GemmChoice& c, bool mapped, bool off32, int prec = 0)
+static bool eligible(const GemmChoice& c, bool mapped, bool off32, int prec = 0, int epi = EPI_DIFF2)
 {
-    if (prec == T2X_PREC_FP16X2 && c.tile != 258) return false;     // 2 products: the pipelined kernel only
-    if (mapped && c.tile != 258 && c.tile != 256) return false;   // row maps: 256x256 kernels only
+    if (c.tile == 259 && (prec != T2X_PREC_FP16X2 || epi != EPI_DIFF2)) return false;   // FP16X2 diff2 only
+    if (prec == T2X_PREC_FP16X2 && c.tile != 258 && c.tile != 259) return false;       // 2-product kernels
+    if (c.tile == 259 && !off32) return false;                                         // 32-bit operand offsets
+    if (mapped && c.tile != 258 && c.tile != 256 && c.tile != 259) return false;   // row maps: pipelined / 256 kernels
     if (c.tile == 258 && !off32) return false;                      // 32-bit operand offsets
     return true;
 }
@@ -1300,7 +1499,7 @@ int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const floa
     }
     if (c.tile != 0) {
         if (!valid_tile(c.tile)) { set_error("gemm_x3: unknown tile variant"); return T2X_EINVAL; }
-        if (!eligible(c, mapped, off32, prec)) c.tile = prec == T2X_PREC_FP16X2 ? 258 : 256;
+        if (!eligible(c, mapped, off32, prec, epi)) c.tile = prec == T2X_PREC_FP16X2 ? 258 : 256;
         if (c.group <= 0) c.group = c.tile == 128 ? 64 : 32;
         if (used) *used = c;
         return launch_x3_variant(prec, epi, A, B, K, Mrows_pad, Nrows_pad, ep, st, c.tile, c.group, rowmap);
@@ -1316,7 +1515,7 @@ int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const floa
             if (it != g_tune.end()) hit = it->second;
         }
         // a cached choice is re-checked against this launch (offsets, tile multiples) before use
-        if (hit.tile != 0 && eligible(hit, mapped, off32, prec) && Mrows_pad % tile_rows(hit.tile) == 0 &&
+        if (hit.tile != 0 && eligible(hit, mapped, off32, prec, epi) && Mrows_pad % tile_rows(hit.tile) == 0 &&
             Nrows_pad % tile_cols(hit.tile) == 0) {
             if (used) *used = hit;
             return launch_x3_variant(prec, epi, A, B, K, Mrows_pad, Nrows_pad, ep, st, hit.tile, hit.group, rowmap);
@@ -1331,7 +1530,7 @@ int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const floa
                                32, rowmap);
     for (const GemmChoice& cand : kCandidates) {
         if (rc != T2X_OK) break;
-        if (!eligible(cand, mapped, off32, prec)) continue;
+        if (!eligible(cand, mapped, off32, prec, epi)) continue;
         if (Mrows_pad % tile_rows(cand.tile) || Nrows_pad % tile_cols(cand.tile)) continue;
         for (int rep = 0; rep < 2 && rc == T2X_OK; rep++) {   // best of two runs per candidate
             if (hipEventRecord(e0, st) != hipSuccess) {

@@ -39,7 +39,7 @@ def test_variant_bit_identical(case, gpu_device, tile, group):
     if tile:
         assert st.gemm_tile_used == tile and (group == 0 or st.gemm_group_used == group)
     else:
-        assert st.gemm_tile_used in (128, 256, 257, 258) and st.gemm_group_used > 0
+        assert st.gemm_tile_used in (128, 256, 257, 258, 259) and st.gemm_group_used > 0
     print(f"\n[gemm variant] requested ({tile}, {group}) used ({st.gemm_tile_used}, {st.gemm_group_used})")
     assert np.array_equal(g["dbg"]["coarse_diff2"], ref["dbg"]["coarse_diff2"])
     assert np.array_equal(g["out"]["best_hidden"], ref["out"]["best_hidden"])
@@ -82,8 +82,40 @@ def test_two_product_kernel_across_k_step_counts(gpu_device):
         assert np.array_equal(valid, g["dbg"]["coarse_diff2"] > -998)
         e = H.rel(g["dbg"]["coarse_diff2"][valid], o["dbg"]["coarse_diff2"][valid])
         print(f"\n[fp16x2] coarse size {cs}: P_c {st.P_coarse}, {nk} K steps, coarse diff2 max rel {e:.2e}")
-        assert st.gemm_tile_used == 258
+        assert st.gemm_tile_used in (258, 259)
         assert e < 2e-4, cs
         assert np.array_equal(g["out"]["best_hidden"], o["out"]["best_hidden"]), cs
         seen.setdefault(nk % 3, cs)
     assert set(seen) == {0, 1, 2}, seen
+
+
+@pytest.fixture(scope="module")
+def case_x2(gpu_device):
+    wl = H.Workload("refine64", n=24, seed_offset=5, precision=T.T2X_PREC_FP16X2)
+    wl.cfg.gemm_tile, wl.cfg.gemm_group = 258, 0
+    ref = wl.run_gpu(dev=gpu_device, n_debug=24, fine_cap=16384)
+    return wl, ref
+
+
+@pytest.mark.parametrize("tile,group", [(259, 16), (259, 32), (259, 64), (258, 64), (0, 0)])
+def test_two_product_variants_bit_identical(case_x2, gpu_device, tile, group):
+    """The two 2-product kernels (k_gemm_x3p<f16, EPI_DIFF2, 2>, tile 258: 256x256 with 8 waves;
+    k_gemm_x2h, tile 259: 128x256 with 4 waves, two workgroups per compute unit) feed every element
+    the same MFMA chain and epilogue: the whole coarse diff2 matrix, poses and dLL are
+    bit-identical across tiles, tile-walk groups and the autotuner's choice."""
+    wl, ref = case_x2
+    lib = T.load()
+    wl.cfg.gemm_tile, wl.cfg.gemm_group = tile, group
+    T.check(lib.t2x_stats_reset(gpu_device))
+    g = wl.run_gpu(dev=gpu_device, n_debug=24, fine_cap=16384)
+    st = T.Stats()
+    T.check(lib.t2x_stats_get(gpu_device, C.byref(st)))
+    wl.cfg.gemm_tile, wl.cfg.gemm_group = 258, 0
+    print(f"\n[fp16x2 variant] requested ({tile}, {group}) used ({st.gemm_tile_used}, {st.gemm_group_used})")
+    if tile:
+        assert st.gemm_tile_used == tile and st.gemm_group_used == group
+    else:
+        assert st.gemm_tile_used in (258, 259)
+    assert np.array_equal(g["dbg"]["coarse_diff2"], ref["dbg"]["coarse_diff2"])
+    assert np.array_equal(g["out"]["best_hidden"], ref["out"]["best_hidden"])
+    assert np.array_equal(g["out"]["dLL"], ref["out"]["dLL"])
