@@ -341,11 +341,13 @@ extern "C" int t2x_backprojector_get(t2x_device* h, int iclass, double* re, doub
     ClassVolumes& c = it->second;
     size_t n = vol_elems(c.b_ref_dim, c.b_pad);
     T2X_HIP(hipSetDevice(d->id));
+    // both volumes in one page-locked staging area (two DMA copies, no driver staging)
+    T2X_TRY(d->rb_host.ensure(n * (sizeof(float2) + sizeof(float))));
+    float2* hd = d->rb_host.as<float2>();
+    float* hw = reinterpret_cast<float*>(hd + n);
+    T2X_HIP(hipMemcpyAsync(hd, c.bp_data, n * sizeof(float2), hipMemcpyDeviceToHost, d->stream));
+    T2X_HIP(hipMemcpyAsync(hw, c.bp_weight, n * sizeof(float), hipMemcpyDeviceToHost, d->stream));
     T2X_HIP(hipStreamSynchronize(d->stream));
-    std::vector<float2> hd(n);
-    std::vector<float> hw(n);
-    T2X_HIP(hipMemcpy(hd.data(), c.bp_data, n * sizeof(float2), hipMemcpyDeviceToHost));
-    T2X_HIP(hipMemcpy(hw.data(), c.bp_weight, n * sizeof(float), hipMemcpyDeviceToHost));
     for (size_t i = 0; i < n; i++) {
         if (re) re[i] = hd[i].x;
         if (im) im[i] = hd[i].y;

@@ -1456,6 +1456,68 @@ static int zero_async(void* p, size_t bytes, hipStream_t st)
     return T2X_OK;
 }
 
+// The batch's readback: up to kPackMax device vectors gathered into one device buffer, each at
+// an 8-byte aligned offset, so that the host reads them with a single copy into page-locked
+// memory (separate copies into pageable vectors each go through a driver blit and a sync).
+constexpr int kPackMax = 12;
+struct PackSeg {
+    const unsigned* src;
+    long long off_words;   // offset in dst
+    long long n_words;
+};
+struct PackList {
+    PackSeg seg[kPackMax];
+};
+
+// grid.y = segment, grid.x strides over the segment's 32-bit words
+__global__ void k_pack(PackList L, unsigned* __restrict__ dst)
+{
+    const PackSeg s = L.seg[blockIdx.y];
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < s.n_words;
+         i += (long long)gridDim.x * blockDim.x)
+        dst[s.off_words + i] = s.src[i];
+}
+
+struct Readback {
+    struct Item {
+        void* dst;
+        const void* src;
+        size_t bytes, off;
+    };
+    std::vector<Item> items;
+    size_t total = 0;
+    void add(void* dst, const void* src, size_t bytes)
+    {
+        items.push_back({dst, src, bytes, total});
+        total += (bytes + 7) & ~size_t(7);
+    }
+    // packs, copies and waits for the stream; then fills the host destinations
+    int run(Device* d, hipStream_t st)
+    {
+        if (items.size() > (size_t)kPackMax) { set_error("readback: too many vectors"); return T2X_EINVAL; }
+        if (items.empty()) return T2X_OK;
+        T2X_TRY(d->rb_dev.ensure(total));
+        T2X_TRY(d->rb_host.ensure(total));
+        PackList L{};
+        long long maxw = 0;
+        for (size_t i = 0; i < items.size(); i++) {
+            if (items[i].bytes % 4) { set_error("readback: size not a multiple of 4 bytes"); return T2X_EINVAL; }
+            L.seg[i] = {reinterpret_cast<const unsigned*>(items[i].src), (long long)(items[i].off / 4),
+                        (long long)(items[i].bytes / 4)};
+            maxw = std::max(maxw, L.seg[i].n_words);
+        }
+        const dim3 grid((unsigned)std::max<long long>(1, std::min<long long>((maxw + 255) / 256, 256)),
+                        (unsigned)items.size());
+        k_pack<<<grid, 256, 0, st>>>(L, d->rb_dev.as<unsigned>());
+        T2X_HIP(hipGetLastError());
+        T2X_HIP(hipMemcpyAsync(d->rb_host.ptr, d->rb_dev.ptr, total, hipMemcpyDeviceToHost, st));
+        T2X_HIP(hipStreamSynchronize(st));
+        for (const Item& it : items)
+            std::memcpy(it.dst, d->rb_host.as<char>() + it.off, it.bytes);
+        return T2X_OK;
+    }
+};
+
 // One pass (adaptive_oversampling 0): the significant pairs' diff2 straight from the coarse
 // matrix (rows (particle, translation), columns orientations), and the coarse pass's weight
 // normalisation as the stored one.
@@ -2376,8 +2438,11 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             lp.Npsi = (int)Npsi;
         }
         // ---- preprocessing ----
-        T2X_TRY(d->img2.ensure(sizeof(float) * 2 * B * NN));
-        T2X_TRY(d->fft_out.ensure(sizeof(float2) * 2 * B * N * xN));
+        // the batched FFTs run over the call's largest batch whatever B (the plans are made once
+        // per call shape, not again for a smaller last batch; transforms beyond B are not read)
+        const int Bfft = std::min(Bmax, n);
+        T2X_TRY(d->img2.ensure(sizeof(float) * 2 * Bfft * NN));
+        T2X_TRY(d->fft_out.ensure(sizeof(float2) * 2 * Bfft * N * xN));
         T2X_TRY(d->Fimg_cur.ensure(sizeof(float2) * B * Pw));
         T2X_TRY(d->Fimg_nm_cur.ensure(sizeof(float2) * B * Pw));
         T2X_TRY(d->Fctf_cur.ensure(sizeof(float) * B * Pw));
@@ -2385,22 +2450,22 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         const float* d_noise = nullptr;
         if (!cfg->do_zero_mask) {
             // the mask's background: per particle a noise image with its optics group's spectrum
-            T2X_TRY(d->noise_spec.ensure(sizeof(float2) * B * N * xN));
-            T2X_TRY(d->noise_img.ensure(sizeof(float) * B * NN));
+            T2X_TRY(d->noise_spec.ensure(sizeof(float2) * Bfft * N * xN));
+            T2X_TRY(d->noise_img.ensure(sizeof(float) * Bfft * NN));
             k_noise_spectrum<<<B, 256, 0, st>>>(N, d_ps, d->sigma2_tab.as<double>(), cfg->sigma2_fudge,
                                                 cfg->random_seed ^ 0x6d61736b6e6f6973ULL, d->noise_spec.as<float2>());
             T2X_HIP(hipGetLastError());
-            if (!d->noise_plan || d->noise_n != N || d->noise_batch != B) {
+            if (!d->noise_plan || d->noise_n != N || d->noise_batch != Bfft) {
                 if (d->noise_plan) { (void)hipfftDestroy(d->noise_plan); d->noise_plan = 0; }
                 int dims[2] = {N, N};
-                if (hipfftPlanMany(&d->noise_plan, 2, dims, nullptr, 1, N * xN, nullptr, 1, N * N, HIPFFT_C2R, B) !=
+                if (hipfftPlanMany(&d->noise_plan, 2, dims, nullptr, 1, N * xN, nullptr, 1, N * N, HIPFFT_C2R, Bfft) !=
                         HIPFFT_SUCCESS ||
                     hipfftSetStream(d->noise_plan, st) != HIPFFT_SUCCESS) {
                     set_error("hipfft C2R plan for the mask background failed");
                     return T2X_EDEVICE;
                 }
                 d->noise_n = N;
-                d->noise_batch = B;
+                d->noise_batch = Bfft;
             }
             if (hipfftExecC2R(d->noise_plan, d->noise_spec.as<hipfftComplex>(), d->noise_img.as<hipfftReal>()) !=
                 HIPFFT_SUCCESS) {
@@ -2412,7 +2477,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         k_prep_real<<<B, 256, 0, st>>>(d_img, N, B, d_ps, cfg->particle_diameter / (2. * pix),
                                       (double)cfg->width_mask_edge, d_noise, d->img2.as<float>());
         T2X_HIP(hipGetLastError());
-        T2X_TRY(ensure_fft(d, N, 2 * B));
+        T2X_TRY(ensure_fft(d, N, 2 * Bfft));
         if (hipfftExecR2C(d->fft_plan, d->img2.as<hipfftReal>(), d->fft_out.as<hipfftComplex>()) != HIPFFT_SUCCESS) {
             set_error("hipfftExecR2C failed");
             return T2X_EDEVICE;
@@ -2901,6 +2966,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         std::vector<long long> hnsig(B);
         std::vector<double> hnorm(3 * B), hpower((size_t)B * xN);
         std::vector<PartScalars> hps2(B);
+        std::vector<int> hflag(B);   // candidate path: particles sent to the radix select
         auto post_part = [&]() -> int {
         T2X_TRY(d->sig_pairs.ensure(sizeof(int2) * cap));
         T2X_TRY(d->sig_orient.ensure(sizeof(int) * cap));
@@ -3221,15 +3287,19 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             T2X_TRY(post_part());
         }
         // ---- per-particle results: the batch's one readback ----
-        T2X_HIP(hipMemcpyAsync(&hplan, d_plan, sizeof(BatchPlan), hipMemcpyDeviceToHost, st));
-        T2X_HIP(hipMemcpyAsync(pair_off.data(), d_pair_off.ptr, sizeof(long long) * (B + 1), hipMemcpyDeviceToHost, st));
-        T2X_HIP(hipMemcpyAsync(hselc.data(), d_selc, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
-        T2X_HIP(hipMemcpyAsync(hself.data(), d_self, sizeof(SelResult) * B, hipMemcpyDeviceToHost, st));
-        T2X_HIP(hipMemcpyAsync(hnsig.data(), d->fine_sel.ptr, sizeof(long long) * B, hipMemcpyDeviceToHost, st));
-        T2X_HIP(hipMemcpyAsync(hnorm.data(), d->best.ptr, sizeof(double) * 3 * B, hipMemcpyDeviceToHost, st));
-        T2X_HIP(hipMemcpyAsync(hpower.data(), d->power.ptr, sizeof(double) * B * xN, hipMemcpyDeviceToHost, st));
-        T2X_HIP(hipMemcpyAsync(hps2.data(), d_ps, sizeof(PartScalars) * B, hipMemcpyDeviceToHost, st));
-        T2X_HIP(hipStreamSynchronize(st));
+        {
+            Readback rb;
+            rb.add(&hplan, d_plan, sizeof(BatchPlan));
+            rb.add(pair_off.data(), d_pair_off.ptr, sizeof(long long) * (B + 1));
+            rb.add(hselc.data(), d_selc, sizeof(SelResult) * B);
+            rb.add(hself.data(), d_self, sizeof(SelResult) * B);
+            rb.add(hnsig.data(), d->fine_sel.ptr, sizeof(long long) * B);
+            rb.add(hnorm.data(), d->best.ptr, sizeof(double) * 3 * B);
+            rb.add(hpower.data(), d->power.ptr, sizeof(double) * B * xN);
+            rb.add(hps2.data(), d_ps, sizeof(PartScalars) * B);
+            if (cand_path) rb.add(hflag.data(), d_radix, sizeof(int) * B);
+            T2X_TRY(rb.run(d, st));
+        }
         if (hplan.err_code) {
             set_error(std::string(hplan.err_code == T2X_EZEROSUM ? "zero sum of weights in the coarse pass"
                                                                   : "no significant coarse sample") +
@@ -3248,8 +3318,6 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         if (!cand_path) {
             stt.n_select_fallbacks += 1;
         } else {   // did any particle of the batch take the radix select?
-            std::vector<int> hflag(B);
-            T2X_HIP(hipMemcpy(hflag.data(), d_radix, sizeof(int) * B, hipMemcpyDeviceToHost));
             for (int i = 0; i < B; i++)
                 if (hflag[i]) { stt.n_select_fallbacks += 1; break; }
         }
@@ -3339,7 +3407,11 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     stt.n_calls += 1;
     // ---- weighted sums back to the caller ----
     std::vector<double> hw(nws);
-    T2X_HIP(hipMemcpy(hw.data(), d->wsum_dev.ptr, sizeof(double) * nws, hipMemcpyDeviceToHost));
+    {
+        Readback rb;
+        rb.add(hw.data(), d->wsum_dev.ptr, sizeof(double) * nws);
+        T2X_TRY(rb.run(d, st));
+    }
     if (wsum) {
         const size_t o_sumw = (size_t)n_og * xN, o_pdfc = o_sumw + n_og, o_pdfd = o_pdfc + K,
                      o_off = o_pdfd + (size_t)K * Nd, o_poff = o_off + 1;

@@ -75,6 +75,37 @@ struct DevBuf {
     }
 };
 
+// Page-locked host memory (hipHostMalloc): the staging area of the batch's one readback, so
+// that it is a single DMA copy rather than a blit into a driver staging buffer per vector.
+struct HostBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    HostBuf() = default;
+    HostBuf(const HostBuf&) = delete;
+    HostBuf& operator=(const HostBuf&) = delete;
+    ~HostBuf() { release(); }
+    int ensure(size_t need)
+    {
+        if (need <= bytes) return T2X_OK;
+        release();
+        size_t b = need + need / 8 + 256;
+        if (hipHostMalloc(&ptr, b, hipHostMallocDefault) != hipSuccess) {
+            ptr = nullptr;
+            set_error("hipHostMalloc of " + std::to_string(b) + " bytes failed");
+            return T2X_ENOMEM;
+        }
+        bytes = b;
+        return T2X_OK;
+    }
+    template <class T> T* as() const { return reinterpret_cast<T*>(ptr); }
+    void release()
+    {
+        if (ptr) (void)hipHostFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
 // One class of the model: the projector (PPref) and the backprojector (BPref).
 struct ClassVolumes {
     // projector
@@ -162,6 +193,9 @@ struct Device {
     DevBuf plan;
     long long plan_cap = 0;
     int n_cu = 0;
+    // the batch's readback: its vectors packed into rb_dev by one kernel, then one copy to rb_host
+    DevBuf rb_dev;
+    HostBuf rb_host;
     // launch_graph: the batch graph captured in call graph_call for batches of graph_B
     // particles (valid while alloc_generation() == graph_gen), and the batch sizes the current
     // call has run eagerly
