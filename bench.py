@@ -9,6 +9,8 @@ same run on the same resident particles:
            references); the fine pass and every other cross term as fp16x3
   fp16x2+graph  the same, each batch's launches after its preprocessing captured once per batch
            size as a hipGraph and replayed (reported beside the eager paths)
+  (+direct on any path: the 3D backprojection scatters each pixel into the volume with global
+           atomics instead of accumulating in LDS bricks; a measurement aid, not run by default)
   learned  fp16x3 + learned pose proposal: the coarse pass scores only the top-16 bins
            (classes x HEALPix order-2 directions) of the proposal network, exhaustive search for
            particles whose top softmax probability is below 0.3. The network is trained on the
@@ -186,11 +188,13 @@ def train_proposal(lib, dev, cfg, sz, ds, w, steps=2000, batch=256, n_proj=20000
 
 
 def configure(cfg, pname):
-    """A path name is a precision / proposal setting, optionally with "+graph": the same path
-    with each batch's launches after the preprocessing run as a replayed hipGraph."""
-    base, _, mode = pname.partition("+")
+    """A path name is a precision / proposal setting, optionally with modifiers: "+graph" (each
+    batch's launches after the preprocessing run as a replayed hipGraph) and "+direct" (the 3D
+    backprojection scatters every pixel with global atomics instead of going through LDS bricks)."""
+    base, *modes = pname.split("+")
     cfg.precision = PRECISIONS[base]
-    cfg.launch_graph = 1 if mode == "graph" else 0
+    cfg.launch_graph = 1 if "graph" in modes else 0
+    cfg.bp_direct = 1 if "direct" in modes else 0
     topk, gate, conf = PROPOSAL.get(base, (0, 0, 0.0))
     cfg.proposal_topk = topk
     cfg.proposal_gate = gate
@@ -577,6 +581,9 @@ def main():
              "ms_per_step": rr["dt"] * 1e3 / args.steps,
              "coarse_gemm_ms": rr["st"].ms_coarse_gemm / args.steps,
              "fine_ms": (rr["st"].ms_fine_diff2) / args.steps,
+             "store_ms": rr["st"].ms_store / args.steps,
+             "bp_items_through_bricks": (rr["st"].bp_brick_items / rr["st"].bp_orientations
+                                         if rr["st"].bp_orientations > 0 else None),
              "gemm_variant_autotuned": ([rr["st"].gemm_tile_used, rr["st"].gemm_group_used]
                                         if rr["st"].gemm_tile_used else None),
              "pose_index_agreement_vs_fp32": (float((rr["out"]["best_hidden"] == runs["fp32"]["out"]["best_hidden"]).mean())
@@ -598,7 +605,7 @@ def main():
                       "train_top16_last": train.top16_last if train else None})
         if rr["st"].local_rows > 0:
             v["local_rows_per_slice"] = rr["st"].local_rows / max(1.0, rr["st"].local_slices)
-        if pname.endswith("+graph"):
+        if "+graph" in pname:
             base = pname.partition("+")[0]
             v.update({"graph_launches_per_step": rr["st"].n_graph_launches / args.steps,
                       "ms_in_graphs_per_step": rr["st"].ms_graph / args.steps,

@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define T2X_ABI_VERSION 4
+#define T2X_ABI_VERSION 5
 
 enum {
     T2X_OK = 0,
@@ -130,6 +130,11 @@ typedef struct t2x_estep_config {
                                    /* together (t2x_stats.ms_graph). Global searches      */
                                    /* without the proposal and without diagnostics only;  */
                                    /* elsewhere ignored                                   */
+    int    bp_direct;              /* 1: every pixel inserted into the 3D backprojector   */
+                                   /* with global atomics (2D references always are);    */
+                                   /* 0: through LDS bricks of 16^3 voxels, as many of a */
+                                   /* batch's items as the brick buffers hold, the rest  */
+                                   /* directly (t2x_stats.bp_brick_items)                */
 } t2x_estep_config;
 
 /* Model arrays of the current iteration (host, caller-owned, read-only). */
@@ -247,6 +252,7 @@ typedef struct t2x_stats {
     double local_slices;         /* of those, distinct slices projected (shared within a batch)    */
     double ms_graph;             /* launch_graph: GPU time of the batches run as a graph           */
     long   n_graph_launches;     /* batches run as a graph (captured, or replayed)                 */
+    double bp_brick_items;       /* of bp_orientations, those inserted through LDS bricks          */
 } t2x_stats;
 
 /* ---------------------------------------------------------------------------------------

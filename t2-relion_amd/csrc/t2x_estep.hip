@@ -1168,6 +1168,12 @@ struct StoreParams {
     double* w_aa;
     int Nd;
     LocalPrior lp;             // local searches (lp.dir null: global)
+    // brick backprojection (3D references; bp_val null: every pixel scattered with global
+    // atomics): per item its plane (unit normal, class), its frame and its pixels' values
+    float4* bp_filt;           // [item_cap] (nx, ny, nz, class as int bits; -1: nothing to insert)
+    double* bp_geom;           // [item_cap][6] pf * Ainv (da0..da5)
+    float4* bp_val;            // [item_cap][npix] (data re, data im, weight, 0); 0 where not inserted
+    long long item_cap;        // items from item_cap on take the direct scatter
 };
 
 #define STORE_MAX_T 2048
@@ -1196,6 +1202,8 @@ __device__ __forceinline__ void store_item(const long long* __restrict__ item_of
     const double sumw = P.fine_res[b].sum_w, sig = P.fine_res[b].sig_w, mind2 = P.fine_res[b].min_d2;
     const int nover = g.over_rot * g.over_trans;
     const int og = P.ps[b].og;
+    const bool brick = P.bp_val && !g.ref2d && item < P.item_cap;
+    if (brick && threadIdx.x == 0) P.bp_filt[item] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
     if (threadIdx.x == 0) s_cnt = 0;
     for (int i = threadIdx.x; i < P.nshell; i += blockDim.x) shell[i] = 0.;
     __syncthreads();
@@ -1267,6 +1275,14 @@ __device__ __forceinline__ void store_item(const long long* __restrict__ item_of
     const double da0 = Ai[0] * pf, da1 = Ai[1] * pf, da2 = Ai[3] * pf, da3 = Ai[4] * pf, da4 = Ai[6] * pf, da5 = Ai[7] * pf;
     const float fa0 = (float)da0, fa1 = (float)da1, fa2 = (float)da2;
     const float fa3 = (float)da3, fa4 = (float)da4, fa5 = (float)da5;
+    if (brick && threadIdx.x == 0) {
+        double* gm = P.bp_geom + (size_t)item * 6;
+        gm[0] = da0; gm[1] = da1; gm[2] = da2; gm[3] = da3; gm[4] = da4; gm[5] = da5;
+        // the plane of the slice: normal (da0, da2, da4) x (da1, da3, da5)
+        const double nx = da2 * da5 - da4 * da3, ny = da4 * da1 - da0 * da5, nz = da0 * da3 - da2 * da1;
+        const double nn = 1. / sqrt(nx * nx + ny * ny + nz * nz);
+        P.bp_filt[item] = make_float4((float)(nx * nn), (float)(ny * nn), (float)(nz * nn), __int_as_float(c));
+    }
     const float2* vol = P.vols[c];
     const int pad = P.pads[c];
     float2* bpd = P.bp_data[c];
@@ -1322,6 +1338,12 @@ __device__ __forceinline__ void store_item(const long long* __restrict__ item_of
         if (scale_sums) {
             xa_acc += (double)xacc;
             aa_acc += (double)wsum_q * ((double)fr * fr + (double)fi * fi);
+        }
+        if (brick) {   // inserted by k_bp_brick
+            const bool ins = (wacc > 0.f) &&
+                             bp_pixel_inside(AtA_xx, AtA_xy, AtA_yy, AtA_xy2, max_r2, da0, da1, da2, da3, da4, da5, x, y, 0);
+            P.bp_val[(size_t)item * P.npix + p] = ins ? make_float4(accr, acci, wacc, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+            continue;
         }
         if (!(wacc > 0.f)) continue;
         if (!bp_pixel_inside(AtA_xx, AtA_xy, AtA_yy, AtA_xy2, max_r2, da0, da1, da2, da3, da4, da5, x, y, g.ref2d))
@@ -1422,6 +1444,173 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
         const long long item = s_item;
         if (item >= total) break;
         store_item(item_off, B, orient_off, orients, orient_pair_start, pair_off, pairs, P, item);
+    }
+}
+
+// Backprojection through LDS bricks (3D references). In brick mode k_store leaves each item's
+// frame and per-pixel values (data re/im and weight: the item's translations summed, before the
+// trilinear insertion of backproject2Dto3D, backprojector.cpp:136-357) in bp_geom / bp_val, and
+// k_bp_brick inserts them: one workgroup per unit (a BP_S^3 voxel brick of one class's volume,
+// and a share of the batch's items), which collects the items whose plane crosses the brick,
+// inserts every pixel of theirs whose trilinear footprint meets the brick into an LDS copy of
+// it (LDS atomics), and adds the brick to the volume at the end: a plain read-modify-write when
+// one unit covers the brick, global atomics when its items are shared among several. The
+// per-pixel arithmetic (positions in double without contraction, the Friedel flip, the bounds
+// and the eight trilinear weights) is the direct scatter's; only the order of the FP32 sums
+// differs. A wave takes one item at a time and walks the pixels whose positions can fall in the
+// brick: the box of the plane coordinates (m0.q, m1.q) / |m0|^2 over the brick's corners (and
+// its mirror image, for pixels inserted at -q).
+#define BP_S 16
+#define BP_VOX (BP_S * BP_S * BP_S)
+#define BP_LIST 1024
+#define BP_THREADS 512
+
+struct BpBrickParams {
+    const long long* item_off;
+    int B;
+    long long item_cap;
+    const float4* filt;
+    const double* geom;
+    const float4* val;
+    const int* pix_index;      // [(y + py0) * pxw + x]: compact pixel of frequency (x, y), or -1
+    int py0, pxw, pyh;
+    int npix;
+    float2* const* bp_data;
+    float* const* bp_weight;
+    const int* bp_pads;
+};
+
+struct BpUnit {
+    int c, bx, by, bz;   // class, brick origin / BP_S in voxel index space (x, y, z)
+    int g, G;            // this unit's share: the items i with i % G == g
+    int pad0, pad1;
+};
+
+__device__ __forceinline__ void bp_insert_item(const BpBrickParams& P, int item, int bxdim, int bpad, int bh,
+                                               int x_lo, int y_lo, int z_lo, const double (&lo)[3],
+                                               const double (&hi)[3], float* s_re, float* s_im, float* s_w,
+                                               int lane)
+{
+    const double* gm = P.geom + (size_t)item * 6;
+    const double a0 = gm[0], a1 = gm[1], a2 = gm[2], a3 = gm[3], a4 = gm[4], a5 = gm[5];
+    const double s2 = a0 * a0 + a2 * a2 + a4 * a4;
+    const double u_lo = (fmin(a0 * lo[0], a0 * hi[0]) + fmin(a2 * lo[1], a2 * hi[1]) + fmin(a4 * lo[2], a4 * hi[2])) / s2;
+    const double u_hi = (fmax(a0 * lo[0], a0 * hi[0]) + fmax(a2 * lo[1], a2 * hi[1]) + fmax(a4 * lo[2], a4 * hi[2])) / s2;
+    const double v_lo = (fmin(a1 * lo[0], a1 * hi[0]) + fmin(a3 * lo[1], a3 * hi[1]) + fmin(a5 * lo[2], a5 * hi[2])) / s2;
+    const double v_hi = (fmax(a1 * lo[0], a1 * hi[0]) + fmax(a3 * lo[1], a3 * hi[1]) + fmax(a5 * lo[2], a5 * hi[2])) / s2;
+    const double e = 1e-6;
+    const int ymin = -P.py0, ymax = P.pyh - 1 - P.py0;
+    // pixels inserted at +q (xp >= 0), then at -q (xp < 0)
+    const int ax0 = max(0, (int)ceil(u_lo - e)), ax1 = min(P.pxw - 1, (int)floor(u_hi + e));
+    const int ay0 = max(ymin, (int)ceil(v_lo - e)), ay1 = min(ymax, (int)floor(v_hi + e));
+    const int bx0 = max(0, (int)ceil(-u_hi - e)), bx1 = min(P.pxw - 1, (int)floor(-u_lo + e));
+    const int by0 = max(ymin, (int)ceil(-v_hi - e)), by1 = min(ymax, (int)floor(-v_lo + e));
+    const int nax = max(0, ax1 - ax0 + 1), nbx = max(0, bx1 - bx0 + 1);
+    const int na = nax * max(0, ay1 - ay0 + 1), nb = nbx * max(0, by1 - by0 + 1);
+    const float4* val = P.val + (size_t)item * P.npix;
+    for (int k = lane; k < na + nb; k += 64) {
+        int x, y;
+        bool flip;
+        if (k < na) { y = ay0 + k / nax; x = ax0 + k % nax; flip = false; }
+        else { const int kk = k - na; y = by0 + kk / nbx; x = bx0 + kk % nbx; flip = true; }
+        const int p = P.pix_index[(y + P.py0) * P.pxw + x];
+        if (p < 0) continue;
+        double xp, yp, zp;
+        {
+#pragma clang fp contract(off)
+            xp = a0 * x + a1 * y;
+            yp = a2 * x + a3 * y;
+            zp = a4 * x + a5 * y;
+        }
+        const bool neg = xp < 0;
+        if (neg != flip) continue;   // inserted in the other walk (or by another brick)
+        const float4 v = val[p];
+        if (!(v.z > 0.f)) continue;
+        if (neg) { xp = -xp; yp = -yp; zp = -zp; }
+        const double fx0 = floor(xp), fy0 = floor(yp), fz0 = floor(zp);
+        const int x0 = (int)fx0, y0 = (int)fy0 + bh, z0 = (int)fz0 + bh;
+        if (x0 < 0 || x0 + 1 >= bxdim || y0 < 0 || y0 + 1 >= bpad || z0 < 0 || z0 + 1 >= bpad) continue;
+        if (x0 + 1 < x_lo || x0 >= x_lo + BP_S || y0 + 1 < y_lo || y0 >= y_lo + BP_S || z0 + 1 < z_lo ||
+            z0 >= z_lo + BP_S)
+            continue;
+        const double fx = xp - fx0, fy = yp - fy0, fz = zp - fz0;
+        const double mfx = 1. - fx, mfy = 1. - fy, mfz = 1. - fz;
+        const float vr = v.x, vi = neg ? -v.y : v.y, wacc = v.z;
+        const double wts[8] = {mfz * mfy * mfx, mfz * mfy * fx, mfz * fy * mfx, mfz * fy * fx,
+                               fz * mfy * mfx, fz * mfy * fx, fz * fy * mfx, fz * fy * fx};
+#pragma unroll
+        for (int q = 0; q < 8; q++) {
+            const int lx = x0 + (q & 1) - x_lo, ly = y0 + ((q >> 1) & 1) - y_lo, lz = z0 + (q >> 2) - z_lo;
+            if ((unsigned)lx >= BP_S || (unsigned)ly >= BP_S || (unsigned)lz >= BP_S) continue;
+            const int li = (lz * BP_S + ly) * BP_S + lx;
+            const float wk = (float)wts[q];
+            atomicAdd(&s_re[li], wk * vr);
+            atomicAdd(&s_im[li], wk * vi);
+            atomicAdd(&s_w[li], wk * wacc);
+        }
+    }
+}
+
+__global__ __launch_bounds__(BP_THREADS) void k_bp_brick(BpBrickParams P, const BpUnit* __restrict__ units)
+{
+    __shared__ float s_re[BP_VOX], s_im[BP_VOX], s_w[BP_VOX];
+    __shared__ int s_list[BP_LIST];
+    __shared__ int s_n;
+    // the batch's items below item_cap (k_store scattered the others directly)
+    const long long total = min(P.item_off[P.B], P.item_cap);
+    if (total == 0) return;
+    const BpUnit u = units[blockIdx.x];
+    const int bpad = P.bp_pads[u.c], bxdim = bpad / 2 + 1, bh = bpad / 2;
+    const int x_lo = u.bx * BP_S, y_lo = u.by * BP_S, z_lo = u.bz * BP_S;
+    for (int v = threadIdx.x; v < BP_VOX; v += blockDim.x) { s_re[v] = 0.f; s_im[v] = 0.f; s_w[v] = 0.f; }
+    if (threadIdx.x == 0) s_n = 0;
+    // positions (centred) whose footprint [floor, floor + 1] meets the brick: [lo, hi) per axis
+    const double lo[3] = {(double)(x_lo - 1), (double)(y_lo - bh - 1), (double)(z_lo - bh - 1)};
+    const double hi[3] = {(double)(x_lo + BP_S), (double)(y_lo - bh + BP_S), (double)(z_lo - bh + BP_S)};
+    const float flo0 = (float)lo[0], flo1 = (float)lo[1], flo2 = (float)lo[2];
+    const float fhi0 = (float)hi[0], fhi1 = (float)hi[1], fhi2 = (float)hi[2];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
+    __syncthreads();
+    for (long long base = 0; base < total; base += blockDim.x) {
+        const long long i = base + threadIdx.x;
+        if (i < total && (int)(i % u.G) == u.g) {
+            const float4 f = P.filt[i];
+            if (__float_as_int(f.w) == u.c) {
+                // the plane n.q = 0 meets the box when n.q takes both signs over its corners
+                const float dmin = fminf(f.x * flo0, f.x * fhi0) + fminf(f.y * flo1, f.y * fhi1) +
+                                   fminf(f.z * flo2, f.z * fhi2);
+                const float dmax = fmaxf(f.x * flo0, f.x * fhi0) + fmaxf(f.y * flo1, f.y * fhi1) +
+                                   fmaxf(f.z * flo2, f.z * fhi2);
+                if (dmin <= 0.01f && dmax >= -0.01f) s_list[atomicAdd(&s_n, 1)] = (int)i;
+            }
+        }
+        __syncthreads();
+        const int n = s_n;
+        __syncthreads();   // every thread has read the count before the next appends
+        if (n > BP_LIST - BP_THREADS || base + blockDim.x >= total) {
+            for (int e = wave; e < n; e += nwaves)
+                bp_insert_item(P, s_list[e], bxdim, bpad, bh, x_lo, y_lo, z_lo, lo, hi, s_re, s_im, s_w, lane);
+            __syncthreads();
+            if (threadIdx.x == 0) s_n = 0;
+            __syncthreads();
+        }
+    }
+    float2* bpd = P.bp_data[u.c];
+    float* bpw = P.bp_weight[u.c];
+    for (int v = threadIdx.x; v < BP_VOX; v += blockDim.x) {
+        const float r = s_re[v], im = s_im[v], w = s_w[v];
+        if (r == 0.f && im == 0.f && w == 0.f) continue;
+        const int lx = v % BP_S, ly = (v / BP_S) % BP_S, lz = v / (BP_S * BP_S);
+        const size_t gi = ((size_t)(z_lo + lz) * bpad + (size_t)(y_lo + ly)) * bxdim + (size_t)(x_lo + lx);
+        if (u.G == 1) {
+            const float2 o = bpd[gi];
+            bpd[gi] = make_float2(o.x + r, o.y + im);
+            bpw[gi] += w;
+        } else {
+            atomicAdd(&bpd[gi].x, r);
+            atomicAdd(&bpd[gi].y, im);
+            atomicAdd(&bpw[gi], w);
+        }
     }
 }
 
@@ -1850,6 +2039,14 @@ static int upload_table(PassTables& t, hipStream_t st)
     T2X_HIP(hipMemcpyAsync(t.d_minv.ptr, t.h_minv.data(), sizeof(float) * t.h_minv.size(), hipMemcpyHostToDevice, st));
     T2X_HIP(hipMemcpyAsync(t.d_ires.ptr, t.h_ires.data(), sizeof(int) * t.npix, hipMemcpyHostToDevice, st));
     T2X_HIP(hipMemcpyAsync(t.d_inproj.ptr, t.h_inproj.data(), t.npix, hipMemcpyHostToDevice, st));
+    // frequency -> compact pixel: x in [0, size/2], y in [-size/2, size/2]
+    t.px_w = t.size / 2 + 1;
+    t.py0 = t.size / 2;
+    t.py_h = t.size + 1;
+    std::vector<int> idx((size_t)t.px_w * t.py_h, -1);
+    for (int i = 0; i < t.npix; i++) idx[(size_t)(t.h_y[i] + t.py0) * t.px_w + t.h_x[i]] = i;
+    T2X_TRY(t.d_pixidx.ensure(sizeof(int) * idx.size()));
+    T2X_HIP(hipMemcpyAsync(t.d_pixidx.ptr, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, st));
     T2X_HIP(hipStreamSynchronize(st));
     return T2X_OK;
 }
@@ -2360,6 +2557,60 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     stt.current_size = cs;
     stt.P_coarse = tc.npix;
     stt.P_fine = tf.npix;
+
+    // ---- brick backprojection (3D references): the call's work units and buffers ----
+    const bool bp_bricks = !ref2d && !cfg->bp_direct;
+    long long bp_item_cap = 0;
+    int bp_nunits = 0;
+    if (bp_bricks) {
+        // a batch's items held for the bricks: up to 256 per particle and 2 GiB of pixel values
+        // (the items beyond are scattered directly)
+        const int Bc = std::min(Bmax, n);
+        bp_item_cap = std::min<long long>((long long)Bc * 256,
+                                          std::max<long long>(1, (2ll << 30) / ((long long)tf.npix * 16)));
+        const char* env_cap = getenv("T2X_BP_ITEM_CAP");   // tests: a small capacity sends items direct
+        if (env_cap && atoll(env_cap) > 0) bp_item_cap = atoll(env_cap);
+        // each unit takes about unit_items of the items that cross its brick: the items of a class
+        // in a batch (from the last batch's items per particle, else two orientations per particle)
+        // times the share of planes through the origin that cross a brick at distance r from it,
+        // about (half-diagonal) / r
+        double unit_items = 384.;
+        const char* env_ui = getenv("T2X_BP_UNIT_ITEMS");
+        if (env_ui && atof(env_ui) > 0.) unit_items = atof(env_ui);
+        const double ipp = d->bp_items_pp > 0. ? d->bp_items_pp : 2. * over_rot;
+        const double items_class = ipp * Bc / K;
+        const double hd = 0.5 * std::sqrt(3.) * (BP_S + 1);
+        std::vector<BpUnit> units;
+        for (int c = 0; c < K; c++) {
+            const int bpad = h_bpads[c], bxdim = bpad / 2 + 1, bh = bpad / 2;
+            const int rr = relion_round(h_bprmax[c] * pf);
+            const int nbx = (bxdim + BP_S - 1) / BP_S, nby = (bpad + BP_S - 1) / BP_S;
+            for (int bz = 0; bz < nby; bz++)
+                for (int by = 0; by < nby; by++)
+                    for (int bx = 0; bx < nbx; bx++) {
+                        // positions (centred) inserted into the brick: [lo, hi) per axis
+                        const double lo[3] = {bx * BP_S - 1., by * BP_S - bh - 1., bz * BP_S - bh - 1.};
+                        const double hi[3] = {bx * BP_S + (double)BP_S, by * BP_S - bh + (double)BP_S,
+                                              bz * BP_S - bh + (double)BP_S};
+                        double dn2 = 0., dc2 = 0.;
+                        for (int k = 0; k < 3; k++) {
+                            const double q = std::max(lo[k], std::min(0., hi[k]));   // nearest to the origin
+                            const double m = 0.5 * (lo[k] + hi[k]);
+                            dn2 += q * q;
+                            dc2 += m * m;
+                        }
+                        if (dn2 > (double)(rr + 1) * (rr + 1)) continue;   // beyond r_max: nothing lands
+                        const double f = std::min(1., hd / std::max(1., std::sqrt(dc2)));
+                        const int G = std::max(1, std::min(16, (int)std::ceil(f * items_class / unit_items)));
+                        for (int g2 = 0; g2 < G; g2++) units.push_back(BpUnit{c, bx, by, bz, g2, G, 0, 0});
+                    }
+        }
+        bp_nunits = (int)units.size();
+        T2X_TRY(upload(d->bp_units, units, st));
+        T2X_TRY(d->bp_filt.ensure(sizeof(float4) * bp_item_cap));
+        T2X_TRY(d->bp_geom.ensure(sizeof(double) * 6 * bp_item_cap));
+        T2X_TRY(d->bp_val.ensure(sizeof(float4) * bp_item_cap * tf.npix));
+    }
 
     for (int b0 = 0; b0 < n; b0 += Bmax) {
         const int B = std::min(Bmax, n - b0);
@@ -3224,6 +3475,12 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         P.dvp = scale_sums ? d->dvp_tab.as<double>() : nullptr;
         P.Nd = (int)Nd;
         P.lp = lp;
+        if (bp_bricks) {
+            P.bp_filt = d->bp_filt.as<float4>();
+            P.bp_geom = d->bp_geom.as<double>();
+            P.bp_val = d->bp_val.as<float4>();
+            P.item_cap = bp_item_cap;
+        }
         if (xN > 520) { set_error("box too large for the shell accumulator"); return T2X_EINVAL; }
         // one persistent launch: 8 workgroups per compute unit take the items from the queue
         k_store<<<(unsigned)(8 * d->n_cu), 256, 0, st>>>(d_item_off.as<long long>(), B, d_orient_off.as<long long>(),
@@ -3231,6 +3488,25 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                                                            d_pair_off.as<long long>(), d->sig_pairs.as<int2>(), P,
                                                            &d_plan->queue[1]);
         T2X_HIP(hipGetLastError());
+        if (bp_bricks && bp_nunits > 0) {
+            BpBrickParams Q{};
+            Q.item_off = d_item_off.as<long long>();
+            Q.B = B;
+            Q.item_cap = bp_item_cap;
+            Q.filt = d->bp_filt.as<float4>();
+            Q.geom = d->bp_geom.as<double>();
+            Q.val = d->bp_val.as<float4>();
+            Q.pix_index = tf.d_pixidx.as<int>();
+            Q.py0 = tf.py0;
+            Q.pxw = tf.px_w;
+            Q.pyh = tf.py_h;
+            Q.npix = tf.npix;
+            Q.bp_data = d_bpd.as<float2*>();
+            Q.bp_weight = d_bpw.as<float*>();
+            Q.bp_pads = d_bpads.as<int>();
+            k_bp_brick<<<(unsigned)bp_nunits, BP_THREADS, 0, st>>>(Q, d->bp_units.as<BpUnit>());
+            T2X_HIP(hipGetLastError());
+        }
         mark();   // 8: store done
         return T2X_OK;
         };
@@ -3315,6 +3591,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         }   // attempt
         npairs = hplan.npairs;
         nitems = hplan.nitems;
+        if (B > 0) d->bp_items_pp = (double)nitems / B;
+        if (bp_bricks) stt.bp_brick_items += (double)std::min(nitems, bp_item_cap);
         if (!cand_path) {
             stt.n_select_fallbacks += 1;
         } else {   // did any particle of the batch take the radix select?
