@@ -645,24 +645,25 @@ def main():
     else:
         # graph-mode paths compute the same results as their eager path; they are reported as
         # variants (their per-stage timings, which the roofline uses, cover eager batches only)
-        eager = [p for p in runs if "+" not in p]
+        eager = [p for p in runs if "+" not in p] or list(runs)
         ok = [p for p in eager if meets_bars(p)] or eager
         head = max(ok, key=lambda p: runs[p]["per_iter"])
     r = runs[head]
+    hb = head.split("+")[0]   # the precision path of the headline (modifiers aside)
     st, dt, value = r["st"], r["iter_dt"], r["per_iter"]
     nsig = st.sig_coarse / max(1, st.n_particles)
     # dominant kernel: the coarse cross-term GEMM; algorithmic 4*T*O*P flops per scored particle
     # (the counting convention: one complex inner product per (translation, orientation) pair);
     # the split-precision forms issue 3 MFMA products for each, reported as their own figure
     launches = max(1, st.n_coarse_diff2_launches)
-    nprod = {"fp32": 1, "fp16x2": 2}.get(head, 3)
+    nprod = {"fp32": 1, "fp16x2": 2}.get(hb, 3)
     flop_total = st.coarse_diff2_flop
     if head.startswith("learned"):   # only the scored bins are computed
         n_bins = cfg.nr_classes * ds["info"].nr_dir // 4
         flop_total *= st.proposal_bins_scored / max(1.0, st.n_proposal_particles * n_bins)
     flop_per_launch = flop_total / launches
     avg_ms = st.ms_coarse_gemm / launches
-    peak = MFMA_PEAK_TFLOPS[head]
+    peak = MFMA_PEAK_TFLOPS[hb]
     achieved = flop_per_launch / (avg_ms * 1e-3) / 1e12 if avg_ms > 0 else 0.0
     # HBM bytes per launch of the same stage from the committed rocprofv3 --pmc measurement
     # (profiles/coarse_gemm_traffic.json: FETCH_SIZE + WRITE_SIZE, copy-kernel calibrated)
@@ -671,7 +672,7 @@ def main():
     tj = os.path.join(REPO, "profiles", "coarse_gemm_traffic.json")
     if traffic is None and os.path.exists(tj) and args.workload == "class256":
         with open(tj) as f:
-            tr = json.load(f).get(head)
+            tr = json.load(f).get(hb)
         if tr:
             traffic = tr["bytes_per_launch"]
             traffic_src = "profiles/coarse_gemm_traffic.json (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, 800-particle batches)"
@@ -680,7 +681,7 @@ def main():
             "traffic_unit": "bytes per launch", "traffic_source": traffic_src,
             "kernel": ("coarse diff2 of local searches: (particle, orientation) rows x translations, k_local_zrows + "
                        + ("k_gemm_tn (FP32 MFMA)" if head == "fp32" else "k_gemm_local_x3 (split MFMA, grouped)") + " + k_local_diff2" if local
-                       else "coarse diff2 cross term, " + KERNELS[head]),
+                       else "coarse diff2 cross term, " + KERNELS[hb]),
             "avg_launch_ms": avg_ms, "launches": launches, "flop_per_launch": flop_per_launch,
             "flop_convention": "algorithmic 4*T*O*P_c per scored particle (one complex product per pair)",
             "mfma_products_per_pair": nprod,
@@ -749,7 +750,7 @@ def main():
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": dt * 1e3 / args.steps,
         "headline_rule": "fastest path, per-iteration costs (proposal training) included, with >= 99% pose "
                          "agreement and FSC-0.143 within 0.1 A of plain FP32",
-        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": DTYPES[head],
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": DTYPES[hb],
         "data": "synthetic (GPU-generated blob volumes and CTF-modulated noisy projections)",
         "config": {"workload": f"{args.workload}: {n} particles {cfg.ori_size}^2 @ {cfg.pixel_size} A/px, "
                                f"K={cfg.nr_classes}, "

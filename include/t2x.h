@@ -130,9 +130,9 @@ typedef struct t2x_estep_config {
                                    /* together (t2x_stats.ms_graph). Global searches      */
                                    /* without the proposal and without diagnostics only;  */
                                    /* elsewhere ignored                                   */
-    int    bp_direct;              /* 1: every pixel inserted into the 3D backprojector   */
-                                   /* with global atomics (2D references always are);    */
-                                   /* 0: through LDS bricks of 16^3 voxels, as many of a */
+    int    bp_direct;              /* 1: every pixel inserted into the backprojector with */
+                                   /* global atomics; 0: through LDS bricks (16^3 voxels */
+                                   /* of a 3D volume, 32^2 of a 2D one), as many of a    */
                                    /* batch's items as the brick buffers hold, the rest  */
                                    /* directly (t2x_stats.bp_brick_items)                */
 } t2x_estep_config;

@@ -1172,7 +1172,9 @@ struct StoreParams {
     // atomics): per item its plane (unit normal, class), its frame and its pixels' values
     float4* bp_filt;           // [item_cap] (nx, ny, nz, class as int bits; -1: nothing to insert)
     double* bp_geom;           // [item_cap][6] pf * Ainv (da0..da5)
-    float4* bp_val;            // [item_cap][npix] (data re, data im, weight, 0); 0 where not inserted
+    float4* bp_val;            // [item_cap][bp_wy][bp_wx] by frequency (x, y + bp_wy0): (data re,
+                               // data im, weight, 0); 0 where not inserted (or not a pixel)
+    int bp_wx, bp_wy, bp_wy0;
     long long item_cap;        // items from item_cap on take the direct scatter
 };
 
@@ -1202,7 +1204,7 @@ __device__ __forceinline__ void store_item(const long long* __restrict__ item_of
     const double sumw = P.fine_res[b].sum_w, sig = P.fine_res[b].sig_w, mind2 = P.fine_res[b].min_d2;
     const int nover = g.over_rot * g.over_trans;
     const int og = P.ps[b].og;
-    const bool brick = P.bp_val && !g.ref2d && item < P.item_cap;
+    const bool brick = P.bp_val && item < P.item_cap;
     if (brick && threadIdx.x == 0) P.bp_filt[item] = make_float4(0.f, 0.f, 0.f, __int_as_float(-1));
     if (threadIdx.x == 0) s_cnt = 0;
     for (int i = threadIdx.x; i < P.nshell; i += blockDim.x) shell[i] = 0.;
@@ -1341,8 +1343,10 @@ __device__ __forceinline__ void store_item(const long long* __restrict__ item_of
         }
         if (brick) {   // inserted by k_bp_brick
             const bool ins = (wacc > 0.f) &&
-                             bp_pixel_inside(AtA_xx, AtA_xy, AtA_yy, AtA_xy2, max_r2, da0, da1, da2, da3, da4, da5, x, y, 0);
-            P.bp_val[(size_t)item * P.npix + p] = ins ? make_float4(accr, acci, wacc, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
+                             bp_pixel_inside(AtA_xx, AtA_xy, AtA_yy, AtA_xy2, max_r2, da0, da1, da2, da3, da4, da5, x, y,
+                                             g.ref2d);
+            P.bp_val[((size_t)item * P.bp_wy + (y + P.bp_wy0)) * P.bp_wx + x] =
+                ins ? make_float4(accr, acci, wacc, 0.f) : make_float4(0.f, 0.f, 0.f, 0.f);
             continue;
         }
         if (!(wacc > 0.f)) continue;
@@ -1447,23 +1451,30 @@ __global__ __launch_bounds__(256) void k_store(const long long* __restrict__ ite
     }
 }
 
-// Backprojection through LDS bricks (3D references). In brick mode k_store leaves each item's
-// frame and per-pixel values (data re/im and weight: the item's translations summed, before the
-// trilinear insertion of backproject2Dto3D, backprojector.cpp:136-357) in bp_geom / bp_val, and
-// k_bp_brick inserts them: one workgroup per unit (a BP_S^3 voxel brick of one class's volume,
-// and a share of the batch's items), which collects the items whose plane crosses the brick,
-// inserts every pixel of theirs whose trilinear footprint meets the brick into an LDS copy of
-// it (LDS atomics), and adds the brick to the volume at the end: a plain read-modify-write when
-// one unit covers the brick, global atomics when its items are shared among several. The
-// per-pixel arithmetic (positions in double without contraction, the Friedel flip, the bounds
-// and the eight trilinear weights) is the direct scatter's; only the order of the FP32 sums
-// differs. A wave takes one item at a time and walks the pixels whose positions can fall in the
-// brick: the box of the plane coordinates (m0.q, m1.q) / |m0|^2 over the brick's corners (and
-// its mirror image, for pixels inserted at -q).
-#define BP_S 16
-#define BP_VOX (BP_S * BP_S * BP_S)
+// Backprojection through LDS bricks. In brick mode k_store leaves each item's frame and
+// per-pixel values (data re/im and weight: the item's translations summed, before the insertion
+// of backproject2Dto3D / backrotate2D, backprojector.cpp:136-357 / 568-643) in bp_geom / bp_val,
+// and k_bp_brick inserts them: one workgroup per unit (a brick of one class's volume, BP_S3^3
+// voxels for 3D references, a tile of BP_S2^2 for 2D ones, and a share of the batch's items),
+// which collects the items that can reach the brick (3D: whose plane crosses it; 2D: every item
+// of the class), inserts every pixel of theirs whose interpolation footprint meets the brick into
+// an LDS copy of it (LDS atomics), and adds the brick to the volume at the end: a plain
+// read-modify-write when one unit covers the brick, global atomics when its items are shared
+// among several. The per-pixel arithmetic (positions in double without contraction, the Friedel
+// flip, the bounds and the trilinear / bilinear weights) is the direct scatter's; only the order
+// of the FP32 sums differs. A wave takes one item at a time and walks the pixels whose positions
+// can fall in the brick: the box of their frequencies (x, y) = M^T q / |m0|^2 (M = pf * the
+// inverse rotation's first columns, m0 its first column) over the brick's corners, and its mirror
+// image for the pixels inserted at -q.
+#define BP_S3 16
+#define BP_S2 32
 #define BP_LIST 1024
 #define BP_THREADS 512
+
+template <int D3> struct BpDims {
+    static constexpr int S = D3 ? BP_S3 : BP_S2;
+    static constexpr int VOX = D3 ? BP_S3 * BP_S3 * BP_S3 : BP_S2 * BP_S2;
+};
 
 struct BpBrickParams {
     const long long* item_off;
@@ -1471,33 +1482,48 @@ struct BpBrickParams {
     long long item_cap;
     const float4* filt;
     const double* geom;
-    const float4* val;
-    const int* pix_index;      // [(y + py0) * pxw + x]: compact pixel of frequency (x, y), or -1
+    const float4* val;         // [item][pyh][pxw]: the values of frequency (x, y) at row y + py0
     int py0, pxw, pyh;
-    int npix;
     float2* const* bp_data;
     float* const* bp_weight;
     const int* bp_pads;
+    const int* cls_off;        // [K+1] the batch's items grouped by class (k_bp_bucket)
+    const int* cls_items;
 };
 
-struct BpUnit {
-    int c, bx, by, bz;   // class, brick origin / BP_S in voxel index space (x, y, z)
-    int g, G;            // this unit's share: the items i with i % G == g
-    int pad0, pad1;
+// A brick of a class's volume within reach of the insertion radius. Its items are split into
+// G shares (units of work) by the batch's count of items of that class: G = ceil(reach * count /
+// unit_items), reach the expected fraction of the class's items that reach the brick.
+struct BpBrick {
+    int c, bx, by, bz;   // class, brick origin / brick side in voxel index space (x, y, z; 2D: z 0)
+    float reach;
+    int pad0, pad1, pad2;
 };
 
+template <int D3>
 __device__ __forceinline__ void bp_insert_item(const BpBrickParams& P, int item, int bxdim, int bpad, int bh,
-                                               int x_lo, int y_lo, int z_lo, const double (&lo)[3],
+                                               const int (&blo)[3], const double (&lo)[3],
                                                const double (&hi)[3], float* s_re, float* s_im, float* s_w,
                                                int lane)
 {
+    constexpr int S = BpDims<D3>::S;
     const double* gm = P.geom + (size_t)item * 6;
     const double a0 = gm[0], a1 = gm[1], a2 = gm[2], a3 = gm[3], a4 = gm[4], a5 = gm[5];
-    const double s2 = a0 * a0 + a2 * a2 + a4 * a4;
-    const double u_lo = (fmin(a0 * lo[0], a0 * hi[0]) + fmin(a2 * lo[1], a2 * hi[1]) + fmin(a4 * lo[2], a4 * hi[2])) / s2;
-    const double u_hi = (fmax(a0 * lo[0], a0 * hi[0]) + fmax(a2 * lo[1], a2 * hi[1]) + fmax(a4 * lo[2], a4 * hi[2])) / s2;
-    const double v_lo = (fmin(a1 * lo[0], a1 * hi[0]) + fmin(a3 * lo[1], a3 * hi[1]) + fmin(a5 * lo[2], a5 * hi[2])) / s2;
-    const double v_hi = (fmax(a1 * lo[0], a1 * hi[0]) + fmax(a3 * lo[1], a3 * hi[1]) + fmax(a5 * lo[2], a5 * hi[2])) / s2;
+    double s2, u_lo, u_hi, v_lo, v_hi;
+    if constexpr (D3) {
+        s2 = a0 * a0 + a2 * a2 + a4 * a4;
+        u_lo = fmin(a0 * lo[0], a0 * hi[0]) + fmin(a2 * lo[1], a2 * hi[1]) + fmin(a4 * lo[2], a4 * hi[2]);
+        u_hi = fmax(a0 * lo[0], a0 * hi[0]) + fmax(a2 * lo[1], a2 * hi[1]) + fmax(a4 * lo[2], a4 * hi[2]);
+        v_lo = fmin(a1 * lo[0], a1 * hi[0]) + fmin(a3 * lo[1], a3 * hi[1]) + fmin(a5 * lo[2], a5 * hi[2]);
+        v_hi = fmax(a1 * lo[0], a1 * hi[0]) + fmax(a3 * lo[1], a3 * hi[1]) + fmax(a5 * lo[2], a5 * hi[2]);
+    } else {
+        s2 = a0 * a0 + a2 * a2;
+        u_lo = fmin(a0 * lo[0], a0 * hi[0]) + fmin(a2 * lo[1], a2 * hi[1]);
+        u_hi = fmax(a0 * lo[0], a0 * hi[0]) + fmax(a2 * lo[1], a2 * hi[1]);
+        v_lo = fmin(a1 * lo[0], a1 * hi[0]) + fmin(a3 * lo[1], a3 * hi[1]);
+        v_hi = fmax(a1 * lo[0], a1 * hi[0]) + fmax(a3 * lo[1], a3 * hi[1]);
+    }
+    u_lo /= s2; u_hi /= s2; v_lo /= s2; v_hi /= s2;
     const double e = 1e-6;
     const int ymin = -P.py0, ymax = P.pyh - 1 - P.py0;
     // pixels inserted at +q (xp >= 0), then at -q (xp < 0)
@@ -1507,102 +1533,199 @@ __device__ __forceinline__ void bp_insert_item(const BpBrickParams& P, int item,
     const int by0 = max(ymin, (int)ceil(-v_hi - e)), by1 = min(ymax, (int)floor(-v_lo + e));
     const int nax = max(0, ax1 - ax0 + 1), nbx = max(0, bx1 - bx0 + 1);
     const int na = nax * max(0, ay1 - ay0 + 1), nb = nbx * max(0, by1 - by0 + 1);
-    const float4* val = P.val + (size_t)item * P.npix;
+    const float4* val = P.val + (size_t)item * P.pyh * P.pxw;
     for (int k = lane; k < na + nb; k += 64) {
         int x, y;
         bool flip;
         if (k < na) { y = ay0 + k / nax; x = ax0 + k % nax; flip = false; }
         else { const int kk = k - na; y = by0 + kk / nbx; x = bx0 + kk % nbx; flip = true; }
-        const int p = P.pix_index[(y + P.py0) * P.pxw + x];
-        if (p < 0) continue;
-        double xp, yp, zp;
+        double xp, yp, zp = 0.;
         {
 #pragma clang fp contract(off)
             xp = a0 * x + a1 * y;
             yp = a2 * x + a3 * y;
-            zp = a4 * x + a5 * y;
+            if constexpr (D3) zp = a4 * x + a5 * y;
         }
         const bool neg = xp < 0;
         if (neg != flip) continue;   // inserted in the other walk (or by another brick)
-        const float4 v = val[p];
-        if (!(v.z > 0.f)) continue;
         if (neg) { xp = -xp; yp = -yp; zp = -zp; }
-        const double fx0 = floor(xp), fy0 = floor(yp), fz0 = floor(zp);
-        const int x0 = (int)fx0, y0 = (int)fy0 + bh, z0 = (int)fz0 + bh;
-        if (x0 < 0 || x0 + 1 >= bxdim || y0 < 0 || y0 + 1 >= bpad || z0 < 0 || z0 + 1 >= bpad) continue;
-        if (x0 + 1 < x_lo || x0 >= x_lo + BP_S || y0 + 1 < y_lo || y0 >= y_lo + BP_S || z0 + 1 < z_lo ||
-            z0 >= z_lo + BP_S)
-            continue;
-        const double fx = xp - fx0, fy = yp - fy0, fz = zp - fz0;
-        const double mfx = 1. - fx, mfy = 1. - fy, mfz = 1. - fz;
+        const double fx0 = floor(xp), fy0 = floor(yp);
+        const int x0 = (int)fx0, y0 = (int)fy0 + bh;
+        if (x0 < 0 || x0 + 1 >= bxdim || y0 < 0 || y0 + 1 >= bpad) continue;
+        if (x0 + 1 < blo[0] || x0 >= blo[0] + S || y0 + 1 < blo[1] || y0 >= blo[1] + S) continue;
+        double fz0 = 0.;
+        int z0 = 0;
+        if constexpr (D3) {
+            fz0 = floor(zp);
+            z0 = (int)fz0 + bh;
+            if (z0 < 0 || z0 + 1 >= bpad || z0 + 1 < blo[2] || z0 >= blo[2] + S) continue;
+        }
+        // the pixel's values (zero for a frequency outside the pixel set or not inserted)
+        const float4 v = val[(y + P.py0) * P.pxw + x];
+        if (!(v.z > 0.f)) continue;
         const float vr = v.x, vi = neg ? -v.y : v.y, wacc = v.z;
-        const double wts[8] = {mfz * mfy * mfx, mfz * mfy * fx, mfz * fy * mfx, mfz * fy * fx,
-                               fz * mfy * mfx, fz * mfy * fx, fz * fy * mfx, fz * fy * fx};
+        const double fx = xp - fx0, fy = yp - fy0;
+        const double mfx = 1. - fx, mfy = 1. - fy;
+        if constexpr (D3) {
+            const double fz = zp - fz0, mfz = 1. - fz;
+            const double wts[8] = {mfz * mfy * mfx, mfz * mfy * fx, mfz * fy * mfx, mfz * fy * fx,
+                                   fz * mfy * mfx, fz * mfy * fx, fz * fy * mfx, fz * fy * fx};
 #pragma unroll
-        for (int q = 0; q < 8; q++) {
-            const int lx = x0 + (q & 1) - x_lo, ly = y0 + ((q >> 1) & 1) - y_lo, lz = z0 + (q >> 2) - z_lo;
-            if ((unsigned)lx >= BP_S || (unsigned)ly >= BP_S || (unsigned)lz >= BP_S) continue;
-            const int li = (lz * BP_S + ly) * BP_S + lx;
-            const float wk = (float)wts[q];
-            atomicAdd(&s_re[li], wk * vr);
-            atomicAdd(&s_im[li], wk * vi);
-            atomicAdd(&s_w[li], wk * wacc);
+            for (int q = 0; q < 8; q++) {
+                const int lx = x0 + (q & 1) - blo[0], ly = y0 + ((q >> 1) & 1) - blo[1], lz = z0 + (q >> 2) - blo[2];
+                if ((unsigned)lx >= S || (unsigned)ly >= S || (unsigned)lz >= S) continue;
+                const int li = (lz * S + ly) * S + lx;
+                const float wk = (float)wts[q];
+                atomicAdd(&s_re[li], wk * vr);
+                atomicAdd(&s_im[li], wk * vi);
+                atomicAdd(&s_w[li], wk * wacc);
+            }
+        } else {
+            const double wts[4] = {mfy * mfx, mfy * fx, fy * mfx, fy * fx};
+#pragma unroll
+            for (int q = 0; q < 4; q++) {
+                const int lx = x0 + (q & 1) - blo[0], ly = y0 + (q >> 1) - blo[1];
+                if ((unsigned)lx >= S || (unsigned)ly >= S) continue;
+                const int li = ly * S + lx;
+                const float wk = (float)wts[q];
+                atomicAdd(&s_re[li], wk * vr);
+                atomicAdd(&s_im[li], wk * vi);
+                atomicAdd(&s_w[li], wk * wacc);
+            }
         }
     }
 }
 
-__global__ __launch_bounds__(BP_THREADS) void k_bp_brick(BpBrickParams P, const BpUnit* __restrict__ units)
+// The batch's brick items grouped by class (a counting sort in one workgroup: counts, their
+// scan, then each item placed at its class's next free slot), so that a unit reads the items of
+// its own class only. Items with nothing to insert (class -1) are left out.
+// Then the units: each brick's share count G from its class's count, their exclusive scan
+// boff[0..nbricks] (boff[nbricks]: the units of the batch), and boff[nbricks + 1], the work-queue
+// counter of k_bp_brick, cleared.
+#define BP_KMAX 8192
+#define BP_GMAX 256
+__global__ __launch_bounds__(1024) void k_bp_bucket(const long long* __restrict__ item_off, int B, long long item_cap,
+                                                    const float4* __restrict__ filt, int K, int* __restrict__ cls_off,
+                                                    int* __restrict__ cls_items, const BpBrick* __restrict__ bricks,
+                                                    int nbricks, float unit_items, int* __restrict__ boff)
 {
-    __shared__ float s_re[BP_VOX], s_im[BP_VOX], s_w[BP_VOX];
+    __shared__ int cnt[BP_KMAX];
+    __shared__ int part[1024];
+    const long long total = min(item_off[B], item_cap);
+    for (int k = threadIdx.x; k < K; k += blockDim.x) cnt[k] = 0;
+    __syncthreads();
+    for (long long i = threadIdx.x; i < total; i += blockDim.x) {
+        const int c = __float_as_int(filt[i].w);
+        if (c >= 0) atomicAdd(&cnt[c], 1);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int k = 0; k < K; k++) {
+            const int v = cnt[k];
+            cnt[k] = run;
+            cls_off[k] = run;
+            run += v;
+        }
+        cls_off[K] = run;
+    }
+    __syncthreads();
+    for (long long i = threadIdx.x; i < total; i += blockDim.x) {
+        const int c = __float_as_int(filt[i].w);
+        if (c >= 0) cls_items[atomicAdd(&cnt[c], 1)] = (int)i;
+    }
+    __syncthreads();   // cls_off is complete
+    // units: thread t takes bricks [t * per, (t + 1) * per)
+    const int per = (nbricks + blockDim.x - 1) / blockDim.x;
+    const int j0 = min(nbricks, (int)threadIdx.x * per), j1 = min(nbricks, j0 + per);
+    int sum = 0;
+    for (int j = j0; j < j1; j++) {
+        const BpBrick br = bricks[j];
+        const int n = cls_off[br.c + 1] - cls_off[br.c];
+        const int G = n == 0 ? 0 : max(1, min(BP_GMAX, (int)ceilf(br.reach * (float)n / unit_items)));
+        boff[j] = G;   // counts, scanned below by the same thread
+        sum += G;
+    }
+    part[threadIdx.x] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int run = 0;
+        for (int t = 0; t < (int)blockDim.x; t++) {
+            const int v = part[t];
+            part[t] = run;
+            run += v;
+        }
+        boff[nbricks] = run;
+        boff[nbricks + 1] = 0;
+    }
+    __syncthreads();
+    int run = part[threadIdx.x];
+    for (int j = j0; j < j1; j++) {
+        const int G = boff[j];
+        boff[j] = run;
+        run += G;
+    }
+}
+
+// One unit: share g of G of the items of brick br's class, inserted into an LDS copy of the
+// brick, which is then added to the volume.
+template <int D3>
+__device__ void bp_brick_unit(const BpBrickParams& P, const BpBrick& br, int g, int G)
+{
+    constexpr int S = BpDims<D3>::S, VOX = BpDims<D3>::VOX;
+    __shared__ float s_re[VOX], s_im[VOX], s_w[VOX];
     __shared__ int s_list[BP_LIST];
     __shared__ int s_n;
-    // the batch's items below item_cap (k_store scattered the others directly)
-    const long long total = min(P.item_off[P.B], P.item_cap);
-    if (total == 0) return;
-    const BpUnit u = units[blockIdx.x];
-    const int bpad = P.bp_pads[u.c], bxdim = bpad / 2 + 1, bh = bpad / 2;
-    const int x_lo = u.bx * BP_S, y_lo = u.by * BP_S, z_lo = u.bz * BP_S;
-    for (int v = threadIdx.x; v < BP_VOX; v += blockDim.x) { s_re[v] = 0.f; s_im[v] = 0.f; s_w[v] = 0.f; }
+    const int bpad = P.bp_pads[br.c], bxdim = bpad / 2 + 1, bh = bpad / 2;
+    const int blo[3] = {br.bx * S, br.by * S, br.bz * S};
+    for (int v = threadIdx.x; v < VOX; v += blockDim.x) { s_re[v] = 0.f; s_im[v] = 0.f; s_w[v] = 0.f; }
     if (threadIdx.x == 0) s_n = 0;
     // positions (centred) whose footprint [floor, floor + 1] meets the brick: [lo, hi) per axis
-    const double lo[3] = {(double)(x_lo - 1), (double)(y_lo - bh - 1), (double)(z_lo - bh - 1)};
-    const double hi[3] = {(double)(x_lo + BP_S), (double)(y_lo - bh + BP_S), (double)(z_lo - bh + BP_S)};
+    const double lo[3] = {(double)(blo[0] - 1), (double)(blo[1] - bh - 1), (double)(blo[2] - bh - 1)};
+    const double hi[3] = {(double)(blo[0] + S), (double)(blo[1] - bh + S), (double)(blo[2] - bh + S)};
     const float flo0 = (float)lo[0], flo1 = (float)lo[1], flo2 = (float)lo[2];
     const float fhi0 = (float)hi[0], fhi1 = (float)hi[1], fhi2 = (float)hi[2];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6, nwaves = blockDim.x >> 6;
     __syncthreads();
-    for (long long base = 0; base < total; base += blockDim.x) {
-        const long long i = base + threadIdx.x;
-        if (i < total && (int)(i % u.G) == u.g) {
+    // the items of the unit's class, its share of them
+    const int beg = P.cls_off[br.c], end = P.cls_off[br.c + 1];
+    for (int base = beg; base < end; base += blockDim.x) {
+        const int idx = base + threadIdx.x;
+        if (idx < end && (idx - beg) % G == g) {
+            const int i = P.cls_items[idx];
             const float4 f = P.filt[i];
-            if (__float_as_int(f.w) == u.c) {
-                // the plane n.q = 0 meets the box when n.q takes both signs over its corners
-                const float dmin = fminf(f.x * flo0, f.x * fhi0) + fminf(f.y * flo1, f.y * fhi1) +
-                                   fminf(f.z * flo2, f.z * fhi2);
-                const float dmax = fmaxf(f.x * flo0, f.x * fhi0) + fmaxf(f.y * flo1, f.y * fhi1) +
-                                   fmaxf(f.z * flo2, f.z * fhi2);
-                if (dmin <= 0.01f && dmax >= -0.01f) s_list[atomicAdd(&s_n, 1)] = (int)i;
+            {
+                bool take = true;
+                if constexpr (D3) {
+                    // the plane n.q = 0 meets the box when n.q takes both signs over its corners
+                    const float dmin = fminf(f.x * flo0, f.x * fhi0) + fminf(f.y * flo1, f.y * fhi1) +
+                                       fminf(f.z * flo2, f.z * fhi2);
+                    const float dmax = fmaxf(f.x * flo0, f.x * fhi0) + fmaxf(f.y * flo1, f.y * fhi1) +
+                                       fmaxf(f.z * flo2, f.z * fhi2);
+                    take = dmin <= 0.01f && dmax >= -0.01f;
+                }
+                if (take) s_list[atomicAdd(&s_n, 1)] = i;
             }
         }
         __syncthreads();
         const int n = s_n;
         __syncthreads();   // every thread has read the count before the next appends
-        if (n > BP_LIST - BP_THREADS || base + blockDim.x >= total) {
+        if (n > BP_LIST - BP_THREADS || base + (int)blockDim.x >= end) {
             for (int e = wave; e < n; e += nwaves)
-                bp_insert_item(P, s_list[e], bxdim, bpad, bh, x_lo, y_lo, z_lo, lo, hi, s_re, s_im, s_w, lane);
+                bp_insert_item<D3>(P, s_list[e], bxdim, bpad, bh, blo, lo, hi, s_re, s_im, s_w, lane);
             __syncthreads();
             if (threadIdx.x == 0) s_n = 0;
             __syncthreads();
         }
     }
-    float2* bpd = P.bp_data[u.c];
-    float* bpw = P.bp_weight[u.c];
-    for (int v = threadIdx.x; v < BP_VOX; v += blockDim.x) {
+    float2* bpd = P.bp_data[br.c];
+    float* bpw = P.bp_weight[br.c];
+    for (int v = threadIdx.x; v < VOX; v += blockDim.x) {
         const float r = s_re[v], im = s_im[v], w = s_w[v];
         if (r == 0.f && im == 0.f && w == 0.f) continue;
-        const int lx = v % BP_S, ly = (v / BP_S) % BP_S, lz = v / (BP_S * BP_S);
-        const size_t gi = ((size_t)(z_lo + lz) * bpad + (size_t)(y_lo + ly)) * bxdim + (size_t)(x_lo + lx);
-        if (u.G == 1) {
+        const int lx = v % S, ly = (v / S) % S, lz = D3 ? v / (S * S) : 0;
+        const size_t gi = ((size_t)(blo[2] + lz) * bpad + (size_t)(blo[1] + ly)) * bxdim + (size_t)(blo[0] + lx);
+        if (G == 1) {
             const float2 o = bpd[gi];
             bpd[gi] = make_float2(o.x + r, o.y + im);
             bpw[gi] += w;
@@ -1611,6 +1734,31 @@ __global__ __launch_bounds__(BP_THREADS) void k_bp_brick(BpBrickParams P, const 
             atomicAdd(&bpd[gi].y, im);
             atomicAdd(&bpw[gi], w);
         }
+    }
+}
+
+// One persistent launch: each workgroup takes the next unit from the queue (boff[nbricks + 1],
+// cleared by k_bp_bucket) until the batch's units (boff[nbricks]) run out.
+template <int D3>
+__global__ __launch_bounds__(BP_THREADS) void k_bp_brick(BpBrickParams P, const BpBrick* __restrict__ bricks,
+                                                         int nbricks, int* __restrict__ boff)
+{
+    __shared__ int s_unit;
+    const int nunits = boff[nbricks];
+    for (;;) {
+        __syncthreads();   // every thread is done with the previous unit's LDS
+        if (threadIdx.x == 0) s_unit = atomicAdd(&boff[nbricks + 1], 1);
+        __syncthreads();
+        const int unit = s_unit;
+        if (unit >= nunits) break;
+        int lo = 0, hi = nbricks;   // boff[lo] <= unit < boff[hi]
+        while (hi - lo > 1) {
+            const int mid = (lo + hi) >> 1;
+            if (boff[mid] <= unit) lo = mid;
+            else hi = mid;
+        }
+        const BpBrick br = bricks[lo];
+        bp_brick_unit<D3>(P, br, unit - boff[lo], boff[lo + 1] - boff[lo]);
     }
 }
 
@@ -2039,14 +2187,10 @@ static int upload_table(PassTables& t, hipStream_t st)
     T2X_HIP(hipMemcpyAsync(t.d_minv.ptr, t.h_minv.data(), sizeof(float) * t.h_minv.size(), hipMemcpyHostToDevice, st));
     T2X_HIP(hipMemcpyAsync(t.d_ires.ptr, t.h_ires.data(), sizeof(int) * t.npix, hipMemcpyHostToDevice, st));
     T2X_HIP(hipMemcpyAsync(t.d_inproj.ptr, t.h_inproj.data(), t.npix, hipMemcpyHostToDevice, st));
-    // frequency -> compact pixel: x in [0, size/2], y in [-size/2, size/2]
+    // the frequency window of the pixels: x in [0, size/2], y in [-size/2, size/2]
     t.px_w = t.size / 2 + 1;
     t.py0 = t.size / 2;
     t.py_h = t.size + 1;
-    std::vector<int> idx((size_t)t.px_w * t.py_h, -1);
-    for (int i = 0; i < t.npix; i++) idx[(size_t)(t.h_y[i] + t.py0) * t.px_w + t.h_x[i]] = i;
-    T2X_TRY(t.d_pixidx.ensure(sizeof(int) * idx.size()));
-    T2X_HIP(hipMemcpyAsync(t.d_pixidx.ptr, idx.data(), sizeof(int) * idx.size(), hipMemcpyHostToDevice, st));
     T2X_HIP(hipStreamSynchronize(st));
     return T2X_OK;
 }
@@ -2558,58 +2702,64 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     stt.P_coarse = tc.npix;
     stt.P_fine = tf.npix;
 
-    // ---- brick backprojection (3D references): the call's work units and buffers ----
-    const bool bp_bricks = !ref2d && !cfg->bp_direct;
+    // ---- brick backprojection: the call's work units and buffers ----
+    const bool bp_bricks = !cfg->bp_direct && K <= BP_KMAX;
     long long bp_item_cap = 0;
-    int bp_nunits = 0;
+    int bp_nbricks = 0;
+    float bp_unit_items = 0.f;
     if (bp_bricks) {
         // a batch's items held for the bricks: up to 256 per particle and 2 GiB of pixel values
         // (the items beyond are scattered directly)
         const int Bc = std::min(Bmax, n);
-        bp_item_cap = std::min<long long>((long long)Bc * 256,
-                                          std::max<long long>(1, (2ll << 30) / ((long long)tf.npix * 16)));
+        const long long win = (long long)tf.px_w * tf.py_h;   // frequencies of the value layout
+        bp_item_cap = std::min<long long>((long long)Bc * 256, std::max<long long>(1, (2ll << 30) / (win * 16)));
         const char* env_cap = getenv("T2X_BP_ITEM_CAP");   // tests: a small capacity sends items direct
         if (env_cap && atoll(env_cap) > 0) bp_item_cap = atoll(env_cap);
-        // each unit takes about unit_items of the items that cross its brick: the items of a class
-        // in a batch (from the last batch's items per particle, else two orientations per particle)
-        // times the share of planes through the origin that cross a brick at distance r from it,
-        // about (half-diagonal) / r
-        double unit_items = 384.;
+        // the bricks within reach of the insertion radius, each with the expected share of its
+        // class's items that reach it: for a 3D brick at distance r from the origin the share of
+        // planes through the origin that cross it, about (half-diagonal) / r; every item of a
+        // 2D class reaches all its tiles. Per batch a unit of work takes about unit_items of the
+        // items that reach its brick.
+        double unit_items = 32.;
         const char* env_ui = getenv("T2X_BP_UNIT_ITEMS");
         if (env_ui && atof(env_ui) > 0.) unit_items = atof(env_ui);
-        const double ipp = d->bp_items_pp > 0. ? d->bp_items_pp : 2. * over_rot;
-        const double items_class = ipp * Bc / K;
-        const double hd = 0.5 * std::sqrt(3.) * (BP_S + 1);
-        std::vector<BpUnit> units;
+        const int S = ref2d ? BP_S2 : BP_S3;
+        const double hd = 0.5 * std::sqrt(3.) * (S + 1);
+        std::vector<BpBrick> bricks;
         for (int c = 0; c < K; c++) {
             const int bpad = h_bpads[c], bxdim = bpad / 2 + 1, bh = bpad / 2;
-            const int rr = relion_round(h_bprmax[c] * pf);
-            const int nbx = (bxdim + BP_S - 1) / BP_S, nby = (bpad + BP_S - 1) / BP_S;
-            for (int bz = 0; bz < nby; bz++)
+            // the insertion radius (backrotate2D truncates r_max * pf, backproject2Dto3D rounds it)
+            const int rr = ref2d ? (int)(h_bprmax[c] * pf) : relion_round(h_bprmax[c] * pf);
+            const int nbx = (bxdim + S - 1) / S, nby = (bpad + S - 1) / S, nbz = ref2d ? 1 : nby;
+            for (int bz = 0; bz < nbz; bz++)
                 for (int by = 0; by < nby; by++)
                     for (int bx = 0; bx < nbx; bx++) {
                         // positions (centred) inserted into the brick: [lo, hi) per axis
-                        const double lo[3] = {bx * BP_S - 1., by * BP_S - bh - 1., bz * BP_S - bh - 1.};
-                        const double hi[3] = {bx * BP_S + (double)BP_S, by * BP_S - bh + (double)BP_S,
-                                              bz * BP_S - bh + (double)BP_S};
+                        const int nd = ref2d ? 2 : 3;
+                        const double lo[3] = {bx * S - 1., by * S - bh - 1., bz * S - bh - 1.};
+                        const double hi[3] = {bx * S + (double)S, by * S - bh + (double)S, bz * S - bh + (double)S};
                         double dn2 = 0., dc2 = 0.;
-                        for (int k = 0; k < 3; k++) {
+                        for (int k = 0; k < nd; k++) {
                             const double q = std::max(lo[k], std::min(0., hi[k]));   // nearest to the origin
                             const double m = 0.5 * (lo[k] + hi[k]);
                             dn2 += q * q;
                             dc2 += m * m;
                         }
                         if (dn2 > (double)(rr + 1) * (rr + 1)) continue;   // beyond r_max: nothing lands
-                        const double f = std::min(1., hd / std::max(1., std::sqrt(dc2)));
-                        const int G = std::max(1, std::min(16, (int)std::ceil(f * items_class / unit_items)));
-                        for (int g2 = 0; g2 < G; g2++) units.push_back(BpUnit{c, bx, by, bz, g2, G, 0, 0});
+                        const double f = ref2d ? 1. : std::min(1., hd / std::max(1., std::sqrt(dc2)));
+                        bricks.push_back(BpBrick{c, bx, by, bz, (float)f, 0, 0, 0});
                     }
         }
-        bp_nunits = (int)units.size();
-        T2X_TRY(upload(d->bp_units, units, st));
+        bp_nbricks = (int)bricks.size();
+        bp_unit_items = (float)unit_items;
+        T2X_TRY(upload(d->bp_units, bricks, st));
+        T2X_TRY(d->bp_boff.ensure(sizeof(int) * (bp_nbricks + 2)));
+        T2X_TRY(d->bp_cls.ensure(sizeof(int) * (K + 1 + bp_item_cap)));
         T2X_TRY(d->bp_filt.ensure(sizeof(float4) * bp_item_cap));
         T2X_TRY(d->bp_geom.ensure(sizeof(double) * 6 * bp_item_cap));
-        T2X_TRY(d->bp_val.ensure(sizeof(float4) * bp_item_cap * tf.npix));
+        // frequencies outside the pixel set are never written: zero for the call's layout
+        T2X_TRY(d->bp_val.ensure(sizeof(float4) * bp_item_cap * win));
+        T2X_TRY(zero_async(d->bp_val.ptr, sizeof(float4) * bp_item_cap * win, st));
     }
 
     for (int b0 = 0; b0 < n; b0 += Bmax) {
@@ -3479,6 +3629,9 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             P.bp_filt = d->bp_filt.as<float4>();
             P.bp_geom = d->bp_geom.as<double>();
             P.bp_val = d->bp_val.as<float4>();
+            P.bp_wx = tf.px_w;
+            P.bp_wy = tf.py_h;
+            P.bp_wy0 = tf.py0;
             P.item_cap = bp_item_cap;
         }
         if (xN > 520) { set_error("box too large for the shell accumulator"); return T2X_EINVAL; }
@@ -3488,7 +3641,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                                                            d_pair_off.as<long long>(), d->sig_pairs.as<int2>(), P,
                                                            &d_plan->queue[1]);
         T2X_HIP(hipGetLastError());
-        if (bp_bricks && bp_nunits > 0) {
+        if (bp_bricks && bp_nbricks > 0) {
             BpBrickParams Q{};
             Q.item_off = d_item_off.as<long long>();
             Q.B = B;
@@ -3496,15 +3649,23 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             Q.filt = d->bp_filt.as<float4>();
             Q.geom = d->bp_geom.as<double>();
             Q.val = d->bp_val.as<float4>();
-            Q.pix_index = tf.d_pixidx.as<int>();
             Q.py0 = tf.py0;
             Q.pxw = tf.px_w;
             Q.pyh = tf.py_h;
-            Q.npix = tf.npix;
             Q.bp_data = d_bpd.as<float2*>();
             Q.bp_weight = d_bpw.as<float*>();
             Q.bp_pads = d_bpads.as<int>();
-            k_bp_brick<<<(unsigned)bp_nunits, BP_THREADS, 0, st>>>(Q, d->bp_units.as<BpUnit>());
+            Q.cls_off = d->bp_cls.as<int>();
+            Q.cls_items = Q.cls_off + K + 1;
+            const BpBrick* d_bricks = d->bp_units.as<BpBrick>();
+            int* d_boff = d->bp_boff.as<int>();
+            k_bp_bucket<<<1, 1024, 0, st>>>(Q.item_off, B, bp_item_cap, Q.filt, K, d->bp_cls.as<int>(),
+                                            d->bp_cls.as<int>() + K + 1, d_bricks, bp_nbricks, bp_unit_items,
+                                            d_boff);
+            // workgroups that fit on a compute unit at once (3D: 52 KB of LDS, 2D: 16 KB) take the
+            // units from the queue
+            if (ref2d) k_bp_brick<0><<<(unsigned)(4 * d->n_cu), BP_THREADS, 0, st>>>(Q, d_bricks, bp_nbricks, d_boff);
+            else k_bp_brick<1><<<(unsigned)(2 * d->n_cu), BP_THREADS, 0, st>>>(Q, d_bricks, bp_nbricks, d_boff);
             T2X_HIP(hipGetLastError());
         }
         mark();   // 8: store done
@@ -3591,7 +3752,6 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         }   // attempt
         npairs = hplan.npairs;
         nitems = hplan.nitems;
-        if (B > 0) d->bp_items_pp = (double)nitems / B;
         if (bp_bricks) stt.bp_brick_items += (double)std::min(nitems, bp_item_cap);
         if (!cand_path) {
             stt.n_select_fallbacks += 1;

@@ -153,8 +153,7 @@ struct PassTables {
     DevBuf d_minv;         // float Minvsigma2 of each compact pixel, one row per optics group
     DevBuf d_ires;         // int shell index of each compact pixel
     DevBuf d_inproj;       // uchar: pixel inside the projected disc x^2+y^2 <= (size/2)^2
-    DevBuf d_pixidx;       // int [(y + py0) * pxw + x]: compact pixel of frequency (x, y), or -1
-    int px_w = 0, py0 = 0, py_h = 0;
+    int px_w = 0, py0 = 0, py_h = 0;   // frequency window (x, y + py0) of the pixels
     std::vector<int> h_x, h_y, h_win, h_ires;
     std::vector<float> h_minv;
     std::vector<unsigned char> h_inproj;
@@ -198,10 +197,9 @@ struct Device {
     // the batch's readback: its vectors packed into rb_dev by one kernel, then one copy to rb_host
     DevBuf rb_dev;
     HostBuf rb_host;
-    // brick backprojection: item planes, frames and pixel values, the call's work units, and the
-    // items per particle of the last batch (sizes the units' item shares)
-    DevBuf bp_filt, bp_geom, bp_val, bp_units;
-    double bp_items_pp = 0.;
+    // brick backprojection: item planes, frames and pixel values, the call's bricks, the batch's
+    // items by class and its units of work
+    DevBuf bp_filt, bp_geom, bp_val, bp_units, bp_cls, bp_boff;
     // launch_graph: the batch graph captured in call graph_call for batches of graph_B
     // particles (valid while alloc_generation() == graph_gen), and the batch sizes the current
     // call has run eagerly

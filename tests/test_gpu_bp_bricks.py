@@ -1,7 +1,7 @@
-"""Backprojection through LDS bricks (t2x_estep_config.bp_direct 0, the default for 3D
-references) against the direct scatter (bp_direct 1: each pixel's eight trilinear contributions
-added to the volume with global atomics, backproject2Dto3D, backprojector.cpp:136-357) and
-against the FP64 oracle.
+"""Backprojection through LDS bricks (t2x_estep_config.bp_direct 0, the default) against the
+direct scatter (bp_direct 1: each pixel's eight trilinear contributions, or four bilinear ones
+for 2D references, added to the volume with global atomics: backproject2Dto3D and backrotate2D,
+backprojector.cpp:136-357 and 568-643) and against the FP64 oracle.
 
 Both paths insert the same FP32 contributions (positions in double without contraction, the
 Friedel flip, the bounds and the trilinear weights are computed alike); only the order of the
@@ -12,7 +12,7 @@ weighted sums to 1e-9 (the run-to-run order of the store's FP64 atomics).
 
 The brick path has three forms, each checked against the direct scatter: one workgroup per
 brick that adds its LDS copy to the volume with a plain read-modify-write (T2X_BP_UNIT_ITEMS
-large), the items of each brick shared among up to 16 workgroups that add theirs with global
+large), the items of each brick shared among up to 256 workgroups that add theirs with global
 atomics (T2X_BP_UNIT_ITEMS 1), and the default mix (shared near the origin, where every plane
 passes). The items of a batch beyond the brick buffers (a capacity forced down to one item with
 T2X_BP_ITEM_CAP) are scattered directly, in the same launch; t2x_stats.bp_brick_items counts
@@ -59,7 +59,8 @@ def same_but_bp_order(g, ref, bp_tol=1e-6):
     return worst
 
 
-@pytest.mark.parametrize("name,n,K,batch", [("refine64", 22, 3, 8), ("class256", 12, 2, 6)])
+@pytest.mark.parametrize("name,n,K,batch", [("refine64", 22, 3, 8), ("class256", 12, 2, 6),
+                                             ("class2d128", 40, 5, 16)])
 @pytest.mark.parametrize("unit_items", ["", "1", "1000000000"])
 def test_bricks_equal_direct(gpu_device, monkeypatch, name, n, K, batch, unit_items):
     monkeypatch.setenv("T2X_BATCH", str(batch))
@@ -88,14 +89,15 @@ def test_items_beyond_capacity_scatter_directly(gpu_device, monkeypatch):
     same_but_bp_order(mixed, brick)
 
 
-def test_bricks_vs_oracle(gpu_device, monkeypatch):
+@pytest.mark.parametrize("name", ["refine64", "class2d128"])
+def test_bricks_vs_oracle(gpu_device, monkeypatch, name):
     monkeypatch.setenv("T2X_BATCH", "6")
-    wl = H.Workload("refine64", n=12, K=2, seed_offset=73)
+    wl = H.Workload(name, n=12, K=2, seed_offset=73)
     o = wl.run_oracle()
     g, cnt = run_counted(wl, gpu_device, False)
     assert cnt[0] == cnt[1]
     for c in range(2):
         e_d = H.rel_norm(g["bp"][c][0], o["bp"][c][0])
         e_w = H.rel_norm(g["bp"][c][1], o["bp"][c][1])
-        print(f"\n[bricks vs oracle] class {c}: data {e_d:.2e}, weight {e_w:.2e}")
+        print(f"\n[bricks vs oracle] {name} class {c}: data {e_d:.2e}, weight {e_w:.2e}")
         assert e_d < 1e-4 and e_w < 1e-4
