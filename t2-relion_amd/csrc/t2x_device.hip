@@ -7,6 +7,7 @@
 // trilinear interpolation, as the CPU path does (projector.cpp:585-640).
 #include "t2x_internal.h"
 #include <algorithm>
+#include <thread>
 
 namespace t2x {
 
@@ -348,11 +349,20 @@ extern "C" int t2x_backprojector_get(t2x_device* h, int iclass, double* re, doub
     T2X_HIP(hipMemcpyAsync(hd, c.bp_data, n * sizeof(float2), hipMemcpyDeviceToHost, d->stream));
     T2X_HIP(hipMemcpyAsync(hw, c.bp_weight, n * sizeof(float), hipMemcpyDeviceToHost, d->stream));
     T2X_HIP(hipStreamSynchronize(d->stream));
-    for (size_t i = 0; i < n; i++) {
-        if (re) re[i] = hd[i].x;
-        if (im) im[i] = hd[i].y;
-        if (w) w[i] = hw[i];
-    }
+    // FP32 -> FP64 into the caller's arrays, split over host threads (a 3D volume holds millions
+    // of voxels: one thread alone is held by memory latency)
+    auto convert = [&](size_t i0, size_t i1) {
+        for (size_t i = i0; i < i1; i++) {
+            if (re) re[i] = hd[i].x;
+            if (im) im[i] = hd[i].y;
+            if (w) w[i] = hw[i];
+        }
+    };
+    const size_t nthr = std::min<size_t>(8, std::max<size_t>(1, n >> 18));
+    std::vector<std::thread> th;
+    for (size_t t = 1; t < nthr; t++) th.emplace_back(convert, n * t / nthr, n * (t + 1) / nthr);
+    convert(0, n / nthr);
+    for (auto& x : th) x.join();
     return T2X_OK;
 }
 

@@ -88,15 +88,68 @@ __host__ __device__ static inline int relion_round(double x) { return (x > 0) ? 
 // Norm correction, integer re-centring with zero fill (selfTranslate DONT_WRAP on the rounded
 // old offset, ml_optimiser.cpp:6097-6098) and the soft circular mask (softMaskOutsideMap,
 // mask.cpp:43-97) whose background is the average value (--zero_mask, noise == null) or the
-// particle's noise image noise[b]. Writes the unmasked image to out[b] and the masked image to
-// out[B + b].
-__global__ __launch_bounds__(256) void k_prep_real(const float* __restrict__ img, int N, int B,
-                                                   const PartScalars* __restrict__ ps,
-                                                   double radius, double cosine_width,
-                                                   const float* __restrict__ noise,
-                                                   float* __restrict__ out)
+// particle's noise image noise[b]. Two launches over (particle, band of N / PREP_BANDS rows):
+// k_prep_bg sums the background weights and values of its band (pixels at r >= radius), and
+// k_prep_mask, with the particle's average from the bands' sums (in band order), writes the
+// unmasked image to out[b] and the masked image to out[B + b].
+#define PREP_BANDS 8
+
+__device__ __forceinline__ float prep_pixel(const float* __restrict__ src, int N, int i, int j, int sx, int sy,
+                                            float scale)
 {
-    const int b = blockIdx.x;
+    const int si = i - sy, sj = j - sx;
+    return (si >= 0 && si < N && sj >= 0 && sj < N) ? src[(size_t)si * N + sj] * scale : 0.f;
+}
+
+__global__ __launch_bounds__(256) void k_prep_bg(const float* __restrict__ img, int N,
+                                                 const PartScalars* __restrict__ ps, double radius,
+                                                 double cosine_width, double* __restrict__ part)
+{
+    const int b = blockIdx.x, band = blockIdx.y;
+    const int r0 = band * N / PREP_BANDS, r1 = (band + 1) * N / PREP_BANDS;
+    const float* src = img + (size_t)b * N * N;
+    const int sx = (int)ps[b].old_x, sy = (int)ps[b].old_y;
+    const float scale = (float)ps[b].norm_scale;
+    const double radius_p = radius + cosine_width;
+    // squared distances clear of both edges by half a pixel need no square root
+    const double in2 = radius > 0.5 ? (radius - 0.5) * (radius - 0.5) : -1., out2 = (radius_p + 0.5) * (radius_p + 0.5);
+    double sbg = 0., sw = 0.;
+    for (int i = r0; i < r1; i++) {
+        const double y = i - N / 2;
+        for (int j = threadIdx.x; j < N; j += blockDim.x) {
+            const double x = j - N / 2;
+            const double n2 = x * x + y * y;
+            if (n2 <= in2) continue;
+            const float v = prep_pixel(src, N, i, j, sx, sy, scale);
+            if (n2 >= out2) { sw += 1.; sbg += v; continue; }
+            const double r = sqrt(n2);
+            if (r < radius) continue;
+            if (r > radius_p) { sw += 1.; sbg += v; }
+            else {
+                const double rc = 0.5 + 0.5 * cos(kPi * (radius_p - r) / cosine_width);
+                sw += rc;
+                sbg += rc * v;
+            }
+        }
+    }
+    __shared__ double r1s[4], r2s[4];
+    for (int o = 32; o > 0; o >>= 1) { sbg += __shfl_xor(sbg, o); sw += __shfl_xor(sw, o); }
+    if ((threadIdx.x & 63) == 0) { r1s[threadIdx.x >> 6] = sbg; r2s[threadIdx.x >> 6] = sw; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double* o = part + ((size_t)b * PREP_BANDS + band) * 2;
+        o[0] = r1s[0] + r1s[1] + r1s[2] + r1s[3];
+        o[1] = r2s[0] + r2s[1] + r2s[2] + r2s[3];
+    }
+}
+
+__global__ __launch_bounds__(256) void k_prep_mask(const float* __restrict__ img, int N, int B,
+                                                   const PartScalars* __restrict__ ps, double radius,
+                                                   double cosine_width, const float* __restrict__ noise,
+                                                   const double* __restrict__ part, float* __restrict__ out)
+{
+    const int b = blockIdx.x, band = blockIdx.y;
+    const int r0 = band * N / PREP_BANDS, r1 = (band + 1) * N / PREP_BANDS;
     const size_t NN = (size_t)N * N;
     const float* src = img + (size_t)b * NN;
     float* um = out + (size_t)b * NN;
@@ -104,43 +157,35 @@ __global__ __launch_bounds__(256) void k_prep_real(const float* __restrict__ img
     const int sx = (int)ps[b].old_x, sy = (int)ps[b].old_y;
     const float scale = (float)ps[b].norm_scale;
     const double radius_p = radius + cosine_width;
-    double sbg = 0., sw = 0.;
-    for (size_t q = threadIdx.x; q < NN; q += blockDim.x) {
-        int i = (int)(q / N), j = (int)(q % N);
-        int si = i - sy, sj = j - sx;
-        float v = 0.f;
-        if (si >= 0 && si < N && sj >= 0 && sj < N) v = src[(size_t)si * N + sj] * scale;
-        um[q] = v;
-        if (noise) continue;
-        double y = i - N / 2, x = j - N / 2;
-        double r = sqrt(x * x + y * y);
-        if (r < radius) continue;
-        if (r > radius_p) { sw += 1.; sbg += v; }
-        else {
-            double rc = 0.5 + 0.5 * cos(kPi * (radius_p - r) / cosine_width);
-            sw += rc;
-            sbg += rc * v;
+    double bg = 0.;
+    if (!noise) {
+        double s1 = 0., s2 = 0.;
+        for (int k = 0; k < PREP_BANDS; k++) {
+            s1 += part[((size_t)b * PREP_BANDS + k) * 2];
+            s2 += part[((size_t)b * PREP_BANDS + k) * 2 + 1];
         }
+        bg = s1 / s2;
     }
-    __shared__ double r1[4], r2[4];
-    for (int o = 32; o > 0; o >>= 1) { sbg += __shfl_xor(sbg, o); sw += __shfl_xor(sw, o); }
-    if ((threadIdx.x & 63) == 0) { r1[threadIdx.x >> 6] = sbg; r2[threadIdx.x >> 6] = sw; }
-    __syncthreads();
-    __shared__ double bg;
-    if (threadIdx.x == 0) bg = noise ? 0. : (r1[0] + r1[1] + r1[2] + r1[3]) / (r2[0] + r2[1] + r2[2] + r2[3]);
-    __syncthreads();
     const float* nz = noise ? noise + (size_t)b * NN : nullptr;
-    for (size_t q = threadIdx.x; q < NN; q += blockDim.x) {
-        int i = (int)(q / N), j = (int)(q % N);
-        double y = i - N / 2, x = j - N / 2;
-        double r = sqrt(x * x + y * y);
-        float v = um[q];
-        const double add = nz ? (double)nz[q] : bg;
-        if (r < radius) mk[q] = v;
-        else if (r > radius_p) mk[q] = (float)add;
-        else {
-            double rc = 0.5 + 0.5 * cos(kPi * (radius_p - r) / cosine_width);
-            mk[q] = (float)((1 - rc) * v + rc * add);
+    const double in2 = radius > 0.5 ? (radius - 0.5) * (radius - 0.5) : -1., out2 = (radius_p + 0.5) * (radius_p + 0.5);
+    for (int i = r0; i < r1; i++) {
+        const double y = i - N / 2;
+        for (int j = threadIdx.x; j < N; j += blockDim.x) {
+            const size_t q = (size_t)i * N + j;
+            const double x = j - N / 2;
+            const double n2 = x * x + y * y;
+            const float v = prep_pixel(src, N, i, j, sx, sy, scale);
+            um[q] = v;
+            if (n2 <= in2) { mk[q] = v; continue; }
+            const double add = nz ? (double)nz[q] : bg;
+            if (n2 >= out2) { mk[q] = (float)add; continue; }
+            const double r = sqrt(n2);
+            if (r < radius) mk[q] = v;
+            else if (r > radius_p) mk[q] = (float)add;
+            else {
+                const double rc = 0.5 + 0.5 * cos(kPi * (radius_p - r) / cosine_width);
+                mk[q] = (float)((1 - rc) * v + rc * add);
+            }
         }
     }
 }
@@ -2463,11 +2508,23 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     const bool scale_sums = cfg->do_scale_correction && model->data_vs_prior_class;
     DevBuf& d_minv_fd = d->minv_fd;
     T2X_TRY(upload(d_minv_fd, minv_diff2, st));
-    // phase tables
-    std::vector<float2> abc = phase_table(tc, S.coarse_tx_px, S.coarse_ty_px, N);
-    std::vector<float2> abf = phase_table(tf, S.fine_tx_px, S.fine_ty_px, N);
-    T2X_TRY(upload(d->ab_coarse, abc, st));
-    T2X_TRY(upload(d->ab_fine, abf, st));
+    // phase tables (kept from the previous call when its sizes and translations were the same)
+    {
+        std::vector<double> key = {(double)N, (double)cc, (double)cs, (double)tc.npix, (double)tf.npix,
+                                   (double)S.coarse_tx_px.size(), (double)S.fine_tx_px.size()};
+        key.insert(key.end(), S.coarse_tx_px.begin(), S.coarse_tx_px.end());
+        key.insert(key.end(), S.coarse_ty_px.begin(), S.coarse_ty_px.end());
+        key.insert(key.end(), S.fine_tx_px.begin(), S.fine_tx_px.end());
+        key.insert(key.end(), S.fine_ty_px.begin(), S.fine_ty_px.end());
+        if (key != d->ab_key || !d->ab_coarse.ptr || !d->ab_fine.ptr) {
+            d->ab_key.clear();
+            std::vector<float2> abc = phase_table(tc, S.coarse_tx_px, S.coarse_ty_px, N);
+            std::vector<float2> abf = phase_table(tf, S.fine_tx_px, S.fine_ty_px, N);
+            T2X_TRY(upload(d->ab_coarse, abc, st));
+            T2X_TRY(upload(d->ab_fine, abf, st));
+            d->ab_key = key;
+        }
+    }
     // the fine pass's GEMM operand: phase rows with K padded to whole pixel chunks
     const int ldk_fine = (int)round_up(tf.npix, FT_KC) * 2;
     {
@@ -2594,17 +2651,26 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         T2X_TRY(d->coarse_B2.ensure(sizeof(float) * O_pad * Kn));
         T2X_HIP(hipMemsetAsync(d->coarse_B.ptr, 0, sizeof(float) * O_pad * Kc, st));
         T2X_HIP(hipMemsetAsync(d->coarse_B2.ptr, 0, sizeof(float) * O_pad * Kn, st));
-        std::vector<float> Ai(Norient * 9);
-        for (long dd = 0; dd < Nd; dd++)
-            for (long p = 0; p < Npsi; p++) {
-                double A[9], R[9];
-                euler_matrix_host(S.rot[dd], S.tilt[dd], S.psi[p], A);
-                inv3(A, R);
-                float* a = &Ai[(dd * Npsi + p) * 9];
-                for (int k = 0; k < 9; k++) a[k] = (float)(R[k] * pf);
-            }
-        DevBuf d_Ai;
-        T2X_TRY(upload(d_Ai, Ai, st));
+        // the projection frames pf * A^-1 of the orientations (kept while the angles are the same)
+        std::vector<double> akey = {pf, (double)Nd, (double)Npsi};
+        akey.insert(akey.end(), S.rot.begin(), S.rot.begin() + Nd);
+        akey.insert(akey.end(), S.tilt.begin(), S.tilt.begin() + Nd);
+        akey.insert(akey.end(), S.psi.begin(), S.psi.begin() + Npsi);
+        DevBuf& d_Ai = d->coarse_Ai;
+        if (akey != d->coarse_Ai_key || !d_Ai.ptr) {
+            d->coarse_Ai_key.clear();
+            std::vector<float> Ai(Norient * 9);
+            for (long dd = 0; dd < Nd; dd++)
+                for (long p = 0; p < Npsi; p++) {
+                    double A[9], R[9];
+                    euler_matrix_host(S.rot[dd], S.tilt[dd], S.psi[p], A);
+                    inv3(A, R);
+                    float* a = &Ai[(dd * Npsi + p) * 9];
+                    for (int k = 0; k < 9; k++) a[k] = (float)(R[k] * pf);
+                }
+            T2X_TRY(upload(d_Ai, Ai, st));
+            d->coarse_Ai_key = akey;
+        }
         k_project_coarse<<<(unsigned)O_all, 256, 0, st>>>(
             d_vols.as<const float2*>(), d_pads.as<int>(), d_Ai.as<float>(), (int)Norient,
             tc.d_pix.as<int2>(), tc.d_inproj.as<unsigned char>(), tc.npix, d->coarse_B.as<float>(), Kc,
@@ -2624,7 +2690,6 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                                  d->coarse_B2l.ptr, Kn, d->coarse_B2s.as<float>(), st));
         }
         T2X_HIP(hipStreamSynchronize(st));
-        d_Ai.release();
         d->coarse_key = key;
     }
     (void)hipEventRecord(d->ev[1], st);
@@ -2875,8 +2940,13 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             }
             d_noise = d->noise_img.as<float>();
         }
-        k_prep_real<<<B, 256, 0, st>>>(d_img, N, B, d_ps, cfg->particle_diameter / (2. * pix),
-                                      (double)cfg->width_mask_edge, d_noise, d->img2.as<float>());
+        T2X_TRY(d->prep_part.ensure(sizeof(double) * 2 * PREP_BANDS * B));
+        if (!d_noise)
+            k_prep_bg<<<dim3(B, PREP_BANDS), 256, 0, st>>>(d_img, N, d_ps, cfg->particle_diameter / (2. * pix),
+                                                          (double)cfg->width_mask_edge, d->prep_part.as<double>());
+        k_prep_mask<<<dim3(B, PREP_BANDS), 256, 0, st>>>(d_img, N, B, d_ps, cfg->particle_diameter / (2. * pix),
+                                                        (double)cfg->width_mask_edge, d_noise,
+                                                        d->prep_part.as<double>(), d->img2.as<float>());
         T2X_HIP(hipGetLastError());
         T2X_TRY(ensure_fft(d, N, 2 * Bfft));
         if (hipfftExecR2C(d->fft_plan, d->img2.as<hipfftReal>(), d->fft_out.as<hipfftComplex>()) != HIPFFT_SUCCESS) {

@@ -200,6 +200,11 @@ struct Device {
     // brick backprojection: item planes, frames and pixel values, the call's bricks, the batch's
     // items by class and its units of work
     DevBuf bp_filt, bp_geom, bp_val, bp_units, bp_cls, bp_boff;
+    // the phase tables' sizes and translations (ab_coarse / ab_fine hold them while unchanged)
+    std::vector<double> ab_key;
+    DevBuf prep_part;   // per particle and row band: background sums of the mask
+    DevBuf coarse_Ai;   // projection frames of the coarse orientations, for the angles in coarse_Ai_key
+    std::vector<double> coarse_Ai_key;
     // launch_graph: the batch graph captured in call graph_call for batches of graph_B
     // particles (valid while alloc_generation() == graph_gen), and the batch sizes the current
     // call has run eagerly
