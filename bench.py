@@ -75,6 +75,7 @@ KERNELS = {"fp32": "k_gemm_tn<EPI_DIFF2> (FP32 MFMA 16x16x4)",
            "bf16x3": "k_gemm_x3p<bf16, EPI_DIFF2> (BF16 MFMA 16x16x32, 3 products, 256x256 pipelined)",
            "fp16x3": "k_gemm_x3p<f16, EPI_DIFF2> (FP16 MFMA 16x16x32, 3 products, 256x256 pipelined)",
            "fp16x2": "k_gemm_x3p<f16, EPI_DIFF2, 2> (FP16 MFMA 16x16x32, 2 products, 256x256 pipelined)",
+           "fp16x2/259": "k_gemm_x2h (FP16 MFMA 16x16x32, 2 products, 128x256 tiles of 4 waves, 2 per CU)",
            "learned": "k_gemm_x3p<f16> on exhaustive-fallback rows + k_gemm_x3_bins<f16> on top-k bins "
                       "(FP16 MFMA 16x16x32, 3 products)",
            "learned_gated": "k_gemm_x3p<f16> on exhaustive-fallback rows + k_gemm_x3_bins<f16> on top-k bins "
@@ -681,7 +682,8 @@ def main():
             "traffic_unit": "bytes per launch", "traffic_source": traffic_src,
             "kernel": ("coarse diff2 of local searches: (particle, orientation) rows x translations, k_local_zrows + "
                        + ("k_gemm_tn (FP32 MFMA)" if head == "fp32" else "k_gemm_local_x3 (split MFMA, grouped)") + " + k_local_diff2" if local
-                       else "coarse diff2 cross term, " + KERNELS[hb]),
+                       else "coarse diff2 cross term, " + KERNELS.get(f"{hb}/{st.gemm_tile_used}", KERNELS[hb])),
+            "gemm_variant": [st.gemm_tile_used, st.gemm_group_used] if st.gemm_tile_used else None,
             "avg_launch_ms": avg_ms, "launches": launches, "flop_per_launch": flop_per_launch,
             "flop_convention": "algorithmic 4*T*O*P_c per scored particle (one complex product per pair)",
             "mfma_products_per_pair": nprod,

@@ -114,10 +114,10 @@ __global__ __launch_bounds__(256) void k_prep_bg(const float* __restrict__ img, 
     // squared distances clear of both edges by half a pixel need no square root
     const double in2 = radius > 0.5 ? (radius - 0.5) * (radius - 0.5) : -1., out2 = (radius_p + 0.5) * (radius_p + 0.5);
     double sbg = 0., sw = 0.;
-    for (int i = r0; i < r1; i++) {
-        const double y = i - N / 2;
-        for (int j = threadIdx.x; j < N; j += blockDim.x) {
-            const double x = j - N / 2;
+    for (int q = threadIdx.x; q < (r1 - r0) * N; q += blockDim.x) {
+        const int i = r0 + q / N, j = q - (q / N) * N;
+        {
+            const double y = i - N / 2, x = j - N / 2;
             const double n2 = x * x + y * y;
             if (n2 <= in2) continue;
             const float v = prep_pixel(src, N, i, j, sx, sy, scale);
@@ -168,11 +168,11 @@ __global__ __launch_bounds__(256) void k_prep_mask(const float* __restrict__ img
     }
     const float* nz = noise ? noise + (size_t)b * NN : nullptr;
     const double in2 = radius > 0.5 ? (radius - 0.5) * (radius - 0.5) : -1., out2 = (radius_p + 0.5) * (radius_p + 0.5);
-    for (int i = r0; i < r1; i++) {
-        const double y = i - N / 2;
-        for (int j = threadIdx.x; j < N; j += blockDim.x) {
+    for (int t = threadIdx.x; t < (r1 - r0) * N; t += blockDim.x) {
+        const int i = r0 + t / N, j = t - (t / N) * N;
+        {
             const size_t q = (size_t)i * N + j;
-            const double x = j - N / 2;
+            const double y = i - N / 2, x = j - N / 2;
             const double n2 = x * x + y * y;
             const float v = prep_pixel(src, N, i, j, sx, sy, scale);
             um[q] = v;
