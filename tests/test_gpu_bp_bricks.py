@@ -78,6 +78,20 @@ def test_bricks_equal_direct(gpu_device, monkeypatch, name, n, K, batch, unit_it
           f"{worst:.2e}, direct vs direct {base:.2e}")
 
 
+def test_bricks_equal_direct_local_search(gpu_device, monkeypatch):
+    """Local angular searches (each particle its own in-range orientations) insert through the
+    same bricks."""
+    from test_gpu_local import localise
+    monkeypatch.setenv("T2X_BATCH", "8")
+    wl = localise(H.Workload("refine64", n=20, seed_offset=74, K=2), 10.0)
+    direct, cnt_d = run_counted(wl, gpu_device, True)
+    brick, cnt_b = run_counted(wl, gpu_device, False)
+    assert cnt_d[0] == 0
+    assert cnt_b[0] == cnt_b[1]
+    worst = same_but_bp_order(brick, direct)
+    print(f"\n[bricks] local search: bricks vs direct {worst:.2e}")
+
+
 def test_items_beyond_capacity_scatter_directly(gpu_device, monkeypatch):
     monkeypatch.setenv("T2X_BATCH", "8")
     wl = H.Workload("refine64", n=16, K=2, seed_offset=72)
