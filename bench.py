@@ -190,12 +190,14 @@ def train_proposal(lib, dev, cfg, sz, ds, w, steps=2000, batch=256, n_proj=20000
 
 def configure(cfg, pname):
     """A path name is a precision / proposal setting, optionally with modifiers: "+graph" (each
-    batch's launches after the preprocessing run as a replayed hipGraph) and "+direct" (the 3D
-    backprojection scatters every pixel with global atomics instead of going through LDS bricks)."""
+    batch's launches after the preprocessing run as a replayed hipGraph), "+direct" (the 3D
+    backprojection scatters every pixel with global atomics instead of going through LDS bricks)
+    and "+exhaustive" (the coarse GEMM computes every tile, without the low-frequency bounds)."""
     base, *modes = pname.split("+")
     cfg.precision = PRECISIONS[base]
     cfg.launch_graph = 1 if "graph" in modes else 0
     cfg.bp_direct = 1 if "direct" in modes else 0
+    cfg.coarse_exhaustive = 1 if "exhaustive" in modes else 0
     topk, gate, conf = PROPOSAL.get(base, (0, 0, 0.0))
     cfg.proposal_topk = topk
     cfg.proposal_gate = gate
@@ -585,6 +587,8 @@ def main():
              "store_ms": rr["st"].ms_store / args.steps,
              "bp_items_through_bricks": (rr["st"].bp_brick_items / rr["st"].bp_orientations
                                          if rr["st"].bp_orientations > 0 else None),
+             "coarse_tiles_pruned": (rr["st"].coarse_tiles_pruned / rr["st"].coarse_tiles_checked
+                                     if rr["st"].coarse_tiles_checked > 0 else None),
              "gemm_variant_autotuned": ([rr["st"].gemm_tile_used, rr["st"].gemm_group_used]
                                         if rr["st"].gemm_tile_used else None),
              "pose_index_agreement_vs_fp32": (float((rr["out"]["best_hidden"] == runs["fp32"]["out"]["best_hidden"]).mean())

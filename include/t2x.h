@@ -30,7 +30,7 @@
 extern "C" {
 #endif
 
-#define T2X_ABI_VERSION 5
+#define T2X_ABI_VERSION 6
 
 enum {
     T2X_OK = 0,
@@ -135,6 +135,15 @@ typedef struct t2x_estep_config {
                                    /* of a 3D volume, 32^2 of a 2D one), as many of a    */
                                    /* batch's items as the brick buffers hold, the rest  */
                                    /* directly (t2x_stats.bp_brick_items)                */
+    int    coarse_exhaustive;      /* 1: the coarse GEMM computes every tile. 0 (global  */
+                                   /* searches in a split precision, without the        */
+                                   /* proposal or diagnostics): a first GEMM over the   */
+                                   /* lowest frequencies bounds each block's diff2 from */
+                                   /* below (every pixel adds a non-negative term), and */
+                                   /* the full GEMM skips the tiles whose blocks all lie */
+                                   /* beyond the negligible-weight cut above a computed */
+                                   /* value; the weights are those of the exhaustive    */
+                                   /* pass (t2x_stats.coarse_tiles_*)                    */
 } t2x_estep_config;
 
 /* Model arrays of the current iteration (host, caller-owned, read-only). */
@@ -253,6 +262,8 @@ typedef struct t2x_stats {
     double ms_graph;             /* launch_graph: GPU time of the batches run as a graph           */
     long   n_graph_launches;     /* batches run as a graph (captured, or replayed)                 */
     double bp_brick_items;       /* of bp_orientations, those inserted through LDS bricks          */
+    double coarse_tiles_checked; /* coarse GEMM tiles tested against the low-frequency bounds      */
+    double coarse_tiles_pruned;  /* of those, skipped (every block beyond the cut)                 */
 } t2x_stats;
 
 /* ---------------------------------------------------------------------------------------

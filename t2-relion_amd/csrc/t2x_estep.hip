@@ -43,6 +43,9 @@ int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const floa
                    long long a_rows, const void* Bh, const void* Bl, const float* Bs, long long ldb, int K,
                    int Mrows_pad, int Nrows_pad, const EpiParams& ep, hipStream_t st, GemmChoice want,
                    const int* rowmap, GemmChoice* used);
+int launch_gemm_h1_bounds(const void* Ah, const float* As, long long lda, long long a_rows, const void* Bh,
+                          const float* Bs, long long ldb, int K, int Mrows_pad, int Nrows_pad, const EpiParams& ep,
+                          hipStream_t st);
 int launch_gemm_local_x3(int prec, const void* Sh, const void* Sl, const float* Ss, const void* Wh, const void* Wl,
                          const float* Ws, long long ld, int K, const int2* items, int n_items, int T,
                          const long long* orient_off, const int* rowu, const double* cb, const double* n1,
@@ -280,12 +283,14 @@ __global__ __launch_bounds__(256) void k_prep_fourier(const float2* __restrict__
 __global__ __launch_bounds__(256) void k_const_term(const float2* __restrict__ Fimg, int Pw,
                                                     const int* __restrict__ win,
                                                     const float* __restrict__ minv, int npix,
-                                                    PartScalars* __restrict__ ps, int which)
+                                                    PartScalars* __restrict__ ps, int which, int np_low = 0)
 {
     const int b = blockIdx.x;
     const float* mv = minv + (size_t)ps[b].og * npix;   // the optics group's Minvsigma2
     double s = 0.;
-    for (int p = threadIdx.x; p < npix; p += blockDim.x) {
+    // which 2: the first np of the npix pixels (the minv rows keep their stride npix)
+    const int np = which == 2 ? np_low : npix;
+    for (int p = threadIdx.x; p < np; p += blockDim.x) {
         float2 x = Fimg[(size_t)b * Pw + win[p]];
         s += 0.5 * (double)mv[p] * ((double)x.x * x.x + (double)x.y * x.y);
     }
@@ -296,7 +301,8 @@ __global__ __launch_bounds__(256) void k_const_term(const float2* __restrict__ F
     if (threadIdx.x == 0) {
         double v = ps[b].highres_Xi2 / 2. + red[0] + red[1] + red[2] + red[3];
         if (which == 0) ps[b].C_coarse = v;
-        else ps[b].C_fine = v;
+        else if (which == 1) ps[b].C_fine = v;
+        else ps[b].C_low = v;   // over a low-frequency prefix of the coarse pixels
     }
 }
 
@@ -1928,6 +1934,106 @@ __global__ void k_gather_ccoarse(const PartScalars* __restrict__ ps, int B, doub
     if (i < B) out[i] = ps[i].C_coarse;
 }
 
+// Pruned coarse pass, the bounds pass's N1 table. The coarse diff2 over the pixels beyond the
+// low-frequency prefix (the high set h) is 1/2 sum_h minv |ctf F - X|^2 = Ch + N1h - G_h, with
+// Ch = 1/2 sum_h minv |X|^2 (C_coarse - C_low, the constant highres term cancelling), N1h =
+// 1/2 sum_h minv ctf^2 |F|^2 and G_h = sum_h minv Re(ctf F conj X); Cauchy-Schwarz bounds
+// G_h <= 2 sqrt(N1h Ch). So C_b + N1 - G_low - 2 sqrt(N1h Ch) is a lower bound of the full diff2,
+// and the bounds GEMM over the prefix gets it with N1 - 2 sqrt(N1h Ch) as its N1. The factor
+// 1 + 2^-8 covers the split operands' rounding (the FP16 reference side's 2^-11 relative) and the
+// FP32 sums. A bounds GEMM of one FP16 product per element (A hi . B hi) drops A lo . B hi (and,
+// under FP16X3, A hi . B lo): each |A lo| <= 2^-11 |A| (the rows are scaled to a maximum in
+// [0.5, 1), so normal FP16 but for negligible elements), so the dropped sum is at most
+// 2 . 2^-11 . sum |A||B| <= 2^-10 . 2 sqrt(N1 Cl), Cl = C_low - highres/2 the image energy of the
+// prefix (Cauchy-Schwarz again); eps1 = 2^-9 allows twice that, for the FP32 sums of both
+// passes. eps1 = 0 for a bounds GEMM in the full split precision. n1h holds N1h on entry (its GEMM
+// over the high set) and the table on return.
+__global__ void k_prune_n1(float* __restrict__ n1h, const float* __restrict__ N1, long long ldn, int O,
+                           const PartScalars* __restrict__ ps, long long total, double eps1)
+{
+    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= total) return;
+    const long long b = i / O, n = i % O;
+    const double ch = fmax(ps[b].C_coarse - ps[b].C_low, 0.);
+    const double cl = fmax(ps[b].C_low - ps[b].highres_Xi2 / 2., 0.);
+    const long long q = b * ldn + n;
+    const double nh = fmax((double)n1h[q], 0.), n1 = fmax((double)N1[q], 0.);
+    n1h[q] = (float)((double)N1[q] - 2. * (1. + 1. / 256.) * sqrt(nh * ch) - 2. * eps1 * sqrt(n1 * cl));
+}
+
+// Pruned coarse pass: per particle an upper bound of its coarse minimum diff2, the full diff2 of
+// the 64 columns of the (row, block) whose low-frequency bound is smallest (where the minimum
+// most likely lies), recomputed from the split operands in FP32 (the GEMM's products and scales,
+// summed in another order), plus a margin of 1 + 1e-4 |d| for that order; +inf where the
+// particle has no scored column. It seeds the pruning test and the lazy stores' running minima.
+template <class H>
+__global__ __launch_bounds__(256) void k_prune_seed(const float* __restrict__ lb, int lb_ld, int nblk, int T,
+                                                    const H* __restrict__ Ah, const H* __restrict__ Al,
+                                                    const float* __restrict__ As, const H* __restrict__ Bh,
+                                                    const H* __restrict__ Bl, const float* __restrict__ Bs,
+                                                    long long ld, int K, const double* __restrict__ Cb,
+                                                    const float* __restrict__ N1, long long ldn,
+                                                    const double* __restrict__ prior_o, long long N,
+                                                    float* __restrict__ seed, unsigned int* __restrict__ pmin)
+{
+    __shared__ float sv[256];
+    __shared__ long long si[256];
+    __shared__ float sd[64];
+    const int b = blockIdx.x, tid = threadIdx.x;
+    float best = 3.0e38f;
+    long long bi = -1;
+    const long long total = (long long)T * nblk;
+    for (long long q = tid; q < total; q += blockDim.x) {
+        const float v = lb[((long long)b * T + q / nblk) * lb_ld + q % nblk];
+        if (v < best) { best = v; bi = q; }
+    }
+    sv[tid] = best;
+    si[tid] = bi;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w && sv[tid + w] < sv[tid]) { sv[tid] = sv[tid + w]; si[tid] = si[tid + w]; }
+        __syncthreads();
+    }
+    const long long q = si[0];
+    if (q < 0) {
+        if (tid == 0) { seed[b] = __builtin_inff(); pmin[b] = float_to_ordered(__builtin_inff()); }
+        return;
+    }
+    const long long r = (long long)b * T + q / nblk;
+    const int c = tid >> 2, part = tid & 3;
+    const long long n = (q % nblk) * 64 + c;
+    float acc = 0.f;
+    if (n < N) {
+        const H* ah = Ah + r * ld;
+        const H* al = Al + r * ld;
+        const H* bh = Bh + n * ld;
+        const H* bl = Bl ? Bl + n * ld : nullptr;
+        for (int k = part; k < K; k += 4) {
+            const float av = (float)ah[k] + (float)al[k];
+            const float bv = bl ? (float)bh[k] + (float)bl[k] : (float)bh[k];
+            acc += av * bv;
+        }
+    }
+    acc += __shfl_xor(acc, 1);
+    acc += __shfl_xor(acc, 2);
+    if (part == 0) {
+        float d = 3.0e38f;
+        if (n < N && prior_o[n] > 0.) {
+            const float v = acc * (As ? As[r] : 1.f) * (Bs ? Bs[n] : 1.f);
+            d = (float)(Cb[b] + (double)N1[(long long)b * ldn + n] - (double)v);
+        }
+        sd[c] = d;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        float m = 3.0e38f;
+        for (int k = 0; k < 64; k++) m = fminf(m, sd[k]);
+        const float sdv = m < 3.0e38f ? m + 1.f + 1e-4f * fabsf(m) : __builtin_inff();
+        seed[b] = sdv;
+        pmin[b] = float_to_ordered(sdv);
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // Local angular searches: HealpixSampling::selectOrientationsWithNonZeroPriorProbability
 // (healpix_sampling.cpp:769-1145) for C1, without bimodal psi or multi-body terms, sigma_cutoff 3,
@@ -2203,6 +2309,26 @@ static void build_table(int size, int cur, const double* sigma2, int nsig2, int 
         }
     }
     t.npix = (int)t.h_x.size();
+    if (!store) {
+        // the diff2 pixels in order of |frequency| (then window order): every prefix is a disc, so
+        // a GEMM over the first K columns gives diff2 over a low-frequency subset, a lower bound
+        // of the full diff2 (each pixel adds minv |ctf F - X|^2 / 2 >= 0)
+        std::vector<int> ord(t.npix);
+        for (int i = 0; i < t.npix; i++) ord[i] = i;
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) {
+            return t.h_x[a] * t.h_x[a] + t.h_y[a] * t.h_y[a] < t.h_x[b] * t.h_x[b] + t.h_y[b] * t.h_y[b];
+        });
+        auto perm = [&](auto& v) {
+            auto w = v;
+            for (int i = 0; i < t.npix; i++) w[i] = v[ord[i]];
+            v.swap(w);
+        };
+        perm(t.h_x);
+        perm(t.h_y);
+        perm(t.h_win);
+        perm(t.h_ires);
+        perm(t.h_inproj);
+    }
     t.n_og = n_og;
     t.h_minv.resize((size_t)n_og * t.npix);
     for (int g = 0; g < n_og; g++) {
@@ -2767,6 +2893,23 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     stt.P_coarse = tc.npix;
     stt.P_fine = tf.npix;
 
+    // ---- pruned coarse pass: the low-frequency prefix of the (|frequency|-ordered) coarse pixels ----
+    // (global searches in a split precision, without the proposal or diagnostics). The prefix holds
+    // the pixels within 0.82 of the coarse radius (two thirds of them), rounded up to 32 pixels.
+    int K_low = 0;
+    bool prune_call = !cfg->coarse_exhaustive && prec != T2X_PREC_FP32 && !local && !use_prop &&
+                      !(dbg && dbg->n_debug > 0);
+    if (prune_call) {
+        double frac = 0.82;
+        const char* env_pr = getenv("T2X_PRUNE_RADIUS");   // fraction of the coarse radius
+        if (env_pr && atof(env_pr) > 0.) frac = atof(env_pr);
+        const double rl = frac * (cc / 2.0);
+        int n_low = 0;
+        while (n_low < tc.npix && tc.h_x[n_low] * tc.h_x[n_low] + tc.h_y[n_low] * tc.h_y[n_low] <= rl * rl) n_low++;
+        K_low = (int)round_up(2LL * n_low, 64);
+        if (n_low < 16 || K_low >= Kc - 64) prune_call = false;
+    }
+
     // ---- brick backprojection: the call's work units and buffers ----
     const bool bp_bricks = !cfg->bp_direct && K <= BP_KMAX;
     long long bp_item_cap = 0;
@@ -2829,6 +2972,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
 
     for (int b0 = 0; b0 < n; b0 += Bmax) {
         const int B = std::min(Bmax, n - b0);
+        const bool prune_batch = prune_call;   // the pruned coarse pass for this batch
         const size_t NN = (size_t)N * N;
         int ev = 0;
         (void)hipEventRecord(d->ev[ev++], st);
@@ -3227,6 +3371,18 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                                            d->coarse_B2h.ptr, d->coarse_B2l.ptr, d->coarse_B2s.as<float>(), Kn, Kn,
                                            (int)B_pad, (int)O_pad, ep, st,
                                            GemmChoice{cfg->gemm_tile, cfg->gemm_group}, nullptr, nullptr));
+                    if (prune_batch) {   // N1h: the same over the pixels beyond the low-frequency prefix
+                        T2X_TRY(d->prune_n1.ensure(sizeof(float) * B_pad * O_pad));
+                        EpiParams el = ep;
+                        el.C = d->prune_n1.as<float>();
+                        const int k0 = K_low / 2;   // a multiple of 32 elements: 64-byte aligned rows
+                        T2X_TRY(launch_gemm_x3(prec == T2X_PREC_FP16X2 ? T2X_PREC_FP16X3 : prec, EPI_STORE,
+                                               (const uint16_t*)d->A1h.ptr + k0, (const uint16_t*)d->A1l.ptr + k0,
+                                               d->A1s.as<float>(), Kn, B_pad, (const uint16_t*)d->coarse_B2h.ptr + k0,
+                                               (const uint16_t*)d->coarse_B2l.ptr + k0, d->coarse_B2s.as<float>(),
+                                               Kn, Kn - k0, (int)B_pad, (int)O_pad, el, st,
+                                               GemmChoice{cfg->gemm_tile, cfg->gemm_group}, nullptr, nullptr));
+                    }
                 }
                 if (local) {
                     const long long ne = (long long)B * O_all;
@@ -3314,6 +3470,64 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                 // rows beyond the valid columns of the last tile keep +inf
                 k_fill_f32<<<(unsigned)((M_pad * rowmin_ld + 255) / 256), 256, 0, st>>>(d_rowmin.as<float>(), 3.0e38f,
                                                                                         M_pad * rowmin_ld);
+                if (prune_batch) {
+                    // 1. bounds: block minima of C_b + N1 - G_low - 2 sqrt(N1h Ch), the cross term over
+                    //    the first K_low columns and the rest bounded by Cauchy-Schwarz, no values stored
+                    k_const_term<<<B, 256, 0, st>>>(d->Fimg_cur.as<float2>(), Pw, tc.d_win.as<int>(),
+                                                    tc.d_minv.as<float>(), tc.npix, d_ps, 2,
+                                                    std::min(K_low / 2, tc.npix));
+                    // one FP16 product per element unless the precision is BF16X3 (whose dropped
+                    // terms would cost 2^-7 of the bound) or the full split is asked for
+                    const char* env_fb = getenv("T2X_PRUNE_FULL_BOUNDS");
+                    const bool h1 = prec != T2X_PREC_BF16X3 && !(env_fb && atoi(env_fb) != 0);
+                    {
+                        const long long ne = (long long)B * O_pad;
+                        k_prune_n1<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(
+                            d->prune_n1.as<float>(), ep.N1, O_pad, (int)O_pad, d_ps, ne, h1 ? 1. / 512. : 0.);
+                    }
+                    T2X_TRY(d->prune_lb.ensure(sizeof(float) * M_pad * rowmin_ld));
+                    k_fill_f32<<<(unsigned)((M_pad * rowmin_ld + 255) / 256), 256, 0, st>>>(
+                        d->prune_lb.as<float>(), 3.0e38f, M_pad * rowmin_ld);
+                    T2X_HIP(hipGetLastError());
+                    EpiParams eb = ep;
+                    eb.C = nullptr;
+                    eb.N1 = d->prune_n1.as<float>();
+                    eb.rowmin = d->prune_lb.as<float>();
+                    eb.pmin = nullptr;
+                    if (h1)
+                        T2X_TRY(launch_gemm_h1_bounds(d->Ah.ptr, d->As.as<float>(), Kc, M_pad, d->coarse_Bh.ptr,
+                                                      d->coarse_Bs.as<float>(), Kc, K_low, (int)M_pad, (int)O_pad,
+                                                      eb, st));
+                    else
+                        T2X_TRY(launch_gemm_x3(prec, EPI_DIFF2, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc, M_pad,
+                                               d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, K_low,
+                                               (int)M_pad, (int)O_pad, eb, st,
+                                               GemmChoice{cfg->gemm_tile, cfg->gemm_group}, nullptr, nullptr));
+                    // 2. seeds: per particle the full diff2 of its most promising block (also the lazy
+                    //    stores' starting minima)
+                    T2X_TRY(d->prune_seed.ensure(sizeof(float) * B));
+                    const int nblk = (int)((O_all + 63) / 64);
+                    if (prec == T2X_PREC_BF16X3)
+                        k_prune_seed<__bf16><<<B, 256, 0, st>>>(
+                            d->prune_lb.as<float>(), rowmin_ld, nblk, (int)T, (const __bf16*)d->Ah.ptr,
+                            (const __bf16*)d->Al.ptr, d->As.as<float>(), (const __bf16*)d->coarse_Bh.ptr,
+                            (const __bf16*)d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc, ep.Cb, ep.N1, O_pad,
+                            ep.prior_o, O_all, d->prune_seed.as<float>(), d->pmin.as<unsigned>());
+                    else
+                        k_prune_seed<_Float16><<<B, 256, 0, st>>>(
+                            d->prune_lb.as<float>(), rowmin_ld, nblk, (int)T, (const _Float16*)d->Ah.ptr,
+                            (const _Float16*)d->Al.ptr, d->As.as<float>(), (const _Float16*)d->coarse_Bh.ptr,
+                            prec == T2X_PREC_FP16X2 ? nullptr : (const _Float16*)d->coarse_Bl.ptr,
+                            d->coarse_Bs.as<float>(), Kc, Kc, ep.Cb, ep.N1, O_pad, ep.prior_o, O_all,
+                            d->prune_seed.as<float>(), d->pmin.as<unsigned>());
+                    T2X_HIP(hipGetLastError());
+                    // 3. the full pass skips the tiles whose blocks all lie beyond the cut
+                    T2X_TRY(d->prune_cnt.ensure(sizeof(unsigned) * 2));
+                    T2X_TRY(zero_async(d->prune_cnt.ptr, sizeof(unsigned) * 2, st));
+                    ep.prune_lb = d->prune_lb.as<float>();
+                    ep.prune_seed = d->prune_seed.as<float>();
+                    ep.prune_count = d->prune_cnt.as<unsigned>();
+                }
                 if (prec == T2X_PREC_FP32) {
                     T2X_TRY(launch_gemm(EPI_DIFF2, d->A.as<float>(), Kc, d->coarse_B.as<float>(), Kc, Kc,
                                         (int)M_pad, (int)O_pad, ep, st));
@@ -3438,6 +3652,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         std::vector<double> hnorm(3 * B), hpower((size_t)B * xN);
         std::vector<PartScalars> hps2(B);
         std::vector<int> hflag(B);   // candidate path: particles sent to the radix select
+        unsigned hprune[2] = {0u, 0u};   // pruned coarse pass: tiles skipped, tiles checked
         auto post_part = [&]() -> int {
         T2X_TRY(d->sig_pairs.ensure(sizeof(int2) * cap));
         T2X_TRY(d->sig_orient.ensure(sizeof(int) * cap));
@@ -3750,7 +3965,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         for (int attempt = 0;; attempt++) {
         if (attempt == 0) {
             const bool replay = gmode && d->graph_exec && d->graph_call == call_id && d->graph_B == B &&
-                                d->graph_gen == alloc_generation();
+                                d->graph_prune == prune_batch && d->graph_gen == alloc_generation();
             if (replay || (gmode && d->eager_B.count(B))) {
                 if (!replay) {
                     destroy_graph(d);
@@ -3776,6 +3991,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                     d->graph = gr;
                     d->graph_exec = ex;
                     d->graph_B = B;
+                    d->graph_prune = prune_batch;
                     d->graph_gen = alloc_generation();
                     d->graph_call = call_id;
                 }
@@ -3805,6 +4021,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             rb.add(hpower.data(), d->power.ptr, sizeof(double) * B * xN);
             rb.add(hps2.data(), d_ps, sizeof(PartScalars) * B);
             if (cand_path) rb.add(hflag.data(), d_radix, sizeof(int) * B);
+            if (prune_batch) rb.add(hprune, d->prune_cnt.ptr, sizeof(unsigned) * 2);
             T2X_TRY(rb.run(d, st));
         }
         if (hplan.err_code) {
@@ -3823,6 +4040,12 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         npairs = hplan.npairs;
         nitems = hplan.nitems;
         if (bp_bricks) stt.bp_brick_items += (double)std::min(nitems, bp_item_cap);
+        if (prune_batch) {
+            stt.coarse_tiles_pruned += hprune[0];
+            stt.coarse_tiles_checked += hprune[1];
+            // a pass that prunes little costs more than the exhaustive one: off for the call's rest
+            if (hprune[1] > 0 && hprune[0] < 0.4 * hprune[1]) prune_call = false;
+        }
         if (!cand_path) {
             stt.n_select_fallbacks += 1;
         } else {   // did any particle of the batch take the radix select?

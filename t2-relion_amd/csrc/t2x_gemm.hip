@@ -38,6 +38,44 @@ __device__ __forceinline__ void tile_coords(int pid, int tiles_m, int tiles_n, i
     tn = in_group / gsize;
 }
 
+// Pruned coarse pass (EpiParams prune_lb): whether the tile rows [m0, m0 + rows) x columns
+// [n0, n0 + cols) can stay uncomputed, i.e. every 64-column block of every valid row has its
+// lower bound above the row's particle's seed by more than the negligible-weight cut; if so the
+// blocks get their bounds as minima. Uniform over the workgroup; called before the tile's first
+// memory operation.
+__device__ bool prune_tile(const EpiParams& ep, long long m0, int rows, long long n0, int cols, int tid,
+                           int nthreads)
+{
+    __shared__ int s_keep;
+    if (tid == 0) s_keep = 0;
+    __syncthreads();
+    const int nblk = cols / 64;
+    const long long blk0 = n0 / 64;
+    bool keep = false;
+    for (int q = tid; q < rows * nblk; q += nthreads) {
+        const long long m = m0 + q / nblk;
+        const long long blk = blk0 + q % nblk;
+        if (m >= ep.M || blk >= ep.rowmin_ld) continue;
+        const float lb = ep.prune_lb[m * ep.rowmin_ld + blk];
+        if (!(lb > ep.prune_seed[m / ep.T] + 61.f)) keep = true;
+    }
+    if (keep) s_keep = 1;
+    __syncthreads();
+    const bool skip = s_keep == 0;
+    if (skip)
+        for (int q = tid; q < rows * nblk; q += nthreads) {
+            const long long m = m0 + q / nblk;
+            const long long blk = blk0 + q % nblk;
+            if (m >= ep.M || blk >= ep.rowmin_ld) continue;
+            ep.rowmin[m * ep.rowmin_ld + blk] = ep.prune_lb[m * ep.rowmin_ld + blk];
+        }
+    if (tid == 0) {
+        atomicAdd(&ep.prune_count[1], 1u);
+        if (skip) atomicAdd(&ep.prune_count[0], 1u);
+    }
+    return skip;
+}
+
 // Shared epilogue of the GEMM kernels for one wave's block of MI x 4 MFMA tiles (lane holds
 // C[4*(lane/16)+r][lane%16] of each 16x16 tile). Rows start at mbase, columns at ncol0 (a
 // multiple of 64). EPI_STORE writes C = acc * row scale * column scale; EPI_DIFF2 writes
@@ -104,12 +142,12 @@ __device__ __forceinline__ void gemm_epilogue(const floatx4 (&acc)[MI][4], long 
                     if (mval[r] && nval[j] && scored[j]) rmin = fminf(rmin, dv[j]);
                 }
                 for (int o = 1; o < 16; o <<= 1) rmin = fminf(rmin, __shfl_xor(rmin, o));
-                bool store = true;
+                bool store = ep.C != nullptr;   // null: a bounds-only launch (block minima alone)
                 if (ep.pmin && mval[r]) {
                     // the particle's final minimum is <= est, so a block more than the cut (60,
                     // plus one for rounding) above est is negligible whatever the final minimum
                     const float est = ordered_to_float(ep.pmin[pb[r]]);
-                    store = !(rmin > est + 61.f);
+                    store = store && !(rmin > est + 61.f);
                     if (rmin < est && col_in == 0) atomicMin(&ep.pmin[pb[r]], float_to_ordered(rmin));
                 }
                 if (store) {
@@ -705,7 +743,7 @@ __device__ __forceinline__ void gemm_epilogue_staged(const floatx4 (&acc)[MI][4]
                     rmin = (nval[j] && scored[j]) ? fminf(rmin, dv[j]) : rmin;
                 }
                 rmin = min_row16(rmin);
-                bool store = rv;
+                bool store = rv && ep.C != nullptr;   // null: a bounds-only launch (block minima alone)
                 if (lazy) {
                     // the particle's final minimum is <= es_q, so a block more than the cut (60,
                     // plus one for rounding) above es_q is negligible whatever the final minimum
@@ -715,7 +753,7 @@ __device__ __forceinline__ void gemm_epilogue_staged(const floatx4 (&acc)[MI][4]
                     es_q = q == 1 ? est[1] : es_q;
                     es_q = q == 2 ? est[2] : es_q;
                     es_q = q == 3 ? est[3] : es_q;
-                    store = rv && !(rmin > es_q + 61.f);
+                    store = store && !(rmin > es_q + 61.f);
                     const bool lower = rv && rmin < es_q;
                     if (lower && col_in == 0) atomicMin(&ep.pmin[e.y], float_to_ordered(rmin));
                     est[0] = (lower && q == 0) ? rmin : est[0];
@@ -751,6 +789,7 @@ __global__ __launch_bounds__(512) void k_gemm_x3p(SplitOperand A, SplitOperand B
     int tm, tn;
     tile_coords(blockIdx.x, tiles_m, tiles_n, group_m, tm, tn);
     const long long m0 = (long long)tm * WBM, n0 = (long long)tn * WBN;
+    if (EPI == EPI_DIFF2 && ep.prune_lb && !rowmap && prune_tile(ep, m0, WBM, n0, WBN, tid, 512)) return;
 
     floatx4 acc[8][4];
 #pragma unroll
@@ -938,6 +977,9 @@ __device__ __forceinline__ bool stage_epilogue_h(EpiStageH& es, long long m0l, l
     return true;
 }
 
+// ONE: the bounds form of the pruned coarse pass, one product (A hi . B hi) per element; A lo is
+// neither loaded nor multiplied (the caller allows for the dropped term in its bound).
+template <bool ONE>
 __global__ __launch_bounds__(256, 2) void k_gemm_x2h(SplitOperand A, SplitOperand B, int K, int tiles_m, int tiles_n,
                                                      int group_m, EpiParams ep, const int* __restrict__ rowmap)
 {
@@ -954,6 +996,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_x2h(SplitOperand A, SplitOperan
     int tm, tn;
     tile_coords(blockIdx.x, tiles_m, tiles_n, group_m, tm, tn);
     const long long m0 = (long long)tm * 128, n0 = (long long)tn * 256;
+    if (ep.prune_lb && !rowmap && prune_tile(ep, m0, 128, n0, 256, tid, 256)) return;
 
     // acc[h][i][j]: rows 16 i of the wave's 64, columns 64 h + 16 j of its 128
     floatx4 acc[2][4][4];
@@ -985,8 +1028,10 @@ __global__ __launch_bounds__(256, 2) void k_gemm_x2h(SplitOperand A, SplitOperan
         const long long kb = (long long)(k) * (long long)sizeof(H);                          \
         lds_dma16<H>(reinterpret_cast<const H*>(Ah + kb + oa0), &SA[0][ra0 * 32]);          \
         lds_dma16<H>(reinterpret_cast<const H*>(Ah + kb + oa1), &SA[0][ra1 * 32]);          \
-        lds_dma16<H>(reinterpret_cast<const H*>(Al + kb + oa0), &SA[1][ra0 * 32]);          \
-        lds_dma16<H>(reinterpret_cast<const H*>(Al + kb + oa1), &SA[1][ra1 * 32]);          \
+        if constexpr (!ONE) {                                                                \
+            lds_dma16<H>(reinterpret_cast<const H*>(Al + kb + oa0), &SA[1][ra0 * 32]);      \
+            lds_dma16<H>(reinterpret_cast<const H*>(Al + kb + oa1), &SA[1][ra1 * 32]);      \
+        }                                                                                    \
         _Pragma("unroll") for (int q = 0; q < 4; q++)                                        \
             lds_dma16<H>(reinterpret_cast<const H*>(Bh + kb + obq + q * ob16), &SB[(rb + 16 * q) * 32]); \
     } while (0)
@@ -996,13 +1041,16 @@ __global__ __launch_bounds__(256, 2) void k_gemm_x2h(SplitOperand A, SplitOperan
 #define X2H_READ_A(SA, i, AH, AL)                                   \
     do {                                                            \
         AH = *reinterpret_cast<const v8*>(&SA[0][oa + (i) * 512]);  \
-        AL = *reinterpret_cast<const v8*>(&SA[1][oa + (i) * 512]);  \
+        if constexpr (ONE) AL = v8{};                               \
+        else AL = *reinterpret_cast<const v8*>(&SA[1][oa + (i) * 512]); \
     } while (0)
     // one A block against the 8 B fragments: lo.hi then hi.hi per element
 #define X2H_GROUP(i, AL, AH)                                                                        \
     do {                                                                                            \
-        _Pragma("unroll") for (int j = 0; j < 8; j++)                                               \
-            acc[j >> 2][i][j & 3] = mfma16x16x32<H>(AL, bh[j], acc[j >> 2][i][j & 3]);              \
+        if constexpr (!ONE) {                                                                       \
+            _Pragma("unroll") for (int j = 0; j < 8; j++)                                           \
+                acc[j >> 2][i][j & 3] = mfma16x16x32<H>(AL, bh[j], acc[j >> 2][i][j & 3]);          \
+        }                                                                                           \
         _Pragma("unroll") for (int j = 0; j < 8; j++)                                               \
             acc[j >> 2][i][j & 3] = mfma16x16x32<H>(AH, bh[j], acc[j >> 2][i][j & 3]);              \
     } while (0)
@@ -1026,7 +1074,7 @@ __global__ __launch_bounds__(256, 2) void k_gemm_x2h(SplitOperand A, SplitOperan
         const bool nxt = (s) + 1 < nk;                                             \
         if (nxt) X2H_READ_A(NA, 0, ah, al);                                        \
         _Pragma("unroll") for (int j = 0; j < 8; j++) {                            \
-            acc[j >> 2][3][j & 3] = mfma16x16x32<H>(a3l, bh[j], acc[j >> 2][3][j & 3]); \
+            if constexpr (!ONE) acc[j >> 2][3][j & 3] = mfma16x16x32<H>(a3l, bh[j], acc[j >> 2][3][j & 3]); \
             acc[j >> 2][3][j & 3] = mfma16x16x32<H>(a3h, bh[j], acc[j >> 2][3][j & 3]); \
             if (nxt) bh[j] = *reinterpret_cast<const v8*>(&NB[ob + j * 512]);      \
         }                                                                          \
@@ -1410,7 +1458,7 @@ static int launch_x3_variant(int prec, int epi, const SplitOperand& A, const Spl
             set_error("gemm_x3: tile 259 serves the FP16X2 diff2 GEMM only");
             return T2X_EINVAL;
         }
-        k_gemm_x2h<<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep, rowmap);
+        k_gemm_x2h<false><<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep, rowmap);
     } else if (tile == 257) {
         if (bf) { if (store) k_gemm_x3d<__bf16, EPI_STORE><<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep);
                   else k_gemm_x3d<__bf16, EPI_DIFF2><<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep); }
@@ -1435,6 +1483,34 @@ static int launch_x3_variant(int prec, int epi, const SplitOperand& A, const Spl
         else    { if (store) k_gemm_x3<_Float16, EPI_STORE><<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep);
                   else k_gemm_x3<_Float16, EPI_DIFF2><<<grid, 256, 0, st>>>(A, B, K, tm, tn, g, ep); }
     }
+    T2X_HIP(hipGetLastError());
+    return T2X_OK;
+}
+
+// Bounds form of the pruned coarse pass: block minima of the diff2 with one FP16 product per
+// element (A hi . B hi), no values stored (ep.C null), 128x256 tiles.
+int launch_gemm_h1_bounds(const void* Ah, const float* As, long long lda, long long a_rows, const void* Bh,
+                          const float* Bs, long long ldb, int K, int Mrows_pad, int Nrows_pad, const EpiParams& ep,
+                          hipStream_t st)
+{
+    const long long lim = 1LL << 32;
+    if (a_rows * lda * 2 >= lim || (long long)Nrows_pad * ldb * 2 >= lim) {
+        set_error("gemm_h1_bounds: operands beyond 4 GiB (32-bit offsets)");
+        return T2X_EINVAL;
+    }
+    if (ep.C != nullptr || ep.pmin != nullptr || ep.prune_lb != nullptr) {
+        set_error("gemm_h1_bounds: a bounds-only launch (no stores, no lazy minima, no pruning)");
+        return T2X_EINVAL;
+    }
+    if (K % XK != 0 || Mrows_pad % 128 != 0 || Nrows_pad % 256 != 0) {
+        set_error("gemm_h1_bounds: operands not padded to tile multiples");
+        return T2X_EINVAL;
+    }
+    const int tm = Mrows_pad / 128, tn = Nrows_pad / 256;
+    const long long nwg = (long long)tm * tn;
+    if (nwg > 0x7fffffffLL) { set_error("gemm: grid too large"); return T2X_EINVAL; }
+    SplitOperand A{Ah, nullptr, As, lda}, B{Bh, nullptr, Bs, ldb};
+    k_gemm_x2h<true><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, 32, ep, nullptr);
     T2X_HIP(hipGetLastError());
     return T2X_OK;
 }

@@ -203,6 +203,10 @@ struct Device {
     // the phase tables' sizes and translations (ab_coarse / ab_fine hold them while unchanged)
     std::vector<double> ab_key;
     DevBuf prep_part;   // per particle and row band: background sums of the mask
+    // pruned coarse pass: N1 over the low-frequency prefix, its constant terms, the block lower
+    // bounds, the per-particle seeds and the tile counters
+    DevBuf prune_n1, prune_lb, prune_seed, prune_cnt;
+    bool graph_prune = false;   // the batch graph was captured with the pruned coarse pass
     DevBuf coarse_Ai;   // projection frames of the coarse orientations, for the angles in coarse_Ai_key
     std::vector<double> coarse_Ai_key;
     // launch_graph: the batch graph captured in call graph_call for batches of graph_B

@@ -18,6 +18,7 @@ struct PartScalars {
     double norm_scale;    // avg_norm / normcorr (1 when norm correction is off)
     double C_coarse;      // highres/2 + 1/2 sum_Mc minv |X|^2
     double C_fine;        // highres/2 + 1/2 sum_Mf minv |X|^2
+    double C_low;         // highres/2 + 1/2 sum minv |X|^2 over a low-frequency prefix of Mc (pruning)
     double scale;         // the image group's scale correction (1 without), folded into the CTF
     double ctf_store;     // factor on that CTF in the weighted sums and the backprojection (the
                           // store clamps the scale at 0.001, ml_optimiser.cpp:8333-8343)
@@ -158,6 +159,15 @@ struct EpiParams {
     // 64-column block whose minimum exceeds it by more than the negligible-weight cut are not
     // stored (the block minimum still is, so every reader skips the block). null: store all.
     unsigned int* pmin;
+    // Pruned coarse pass. C null: a bounds-only launch (EPI_DIFF2 writes the 64-column block
+    // minima, no values). prune_lb non-null: a tile whose rows' 64-column blocks all have a lower
+    // bound prune_lb[m][blk] (the low-frequency diff2 block minimum) above the row's particle's
+    // prune_seed (an upper bound of its minimum) by more than the negligible-weight cut is not
+    // computed; its blocks get the bound as their minimum. prune_count[0] counts skipped tiles,
+    // prune_count[1] the tiles that checked.
+    const float* prune_lb;
+    const float* prune_seed;
+    unsigned int* prune_count;
 };
 
 struct SelParams {

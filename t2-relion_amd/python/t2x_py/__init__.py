@@ -55,7 +55,7 @@ class EstepConfig(C.Structure):
         ("proposal_topk", C.c_int), ("proposal_gate", C.c_int),
         ("proposal_min_conf", C.c_double), ("gemm_tile", C.c_int), ("gemm_group", C.c_int),
         ("sigma_rot", C.c_double), ("sigma_tilt", C.c_double), ("sigma_psi", C.c_double),
-        ("launch_graph", C.c_int), ("bp_direct", C.c_int),
+        ("launch_graph", C.c_int), ("bp_direct", C.c_int), ("coarse_exhaustive", C.c_int),
     ]
 
 
@@ -129,7 +129,8 @@ class Stats(C.Structure):
          ("proposal_bins_scored", C.c_double), ("gemm_tile_used", C.c_int), ("gemm_group_used", C.c_int),
                 ("local_rows", C.c_double), ("local_slices", C.c_double),
                 ("ms_graph", C.c_double), ("n_graph_launches", C.c_long),
-                ("bp_brick_items", C.c_double)]
+                ("bp_brick_items", C.c_double), ("coarse_tiles_checked", C.c_double),
+                ("coarse_tiles_pruned", C.c_double)]
 
 
 class MrcInfo(C.Structure):

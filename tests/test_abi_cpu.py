@@ -29,7 +29,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 20
     for n in names:
         assert hasattr(lib, n), n
-    assert lib.t2x_abi_version() == 5
+    assert lib.t2x_abi_version() == 6
 
 
 def test_bad_arguments_return_status():
