@@ -6,7 +6,11 @@ same run on the same resident particles:
   fp32     plain FP32 cross terms (the parity path)
   fp16x3   split-precision cross terms (FP16 hi+lo on MFMA, FP32 accumulate, FP64 normalisation)
   fp16x2   the dense coarse cross term with 2 MFMA products (FP16 hi+lo images x FP16 hi
-           references); the fine pass and every other cross term as fp16x3
+           references); the fine pass and every other cross term as fp16x3. Global split-precision
+           searches take the pruned coarse pass by default: a one-product bounds GEMM over the
+           low-frequency prefix of the coarse pixels, then the full GEMM only on tiles not beyond
+           the negligible-weight cut (identical outputs)
+  fp16x2+exhaustive  the same with the coarse GEMM over every tile (coarse_exhaustive 1)
   fp16x2+graph  the same, each batch's launches after its preprocessing captured once per batch
            size as a hipGraph and replayed (reported beside the eager paths)
   (+direct on any path: the 3D backprojection scatters each pixel into the volume with global
@@ -30,7 +34,9 @@ The headline value is the fastest path, per-iteration costs included, that keeps
 agreement with exhaustive search and the plain path's FSC-0.143 within 0.1 A (--headline pins
 one). The dominant kernel of the headline path is timed with HIP events on the library's stream;
 its roofline fraction counts the algorithmic 4*T*O*P_c flops (the MFMA-product utilisation of the
-split forms is reported beside it). A quality phase reconstructs half-maps (first and second half
+split forms is reported beside it); for the pruned pass, the exhaustive pass's count over the
+pruned stage's time (an effective rate), with the work actually executed and its MFMA
+utilisation beside it. A quality phase reconstructs half-maps (first and second half
 of all particles) of every class for each path on the GPU and reports the FSC-0.143 resolution
 per path; the pose agreement of each path with exhaustive search is reported per variant.
 
@@ -452,8 +458,8 @@ def main():
     ap.add_argument("--precision", default="",
                     help="comma list of paths to time (fp32, bf16x3, fp16x3, fp16x2, learned, learned_gated; "
                          "a precision path + \"+graph\" runs its batches as replayed hipGraphs) (default: "
-                         "fp32,fp16x3,fp16x2,fp16x2+graph,learned,learned_gated for 3D references, "
-                         "fp32,fp16x3,fp16x2,fp16x2+graph for 2D, fp32,bf16x3,fp16x3 for local searches, whose "
+                         "fp32,fp16x3,fp16x2,fp16x2+exhaustive,fp16x2+graph,learned,learned_gated for 3D "
+                         "references, fp32,fp16x3,fp16x2,fp16x2+exhaustive,fp16x2+graph for 2D, fp32,bf16x3,fp16x3 for local searches, whose "
                          "grouped GEMM has no 2-product form)")
     ap.add_argument("--headline", default="", help="path of the headline value (default: the fastest path, "
                     "per-iteration costs included, that meets the pose-agreement and FSC bars)")
@@ -522,8 +528,8 @@ def main():
         log("copy calibration: 1 GiB device-to-device copy (t2x::k_copy16) done")
     local = cfg.orientational_prior_mode == T.T2X_PRIOR_ROTTILT_PSI
     paths = (args.precision or ("fp32,bf16x3,fp16x3" if local else
-                                "fp32,fp16x3,fp16x2,fp16x2+graph" if cfg.ref_dim == 2 else
-                                "fp32,fp16x3,fp16x2,fp16x2+graph,learned,learned_gated")).split(",")
+                                "fp32,fp16x3,fp16x2,fp16x2+exhaustive,fp16x2+graph" if cfg.ref_dim == 2 else
+                                "fp32,fp16x3,fp16x2,fp16x2+exhaustive,fp16x2+graph,learned,learned_gated")).split(",")
     train = None
     use_net = any(p.startswith("learned") for p in paths)
     if use_net and args.load_proposal:
@@ -675,9 +681,10 @@ def main():
     traffic = os.environ.get("T2X_GEMM_TRAFFIC_BYTES")
     traffic_src = "env T2X_GEMM_TRAFFIC_BYTES" if traffic else None
     tj = os.path.join(REPO, "profiles", "coarse_gemm_traffic.json")
+    pruned = st.coarse_tiles_checked > 0 and not local and not head.startswith("learned")
     if traffic is None and os.path.exists(tj) and args.workload == "class256":
         with open(tj) as f:
-            tr = json.load(f).get(hb)
+            tr = json.load(f).get(hb + ("_pruned" if pruned else ""))
         if tr:
             traffic = tr["bytes_per_launch"]
             traffic_src = "profiles/coarse_gemm_traffic.json (rocprofv3 --pmc FETCH_SIZE + WRITE_SIZE, 800-particle batches)"
@@ -692,6 +699,24 @@ def main():
             "flop_convention": "algorithmic 4*T*O*P_c per scored particle (one complex product per pair)",
             "mfma_products_per_pair": nprod,
             "mfma_product_utilisation": nprod * achieved / peak}
+    if pruned:
+        # the pruned pass computes part of the exhaustive pass's products for identical outputs:
+        # `achieved` stays the counting convention's (effective) rate; the work the MFMA units
+        # actually did is reported beside it
+        f_kept = 1.0 - st.coarse_tiles_pruned / st.coarse_tiles_checked
+        exec_pairs = (st.coarse_bound_flop + flop_total * f_kept) / launches
+        exec_products = (st.coarse_bound_flop * 1 + flop_total * f_kept * nprod) / launches
+        roof.update({
+            "kernel": ("coarse diff2 cross term, pruned pass: k_gemm_x2h<true> (bounds: 1 FP16 product over the "
+                       "low-frequency prefix of the coarse pixels, block minima only) + k_prune_seed + "
+                       "k_gemm_x2h<false> (2 products, the tiles not beyond the negligible-weight cut)"),
+            "achieved_convention": ("effective: the exhaustive pass's algorithmic flops per batch over the pruned "
+                                    "stage's time per batch (same outputs)"),
+            "tiles_pruned_fraction": 1.0 - f_kept,
+            "executed_flop_per_launch": exec_pairs,
+            "executed_mfma_product_utilisation": exec_products / (avg_ms * 1e-3) / 1e12 / peak if avg_ms > 0 else 0.0,
+        })
+        del roof["mfma_product_utilisation"]
     configure(cfg, head)
     # the unbatched per-particle launch path (the reference GPU path launches every particle's
     # kernels separately): the same library with batches of one particle, on a sample

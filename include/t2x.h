@@ -264,6 +264,7 @@ typedef struct t2x_stats {
     double bp_brick_items;       /* of bp_orientations, those inserted through LDS bricks          */
     double coarse_tiles_checked; /* coarse GEMM tiles tested against the low-frequency bounds      */
     double coarse_tiles_pruned;  /* of those, skipped (every block beyond the cut)                 */
+    double coarse_bound_flop;    /* the bounds GEMMs' work, 4 flops per (row, orientation, pixel)  */
 } t2x_stats;
 
 /* ---------------------------------------------------------------------------------------

@@ -46,6 +46,10 @@ int launch_gemm_x3(int prec, int epi, const void* Ah, const void* Al, const floa
 int launch_gemm_h1_bounds(const void* Ah, const float* As, long long lda, long long a_rows, const void* Bh,
                           const float* Bs, long long ldb, int K, int Mrows_pad, int Nrows_pad, const EpiParams& ep,
                           hipStream_t st);
+int launch_gemm_pruned(int prec, const void* Ah, const void* Al, const float* As, long long lda, long long a_rows,
+                       const void* Bh, const void* Bl, const float* Bs, long long ldb, int K, int Mrows_pad,
+                       int Nrows_pad, const EpiParams& ep, const float* lb, const float* seed, unsigned* list,
+                       long long list_cap, unsigned* count, int n_cu, hipStream_t st);
 int launch_gemm_local_x3(int prec, const void* Sh, const void* Sl, const float* Ss, const void* Wh, const void* Wl,
                          const float* Ws, long long ld, int K, const int2* items, int n_items, int T,
                          const long long* orient_off, const int* rowu, const double* cb, const double* n1,
@@ -2973,6 +2977,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     for (int b0 = 0; b0 < n; b0 += Bmax) {
         const int B = std::min(Bmax, n - b0);
         const bool prune_batch = prune_call;   // the pruned coarse pass for this batch
+        long long prune_ntiles = 0;            // its full GEMM's tiles
         const size_t NN = (size_t)N * N;
         int ev = 0;
         (void)hipEventRecord(d->ev[ev++], st);
@@ -3521,12 +3526,11 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                             d->coarse_Bs.as<float>(), Kc, Kc, ep.Cb, ep.N1, O_pad, ep.prior_o, O_all,
                             d->prune_seed.as<float>(), d->pmin.as<unsigned>());
                     T2X_HIP(hipGetLastError());
-                    // 3. the full pass skips the tiles whose blocks all lie beyond the cut
-                    T2X_TRY(d->prune_cnt.ensure(sizeof(unsigned) * 2));
-                    T2X_TRY(zero_async(d->prune_cnt.ptr, sizeof(unsigned) * 2, st));
-                    ep.prune_lb = d->prune_lb.as<float>();
-                    ep.prune_seed = d->prune_seed.as<float>();
-                    ep.prune_count = d->prune_cnt.as<unsigned>();
+                    // 3. the full pass computes the tiles with a block not beyond the cut (below)
+                    T2X_TRY(d->prune_cnt.ensure(sizeof(unsigned)));
+                    T2X_TRY(zero_async(d->prune_cnt.ptr, sizeof(unsigned), st));
+                    prune_ntiles = (M_pad / 256) * (O_pad / 256);   // the 256x256 tiles of k_gemm_x3p
+                    T2X_TRY(d->prune_list.ensure(sizeof(unsigned) * prune_ntiles));
                 }
                 if (prec == T2X_PREC_FP32) {
                     T2X_TRY(launch_gemm(EPI_DIFF2, d->A.as<float>(), Kc, d->coarse_B.as<float>(), Kc, Kc,
@@ -3547,6 +3551,13 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                                                     d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc,
                                                     d->prop_plist.as<int>(), d->prop_work.ptr, n_work, (int)T,
                                                     O_pad - 1, ep, st));
+                    } else if (prune_batch) {
+                        T2X_TRY(launch_gemm_pruned(prec, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc, M_pad,
+                                                   d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc,
+                                                   (int)M_pad, (int)O_pad, ep, d->prune_lb.as<float>(),
+                                                   d->prune_seed.as<float>(), d->prune_list.as<unsigned>(),
+                                                   prune_ntiles, d->prune_cnt.as<unsigned>(), d->n_cu, st));
+                        gused = GemmChoice{258, 0};   // group 0: the tile list
                     } else
                         T2X_TRY(launch_gemm_x3(prec, EPI_DIFF2, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc,
                                                M_pad, d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc,
@@ -3652,7 +3663,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         std::vector<double> hnorm(3 * B), hpower((size_t)B * xN);
         std::vector<PartScalars> hps2(B);
         std::vector<int> hflag(B);   // candidate path: particles sent to the radix select
-        unsigned hprune[2] = {0u, 0u};   // pruned coarse pass: tiles skipped, tiles checked
+        unsigned hprune = 0u;   // pruned coarse pass: the tiles its full GEMM computed
         auto post_part = [&]() -> int {
         T2X_TRY(d->sig_pairs.ensure(sizeof(int2) * cap));
         T2X_TRY(d->sig_orient.ensure(sizeof(int) * cap));
@@ -4021,7 +4032,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
             rb.add(hpower.data(), d->power.ptr, sizeof(double) * B * xN);
             rb.add(hps2.data(), d_ps, sizeof(PartScalars) * B);
             if (cand_path) rb.add(hflag.data(), d_radix, sizeof(int) * B);
-            if (prune_batch) rb.add(hprune, d->prune_cnt.ptr, sizeof(unsigned) * 2);
+            if (prune_batch) rb.add(&hprune, d->prune_cnt.ptr, sizeof(unsigned));
             T2X_TRY(rb.run(d, st));
         }
         if (hplan.err_code) {
@@ -4041,10 +4052,12 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
         nitems = hplan.nitems;
         if (bp_bricks) stt.bp_brick_items += (double)std::min(nitems, bp_item_cap);
         if (prune_batch) {
-            stt.coarse_tiles_pruned += hprune[0];
-            stt.coarse_tiles_checked += hprune[1];
+            const double skipped = (double)prune_ntiles - (double)hprune;
+            stt.coarse_tiles_pruned += skipped;
+            stt.coarse_tiles_checked += (double)prune_ntiles;
+            stt.coarse_bound_flop += 4.0 * (double)T * (double)O_all * (double)(K_low / 2) * B;
             // a pass that prunes little costs more than the exhaustive one: off for the call's rest
-            if (hprune[1] > 0 && hprune[0] < 0.4 * hprune[1]) prune_call = false;
+            if (prune_ntiles > 0 && skipped < 0.4 * (double)prune_ntiles) prune_call = false;
         }
         if (!cand_path) {
             stt.n_select_fallbacks += 1;

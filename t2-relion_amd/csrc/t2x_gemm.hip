@@ -38,42 +38,37 @@ __device__ __forceinline__ void tile_coords(int pid, int tiles_m, int tiles_n, i
     tn = in_group / gsize;
 }
 
-// Pruned coarse pass (EpiParams prune_lb): whether the tile rows [m0, m0 + rows) x columns
-// [n0, n0 + cols) can stay uncomputed, i.e. every 64-column block of every valid row has its
-// lower bound above the row's particle's seed by more than the negligible-weight cut; if so the
-// blocks get their bounds as minima. Uniform over the workgroup; called before the tile's first
-// memory operation.
-__device__ bool prune_tile(const EpiParams& ep, long long m0, int rows, long long n0, int cols, int tid,
-                           int nthreads)
+// Pruned coarse pass, tile plan: one wave per tile of tile_rows x tile_cols (tile_cols a multiple
+// of 64). A tile stays uncomputed when every 64-column block of every valid row has its lower
+// bound lb more than the negligible-weight cut (60, plus 1 for rounding) above the row's
+// particle's seed (an upper bound of its minimum diff2); its blocks then get their bounds as
+// minima. The other tiles are appended to `list` (count[0] of them), in no particular order.
+__global__ __launch_bounds__(256) void k_prune_plan(const float* __restrict__ lb, const float* __restrict__ seed,
+                                                    float* __restrict__ rowmin, int rowmin_ld, long long M, int T,
+                                                    int tile_rows, int tile_cols, int tiles_m, int tiles_n,
+                                                    unsigned* __restrict__ list, unsigned* __restrict__ count)
 {
-    __shared__ int s_keep;
-    if (tid == 0) s_keep = 0;
-    __syncthreads();
-    const int nblk = cols / 64;
-    const long long blk0 = n0 / 64;
+    const unsigned tile = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (tile >= (unsigned)tiles_m * (unsigned)tiles_n) return;   // uniform over the wave
+    const int nblk = tile_cols / 64;
+    const long long m0 = (long long)(tile / (unsigned)tiles_n) * tile_rows;
+    const long long blk0 = (long long)(tile % (unsigned)tiles_n) * nblk;
     bool keep = false;
-    for (int q = tid; q < rows * nblk; q += nthreads) {
-        const long long m = m0 + q / nblk;
-        const long long blk = blk0 + q % nblk;
-        if (m >= ep.M || blk >= ep.rowmin_ld) continue;
-        const float lb = ep.prune_lb[m * ep.rowmin_ld + blk];
-        if (!(lb > ep.prune_seed[m / ep.T] + 61.f)) keep = true;
+    for (int q = lane; q < tile_rows * nblk; q += 64) {
+        const long long m = m0 + q / nblk, blk = blk0 + q % nblk;
+        if (m >= M || blk >= rowmin_ld) continue;
+        if (!(lb[m * rowmin_ld + blk] > seed[m / T] + 61.f)) keep = true;
     }
-    if (keep) s_keep = 1;
-    __syncthreads();
-    const bool skip = s_keep == 0;
-    if (skip)
-        for (int q = tid; q < rows * nblk; q += nthreads) {
-            const long long m = m0 + q / nblk;
-            const long long blk = blk0 + q % nblk;
-            if (m >= ep.M || blk >= ep.rowmin_ld) continue;
-            ep.rowmin[m * ep.rowmin_ld + blk] = ep.prune_lb[m * ep.rowmin_ld + blk];
-        }
-    if (tid == 0) {
-        atomicAdd(&ep.prune_count[1], 1u);
-        if (skip) atomicAdd(&ep.prune_count[0], 1u);
+    if (__any(keep)) {
+        if (lane == 0) list[atomicAdd(&count[0], 1u)] = tile;
+        return;
     }
-    return skip;
+    for (int q = lane; q < tile_rows * nblk; q += 64) {
+        const long long m = m0 + q / nblk, blk = blk0 + q % nblk;
+        if (m >= M || blk >= rowmin_ld) continue;
+        rowmin[m * rowmin_ld + blk] = lb[m * rowmin_ld + blk];
+    }
 }
 
 // Shared epilogue of the GEMM kernels for one wave's block of MI x 4 MFMA tiles (lane holds
@@ -772,24 +767,18 @@ __device__ __forceinline__ void gemm_epilogue_staged(const floatx4 (&acc)[MI][4]
 
 // NP = 3: Al Bh + Ah Bl + Ah Bh; NP = 2 (FP16X2): Al Bh + Ah Bh, the B operand's hi half alone
 // (its lo half neither loaded nor read).
-template <class H, int EPI, int NP = 3>
-__global__ __launch_bounds__(512) void k_gemm_x3p(SplitOperand A, SplitOperand B, int K, int tiles_m,
-                                                  int tiles_n, int group_m, EpiParams ep,
-                                                  const int* __restrict__ rowmap)
+// One tile, rows [m0, m0 + WBM) x columns [n0, n0 + WBN): stage s of the K loop lives in st0
+// (s even) or st1 (s odd); [operand][row * 32 + chunk] with operands Ah, Al, Bh, Bl and 16-byte
+// chunks swizzled as swz().
+template <class H, int EPI, int NP>
+__device__ __forceinline__ void x3p_tile(H (&st0)[4][WBM * 32], H (&st1)[4][WBM * 32], EpiStage& es,
+                                         const SplitOperand& A, const SplitOperand& B, int K, long long m0,
+                                         long long n0, const EpiParams& ep, const int* __restrict__ rowmap)
 {
     typedef typename Vec8<H>::type v8;
-    // stage s of the K loop lives in st0 (s even) or st1 (s odd); [operand][row * 32 + chunk]
-    // with operands Ah, Al, Bh, Bl and 16-byte chunks swizzled as swz()
-    __shared__ __attribute__((aligned(16))) H st0[4][WBM * 32];
-    __shared__ __attribute__((aligned(16))) H st1[4][WBM * 32];
-    __shared__ EpiStage es;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave & 1, wn = wave >> 1;
-    int tm, tn;
-    tile_coords(blockIdx.x, tiles_m, tiles_n, group_m, tm, tn);
-    const long long m0 = (long long)tm * WBM, n0 = (long long)tn * WBN;
-    if (EPI == EPI_DIFF2 && ep.prune_lb && !rowmap && prune_tile(ep, m0, WBM, n0, WBN, tid, 512)) return;
 
     floatx4 acc[8][4];
 #pragma unroll
@@ -911,6 +900,31 @@ __global__ __launch_bounds__(512) void k_gemm_x3p(SplitOperand A, SplitOperand B
         gemm_epilogue<8, EPI>(acc, m0 + wm * 128, n0 + wn * 64, lane, A.inv_scale, B.inv_scale, ep, rowmap);
 }
 
+// LIST (the pruned coarse pass): a persistent grid over the tiles listed in ep.tile_list
+// (*ep.tile_count of them, indices tm * tiles_n + tn), no row map.
+template <class H, int EPI, int NP = 3, bool LIST = false>
+__global__ __launch_bounds__(512) void k_gemm_x3p(SplitOperand A, SplitOperand B, int K, int tiles_m,
+                                                  int tiles_n, int group_m, EpiParams ep,
+                                                  const int* __restrict__ rowmap)
+{
+    __shared__ __attribute__((aligned(16))) H st0[4][WBM * 32];
+    __shared__ __attribute__((aligned(16))) H st1[4][WBM * 32];
+    __shared__ EpiStage es;
+    if constexpr (LIST) {
+        const unsigned n = *ep.tile_count;
+        for (unsigned it = blockIdx.x; it < n; it += gridDim.x) {
+            const unsigned t = ep.tile_list[it];
+            x3p_tile<H, EPI, NP>(st0, st1, es, A, B, K, (long long)(t / (unsigned)tiles_n) * WBM,
+                                 (long long)(t % (unsigned)tiles_n) * WBN, ep, nullptr);
+            __syncthreads();   // the next tile reuses the LDS stages
+        }
+    } else {
+        int tm, tn;
+        tile_coords(blockIdx.x, tiles_m, tiles_n, group_m, tm, tn);
+        x3p_tile<H, EPI, NP>(st0, st1, es, A, B, K, (long long)tm * WBM, (long long)tn * WBN, ep, rowmap);
+    }
+}
+
 // Two-product half-height variant (FP16X2 only): tile 128 (M) x 256 (N), 4 waves of 64 x 128,
 // 2 stages of 32 KB (A hi, A lo: 128 rows; B hi: 256 rows) + a 9 KB epilogue stage, so two
 // workgroups share a compute unit: one workgroup's K-step barriers and epilogue overlap the
@@ -980,23 +994,16 @@ __device__ __forceinline__ bool stage_epilogue_h(EpiStageH& es, long long m0l, l
 // ONE: the bounds form of the pruned coarse pass, one product (A hi . B hi) per element; A lo is
 // neither loaded nor multiplied (the caller allows for the dropped term in its bound).
 template <bool ONE>
-__global__ __launch_bounds__(256, 2) void k_gemm_x2h(SplitOperand A, SplitOperand B, int K, int tiles_m, int tiles_n,
-                                                     int group_m, EpiParams ep, const int* __restrict__ rowmap)
+__device__ __forceinline__ void x2h_tile(_Float16 (&sa0)[2][128 * 32], _Float16 (&sb0)[256 * 32],
+                                         _Float16 (&sa1)[2][128 * 32], _Float16 (&sb1)[256 * 32], EpiStageH& es,
+                                         const SplitOperand& A, const SplitOperand& B, int K, long long m0,
+                                         long long n0, const EpiParams& ep, const int* __restrict__ rowmap)
 {
     typedef _Float16 H;
     typedef Vec8<H>::type v8;
-    __shared__ __attribute__((aligned(16))) H sa0[2][128 * 32];   // stage 0: A hi, A lo
-    __shared__ __attribute__((aligned(16))) H sb0[256 * 32];      // stage 0: B hi
-    __shared__ __attribute__((aligned(16))) H sa1[2][128 * 32];
-    __shared__ __attribute__((aligned(16))) H sb1[256 * 32];
-    __shared__ EpiStageH es;
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave & 1, wn = wave >> 1;
-    int tm, tn;
-    tile_coords(blockIdx.x, tiles_m, tiles_n, group_m, tm, tn);
-    const long long m0 = (long long)tm * 128, n0 = (long long)tn * 256;
-    if (ep.prune_lb && !rowmap && prune_tile(ep, m0, 128, n0, 256, tid, 256)) return;
 
     // acc[h][i][j]: rows 16 i of the wave's 64, columns 64 h + 16 j of its 128
     floatx4 acc[2][4][4];
@@ -1106,6 +1113,20 @@ __global__ __launch_bounds__(256, 2) void k_gemm_x2h(SplitOperand A, SplitOperan
         gemm_epilogue<4, EPI_DIFF2>(acc[1], m0 + wm * 64, n0 + wn * 128 + 64, lane, A.inv_scale, B.inv_scale, ep,
                                     rowmap);
     }
+}
+
+template <bool ONE>
+__global__ __launch_bounds__(256, 2) void k_gemm_x2h(SplitOperand A, SplitOperand B, int K, int tiles_m, int tiles_n,
+                                                     int group_m, EpiParams ep, const int* __restrict__ rowmap)
+{
+    __shared__ __attribute__((aligned(16))) _Float16 sa0[2][128 * 32];   // stage 0: A hi, A lo
+    __shared__ __attribute__((aligned(16))) _Float16 sb0[256 * 32];      // stage 0: B hi
+    __shared__ __attribute__((aligned(16))) _Float16 sa1[2][128 * 32];
+    __shared__ __attribute__((aligned(16))) _Float16 sb1[256 * 32];
+    __shared__ EpiStageH es;
+    int tm, tn;
+    tile_coords(blockIdx.x, tiles_m, tiles_n, group_m, tm, tn);
+    x2h_tile<ONE>(sa0, sb0, sa1, sb1, es, A, B, K, (long long)tm * 128, (long long)tn * 256, ep, rowmap);
 }
 
 // Learned-proposal variant: one workgroup per work item = (bin, row tile, column tile). The
@@ -1498,8 +1519,8 @@ int launch_gemm_h1_bounds(const void* Ah, const float* As, long long lda, long l
         set_error("gemm_h1_bounds: operands beyond 4 GiB (32-bit offsets)");
         return T2X_EINVAL;
     }
-    if (ep.C != nullptr || ep.pmin != nullptr || ep.prune_lb != nullptr) {
-        set_error("gemm_h1_bounds: a bounds-only launch (no stores, no lazy minima, no pruning)");
+    if (ep.C != nullptr || ep.pmin != nullptr) {
+        set_error("gemm_h1_bounds: a bounds-only launch (no stores, no lazy minima)");
         return T2X_EINVAL;
     }
     if (K % XK != 0 || Mrows_pad % 128 != 0 || Nrows_pad % 256 != 0) {
@@ -1511,6 +1532,50 @@ int launch_gemm_h1_bounds(const void* Ah, const float* As, long long lda, long l
     if (nwg > 0x7fffffffLL) { set_error("gemm: grid too large"); return T2X_EINVAL; }
     SplitOperand A{Ah, nullptr, As, lda}, B{Bh, nullptr, Bs, ldb};
     k_gemm_x2h<true><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, 32, ep, nullptr);
+    T2X_HIP(hipGetLastError());
+    return T2X_OK;
+}
+
+// Pruned coarse pass, full GEMM: the tile plan (k_prune_plan over the tiles of the kernel that
+// follows, from the block bounds lb and the per-particle seeds) and then the diff2 GEMM over the
+// kept tiles: k_gemm_x3p (256x256 tiles; 2 products under FP16X2, else 3) on a persistent grid of
+// one workgroup per compute unit. list holds list_cap entries; count (one word) must be zero on
+// entry and holds the kept tiles' number on return.
+int launch_gemm_pruned(int prec, const void* Ah, const void* Al, const float* As, long long lda, long long a_rows,
+                       const void* Bh, const void* Bl, const float* Bs, long long ldb, int K, int Mrows_pad,
+                       int Nrows_pad, const EpiParams& ep, const float* lb, const float* seed, unsigned* list,
+                       long long list_cap, unsigned* count, int n_cu, hipStream_t st)
+{
+    const long long lim = 1LL << 32;
+    if (a_rows * lda * 2 >= lim || (long long)Nrows_pad * ldb * 2 >= lim) {
+        set_error("gemm_pruned: operands beyond 4 GiB (32-bit offsets)");
+        return T2X_EINVAL;
+    }
+    const int tr = WBM, tc = WBN;
+    if (K % XK != 0 || Mrows_pad % tr != 0 || Nrows_pad % tc != 0) {
+        set_error("gemm_pruned: operands not padded to tile multiples");
+        return T2X_EINVAL;
+    }
+    if (ep.C == nullptr || ep.rowmin == nullptr || ep.T <= 0 || n_cu <= 0) {
+        set_error("gemm_pruned: a diff2 launch with block minima");
+        return T2X_EINVAL;
+    }
+    const int tm = Mrows_pad / tr, tn = Nrows_pad / tc;
+    const long long ntiles = (long long)tm * tn;
+    if (ntiles > list_cap || ntiles > 0x7fffffffLL) { set_error("gemm_pruned: tile list too small"); return T2X_EINVAL; }
+    k_prune_plan<<<(unsigned)((ntiles + 3) / 4), 256, 0, st>>>(lb, seed, ep.rowmin, ep.rowmin_ld, ep.M, ep.T, tr, tc,
+                                                               tm, tn, list, count);
+    T2X_HIP(hipGetLastError());
+    EpiParams el = ep;
+    el.tile_list = list;
+    el.tile_count = count;
+    SplitOperand A{Ah, Al, As, lda}, B{Bh, Bl, Bs, ldb};
+    if (prec == T2X_PREC_FP16X2)
+        k_gemm_x3p<_Float16, EPI_DIFF2, 2, true><<<(unsigned)n_cu, 512, 0, st>>>(A, B, K, tm, tn, 0, el, nullptr);
+    else if (prec == T2X_PREC_BF16X3)
+        k_gemm_x3p<__bf16, EPI_DIFF2, 3, true><<<(unsigned)n_cu, 512, 0, st>>>(A, B, K, tm, tn, 0, el, nullptr);
+    else
+        k_gemm_x3p<_Float16, EPI_DIFF2, 3, true><<<(unsigned)n_cu, 512, 0, st>>>(A, B, K, tm, tn, 0, el, nullptr);
     T2X_HIP(hipGetLastError());
     return T2X_OK;
 }

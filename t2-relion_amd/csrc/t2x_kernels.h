@@ -160,14 +160,10 @@ struct EpiParams {
     // stored (the block minimum still is, so every reader skips the block). null: store all.
     unsigned int* pmin;
     // Pruned coarse pass. C null: a bounds-only launch (EPI_DIFF2 writes the 64-column block
-    // minima, no values). prune_lb non-null: a tile whose rows' 64-column blocks all have a lower
-    // bound prune_lb[m][blk] (the low-frequency diff2 block minimum) above the row's particle's
-    // prune_seed (an upper bound of its minimum) by more than the negligible-weight cut is not
-    // computed; its blocks get the bound as their minimum. prune_count[0] counts skipped tiles,
-    // prune_count[1] the tiles that checked.
-    const float* prune_lb;
-    const float* prune_seed;
-    unsigned int* prune_count;
+    // minima, no values). tile_list: the tiles a list-driven (persistent) launch computes,
+    // *tile_count of them (read by the kernels of the LIST form only).
+    const unsigned int* tile_list;
+    const unsigned int* tile_count;
 };
 
 struct SelParams {

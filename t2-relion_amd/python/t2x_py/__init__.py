@@ -130,7 +130,7 @@ class Stats(C.Structure):
                 ("local_rows", C.c_double), ("local_slices", C.c_double),
                 ("ms_graph", C.c_double), ("n_graph_launches", C.c_long),
                 ("bp_brick_items", C.c_double), ("coarse_tiles_checked", C.c_double),
-                ("coarse_tiles_pruned", C.c_double)]
+                ("coarse_tiles_pruned", C.c_double), ("coarse_bound_flop", C.c_double)]
 
 
 class MrcInfo(C.Structure):
