@@ -2977,7 +2977,8 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
     for (int b0 = 0; b0 < n; b0 += Bmax) {
         const int B = std::min(Bmax, n - b0);
         const bool prune_batch = prune_call;   // the pruned coarse pass for this batch
-        long long prune_ntiles = 0;            // its full GEMM's tiles
+        // its full GEMM's tiles (the 256x256 tiles of k_gemm_x3p), known here for graph replays too
+        const long long prune_ntiles = prune_batch ? (round_up((long long)B * T, 256) / 256) * (O_pad / 256) : 0;
         const size_t NN = (size_t)N * N;
         int ev = 0;
         (void)hipEventRecord(d->ev[ev++], st);
@@ -3529,7 +3530,6 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                     // 3. the full pass computes the tiles with a block not beyond the cut (below)
                     T2X_TRY(d->prune_cnt.ensure(sizeof(unsigned)));
                     T2X_TRY(zero_async(d->prune_cnt.ptr, sizeof(unsigned), st));
-                    prune_ntiles = (M_pad / 256) * (O_pad / 256);   // the 256x256 tiles of k_gemm_x3p
                     T2X_TRY(d->prune_list.ensure(sizeof(unsigned) * prune_ntiles));
                 }
                 if (prec == T2X_PREC_FP32) {

@@ -590,10 +590,12 @@ __device__ __forceinline__ void lds_dma16(const H* src, H* lds_dst)
 template <class H>
 __device__ __forceinline__ void x3_group(floatx4 (&acc)[4], typename Vec8<H>::type al, typename Vec8<H>::type ah,
                                          const typename Vec8<H>::type (&bh)[4], const typename Vec8<H>::type (&bl)[4],
-                                         bool with_bl = true)
+                                         bool with_bl = true, bool with_al = true)
 {
+    if (with_al) {
 #pragma unroll
-    for (int j = 0; j < 4; j++) acc[j] = mfma16x16x32<H>(al, bh[j], acc[j]);
+        for (int j = 0; j < 4; j++) acc[j] = mfma16x16x32<H>(al, bh[j], acc[j]);
+    }
     if (with_bl) {
 #pragma unroll
         for (int j = 0; j < 4; j++) acc[j] = mfma16x16x32<H>(ah, bl[j], acc[j]);
@@ -766,7 +768,8 @@ __device__ __forceinline__ void gemm_epilogue_staged(const floatx4 (&acc)[MI][4]
 }
 
 // NP = 3: Al Bh + Ah Bl + Ah Bh; NP = 2 (FP16X2): Al Bh + Ah Bh, the B operand's hi half alone
-// (its lo half neither loaded nor read).
+// (its lo half neither loaded nor read); NP = 1 (the bounds form of the pruned coarse pass):
+// Ah Bh alone.
 // One tile, rows [m0, m0 + WBM) x columns [n0, n0 + WBN): stage s of the K loop lives in st0
 // (s even) or st1 (s odd); [operand][row * 32 + chunk] with operands Ah, Al, Bh, Bl and 16-byte
 // chunks swizzled as swz().
@@ -811,8 +814,10 @@ __device__ __forceinline__ void x3p_tile(H (&st0)[4][WBM * 32], H (&st1)[4][WBM 
         const long long kb = (long long)(k) * (long long)sizeof(H);                               \
         lds_dma16<H>(reinterpret_cast<const H*>(Ah + kb + oa0), &ST[0][rb0 * 32]);               \
         lds_dma16<H>(reinterpret_cast<const H*>(Ah + kb + oa1), &ST[0][rb1 * 32]);               \
-        lds_dma16<H>(reinterpret_cast<const H*>(Al + kb + oa0), &ST[1][rb0 * 32]);               \
-        lds_dma16<H>(reinterpret_cast<const H*>(Al + kb + oa1), &ST[1][rb1 * 32]);               \
+        if (NP >= 2) {                                                                            \
+            lds_dma16<H>(reinterpret_cast<const H*>(Al + kb + oa0), &ST[1][rb0 * 32]);           \
+            lds_dma16<H>(reinterpret_cast<const H*>(Al + kb + oa1), &ST[1][rb1 * 32]);           \
+        }                                                                                         \
         lds_dma16<H>(reinterpret_cast<const H*>(Bh + kb + ob0), &ST[2][rb0 * 32]);               \
         lds_dma16<H>(reinterpret_cast<const H*>(Bh + kb + ob1), &ST[2][rb1 * 32]);               \
         if (NP == 3) {                                                                            \
@@ -835,7 +840,8 @@ __device__ __forceinline__ void x3p_tile(H (&st0)[4][WBM * 32], H (&st1)[4][WBM 
 #define X3P_READ_A(ST, i, AH, AL)                                                \
     do {                                                                         \
         AH = *reinterpret_cast<const v8*>(&ST[0][oa + (i) * 512]);               \
-        AL = *reinterpret_cast<const v8*>(&ST[1][oa + (i) * 512]);               \
+        if (NP >= 2) AL = *reinterpret_cast<const v8*>(&ST[1][oa + (i) * 512]);  \
+        else AL = v8{};                                                          \
     } while (0)
 
     // One K step on stage CUR (fragments bh/bl and A block 0 already in registers): A blocks
@@ -853,7 +859,7 @@ __device__ __forceinline__ void x3p_tile(H (&st0)[4][WBM * 32], H (&st1)[4][WBM 
                 X3P_READ_A(CUR, 6, a6h, a6l);                                    \
                 X3P_READ_A(CUR, 7, a7h, a7l);                                    \
             }                                                                    \
-            x3_group<H>(acc[i], al, ah, bh, bl, NP == 3);                        \
+            x3_group<H>(acc[i], al, ah, bh, bl, NP == 3, NP >= 2);               \
             if (i < 5) { ah = nh; al = nl; }                                     \
         }                                                                        \
         __builtin_amdgcn_s_waitcnt(0);                                           \
@@ -864,10 +870,10 @@ __device__ __forceinline__ void x3p_tile(H (&st0)[4][WBM * 32], H (&st1)[4][WBM 
         /* groups 6 and 7 column by column: B fragment j of this step is dead */ \
         /* after column j, and fragment j of the next step takes its registers */ \
         _Pragma("unroll") for (int j = 0; j < 4; j++) {                          \
-            acc[6][j] = mfma16x16x32<H>(a6l, bh[j], acc[6][j]);                  \
+            if (NP >= 2) acc[6][j] = mfma16x16x32<H>(a6l, bh[j], acc[6][j]);     \
             if (NP == 3) acc[6][j] = mfma16x16x32<H>(a6h, bl[j], acc[6][j]);     \
             acc[6][j] = mfma16x16x32<H>(a6h, bh[j], acc[6][j]);                  \
-            acc[7][j] = mfma16x16x32<H>(a7l, bh[j], acc[7][j]);                  \
+            if (NP >= 2) acc[7][j] = mfma16x16x32<H>(a7l, bh[j], acc[7][j]);     \
             if (NP == 3) acc[7][j] = mfma16x16x32<H>(a7h, bl[j], acc[7][j]);     \
             acc[7][j] = mfma16x16x32<H>(a7h, bh[j], acc[7][j]);                  \
             if (nxt) {                                                           \
@@ -1527,11 +1533,18 @@ int launch_gemm_h1_bounds(const void* Ah, const float* As, long long lda, long l
         set_error("gemm_h1_bounds: operands not padded to tile multiples");
         return T2X_EINVAL;
     }
-    const int tm = Mrows_pad / 128, tn = Nrows_pad / 256;
+    const char* env_t = getenv("T2X_BOUNDS_TILE");   // 259: k_gemm_x2h (128x256); 258: k_gemm_x3p (256x256)
+    const bool wide = env_t && atoi(env_t) == 258 && Mrows_pad % 256 == 0;
+    const int tm = Mrows_pad / (wide ? 256 : 128), tn = Nrows_pad / 256;
     const long long nwg = (long long)tm * tn;
     if (nwg > 0x7fffffffLL) { set_error("gemm: grid too large"); return T2X_EINVAL; }
     SplitOperand A{Ah, nullptr, As, lda}, B{Bh, nullptr, Bs, ldb};
-    k_gemm_x2h<true><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, 32, ep, nullptr);
+    const char* env_g = getenv("T2X_BOUNDS_GROUP");   // tile-walk group (tile rows per group)
+    const int g = env_g && atoi(env_g) > 0 ? atoi(env_g) : 32;
+    if (wide)
+        k_gemm_x3p<_Float16, EPI_DIFF2, 1><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, g, ep, nullptr);
+    else
+        k_gemm_x2h<true><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, g, ep, nullptr);
     T2X_HIP(hipGetLastError());
     return T2X_OK;
 }

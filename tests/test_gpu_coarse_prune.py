@@ -93,3 +93,21 @@ def test_pruned_full_split_bounds(gpu_device, monkeypatch):
     print(f"\n[prune] class256, full-split bounds: {cnt[0]:.0f} of {cnt[1]:.0f} tiles pruned")
     assert cnt[0] > 0.5 * cnt[1]
     same(pr, ex)
+
+
+def test_pruned_graph_replays_count_tiles(gpu_device, monkeypatch):
+    """Batches replayed from a graph prune as the eager ones, and the tile counters see them."""
+    lib = T.load()
+    monkeypatch.setenv("T2X_BATCH", "4")   # 16 particles: a captured batch and two replays
+    wl = H.Workload("class256", n=16, seed_offset=85, precision=T.T2X_PREC_FP16X2)
+    eager, cnt_e = run(wl, gpu_device, False)
+    wl.cfg.launch_graph = 1
+    graph, cnt_g = run(wl, gpu_device, False)
+    wl.cfg.launch_graph = 0
+    st = T.Stats()
+    T.check(lib.t2x_stats_get(gpu_device, C.byref(st)))
+    print(f"\n[prune] graph: {st.n_graph_launches} graph launches, {cnt_g[0]:.0f} of {cnt_g[1]:.0f} tiles "
+          f"pruned (eager {cnt_e[0]:.0f} of {cnt_e[1]:.0f})")
+    assert st.n_graph_launches >= 2
+    assert cnt_g == cnt_e
+    same(graph, eager)
