@@ -3491,14 +3491,12 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                         k_prune_n1<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(
                             d->prune_n1.as<float>(), ep.N1, O_pad, (int)O_pad, d_ps, ne, h1 ? 1. / 512. : 0.);
                     }
-                    T2X_TRY(d->prune_lb.ensure(sizeof(float) * M_pad * rowmin_ld));
-                    k_fill_f32<<<(unsigned)((M_pad * rowmin_ld + 255) / 256), 256, 0, st>>>(
-                        d->prune_lb.as<float>(), 3.0e38f, M_pad * rowmin_ld);
                     T2X_HIP(hipGetLastError());
+                    // the bounds go into the block minima themselves: a pruned tile keeps them, a
+                    // kept tile's full GEMM overwrites them
                     EpiParams eb = ep;
                     eb.C = nullptr;
                     eb.N1 = d->prune_n1.as<float>();
-                    eb.rowmin = d->prune_lb.as<float>();
                     eb.pmin = nullptr;
                     if (h1)
                         T2X_TRY(launch_gemm_h1_bounds(d->Ah.ptr, d->As.as<float>(), Kc, M_pad, d->coarse_Bh.ptr,
@@ -3515,13 +3513,13 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                     const int nblk = (int)((O_all + 63) / 64);
                     if (prec == T2X_PREC_BF16X3)
                         k_prune_seed<__bf16><<<B, 256, 0, st>>>(
-                            d->prune_lb.as<float>(), rowmin_ld, nblk, (int)T, (const __bf16*)d->Ah.ptr,
+                            d_rowmin.as<float>(), rowmin_ld, nblk, (int)T, (const __bf16*)d->Ah.ptr,
                             (const __bf16*)d->Al.ptr, d->As.as<float>(), (const __bf16*)d->coarse_Bh.ptr,
                             (const __bf16*)d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc, ep.Cb, ep.N1, O_pad,
                             ep.prior_o, O_all, d->prune_seed.as<float>(), d->pmin.as<unsigned>());
                     else
                         k_prune_seed<_Float16><<<B, 256, 0, st>>>(
-                            d->prune_lb.as<float>(), rowmin_ld, nblk, (int)T, (const _Float16*)d->Ah.ptr,
+                            d_rowmin.as<float>(), rowmin_ld, nblk, (int)T, (const _Float16*)d->Ah.ptr,
                             (const _Float16*)d->Al.ptr, d->As.as<float>(), (const _Float16*)d->coarse_Bh.ptr,
                             prec == T2X_PREC_FP16X2 ? nullptr : (const _Float16*)d->coarse_Bl.ptr,
                             d->coarse_Bs.as<float>(), Kc, Kc, ep.Cb, ep.N1, O_pad, ep.prior_o, O_all,
@@ -3554,7 +3552,7 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                     } else if (prune_batch) {
                         T2X_TRY(launch_gemm_pruned(prec, d->Ah.ptr, d->Al.ptr, d->As.as<float>(), Kc, M_pad,
                                                    d->coarse_Bh.ptr, d->coarse_Bl.ptr, d->coarse_Bs.as<float>(), Kc, Kc,
-                                                   (int)M_pad, (int)O_pad, ep, d->prune_lb.as<float>(),
+                                                   (int)M_pad, (int)O_pad, ep, d_rowmin.as<float>(),
                                                    d->prune_seed.as<float>(), d->prune_list.as<unsigned>(),
                                                    prune_ntiles, d->prune_cnt.as<unsigned>(), d->n_cu, st));
                         gused = GemmChoice{258, 0};   // group 0: the tile list

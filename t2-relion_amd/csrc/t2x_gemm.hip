@@ -42,9 +42,10 @@ __device__ __forceinline__ void tile_coords(int pid, int tiles_m, int tiles_n, i
 // of 64). A tile stays uncomputed when every 64-column block of every valid row has its lower
 // bound lb more than the negligible-weight cut (60, plus 1 for rounding) above the row's
 // particle's seed (an upper bound of its minimum diff2); its blocks then get their bounds as
-// minima. The other tiles are appended to `list` (count[0] of them), in no particular order.
-__global__ __launch_bounds__(256) void k_prune_plan(const float* __restrict__ lb, const float* __restrict__ seed,
-                                                    float* __restrict__ rowmin, int rowmin_ld, long long M, int T,
+// minima (already in place when lb is the minima buffer itself). The other tiles are appended to
+// `list` (count[0] of them), in no particular order.
+__global__ __launch_bounds__(256) void k_prune_plan(const float* lb, const float* __restrict__ seed,
+                                                    float* rowmin, int rowmin_ld, long long M, int T,
                                                     int tile_rows, int tile_cols, int tiles_m, int tiles_n,
                                                     unsigned* __restrict__ list, unsigned* __restrict__ count)
 {
@@ -64,6 +65,7 @@ __global__ __launch_bounds__(256) void k_prune_plan(const float* __restrict__ lb
         if (lane == 0) list[atomicAdd(&count[0], 1u)] = tile;
         return;
     }
+    if (rowmin == lb) return;
     for (int q = lane; q < tile_rows * nblk; q += 64) {
         const long long m = m0 + q / nblk, blk = blk0 + q % nblk;
         if (m >= M || blk >= rowmin_ld) continue;
