@@ -606,6 +606,18 @@ __device__ __forceinline__ void x3_group(floatx4 (&acc)[4], typename Vec8<H>::ty
     for (int j = 0; j < 4; j++) acc[j] = mfma16x16x32<H>(ah, bh[j], acc[j]);
 }
 
+// NP = 4 (one product over two K chunks per stage): the hi halves of chunk 0 (ah, bh) and of
+// chunk 1 (al, bl, held in the lo slots).
+template <class H>
+__device__ __forceinline__ void x3_group2(floatx4 (&acc)[4], typename Vec8<H>::type a1, typename Vec8<H>::type a0,
+                                          const typename Vec8<H>::type (&b0)[4], const typename Vec8<H>::type (&b1)[4])
+{
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[j] = mfma16x16x32<H>(a1, b1[j], acc[j]);
+#pragma unroll
+    for (int j = 0; j < 4; j++) acc[j] = mfma16x16x32<H>(a0, b0[j], acc[j]);
+}
+
 // Epilogue inputs of one 256x256 tile, staged in LDS before the K loop so that their global
 // latency hides behind it (read per 16-row block after the loop, they left every wave waiting
 // on memory eight times per tile): per tile row its output row (-1: padding), particle, slot
@@ -770,8 +782,9 @@ __device__ __forceinline__ void gemm_epilogue_staged(const floatx4 (&acc)[MI][4]
 }
 
 // NP = 3: Al Bh + Ah Bl + Ah Bh; NP = 2 (FP16X2): Al Bh + Ah Bh, the B operand's hi half alone
-// (its lo half neither loaded nor read); NP = 1 (the bounds form of the pruned coarse pass):
-// Ah Bh alone.
+// (its lo half neither loaded nor read); NP = 1 (a bounds form of the pruned coarse pass):
+// Ah Bh alone; NP = 4 (the other bounds form): Ah Bh alone over two K chunks per stage, the lo
+// slots holding the hi halves of the second chunk (K a multiple of 64).
 // One tile, rows [m0, m0 + WBM) x columns [n0, n0 + WBN): stage s of the K loop lives in st0
 // (s even) or st1 (s odd); [operand][row * 32 + chunk] with operands Ah, Al, Bh, Bl and 16-byte
 // chunks swizzled as swz().
@@ -784,6 +797,7 @@ __device__ __forceinline__ void x3p_tile(H (&st0)[4][WBM * 32], H (&st1)[4][WBM 
     const int tid = threadIdx.x, lane = tid & 63;
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int wm = wave & 1, wn = wave >> 1;
+    constexpr int KS = NP == 4 ? 2 * XK : XK;   // K columns per stage
 
     floatx4 acc[8][4];
 #pragma unroll
@@ -816,7 +830,7 @@ __device__ __forceinline__ void x3p_tile(H (&st0)[4][WBM * 32], H (&st1)[4][WBM 
         const long long kb = (long long)(k) * (long long)sizeof(H);                               \
         lds_dma16<H>(reinterpret_cast<const H*>(Ah + kb + oa0), &ST[0][rb0 * 32]);               \
         lds_dma16<H>(reinterpret_cast<const H*>(Ah + kb + oa1), &ST[0][rb1 * 32]);               \
-        if (NP >= 2) {                                                                            \
+        if (NP == 2 || NP == 3) {                                                                 \
             lds_dma16<H>(reinterpret_cast<const H*>(Al + kb + oa0), &ST[1][rb0 * 32]);           \
             lds_dma16<H>(reinterpret_cast<const H*>(Al + kb + oa1), &ST[1][rb1 * 32]);           \
         }                                                                                         \
@@ -825,6 +839,13 @@ __device__ __forceinline__ void x3p_tile(H (&st0)[4][WBM * 32], H (&st1)[4][WBM 
         if (NP == 3) {                                                                            \
             lds_dma16<H>(reinterpret_cast<const H*>(Bl + kb + ob0), &ST[3][rb0 * 32]);           \
             lds_dma16<H>(reinterpret_cast<const H*>(Bl + kb + ob1), &ST[3][rb1 * 32]);           \
+        }                                                                                         \
+        if (NP == 4) {   /* the hi halves' next K chunk into the lo slots */                      \
+            const long long kc = kb + XK * (long long)sizeof(H);                                  \
+            lds_dma16<H>(reinterpret_cast<const H*>(Ah + kc + oa0), &ST[1][rb0 * 32]);           \
+            lds_dma16<H>(reinterpret_cast<const H*>(Ah + kc + oa1), &ST[1][rb1 * 32]);           \
+            lds_dma16<H>(reinterpret_cast<const H*>(Bh + kc + ob0), &ST[3][rb0 * 32]);           \
+            lds_dma16<H>(reinterpret_cast<const H*>(Bh + kc + ob1), &ST[3][rb1 * 32]);           \
         }                                                                                         \
     } while (0)
 
@@ -836,7 +857,7 @@ __device__ __forceinline__ void x3p_tile(H (&st0)[4][WBM * 32], H (&st1)[4][WBM 
     do {                                                                         \
         _Pragma("unroll") for (int j = 0; j < 4; j++) {                          \
             BH[j] = *reinterpret_cast<const v8*>(&ST[2][ob + j * 512]);          \
-            if (NP == 3) BL[j] = *reinterpret_cast<const v8*>(&ST[3][ob + j * 512]); \
+            if (NP >= 3) BL[j] = *reinterpret_cast<const v8*>(&ST[3][ob + j * 512]); \
         }                                                                        \
     } while (0)
 #define X3P_READ_A(ST, i, AH, AL)                                                \
@@ -861,36 +882,39 @@ __device__ __forceinline__ void x3p_tile(H (&st0)[4][WBM * 32], H (&st1)[4][WBM 
                 X3P_READ_A(CUR, 6, a6h, a6l);                                    \
                 X3P_READ_A(CUR, 7, a7h, a7l);                                    \
             }                                                                    \
-            x3_group<H>(acc[i], al, ah, bh, bl, NP == 3, NP >= 2);               \
+            if (NP == 4) x3_group2<H>(acc[i], al, ah, bh, bl);                    \
+            else x3_group<H>(acc[i], al, ah, bh, bl, NP == 3, NP >= 2);          \
             if (i < 5) { ah = nh; al = nl; }                                     \
         }                                                                        \
         __builtin_amdgcn_s_waitcnt(0);                                           \
         __syncthreads();                                                         \
-        if ((s) + 2 < nk) X3P_LOAD(CUR, ((s) + 2) * XK);                         \
+        if ((s) + 2 < nk) X3P_LOAD(CUR, ((s) + 2) * KS);                         \
         const bool nxt = (s) + 1 < nk;                                           \
         if (nxt) X3P_READ_A(NXT, 0, ah, al);                                     \
         /* groups 6 and 7 column by column: B fragment j of this step is dead */ \
         /* after column j, and fragment j of the next step takes its registers */ \
         _Pragma("unroll") for (int j = 0; j < 4; j++) {                          \
-            if (NP >= 2) acc[6][j] = mfma16x16x32<H>(a6l, bh[j], acc[6][j]);     \
+            if (NP == 2 || NP == 3) acc[6][j] = mfma16x16x32<H>(a6l, bh[j], acc[6][j]); \
             if (NP == 3) acc[6][j] = mfma16x16x32<H>(a6h, bl[j], acc[6][j]);     \
+            if (NP == 4) acc[6][j] = mfma16x16x32<H>(a6l, bl[j], acc[6][j]);     \
             acc[6][j] = mfma16x16x32<H>(a6h, bh[j], acc[6][j]);                  \
-            if (NP >= 2) acc[7][j] = mfma16x16x32<H>(a7l, bh[j], acc[7][j]);     \
+            if (NP == 2 || NP == 3) acc[7][j] = mfma16x16x32<H>(a7l, bh[j], acc[7][j]); \
             if (NP == 3) acc[7][j] = mfma16x16x32<H>(a7h, bl[j], acc[7][j]);     \
+            if (NP == 4) acc[7][j] = mfma16x16x32<H>(a7l, bl[j], acc[7][j]);     \
             acc[7][j] = mfma16x16x32<H>(a7h, bh[j], acc[7][j]);                  \
             if (nxt) {                                                           \
                 bh[j] = *reinterpret_cast<const v8*>(&NXT[2][ob + j * 512]);     \
-                if (NP == 3) bl[j] = *reinterpret_cast<const v8*>(&NXT[3][ob + j * 512]); \
+                if (NP >= 3) bl[j] = *reinterpret_cast<const v8*>(&NXT[3][ob + j * 512]); \
             }                                                                    \
         }                                                                        \
     } while (0)
 
-    const int nk = K / XK;
+    const int nk = K / KS;
     X3P_LOAD(st0, 0);
     const bool staged = stage_epilogue<EPI>(es, m0, n0, A.inv_scale, B.inv_scale, ep, rowmap, tid);
     __builtin_amdgcn_s_waitcnt(0);
     __syncthreads();
-    if (nk > 1) X3P_LOAD(st1, XK);
+    if (nk > 1) X3P_LOAD(st1, KS);
     v8 bh[4], bl[4], ah, al;
     X3P_READ_B(st0, bh, bl);
     X3P_READ_A(st0, 0, ah, al);
@@ -1517,7 +1541,9 @@ static int launch_x3_variant(int prec, int epi, const SplitOperand& A, const Spl
 }
 
 // Bounds form of the pruned coarse pass: block minima of the diff2 with one FP16 product per
-// element (A hi . B hi), no values stored (ep.C null), 128x256 tiles.
+// element (A hi . B hi), no values stored (ep.C null): k_gemm_x3p with two K chunks per LDS stage
+// (the stage's lo slots hold the second chunk's hi halves), so each barrier and operand wait
+// covers twice the MFMA work of a one-chunk stage.
 int launch_gemm_h1_bounds(const void* Ah, const float* As, long long lda, long long a_rows, const void* Bh,
                           const float* Bs, long long ldb, int K, int Mrows_pad, int Nrows_pad, const EpiParams& ep,
                           hipStream_t st)
@@ -1535,15 +1561,20 @@ int launch_gemm_h1_bounds(const void* Ah, const float* As, long long lda, long l
         set_error("gemm_h1_bounds: operands not padded to tile multiples");
         return T2X_EINVAL;
     }
-    const char* env_t = getenv("T2X_BOUNDS_TILE");   // 259: k_gemm_x2h (128x256); 258: k_gemm_x3p (256x256)
-    const bool wide = env_t && atoi(env_t) == 258 && Mrows_pad % 256 == 0;
+    // 260 (default): k_gemm_x3p NP = 4 (256x256, two K chunks per stage; K a multiple of 64);
+    // 258: k_gemm_x3p NP = 1; 259 (and the fallback): k_gemm_x2h<true> (128x256)
+    const char* env_t = getenv("T2X_BOUNDS_TILE");
+    const int bt = env_t ? atoi(env_t) : 260;
+    const bool wide = (bt == 258 || (bt == 260 && K % (2 * XK) == 0)) && Mrows_pad % 256 == 0;
     const int tm = Mrows_pad / (wide ? 256 : 128), tn = Nrows_pad / 256;
     const long long nwg = (long long)tm * tn;
     if (nwg > 0x7fffffffLL) { set_error("gemm: grid too large"); return T2X_EINVAL; }
     SplitOperand A{Ah, nullptr, As, lda}, B{Bh, nullptr, Bs, ldb};
     const char* env_g = getenv("T2X_BOUNDS_GROUP");   // tile-walk group (tile rows per group)
     const int g = env_g && atoi(env_g) > 0 ? atoi(env_g) : 32;
-    if (wide)
+    if (wide && bt == 260)
+        k_gemm_x3p<_Float16, EPI_DIFF2, 4><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, g, ep, nullptr);
+    else if (wide)
         k_gemm_x3p<_Float16, EPI_DIFF2, 1><<<(unsigned)nwg, 512, 0, st>>>(A, B, K, tm, tn, g, ep, nullptr);
     else
         k_gemm_x2h<true><<<(unsigned)nwg, 256, 0, st>>>(A, B, K, tm, tn, g, ep, nullptr);

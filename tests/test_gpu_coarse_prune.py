@@ -1,9 +1,9 @@
 """The pruned coarse pass (t2x_estep_config.coarse_exhaustive 0) against the exhaustive one.
 
 The coarse diff2 of a sample is a sum of non-negative pixel terms (1/2 minv |ctf F - X_t|^2,
-plus a constant), so the diff2 over the lowest frequencies, a prefix of the |frequency|-ordered
-coarse pixels, bounds it from below. A first GEMM over that prefix gives each 64-column block's
-lower bound; per particle the full diff2 of its most promising block gives an upper bound of its
+plus a constant). A first GEMM over a prefix of the |frequency|-ordered coarse pixels (the low
+frequencies), with the rest bounded by Cauchy-Schwarz and a margin for the FP16 product it
+drops, gives each 64-column block's lower bound; per particle the full diff2 of its most promising block gives an upper bound of its
 minimum (the seed); the full GEMM then skips the tiles whose blocks all lie more than the
 negligible-weight cut (60, plus one for rounding) above the seed. Every skipped block is one the
 selection already treats as negligible (its minimum beyond the cut), so poses, dLL, the
@@ -13,7 +13,8 @@ backprojections 1e-6).
 
 Covered: the three split precisions at class256 shapes (where most tiles are pruned), a
 workload where few are (refine64: the pass turns itself off after a batch that prunes under
-40% of its tiles), a prefix radius forced small (T2X_PRUNE_RADIUS), and ragged batches.
+40% of its tiles), a prefix radius forced small (T2X_PRUNE_RADIUS), ragged batches, the bounds
+GEMM in the full split precision and as each of its kernels, and graph replays.
 """
 import ctypes as C
 
@@ -111,3 +112,17 @@ def test_pruned_graph_replays_count_tiles(gpu_device, monkeypatch):
     assert st.n_graph_launches >= 2
     assert cnt_g == cnt_e
     same(graph, eager)
+
+
+@pytest.mark.parametrize("tile", ["259", "258"])
+def test_pruned_other_bounds_kernels(gpu_device, monkeypatch, tile):
+    """The bounds GEMM as k_gemm_x2h<true> (128x256 tiles) or as k_gemm_x3p with one K chunk per
+    stage (the default takes two)."""
+    monkeypatch.setenv("T2X_BATCH", "6")
+    monkeypatch.setenv("T2X_BOUNDS_TILE", tile)
+    wl = H.Workload("class256", n=12, seed_offset=86, precision=T.T2X_PREC_FP16X2)
+    ex, _ = run(wl, gpu_device, True)
+    pr, cnt = run(wl, gpu_device, False)
+    print(f"\n[prune] class256, bounds tile {tile}: {cnt[0]:.0f} of {cnt[1]:.0f} tiles pruned")
+    assert cnt[0] > 0.5 * cnt[1]
+    same(pr, ex)
