@@ -1938,7 +1938,7 @@ __global__ void k_gather_ccoarse(const PartScalars* __restrict__ ps, int B, doub
     if (i < B) out[i] = ps[i].C_coarse;
 }
 
-// Pruned coarse pass, the bounds pass's N1 table. The coarse diff2 over the pixels beyond the
+// Pruned coarse pass, the bounds pass's N1 term. The coarse diff2 over the pixels beyond the
 // low-frequency prefix (the high set h) is 1/2 sum_h minv |ctf F - X|^2 = Ch + N1h - G_h, with
 // Ch = 1/2 sum_h minv |X|^2 (C_coarse - C_low, the constant highres term cancelling), N1h =
 // 1/2 sum_h minv ctf^2 |F|^2 and G_h = sum_h minv Re(ctf F conj X); Cauchy-Schwarz bounds
@@ -1950,19 +1950,17 @@ __global__ void k_gather_ccoarse(const PartScalars* __restrict__ ps, int B, doub
 // [0.5, 1), so normal FP16 but for negligible elements), so the dropped sum is at most
 // 2 . 2^-11 . sum |A||B| <= 2^-10 . 2 sqrt(N1 Cl), Cl = C_low - highres/2 the image energy of the
 // prefix (Cauchy-Schwarz again); eps1 = 2^-9 allows twice that, for the FP32 sums of both
-// passes. eps1 = 0 for a bounds GEMM in the full split precision. n1h holds N1h on entry (its GEMM
-// over the high set) and the table on return.
-__global__ void k_prune_n1(float* __restrict__ n1h, const float* __restrict__ N1, long long ldn, int O,
-                           const PartScalars* __restrict__ ps, long long total, double eps1)
+// passes. eps1 = 0 for a bounds GEMM in the full split precision. The GEMM's epilogue forms
+// N1 - c0 sqrt(N1h) - c1 sqrt(N1) where it reads N1 (N1h from its GEMM over the high set); this
+// kernel gives each particle's c0 = 2 (1 + 2^-8) sqrt(Ch) and c1 = 2 eps1 sqrt(Cl).
+__global__ void k_prune_c(const PartScalars* __restrict__ ps, int B, double eps1, double* __restrict__ c)
 {
-    const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= total) return;
-    const long long b = i / O, n = i % O;
+    const int b = blockIdx.x * blockDim.x + threadIdx.x;
+    if (b >= B) return;
     const double ch = fmax(ps[b].C_coarse - ps[b].C_low, 0.);
     const double cl = fmax(ps[b].C_low - ps[b].highres_Xi2 / 2., 0.);
-    const long long q = b * ldn + n;
-    const double nh = fmax((double)n1h[q], 0.), n1 = fmax((double)N1[q], 0.);
-    n1h[q] = (float)((double)N1[q] - 2. * (1. + 1. / 256.) * sqrt(nh * ch) - 2. * eps1 * sqrt(n1 * cl));
+    c[2 * b] = 2. * (1. + 1. / 256.) * sqrt(ch);
+    c[2 * b + 1] = 2. * eps1 * sqrt(cl);
 }
 
 // Pruned coarse pass: per particle an upper bound of its coarse minimum diff2, the full diff2 of
@@ -3486,17 +3484,16 @@ extern "C" int t2x_estep(t2x_device* h, const t2x_estep_config* cfg, const t2x_m
                     // terms would cost 2^-7 of the bound) or the full split is asked for
                     const char* env_fb = getenv("T2X_PRUNE_FULL_BOUNDS");
                     const bool h1 = prec != T2X_PREC_BF16X3 && !(env_fb && atoi(env_fb) != 0);
-                    {
-                        const long long ne = (long long)B * O_pad;
-                        k_prune_n1<<<(unsigned)((ne + 255) / 256), 256, 0, st>>>(
-                            d->prune_n1.as<float>(), ep.N1, O_pad, (int)O_pad, d_ps, ne, h1 ? 1. / 512. : 0.);
-                    }
+                    T2X_TRY(d->prune_c.ensure(sizeof(double) * 2 * B));
+                    k_prune_c<<<(unsigned)((B + 255) / 256), 256, 0, st>>>(d_ps, B, h1 ? 1. / 512. : 0.,
+                                                                           d->prune_c.as<double>());
                     T2X_HIP(hipGetLastError());
                     // the bounds go into the block minima themselves: a pruned tile keeps them, a
                     // kept tile's full GEMM overwrites them
                     EpiParams eb = ep;
                     eb.C = nullptr;
-                    eb.N1 = d->prune_n1.as<float>();
+                    eb.N1h = d->prune_n1.as<float>();
+                    eb.prune_c = d->prune_c.as<double>();
                     eb.pmin = nullptr;
                     if (h1)
                         T2X_TRY(launch_gemm_h1_bounds(d->Ah.ptr, d->As.as<float>(), Kc, M_pad, d->coarse_Bh.ptr,

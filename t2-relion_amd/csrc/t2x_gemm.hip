@@ -27,6 +27,18 @@ enum { EPI_STORE = 0, EPI_DIFF2 = 1 };
 // gets 8 rows x 8 columns of them: every A and B tile slice fetched into an XCD's L2 feeds 8
 // workgroups there, and all XCDs walk the same B columns at the same time (one HBM read of B
 // per row group, shared through the MALL). The map is a bijection for any tiles_m, tiles_n.
+// The epilogue's N1 term of particle p, column n: N1 itself, or in a bounds launch of the pruned
+// coarse pass N1 - c0 sqrt(N1h) - c1 sqrt(N1) (the Cauchy-Schwarz bound of the cross term beyond
+// the low-frequency prefix and the margin for the dropped FP16 product; c0, c1 per particle).
+__device__ __forceinline__ float n1_at(const EpiParams& ep, long long p, long long n)
+{
+    const long long q = p * ep.ldn + n;
+    const float v = ep.N1[q];
+    if (ep.N1h == nullptr) return v;
+    const double c0 = ep.prune_c[2 * p], c1 = ep.prune_c[2 * p + 1];
+    return (float)((double)v - c0 * sqrt(fmax((double)ep.N1h[q], 0.)) - c1 * sqrt(fmax((double)v, 0.)));
+}
+
 __device__ __forceinline__ void tile_coords(int pid, int tiles_m, int tiles_n, int group_m, int& tm, int& tn)
 {
     const int per_group = group_m * tiles_n;
@@ -117,9 +129,8 @@ __device__ __forceinline__ void gemm_epilogue(const floatx4 (&acc)[MI][4], long 
                 while ((b + 1) * ep.T <= mm) b++;
                 pb[r] = b;
                 cb[r] = ep.Cb[b];
-                const float* n1row = ep.N1 + (long long)b * ep.ldn;
 #pragma unroll
-                for (int j = 0; j < 4; j++) n1[r][j] = n1row[nval[j] ? ncol[j] : 0];
+                for (int j = 0; j < 4; j++) n1[r][j] = n1_at(ep, b, nval[j] ? ncol[j] : 0);
             }
         }
 #pragma unroll
@@ -670,7 +681,7 @@ __device__ __forceinline__ bool stage_epilogue(EpiStage& es, long long m0l, long
             const unsigned ms = max(m0, (p0 + s) * T);   // a valid row of slot s
             const unsigned p = rowmap ? (unsigned)rowmap[ms] / T : p0 + s;
             const long long n = n0 + c;
-            es.cbn1[s][c] = ep.Cb[p] + (double)(n < ep.N ? ep.N1[(long long)p * ep.ldn + n] : 0.f);
+            es.cbn1[s][c] = ep.Cb[p] + (double)(n < ep.N ? n1_at(ep, p, n) : 0.f);
         }
         if (tid < EPI_SLOTS) {
             int p = -1;   // no particle: slot beyond the tile's valid rows
@@ -1009,7 +1020,7 @@ __device__ __forceinline__ bool stage_epilogue_h(EpiStageH& es, long long m0l, l
         const unsigned ms = max(m0, (p0 + sl) * T);   // a valid row of slot sl
         const unsigned p = rowmap ? (unsigned)rowmap[ms] / T : p0 + sl;
         const long long n = n0 + c;
-        es.n1[sl][c] = n < ep.N ? ep.N1[(long long)p * ep.ldn + n] : 0.f;
+        es.n1[sl][c] = n < ep.N ? n1_at(ep, p, n) : 0.f;
     }
     if (tid < EPI_SLOTS) {
         int p = -1;

@@ -205,7 +205,7 @@ struct Device {
     DevBuf prep_part;   // per particle and row band: background sums of the mask
     // pruned coarse pass: N1 over the low-frequency prefix, its constant terms, the block lower
     // bounds, the per-particle seeds and the tile counters
-    DevBuf prune_n1, prune_seed, prune_cnt, prune_list;
+    DevBuf prune_n1, prune_c, prune_seed, prune_cnt, prune_list;
     bool graph_prune = false;   // the batch graph was captured with the pruned coarse pass
     DevBuf coarse_Ai;   // projection frames of the coarse orientations, for the angles in coarse_Ai_key
     std::vector<double> coarse_Ai_key;

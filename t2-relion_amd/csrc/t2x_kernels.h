@@ -164,6 +164,11 @@ struct EpiParams {
     // *tile_count of them (read by the kernels of the LIST form only).
     const unsigned int* tile_list;
     const unsigned int* tile_count;
+    // Bounds launch of the pruned coarse pass: N1h (the N1 term over the pixels beyond the
+    // low-frequency prefix, ld ldn) non-null turns each N1 read into N1 - c0 sqrt(N1h) -
+    // c1 sqrt(N1), with prune_c[2 p], prune_c[2 p + 1] = c0, c1 of particle p.
+    const float* N1h;
+    const double* prune_c;
 };
 
 struct SelParams {
