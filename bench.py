@@ -709,7 +709,7 @@ def main():
         roof.update({
             "kernel": ("coarse diff2 cross term, pruned pass: k_gemm_x3p<f16, 1, 4> (bounds: 1 FP16 product over "
                        "the low-frequency prefix of the coarse pixels, two K chunks per LDS stage, block minima only) "
-                       "+ k_prune_n1 + k_prune_seed + k_prune_plan + "
+                       "+ k_prune_c + k_prune_seed + k_prune_plan + "
                        f"k_gemm_x3p<LIST> ({nprod} products, persistent over the 256x256 tiles not beyond the "
                        "negligible-weight cut)"),
             "achieved_convention": ("effective: the exhaustive pass's algorithmic flops per batch over the pruned "
