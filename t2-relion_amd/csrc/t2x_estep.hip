@@ -1984,10 +1984,17 @@ __global__ __launch_bounds__(256) void k_prune_seed(const float* __restrict__ lb
     const int b = blockIdx.x, tid = threadIdx.x;
     float best = 3.0e38f;
     long long bi = -1;
-    const long long total = (long long)T * nblk;
-    for (long long q = tid; q < total; q += blockDim.x) {
-        const float v = lb[((long long)b * T + q / nblk) * lb_ld + q % nblk];
-        if (v < best) { best = v; bi = q; }
+    // the particle's T rows of bounds, four blocks per load (lb_ld is a multiple of 4; blocks at
+    // or beyond nblk are padding)
+    const int w4 = lb_ld / 4;
+    for (long long q4 = tid; q4 < (long long)T * w4; q4 += blockDim.x) {
+        const long long t = q4 / w4;
+        const int k = (int)(q4 - t * w4) * 4;
+        const float4 v = *reinterpret_cast<const float4*>(lb + ((long long)b * T + t) * lb_ld + k);
+        const float vv[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int e = 0; e < 4; e++)
+            if (k + e < nblk && vv[e] < best) { best = vv[e]; bi = t * nblk + k + e; }
     }
     sv[tid] = best;
     si[tid] = bi;
